@@ -1,0 +1,139 @@
+/*
+ * Test-infrastructure driver around the reference library (WFA2-lib v2.3),
+ * compiled together with the reference's own sources by oracle/Makefile into
+ * oracle/_ref/.  It is never linked into the product library.
+ *
+ * It exposes one flat entry point so that Python (ctypes) can run the
+ * reference's wavefront_align over a batch of pairs, one aligner per OpenMP
+ * thread, exactly as tools/align_benchmark/align_benchmark.c:504-586 does, and
+ * time only the wavefront_align calls (benchmark_gap_affine.c:128-154).
+ */
+#include <stdint.h>
+#include <string.h>
+#include <stdlib.h>
+#include <omp.h>
+#include "wavefront/wavefront_align.h"
+
+/* Penalty vector layout: {match, mismatch, gap_open1, gap_ext1, gap_open2, gap_ext2}. */
+static void ref_shim_attributes(
+    wavefront_aligner_attr_t* const attr,
+    const int metric,
+    const int* const pen,
+    const int memory_mode,
+    const int score_only,
+    const int max_steps) {
+  *attr = wavefront_aligner_attr_default;
+  attr->heuristic.strategy = wf_heuristic_none;
+  attr->alignment_scope = score_only ? compute_score : compute_alignment;
+  attr->memory_mode = (wavefront_memory_t)memory_mode;
+  if (max_steps > 0) attr->system.max_alignment_steps = max_steps;
+  if (metric == 4) {
+    attr->distance_metric = gap_affine_2p;
+    attr->affine2p_penalties.match = pen[0];
+    attr->affine2p_penalties.mismatch = pen[1];
+    attr->affine2p_penalties.gap_opening1 = pen[2];
+    attr->affine2p_penalties.gap_extension1 = pen[3];
+    attr->affine2p_penalties.gap_opening2 = pen[4];
+    attr->affine2p_penalties.gap_extension2 = pen[5];
+  } else {
+    attr->distance_metric = gap_affine;
+    attr->affine_penalties.match = pen[0];
+    attr->affine_penalties.mismatch = pen[1];
+    attr->affine_penalties.gap_opening = pen[2];
+    attr->affine_penalties.gap_extension = pen[3];
+  }
+}
+
+/*
+ * Align n pairs.  Pair i is pattern = buf[p_off[i] .. p_off[i]+plen[i]) and
+ * text = buf[t_off[i] .. t_off[i]+tlen[i]).  Outputs per pair: the returned
+ * status, cigar->score, the CIGAR re-scored as align_benchmark writes it to
+ * .alg files (benchmark_utils.c:143-179), and the raw operations string
+ * (one of M/X/I/D per column) stored at ops_out[ops_off[i]..+ops_len[i]).
+ * ops_off[i] must leave room for plen[i]+tlen[i] bytes.  Returns the wall
+ * time in seconds of the parallel alignment loop.
+ */
+double ref_align_batch(
+    const int metric,
+    const int* const pen,
+    const int memory_mode,
+    const int score_only,
+    const int max_steps,
+    const int num_threads,
+    const int n,
+    const char* const buf,
+    const int64_t* const p_off,
+    const int* const plen,
+    const int64_t* const t_off,
+    const int* const tlen,
+    int* const status,
+    int* const score,
+    int* const alg_score,
+    char* const ops_out,
+    const int64_t* const ops_off,
+    int* const ops_len) {
+  wavefront_aligner_attr_t attr;
+  ref_shim_attributes(&attr,metric,pen,memory_mode,score_only,max_steps);
+  const int nt = (num_threads > 0) ? num_threads : 1;
+  wavefront_aligner_t** aligners = calloc(nt,sizeof(wavefront_aligner_t*));
+  int t;
+  for (t=0;t<nt;++t) aligners[t] = wavefront_aligner_new(&attr);
+  const double t0 = omp_get_wtime();
+  int i;
+  #pragma omp parallel for num_threads(nt) schedule(dynamic,64)
+  for (i=0;i<n;++i) {
+    wavefront_aligner_t* const wf = aligners[omp_get_thread_num()];
+    status[i] = wavefront_align(wf,buf+p_off[i],plen[i],buf+t_off[i],tlen[i]);
+    cigar_t* const cigar = wf->cigar;
+    score[i] = cigar->score;
+    const int len = (score_only) ? 0 : cigar->end_offset - cigar->begin_offset;
+    ops_len[i] = (len > 0) ? len : 0;
+    if (len > 0) {
+      memcpy(ops_out+ops_off[i],cigar->operations+cigar->begin_offset,len);
+      alg_score[i] = (metric == 4) ?
+          cigar_score_gap_affine2p(cigar,&attr.affine2p_penalties) :
+          cigar_score_gap_affine(cigar,&attr.affine_penalties);
+    } else {
+      alg_score[i] = (score_only) ? cigar->score : -1;
+    }
+  }
+  const double t1 = omp_get_wtime();
+  for (t=0;t<nt;++t) wavefront_aligner_delete(aligners[t]);
+  free(aligners);
+  return t1 - t0;
+}
+
+/* Time-only variant for the CPU baseline: aligns and discards the results. */
+double ref_align_time(
+    const int metric,
+    const int* const pen,
+    const int memory_mode,
+    const int num_threads,
+    const int n,
+    const char* const buf,
+    const int64_t* const p_off,
+    const int* const plen,
+    const int64_t* const t_off,
+    const int* const tlen,
+    int64_t* const checksum) {
+  wavefront_aligner_attr_t attr;
+  ref_shim_attributes(&attr,metric,pen,memory_mode,0,0);
+  const int nt = (num_threads > 0) ? num_threads : 1;
+  wavefront_aligner_t** aligners = calloc(nt,sizeof(wavefront_aligner_t*));
+  int t;
+  for (t=0;t<nt;++t) aligners[t] = wavefront_aligner_new(&attr);
+  int64_t sum = 0;
+  const double t0 = omp_get_wtime();
+  int i;
+  #pragma omp parallel for num_threads(nt) schedule(dynamic,64) reduction(+:sum)
+  for (i=0;i<n;++i) {
+    wavefront_aligner_t* const wf = aligners[omp_get_thread_num()];
+    wavefront_align(wf,buf+p_off[i],plen[i],buf+t_off[i],tlen[i]);
+    sum += wf->cigar->score;
+  }
+  const double t1 = omp_get_wtime();
+  for (t=0;t<nt;++t) wavefront_aligner_delete(aligners[t]);
+  free(aligners);
+  *checksum = sum;
+  return t1 - t0;
+}

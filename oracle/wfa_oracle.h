@@ -1,0 +1,56 @@
+/*
+ * CPU oracle for the exact WFA hot path (TEST INFRASTRUCTURE ONLY).
+ *
+ * A plain-C restatement of WFA2-lib v2.3's wavefront_align for gap-affine and
+ * dual-cost gap-affine, end-to-end, in memory mode high (and med/low, whose
+ * CIGARs the reference asserts equal memory-high: tests/wfa.utest.sh:67-78)
+ * and ultralow (BiWFA).  It is the checker for the HIP path; only tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py may load it.
+ *
+ * Pinning: every function here is checked against the reference's golden
+ * vectors (the .alg files under tests/golden/, copied from tests/wfa.utest.check/), against
+ * the reference itself compiled from source into oracle/_ref/, and against
+ * an independent O(nm) dynamic-programming scorer (tests/test_oracle.py).
+ */
+#ifndef WFA_ORACLE_H_
+#define WFA_ORACLE_H_
+
+#include <stdint.h>
+
+typedef struct {
+  int metric;          /* 3 = gap-affine, 4 = gap-affine-2p (wavefront_penalties.h:41-47) */
+  int match;           /* user penalties, as in affine{,2p}_penalties_t */
+  int mismatch;
+  int gap_opening1;
+  int gap_extension1;
+  int gap_opening2;
+  int gap_extension2;
+  int memory_mode;     /* 0 high, 1 med, 2 low, 3 ultralow (wavefront_attributes.h:104-109) */
+  int score_only;      /* alignment_scope == compute_score */
+  int max_steps;       /* <= 0 means unlimited (INT_MAX) */
+} oracle_conf_t;
+
+/*
+ * Align one pair.  Returns the reference's status code (wfa.h:45-55).
+ * score      <- cigar->score (classic score, wavefront_compute.c:108-120)
+ * ops        <- CIGAR operations (M/X/I/D, one per column), capacity plen+tlen
+ * ops_len    <- number of operations written
+ * dsteps     <- diagonal-steps: sum of (hi-lo+1) over every non-null compute
+ */
+int oracle_align(
+    const oracle_conf_t* conf,
+    const char* pattern, int plen,
+    const char* text, int tlen,
+    int* score, char* ops, int* ops_len, int64_t* dsteps, int* end_score);
+
+/* Batch form over a shared byte buffer (see oracle/oracle.py). */
+void oracle_align_batch(
+    const oracle_conf_t* conf, int n, const char* buf,
+    const int64_t* p_off, const int* plen, const int64_t* t_off, const int* tlen,
+    int* status, int* score, char* ops_out, const int64_t* ops_off, int* ops_len,
+    int64_t* dsteps, int* end_score);
+
+/* cigar_score_gap_affine / cigar_score_gap_affine2p (alignment/cigar.c:277-342). */
+int oracle_cigar_score(const oracle_conf_t* conf, const char* ops, int len);
+
+#endif
