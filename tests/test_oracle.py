@@ -1,0 +1,159 @@
+"""Pins the CPU oracle (oracle/wfa_oracle.c) before it is trusted as checker.
+
+1. Golden vectors: the reference's own regression outputs for the 305-pair
+   suite (tests/wfa.utest.check/*.alg) in every exact mode the device path
+   covers: memory high / med / score-only / BiWFA / BiWFA score-only,
+   gap-affine with seven penalty sets (match < 0 included) and gap-affine-2p.
+2. The reference itself, compiled from its sources into oracle/_ref/, on
+   seeded random pairs of the benchmark shapes (skipped if not built).
+3. Mathematics: an independent Gotoh DP (tests/dp.py) for the optimal score,
+   CIGAR validity and re-scoring, and closed forms.
+"""
+import random
+
+import numpy as np
+import pytest
+
+import dp
+import golden_suite as G
+import oracle as O
+
+
+def _pairs(pats, texts):
+    return O.Pairs(pats, texts)
+
+
+@pytest.fixture(scope="module")
+def suite():
+    return G.read_seq()
+
+
+@pytest.mark.parametrize("case", G.cases(), ids=[c[0] for c in G.cases()])
+def test_oracle_matches_golden(suite, case):
+    name, metric, pen, mem, so = case
+    pats, texts = suite
+    conf = O.Conf(metric, pen, mem, so)
+    r = O.oracle_align(conf, _pairs(pats, texts))
+    assert O.alg_lines(r, so) == G.golden_lines(name)
+
+
+def _rand_seq(rng, n, alphabet=b"ACGT"):
+    return bytes(rng.choice(alphabet) for _ in range(n))
+
+
+def _mutate(rng, s, nerr, max_indel=1):
+    s = bytearray(s)
+    for _ in range(nerr):
+        k = rng.randrange(3)
+        if k == 0 and s:
+            p = rng.randrange(len(s))
+            s[p] = rng.choice([c for c in b"ACGT" if c != s[p]])
+        elif k == 1:
+            p = rng.randrange(len(s) + 1)
+            L = rng.randint(1, max_indel)
+            s[p:p] = bytes(rng.choice(b"ACGT") for _ in range(L))
+        elif s:
+            p = rng.randrange(len(s))
+            L = rng.randint(1, max_indel)
+            del s[p:p + L]
+    return bytes(s)
+
+
+def _random_pairs(seed, count, lo, hi, maxerr, max_indel=1):
+    rng = random.Random(seed)
+    pats, texts = [], []
+    for _ in range(count):
+        t = _rand_seq(rng, rng.randint(lo, hi))
+        p = _mutate(rng, t, rng.randint(0, maxerr), max_indel)
+        pats.append(p)
+        texts.append(t)
+    return pats, texts
+
+
+PEN_SETS = [
+    ("affine", (0, 4, 6, 2)),
+    ("affine", (0, 1, 2, 1)),
+    ("affine", (-3, 5, 3, 2)),
+    ("affine", (-5, 1, 2, 1)),
+    ("affine2p", (0, 4, 6, 2, 24, 1)),
+    ("affine2p", (-2, 3, 4, 3, 12, 1)),
+]
+
+
+@pytest.mark.parametrize("metric,pen", PEN_SETS)
+def test_oracle_score_is_dp_optimum(metric, pen):
+    pats, texts = _random_pairs(hash((metric, pen)) & 0xFFFF, 60, 0, 36, 8, max_indel=4)
+    for memory in ("high", "ultralow"):
+        r = O.oracle_align(O.Conf(metric, pen, memory), _pairs(pats, texts))
+        for i, (p, t) in enumerate(zip(pats, texts)):
+            assert r.status[i] == 0
+            best = dp.gotoh(p, t, *pen)
+            ops = r.ops(i)
+            assert dp.cigar_valid(p, t, ops), (p, t, ops)
+            assert dp.cigar_score(ops, *pen) == best, (p, t, ops)
+            if memory == "high" or (len(p) and len(t)):
+                assert r.score[i] == best
+
+
+def test_oracle_closed_forms():
+    conf = O.Conf("affine", (0, 4, 6, 2))
+    t = b"ACGTTGCAACGTACGGATCCATGACCAGTACGATCG"
+    sub = t[:10] + (b"A" if t[10:11] != b"A" else b"C") + t[11:]
+    ins3 = t[:15] + b"GGG" + t[15:]
+    r = O.oracle_align(conf, _pairs([t, sub, ins3, t[:20] + t[27:]], [t, t, t, t]))
+    assert list(r.score) == [0, -4, -(6 + 3 * 2), -(6 + 7 * 2)]
+    assert r.ops(0) == b"M" * len(t)
+    # dual cost: a long gap takes the cheaper second piece
+    conf2 = O.Conf("affine2p", (0, 4, 6, 2, 24, 1))
+    long_del = t[:5] + b"A" * 30 + t[5:]
+    r2 = O.oracle_align(conf2, _pairs([long_del, ins3], [t, t]))
+    assert r2.score[0] == -min(6 + 30 * 2, 24 + 30)
+    assert r2.score[1] == -min(6 + 3 * 2, 24 + 3)
+
+
+def test_oracle_empty_and_trivial():
+    conf = O.Conf("affine", (0, 4, 6, 2))
+    r = O.oracle_align(conf, _pairs([b"", b"ACGT", b"", b"A"], [b"", b"", b"ACG", b"A"]))
+    assert list(r.status) == [0, 0, 0, 0]
+    assert r.ops(0) == b"" and r.ops(1) == b"DDDD" and r.ops(2) == b"III" and r.ops(3) == b"M"
+    assert list(r.score) == [0, -(6 + 8), -(6 + 6), 0]
+
+
+def test_oracle_max_steps():
+    conf = O.Conf("affine", (0, 4, 6, 2), max_steps=10)
+    t = b"ACGTACGTTTGACCA" * 4
+    p = _mutate(random.Random(3), t, 12)
+    r = O.oracle_align(conf, _pairs([p], [t]))
+    assert r.status[0] == O.WF_STATUS_MAX_STEPS_REACHED and r.score[0] == -10
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("metric,pen,memory,shape", [
+    ("affine", (0, 4, 6, 2), "high", "short"),
+    ("affine", (0, 4, 6, 2), "high", "kbp"),
+    ("affine2p", (0, 4, 6, 2, 24, 1), "high", "kbp"),
+    ("affine", (0, 4, 6, 2), "ultralow", "kbp"),
+    ("affine2p", (0, 4, 6, 2, 24, 1), "ultralow", "kbp"),
+    ("affine", (-3, 5, 3, 2), "high", "short"),
+])
+def test_oracle_matches_reference_build(metric, pen, memory, shape):
+    from wfa_amd import datagen
+    ps = datagen.generate(1, 11, 600) if shape == "short" else datagen.generate(2, 12, 6)
+    pairs = _pairs(ps.patterns(), ps.texts())
+    conf = O.Conf(metric, pen, memory)
+    a = O.oracle_align(conf, pairs)
+    b = O.ref_align(conf, pairs, threads=4)
+    assert list(a.status) == list(b.status)
+    assert list(a.score) == list(b.score)
+    for i in range(pairs.n):
+        assert a.ops(i) == b.ops(i), i
+
+
+def test_oracle_counts_diagonal_steps():
+    # W for a fixed pair is deterministic; two identical sequences need one
+    # extension and no compute step at all.
+    conf = O.Conf("affine", (0, 4, 6, 2))
+    t = b"ACGTACGGTCA" * 9
+    r = O.oracle_align(conf, _pairs([t, t[:50] + b"T" + t[51:]], [t, t]))
+    assert r.dsteps[0] == 0
+    assert r.dsteps[1] > 0 and r.end_score[1] == 4

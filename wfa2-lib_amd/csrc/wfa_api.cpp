@@ -1,0 +1,829 @@
+// C-ABI host layer: the drop-in wavefront_aligner_new / wavefront_align /
+// wavefront_aligner_delete entry points (wavefront/wavefront_align.c:212-241,
+// wavefront/wavefront_aligner.c:422-504) plus wavefront_align_batch and the
+// device-resident batch API, all driving the HIP kernels of wfa_kernels.hip.
+//
+// One aligner owns one HIP stream and a grow-only device arena.  A batch is
+// packed on the host (2 bits per base when every byte is A/C/G/T, else one
+// byte per symbol), copied to HBM once, and aligned by:
+//   1. forward kernels, one launch per kernel family (pairs sorted by length;
+//      short pairs on the LDS-resident one-wavefront family, long pairs on the
+//      workgroup family with the wavefront ring in HBM);
+//   2. one backtrace kernel over the pairs that completed;
+//   3. a retry pass, only for pairs that overflowed the LDS ring (moved to the
+//      HBM-ring family) or the record arena (arena reset, smaller pass).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <thread>
+#include <vector>
+
+#include "wfa_device.h"
+#include "wfa_mi355x.h"
+
+namespace {
+
+using wfa::ForwardArgs;
+using wfa::BacktraceArgs;
+using wfa::LaunchShape;
+using wfa::Penalties;
+
+bool hip_ok(hipError_t e, const char* what) {
+  if (e == hipSuccess) return true;
+  fprintf(stderr, "[WFA] HIP error in %s: %s\n", what, hipGetErrorString(e));
+  return false;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  bool ensure(size_t bytes) {
+    if (bytes <= cap && p) return true;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max<size_t>(bytes, 256);
+    if (hipMalloc(&p, want) != hipSuccess) { p = nullptr; return false; }
+    cap = want;
+    return true;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+// Per-aligner device state.
+struct Engine {
+  int device = 0;
+  int num_cus = 0;
+  size_t lds_per_cu = 0;
+  hipStream_t stream = nullptr;
+  DevBuf rec;        // backtrace-record arena
+  DevBuf ring;       // HBM wavefront rings of the workgroup family
+  DevBuf counters;   // queue counter + arena bump pointer
+  size_t rec_bytes = 0;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool ok = false;
+};
+
+int gcd(int a, int b) { while (b) { int t = a % b; a = b; b = t; } return a; }
+
+}  // namespace
+
+// ----------------------------------------------------------------- batches
+
+struct wfa_device_batch_t {
+  int n = 0;
+  int bits = 2;
+  int64_t total_ops = 0;
+  std::vector<int> plen, tlen;
+  std::vector<int64_t> ops_off;
+  std::vector<int> sorted;           // pair ids, longest first
+  DevBuf words, d_plen, d_tlen, d_patw, d_txtw, d_order, d_status, d_score, d_last, d_dsteps,
+         d_ops_off, d_ops_begin, d_ops;
+  // results of the last run (host)
+  std::vector<int> status, score, ops_begin;
+  std::vector<long long> dsteps;
+  bool have_results = false;
+  bool ops_on_device = false;
+  wfa_run_stats_t stats{};
+  // the aligner configuration it was run with
+  int records = 1;
+  ~wfa_device_batch_t() {
+    for (DevBuf* b : {&words, &d_plen, &d_tlen, &d_patw, &d_txtw, &d_order, &d_status, &d_score, &d_last,
+                      &d_dsteps, &d_ops_off, &d_ops_begin, &d_ops})
+      b->release();
+  }
+};
+
+namespace {
+
+Engine* engine_of(wavefront_aligner_t* wf) { return static_cast<Engine*>(wf->device); }
+
+bool engine_init(Engine* e) {
+  if (e->ok) return true;
+  if (!hip_ok(hipGetDevice(&e->device), "hipGetDevice")) return false;
+  hipDeviceProp_t prop;
+  if (!hip_ok(hipGetDeviceProperties(&prop, e->device), "hipGetDeviceProperties")) return false;
+  e->num_cus = prop.multiProcessorCount;
+  e->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
+  if (!hip_ok(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "hipStreamCreate")) return false;
+  for (auto& ev : e->ev)
+    if (!hip_ok(hipEventCreate(&ev), "hipEventCreate")) return false;
+  if (!e->counters.ensure(64)) return false;
+  const char* env = getenv("WFA_RECORD_ARENA_MB");
+  size_t mb = env ? strtoull(env, nullptr, 10) : 8192;
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0)
+    mb = std::min<size_t>(mb, (free_b / 4) >> 20);
+  e->rec_bytes = std::max<size_t>(mb, 64) << 20;
+  e->ok = true;
+  return true;
+}
+
+void engine_free(Engine* e) {
+  if (!e) return;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  e->rec.release();
+  e->ring.release();
+  e->counters.release();
+  for (auto& ev : e->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+Penalties make_penalties(const wavefront_penalties_t& p, bool two) {
+  Penalties r{};
+  r.x = p.mismatch;
+  r.o1 = p.gap_opening1;
+  r.e1 = p.gap_extension1;
+  r.o2 = two ? p.gap_opening2 : 0;
+  r.e2 = two ? p.gap_extension2 : 1;
+  int indel = r.o1 + r.e1;
+  if (two) indel = std::max(indel, r.o2 + r.e2);
+  r.scope = std::max(indel, r.x) + 1;
+  int g = gcd(r.x, gcd(r.o1 + r.e1, r.e1));
+  if (two) g = gcd(g, gcd(r.o2 + r.e2, r.e2));
+  r.g = std::max(g, 1);
+  r.nslots = (r.scope - 1) / r.g + 1;
+  return r;
+}
+
+// wavefront_penalties_set_affine{,2p} (wavefront_penalties.c:95-173).
+void set_penalties(wavefront_penalties_t* wp, const wavefront_aligner_attr_t* attr) {
+  memset(wp, 0, sizeof(*wp));
+  wp->distance_metric = attr->distance_metric;
+  if (attr->distance_metric == gap_affine_2p) {
+    const affine2p_penalties_t& a = attr->affine2p_penalties;
+    if (a.match > 0) {
+      fprintf(stderr, "[WFA::Penalties] Match score must be negative or zero (M=%d)\n", a.match);
+      exit(1);
+    }
+    if (a.mismatch <= 0 || a.gap_opening1 < 0 || a.gap_extension1 <= 0 || a.gap_opening2 < 0 ||
+        a.gap_extension2 <= 0) {
+      fprintf(stderr, "[WFA::Penalties] Penalties (X=%d,O1=%d,E1=%d,O2=%d,E2=%d) must be (X>0,O1>=0,E1>0,O2>=0,E2>0)\n",
+              a.mismatch, a.gap_opening1, a.gap_extension1, a.gap_opening2, a.gap_extension2);
+      exit(1);
+    }
+    if (a.match < 0) {
+      wp->match = a.match;
+      wp->mismatch = 2 * a.mismatch - 2 * a.match;
+      wp->gap_opening1 = 2 * a.gap_opening1;
+      wp->gap_extension1 = 2 * a.gap_extension1 - a.match;
+      wp->gap_opening2 = 2 * a.gap_opening2;
+      wp->gap_extension2 = 2 * a.gap_extension2 - a.match;
+    } else {
+      wp->match = 0;
+      wp->mismatch = a.mismatch;
+      wp->gap_opening1 = a.gap_opening1;
+      wp->gap_extension1 = a.gap_extension1;
+      wp->gap_opening2 = a.gap_opening2;
+      wp->gap_extension2 = a.gap_extension2;
+    }
+    wp->affine2p_penalties = a;
+    wp->internal_gap_e = a.gap_extension1;
+  } else if (attr->distance_metric == gap_affine) {
+    const affine_penalties_t& a = attr->affine_penalties;
+    if (a.match > 0) {
+      fprintf(stderr, "[WFA::Penalties] Match score must be negative or zero (M=%d)\n", a.match);
+      exit(1);
+    }
+    if (a.mismatch <= 0 || a.gap_opening < 0 || a.gap_extension <= 0) {
+      fprintf(stderr, "[WFA::Penalties] Penalties (X=%d,O=%d,E=%d) must be (X>0,O>=0,E>0)\n",
+              a.mismatch, a.gap_opening, a.gap_extension);
+      exit(1);
+    }
+    if (a.match < 0) {
+      wp->match = a.match;
+      wp->mismatch = 2 * a.mismatch - 2 * a.match;
+      wp->gap_opening1 = 2 * a.gap_opening;
+      wp->gap_extension1 = 2 * a.gap_extension - a.match;
+    } else {
+      wp->match = 0;
+      wp->mismatch = a.mismatch;
+      wp->gap_opening1 = a.gap_opening;
+      wp->gap_extension1 = a.gap_extension;
+    }
+    wp->gap_opening2 = -1;
+    wp->gap_extension2 = -1;
+    wp->affine_penalties = a;
+    wp->internal_gap_e = a.gap_extension;
+  } else {
+    wp->linear_penalties = attr->linear_penalties;
+  }
+}
+
+// wavefront_compute_classic_score (wavefront_compute.c:108-120).
+int classic_score(const wavefront_penalties_t& p, int plen, int tlen, int s) {
+  const int swg_match = -p.match;
+  if (swg_match == 0) return -s;
+  return (swg_match * (plen + tlen) - s) / 2;
+}
+
+// Configurations the exact device path covers; everything else is refused.
+int check_supported(wavefront_aligner_t* wf) {
+  const distance_metric_t m = wf->penalties.distance_metric;
+  const char* why = nullptr;
+  if (m != gap_affine && m != gap_affine_2p) why = "only gap-affine and gap-affine-2p metrics";
+  else if (wf->alignment_form.span != alignment_end2end || wf->alignment_form.extension)
+    why = "only end-to-end alignment";
+  else if (wf->heuristic.strategy != wf_heuristic_none) why = "only heuristic none (exact alignment)";
+  else if (wf->memory_mode == wavefront_memory_ultralow) why = "memory mode ultralow (BiWFA) is not enabled in this build";
+  if (!why) return WF_STATUS_OK;
+  if (wf->system.verbose >= 0)
+    fprintf(stderr, "[WFA] Unsupported configuration on the device path: %s\n", why);
+  return WF_STATUS_UNSUPPORTED;
+}
+
+bool all_acgt(const wfa_pair_t* pairs, int n) {
+  static bool lut_init = false;
+  static bool lut[256];
+  if (!lut_init) {
+    for (int i = 0; i < 256; ++i) lut[i] = false;
+    lut['A'] = lut['C'] = lut['G'] = lut['T'] = true;
+    lut_init = true;
+  }
+  for (int i = 0; i < n; ++i) {
+    const unsigned char* p = reinterpret_cast<const unsigned char*>(pairs[i].pattern);
+    for (int j = 0; j < pairs[i].pattern_length; ++j) if (!lut[p[j]]) return false;
+    const unsigned char* t = reinterpret_cast<const unsigned char*>(pairs[i].text);
+    for (int j = 0; j < pairs[i].text_length; ++j) if (!lut[t[j]]) return false;
+  }
+  return true;
+}
+
+inline int64_t nwords(int len, int bits) { return len / (64 / bits) + 2; }
+
+void pack_seq(uint64_t* dst, const char* src, int len, int bits) {
+  const int64_t nw = nwords(len, bits);
+  memset(dst, 0, nw * sizeof(uint64_t));
+  if (bits == 8) {
+    memcpy(dst, src, len);
+    return;
+  }
+  // A,C,G,T -> 0,1,3,2 ((c >> 1) & 3); any bijection works for equality tests.
+  for (int i = 0; i < len; ++i) {
+    const uint64_t code = (uint64_t)((static_cast<unsigned char>(src[i]) >> 1) & 3);
+    dst[i >> 5] |= code << (2 * (i & 31));
+  }
+}
+
+// Kernel-family plan for one pass over a subset of pairs.
+struct Group {
+  int family;
+  int block;
+  int wcap;
+  int lmax_words;
+  std::vector<int> ids;
+};
+
+constexpr int kSmallLdsBudget = 16 * 1024;  // per workgroup: keeps ~9 wavefronts per CU resident
+
+int small_wcap(const Penalties& pen, int ncomp) {
+  int w = kSmallLdsBudget / (pen.nslots * ncomp * 4);
+  w = (w / 32) * 32;
+  return w;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ lifecycle
+
+extern "C" wavefront_aligner_t* wavefront_aligner_new(wavefront_aligner_attr_t* attributes) {
+  if (attributes == nullptr) attributes = &wavefront_aligner_attr_default;
+  wavefront_aligner_t* wf = static_cast<wavefront_aligner_t*>(calloc(1, sizeof(wavefront_aligner_t)));
+  const bool bi = attributes->memory_mode == wavefront_memory_ultralow;
+  wf->align_mode = bi ? wf_align_biwfa : wf_align_regular;
+  wf->align_status.status = WF_STATUS_OK;
+  wf->alignment_scope = attributes->alignment_scope;
+  wf->alignment_form = attributes->alignment_form;
+  set_penalties(&wf->penalties, attributes);
+  wf->heuristic = attributes->heuristic;
+  if (wf->heuristic.strategy == wf_heuristic_none) memset(&wf->heuristic, 0, sizeof(wf->heuristic));
+  wf->memory_mode = attributes->memory_mode;
+  wf->component_begin = affine2p_matrix_M;
+  wf->component_end = affine2p_matrix_M;
+  wf->system = attributes->system;
+  wf->cigar = static_cast<cigar_t*>(calloc(1, sizeof(cigar_t)));
+  wf->cigar->max_operations = 2 * (1000 + 1000);
+  wf->cigar->operations = static_cast<char*>(malloc(wf->cigar->max_operations));
+  wf->cigar->score = INT32_MIN;
+  wf->cigar->end_v = wf->cigar->end_h = -1;
+  wf->device = new (std::nothrow) Engine();
+  return wf;
+}
+
+extern "C" void wavefront_aligner_reap(wavefront_aligner_t* const wf) {
+  Engine* e = engine_of(wf);
+  if (!e) return;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  e->rec.release();
+  e->ring.release();
+}
+
+extern "C" void wavefront_aligner_delete(wavefront_aligner_t* const wf) {
+  if (!wf) return;
+  engine_free(engine_of(wf));
+  free(wf->sequences.pattern);
+  free(wf->sequences.text);
+  if (wf->cigar) {
+    free(wf->cigar->operations);
+    free(wf->cigar->cigar_buffer);
+    free(wf->cigar);
+  }
+  free(wf);
+}
+
+// ------------------------------------------------------------ batches
+
+extern "C" size_t wavefront_align_batch_arena_bytes(const wfa_pair_t* pairs, int num_pairs) {
+  size_t b = 0;
+  for (int i = 0; i < num_pairs; ++i) b += (size_t)pairs[i].pattern_length + (size_t)pairs[i].text_length;
+  return b;
+}
+
+extern "C" wfa_device_batch_t* wfa_device_batch_upload(wavefront_aligner_t* const wf,
+                                                      const wfa_pair_t* pairs, int n) {
+  Engine* e = engine_of(wf);
+  if (!e || !engine_init(e) || n < 0) return nullptr;
+  wfa_device_batch_t* b = new (std::nothrow) wfa_device_batch_t();
+  if (!b) return nullptr;
+  b->n = n;
+  b->bits = all_acgt(pairs, n) ? 2 : 8;
+  b->plen.resize(n);
+  b->tlen.resize(n);
+  b->ops_off.resize(n);
+  std::vector<int64_t> patw(n), txtw(n);
+  int64_t words = 0, ops = 0;
+  for (int i = 0; i < n; ++i) {
+    b->plen[i] = pairs[i].pattern_length;
+    b->tlen[i] = pairs[i].text_length;
+    patw[i] = words;
+    words += nwords(b->plen[i], b->bits);
+    txtw[i] = words;
+    words += nwords(b->tlen[i], b->bits);
+    b->ops_off[i] = ops;
+    ops += (int64_t)b->plen[i] + b->tlen[i];
+  }
+  b->total_ops = ops;
+  std::vector<uint64_t> host_words((size_t)words);
+  {
+    const int nt = std::max(1, std::min<int>(16, (int)std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t) {
+      th.emplace_back([&, t]() {
+        for (int i = t; i < n; i += nt) {
+          pack_seq(host_words.data() + patw[i], pairs[i].pattern, b->plen[i], b->bits);
+          pack_seq(host_words.data() + txtw[i], pairs[i].text, b->tlen[i], b->bits);
+        }
+      });
+    }
+    for (auto& x : th) x.join();
+  }
+  b->sorted.resize(n);
+  std::iota(b->sorted.begin(), b->sorted.end(), 0);
+  std::stable_sort(b->sorted.begin(), b->sorted.end(), [&](int x, int y) {
+    return (int64_t)b->plen[x] + b->tlen[x] > (int64_t)b->plen[y] + b->tlen[y];
+  });
+  const size_t nn = std::max(n, 1);
+  bool ok = b->words.ensure(std::max<size_t>(words, 1) * 8) && b->d_plen.ensure(nn * 4) &&
+            b->d_tlen.ensure(nn * 4) && b->d_patw.ensure(nn * 8) && b->d_txtw.ensure(nn * 8) &&
+            b->d_order.ensure(nn * 4) && b->d_status.ensure(nn * 4) && b->d_score.ensure(nn * 4) &&
+            b->d_last.ensure(nn * 4) && b->d_dsteps.ensure(nn * 8) && b->d_ops_off.ensure(nn * 8) &&
+            b->d_ops_begin.ensure(nn * 4);
+  if (!ok) {
+    fprintf(stderr, "[WFA] device allocation failed for a batch of %d pairs\n", n);
+    delete b;
+    return nullptr;
+  }
+  hipStream_t st = e->stream;
+  ok = hip_ok(hipMemcpyAsync(b->words.p, host_words.data(), words * 8, hipMemcpyHostToDevice, st), "upload") &&
+       hip_ok(hipMemcpyAsync(b->d_plen.p, b->plen.data(), n * 4, hipMemcpyHostToDevice, st), "upload") &&
+       hip_ok(hipMemcpyAsync(b->d_tlen.p, b->tlen.data(), n * 4, hipMemcpyHostToDevice, st), "upload") &&
+       hip_ok(hipMemcpyAsync(b->d_patw.p, patw.data(), n * 8, hipMemcpyHostToDevice, st), "upload") &&
+       hip_ok(hipMemcpyAsync(b->d_txtw.p, txtw.data(), n * 8, hipMemcpyHostToDevice, st), "upload") &&
+       hip_ok(hipMemcpyAsync(b->d_ops_off.p, b->ops_off.data(), n * 8, hipMemcpyHostToDevice, st), "upload") &&
+       hip_ok(hipStreamSynchronize(st), "upload sync");
+  if (!ok) { delete b; return nullptr; }
+  return b;
+}
+
+extern "C" void wfa_device_batch_free(wfa_device_batch_t* b) { delete b; }
+
+namespace {
+
+// One forward launch over `ids` followed by the backtrace of those pairs.
+int run_group(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, const Group& grp,
+              bool two, const Penalties& pen, int records, float* fwd_ms, float* bt_ms) {
+  const int n = (int)grp.ids.size();
+  if (n == 0) return WF_STATUS_OK;
+  hipStream_t st = e->stream;
+  const int ncomp = two ? 5 : 3;
+  LaunchShape shape;
+  shape.family = grp.family;
+  shape.block = grp.block;
+  shape.lds_bytes = wfa::forward_lds_bytes(grp.family, ncomp, pen.nslots, grp.wcap, grp.lmax_words);
+  int per_cu = wfa::forward_max_blocks_per_cu(grp.family, grp.block, two, b->bits, shape.lds_bytes);
+  if (per_cu <= 0) per_cu = 1;
+  shape.grid = std::min<int64_t>(n, (int64_t)per_cu * e->num_cus);
+  if (!hip_ok(hipMemcpyAsync(b->d_order.p, grp.ids.data(), n * 4, hipMemcpyHostToDevice, st), "order upload"))
+    return WF_STATUS_OOM;
+  if (records && !e->rec.ensure(e->rec_bytes)) {
+    fprintf(stderr, "[WFA] cannot allocate the %zu MB record arena\n", e->rec_bytes >> 20);
+    return WF_STATUS_OOM;
+  }
+  int* ring = nullptr;
+  long long ring_stride = 0;
+  if (grp.family == wfa::kLarge) {
+    ring_stride = (long long)pen.nslots * ncomp * grp.wcap;
+    if (!e->ring.ensure((size_t)ring_stride * shape.grid * sizeof(int))) {
+      fprintf(stderr, "[WFA] cannot allocate %lld MB of wavefront rings\n",
+              (long long)((size_t)ring_stride * shape.grid * 4 >> 20));
+      return WF_STATUS_OOM;
+    }
+    ring = e->ring.as<int>();
+  }
+  unsigned int* queue = e->counters.as<unsigned int>();
+  unsigned long long* rec_top = reinterpret_cast<unsigned long long*>(e->counters.as<char>() + 8);
+  ForwardArgs fa{};
+  fa.n_pairs = n;
+  fa.order = b->d_order.as<int>();
+  fa.plen = b->d_plen.as<int>();
+  fa.tlen = b->d_tlen.as<int>();
+  fa.words = b->words.as<uint64_t>();
+  fa.pat_word = b->d_patw.as<int64_t>();
+  fa.txt_word = b->d_txtw.as<int64_t>();
+  fa.out_status = b->d_status.as<int>();
+  fa.out_score = b->d_score.as<int>();
+  fa.out_dsteps = b->d_dsteps.as<long long>();
+  fa.out_last_block = b->d_last.as<uint32_t>();
+  fa.rec_arena = e->rec.as<uint4>();
+  fa.rec_units = records ? std::min<unsigned long long>(e->rec.cap / 16, 0xFFFFFFF0ull) : 0;
+  fa.rec_top = rec_top;
+  fa.ring_global = ring;
+  fa.ring_stride = ring_stride;
+  fa.queue = queue;
+  fa.wcap = grp.wcap;
+  fa.lmax_words = grp.lmax_words;
+  fa.max_steps = wf->system.max_alignment_steps > 0 ? wf->system.max_alignment_steps : INT_MAX;
+  fa.records = records;
+  fa.comp_begin = wfa::kM;
+  fa.comp_end = wfa::kM;
+  fa.pen = pen;
+  if (!hip_ok(wfa::launch_reset_counters(queue, rec_top, st), "reset")) return WF_STATUS_OOM;
+  (void)hipEventRecord(e->ev[0], st);
+  if (!hip_ok(wfa::launch_forward(fa, two, b->bits, shape, st), "forward launch")) return WF_STATUS_OOM;
+  (void)hipEventRecord(e->ev[1], st);
+  if (records) {
+    BacktraceArgs ba{};
+    ba.n_pairs = n;
+    ba.order = b->d_order.as<int>();
+    ba.plen = b->d_plen.as<int>();
+    ba.tlen = b->d_tlen.as<int>();
+    ba.status = b->d_status.as<int>();
+    ba.score = b->d_score.as<int>();
+    ba.last_block = b->d_last.as<uint32_t>();
+    ba.rec_arena = e->rec.as<uint4>();
+    ba.rec_units = fa.rec_units;
+    ba.ops = b->d_ops.as<char>();
+    ba.ops_off = b->d_ops_off.as<int64_t>();
+    ba.ops_begin = b->d_ops_begin.as<int>();
+    ba.comp_begin = wfa::kM;
+    ba.comp_end = wfa::kM;
+    ba.pen = pen;
+    if (!hip_ok(wfa::launch_backtrace(ba, st), "backtrace launch")) return WF_STATUS_OOM;
+  }
+  (void)hipEventRecord(e->ev[2], st);
+  if (!hip_ok(hipStreamSynchronize(st), "alignment")) return WF_STATUS_OOM;
+  float f = 0, t = 0;
+  (void)hipEventElapsedTime(&f, e->ev[0], e->ev[1]);
+  (void)hipEventElapsedTime(&t, e->ev[1], e->ev[2]);
+  *fwd_ms += f;
+  *bt_ms += t;
+  return WF_STATUS_OK;
+}
+
+Group large_group(const wfa_device_batch_t* b, std::vector<int> ids) {
+  Group g;
+  g.family = wfa::kLarge;
+  int pmax = 0, tmax = 0;
+  for (int i : ids) { pmax = std::max(pmax, b->plen[i]); tmax = std::max(tmax, b->tlen[i]); }
+  g.wcap = pmax + tmax + 8;
+  g.lmax_words = 0;
+  g.block = wfa::pick_block(wfa::kLarge, pmax + tmax);
+  g.ids = std::move(ids);
+  return g;
+}
+
+}  // namespace
+
+extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_batch_t* b) {
+  if (!b) return WF_STATUS_OOM;
+  const int sup = check_supported(wf);
+  if (sup != WF_STATUS_OK) return sup;
+  Engine* e = engine_of(wf);
+  if (!e || !engine_init(e)) return WF_STATUS_OOM;
+  const bool two = wf->penalties.distance_metric == gap_affine_2p;
+  const Penalties pen = make_penalties(wf->penalties, two);
+  const int ncomp = two ? 5 : 3;
+  const int records = (wf->alignment_scope == compute_alignment) ? 1 : 0;
+  b->records = records;
+  b->stats = wfa_run_stats_t{};
+  b->have_results = false;
+  if (records && !b->d_ops.ensure(std::max<int64_t>(b->total_ops, 1))) {
+    fprintf(stderr, "[WFA] cannot allocate %lld bytes of CIGAR output\n", (long long)b->total_ops);
+    return WF_STATUS_OOM;
+  }
+  b->ops_on_device = records != 0;
+  // Plan: short pairs on the LDS family, the rest on the HBM-ring family.
+  const int wcap_small = small_wcap(pen, ncomp);
+  const int spw = 64 / b->bits;
+  std::vector<int> small_ids, large_ids;
+  int lmax_small = 0;
+  for (int i : b->sorted) {
+    const int L = std::max(b->plen[i], b->tlen[i]);
+    if (wcap_small >= 64 && L <= 1024) {
+      small_ids.push_back(i);
+      lmax_small = std::max<int>(lmax_small, L / spw + 2);
+    } else {
+      large_ids.push_back(i);
+    }
+  }
+  std::vector<Group> groups;
+  if (!small_ids.empty()) {
+    Group g;
+    g.family = wfa::kSmall;
+    g.block = 64;
+    g.wcap = wcap_small;
+    g.lmax_words = lmax_small;
+    g.ids = std::move(small_ids);
+    groups.push_back(std::move(g));
+  }
+  // Split the long pairs by workgroup size.
+  {
+    std::vector<int> cls[3];
+    for (int i : large_ids) {
+      const int w = b->plen[i] + b->tlen[i];
+      cls[w <= 256 ? 0 : (w <= 4096 ? 1 : 2)].push_back(i);
+    }
+    for (auto& c : cls)
+      if (!c.empty()) groups.push_back(large_group(b, std::move(c)));
+  }
+  b->status.assign(b->n, wfa::kDevArenaFull);
+  b->score.assign(b->n, 0);
+  b->ops_begin.assign(b->n, 0);
+  b->dsteps.assign(b->n, 0);
+  float fwd_ms = 0, bt_ms = 0;
+  int launches = 0;
+  std::vector<int> st_tmp, sc_tmp, ob_tmp;
+  std::vector<long long> ds_tmp;
+  int guard = 0;
+  while (!groups.empty()) {
+    if (++guard > 64) {
+      fprintf(stderr, "[WFA] batch did not converge (record arena too small?)\n");
+      return WF_STATUS_OOM;
+    }
+    Group grp = std::move(groups.front());
+    groups.erase(groups.begin());
+    const int rc = run_group(wf, e, b, grp, two, pen, records, &fwd_ms, &bt_ms);
+    ++launches;
+    if (rc != WF_STATUS_OK) return rc;
+    // Collect this group's results.
+    hipStream_t st = e->stream;
+    bool ok = true;
+    st_tmp.resize(b->n);
+    sc_tmp.resize(b->n);
+    ob_tmp.resize(b->n);
+    ds_tmp.resize(b->n);
+    ok = hip_ok(hipMemcpyAsync(st_tmp.data(), b->d_status.p, b->n * 4, hipMemcpyDeviceToHost, st), "download") &&
+         hip_ok(hipMemcpyAsync(sc_tmp.data(), b->d_score.p, b->n * 4, hipMemcpyDeviceToHost, st), "download") &&
+         hip_ok(hipMemcpyAsync(ob_tmp.data(), b->d_ops_begin.p, b->n * 4, hipMemcpyDeviceToHost, st), "download") &&
+         hip_ok(hipMemcpyAsync(ds_tmp.data(), b->d_dsteps.p, b->n * 8, hipMemcpyDeviceToHost, st), "download") &&
+         hip_ok(hipStreamSynchronize(st), "download sync");
+    if (!ok) return WF_STATUS_OOM;
+    std::vector<int> ring_over, arena_full;
+    for (int i : grp.ids) {
+      b->status[i] = st_tmp[i];
+      b->score[i] = sc_tmp[i];
+      b->ops_begin[i] = ob_tmp[i];
+      b->dsteps[i] = ds_tmp[i];
+      if (st_tmp[i] == wfa::kDevRingOverflow) ring_over.push_back(i);
+      else if (st_tmp[i] == wfa::kDevArenaFull) arena_full.push_back(i);
+    }
+    if (!ring_over.empty()) groups.push_back(large_group(b, std::move(ring_over)));
+    if (!arena_full.empty()) {
+      if (arena_full.size() == grp.ids.size()) {
+        // Not even one pair fitted: grow the arena before retrying.
+        size_t free_b = 0, total_b = 0;
+        (void)hipMemGetInfo(&free_b, &total_b);
+        const size_t want = e->rec_bytes * 2;
+        if (want > e->rec.cap + free_b / 2) {
+          fprintf(stderr, "[WFA] record arena exhausted (%zu MB)\n", e->rec_bytes >> 20);
+          for (int i : arena_full) b->status[i] = WF_STATUS_OOM;
+          arena_full.clear();
+        } else {
+          e->rec.release();
+          e->rec_bytes = want;
+        }
+      }
+      if (!arena_full.empty()) {
+        Group g = grp;
+        g.ids = std::move(arena_full);
+        groups.insert(groups.begin(), std::move(g));
+      }
+    }
+  }
+  int64_t dsum = 0;
+  for (long long d : b->dsteps) dsum += d;
+  b->stats.diagonal_steps = dsum;
+  b->stats.forward_ms = fwd_ms;
+  b->stats.backtrace_ms = bt_ms;
+  b->stats.forward_launches = launches;
+  b->have_results = true;
+  int first_fail = WF_STATUS_ALG_COMPLETED;
+  for (int i = 0; i < b->n; ++i) {
+    if (b->status[i] == wfa::kDevBadTrace) {
+      fprintf(stderr, "[WFA] internal error: inconsistent backtrace for pair %d\n", i);
+      b->status[i] = WF_STATUS_UNATTAINABLE;
+    }
+    if (b->status[i] != WF_STATUS_ALG_COMPLETED && first_fail == WF_STATUS_ALG_COMPLETED)
+      first_fail = b->status[i];
+  }
+  return first_fail;
+}
+
+extern "C" int wfa_device_batch_stats(wfa_device_batch_t* b, wfa_run_stats_t* stats) {
+  if (!b || !stats || !b->have_results) return -1;
+  *stats = b->stats;
+  return 0;
+}
+
+extern "C" int wfa_device_batch_download(wavefront_aligner_t* const wf, wfa_device_batch_t* b,
+                                         wfa_result_t* results, char* cigar_arena,
+                                         size_t cigar_arena_bytes) {
+  if (!b || !b->have_results) return WF_STATUS_UNATTAINABLE;
+  Engine* e = engine_of(wf);
+  const bool want_ops = cigar_arena != nullptr && b->records && b->ops_on_device;
+  if (want_ops) {
+    if (cigar_arena_bytes < (size_t)b->total_ops) {
+      fprintf(stderr, "[WFA] cigar arena too small (%zu < %lld bytes)\n", cigar_arena_bytes,
+              (long long)b->total_ops);
+      return WF_STATUS_OOM;
+    }
+    if (b->total_ops > 0 &&
+        !hip_ok(hipMemcpyAsync(cigar_arena, b->d_ops.p, b->total_ops, hipMemcpyDeviceToHost, e->stream), "cigar download"))
+      return WF_STATUS_OOM;
+    if (!hip_ok(hipStreamSynchronize(e->stream), "cigar download")) return WF_STATUS_OOM;
+  }
+  int first_fail = WF_STATUS_ALG_COMPLETED;
+  for (int i = 0; i < b->n; ++i) {
+    wfa_result_t& r = results[i];
+    const int st = b->status[i];
+    r.status = st;
+    r.end_v = b->plen[i];
+    r.end_h = b->tlen[i];
+    r.cigar_offset = b->ops_off[i];
+    r.cigar_length = 0;
+    if (st == WF_STATUS_ALG_COMPLETED) {
+      r.score = classic_score(wf->penalties, b->plen[i], b->tlen[i], b->score[i]);
+      if (b->records) {
+        r.cigar_offset = b->ops_off[i] + b->ops_begin[i];
+        r.cigar_length = b->plen[i] + b->tlen[i] - b->ops_begin[i];
+      }
+    } else if (st == WF_STATUS_MAX_STEPS_REACHED) {
+      r.score = -wf->system.max_alignment_steps;  // wavefront_unialign.c:102-107
+    } else {
+      r.score = INT32_MIN;
+    }
+    if (st != WF_STATUS_ALG_COMPLETED && first_fail == WF_STATUS_ALG_COMPLETED) first_fail = st;
+  }
+  return first_fail;
+}
+
+extern "C" int wavefront_align_batch(wavefront_aligner_t* const wf, const wfa_pair_t* pairs, int num_pairs,
+                                     wfa_result_t* results, char* cigar_arena, size_t cigar_arena_bytes) {
+  const int sup = check_supported(wf);
+  if (sup != WF_STATUS_OK) {
+    for (int i = 0; i < num_pairs; ++i) {
+      results[i] = wfa_result_t{sup, INT32_MIN, -1, -1, 0, 0};
+    }
+    return sup;
+  }
+  if (num_pairs <= 0) return WF_STATUS_ALG_COMPLETED;
+  if (wf->alignment_scope == compute_alignment &&
+      (cigar_arena == nullptr || cigar_arena_bytes < wavefront_align_batch_arena_bytes(pairs, num_pairs))) {
+    fprintf(stderr, "[WFA] wavefront_align_batch: cigar_arena must hold %zu bytes\n",
+            wavefront_align_batch_arena_bytes(pairs, num_pairs));
+    return WF_STATUS_OOM;
+  }
+  wfa_device_batch_t* b = wfa_device_batch_upload(wf, pairs, num_pairs);
+  if (!b) return WF_STATUS_OOM;
+  int rc = wfa_device_batch_run(wf, b);
+  if (rc == WF_STATUS_OOM && !b->have_results) { wfa_device_batch_free(b); return rc; }
+  rc = wfa_device_batch_download(wf, b, results, cigar_arena, cigar_arena_bytes);
+  wfa_device_batch_free(b);
+  return rc;
+}
+
+// ------------------------------------------------------------ single pair
+
+namespace {
+
+void remember_sequences(wavefront_aligner_t* wf, const char* p, int plen, const char* t, int tlen) {
+  free(wf->sequences.pattern);
+  free(wf->sequences.text);
+  wf->sequences.pattern = static_cast<char*>(malloc(plen + 1));
+  wf->sequences.text = static_cast<char*>(malloc(tlen + 1));
+  memcpy(wf->sequences.pattern, p, plen);
+  wf->sequences.pattern[plen] = '\0';
+  memcpy(wf->sequences.text, t, tlen);
+  wf->sequences.text[tlen] = '\0';
+  wf->sequences.pattern_length = plen;
+  wf->sequences.text_length = tlen;
+}
+
+int align_one(wavefront_aligner_t* wf) {
+  cigar_t* cigar = wf->cigar;
+  const int plen = wf->sequences.pattern_length, tlen = wf->sequences.text_length;
+  cigar->begin_offset = cigar->end_offset = 0;
+  cigar->score = INT32_MIN;
+  cigar->end_v = cigar->end_h = -1;
+  wf->align_status.dropped = false;
+  const int sup = check_supported(wf);
+  if (sup != WF_STATUS_OK) {
+    wf->align_status.status = sup;
+    return sup;
+  }
+  wfa_pair_t pr{wf->sequences.pattern, plen, wf->sequences.text, tlen};
+  const bool cig = wf->alignment_scope == compute_alignment;
+  if (cig && cigar->max_operations < plen + tlen + 1) {
+    free(cigar->operations);
+    cigar->max_operations = 2 * (plen + tlen) + 1;
+    cigar->operations = static_cast<char*>(malloc(cigar->max_operations));
+  }
+  wfa_result_t r;
+  const int rc = wavefront_align_batch(wf, &pr, 1, &r, cig ? cigar->operations : nullptr,
+                                       cig ? (size_t)cigar->max_operations : 0);
+  (void)rc;
+  wf->align_status.status = r.status;
+  wf->align_status.score = r.score;
+  if (r.status == WF_STATUS_ALG_COMPLETED || r.status == WF_STATUS_MAX_STEPS_REACHED) {
+    cigar->score = r.score;
+  }
+  if (r.status == WF_STATUS_ALG_COMPLETED) {
+    cigar->end_v = r.end_v;
+    cigar->end_h = r.end_h;
+    if (cig) {
+      cigar->begin_offset = (int)r.cigar_offset;
+      cigar->end_offset = (int)r.cigar_offset + r.cigar_length;
+    }
+  }
+  return r.status;
+}
+
+}  // namespace
+
+extern "C" int wavefront_align(wavefront_aligner_t* const wf, const char* const pattern, const int plen,
+                               const char* const text, const int tlen) {
+  remember_sequences(wf, pattern, plen, text, tlen);
+  return align_one(wf);
+}
+
+extern "C" int wavefront_align_packed2bits(wavefront_aligner_t* const wf, const uint8_t* const pattern,
+                                           const int plen, const uint8_t* const text, const int tlen) {
+  // Decodes every base (the layout of wavefront_sequences.c:102-140).
+  static const char dec[4] = {'A', 'C', 'G', 'T'};
+  std::vector<char> p(plen), t(tlen);
+  for (int i = 0; i < plen; ++i) p[i] = dec[(pattern[i >> 2] >> (2 * (i & 3))) & 3];
+  for (int i = 0; i < tlen; ++i) t[i] = dec[(text[i >> 2] >> (2 * (i & 3))) & 3];
+  remember_sequences(wf, p.data(), plen, t.data(), tlen);
+  return align_one(wf);
+}
+
+extern "C" int wavefront_align_lambda(wavefront_aligner_t* const wf, alignment_match_funct_t const,
+                                      void*, const int, const int) {
+  fprintf(stderr, "[WFA] Unsupported configuration on the device path: lambda sequences\n");
+  wf->align_status.status = WF_STATUS_UNSUPPORTED;
+  return WF_STATUS_UNSUPPORTED;
+}
+
+// wavefront_align.c:245-266: continue a pair paused by the step limit.  The
+// device recomputes it from score 0 under the current limits, which yields
+// the same result as continuing.
+extern "C" int wavefront_align_resume(wavefront_aligner_t* const wf) {
+  if (wf->align_status.status != WF_STATUS_MAX_STEPS_REACHED || wf->memory_mode == wavefront_memory_ultralow) {
+    fprintf(stderr, "[WFA] Alignment cannot be resumed\n");
+    exit(1);
+  }
+  return align_one(wf);
+}

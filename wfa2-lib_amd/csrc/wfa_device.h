@@ -1,0 +1,113 @@
+// Internal interface between the C-ABI host layer (wfa_api.cpp) and the HIP
+// kernels (wfa_kernels.hip).  Not installed; no torch types anywhere.
+#pragma once
+#include <stdint.h>
+#include <hip/hip_runtime.h>
+
+namespace wfa {
+
+constexpr int32_t kOffsetNull = INT32_MIN / 2;  // WAVEFRONT_OFFSET_NULL
+
+// Component indices (M, I1, D1, I2, D2), as affine2p_matrix_type order-free ids.
+enum Comp : int { kM = 0, kI1 = 1, kD1 = 2, kI2 = 3, kD2 = 4 };
+
+// Per-pair device status codes.  0 / -100 mirror WF_STATUS_ALG_COMPLETED /
+// WF_STATUS_MAX_STEPS_REACHED; the others are internal and make the host
+// re-run the pair with a larger configuration.
+enum DevStatus : int {
+  kDevCompleted = 0,
+  kDevMaxSteps = -100,
+  kDevRingOverflow = -2001,   // a wavefront wider than the launch's ring capacity
+  kDevArenaFull = -2002,      // the backtrace-record arena ran out
+  kDevBadTrace = -2003,       // backtrace found an inconsistent record (bug guard)
+};
+
+// Internal (Eizenga-transformed) penalties plus derived scalars.
+struct Penalties {
+  int x, o1, e1, o2, e2;   // o2/e2 unused for gap-affine
+  int scope;               // max_score_scope
+  int g;                   // gcd of all score increments: only multiples of g can be non-null
+  int nslots;              // ring slots = (scope-1)/g + 1
+};
+
+// Record word (one per diagonal of every non-null step):
+//   bits  0..24  M offset before extension (the max of its candidates)
+//   bits 25..27  winning M source: 0 X, 1 D2, 2 D1, 3 I2, 4 I1
+//   bit  28..31  gap-extension flags for I1, D1, I2, D2 at this diagonal
+constexpr uint32_t kRecOffMask = 0x1FFFFFFu;
+constexpr int kRecSrcShift = 25;
+constexpr uint32_t kRecI1Ext = 1u << 28;
+constexpr uint32_t kRecD1Ext = 1u << 29;
+constexpr uint32_t kRecI2Ext = 1u << 30;
+constexpr uint32_t kRecD2Ext = 1u << 31;
+enum RecSrc : uint32_t { kSrcX = 0, kSrcD2 = 1, kSrcD1 = 2, kSrcI2 = 3, kSrcI1 = 4 };
+constexpr uint32_t kNoBlock = 0xFFFFFFFFu;
+constexpr uint32_t kChunkUnits = 4096;   // record chunk grabbed per workgroup, in 16-byte units
+
+struct ForwardArgs {
+  int n_pairs;
+  const int* order;            // processing order (pair ids), longest first
+  const int* plen;
+  const int* tlen;
+  const uint64_t* words;       // packed sequences
+  const int64_t* pat_word;     // first word of each pattern
+  const int64_t* txt_word;     // first word of each text
+  int* out_status;
+  int* out_score;              // wavefront score at the end (s_end)
+  long long* out_dsteps;       // diagonal-steps of the pair (sum of hi-lo+1)
+  uint32_t* out_last_block;    // last record block of the pair, 16-byte units
+  uint4* rec_arena;            // record arena (16-byte units)
+  unsigned long long rec_units;
+  unsigned long long* rec_top; // bump pointer into the arena, 16-byte units
+  int* ring_global;            // per-workgroup ring scratch (global-ring kernels)
+  long long ring_stride;       // ints per workgroup
+  unsigned int* queue;         // work-queue counter
+  int wcap;                    // ring capacity in diagonals per slot
+  int lmax_words;              // LDS capacity per sequence, in 64-bit words
+  int max_steps;               // INT_MAX if unlimited
+  int records;                 // 0 = score only
+  int comp_begin, comp_end;
+  Penalties pen;
+};
+
+struct BacktraceArgs {
+  int n_pairs;
+  const int* order;            // pairs to trace (the forward pass's subset)
+  const int* plen;
+  const int* tlen;
+  int* status;                 // in/out (kDevBadTrace on a corrupt walk)
+  const int* score;
+  const uint32_t* last_block;
+  const uint4* rec_arena;
+  unsigned long long rec_units;
+  char* ops;                   // operations arena
+  const int64_t* ops_off;      // slot of pair i: ops[ops_off[i] .. +plen+tlen)
+  int* ops_begin;              // CIGAR = ops[ops_off + ops_begin .. ops_off + plen + tlen)
+  int comp_begin, comp_end;
+  Penalties pen;
+};
+
+// Kernel families.  kSmall: one 64-lane wavefront per pair, wavefront ring
+// and both sequences in LDS.  kLarge: one workgroup per pair, ring in HBM.
+enum Family : int { kSmall = 0, kLarge = 1 };
+
+struct LaunchShape {
+  int family;
+  int block;        // threads per workgroup
+  int grid;         // persistent workgroups
+  size_t lds_bytes; // dynamic LDS
+};
+
+// Returns hipSuccess or the launch error.  bits: 2 (ACGT packed) or 8 (bytes).
+hipError_t launch_forward(const ForwardArgs& a, int two_piece, int bits,
+                          const LaunchShape& shape, hipStream_t stream);
+hipError_t launch_backtrace(const BacktraceArgs& a, hipStream_t stream);
+hipError_t launch_reset_counters(unsigned int* queue, unsigned long long* rec_top,
+                                 hipStream_t stream);
+
+// Dynamic-LDS bytes and occupancy for a configuration.
+size_t forward_lds_bytes(int family, int ncomp, int nslots, int wcap, int lmax_words);
+int forward_max_blocks_per_cu(int family, int block, int two_piece, int bits, size_t lds_bytes);
+int pick_block(int family, int max_width);
+
+}  // namespace wfa

@@ -1,0 +1,561 @@
+// HIP kernels for the exact WFA hot path on CDNA4 (gfx950).
+//
+// Forward kernel: one pair per workgroup, persistent over a device work queue.
+// Each non-null score step s is one fused pass over the diagonals [lo,hi]:
+// read the s-x / s-o-e / s-e wavefronts from a ring of slots, compute I1, D1,
+// (I2, D2) and M (wavefront_compute_affine.c:44-86 and
+// wavefront_compute_affine2p.c:45-106), extend M along exact matches over the
+// 2-bit packed sequences (wavefront_extend_kernels.c:68-123), write the step's
+// backtrace records, then trim the ends with wave ballots
+// (wavefront_compute.c:579-613) and test termination
+// (wavefront_termination.c:37-114).
+//
+// Records replace the reference's stored wavefronts (memory mode high).  The
+// reference backtrace (wavefront_backtrace.c:320-529) takes, at every cell it
+// visits, the maximum of (offset << 4 | type) over the candidate sources; the
+// candidate offsets it probes are exactly the values the forward pass read to
+// compute that cell, so the winner (and its tie-break M > D2 > D1 > I2 > I1,
+// extension before opening) is decided here once, and stored as 7 bits next
+// to the cell's pre-extension M offset.  The backtrace kernel replays them.
+#include "wfa_device.h"
+#include <climits>
+
+namespace wfa {
+namespace {
+
+struct SlotMeta {      // 64 bytes, one per ring slot
+  int tag;             // score held by the slot, -1 if none
+  int alo;             // lo of the computed range: array origin of all components
+  int lo[5];           // trimmed ranges
+  int hi[5];
+  unsigned flags;      // bit c: component present; bit 8+c: component null (trimmed empty)
+  int pad[3];
+};
+static_assert(sizeof(SlotMeta) == 64, "SlotMeta layout");
+
+// A compute input: reads at k in [lo,hi] return ring[off + k], else NULL.
+// The shared null wavefront of the reference has lo = 1, hi = -1.
+struct Src {
+  int lo, hi, off;
+  __device__ __forceinline__ bool valid() const { return lo <= hi; }
+};
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ Src get_src(const SlotMeta* meta, int s_src, int slot, int R, int c,
+                                       int ncomp, int wcap) {
+  Src r{1, -1, 0};
+  if (s_src < 0) return r;
+  if (slot < 0) slot += R;
+  const int tag = uniform(meta[slot].tag);
+  const unsigned fl = (unsigned)uniform((int)meta[slot].flags);
+  if (tag != s_src) return r;
+  if (!(fl & (1u << c)) || (fl & (0x100u << c))) return r;
+  r.lo = uniform(meta[slot].lo[c]);
+  r.hi = uniform(meta[slot].hi[c]);
+  r.off = (slot * ncomp + c) * wcap - uniform(meta[slot].alo);
+  return r;
+}
+
+template <typename RingPtr>
+__device__ __forceinline__ int rd(RingPtr ring, const Src& s, int k) {
+  return (k >= s.lo && k <= s.hi) ? ring[s.off + k] : kOffsetNull;
+}
+
+__device__ __forceinline__ bool in_bounds(int off, int k, int plen, int tlen) {
+  return (unsigned)off <= (unsigned)tlen && (unsigned)(off - k) <= (unsigned)plen;
+}
+
+// 64-bit window of symbols starting at symbol `pos` (BITS bits per symbol,
+// little-endian inside each word).  Reads words w and w+1: every packed
+// sequence carries at least one padding word.
+template <int BITS, typename WordPtr>
+__device__ __forceinline__ uint64_t window(WordPtr W, int pos) {
+  constexpr int SPW = 64 / BITS;
+  const int w = pos / SPW;
+  const int sh = (pos % SPW) * BITS;
+  const uint64_t lo = W[w];
+  const uint64_t hi = W[w + 1];
+  return sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
+}
+
+// Longest common extension of pattern[v..] and text[h..], bounded by both ends.
+template <int BITS, typename WordPtr>
+__device__ __forceinline__ int lcp_extend(WordPtr P, WordPtr T, int plen, int tlen, int k, int h) {
+  constexpr int SPW = 64 / BITS;
+  int v = h - k;
+  for (;;) {
+    const int rem = min(plen - v, tlen - h);
+    if (rem <= 0) return h;
+    const uint64_t x = window<BITS>(P, v) ^ window<BITS>(T, h);
+    const int n = x ? (int)(__builtin_ctzll(x) / BITS) : SPW;
+    if (n >= rem) return h + rem;
+    h += n;
+    v += n;
+    if (n < SPW) return h;
+  }
+}
+
+struct WaveRange {  // first/last valid diagonal seen by one wave (uniform)
+  int first, last;
+  __device__ __forceinline__ void init() { first = INT_MAX; last = INT_MIN; }
+  __device__ __forceinline__ void add(unsigned long long ballot, int k_lane0) {
+    if (ballot) {
+      if (first == INT_MAX) first = k_lane0 + (int)__builtin_ctzll(ballot);
+      last = k_lane0 + 63 - (int)__builtin_clzll(ballot);
+    }
+  }
+  __device__ __forceinline__ void merge(int f, int l) { first = min(first, f); last = max(last, l); }
+  __device__ __forceinline__ void store(SlotMeta& m, int c, int lo, unsigned& fl) const {
+    fl |= 1u << c;
+    if (first == INT_MAX) { fl |= 0x100u << c; m.lo[c] = lo; m.hi[c] = lo - 1; }
+    else { m.lo[c] = first; m.hi[c] = last; }
+  }
+};
+
+template <int NC, int BLOCK, bool SMALL, int BITS>
+__global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NW = BLOCK / 64;
+  constexpr int SPW = 64 / BITS;
+  __shared__ int sh_pair;
+  __shared__ unsigned long long sh_chunk;
+  __shared__ int sh_red[NW > 1 ? NW : 1][2 * NC];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wid = tid >> 6;
+  const Penalties pen = a.pen;
+  const int R = pen.nslots;
+  const int wcap = a.wcap;
+  const int g = pen.g;
+  const int dq_x = pen.x / g, dq_o1 = (pen.o1 + pen.e1) / g, dq_e1 = pen.e1 / g;
+  const int dq_o2 = (NC == 5) ? (pen.o2 + pen.e2) / g : 0;
+  const int dq_e2 = (NC == 5) ? pen.e2 / g : 0;
+
+  SlotMeta* meta = reinterpret_cast<SlotMeta*>(smem);
+  uint64_t* seqP = reinterpret_cast<uint64_t*>(smem + (size_t)R * sizeof(SlotMeta));
+  uint64_t* seqT = seqP + a.lmax_words;
+  int* ring;
+  if constexpr (SMALL) {
+    ring = reinterpret_cast<int*>(seqT + a.lmax_words);
+  } else {
+    ring = a.ring_global + (long long)blockIdx.x * a.ring_stride;
+  }
+
+  unsigned long long chunk_cur = 0, chunk_end = 0;  // record chunk of this workgroup
+
+  for (;;) {
+    if (tid == 0) sh_pair = (int)atomicAdd(a.queue, 1u);
+    __syncthreads();
+    const int qi = sh_pair;
+    __syncthreads();
+    if (qi >= a.n_pairs) break;
+    const int p = a.order ? a.order[qi] : qi;
+    const int plen = a.plen[p];
+    const int tlen = a.tlen[p];
+    const int kend = tlen - plen;
+    // Generous bound on the end score of any end-to-end alignment; guarantees
+    // that every workgroup leaves the loop even on inconsistent input.
+    const int s_cap = (pen.x + pen.o1 + pen.e1 + (NC == 5 ? pen.o2 + pen.e2 : 0) + 2) * (plen + tlen + 2);
+
+    const uint64_t* P;
+    const uint64_t* T;
+    if constexpr (SMALL) {
+      const uint64_t* gP = a.words + a.pat_word[p];
+      const uint64_t* gT = a.words + a.txt_word[p];
+      const int nwp = plen / SPW + 2, nwt = tlen / SPW + 2;
+      for (int i = tid; i < nwp; i += BLOCK) seqP[i] = gP[i];
+      for (int i = tid; i < nwt; i += BLOCK) seqT[i] = gT[i];
+      P = seqP;
+      T = seqT;
+    } else {
+      P = a.words + a.pat_word[p];
+      T = a.words + a.txt_word[p];
+    }
+    for (int r = tid; r < R; r += BLOCK) meta[r].tag = -1;
+    __syncthreads();
+
+    // Score 0: the begin component holds offset 0 at k = 0 (wavefront_aligner.c:252-380).
+    int cur = 0;
+    int s = 0;
+    int status = 1;  // running
+    uint32_t prev_blk = kNoBlock;
+    long long dsteps = 0;
+    if (tid == 0) {
+      SlotMeta& m = meta[0];
+      m.tag = 0;
+      m.alo = 0;
+      for (int c = 0; c < NC; ++c) { m.lo[c] = 0; m.hi[c] = 0; }
+      m.flags = 1u << a.comp_begin;
+      int v0 = 0;
+      if (a.comp_begin == kM) v0 = lcp_extend<BITS>(P, T, plen, tlen, 0, 0);
+      ring[(0 * NC + a.comp_begin) * wcap] = v0;
+      sh_pair = (a.comp_begin == kM && a.comp_end == kM && kend == 0 && v0 >= tlen) ? 1 : 0;
+    }
+    __syncthreads();
+    if (sh_pair) status = kDevCompleted;
+    __syncthreads();
+
+    while (status == 1) {
+      s += g;
+      if (s >= a.max_steps) { status = kDevMaxSteps; break; }
+      cur = (cur + 1 == R) ? 0 : cur + 1;
+      const Src mx  = get_src(meta, s - pen.x, cur - dq_x, R, kM, NC, wcap);
+      const Src mo1 = get_src(meta, s - pen.o1 - pen.e1, cur - dq_o1, R, kM, NC, wcap);
+      const Src i1  = get_src(meta, s - pen.e1, cur - dq_e1, R, kI1, NC, wcap);
+      const Src d1  = get_src(meta, s - pen.e1, cur - dq_e1, R, kD1, NC, wcap);
+      Src mo2{1, -1, 0}, i2{1, -1, 0}, d2{1, -1, 0};
+      if constexpr (NC == 5) {
+        mo2 = get_src(meta, s - pen.o2 - pen.e2, cur - dq_o2, R, kM, NC, wcap);
+        i2  = get_src(meta, s - pen.e2, cur - dq_e2, R, kI2, NC, wcap);
+        d2  = get_src(meta, s - pen.e2, cur - dq_e2, R, kD2, NC, wcap);
+      }
+      const bool any = mx.valid() || mo1.valid() || i1.valid() || d1.valid() ||
+                       mo2.valid() || i2.valid() || d2.valid();
+      if (!any) {  // null step (wavefront_compute_affine.c:236-243)
+        if (s >= s_cap) { status = kDevBadTrace; break; }
+        if (tid == 0) { meta[cur].tag = s; meta[cur].flags = 0; }
+        __syncthreads();
+        continue;
+      }
+      // wavefront_compute_limits_input (wavefront_compute.c:40-86)
+      int lo = mx.lo, hi = mx.hi;
+      lo = min(lo, mo1.lo - 1); hi = max(hi, mo1.hi + 1);
+      lo = min(lo, i1.lo + 1);  hi = max(hi, i1.hi + 1);
+      lo = min(lo, d1.lo - 1);  hi = max(hi, d1.hi - 1);
+      if constexpr (NC == 5) {
+        lo = min(lo, mo2.lo - 1); hi = max(hi, mo2.hi + 1);
+        lo = min(lo, i2.lo + 1);  hi = max(hi, i2.hi + 1);
+        lo = min(lo, d2.lo - 1);  hi = max(hi, d2.hi - 1);
+      }
+      lo = uniform(lo);
+      hi = uniform(hi);
+      const int width = hi - lo + 1;
+      if (width > wcap) { status = kDevRingOverflow; break; }
+      dsteps += width;
+      // outputs present (wavefront_compute.c:446-493)
+      const bool has_i1 = mo1.valid() || i1.valid();
+      const bool has_d1 = mo1.valid() || d1.valid();
+      const bool has_i2 = (NC == 5) && (mo2.valid() || i2.valid());
+      const bool has_d2 = (NC == 5) && (mo2.valid() || d2.valid());
+
+      uint32_t* rec = nullptr;
+      if (a.records) {
+        const unsigned long long units = 1 + (unsigned long long)((width + 3) / 4);
+        if (chunk_cur + units > chunk_end) {
+          const unsigned long long want = units > kChunkUnits ? units : kChunkUnits;
+          __syncthreads();
+          if (tid == 0) sh_chunk = atomicAdd(a.rec_top, want);
+          __syncthreads();
+          chunk_cur = sh_chunk;
+          chunk_end = chunk_cur + want;
+          if (chunk_end > a.rec_units) {
+            chunk_cur = chunk_end = 0;
+            status = kDevArenaFull;
+            break;
+          }
+        }
+        const uint32_t blk = (uint32_t)chunk_cur;
+        chunk_cur += units;
+        if (tid == 0) a.rec_arena[blk] = make_uint4((unsigned)s, (unsigned)lo, (unsigned)hi, prev_blk);
+        prev_blk = blk;
+        rec = reinterpret_cast<uint32_t*>(a.rec_arena + blk + 1);
+      }
+
+      int* outM  = ring + (cur * NC + kM) * wcap - lo;
+      int* outI1 = ring + (cur * NC + kI1) * wcap - lo;
+      int* outD1 = ring + (cur * NC + kD1) * wcap - lo;
+      int* outI2 = ring + (cur * NC + (NC == 5 ? kI2 : 0)) * wcap - lo;
+      int* outD2 = ring + (cur * NC + (NC == 5 ? kD2 : 0)) * wcap - lo;
+
+      WaveRange rM, rI1, rD1, rI2, rD2;
+      rM.init(); rI1.init(); rD1.init(); rI2.init(); rD2.init();
+      for (int kb = lo; kb <= hi; kb += BLOCK) {
+        const int k = kb + tid;
+        bool vM = false, vI1 = false, vD1 = false, vI2 = false, vD2 = false;
+        if (k <= hi) {
+          const int om_l = rd(ring, mo1, k - 1);
+          const int om_r = rd(ring, mo1, k + 1);
+          const int ei1 = rd(ring, i1, k - 1);
+          const int ed1 = rd(ring, d1, k + 1);
+          const int ins1 = max(om_l, ei1) + 1;
+          const int del1 = max(om_r, ed1);
+          int best = rd(ring, mx, k) + 1;
+          uint32_t src = kSrcX;
+          uint32_t bits = (ei1 >= om_l ? kRecI1Ext : 0u) | (ed1 >= om_r ? kRecD1Ext : 0u);
+          int ins2 = kOffsetNull, del2 = kOffsetNull;
+          if constexpr (NC == 5) {
+            const int o2l = rd(ring, mo2, k - 1);
+            const int o2r = rd(ring, mo2, k + 1);
+            const int ei2 = rd(ring, i2, k - 1);
+            const int ed2 = rd(ring, d2, k + 1);
+            ins2 = max(o2l, ei2) + 1;
+            del2 = max(o2r, ed2);
+            bits |= (ei2 >= o2l ? kRecI2Ext : 0u) | (ed2 >= o2r ? kRecD2Ext : 0u);
+            if (del2 > best) { best = del2; src = kSrcD2; }
+          }
+          // argmax with ties to the higher backtrace type: M > D2 > D1 > I2 > I1
+          if (del1 > best) { best = del1; src = kSrcD1; }
+          if constexpr (NC == 5) { if (ins2 > best) { best = ins2; src = kSrcI2; } }
+          if (ins1 > best) { best = ins1; src = kSrcI1; }
+          int m = best;
+          if (!in_bounds(m, k, plen, tlen)) m = kOffsetNull;  // null-after-max
+          if (rec) rec[k - lo] = ((uint32_t)best & kRecOffMask) | (src << kRecSrcShift) | bits;
+          if (m != kOffsetNull) m = lcp_extend<BITS>(P, T, plen, tlen, k, m);
+          outM[k] = m;
+          vM = (m != kOffsetNull);
+          if (has_i1) { outI1[k] = ins1; vI1 = in_bounds(ins1, k, plen, tlen); }
+          if (has_d1) { outD1[k] = del1; vD1 = in_bounds(del1, k, plen, tlen); }
+          if constexpr (NC == 5) {
+            if (has_i2) { outI2[k] = ins2; vI2 = in_bounds(ins2, k, plen, tlen); }
+            if (has_d2) { outD2[k] = del2; vD2 = in_bounds(del2, k, plen, tlen); }
+          }
+        }
+        const int kw = kb + wid * 64;
+        rM.add(__ballot(vM), kw);
+        rI1.add(__ballot(vI1), kw);
+        rD1.add(__ballot(vD1), kw);
+        if constexpr (NC == 5) {
+          rI2.add(__ballot(vI2), kw);
+          rD2.add(__ballot(vD2), kw);
+        }
+      }
+      if constexpr (NW > 1) {
+        if (lane == 0) {
+          sh_red[wid][0] = rM.first;  sh_red[wid][1] = rM.last;
+          sh_red[wid][2] = rI1.first; sh_red[wid][3] = rI1.last;
+          sh_red[wid][4] = rD1.first; sh_red[wid][5] = rD1.last;
+          if constexpr (NC == 5) {
+            sh_red[wid][6] = rI2.first; sh_red[wid][7] = rI2.last;
+            sh_red[wid][8] = rD2.first; sh_red[wid][9] = rD2.last;
+          }
+        }
+        __syncthreads();
+        rM.init(); rI1.init(); rD1.init(); rI2.init(); rD2.init();
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+          rM.merge(sh_red[w][0], sh_red[w][1]);
+          rI1.merge(sh_red[w][2], sh_red[w][3]);
+          rD1.merge(sh_red[w][4], sh_red[w][5]);
+          if constexpr (NC == 5) {
+            rI2.merge(sh_red[w][6], sh_red[w][7]);
+            rD2.merge(sh_red[w][8], sh_red[w][9]);
+          }
+        }
+      }
+      __syncthreads();  // every lane is done with the ring and sh_red of this step
+      // Trimmed ranges: [first valid, last valid], or null (lo, lo-1).
+      if (tid == 0) {
+        SlotMeta& m = meta[cur];
+        m.tag = s;
+        m.alo = lo;
+        unsigned fl = 0;
+        rM.store(m, kM, lo, fl);
+        if (has_i1) rI1.store(m, kI1, lo, fl);
+        if (has_d1) rD1.store(m, kD1, lo, fl);
+        if constexpr (NC == 5) {
+          if (has_i2) rI2.store(m, kI2, lo, fl);
+          if (has_d2) rD2.store(m, kD2, lo, fl);
+        }
+        m.flags = fl;
+      }
+      // Termination (wavefront_termination.c:37-114): k_end inside the trimmed
+      // range of the end component and its offset reaching the text end.
+      {
+        const int ce = a.comp_end;
+        bool pce = true;
+        WaveRange rce = rM;
+        if (ce == kI1) { pce = has_i1; rce = rI1; }
+        if (ce == kD1) { pce = has_d1; rce = rD1; }
+        if constexpr (NC == 5) {
+          if (ce == kI2) { pce = has_i2; rce = rI2; }
+          if (ce == kD2) { pce = has_d2; rce = rD2; }
+        }
+        if (pce && rce.first != INT_MAX && kend >= rce.first && kend <= rce.last) {
+          const int vend = ring[(cur * NC + ce) * wcap + (kend - lo)];
+          if (vend >= tlen) status = kDevCompleted;
+        }
+      }
+      if (status == 1 && s >= s_cap) status = kDevBadTrace;  // unreachable for valid input
+      __syncthreads();
+    }
+    if (tid == 0) {
+      a.out_status[p] = (status == 1) ? kDevCompleted : status;
+      a.out_score[p] = s;
+      a.out_last_block[p] = prev_blk;
+      a.out_dsteps[p] = dsteps;
+    }
+    __syncthreads();
+  }
+}
+
+// One lane per pair: replays the records from the end cell back to the origin
+// (wavefront_backtrace.c:320-529) and writes the operations backwards into the
+// pair's slot, like cigar->operations[begin_offset..end_offset).
+__global__ void __launch_bounds__(256) wfa_backtrace_kernel(BacktraceArgs a) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.n_pairs) return;
+  const int p = a.order ? a.order[q] : q;
+  const int plen = a.plen[p], tlen = a.tlen[p];
+  const int cap = plen + tlen;
+  char* out = a.ops + a.ops_off[p];
+  if (a.status[p] != kDevCompleted) { a.ops_begin[p] = cap; return; }
+  const Penalties pen = a.pen;
+  int comp = a.comp_end;
+  int s = a.score[p];
+  int k = tlen - plen;
+  int off = tlen;
+  int h = off, v = off - k;
+  int pos = cap;
+  bool bad = false;
+  if (comp == kM) {
+    for (int i = plen - v; i > 0; --i) out[--pos] = 'D';
+    for (int i = tlen - h; i > 0; --i) out[--pos] = 'I';
+  }
+  uint32_t blk = a.last_block[p];
+  uint4 hdr = make_uint4(0, 0, 0, kNoBlock);
+  if (blk != kNoBlock && blk < a.rec_units) hdr = a.rec_arena[blk];
+  while (v > 0 && h > 0 && s > 0) {
+    while (blk != kNoBlock && blk < a.rec_units && (int)hdr.x > s) {
+      blk = hdr.w;
+      if (blk != kNoBlock && blk < a.rec_units) hdr = a.rec_arena[blk];
+    }
+    if (blk == kNoBlock || blk >= a.rec_units || (int)hdr.x != s || k < (int)hdr.y || k > (int)hdr.z) {
+      bad = true;
+      break;
+    }
+    const uint32_t rec = reinterpret_cast<const uint32_t*>(a.rec_arena + blk + 1)[k - (int)hdr.y];
+    if (comp == kM) {
+      const int mpre = (int)(rec & kRecOffMask);
+      const int nm = off - mpre;
+      if (nm < 0 || nm >= pos) { bad = true; break; }
+      for (int n = nm; n > 0; --n) out[--pos] = 'M';
+      off = mpre;
+      v = off - k;
+      h = off;
+      if (v <= 0 || h <= 0) break;
+      switch ((rec >> kRecSrcShift) & 7u) {
+        case kSrcX:  s -= pen.x; out[--pos] = 'X'; --off; break;
+        case kSrcD2:
+          if (rec & kRecD2Ext) { s -= pen.e2; comp = kD2; } else { s -= pen.o2 + pen.e2; }
+          out[--pos] = 'D'; ++k; break;
+        case kSrcD1:
+          if (rec & kRecD1Ext) { s -= pen.e1; comp = kD1; } else { s -= pen.o1 + pen.e1; }
+          out[--pos] = 'D'; ++k; break;
+        case kSrcI2:
+          if (rec & kRecI2Ext) { s -= pen.e2; comp = kI2; } else { s -= pen.o2 + pen.e2; }
+          out[--pos] = 'I'; --k; --off; break;
+        default:
+          if (rec & kRecI1Ext) { s -= pen.e1; comp = kI1; } else { s -= pen.o1 + pen.e1; }
+          out[--pos] = 'I'; --k; --off; break;
+      }
+    } else if (comp == kI1 || comp == kI2) {
+      const bool ext = (comp == kI1) ? (rec & kRecI1Ext) : (rec & kRecI2Ext);
+      const int e = (comp == kI1) ? pen.e1 : pen.e2;
+      const int o = (comp == kI1) ? pen.o1 : pen.o2;
+      if (ext) { s -= e; } else { s -= o + e; comp = kM; }
+      out[--pos] = 'I'; --k; --off;
+    } else {
+      const bool ext = (comp == kD1) ? (rec & kRecD1Ext) : (rec & kRecD2Ext);
+      const int e = (comp == kD1) ? pen.e1 : pen.e2;
+      const int o = (comp == kD1) ? pen.o1 : pen.o2;
+      if (ext) { s -= e; } else { s -= o + e; comp = kM; }
+      out[--pos] = 'D'; ++k;
+    }
+    v = off - k;
+    h = off;
+    if (pos <= 0 && (v > 0 || h > 0)) { bad = true; break; }
+  }
+  if (!bad && comp == kM) {
+    if (v > 0 && h > 0) {
+      int n = min(v, h);
+      v -= n;
+      h -= n;
+      if (n > pos) n = pos, bad = true;
+      while (n-- > 0) out[--pos] = 'M';
+    }
+    if (v + h > pos) bad = true;
+    else {
+      while (v > 0) { out[--pos] = 'D'; --v; }
+      while (h > 0) { out[--pos] = 'I'; --h; }
+    }
+  }
+  if (bad) { a.status[p] = kDevBadTrace; pos = cap; }
+  a.ops_begin[p] = pos;
+}
+
+__global__ void wfa_reset_kernel(unsigned int* queue, unsigned long long* rec_top) {
+  if (threadIdx.x == 0) { *queue = 0u; *rec_top = 0ull; }
+}
+
+template <int NC, int BLOCK, bool SMALL, int BITS>
+const void* forward_fn() {
+  return reinterpret_cast<const void*>(&wfa_forward_kernel<NC, BLOCK, SMALL, BITS>);
+}
+
+// Every instance the host can select.
+const void* select_forward(int family, int two_piece, int bits, int block) {
+#define WFA_SEL(NC, BL, SM, BI)                                                        \
+  if ((two_piece ? 5 : 3) == NC && block == BL && (family == kSmall) == SM && bits == BI) \
+    return forward_fn<NC, BL, SM, BI>();
+  WFA_SEL(3, 64, true, 2) WFA_SEL(3, 64, true, 8) WFA_SEL(5, 64, true, 2) WFA_SEL(5, 64, true, 8)
+  WFA_SEL(3, 64, false, 2) WFA_SEL(3, 64, false, 8) WFA_SEL(5, 64, false, 2) WFA_SEL(5, 64, false, 8)
+  WFA_SEL(3, 256, false, 2) WFA_SEL(3, 256, false, 8) WFA_SEL(5, 256, false, 2) WFA_SEL(5, 256, false, 8)
+  WFA_SEL(3, 1024, false, 2) WFA_SEL(3, 1024, false, 8) WFA_SEL(5, 1024, false, 2) WFA_SEL(5, 1024, false, 8)
+#undef WFA_SEL
+  return nullptr;
+}
+
+}  // namespace
+
+size_t forward_lds_bytes(int family, int ncomp, int nslots, int wcap, int lmax_words) {
+  size_t b = (size_t)nslots * sizeof(SlotMeta);
+  if (family == kSmall) {
+    b += (size_t)2 * lmax_words * sizeof(uint64_t);
+    b += (size_t)nslots * ncomp * wcap * sizeof(int);
+  }
+  return b;
+}
+
+int pick_block(int family, int max_width) {
+  if (family == kSmall) return 64;
+  if (max_width <= 256) return 64;
+  if (max_width <= 4096) return 256;
+  return 1024;
+}
+
+int forward_max_blocks_per_cu(int family, int block, int two_piece, int bits, size_t lds_bytes) {
+  const void* f = select_forward(family, two_piece, bits, block);
+  if (!f) return 0;
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, lds_bytes) != hipSuccess) return 0;
+  return n;
+}
+
+hipError_t launch_forward(const ForwardArgs& a, int two_piece, int bits, const LaunchShape& shape,
+                          hipStream_t stream) {
+  const void* f = select_forward(shape.family, two_piece, bits, shape.block);
+  if (!f) return hipErrorInvalidValue;
+  if (shape.lds_bytes > 48 * 1024) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shape.lds_bytes);
+    if (e != hipSuccess) return e;
+  }
+  void* args[] = {const_cast<ForwardArgs*>(&a)};
+  return hipLaunchKernel(f, dim3(shape.grid), dim3(shape.block), args, shape.lds_bytes, stream);
+}
+
+hipError_t launch_backtrace(const BacktraceArgs& a, hipStream_t stream) {
+  if (a.n_pairs <= 0) return hipSuccess;
+  const int block = 256;
+  const int grid = (a.n_pairs + block - 1) / block;
+  hipLaunchKernelGGL(wfa_backtrace_kernel, dim3(grid), dim3(block), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_reset_counters(unsigned int* queue, unsigned long long* rec_top, hipStream_t stream) {
+  hipLaunchKernelGGL(wfa_reset_kernel, dim3(1), dim3(64), 0, stream, queue, rec_top);
+  return hipGetLastError();
+}
+
+}  // namespace wfa

@@ -1,0 +1,352 @@
+"""Python host bindings of libwfa2_amd.so (the MI355X exact-WFA C ABI).
+
+This is a thin ctypes mirror of ``include/wfa_mi355x.h`` used by the tests,
+the benchmark and the serving loop.  It loads only the in-tree library and
+fails loudly when it is missing: there is no CPU fallback.
+
+    al = Aligner(metric="affine", penalties=(0, 4, 6, 2))
+    status, score, ops = al.align(b"ACGT", b"ACTT")      # wavefront_align
+    res = al.align_batch(patterns, texts)                 # wavefront_align_batch
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import (POINTER, Structure, c_bool, c_char_p, c_double, c_int, c_int32, c_int64,
+                    c_uint, c_uint32, c_uint64, c_void_p)
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwfa2_amd.so")
+
+# wavefront/wfa.h:45-55 (+ WF_STATUS_UNSUPPORTED, library-specific)
+WF_STATUS_ALG_COMPLETED = 0
+WF_STATUS_ALG_PARTIAL = 1
+WF_STATUS_MAX_STEPS_REACHED = -100
+WF_STATUS_OOM = -200
+WF_STATUS_UNATTAINABLE = -300
+WF_STATUS_UNSUPPORTED = -400
+
+METRICS = {"indel": 0, "edit": 1, "linear": 2, "affine": 3, "affine2p": 4}
+MEMORY = {"high": 0, "med": 1, "low": 2, "ultralow": 3}
+
+
+class alignment_form_t(Structure):
+    _fields_ = [("span", c_int), ("extension", c_bool), ("pattern_begin_free", c_int),
+                ("pattern_end_free", c_int), ("text_begin_free", c_int), ("text_end_free", c_int)]
+
+
+class linear_penalties_t(Structure):
+    _fields_ = [("match", c_int), ("mismatch", c_int), ("indel", c_int)]
+
+
+class affine_penalties_t(Structure):
+    _fields_ = [("match", c_int), ("mismatch", c_int), ("gap_opening", c_int), ("gap_extension", c_int)]
+
+
+class affine2p_penalties_t(Structure):
+    _fields_ = [("match", c_int), ("mismatch", c_int), ("gap_opening1", c_int), ("gap_extension1", c_int),
+                ("gap_opening2", c_int), ("gap_extension2", c_int)]
+
+
+class wavefront_heuristic_t(Structure):
+    _fields_ = [("strategy", c_uint)] + [(n, c_int) for n in (
+        "steps_between_cutoffs", "min_k", "max_k", "min_wavefront_length", "max_distance_threshold",
+        "xdrop", "zdrop", "steps_wait", "max_sw_score", "max_wf_score", "max_sw_score_offset",
+        "max_sw_score_k")]
+
+
+class timespec(Structure):
+    _fields_ = [("tv_sec", c_int64), ("tv_nsec", c_int64)]
+
+
+class profiler_counter_t(Structure):
+    _fields_ = [("total", c_uint64), ("samples", c_uint64), ("min", c_uint64), ("max", c_uint64),
+                ("m_oldM", c_double), ("m_newM", c_double), ("m_oldS", c_double), ("m_newS", c_double)]
+
+
+class profiler_timer_t(Structure):
+    _fields_ = [("begin_timer", timespec), ("time_ns", profiler_counter_t), ("accumulated", c_uint64)]
+
+
+class alignment_system_t(Structure):
+    _fields_ = [("max_alignment_steps", c_int), ("probe_interval_global", c_int),
+                ("probe_interval_compact", c_int), ("max_partial_compacts", c_uint64),
+                ("max_memory_compact", c_uint64), ("max_memory_resident", c_uint64),
+                ("max_memory_abort", c_uint64), ("verbose", c_int), ("check_alignment_correct", c_bool),
+                ("timer", profiler_timer_t), ("max_num_threads", c_int), ("min_offsets_per_thread", c_int)]
+
+
+class wavefront_plot_attr_t(Structure):
+    _fields_ = [("enabled", c_bool), ("resolution_points", c_int), ("align_level", c_int)]
+
+
+class wavefront_aligner_attr_t(Structure):
+    _fields_ = [("distance_metric", c_int), ("alignment_scope", c_int), ("alignment_form", alignment_form_t),
+                ("linear_penalties", linear_penalties_t), ("affine_penalties", affine_penalties_t),
+                ("affine2p_penalties", affine2p_penalties_t), ("heuristic", wavefront_heuristic_t),
+                ("memory_mode", c_int), ("mm_allocator", c_void_p), ("plot", wavefront_plot_attr_t),
+                ("system", alignment_system_t)]
+
+
+class cigar_t(Structure):
+    _fields_ = [("operations", c_void_p), ("max_operations", c_int), ("begin_offset", c_int),
+                ("end_offset", c_int), ("score", c_int), ("end_v", c_int), ("end_h", c_int),
+                ("has_misms", c_bool), ("cigar_buffer", c_void_p), ("cigar_length", c_int)]
+
+
+class wavefront_align_status_t(Structure):
+    _fields_ = [("status", c_int), ("score", c_int), ("dropped", c_bool), ("num_null_steps", c_int),
+                ("memory_used", c_uint64), ("wf_align_compute", c_void_p), ("wf_align_extend", c_void_p)]
+
+
+class wavefront_penalties_t(Structure):
+    _fields_ = [("distance_metric", c_int), ("match", c_int), ("mismatch", c_int), ("gap_opening1", c_int),
+                ("gap_extension1", c_int), ("gap_opening2", c_int), ("gap_extension2", c_int),
+                ("linear_penalties", linear_penalties_t), ("affine_penalties", affine_penalties_t),
+                ("affine2p_penalties", affine2p_penalties_t), ("internal_gap_e", c_int)]
+
+
+class wavefront_pos_t(Structure):
+    _fields_ = [("score", c_int), ("k", c_int), ("offset", c_int32)]
+
+
+class wavefront_sequences_t(Structure):
+    _fields_ = [("pattern", c_void_p), ("pattern_length", c_int), ("text", c_void_p), ("text_length", c_int)]
+
+
+class wavefront_aligner_t(Structure):
+    _fields_ = [("align_mode", c_int), ("align_mode_tag", c_char_p), ("align_status", wavefront_align_status_t),
+                ("sequences", wavefront_sequences_t), ("alignment_scope", c_int),
+                ("alignment_form", alignment_form_t), ("penalties", wavefront_penalties_t),
+                ("heuristic", wavefront_heuristic_t), ("memory_mode", c_int), ("component_begin", c_int),
+                ("component_end", c_int), ("alignment_end_pos", wavefront_pos_t), ("cigar", POINTER(cigar_t)),
+                ("system", alignment_system_t), ("device", c_void_p)]
+
+
+class wfa_pair_t(Structure):
+    _fields_ = [("pattern", c_void_p), ("pattern_length", c_int), ("text", c_void_p), ("text_length", c_int)]
+
+
+class wfa_result_t(Structure):
+    _fields_ = [("status", c_int), ("score", c_int), ("end_v", c_int), ("end_h", c_int),
+                ("cigar_offset", c_int64), ("cigar_length", c_int)]
+
+
+class wfa_run_stats_t(Structure):
+    _fields_ = [("diagonal_steps", c_int64), ("forward_ms", c_double), ("backtrace_ms", c_double),
+                ("forward_launches", c_int)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded C-ABI library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libwfa2_amd.so is not built: run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        L.wavefront_aligner_new.restype = POINTER(wavefront_aligner_t)
+        L.wavefront_aligner_new.argtypes = [POINTER(wavefront_aligner_attr_t)]
+        L.wavefront_aligner_delete.argtypes = [POINTER(wavefront_aligner_t)]
+        L.wavefront_align.argtypes = [POINTER(wavefront_aligner_t), c_char_p, c_int, c_char_p, c_int]
+        L.wavefront_align_batch.argtypes = [POINTER(wavefront_aligner_t), c_void_p, c_int, c_void_p,
+                                            c_void_p, ctypes.c_size_t]
+        L.wfa_device_batch_upload.restype = c_void_p
+        L.wfa_device_batch_upload.argtypes = [POINTER(wavefront_aligner_t), c_void_p, c_int]
+        L.wfa_device_batch_run.argtypes = [POINTER(wavefront_aligner_t), c_void_p]
+        L.wfa_device_batch_download.argtypes = [POINTER(wavefront_aligner_t), c_void_p, c_void_p, c_void_p,
+                                                ctypes.c_size_t]
+        L.wfa_device_batch_free.argtypes = [c_void_p]
+        L.wfa_device_batch_stats.argtypes = [c_void_p, POINTER(wfa_run_stats_t)]
+        L.wavefront_align_strerror.restype = c_char_p
+        L.wavefront_align_strerror_short.restype = c_char_p
+        _lib = L
+    return _lib
+
+
+def default_attributes() -> wavefront_aligner_attr_t:
+    """A copy of the library's wavefront_aligner_attr_default."""
+    return wavefront_aligner_attr_t.from_buffer_copy(wavefront_aligner_attr_t.in_dll(lib(), "wavefront_aligner_attr_default"))
+
+
+class BatchResult:
+    """Per-pair results of one batch: numpy arrays plus the CIGAR arena."""
+
+    def __init__(self, n, results, arena):
+        self.n = n
+        self._r = results
+        self.arena = arena
+        self.status = np.array([results[i].status for i in range(n)], dtype=np.int32)
+        self.score = np.array([results[i].score for i in range(n)], dtype=np.int32)
+        self.cigar_offset = np.array([results[i].cigar_offset for i in range(n)], dtype=np.int64)
+        self.cigar_length = np.array([results[i].cigar_length for i in range(n)], dtype=np.int32)
+
+    def ops(self, i) -> bytes:
+        o = int(self.cigar_offset[i])
+        return self.arena[o:o + int(self.cigar_length[i])] if self.arena is not None else b""
+
+
+class _Pairs:
+    """A wfa_pair_t array over one byte buffer (kept alive while in use).
+
+    Accepts a PairSet-like object (buf, p_off, plen, t_off, tlen) or two
+    lists of byte strings."""
+
+    _dtype = np.dtype([("pattern", np.uint64), ("plen", np.int32), ("pad0", np.int32),
+                       ("text", np.uint64), ("tlen", np.int32), ("pad1", np.int32)])
+
+    def __init__(self, patterns, texts=None):
+        if texts is None:
+            ps = patterns
+        else:
+            from .datagen import PairSet
+            ps = PairSet.from_lists(patterns, texts)
+        self.n = int(ps.n)
+        self.buf = ctypes.create_string_buffer(ps.buf, len(ps.buf) + 1)
+        base = ctypes.addressof(self.buf)
+        arr = np.zeros(max(self.n, 1), dtype=self._dtype)
+        if self.n:
+            arr["pattern"][:self.n] = base + ps.p_off.astype(np.uint64)
+            arr["plen"][:self.n] = ps.plen
+            arr["text"][:self.n] = base + ps.t_off.astype(np.uint64)
+            arr["tlen"][:self.n] = ps.tlen
+        self.arr = arr
+        self.total = int(np.sum(ps.plen.astype(np.int64)) + np.sum(ps.tlen.astype(np.int64)))
+
+    @property
+    def address(self):
+        return self.arr.ctypes.data
+
+
+class DeviceBatch:
+    """A batch resident in HBM (wfa_device_batch_upload / _run / _download)."""
+
+    def __init__(self, aligner: "Aligner", patterns, texts=None):
+        self.al = aligner
+        self.pairs = _Pairs(patterns, texts)
+        self.h = lib().wfa_device_batch_upload(aligner.wf, self.pairs.address, self.pairs.n)
+        if not self.h:
+            raise RuntimeError("wfa_device_batch_upload failed")
+
+    def run(self) -> int:
+        return lib().wfa_device_batch_run(self.al.wf, self.h)
+
+    def stats(self) -> wfa_run_stats_t:
+        s = wfa_run_stats_t()
+        lib().wfa_device_batch_stats(self.h, ctypes.byref(s))
+        return s
+
+    def download(self, cigars=True) -> BatchResult:
+        n = self.pairs.n
+        res = (wfa_result_t * max(n, 1))()
+        arena = ctypes.create_string_buffer(self.pairs.total + 1) if cigars else None
+        lib().wfa_device_batch_download(self.al.wf, self.h, ctypes.addressof(res),
+                                        ctypes.addressof(arena) if arena is not None else None,
+                                        self.pairs.total + 1 if arena is not None else 0)
+        return BatchResult(n, res, arena.raw if arena is not None else None)
+
+    def free(self):
+        if self.h:
+            lib().wfa_device_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Aligner:
+    """wavefront_aligner_new with attributes built from keyword arguments."""
+
+    def __init__(self, metric="affine", penalties=(0, 4, 6, 2), memory="high", score_only=False,
+                 max_steps=0, heuristic_none=True):
+        a = default_attributes()
+        a.distance_metric = METRICS[metric]
+        a.alignment_scope = 0 if score_only else 1
+        a.memory_mode = MEMORY[memory]
+        if metric == "affine2p":
+            p = tuple(penalties)
+            a.affine2p_penalties = affine2p_penalties_t(*p)
+        elif metric == "affine":
+            a.affine_penalties = affine_penalties_t(*tuple(penalties)[:4])
+        if heuristic_none:
+            a.heuristic.strategy = 0
+        if max_steps:
+            a.system.max_alignment_steps = int(max_steps)
+        self.attr = a
+        self.score_only = score_only
+        self.wf = lib().wavefront_aligner_new(ctypes.byref(a))
+        if not self.wf:
+            raise RuntimeError("wavefront_aligner_new failed")
+
+    def close(self):
+        if self.wf:
+            lib().wavefront_aligner_delete(self.wf)
+            self.wf = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def align(self, pattern: bytes, text: bytes):
+        """wavefront_align: returns (status, cigar->score, CIGAR operations)."""
+        st = lib().wavefront_align(self.wf, bytes(pattern), len(pattern), bytes(text), len(text))
+        cg = self.wf.contents.cigar.contents
+        ops = b""
+        if cg.end_offset > cg.begin_offset and cg.operations:
+            ops = ctypes.string_at(cg.operations + cg.begin_offset, cg.end_offset - cg.begin_offset)
+        return st, cg.score, ops
+
+    def align_batch(self, patterns, texts=None) -> BatchResult:
+        """wavefront_align_batch over host buffers."""
+        pr = _Pairs(patterns, texts)
+        n = pr.n
+        res = (wfa_result_t * max(n, 1))()
+        arena = None if self.score_only else ctypes.create_string_buffer(pr.total + 1)
+        lib().wavefront_align_batch(self.wf, pr.address, n, ctypes.addressof(res),
+                                    ctypes.addressof(arena) if arena is not None else None,
+                                    pr.total + 1 if arena is not None else 0)
+        return BatchResult(n, res, arena.raw if arena is not None else None)
+
+    def upload(self, patterns, texts=None) -> DeviceBatch:
+        return DeviceBatch(self, patterns, texts)
+
+
+def rle(ops: bytes) -> str:
+    """CIGAR run-length form as cigar_sprint(print_matches=true) prints it."""
+    if not ops:
+        return "-"
+    out = []
+    i, n = 0, len(ops)
+    while i < n:
+        j = i
+        while j < n and ops[j] == ops[i]:
+            j += 1
+        out.append("%d%c" % (j - i, ops[i]))
+        i = j
+    return "".join(out)
+
+
+def declared_functions(header_path):
+    """Function names declared in a C header of include/ (for export checks)."""
+    import re
+    text = open(header_path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text)
+    skip = {"if", "for", "while", "switch", "return", "sizeof", "defined"}
+    decl = []
+    for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", text, flags=re.M | re.S):
+        if m.group(0).lstrip().startswith("typedef"):
+            continue
+        nm = m.group(1)
+        if nm not in skip and not nm.endswith("_t"):
+            decl.append(nm)
+    return sorted(set(decl)) if decl else sorted(set(n for n in names if n not in skip))
