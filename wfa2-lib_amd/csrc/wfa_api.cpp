@@ -81,6 +81,22 @@ int gcd(int a, int b) { while (b) { int t = a % b; a = b; b = t; } return a; }
 
 // ----------------------------------------------------------------- batches
 
+struct PlanGroup {
+  int family = 0;
+  int block = 64;
+  int wcap = 0;
+  int lmax_words = 0;
+  std::vector<int> ids;
+  DevBuf d_ids;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  void release() {
+    d_ids.release();
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    ev[0] = ev[1] = ev[2] = nullptr;
+  }
+};
+
 struct wfa_device_batch_t {
   int n = 0;
   int bits = 2;
@@ -88,18 +104,22 @@ struct wfa_device_batch_t {
   std::vector<int> plen, tlen;
   std::vector<int64_t> ops_off;
   std::vector<int> sorted;           // pair ids, longest first
-  DevBuf words, d_plen, d_tlen, d_patw, d_txtw, d_order, d_status, d_score, d_last, d_dsteps,
+  DevBuf words, d_plen, d_tlen, d_patw, d_txtw, d_status, d_score, d_last, d_dsteps,
          d_ops_off, d_ops_begin, d_ops;
-  // results of the last run (host)
+  // launch plan, cached for the penalties it was made for
+  std::vector<PlanGroup> plan;
+  std::vector<int> plan_key;
+  // results: on the device after a run; copied to the host on demand
   std::vector<int> status, score, ops_begin;
-  std::vector<long long> dsteps;
   bool have_results = false;
+  bool host_results = false;
   bool ops_on_device = false;
+  int first_fail = 0;
   wfa_run_stats_t stats{};
-  // the aligner configuration it was run with
   int records = 1;
   ~wfa_device_batch_t() {
-    for (DevBuf* b : {&words, &d_plen, &d_tlen, &d_patw, &d_txtw, &d_order, &d_status, &d_score, &d_last,
+    for (auto& g : plan) g.release();
+    for (DevBuf* b : {&words, &d_plen, &d_tlen, &d_patw, &d_txtw, &d_status, &d_score, &d_last,
                       &d_dsteps, &d_ops_off, &d_ops_begin, &d_ops})
       b->release();
   }
@@ -239,6 +259,8 @@ int check_supported(wavefront_aligner_t* wf) {
     why = "only end-to-end alignment";
   else if (wf->heuristic.strategy != wf_heuristic_none) why = "only heuristic none (exact alignment)";
   else if (wf->memory_mode == wavefront_memory_ultralow) why = "memory mode ultralow (BiWFA) is not enabled in this build";
+  else if (make_penalties(wf->penalties, m == gap_affine_2p).nslots > 64)
+    why = "penalties whose score scope needs more than 64 wavefront slots";
   if (!why) return WF_STATUS_OK;
   if (wf->system.verbose >= 0)
     fprintf(stderr, "[WFA] Unsupported configuration on the device path: %s\n", why);
@@ -278,19 +300,12 @@ void pack_seq(uint64_t* dst, const char* src, int len, int bits) {
   }
 }
 
-// Kernel-family plan for one pass over a subset of pairs.
-struct Group {
-  int family;
-  int block;
-  int wcap;
-  int lmax_words;
-  std::vector<int> ids;
-};
-
-constexpr int kSmallLdsBudget = 16 * 1024;  // per workgroup: keeps ~9 wavefronts per CU resident
+// LDS ring budget per workgroup of the one-wavefront family (16-bit ring
+// elements): ~8.5 KB with the sequences keeps ~18 wavefronts per CU resident.
+constexpr int kSmallLdsBudget = 8 * 1024;
 
 int small_wcap(const Penalties& pen, int ncomp) {
-  int w = kSmallLdsBudget / (pen.nslots * ncomp * 4);
+  int w = kSmallLdsBudget / (pen.nslots * ncomp * 2);
   w = (w / 32) * 32;
   return w;
 }
@@ -398,7 +413,7 @@ extern "C" wfa_device_batch_t* wfa_device_batch_upload(wavefront_aligner_t* cons
   const size_t nn = std::max(n, 1);
   bool ok = b->words.ensure(std::max<size_t>(words, 1) * 8) && b->d_plen.ensure(nn * 4) &&
             b->d_tlen.ensure(nn * 4) && b->d_patw.ensure(nn * 8) && b->d_txtw.ensure(nn * 8) &&
-            b->d_order.ensure(nn * 4) && b->d_status.ensure(nn * 4) && b->d_score.ensure(nn * 4) &&
+            b->d_status.ensure(nn * 4) && b->d_score.ensure(nn * 4) &&
             b->d_last.ensure(nn * 4) && b->d_dsteps.ensure(nn * 8) && b->d_ops_off.ensure(nn * 8) &&
             b->d_ops_begin.ensure(nn * 4);
   if (!ok) {
@@ -422,42 +437,82 @@ extern "C" void wfa_device_batch_free(wfa_device_batch_t* b) { delete b; }
 
 namespace {
 
-// One forward launch over `ids` followed by the backtrace of those pairs.
-int run_group(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, const Group& grp,
-              bool two, const Penalties& pen, int records, float* fwd_ms, float* bt_ms) {
+wfa::Counters* counters_of(Engine* e) { return e->counters.as<wfa::Counters>(); }
+
+PlanGroup make_large(const wfa_device_batch_t* b, std::vector<int> ids) {
+  PlanGroup g;
+  g.family = wfa::kLarge;
+  int pmax = 0, tmax = 0;
+  for (int i : ids) { pmax = std::max(pmax, b->plen[i]); tmax = std::max(tmax, b->tlen[i]); }
+  g.wcap = pmax + tmax + 8;
+  g.lmax_words = 0;
+  g.block = wfa::pick_block(wfa::kLarge, pmax + tmax);
+  g.ids = std::move(ids);
+  return g;
+}
+
+// Splits the pairs over kernel families: short pairs on the LDS family, the
+// rest on the HBM-ring family by workgroup size.
+std::vector<PlanGroup> make_plan(const wfa_device_batch_t* b, const std::vector<int>& ids,
+                                 const Penalties& pen, int ncomp) {
+  const int wcap_small = small_wcap(pen, ncomp);
+  const int spw = 64 / b->bits;
+  std::vector<int> small_ids, cls[3];
+  int lmax_small = 0;
+  for (int i : ids) {
+    const int L = std::max(b->plen[i], b->tlen[i]);
+    if (wcap_small >= 64 && L <= 1024) {
+      small_ids.push_back(i);
+      lmax_small = std::max<int>(lmax_small, L / spw + 2);
+    } else {
+      const int w = b->plen[i] + b->tlen[i];
+      cls[w <= 256 ? 0 : (w <= 4096 ? 1 : 2)].push_back(i);
+    }
+  }
+  std::vector<PlanGroup> plan;
+  if (!small_ids.empty()) {
+    PlanGroup g;
+    g.family = wfa::kSmall;
+    g.block = 64;
+    g.wcap = wcap_small;
+    g.lmax_words = lmax_small;
+    g.ids = std::move(small_ids);
+    plan.push_back(std::move(g));
+  }
+  for (auto& c : cls)
+    if (!c.empty()) plan.push_back(make_large(b, std::move(c)));
+  return plan;
+}
+
+LaunchShape shape_of(Engine* e, const PlanGroup& g, int two, int bits, const Penalties& pen) {
+  LaunchShape shape;
+  shape.family = g.family;
+  shape.block = g.block;
+  shape.lds_bytes = wfa::forward_lds_bytes(g.family, two ? 5 : 3, pen.nslots, g.wcap, g.lmax_words);
+  int per_cu = wfa::forward_max_blocks_per_cu(g.family, g.block, two, bits, shape.lds_bytes);
+  if (per_cu <= 0) per_cu = 1;
+  shape.grid = (int)std::min<int64_t>((int64_t)g.ids.size(), (int64_t)per_cu * e->num_cus);
+  return shape;
+}
+
+// Enqueues forward + backtrace of one group on the aligner's stream.
+int enqueue_group(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, PlanGroup& grp,
+                  int two, const Penalties& pen, int records) {
   const int n = (int)grp.ids.size();
   if (n == 0) return WF_STATUS_OK;
   hipStream_t st = e->stream;
-  const int ncomp = two ? 5 : 3;
-  LaunchShape shape;
-  shape.family = grp.family;
-  shape.block = grp.block;
-  shape.lds_bytes = wfa::forward_lds_bytes(grp.family, ncomp, pen.nslots, grp.wcap, grp.lmax_words);
-  int per_cu = wfa::forward_max_blocks_per_cu(grp.family, grp.block, two, b->bits, shape.lds_bytes);
-  if (per_cu <= 0) per_cu = 1;
-  shape.grid = std::min<int64_t>(n, (int64_t)per_cu * e->num_cus);
-  if (!hip_ok(hipMemcpyAsync(b->d_order.p, grp.ids.data(), n * 4, hipMemcpyHostToDevice, st), "order upload"))
-    return WF_STATUS_OOM;
-  if (records && !e->rec.ensure(e->rec_bytes)) {
-    fprintf(stderr, "[WFA] cannot allocate the %zu MB record arena\n", e->rec_bytes >> 20);
-    return WF_STATUS_OOM;
-  }
-  int* ring = nullptr;
-  long long ring_stride = 0;
-  if (grp.family == wfa::kLarge) {
-    ring_stride = (long long)pen.nslots * ncomp * grp.wcap;
-    if (!e->ring.ensure((size_t)ring_stride * shape.grid * sizeof(int))) {
-      fprintf(stderr, "[WFA] cannot allocate %lld MB of wavefront rings\n",
-              (long long)((size_t)ring_stride * shape.grid * 4 >> 20));
+  const LaunchShape shape = shape_of(e, grp, two, b->bits, pen);
+  if (!grp.d_ids.p) {
+    if (!grp.d_ids.ensure((size_t)n * 4)) return WF_STATUS_OOM;
+    if (!hip_ok(hipMemcpyAsync(grp.d_ids.p, grp.ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, st), "plan upload"))
       return WF_STATUS_OOM;
-    }
-    ring = e->ring.as<int>();
   }
-  unsigned int* queue = e->counters.as<unsigned int>();
-  unsigned long long* rec_top = reinterpret_cast<unsigned long long*>(e->counters.as<char>() + 8);
+  for (auto& ev : grp.ev)
+    if (!ev && !hip_ok(hipEventCreate(&ev), "hipEventCreate")) return WF_STATUS_OOM;
+  wfa::Counters* c = counters_of(e);
   ForwardArgs fa{};
   fa.n_pairs = n;
-  fa.order = b->d_order.as<int>();
+  fa.order = grp.d_ids.as<int>();
   fa.plen = b->d_plen.as<int>();
   fa.tlen = b->d_tlen.as<int>();
   fa.words = b->words.as<uint64_t>();
@@ -469,10 +524,13 @@ int run_group(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, const G
   fa.out_last_block = b->d_last.as<uint32_t>();
   fa.rec_arena = e->rec.as<uint4>();
   fa.rec_units = records ? std::min<unsigned long long>(e->rec.cap / 16, 0xFFFFFFF0ull) : 0;
-  fa.rec_top = rec_top;
-  fa.ring_global = ring;
-  fa.ring_stride = ring_stride;
-  fa.queue = queue;
+  fa.rec_top = &c->rec_top;
+  fa.ring_global = grp.family == wfa::kLarge ? e->ring.as<int>() : nullptr;
+  fa.ring_stride = grp.family == wfa::kLarge ? (long long)pen.nslots * (two ? 5 : 3) * grp.wcap : 0;
+  fa.queue = &c->queue;
+  fa.dsteps_total = &c->dsteps;
+  fa.retry_count = &c->retry;
+  fa.fail_count = &c->fail;
   fa.wcap = grp.wcap;
   fa.lmax_words = grp.lmax_words;
   fa.max_steps = wf->system.max_alignment_steps > 0 ? wf->system.max_alignment_steps : INT_MAX;
@@ -480,14 +538,14 @@ int run_group(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, const G
   fa.comp_begin = wfa::kM;
   fa.comp_end = wfa::kM;
   fa.pen = pen;
-  if (!hip_ok(wfa::launch_reset_counters(queue, rec_top, st), "reset")) return WF_STATUS_OOM;
-  (void)hipEventRecord(e->ev[0], st);
+  if (!hip_ok(wfa::launch_reset_counters(c, 0, st), "reset")) return WF_STATUS_OOM;
+  (void)hipEventRecord(grp.ev[0], st);
   if (!hip_ok(wfa::launch_forward(fa, two, b->bits, shape, st), "forward launch")) return WF_STATUS_OOM;
-  (void)hipEventRecord(e->ev[1], st);
+  (void)hipEventRecord(grp.ev[1], st);
   if (records) {
     BacktraceArgs ba{};
     ba.n_pairs = n;
-    ba.order = b->d_order.as<int>();
+    ba.order = grp.d_ids.as<int>();
     ba.plen = b->d_plen.as<int>();
     ba.tlen = b->d_tlen.as<int>();
     ba.status = b->d_status.as<int>();
@@ -498,31 +556,49 @@ int run_group(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, const G
     ba.ops = b->d_ops.as<char>();
     ba.ops_off = b->d_ops_off.as<int64_t>();
     ba.ops_begin = b->d_ops_begin.as<int>();
+    ba.fail_count = &c->fail;
     ba.comp_begin = wfa::kM;
     ba.comp_end = wfa::kM;
     ba.pen = pen;
     if (!hip_ok(wfa::launch_backtrace(ba, st), "backtrace launch")) return WF_STATUS_OOM;
   }
-  (void)hipEventRecord(e->ev[2], st);
-  if (!hip_ok(hipStreamSynchronize(st), "alignment")) return WF_STATUS_OOM;
-  float f = 0, t = 0;
-  (void)hipEventElapsedTime(&f, e->ev[0], e->ev[1]);
-  (void)hipEventElapsedTime(&t, e->ev[1], e->ev[2]);
-  *fwd_ms += f;
-  *bt_ms += t;
+  (void)hipEventRecord(grp.ev[2], st);
   return WF_STATUS_OK;
 }
 
-Group large_group(const wfa_device_batch_t* b, std::vector<int> ids) {
-  Group g;
-  g.family = wfa::kLarge;
-  int pmax = 0, tmax = 0;
-  for (int i : ids) { pmax = std::max(pmax, b->plen[i]); tmax = std::max(tmax, b->tlen[i]); }
-  g.wcap = pmax + tmax + 8;
-  g.lmax_words = 0;
-  g.block = wfa::pick_block(wfa::kLarge, pmax + tmax);
-  g.ids = std::move(ids);
-  return g;
+// Device memory every group of the plan needs, allocated before any launch.
+bool reserve_for(Engine* e, std::vector<PlanGroup>& plan, int two, int bits, const Penalties& pen, int records) {
+  if (records && !e->rec.ensure(e->rec_bytes)) {
+    fprintf(stderr, "[WFA] cannot allocate the %zu MB record arena\n", e->rec_bytes >> 20);
+    return false;
+  }
+  size_t ring = 0;
+  for (const PlanGroup& g : plan) {
+    if (g.family != wfa::kLarge) continue;
+    const LaunchShape sh = shape_of(e, g, two, bits, pen);
+    ring = std::max(ring, (size_t)pen.nslots * (two ? 5 : 3) * g.wcap * sh.grid * sizeof(int));
+  }
+  if (ring && !e->ring.ensure(ring)) {
+    fprintf(stderr, "[WFA] cannot allocate %zu MB of wavefront rings\n", ring >> 20);
+    return false;
+  }
+  return true;
+}
+
+bool download_results(Engine* e, wfa_device_batch_t* b) {
+  if (b->host_results) return true;
+  hipStream_t st = e->stream;
+  const size_t n = (size_t)b->n;
+  b->status.resize(n);
+  b->score.resize(n);
+  b->ops_begin.resize(n);
+  bool ok = n == 0 ||
+            (hip_ok(hipMemcpyAsync(b->status.data(), b->d_status.p, n * 4, hipMemcpyDeviceToHost, st), "download") &&
+             hip_ok(hipMemcpyAsync(b->score.data(), b->d_score.p, n * 4, hipMemcpyDeviceToHost, st), "download") &&
+             hip_ok(hipMemcpyAsync(b->ops_begin.data(), b->d_ops_begin.p, n * 4, hipMemcpyDeviceToHost, st), "download"));
+  ok = ok && hip_ok(hipStreamSynchronize(st), "download sync");
+  b->host_results = ok;
+  return ok;
 }
 
 }  // namespace
@@ -533,123 +609,109 @@ extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_ba
   if (sup != WF_STATUS_OK) return sup;
   Engine* e = engine_of(wf);
   if (!e || !engine_init(e)) return WF_STATUS_OOM;
-  const bool two = wf->penalties.distance_metric == gap_affine_2p;
+  const int two = wf->penalties.distance_metric == gap_affine_2p;
   const Penalties pen = make_penalties(wf->penalties, two);
   const int ncomp = two ? 5 : 3;
   const int records = (wf->alignment_scope == compute_alignment) ? 1 : 0;
   b->records = records;
   b->stats = wfa_run_stats_t{};
   b->have_results = false;
+  b->host_results = false;
+  b->first_fail = WF_STATUS_ALG_COMPLETED;
+  if (b->n == 0) { b->have_results = true; b->host_results = true; return WF_STATUS_ALG_COMPLETED; }
   if (records && !b->d_ops.ensure(std::max<int64_t>(b->total_ops, 1))) {
     fprintf(stderr, "[WFA] cannot allocate %lld bytes of CIGAR output\n", (long long)b->total_ops);
     return WF_STATUS_OOM;
   }
   b->ops_on_device = records != 0;
-  // Plan: short pairs on the LDS family, the rest on the HBM-ring family.
-  const int wcap_small = small_wcap(pen, ncomp);
-  const int spw = 64 / b->bits;
-  std::vector<int> small_ids, large_ids;
-  int lmax_small = 0;
-  for (int i : b->sorted) {
-    const int L = std::max(b->plen[i], b->tlen[i]);
-    if (wcap_small >= 64 && L <= 1024) {
-      small_ids.push_back(i);
-      lmax_small = std::max<int>(lmax_small, L / spw + 2);
-    } else {
-      large_ids.push_back(i);
-    }
+  const std::vector<int> key = {pen.x, pen.o1, pen.e1, pen.o2, pen.e2, two, b->bits};
+  if (b->plan_key != key) {
+    for (auto& g : b->plan) g.release();
+    b->plan = make_plan(b, b->sorted, pen, ncomp);
+    b->plan_key = key;
   }
-  std::vector<Group> groups;
-  if (!small_ids.empty()) {
-    Group g;
-    g.family = wfa::kSmall;
-    g.block = 64;
-    g.wcap = wcap_small;
-    g.lmax_words = lmax_small;
-    g.ids = std::move(small_ids);
-    groups.push_back(std::move(g));
-  }
-  // Split the long pairs by workgroup size.
-  {
-    std::vector<int> cls[3];
-    for (int i : large_ids) {
-      const int w = b->plen[i] + b->tlen[i];
-      cls[w <= 256 ? 0 : (w <= 4096 ? 1 : 2)].push_back(i);
-    }
-    for (auto& c : cls)
-      if (!c.empty()) groups.push_back(large_group(b, std::move(c)));
-  }
-  b->status.assign(b->n, wfa::kDevArenaFull);
-  b->score.assign(b->n, 0);
-  b->ops_begin.assign(b->n, 0);
-  b->dsteps.assign(b->n, 0);
+  hipStream_t st = e->stream;
+  wfa::Counters host_c{};
+  std::vector<PlanGroup> retry_groups;
+  std::vector<PlanGroup>* pending = &b->plan;
   float fwd_ms = 0, bt_ms = 0;
   int launches = 0;
-  std::vector<int> st_tmp, sc_tmp, ob_tmp;
-  std::vector<long long> ds_tmp;
-  int guard = 0;
-  while (!groups.empty()) {
-    if (++guard > 64) {
+  if (!hip_ok(wfa::launch_reset_counters(counters_of(e), 2, st), "reset")) return WF_STATUS_OOM;
+  for (int round = 0;; ++round) {
+    if (round > 48) {
       fprintf(stderr, "[WFA] batch did not converge (record arena too small?)\n");
       return WF_STATUS_OOM;
     }
-    Group grp = std::move(groups.front());
-    groups.erase(groups.begin());
-    const int rc = run_group(wf, e, b, grp, two, pen, records, &fwd_ms, &bt_ms);
-    ++launches;
-    if (rc != WF_STATUS_OK) return rc;
-    // Collect this group's results.
-    hipStream_t st = e->stream;
-    bool ok = true;
-    st_tmp.resize(b->n);
-    sc_tmp.resize(b->n);
-    ob_tmp.resize(b->n);
-    ds_tmp.resize(b->n);
-    ok = hip_ok(hipMemcpyAsync(st_tmp.data(), b->d_status.p, b->n * 4, hipMemcpyDeviceToHost, st), "download") &&
-         hip_ok(hipMemcpyAsync(sc_tmp.data(), b->d_score.p, b->n * 4, hipMemcpyDeviceToHost, st), "download") &&
-         hip_ok(hipMemcpyAsync(ob_tmp.data(), b->d_ops_begin.p, b->n * 4, hipMemcpyDeviceToHost, st), "download") &&
-         hip_ok(hipMemcpyAsync(ds_tmp.data(), b->d_dsteps.p, b->n * 8, hipMemcpyDeviceToHost, st), "download") &&
-         hip_ok(hipStreamSynchronize(st), "download sync");
-    if (!ok) return WF_STATUS_OOM;
-    std::vector<int> ring_over, arena_full;
-    for (int i : grp.ids) {
-      b->status[i] = st_tmp[i];
-      b->score[i] = sc_tmp[i];
-      b->ops_begin[i] = ob_tmp[i];
-      b->dsteps[i] = ds_tmp[i];
-      if (st_tmp[i] == wfa::kDevRingOverflow) ring_over.push_back(i);
-      else if (st_tmp[i] == wfa::kDevArenaFull) arena_full.push_back(i);
+    if (!reserve_for(e, *pending, two, b->bits, pen, records)) return WF_STATUS_OOM;
+    if (!hip_ok(wfa::launch_reset_counters(counters_of(e), 1, st), "reset")) return WF_STATUS_OOM;
+    for (PlanGroup& g : *pending) {
+      const int rc = enqueue_group(wf, e, b, g, two, pen, records);
+      if (rc != WF_STATUS_OK) return rc;
+      ++launches;
     }
-    if (!ring_over.empty()) groups.push_back(large_group(b, std::move(ring_over)));
-    if (!arena_full.empty()) {
-      if (arena_full.size() == grp.ids.size()) {
-        // Not even one pair fitted: grow the arena before retrying.
-        size_t free_b = 0, total_b = 0;
-        (void)hipMemGetInfo(&free_b, &total_b);
-        const size_t want = e->rec_bytes * 2;
-        if (want > e->rec.cap + free_b / 2) {
-          fprintf(stderr, "[WFA] record arena exhausted (%zu MB)\n", e->rec_bytes >> 20);
-          for (int i : arena_full) b->status[i] = WF_STATUS_OOM;
-          arena_full.clear();
-        } else {
-          e->rec.release();
-          e->rec_bytes = want;
+    if (!hip_ok(hipMemcpyAsync(&host_c, e->counters.p, sizeof(host_c), hipMemcpyDeviceToHost, st), "counters") ||
+        !hip_ok(hipStreamSynchronize(st), "alignment"))
+      return WF_STATUS_OOM;
+    for (PlanGroup& g : *pending) {
+      float f = 0, t = 0;
+      (void)hipEventElapsedTime(&f, g.ev[0], g.ev[1]);
+      (void)hipEventElapsedTime(&t, g.ev[1], g.ev[2]);
+      fwd_ms += f;
+      bt_ms += t;
+    }
+    if (host_c.retry == 0) break;
+    // Some pairs overflowed the LDS ring or the record arena: re-run only them.
+    b->host_results = false;
+    if (!download_results(e, b)) return WF_STATUS_OOM;
+    std::vector<PlanGroup> next;
+    for (PlanGroup& g : *pending) {
+      std::vector<int> ring_over, arena_full;
+      for (int i : g.ids) {
+        if (b->status[i] == wfa::kDevRingOverflow) ring_over.push_back(i);
+        else if (b->status[i] == wfa::kDevArenaFull) arena_full.push_back(i);
+      }
+      if (!ring_over.empty()) next.push_back(make_large(b, std::move(ring_over)));
+      if (!arena_full.empty()) {
+        if (arena_full.size() == g.ids.size()) {
+          // not a single pair of the group fitted: grow the arena
+          size_t free_b = 0, total_b = 0;
+          (void)hipMemGetInfo(&free_b, &total_b);
+          if (e->rec_bytes * 2 > e->rec.cap + free_b / 2) {
+            fprintf(stderr, "[WFA] record arena exhausted (%zu MB)\n", e->rec_bytes >> 20);
+            for (int i : arena_full) b->status[i] = WF_STATUS_OOM;
+            if (!hip_ok(hipMemcpy(b->d_status.p, b->status.data(), (size_t)b->n * 4, hipMemcpyHostToDevice), "status"))
+              return WF_STATUS_OOM;
+            arena_full.clear();
+          } else {
+            e->rec.release();
+            e->rec_bytes *= 2;
+          }
+        }
+        if (!arena_full.empty()) {
+          PlanGroup h;
+          h.family = g.family;
+          h.block = g.block;
+          h.wcap = g.wcap;
+          h.lmax_words = g.lmax_words;
+          h.ids = std::move(arena_full);
+          next.push_back(std::move(h));
         }
       }
-      if (!arena_full.empty()) {
-        Group g = grp;
-        g.ids = std::move(arena_full);
-        groups.insert(groups.begin(), std::move(g));
-      }
     }
+    for (auto& g : retry_groups) g.release();
+    retry_groups = std::move(next);
+    pending = &retry_groups;
+    if (retry_groups.empty()) break;
   }
-  int64_t dsum = 0;
-  for (long long d : b->dsteps) dsum += d;
-  b->stats.diagonal_steps = dsum;
+  for (auto& g : retry_groups) g.release();
+  b->stats.diagonal_steps = (int64_t)host_c.dsteps;
   b->stats.forward_ms = fwd_ms;
   b->stats.backtrace_ms = bt_ms;
   b->stats.forward_launches = launches;
   b->have_results = true;
+  b->host_results = false;
+  if (host_c.fail == 0) return WF_STATUS_ALG_COMPLETED;
+  if (!download_results(e, b)) return WF_STATUS_OOM;
   int first_fail = WF_STATUS_ALG_COMPLETED;
   for (int i = 0; i < b->n; ++i) {
     if (b->status[i] == wfa::kDevBadTrace) {
@@ -659,6 +721,7 @@ extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_ba
     if (b->status[i] != WF_STATUS_ALG_COMPLETED && first_fail == WF_STATUS_ALG_COMPLETED)
       first_fail = b->status[i];
   }
+  b->first_fail = first_fail;
   return first_fail;
 }
 
@@ -673,6 +736,7 @@ extern "C" int wfa_device_batch_download(wavefront_aligner_t* const wf, wfa_devi
                                          size_t cigar_arena_bytes) {
   if (!b || !b->have_results) return WF_STATUS_UNATTAINABLE;
   Engine* e = engine_of(wf);
+  if (!download_results(e, b)) return WF_STATUS_OOM;
   const bool want_ops = cigar_arena != nullptr && b->records && b->ops_on_device;
   if (want_ops) {
     if (cigar_arena_bytes < (size_t)b->total_ops) {
@@ -688,7 +752,8 @@ extern "C" int wfa_device_batch_download(wavefront_aligner_t* const wf, wfa_devi
   int first_fail = WF_STATUS_ALG_COMPLETED;
   for (int i = 0; i < b->n; ++i) {
     wfa_result_t& r = results[i];
-    const int st = b->status[i];
+    int st = b->status[i];
+    if (st == wfa::kDevBadTrace) st = WF_STATUS_UNATTAINABLE;
     r.status = st;
     r.end_v = b->plen[i];
     r.end_h = b->tlen[i];

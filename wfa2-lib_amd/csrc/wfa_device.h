@@ -62,6 +62,9 @@ struct ForwardArgs {
   int* ring_global;            // per-workgroup ring scratch (global-ring kernels)
   long long ring_stride;       // ints per workgroup
   unsigned int* queue;         // work-queue counter
+  unsigned long long* dsteps_total;  // sum of diagonal-steps over the launch
+  unsigned int* retry_count;   // pairs that must be re-run (ring overflow / arena full)
+  unsigned int* fail_count;    // pairs that finished with another non-completed status
   int wcap;                    // ring capacity in diagonals per slot
   int lmax_words;              // LDS capacity per sequence, in 64-bit words
   int max_steps;               // INT_MAX if unlimited
@@ -83,6 +86,7 @@ struct BacktraceArgs {
   char* ops;                   // operations arena
   const int64_t* ops_off;      // slot of pair i: ops[ops_off[i] .. +plen+tlen)
   int* ops_begin;              // CIGAR = ops[ops_off + ops_begin .. ops_off + plen + tlen)
+  unsigned int* fail_count;    // incremented for a pair whose trace is inconsistent
   int comp_begin, comp_end;
   Penalties pen;
 };
@@ -102,8 +106,20 @@ struct LaunchShape {
 hipError_t launch_forward(const ForwardArgs& a, int two_piece, int bits,
                           const LaunchShape& shape, hipStream_t stream);
 hipError_t launch_backtrace(const BacktraceArgs& a, hipStream_t stream);
-hipError_t launch_reset_counters(unsigned int* queue, unsigned long long* rec_top,
-                                 hipStream_t stream);
+// Device counters of one run (one 64-byte block).
+struct Counters {
+  unsigned int queue;            // work queue of the current launch
+  unsigned int retry;            // pairs to re-run (ring overflow / arena full)
+  unsigned long long rec_top;    // record arena bump pointer (16-byte units)
+  unsigned long long dsteps;     // diagonal-steps of the pairs that finished
+  unsigned int fail;             // pairs that finished with a status other than completed
+  unsigned int pad0;
+  unsigned long long pad[4];
+};
+static_assert(sizeof(Counters) == 64, "Counters layout");
+// Zeroes the queue and the arena pointer; with all=1 also retry; with all=2
+// also the run totals (diagonal-steps, failures).
+hipError_t launch_reset_counters(Counters* c, int all, hipStream_t stream);
 
 // Dynamic-LDS bytes and occupancy for a configuration.
 size_t forward_lds_bytes(int family, int ncomp, int nslots, int wcap, int lmax_words);

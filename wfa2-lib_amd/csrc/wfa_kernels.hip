@@ -19,48 +19,73 @@
 // to the cell's pre-extension M offset.  The backtrace kernel replays them.
 #include "wfa_device.h"
 #include <climits>
+#include <type_traits>
 
 namespace wfa {
 namespace {
 
-struct SlotMeta {      // 64 bytes, one per ring slot
+// Ring metadata lives in registers: lane r of every wave holds the state of
+// ring slot r (score tag, computed-range origin, trimmed [lo,hi] of each
+// component, present/null flags).  A source is read with v_readlane at a
+// uniform slot index, so no memory round trip sits on the per-step critical
+// path; every wave of a workgroup keeps an identical copy.
+template <int NC>
+struct MetaRegs {
   int tag;             // score held by the slot, -1 if none
   int alo;             // lo of the computed range: array origin of all components
-  int lo[5];           // trimmed ranges
-  int hi[5];
   unsigned flags;      // bit c: component present; bit 8+c: component null (trimmed empty)
-  int pad[3];
+  int lo[NC];
+  int hi[NC];
 };
-static_assert(sizeof(SlotMeta) == 64, "SlotMeta layout");
 
-// A compute input: reads at k in [lo,hi] return ring[off + k], else NULL.
-// The shared null wavefront of the reference has lo = 1, hi = -1.
+// A compute input.  lim_lo/lim_hi are the limits it contributes to the output
+// range (the shared null wavefront of the reference has lo = 1, hi = -1).
+// Reads at k in [lo, lo+span] return ring[base + k - lo], else NULL; an
+// invalid source gets an empty read range far from any diagonal.
 struct Src {
-  int lo, hi, off;
-  __device__ __forceinline__ bool valid() const { return lo <= hi; }
+  int lim_lo, lim_hi;
+  int lo, span, base;
+  __device__ __forceinline__ bool valid() const { return lim_lo <= lim_hi; }
 };
+constexpr int kFarDiagonal = 0x40000000;
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int lane_of(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 
-__device__ __forceinline__ Src get_src(const SlotMeta* meta, int s_src, int slot, int R, int c,
-                                       int ncomp, int wcap) {
-  Src r{1, -1, 0};
+template <int NC>
+__device__ __forceinline__ Src get_src(const MetaRegs<NC>& m, int s_src, int slot, int R, int c, int wcap) {
+  Src r{1, -1, kFarDiagonal, 0, 0};
   if (s_src < 0) return r;
   if (slot < 0) slot += R;
-  const int tag = uniform(meta[slot].tag);
-  const unsigned fl = (unsigned)uniform((int)meta[slot].flags);
+  slot = uniform(slot);
+  const int tag = lane_of(m.tag, slot);
+  const unsigned fl = (unsigned)lane_of((int)m.flags, slot);
   if (tag != s_src) return r;
   if (!(fl & (1u << c)) || (fl & (0x100u << c))) return r;
-  r.lo = uniform(meta[slot].lo[c]);
-  r.hi = uniform(meta[slot].hi[c]);
-  r.off = (slot * ncomp + c) * wcap - uniform(meta[slot].alo);
+  r.lim_lo = r.lo = lane_of(m.lo[c], slot);
+  r.lim_hi = lane_of(m.hi[c], slot);
+  r.span = r.lim_hi - r.lim_lo;
+  r.base = (slot * NC + c) * wcap + (r.lo - lane_of(m.alo, slot));
   return r;
 }
 
+// Branch-free predicated read: the load address is always inside the ring.
 template <typename RingPtr>
 __device__ __forceinline__ int rd(RingPtr ring, const Src& s, int k) {
-  return (k >= s.lo && k <= s.hi) ? ring[s.off + k] : kOffsetNull;
+  const int i = k - s.lo;
+  const bool in = (unsigned)i <= (unsigned)s.span;
+  const int v = (int)ring[s.base + (in ? i : 0)];
+  return in ? v : kOffsetNull;
 }
+
+// Ring element stores.  The 16-bit LDS ring saturates to [-16384, 16383]:
+// values below 0 are all NULL-like and values above the text length are all
+// out of bounds, so every comparison the recurrence makes keeps its outcome
+// (only used when both sequences are shorter than 8192).
+__device__ __forceinline__ int16_t to_ring(int v, int16_t*) {
+  return (int16_t)(v < -16384 ? -16384 : (v > 16383 ? 16383 : v));
+}
+__device__ __forceinline__ int to_ring(int v, int*) { return v; }
 
 __device__ __forceinline__ bool in_bounds(int off, int k, int plen, int tlen) {
   return (unsigned)off <= (unsigned)tlen && (unsigned)(off - k) <= (unsigned)plen;
@@ -71,12 +96,13 @@ __device__ __forceinline__ bool in_bounds(int off, int k, int plen, int tlen) {
 // sequence carries at least one padding word.
 template <int BITS, typename WordPtr>
 __device__ __forceinline__ uint64_t window(WordPtr W, int pos) {
-  constexpr int SPW = 64 / BITS;
-  const int w = pos / SPW;
-  const int sh = (pos % SPW) * BITS;
+  constexpr unsigned SPW = 64 / BITS;
+  const unsigned w = (unsigned)pos / SPW;
+  const unsigned sh = ((unsigned)pos % SPW) * BITS;
   const uint64_t lo = W[w];
   const uint64_t hi = W[w + 1];
-  return sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
+  // (hi << 1) << (63 - sh) is hi << (64 - sh) for sh > 0 and 0 for sh == 0.
+  return (lo >> sh) | ((hi << 1) << (63 - sh));
 }
 
 // Longest common extension of pattern[v..] and text[h..], bounded by both ends.
@@ -106,50 +132,76 @@ struct WaveRange {  // first/last valid diagonal seen by one wave (uniform)
     }
   }
   __device__ __forceinline__ void merge(int f, int l) { first = min(first, f); last = max(last, l); }
-  __device__ __forceinline__ void store(SlotMeta& m, int c, int lo, unsigned& fl) const {
-    fl |= 1u << c;
-    if (first == INT_MAX) { fl |= 0x100u << c; m.lo[c] = lo; m.hi[c] = lo - 1; }
-    else { m.lo[c] = first; m.hi[c] = last; }
-  }
+  __device__ __forceinline__ bool null() const { return first == INT_MAX; }
+  // Trimmed range as the reference stores it: [first valid, last valid], or (lo, lo-1).
+  __device__ __forceinline__ int tlo(int lo) const { return null() ? lo : first; }
+  __device__ __forceinline__ int thi(int lo) const { return null() ? lo - 1 : last; }
 };
+
+// Orders LDS traffic between the lanes of the (single) wave of a workgroup:
+// a wave executes its LDS instructions in program order, so only the
+// compiler has to be kept from moving accesses across this point.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 template <int NC, int BLOCK, bool SMALL, int BITS>
 __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
+  // The LDS family stores the ring as int16, the HBM family as int32.
+  using RT = typename std::conditional<SMALL, int16_t, int>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int NW = BLOCK / 64;
   constexpr int SPW = 64 / BITS;
-  __shared__ int sh_pair;
-  __shared__ unsigned long long sh_chunk;
+  // One wave and the ring in LDS: no workgroup barrier is ever needed.
+  constexpr bool WAVE_ONLY = SMALL && NW == 1;
+  __shared__ int sh_bcast[2];
   __shared__ int sh_red[NW > 1 ? NW : 1][2 * NC];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wid = tid >> 6;
   const Penalties pen = a.pen;
-  const int R = pen.nslots;
+  const int R = pen.nslots;  // <= 64 (checked by the host)
   const int wcap = a.wcap;
   const int g = pen.g;
   const int dq_x = pen.x / g, dq_o1 = (pen.o1 + pen.e1) / g, dq_e1 = pen.e1 / g;
   const int dq_o2 = (NC == 5) ? (pen.o2 + pen.e2) / g : 0;
   const int dq_e2 = (NC == 5) ? pen.e2 / g : 0;
 
-  SlotMeta* meta = reinterpret_cast<SlotMeta*>(smem);
-  uint64_t* seqP = reinterpret_cast<uint64_t*>(smem + (size_t)R * sizeof(SlotMeta));
+  uint64_t* seqP = reinterpret_cast<uint64_t*>(smem);
   uint64_t* seqT = seqP + a.lmax_words;
-  int* ring;
+  RT* ring;
   if constexpr (SMALL) {
-    ring = reinterpret_cast<int*>(seqT + a.lmax_words);
+    ring = reinterpret_cast<RT*>(seqT + a.lmax_words);
   } else {
     ring = a.ring_global + (long long)blockIdx.x * a.ring_stride;
   }
+  auto block_sync = [&]() {
+    if constexpr (WAVE_ONLY) wave_sync(); else __syncthreads();
+  };
+  // Lane 0 computes v; every thread of the workgroup receives it.
+  auto broadcast0 = [&](int v, int slot) -> int {
+    if constexpr (NW == 1) {
+      return lane_of(v, 0);
+    } else {
+      if (tid == 0) sh_bcast[slot] = v;
+      __syncthreads();
+      const int r = sh_bcast[slot];
+      __syncthreads();
+      return r;
+    }
+  };
 
-  unsigned long long chunk_cur = 0, chunk_end = 0;  // record chunk of this workgroup
+  unsigned long long chunk_cur = 0, chunk_end = 0;  // record chunk of this workgroup (16-byte units)
+  unsigned long long wg_dsteps = 0;                  // diagonal-steps of this workgroup
+  MetaRegs<NC> meta;
 
   for (;;) {
-    if (tid == 0) sh_pair = (int)atomicAdd(a.queue, 1u);
-    __syncthreads();
-    const int qi = sh_pair;
-    __syncthreads();
+    int qv = 0;
+    if (tid == 0) qv = (int)atomicAdd(a.queue, 1u);
+    const int qi = broadcast0(qv, 0);
     if (qi >= a.n_pairs) break;
     const int p = a.order ? a.order[qi] : qi;
     const int plen = a.plen[p];
@@ -165,6 +217,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
       const uint64_t* gP = a.words + a.pat_word[p];
       const uint64_t* gT = a.words + a.txt_word[p];
       const int nwp = plen / SPW + 2, nwt = tlen / SPW + 2;
+      block_sync();  // the previous pair is done reading the sequences
       for (int i = tid; i < nwp; i += BLOCK) seqP[i] = gP[i];
       for (int i = tid; i < nwt; i += BLOCK) seqT[i] = gT[i];
       P = seqP;
@@ -173,61 +226,57 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
       P = a.words + a.pat_word[p];
       T = a.words + a.txt_word[p];
     }
-    for (int r = tid; r < R; r += BLOCK) meta[r].tag = -1;
-    __syncthreads();
-
     // Score 0: the begin component holds offset 0 at k = 0 (wavefront_aligner.c:252-380).
+    meta.tag = (lane == 0) ? 0 : -1;
+    meta.alo = 0;
+    meta.flags = (lane == 0) ? (1u << a.comp_begin) : 0u;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) { meta.lo[c] = 0; meta.hi[c] = 0; }
     int cur = 0;
     int s = 0;
     int status = 1;  // running
     uint32_t prev_blk = kNoBlock;
     long long dsteps = 0;
-    if (tid == 0) {
-      SlotMeta& m = meta[0];
-      m.tag = 0;
-      m.alo = 0;
-      for (int c = 0; c < NC; ++c) { m.lo[c] = 0; m.hi[c] = 0; }
-      m.flags = 1u << a.comp_begin;
+    block_sync();  // sequences visible to every lane
+    {
       int v0 = 0;
-      if (a.comp_begin == kM) v0 = lcp_extend<BITS>(P, T, plen, tlen, 0, 0);
-      ring[(0 * NC + a.comp_begin) * wcap] = v0;
-      sh_pair = (a.comp_begin == kM && a.comp_end == kM && kend == 0 && v0 >= tlen) ? 1 : 0;
+      if (tid == 0 && a.comp_begin == kM) v0 = lcp_extend<BITS>(P, T, plen, tlen, 0, 0);
+      v0 = broadcast0(v0, 1);
+      if (tid == 0) ring[(0 * NC + a.comp_begin) * wcap] = to_ring(v0, ring);
+      if (a.comp_begin == kM && a.comp_end == kM && kend == 0 && v0 >= tlen) status = kDevCompleted;
     }
-    __syncthreads();
-    if (sh_pair) status = kDevCompleted;
-    __syncthreads();
+    block_sync();
 
     while (status == 1) {
       s += g;
       if (s >= a.max_steps) { status = kDevMaxSteps; break; }
       cur = (cur + 1 == R) ? 0 : cur + 1;
-      const Src mx  = get_src(meta, s - pen.x, cur - dq_x, R, kM, NC, wcap);
-      const Src mo1 = get_src(meta, s - pen.o1 - pen.e1, cur - dq_o1, R, kM, NC, wcap);
-      const Src i1  = get_src(meta, s - pen.e1, cur - dq_e1, R, kI1, NC, wcap);
-      const Src d1  = get_src(meta, s - pen.e1, cur - dq_e1, R, kD1, NC, wcap);
-      Src mo2{1, -1, 0}, i2{1, -1, 0}, d2{1, -1, 0};
+      const Src mx  = get_src(meta, s - pen.x, cur - dq_x, R, kM, wcap);
+      const Src mo1 = get_src(meta, s - pen.o1 - pen.e1, cur - dq_o1, R, kM, wcap);
+      const Src i1  = get_src(meta, s - pen.e1, cur - dq_e1, R, kI1, wcap);
+      const Src d1  = get_src(meta, s - pen.e1, cur - dq_e1, R, kD1, wcap);
+      Src mo2{1, -1, kFarDiagonal, 0, 0}, i2 = mo2, d2 = mo2;
       if constexpr (NC == 5) {
-        mo2 = get_src(meta, s - pen.o2 - pen.e2, cur - dq_o2, R, kM, NC, wcap);
-        i2  = get_src(meta, s - pen.e2, cur - dq_e2, R, kI2, NC, wcap);
-        d2  = get_src(meta, s - pen.e2, cur - dq_e2, R, kD2, NC, wcap);
+        mo2 = get_src(meta, s - pen.o2 - pen.e2, cur - dq_o2, R, kM, wcap);
+        i2  = get_src(meta, s - pen.e2, cur - dq_e2, R, kI2, wcap);
+        d2  = get_src(meta, s - pen.e2, cur - dq_e2, R, kD2, wcap);
       }
       const bool any = mx.valid() || mo1.valid() || i1.valid() || d1.valid() ||
                        mo2.valid() || i2.valid() || d2.valid();
       if (!any) {  // null step (wavefront_compute_affine.c:236-243)
         if (s >= s_cap) { status = kDevBadTrace; break; }
-        if (tid == 0) { meta[cur].tag = s; meta[cur].flags = 0; }
-        __syncthreads();
+        if (lane == cur) { meta.tag = s; meta.flags = 0u; }
         continue;
       }
       // wavefront_compute_limits_input (wavefront_compute.c:40-86)
-      int lo = mx.lo, hi = mx.hi;
-      lo = min(lo, mo1.lo - 1); hi = max(hi, mo1.hi + 1);
-      lo = min(lo, i1.lo + 1);  hi = max(hi, i1.hi + 1);
-      lo = min(lo, d1.lo - 1);  hi = max(hi, d1.hi - 1);
+      int lo = mx.lim_lo, hi = mx.lim_hi;
+      lo = min(lo, mo1.lim_lo - 1); hi = max(hi, mo1.lim_hi + 1);
+      lo = min(lo, i1.lim_lo + 1);  hi = max(hi, i1.lim_hi + 1);
+      lo = min(lo, d1.lim_lo - 1);  hi = max(hi, d1.lim_hi - 1);
       if constexpr (NC == 5) {
-        lo = min(lo, mo2.lo - 1); hi = max(hi, mo2.hi + 1);
-        lo = min(lo, i2.lo + 1);  hi = max(hi, i2.hi + 1);
-        lo = min(lo, d2.lo - 1);  hi = max(hi, d2.hi - 1);
+        lo = min(lo, mo2.lim_lo - 1); hi = max(hi, mo2.lim_hi + 1);
+        lo = min(lo, i2.lim_lo + 1);  hi = max(hi, i2.lim_hi + 1);
+        lo = min(lo, d2.lim_lo - 1);  hi = max(hi, d2.lim_hi - 1);
       }
       lo = uniform(lo);
       hi = uniform(hi);
@@ -245,10 +294,15 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
         const unsigned long long units = 1 + (unsigned long long)((width + 3) / 4);
         if (chunk_cur + units > chunk_end) {
           const unsigned long long want = units > kChunkUnits ? units : kChunkUnits;
-          __syncthreads();
-          if (tid == 0) sh_chunk = atomicAdd(a.rec_top, want);
-          __syncthreads();
-          chunk_cur = sh_chunk;
+          int base_lo = 0, base_hi = 0;
+          if (tid == 0) {
+            const unsigned long long b0 = atomicAdd(a.rec_top, want);
+            base_lo = (int)(unsigned)(b0 & 0xFFFFFFFFull);
+            base_hi = (int)(unsigned)(b0 >> 32);
+          }
+          base_lo = broadcast0(base_lo, 0);
+          base_hi = broadcast0(base_hi, 1);
+          chunk_cur = ((unsigned long long)(unsigned)base_hi << 32) | (unsigned)base_lo;
           chunk_end = chunk_cur + want;
           if (chunk_end > a.rec_units) {
             chunk_cur = chunk_end = 0;
@@ -263,11 +317,11 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
         rec = reinterpret_cast<uint32_t*>(a.rec_arena + blk + 1);
       }
 
-      int* outM  = ring + (cur * NC + kM) * wcap - lo;
-      int* outI1 = ring + (cur * NC + kI1) * wcap - lo;
-      int* outD1 = ring + (cur * NC + kD1) * wcap - lo;
-      int* outI2 = ring + (cur * NC + (NC == 5 ? kI2 : 0)) * wcap - lo;
-      int* outD2 = ring + (cur * NC + (NC == 5 ? kD2 : 0)) * wcap - lo;
+      RT* outM  = ring + (cur * NC + kM) * wcap - lo;
+      RT* outI1 = ring + (cur * NC + kI1) * wcap - lo;
+      RT* outD1 = ring + (cur * NC + kD1) * wcap - lo;
+      RT* outI2 = ring + (cur * NC + (NC == 5 ? kI2 : 0)) * wcap - lo;
+      RT* outD2 = ring + (cur * NC + (NC == 5 ? kD2 : 0)) * wcap - lo;
 
       WaveRange rM, rI1, rD1, rI2, rD2;
       rM.init(); rI1.init(); rD1.init(); rI2.init(); rD2.init();
@@ -303,13 +357,13 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
           if (!in_bounds(m, k, plen, tlen)) m = kOffsetNull;  // null-after-max
           if (rec) rec[k - lo] = ((uint32_t)best & kRecOffMask) | (src << kRecSrcShift) | bits;
           if (m != kOffsetNull) m = lcp_extend<BITS>(P, T, plen, tlen, k, m);
-          outM[k] = m;
+          outM[k] = to_ring(m, ring);
           vM = (m != kOffsetNull);
-          if (has_i1) { outI1[k] = ins1; vI1 = in_bounds(ins1, k, plen, tlen); }
-          if (has_d1) { outD1[k] = del1; vD1 = in_bounds(del1, k, plen, tlen); }
+          if (has_i1) { outI1[k] = to_ring(ins1, ring); vI1 = in_bounds(ins1, k, plen, tlen); }
+          if (has_d1) { outD1[k] = to_ring(del1, ring); vD1 = in_bounds(del1, k, plen, tlen); }
           if constexpr (NC == 5) {
-            if (has_i2) { outI2[k] = ins2; vI2 = in_bounds(ins2, k, plen, tlen); }
-            if (has_d2) { outD2[k] = del2; vD2 = in_bounds(del2, k, plen, tlen); }
+            if (has_i2) { outI2[k] = to_ring(ins2, ring); vI2 = in_bounds(ins2, k, plen, tlen); }
+            if (has_d2) { outD2[k] = to_ring(del2, ring); vD2 = in_bounds(del2, k, plen, tlen); }
           }
         }
         const int kw = kb + wid * 64;
@@ -344,21 +398,21 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
           }
         }
       }
-      __syncthreads();  // every lane is done with the ring and sh_red of this step
-      // Trimmed ranges: [first valid, last valid], or null (lo, lo-1).
-      if (tid == 0) {
-        SlotMeta& m = meta[cur];
-        m.tag = s;
-        m.alo = lo;
-        unsigned fl = 0;
-        rM.store(m, kM, lo, fl);
-        if (has_i1) rI1.store(m, kI1, lo, fl);
-        if (has_d1) rD1.store(m, kD1, lo, fl);
+      block_sync();  // ring values of this step visible to every lane; sh_red free again
+      // Record the step in the slot registers (every wave keeps a copy).
+      if (lane == cur) {
+        meta.tag = s;
+        meta.alo = lo;
+        unsigned fl = 1u | (rM.null() ? 0x100u : 0u);
+        meta.lo[kM] = rM.tlo(lo);
+        meta.hi[kM] = rM.thi(lo);
+        if (has_i1) { fl |= 2u | (rI1.null() ? 0x200u : 0u); meta.lo[kI1] = rI1.tlo(lo); meta.hi[kI1] = rI1.thi(lo); }
+        if (has_d1) { fl |= 4u | (rD1.null() ? 0x400u : 0u); meta.lo[kD1] = rD1.tlo(lo); meta.hi[kD1] = rD1.thi(lo); }
         if constexpr (NC == 5) {
-          if (has_i2) rI2.store(m, kI2, lo, fl);
-          if (has_d2) rD2.store(m, kD2, lo, fl);
+          if (has_i2) { fl |= 8u | (rI2.null() ? 0x800u : 0u); meta.lo[kI2] = rI2.tlo(lo); meta.hi[kI2] = rI2.thi(lo); }
+          if (has_d2) { fl |= 16u | (rD2.null() ? 0x1000u : 0u); meta.lo[kD2] = rD2.tlo(lo); meta.hi[kD2] = rD2.thi(lo); }
         }
-        m.flags = fl;
+        meta.flags = fl;
       }
       // Termination (wavefront_termination.c:37-114): k_end inside the trimmed
       // range of the end component and its offset reaching the text end.
@@ -372,22 +426,24 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
           if (ce == kI2) { pce = has_i2; rce = rI2; }
           if (ce == kD2) { pce = has_d2; rce = rD2; }
         }
-        if (pce && rce.first != INT_MAX && kend >= rce.first && kend <= rce.last) {
-          const int vend = ring[(cur * NC + ce) * wcap + (kend - lo)];
+        if (pce && !rce.null() && kend >= rce.first && kend <= rce.last) {
+          const int vend = (int)ring[(cur * NC + ce) * wcap + (kend - lo)];
           if (vend >= tlen) status = kDevCompleted;
         }
       }
       if (status == 1 && s >= s_cap) status = kDevBadTrace;  // unreachable for valid input
-      __syncthreads();
     }
     if (tid == 0) {
       a.out_status[p] = (status == 1) ? kDevCompleted : status;
       a.out_score[p] = s;
       a.out_last_block[p] = prev_blk;
       a.out_dsteps[p] = dsteps;
+      if (status == kDevRingOverflow || status == kDevArenaFull) atomicAdd(a.retry_count, 1u);
+      else if (status != 1 && status != kDevCompleted) atomicAdd(a.fail_count, 1u);
     }
-    __syncthreads();
+    if (status == 1 || status == kDevCompleted || status == kDevMaxSteps) wg_dsteps += (unsigned long long)dsteps;
   }
+  if (tid == 0 && wg_dsteps) atomicAdd(a.dsteps_total, wg_dsteps);
 }
 
 // One lane per pair: replays the records from the end cell back to the origin
@@ -481,12 +537,21 @@ __global__ void __launch_bounds__(256) wfa_backtrace_kernel(BacktraceArgs a) {
       while (h > 0) { out[--pos] = 'I'; --h; }
     }
   }
-  if (bad) { a.status[p] = kDevBadTrace; pos = cap; }
+  if (bad) {
+    a.status[p] = kDevBadTrace;
+    pos = cap;
+    atomicAdd(a.fail_count, 1u);
+  }
   a.ops_begin[p] = pos;
 }
 
-__global__ void wfa_reset_kernel(unsigned int* queue, unsigned long long* rec_top) {
-  if (threadIdx.x == 0) { *queue = 0u; *rec_top = 0ull; }
+__global__ void wfa_reset_kernel(Counters* c, int all) {
+  if (threadIdx.x == 0) {
+    c->queue = 0u;
+    c->rec_top = 0ull;
+    if (all >= 1) c->retry = 0u;
+    if (all >= 2) { c->dsteps = 0ull; c->fail = 0u; }
+  }
 }
 
 template <int NC, int BLOCK, bool SMALL, int BITS>
@@ -510,10 +575,10 @@ const void* select_forward(int family, int two_piece, int bits, int block) {
 }  // namespace
 
 size_t forward_lds_bytes(int family, int ncomp, int nslots, int wcap, int lmax_words) {
-  size_t b = (size_t)nslots * sizeof(SlotMeta);
+  size_t b = 0;
   if (family == kSmall) {
     b += (size_t)2 * lmax_words * sizeof(uint64_t);
-    b += (size_t)nslots * ncomp * wcap * sizeof(int);
+    b += (size_t)nslots * ncomp * wcap * sizeof(int16_t);
   }
   return b;
 }
@@ -553,8 +618,8 @@ hipError_t launch_backtrace(const BacktraceArgs& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_reset_counters(unsigned int* queue, unsigned long long* rec_top, hipStream_t stream) {
-  hipLaunchKernelGGL(wfa_reset_kernel, dim3(1), dim3(64), 0, stream, queue, rec_top);
+hipError_t launch_reset_counters(Counters* c, int all, hipStream_t stream) {
+  hipLaunchKernelGGL(wfa_reset_kernel, dim3(1), dim3(64), 0, stream, c, all);
   return hipGetLastError();
 }
 
