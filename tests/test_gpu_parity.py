@@ -38,7 +38,7 @@ def assert_same(got, ref, ps, check_dsteps=None):
         assert np.array_equal(check_dsteps, ref.dsteps)
 
 
-GOLDEN_GPU = [c for c in G.cases() if c[3] != "ultralow"]
+GOLDEN_GPU = G.cases()
 
 
 @pytest.fixture(scope="module")
@@ -169,3 +169,45 @@ def test_device_batch_rerun_is_identical():
     b = db.download()
     assert np.array_equal(a.score, b.score)
     assert all(a.ops(i) == b.ops(i) for i in range(ps.n))
+
+
+@pytest.mark.parametrize("metric,pen,shape,n,seed", [
+    ("affine", (0, 4, 6, 2), 2, 6, 801),
+    ("affine2p", (0, 4, 6, 2, 24, 1), 2, 4, 802),
+    ("affine", (0, 4, 6, 2), 1, 300, 803),
+    ("affine", (-3, 5, 3, 2), 2, 4, 804),
+])
+def test_biwfa_vs_oracle(metric, pen, shape, n, seed):
+    say("BiWFA %s %s shape %d n=%d" % (metric, pen, shape, n))
+    ps = datagen.generate(shape, seed, n)
+    if shape == 2:
+        keep = [i for i in range(ps.n) if ps.plen[i] <= 12000]
+        ps = ps.subset(keep)
+    al = wfa_amd.Aligner(metric=metric, penalties=pen, memory="ultralow")
+    db = al.upload(ps)
+    assert db.run() == 0
+    got = db.download()
+    ref = oracle_run(metric, pen, ps, memory="ultralow")
+    assert_same(got, ref, ps)
+    assert db.stats().diagonal_steps == int(ref.dsteps.sum())
+
+
+def test_biwfa_score_only_vs_oracle():
+    say("BiWFA score-only")
+    ps = datagen.generate(2, 805, 4)
+    al = wfa_amd.Aligner(metric="affine", penalties=(0, 4, 6, 2), memory="ultralow", score_only=True)
+    got = al.align_batch(ps)
+    ref = oracle_run("affine", (0, 4, 6, 2), ps, memory="ultralow", score_only=True)
+    assert np.array_equal(got.status, ref.status) and np.array_equal(got.score, ref.score)
+
+
+def test_biwfa_long_noisy_pair_vs_oracle():
+    say("BiWFA 30 kbp 15% pair")
+    # a 100 kbp / 15% pair cut to 30 kbp keeps the run short on the oracle
+    full = datagen.generate(3, 806, 1)
+    p, t = full.pattern(0)[:30000], full.text(0)[:30000]
+    ps = datagen.PairSet.from_lists([p], [t])
+    al = wfa_amd.Aligner(metric="affine", penalties=(0, 4, 6, 2), memory="ultralow")
+    got = al.align_batch(ps)
+    ref = oracle_run("affine", (0, 4, 6, 2), ps, memory="ultralow")
+    assert_same(got, ref, ps)

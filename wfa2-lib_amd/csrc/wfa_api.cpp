@@ -1,7 +1,7 @@
 // C-ABI host layer: the drop-in wavefront_aligner_new / wavefront_align /
 // wavefront_aligner_delete entry points (wavefront/wavefront_align.c:212-241,
 // wavefront/wavefront_aligner.c:422-504) plus wavefront_align_batch and the
-// device-resident batch API, all driving the HIP kernels of wfa_kernels.hip.
+// device-resident batch API, all driving the HIP kernels.
 //
 // One aligner owns one HIP stream and a grow-only device arena.  A batch is
 // packed on the host (2 bits per base when every byte is A/C/G/T, else one
@@ -9,9 +9,10 @@
 //   1. forward kernels, one launch per kernel family (pairs sorted by length;
 //      short pairs on the LDS-resident one-wavefront family, long pairs on the
 //      workgroup family with the wavefront ring in HBM);
-//   2. one backtrace kernel over the pairs that completed;
+//   2. one backtrace kernel per family launch;
 //   3. a retry pass, only for pairs that overflowed the LDS ring (moved to the
 //      HBM-ring family) or the record arena (arena reset, smaller pass).
+// Memory mode ultralow (BiWFA) is driven by wfa_bialign.cpp on the same jobs.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,13 +25,12 @@
 #include <thread>
 #include <vector>
 
-#include "wfa_device.h"
-#include "wfa_mi355x.h"
+#include "wfa_internal.h"
 
-namespace {
+namespace wfa_host {
 
-using wfa::ForwardArgs;
 using wfa::BacktraceArgs;
+using wfa::ForwardArgs;
 using wfa::LaunchShape;
 using wfa::Penalties;
 
@@ -40,94 +40,35 @@ bool hip_ok(hipError_t e, const char* what) {
   return false;
 }
 
-struct DevBuf {
-  void* p = nullptr;
-  size_t cap = 0;
-  bool ensure(size_t bytes) {
-    if (bytes <= cap && p) return true;
-    if (p) (void)hipFree(p);
+bool DevBuf::ensure(size_t bytes) {
+  if (bytes <= cap && p) return true;
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  cap = 0;
+  const size_t want = std::max<size_t>(bytes, 256);
+  if (hipMalloc(&p, want) != hipSuccess) {
     p = nullptr;
-    cap = 0;
-    size_t want = std::max<size_t>(bytes, 256);
-    if (hipMalloc(&p, want) != hipSuccess) { p = nullptr; return false; }
-    cap = want;
-    return true;
+    return false;
   }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    cap = 0;
-  }
-  template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
-};
+  cap = want;
+  return true;
+}
 
-// Per-aligner device state.
-struct Engine {
-  int device = 0;
-  int num_cus = 0;
-  size_t lds_per_cu = 0;
-  hipStream_t stream = nullptr;
-  DevBuf rec;        // backtrace-record arena
-  DevBuf ring;       // HBM wavefront rings of the workgroup family
-  DevBuf counters;   // queue counter + arena bump pointer
-  size_t rec_bytes = 0;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  bool ok = false;
-};
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  cap = 0;
+}
 
-int gcd(int a, int b) { while (b) { int t = a % b; a = b; b = t; } return a; }
-
-}  // namespace
-
-// ----------------------------------------------------------------- batches
-
-struct PlanGroup {
-  int family = 0;
-  int block = 64;
-  int wcap = 0;
-  int lmax_words = 0;
-  std::vector<int> ids;
-  DevBuf d_ids;
-  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-  void release() {
-    d_ids.release();
-    for (auto& e : ev)
-      if (e) (void)hipEventDestroy(e);
-    ev[0] = ev[1] = ev[2] = nullptr;
-  }
-};
-
-struct wfa_device_batch_t {
-  int n = 0;
-  int bits = 2;
-  int64_t total_ops = 0;
-  std::vector<int> plen, tlen;
-  std::vector<int64_t> ops_off;
-  std::vector<int> sorted;           // pair ids, longest first
-  DevBuf words, d_plen, d_tlen, d_patw, d_txtw, d_status, d_score, d_last, d_dsteps,
-         d_ops_off, d_ops_begin, d_ops;
-  // launch plan, cached for the penalties it was made for
-  std::vector<PlanGroup> plan;
-  std::vector<int> plan_key;
-  // results: on the device after a run; copied to the host on demand
-  std::vector<int> status, score, ops_begin;
-  bool have_results = false;
-  bool host_results = false;
-  bool ops_on_device = false;
-  int first_fail = 0;
-  wfa_run_stats_t stats{};
-  int records = 1;
-  ~wfa_device_batch_t() {
-    for (auto& g : plan) g.release();
-    for (DevBuf* b : {&words, &d_plen, &d_tlen, &d_patw, &d_txtw, &d_status, &d_score, &d_last,
-                      &d_dsteps, &d_ops_off, &d_ops_begin, &d_ops})
-      b->release();
-  }
-};
-
-namespace {
+void PlanGroup::release() {
+  d_ids.release();
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  ev[0] = ev[1] = ev[2] = nullptr;
+}
 
 Engine* engine_of(wavefront_aligner_t* wf) { return static_cast<Engine*>(wf->device); }
+wfa::Counters* counters_of(Engine* e) { return e->counters.as<wfa::Counters>(); }
 
 bool engine_init(Engine* e) {
   if (e->ok) return true;
@@ -135,10 +76,7 @@ bool engine_init(Engine* e) {
   hipDeviceProp_t prop;
   if (!hip_ok(hipGetDeviceProperties(&prop, e->device), "hipGetDeviceProperties")) return false;
   e->num_cus = prop.multiProcessorCount;
-  e->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
   if (!hip_ok(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "hipStreamCreate")) return false;
-  for (auto& ev : e->ev)
-    if (!hip_ok(hipEventCreate(&ev), "hipEventCreate")) return false;
   if (!e->counters.ensure(64)) return false;
   const char* env = getenv("WFA_RECORD_ARENA_MB");
   size_t mb = env ? strtoull(env, nullptr, 10) : 8192;
@@ -150,20 +88,33 @@ bool engine_init(Engine* e) {
   return true;
 }
 
+namespace {
+
 void engine_free(Engine* e) {
   if (!e) return;
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   e->rec.release();
   e->ring.release();
   e->counters.release();
-  for (auto& ev : e->ev)
-    if (ev) (void)hipEventDestroy(ev);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
 }
 
-Penalties make_penalties(const wavefront_penalties_t& p, bool two) {
-  Penalties r{};
+}  // namespace
+
+namespace {
+int gcd(int a, int b) {
+  while (b) {
+    const int t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+}  // namespace
+
+wfa::Penalties make_penalties(const wavefront_penalties_t& p, bool two) {
+  wfa::Penalties r{};
   r.x = p.mismatch;
   r.o1 = p.gap_opening1;
   r.e1 = p.gap_extension1;
@@ -250,6 +201,9 @@ int classic_score(const wavefront_penalties_t& p, int plen, int tlen, int s) {
   return (swg_match * (plen + tlen) - s) / 2;
 }
 
+
+namespace {
+
 // Configurations the exact device path covers; everything else is refused.
 int check_supported(wavefront_aligner_t* wf) {
   const distance_metric_t m = wf->penalties.distance_metric;
@@ -258,7 +212,6 @@ int check_supported(wavefront_aligner_t* wf) {
   else if (wf->alignment_form.span != alignment_end2end || wf->alignment_form.extension)
     why = "only end-to-end alignment";
   else if (wf->heuristic.strategy != wf_heuristic_none) why = "only heuristic none (exact alignment)";
-  else if (wf->memory_mode == wavefront_memory_ultralow) why = "memory mode ultralow (BiWFA) is not enabled in this build";
   else if (make_penalties(wf->penalties, m == gap_affine_2p).nslots > 64)
     why = "penalties whose score scope needs more than 64 wavefront slots";
   if (!why) return WF_STATUS_OK;
@@ -311,6 +264,292 @@ int small_wcap(const Penalties& pen, int ncomp) {
 }
 
 }  // namespace
+
+// ------------------------------------------------------------ planning
+
+namespace {
+
+PlanGroup make_large(const JobSet& js, std::vector<int> ids) {
+  PlanGroup g;
+  g.family = wfa::kLarge;
+  int pmax = 0, tmax = 0;
+  for (int i : ids) { pmax = std::max(pmax, js.h_plen[i]); tmax = std::max(tmax, js.h_tlen[i]); }
+  g.wcap = pmax + tmax + 8;
+  g.lmax_words = 0;
+  g.block = wfa::pick_block(wfa::kLarge, pmax + tmax);
+  g.ids = std::move(ids);
+  return g;
+}
+
+LaunchShape shape_of(Engine* e, const PlanGroup& g, int two, int bits, const Penalties& pen) {
+  LaunchShape shape;
+  shape.family = g.family;
+  shape.block = g.block;
+  shape.lds_bytes = wfa::forward_lds_bytes(g.family, two ? 5 : 3, pen.nslots, g.wcap, g.lmax_words);
+  int per_cu = wfa::forward_max_blocks_per_cu(g.family, g.block, two, bits, shape.lds_bytes);
+  if (per_cu <= 0) per_cu = 1;
+  shape.grid = (int)std::min<int64_t>((int64_t)g.ids.size(), (int64_t)per_cu * e->num_cus);
+  return shape;
+}
+
+// Enqueues forward + backtrace of one group on the aligner's stream.
+int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGroup& grp, int two,
+                  const Penalties& pen, int records) {
+  const int n = (int)grp.ids.size();
+  if (n == 0) return WF_STATUS_OK;
+  hipStream_t st = e->stream;
+  const LaunchShape shape = shape_of(e, grp, two, js.bits, pen);
+  if (!grp.d_ids.p) {
+    if (!grp.d_ids.ensure((size_t)n * 4)) return WF_STATUS_OOM;
+    if (!hip_ok(hipMemcpyAsync(grp.d_ids.p, grp.ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, st), "plan upload"))
+      return WF_STATUS_OOM;
+  }
+  for (auto& ev : grp.ev)
+    if (!ev && !hip_ok(hipEventCreate(&ev), "hipEventCreate")) return WF_STATUS_OOM;
+  wfa::Counters* c = counters_of(e);
+  ForwardArgs fa{};
+  fa.n_pairs = n;
+  fa.order = grp.d_ids.as<int>();
+  fa.plen = js.plen;
+  fa.tlen = js.tlen;
+  fa.words = js.words;
+  fa.pat_pos = js.ppos;
+  fa.txt_pos = js.tpos;
+  fa.comp_begin_p = js.cbeg;
+  fa.comp_end_p = js.cend;
+  fa.out_status = js.status;
+  fa.out_score = js.score;
+  fa.out_dsteps = js.dsteps;
+  fa.out_last_block = js.last;
+  fa.rec_arena = e->rec.as<uint4>();
+  fa.rec_units = records ? std::min<unsigned long long>(e->rec.cap / 16, 0xFFFFFFF0ull) : 0;
+  fa.rec_top = &c->rec_top;
+  fa.ring_global = grp.family == wfa::kLarge ? e->ring.as<int>() : nullptr;
+  fa.ring_stride = grp.family == wfa::kLarge ? (long long)pen.nslots * (two ? 5 : 3) * grp.wcap : 0;
+  fa.queue = &c->queue;
+  fa.dsteps_total = &c->dsteps;
+  fa.retry_count = &c->retry;
+  fa.fail_count = &c->fail;
+  fa.wcap = grp.wcap;
+  fa.lmax_words = grp.lmax_words;
+  fa.max_steps = wf->system.max_alignment_steps > 0 ? wf->system.max_alignment_steps : INT_MAX;
+  fa.records = records;
+  fa.comp_begin = wfa::kM;
+  fa.comp_end = wfa::kM;
+  fa.pen = pen;
+  if (!hip_ok(wfa::launch_reset_counters(c, 0, st), "reset")) return WF_STATUS_OOM;
+  (void)hipEventRecord(grp.ev[0], st);
+  if (!hip_ok(wfa::launch_forward(fa, two, js.bits, shape, st), "forward launch")) return WF_STATUS_OOM;
+  (void)hipEventRecord(grp.ev[1], st);
+  if (records) {
+    BacktraceArgs ba{};
+    ba.n_pairs = n;
+    ba.order = grp.d_ids.as<int>();
+    ba.plen = js.plen;
+    ba.tlen = js.tlen;
+    ba.status = js.status;
+    ba.score = js.score;
+    ba.last_block = js.last;
+    ba.rec_arena = e->rec.as<uint4>();
+    ba.rec_units = fa.rec_units;
+    ba.ops = js.ops;
+    ba.ops_off = js.ops_off;
+    ba.ops_begin = js.ops_begin;
+    ba.fail_count = &c->fail;
+    ba.comp_end_p = js.cend;
+    ba.comp_begin = wfa::kM;
+    ba.comp_end = wfa::kM;
+    ba.pen = pen;
+    if (!hip_ok(wfa::launch_backtrace(ba, st), "backtrace launch")) return WF_STATUS_OOM;
+  }
+  (void)hipEventRecord(grp.ev[2], st);
+  return WF_STATUS_OK;
+}
+
+// Device memory every group of the plan needs, allocated before any launch.
+bool reserve_for(Engine* e, std::vector<PlanGroup>& plan, int two, int bits, const Penalties& pen, int records) {
+  if (records && !e->rec.ensure(e->rec_bytes)) {
+    fprintf(stderr, "[WFA] cannot allocate the %zu MB record arena\n", e->rec_bytes >> 20);
+    return false;
+  }
+  size_t ring = 0;
+  for (const PlanGroup& g : plan) {
+    if (g.family != wfa::kLarge) continue;
+    const LaunchShape sh = shape_of(e, g, two, bits, pen);
+    ring = std::max(ring, (size_t)pen.nslots * (two ? 5 : 3) * g.wcap * sh.grid * sizeof(int));
+  }
+  if (ring && !e->ring.ensure(ring)) {
+    fprintf(stderr, "[WFA] cannot allocate %zu MB of wavefront rings\n", ring >> 20);
+    return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+// Splits the jobs over kernel families: short jobs on the LDS family, the
+// rest on the HBM-ring family by workgroup size.
+std::vector<PlanGroup> make_plan(const JobSet& js, const std::vector<int>& ids, const Penalties& pen, int ncomp) {
+  const int wcap_small = small_wcap(pen, ncomp);
+  const int spw = 64 / js.bits;
+  std::vector<int> small_ids, cls[3];
+  int lmax_small = 0;
+  for (int i : ids) {
+    const int L = std::max(js.h_plen[i], js.h_tlen[i]);
+    if (wcap_small >= 64 && L <= 1024) {
+      small_ids.push_back(i);
+      lmax_small = std::max<int>(lmax_small, L / spw + 3);
+    } else {
+      const int w = js.h_plen[i] + js.h_tlen[i];
+      cls[w <= 256 ? 0 : (w <= 4096 ? 1 : 2)].push_back(i);
+    }
+  }
+  std::vector<PlanGroup> plan;
+  if (!small_ids.empty()) {
+    PlanGroup g;
+    g.family = wfa::kSmall;
+    g.block = 64;
+    g.wcap = wcap_small;
+    g.lmax_words = lmax_small;
+    g.ids = std::move(small_ids);
+    plan.push_back(std::move(g));
+  }
+  for (auto& c : cls)
+    if (!c.empty()) plan.push_back(make_large(js, std::move(c)));
+  return plan;
+}
+
+int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<PlanGroup>& plan, int records,
+             RunTotals* tot, std::vector<int>& h_status) {
+  const int two = wf->penalties.distance_metric == gap_affine_2p;
+  const Penalties pen = make_penalties(wf->penalties, two);
+  hipStream_t st = e->stream;
+  wfa::Counters host_c{};
+  std::vector<PlanGroup> retry_groups;
+  std::vector<PlanGroup>* pending = &plan;
+  if (!hip_ok(wfa::launch_reset_counters(counters_of(e), 2, st), "reset")) return WF_STATUS_OOM;
+  for (int round = 0;; ++round) {
+    if (round > 48) {
+      fprintf(stderr, "[WFA] batch did not converge (record arena too small?)\n");
+      return WF_STATUS_OOM;
+    }
+    if (!reserve_for(e, *pending, two, js.bits, pen, records)) return WF_STATUS_OOM;
+    if (!hip_ok(wfa::launch_reset_counters(counters_of(e), 1, st), "reset")) return WF_STATUS_OOM;
+    for (PlanGroup& g : *pending) {
+      const int rc = enqueue_group(wf, e, js, g, two, pen, records);
+      if (rc != WF_STATUS_OK) return rc;
+      ++tot->launches;
+    }
+    if (!hip_ok(hipMemcpyAsync(&host_c, e->counters.p, sizeof(host_c), hipMemcpyDeviceToHost, st), "counters") ||
+        !hip_ok(hipStreamSynchronize(st), "alignment"))
+      return WF_STATUS_OOM;
+    for (PlanGroup& g : *pending) {
+      float f = 0, t = 0;
+      if (g.ids.empty()) continue;
+      (void)hipEventElapsedTime(&f, g.ev[0], g.ev[1]);
+      (void)hipEventElapsedTime(&t, g.ev[1], g.ev[2]);
+      tot->forward_ms += f;
+      tot->backtrace_ms += t;
+    }
+    if (host_c.retry == 0) break;
+    // Some jobs overflowed the LDS ring or the record arena: re-run only them.
+    h_status.resize(js.n);
+    if (!hip_ok(hipMemcpy(h_status.data(), js.status, (size_t)js.n * 4, hipMemcpyDeviceToHost), "status"))
+      return WF_STATUS_OOM;
+    std::vector<PlanGroup> next;
+    for (PlanGroup& g : *pending) {
+      std::vector<int> ring_over, arena_full;
+      for (int i : g.ids) {
+        if (h_status[i] == wfa::kDevRingOverflow) ring_over.push_back(i);
+        else if (h_status[i] == wfa::kDevArenaFull) arena_full.push_back(i);
+      }
+      if (!ring_over.empty()) next.push_back(make_large(js, std::move(ring_over)));
+      if (!arena_full.empty()) {
+        if (arena_full.size() == g.ids.size()) {
+          // not a single job of the group fitted: grow the arena
+          size_t free_b = 0, total_b = 0;
+          (void)hipMemGetInfo(&free_b, &total_b);
+          if (e->rec_bytes * 2 > e->rec.cap + free_b / 2) {
+            fprintf(stderr, "[WFA] record arena exhausted (%zu MB)\n", e->rec_bytes >> 20);
+            for (int i : arena_full) h_status[i] = WF_STATUS_OOM;
+            if (!hip_ok(hipMemcpy(js.status, h_status.data(), (size_t)js.n * 4, hipMemcpyHostToDevice), "status"))
+              return WF_STATUS_OOM;
+            host_c.fail += (unsigned)arena_full.size();
+            arena_full.clear();
+          } else {
+            e->rec.release();
+            e->rec_bytes *= 2;
+          }
+        }
+        if (!arena_full.empty()) {
+          PlanGroup h;
+          h.family = g.family;
+          h.block = g.block;
+          h.wcap = g.wcap;
+          h.lmax_words = g.lmax_words;
+          h.ids = std::move(arena_full);
+          next.push_back(std::move(h));
+        }
+      }
+    }
+    for (auto& g : retry_groups) g.release();
+    retry_groups = std::move(next);
+    pending = &retry_groups;
+    if (retry_groups.empty()) break;
+  }
+  for (auto& g : retry_groups) g.release();
+  tot->dsteps += host_c.dsteps;
+  tot->fail += host_c.fail;
+  return WF_STATUS_OK;
+}
+
+bool download_results(Engine* e, wfa_device_batch_t* b) {
+  if (b->host_results) return true;
+  hipStream_t st = e->stream;
+  const size_t n = (size_t)b->n;
+  b->status.resize(n);
+  b->score.resize(n);
+  b->ops_begin.resize(n);
+  bool ok = n == 0 ||
+            (hip_ok(hipMemcpyAsync(b->status.data(), b->d_status.p, n * 4, hipMemcpyDeviceToHost, st), "download") &&
+             hip_ok(hipMemcpyAsync(b->score.data(), b->d_score.p, n * 4, hipMemcpyDeviceToHost, st), "download") &&
+             hip_ok(hipMemcpyAsync(b->ops_begin.data(), b->d_ops_begin.p, n * 4, hipMemcpyDeviceToHost, st), "download"));
+  ok = ok && hip_ok(hipStreamSynchronize(st), "download sync");
+  b->host_results = ok;
+  return ok;
+}
+
+}  // namespace wfa_host
+
+using namespace wfa_host;
+
+wfa_host::JobSet wfa_device_batch_t::jobs() const {
+  JobSet js;
+  js.n = n;
+  js.bits = bits;
+  js.h_plen = plen.data();
+  js.h_tlen = tlen.data();
+  js.words = words.as<uint64_t>();
+  js.plen = d_plen.as<int>();
+  js.tlen = d_tlen.as<int>();
+  js.ppos = d_patw.as<int64_t>();
+  js.tpos = d_txtw.as<int64_t>();
+  js.status = d_status.as<int>();
+  js.score = d_score.as<int>();
+  js.last = d_last.as<uint32_t>();
+  js.dsteps = d_dsteps.as<long long>();
+  js.ops = d_ops.as<char>();
+  js.ops_off = d_ops_off.as<int64_t>();
+  js.ops_begin = d_ops_begin.as<int>();
+  return js;
+}
+
+wfa_device_batch_t::~wfa_device_batch_t() {
+  for (auto& g : plan) g.release();
+  for (wfa_host::DevBuf* x : {&words, &d_plen, &d_tlen, &d_patw, &d_txtw, &d_status, &d_score, &d_last, &d_dsteps,
+                              &d_ops_off, &d_ops_begin, &d_ops, &rev_words, &d_rev_ppos, &d_rev_tpos})
+    x->release();
+}
 
 // ------------------------------------------------------------ lifecycle
 
@@ -367,8 +606,8 @@ extern "C" size_t wavefront_align_batch_arena_bytes(const wfa_pair_t* pairs, int
   return b;
 }
 
-extern "C" wfa_device_batch_t* wfa_device_batch_upload(wavefront_aligner_t* const wf,
-                                                      const wfa_pair_t* pairs, int n) {
+extern "C" wfa_device_batch_t* wfa_device_batch_upload(wavefront_aligner_t* const wf, const wfa_pair_t* pairs,
+                                                      int n) {
   Engine* e = engine_of(wf);
   if (!e || !engine_init(e) || n < 0) return nullptr;
   wfa_device_batch_t* b = new (std::nothrow) wfa_device_batch_t();
@@ -378,7 +617,9 @@ extern "C" wfa_device_batch_t* wfa_device_batch_upload(wavefront_aligner_t* cons
   b->plen.resize(n);
   b->tlen.resize(n);
   b->ops_off.resize(n);
-  std::vector<int64_t> patw(n), txtw(n);
+  b->pat_pos.resize(n);
+  b->txt_pos.resize(n);
+  std::vector<int64_t> patw(n), txtw(n);  // word index of each sequence
   int64_t words = 0, ops = 0;
   for (int i = 0; i < n; ++i) {
     b->plen[i] = pairs[i].pattern_length;
@@ -391,6 +632,7 @@ extern "C" wfa_device_batch_t* wfa_device_batch_upload(wavefront_aligner_t* cons
     ops += (int64_t)b->plen[i] + b->tlen[i];
   }
   b->total_ops = ops;
+  b->words_used = words;
   std::vector<uint64_t> host_words((size_t)words);
   {
     const int nt = std::max(1, std::min<int>(16, (int)std::thread::hardware_concurrency()));
@@ -413,195 +655,36 @@ extern "C" wfa_device_batch_t* wfa_device_batch_upload(wavefront_aligner_t* cons
   const size_t nn = std::max(n, 1);
   bool ok = b->words.ensure(std::max<size_t>(words, 1) * 8) && b->d_plen.ensure(nn * 4) &&
             b->d_tlen.ensure(nn * 4) && b->d_patw.ensure(nn * 8) && b->d_txtw.ensure(nn * 8) &&
-            b->d_status.ensure(nn * 4) && b->d_score.ensure(nn * 4) &&
-            b->d_last.ensure(nn * 4) && b->d_dsteps.ensure(nn * 8) && b->d_ops_off.ensure(nn * 8) &&
-            b->d_ops_begin.ensure(nn * 4);
+            b->d_status.ensure(nn * 4) && b->d_score.ensure(nn * 4) && b->d_last.ensure(nn * 4) &&
+            b->d_dsteps.ensure(nn * 8) && b->d_ops_off.ensure(nn * 8) && b->d_ops_begin.ensure(nn * 4);
   if (!ok) {
     fprintf(stderr, "[WFA] device allocation failed for a batch of %d pairs\n", n);
     delete b;
     return nullptr;
   }
+  // Symbol positions (word index times symbols per word) for the kernels.
+  const int64_t spw = 64 / b->bits;
+  for (int i = 0; i < n; ++i) {
+    b->pat_pos[i] = patw[i] * spw;
+    b->txt_pos[i] = txtw[i] * spw;
+  }
   hipStream_t st = e->stream;
-  ok = hip_ok(hipMemcpyAsync(b->words.p, host_words.data(), words * 8, hipMemcpyHostToDevice, st), "upload") &&
-       hip_ok(hipMemcpyAsync(b->d_plen.p, b->plen.data(), n * 4, hipMemcpyHostToDevice, st), "upload") &&
-       hip_ok(hipMemcpyAsync(b->d_tlen.p, b->tlen.data(), n * 4, hipMemcpyHostToDevice, st), "upload") &&
-       hip_ok(hipMemcpyAsync(b->d_patw.p, patw.data(), n * 8, hipMemcpyHostToDevice, st), "upload") &&
-       hip_ok(hipMemcpyAsync(b->d_txtw.p, txtw.data(), n * 8, hipMemcpyHostToDevice, st), "upload") &&
-       hip_ok(hipMemcpyAsync(b->d_ops_off.p, b->ops_off.data(), n * 8, hipMemcpyHostToDevice, st), "upload") &&
-       hip_ok(hipStreamSynchronize(st), "upload sync");
-  if (!ok) { delete b; return nullptr; }
+  ok = n == 0 ||
+       (hip_ok(hipMemcpyAsync(b->words.p, host_words.data(), words * 8, hipMemcpyHostToDevice, st), "upload") &&
+        hip_ok(hipMemcpyAsync(b->d_plen.p, b->plen.data(), n * 4, hipMemcpyHostToDevice, st), "upload") &&
+        hip_ok(hipMemcpyAsync(b->d_tlen.p, b->tlen.data(), n * 4, hipMemcpyHostToDevice, st), "upload") &&
+        hip_ok(hipMemcpyAsync(b->d_patw.p, b->pat_pos.data(), n * 8, hipMemcpyHostToDevice, st), "upload") &&
+        hip_ok(hipMemcpyAsync(b->d_txtw.p, b->txt_pos.data(), n * 8, hipMemcpyHostToDevice, st), "upload") &&
+        hip_ok(hipMemcpyAsync(b->d_ops_off.p, b->ops_off.data(), n * 8, hipMemcpyHostToDevice, st), "upload"));
+  ok = ok && hip_ok(hipStreamSynchronize(st), "upload sync");
+  if (!ok) {
+    delete b;
+    return nullptr;
+  }
   return b;
 }
 
 extern "C" void wfa_device_batch_free(wfa_device_batch_t* b) { delete b; }
-
-namespace {
-
-wfa::Counters* counters_of(Engine* e) { return e->counters.as<wfa::Counters>(); }
-
-PlanGroup make_large(const wfa_device_batch_t* b, std::vector<int> ids) {
-  PlanGroup g;
-  g.family = wfa::kLarge;
-  int pmax = 0, tmax = 0;
-  for (int i : ids) { pmax = std::max(pmax, b->plen[i]); tmax = std::max(tmax, b->tlen[i]); }
-  g.wcap = pmax + tmax + 8;
-  g.lmax_words = 0;
-  g.block = wfa::pick_block(wfa::kLarge, pmax + tmax);
-  g.ids = std::move(ids);
-  return g;
-}
-
-// Splits the pairs over kernel families: short pairs on the LDS family, the
-// rest on the HBM-ring family by workgroup size.
-std::vector<PlanGroup> make_plan(const wfa_device_batch_t* b, const std::vector<int>& ids,
-                                 const Penalties& pen, int ncomp) {
-  const int wcap_small = small_wcap(pen, ncomp);
-  const int spw = 64 / b->bits;
-  std::vector<int> small_ids, cls[3];
-  int lmax_small = 0;
-  for (int i : ids) {
-    const int L = std::max(b->plen[i], b->tlen[i]);
-    if (wcap_small >= 64 && L <= 1024) {
-      small_ids.push_back(i);
-      lmax_small = std::max<int>(lmax_small, L / spw + 2);
-    } else {
-      const int w = b->plen[i] + b->tlen[i];
-      cls[w <= 256 ? 0 : (w <= 4096 ? 1 : 2)].push_back(i);
-    }
-  }
-  std::vector<PlanGroup> plan;
-  if (!small_ids.empty()) {
-    PlanGroup g;
-    g.family = wfa::kSmall;
-    g.block = 64;
-    g.wcap = wcap_small;
-    g.lmax_words = lmax_small;
-    g.ids = std::move(small_ids);
-    plan.push_back(std::move(g));
-  }
-  for (auto& c : cls)
-    if (!c.empty()) plan.push_back(make_large(b, std::move(c)));
-  return plan;
-}
-
-LaunchShape shape_of(Engine* e, const PlanGroup& g, int two, int bits, const Penalties& pen) {
-  LaunchShape shape;
-  shape.family = g.family;
-  shape.block = g.block;
-  shape.lds_bytes = wfa::forward_lds_bytes(g.family, two ? 5 : 3, pen.nslots, g.wcap, g.lmax_words);
-  int per_cu = wfa::forward_max_blocks_per_cu(g.family, g.block, two, bits, shape.lds_bytes);
-  if (per_cu <= 0) per_cu = 1;
-  shape.grid = (int)std::min<int64_t>((int64_t)g.ids.size(), (int64_t)per_cu * e->num_cus);
-  return shape;
-}
-
-// Enqueues forward + backtrace of one group on the aligner's stream.
-int enqueue_group(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, PlanGroup& grp,
-                  int two, const Penalties& pen, int records) {
-  const int n = (int)grp.ids.size();
-  if (n == 0) return WF_STATUS_OK;
-  hipStream_t st = e->stream;
-  const LaunchShape shape = shape_of(e, grp, two, b->bits, pen);
-  if (!grp.d_ids.p) {
-    if (!grp.d_ids.ensure((size_t)n * 4)) return WF_STATUS_OOM;
-    if (!hip_ok(hipMemcpyAsync(grp.d_ids.p, grp.ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, st), "plan upload"))
-      return WF_STATUS_OOM;
-  }
-  for (auto& ev : grp.ev)
-    if (!ev && !hip_ok(hipEventCreate(&ev), "hipEventCreate")) return WF_STATUS_OOM;
-  wfa::Counters* c = counters_of(e);
-  ForwardArgs fa{};
-  fa.n_pairs = n;
-  fa.order = grp.d_ids.as<int>();
-  fa.plen = b->d_plen.as<int>();
-  fa.tlen = b->d_tlen.as<int>();
-  fa.words = b->words.as<uint64_t>();
-  fa.pat_word = b->d_patw.as<int64_t>();
-  fa.txt_word = b->d_txtw.as<int64_t>();
-  fa.out_status = b->d_status.as<int>();
-  fa.out_score = b->d_score.as<int>();
-  fa.out_dsteps = b->d_dsteps.as<long long>();
-  fa.out_last_block = b->d_last.as<uint32_t>();
-  fa.rec_arena = e->rec.as<uint4>();
-  fa.rec_units = records ? std::min<unsigned long long>(e->rec.cap / 16, 0xFFFFFFF0ull) : 0;
-  fa.rec_top = &c->rec_top;
-  fa.ring_global = grp.family == wfa::kLarge ? e->ring.as<int>() : nullptr;
-  fa.ring_stride = grp.family == wfa::kLarge ? (long long)pen.nslots * (two ? 5 : 3) * grp.wcap : 0;
-  fa.queue = &c->queue;
-  fa.dsteps_total = &c->dsteps;
-  fa.retry_count = &c->retry;
-  fa.fail_count = &c->fail;
-  fa.wcap = grp.wcap;
-  fa.lmax_words = grp.lmax_words;
-  fa.max_steps = wf->system.max_alignment_steps > 0 ? wf->system.max_alignment_steps : INT_MAX;
-  fa.records = records;
-  fa.comp_begin = wfa::kM;
-  fa.comp_end = wfa::kM;
-  fa.pen = pen;
-  if (!hip_ok(wfa::launch_reset_counters(c, 0, st), "reset")) return WF_STATUS_OOM;
-  (void)hipEventRecord(grp.ev[0], st);
-  if (!hip_ok(wfa::launch_forward(fa, two, b->bits, shape, st), "forward launch")) return WF_STATUS_OOM;
-  (void)hipEventRecord(grp.ev[1], st);
-  if (records) {
-    BacktraceArgs ba{};
-    ba.n_pairs = n;
-    ba.order = grp.d_ids.as<int>();
-    ba.plen = b->d_plen.as<int>();
-    ba.tlen = b->d_tlen.as<int>();
-    ba.status = b->d_status.as<int>();
-    ba.score = b->d_score.as<int>();
-    ba.last_block = b->d_last.as<uint32_t>();
-    ba.rec_arena = e->rec.as<uint4>();
-    ba.rec_units = fa.rec_units;
-    ba.ops = b->d_ops.as<char>();
-    ba.ops_off = b->d_ops_off.as<int64_t>();
-    ba.ops_begin = b->d_ops_begin.as<int>();
-    ba.fail_count = &c->fail;
-    ba.comp_begin = wfa::kM;
-    ba.comp_end = wfa::kM;
-    ba.pen = pen;
-    if (!hip_ok(wfa::launch_backtrace(ba, st), "backtrace launch")) return WF_STATUS_OOM;
-  }
-  (void)hipEventRecord(grp.ev[2], st);
-  return WF_STATUS_OK;
-}
-
-// Device memory every group of the plan needs, allocated before any launch.
-bool reserve_for(Engine* e, std::vector<PlanGroup>& plan, int two, int bits, const Penalties& pen, int records) {
-  if (records && !e->rec.ensure(e->rec_bytes)) {
-    fprintf(stderr, "[WFA] cannot allocate the %zu MB record arena\n", e->rec_bytes >> 20);
-    return false;
-  }
-  size_t ring = 0;
-  for (const PlanGroup& g : plan) {
-    if (g.family != wfa::kLarge) continue;
-    const LaunchShape sh = shape_of(e, g, two, bits, pen);
-    ring = std::max(ring, (size_t)pen.nslots * (two ? 5 : 3) * g.wcap * sh.grid * sizeof(int));
-  }
-  if (ring && !e->ring.ensure(ring)) {
-    fprintf(stderr, "[WFA] cannot allocate %zu MB of wavefront rings\n", ring >> 20);
-    return false;
-  }
-  return true;
-}
-
-bool download_results(Engine* e, wfa_device_batch_t* b) {
-  if (b->host_results) return true;
-  hipStream_t st = e->stream;
-  const size_t n = (size_t)b->n;
-  b->status.resize(n);
-  b->score.resize(n);
-  b->ops_begin.resize(n);
-  bool ok = n == 0 ||
-            (hip_ok(hipMemcpyAsync(b->status.data(), b->d_status.p, n * 4, hipMemcpyDeviceToHost, st), "download") &&
-             hip_ok(hipMemcpyAsync(b->score.data(), b->d_score.p, n * 4, hipMemcpyDeviceToHost, st), "download") &&
-             hip_ok(hipMemcpyAsync(b->ops_begin.data(), b->d_ops_begin.p, n * 4, hipMemcpyDeviceToHost, st), "download"));
-  ok = ok && hip_ok(hipStreamSynchronize(st), "download sync");
-  b->host_results = ok;
-  return ok;
-}
-
-}  // namespace
 
 extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_batch_t* b) {
   if (!b) return WF_STATUS_OOM;
@@ -618,99 +701,40 @@ extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_ba
   b->have_results = false;
   b->host_results = false;
   b->first_fail = WF_STATUS_ALG_COMPLETED;
-  if (b->n == 0) { b->have_results = true; b->host_results = true; return WF_STATUS_ALG_COMPLETED; }
+  if (b->n == 0) {
+    b->have_results = true;
+    b->host_results = true;
+    return WF_STATUS_ALG_COMPLETED;
+  }
   if (records && !b->d_ops.ensure(std::max<int64_t>(b->total_ops, 1))) {
     fprintf(stderr, "[WFA] cannot allocate %lld bytes of CIGAR output\n", (long long)b->total_ops);
     return WF_STATUS_OOM;
   }
   b->ops_on_device = records != 0;
-  const std::vector<int> key = {pen.x, pen.o1, pen.e1, pen.o2, pen.e2, two, b->bits};
-  if (b->plan_key != key) {
-    for (auto& g : b->plan) g.release();
-    b->plan = make_plan(b, b->sorted, pen, ncomp);
-    b->plan_key = key;
+  RunTotals tot;
+  int rc;
+  b->bi_mode = wf->memory_mode == wavefront_memory_ultralow;
+  if (b->bi_mode) {
+    rc = bialign_run(wf, e, b, &tot);
+    if (rc == WF_STATUS_OOM) return rc;
+  } else {
+    const std::vector<int> key = {pen.x, pen.o1, pen.e1, pen.o2, pen.e2, two, b->bits};
+    if (b->plan_key != key) {
+      for (auto& g : b->plan) g.release();
+      b->plan = make_plan(b->jobs(), b->sorted, pen, ncomp);
+      b->plan_key = key;
+    }
+    std::vector<int> scratch;
+    rc = run_jobs(wf, e, b->jobs(), b->plan, records, &tot, scratch);
+    if (rc != WF_STATUS_OK) return rc;
   }
-  hipStream_t st = e->stream;
-  wfa::Counters host_c{};
-  std::vector<PlanGroup> retry_groups;
-  std::vector<PlanGroup>* pending = &b->plan;
-  float fwd_ms = 0, bt_ms = 0;
-  int launches = 0;
-  if (!hip_ok(wfa::launch_reset_counters(counters_of(e), 2, st), "reset")) return WF_STATUS_OOM;
-  for (int round = 0;; ++round) {
-    if (round > 48) {
-      fprintf(stderr, "[WFA] batch did not converge (record arena too small?)\n");
-      return WF_STATUS_OOM;
-    }
-    if (!reserve_for(e, *pending, two, b->bits, pen, records)) return WF_STATUS_OOM;
-    if (!hip_ok(wfa::launch_reset_counters(counters_of(e), 1, st), "reset")) return WF_STATUS_OOM;
-    for (PlanGroup& g : *pending) {
-      const int rc = enqueue_group(wf, e, b, g, two, pen, records);
-      if (rc != WF_STATUS_OK) return rc;
-      ++launches;
-    }
-    if (!hip_ok(hipMemcpyAsync(&host_c, e->counters.p, sizeof(host_c), hipMemcpyDeviceToHost, st), "counters") ||
-        !hip_ok(hipStreamSynchronize(st), "alignment"))
-      return WF_STATUS_OOM;
-    for (PlanGroup& g : *pending) {
-      float f = 0, t = 0;
-      (void)hipEventElapsedTime(&f, g.ev[0], g.ev[1]);
-      (void)hipEventElapsedTime(&t, g.ev[1], g.ev[2]);
-      fwd_ms += f;
-      bt_ms += t;
-    }
-    if (host_c.retry == 0) break;
-    // Some pairs overflowed the LDS ring or the record arena: re-run only them.
-    b->host_results = false;
-    if (!download_results(e, b)) return WF_STATUS_OOM;
-    std::vector<PlanGroup> next;
-    for (PlanGroup& g : *pending) {
-      std::vector<int> ring_over, arena_full;
-      for (int i : g.ids) {
-        if (b->status[i] == wfa::kDevRingOverflow) ring_over.push_back(i);
-        else if (b->status[i] == wfa::kDevArenaFull) arena_full.push_back(i);
-      }
-      if (!ring_over.empty()) next.push_back(make_large(b, std::move(ring_over)));
-      if (!arena_full.empty()) {
-        if (arena_full.size() == g.ids.size()) {
-          // not a single pair of the group fitted: grow the arena
-          size_t free_b = 0, total_b = 0;
-          (void)hipMemGetInfo(&free_b, &total_b);
-          if (e->rec_bytes * 2 > e->rec.cap + free_b / 2) {
-            fprintf(stderr, "[WFA] record arena exhausted (%zu MB)\n", e->rec_bytes >> 20);
-            for (int i : arena_full) b->status[i] = WF_STATUS_OOM;
-            if (!hip_ok(hipMemcpy(b->d_status.p, b->status.data(), (size_t)b->n * 4, hipMemcpyHostToDevice), "status"))
-              return WF_STATUS_OOM;
-            arena_full.clear();
-          } else {
-            e->rec.release();
-            e->rec_bytes *= 2;
-          }
-        }
-        if (!arena_full.empty()) {
-          PlanGroup h;
-          h.family = g.family;
-          h.block = g.block;
-          h.wcap = g.wcap;
-          h.lmax_words = g.lmax_words;
-          h.ids = std::move(arena_full);
-          next.push_back(std::move(h));
-        }
-      }
-    }
-    for (auto& g : retry_groups) g.release();
-    retry_groups = std::move(next);
-    pending = &retry_groups;
-    if (retry_groups.empty()) break;
-  }
-  for (auto& g : retry_groups) g.release();
-  b->stats.diagonal_steps = (int64_t)host_c.dsteps;
-  b->stats.forward_ms = fwd_ms;
-  b->stats.backtrace_ms = bt_ms;
-  b->stats.forward_launches = launches;
+  b->stats.diagonal_steps = (int64_t)tot.dsteps;
+  b->stats.forward_ms = tot.forward_ms;
+  b->stats.backtrace_ms = tot.backtrace_ms;
+  b->stats.forward_launches = tot.launches;
   b->have_results = true;
   b->host_results = false;
-  if (host_c.fail == 0) return WF_STATUS_ALG_COMPLETED;
+  if (tot.fail == 0) return WF_STATUS_ALG_COMPLETED;
   if (!download_results(e, b)) return WF_STATUS_OOM;
   int first_fail = WF_STATUS_ALG_COMPLETED;
   for (int i = 0; i < b->n; ++i) {
@@ -718,8 +742,7 @@ extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_ba
       fprintf(stderr, "[WFA] internal error: inconsistent backtrace for pair %d\n", i);
       b->status[i] = WF_STATUS_UNATTAINABLE;
     }
-    if (b->status[i] != WF_STATUS_ALG_COMPLETED && first_fail == WF_STATUS_ALG_COMPLETED)
-      first_fail = b->status[i];
+    if (b->status[i] != WF_STATUS_ALG_COMPLETED && first_fail == WF_STATUS_ALG_COMPLETED) first_fail = b->status[i];
   }
   b->first_fail = first_fail;
   return first_fail;
@@ -760,12 +783,12 @@ extern "C" int wfa_device_batch_download(wavefront_aligner_t* const wf, wfa_devi
     r.cigar_offset = b->ops_off[i];
     r.cigar_length = 0;
     if (st == WF_STATUS_ALG_COMPLETED) {
-      r.score = classic_score(wf->penalties, b->plen[i], b->tlen[i], b->score[i]);
+      r.score = b->bi_mode ? b->bi_score[i] : classic_score(wf->penalties, b->plen[i], b->tlen[i], b->score[i]);
       if (b->records) {
         r.cigar_offset = b->ops_off[i] + b->ops_begin[i];
         r.cigar_length = b->plen[i] + b->tlen[i] - b->ops_begin[i];
       }
-    } else if (st == WF_STATUS_MAX_STEPS_REACHED) {
+    } else if (st == WF_STATUS_MAX_STEPS_REACHED && !b->bi_mode) {
       r.score = -wf->system.max_alignment_steps;  // wavefront_unialign.c:102-107
     } else {
       r.score = INT32_MIN;

@@ -1,0 +1,571 @@
+// BiWFA (memory mode ultralow) breakpoint search on the device.
+//
+// One workgroup per sub-problem runs wavefront_bialign_find_breakpoint
+// (wavefront_bialign.c:975-1083): a forward and a reverse score-only
+// wavefront (rings of the last max_score_scope scores, in HBM) advanced in the
+// reference's alternation order, phase 1 until the furthest antidiagonals of
+// the two sides meet (wavefront_extend_end2end_max, wavefront_extend.c:173-211),
+// then phase 2 with the overlap scans of wavefront_bialign_overlap
+// (:878-956): for each of the last scores of the other side, D2, I2, D1, I1
+// (gap-opening adjusted) and then M, each taking the first diagonal k0 in
+// ascending order (breakpoint_indel2indel :477-573, breakpoint_m2m :798-874).
+// Each scan is a block-parallel min-reduction over k0.  The host recurses on
+// the two halves, level by level across all pairs (wfa_bialign.cpp).
+#include "wfa_common.h"
+
+namespace wfa {
+namespace {
+
+template <int NC>
+struct Side {
+  MetaRegs<NC> meta;
+  int cur;                 // ring slot of the latest multiple-of-g score
+  int* ring;
+  const uint64_t* P;
+  int pb;
+  const uint64_t* T;
+  int tb;
+  int cb, ce;              // begin / end component of this direction
+  int end_score;           // score at which this side reached its end, -1 if not
+};
+
+template <int BLOCK>
+struct BlockRed {
+  static constexpr int NW = BLOCK / 64;
+  int* sh;                 // NW ints of LDS
+  // Block-wide min of per-thread values (uniform result).
+  __device__ __forceinline__ int min_all(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    if constexpr (NW == 1) return v;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    int r = sh[0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) r = min(r, sh[w]);
+    return r;
+  }
+  __device__ __forceinline__ int max_all(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    if constexpr (NW == 1) return v;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    int r = sh[0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) r = max(r, sh[w]);
+    return r;
+  }
+};
+
+struct Bp {  // wf_bialign_breakpoint_t
+  int score, score_forward, score_reverse;
+  int k_forward, k_reverse, offset_forward, offset_reverse;
+  int component;
+};
+
+// The stored wavefront of component c at score s of a side, if present:
+// its trimmed range and the ring offset of diagonal 0.
+template <int NC>
+struct WfRef {
+  bool present;
+  int lo, hi, base;  // value at k is ring[base + k]
+};
+
+template <int NC>
+__device__ __forceinline__ WfRef<NC> wf_ref(const Side<NC>& sd, int s, int c, const Penalties& pen, int wcap) {
+  WfRef<NC> r{false, 1, 0, 0};
+  if (s < 0 || (s % pen.g) != 0) return r;
+  // the slot of score s, counted back from the latest multiple of g (slot cur)
+  const int tag_cur = lane_of(sd.meta.tag, sd.cur);
+  const int back = (tag_cur - s) / pen.g;
+  if (back < 0 || back >= pen.nslots) return r;
+  int slot = sd.cur - back;
+  if (slot < 0) slot += pen.nslots;
+  slot = uniform(slot);
+  if (lane_of(sd.meta.tag, slot) != s) return r;
+  const unsigned fl = (unsigned)lane_of((int)sd.meta.flags, slot);
+  if (!(fl & (1u << c))) return r;
+  r.present = true;
+  r.lo = lane_of(sd.meta.lo[c], slot);
+  r.hi = lane_of(sd.meta.hi[c], slot);
+  r.base = (slot * NC + c) * wcap - lane_of(sd.meta.alo, slot);
+  return r;
+}
+
+// Compute score s of one side (no backtrace records) fused with its
+// extension; returns 1 when the end is reached (wavefront_termination.c),
+// -1 on an internal error, 0 otherwise.  *max_ak gets the largest
+// antidiagonal 2*offset-k of the extended M wavefront (0 when absent).
+template <int NC, int BLOCK, int BITS>
+__device__ int side_step(Side<NC>& sd, int s, int plen, int tlen, const Penalties& pen, int wcap,
+                         bool want_max, int* max_ak, long long* dsteps, int (*sh_red)[2 * NC + 2],
+                         BlockRed<BLOCK>& red) {
+  constexpr int NW = BLOCK / 64;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int R = pen.nslots, g = pen.g;
+  *max_ak = 0;
+  if (s % g) return 0;  // only multiples of g can be non-null
+  sd.cur = (sd.cur + 1 == R) ? 0 : sd.cur + 1;
+  const int cur = sd.cur;
+  const int dq_x = pen.x / g, dq_o1 = (pen.o1 + pen.e1) / g, dq_e1 = pen.e1 / g;
+  const Src mx  = get_src(sd.meta, s - pen.x, cur - dq_x, R, kM, wcap);
+  const Src mo1 = get_src(sd.meta, s - pen.o1 - pen.e1, cur - dq_o1, R, kM, wcap);
+  const Src i1  = get_src(sd.meta, s - pen.e1, cur - dq_e1, R, kI1, wcap);
+  const Src d1  = get_src(sd.meta, s - pen.e1, cur - dq_e1, R, kD1, wcap);
+  Src mo2{1, -1, kFarDiagonal, 0, 0}, i2 = mo2, d2 = mo2;
+  if constexpr (NC == 5) {
+    const int dq_o2 = (pen.o2 + pen.e2) / g, dq_e2 = pen.e2 / g;
+    mo2 = get_src(sd.meta, s - pen.o2 - pen.e2, cur - dq_o2, R, kM, wcap);
+    i2  = get_src(sd.meta, s - pen.e2, cur - dq_e2, R, kI2, wcap);
+    d2  = get_src(sd.meta, s - pen.e2, cur - dq_e2, R, kD2, wcap);
+  }
+  const bool any = mx.valid() || mo1.valid() || i1.valid() || d1.valid() || mo2.valid() || i2.valid() || d2.valid();
+  if (!any) {
+    if (lane == cur) { sd.meta.tag = s; sd.meta.flags = 0u; }
+    return 0;
+  }
+  int lo = mx.lim_lo, hi = mx.lim_hi;
+  lo = min(lo, mo1.lim_lo - 1); hi = max(hi, mo1.lim_hi + 1);
+  lo = min(lo, i1.lim_lo + 1);  hi = max(hi, i1.lim_hi + 1);
+  lo = min(lo, d1.lim_lo - 1);  hi = max(hi, d1.lim_hi - 1);
+  if constexpr (NC == 5) {
+    lo = min(lo, mo2.lim_lo - 1); hi = max(hi, mo2.lim_hi + 1);
+    lo = min(lo, i2.lim_lo + 1);  hi = max(hi, i2.lim_hi + 1);
+    lo = min(lo, d2.lim_lo - 1);  hi = max(hi, d2.lim_hi - 1);
+  }
+  lo = uniform(lo);
+  hi = uniform(hi);
+  if (hi - lo + 1 > wcap) return -1;
+  *dsteps += hi - lo + 1;
+  const bool has_i1 = mo1.valid() || i1.valid();
+  const bool has_d1 = mo1.valid() || d1.valid();
+  const bool has_i2 = (NC == 5) && (mo2.valid() || i2.valid());
+  const bool has_d2 = (NC == 5) && (mo2.valid() || d2.valid());
+  int* ring = sd.ring;
+  int* outM  = ring + (cur * NC + kM) * wcap - lo;
+  int* outI1 = ring + (cur * NC + kI1) * wcap - lo;
+  int* outD1 = ring + (cur * NC + kD1) * wcap - lo;
+  int* outI2 = ring + (cur * NC + (NC == 5 ? kI2 : 0)) * wcap - lo;
+  int* outD2 = ring + (cur * NC + (NC == 5 ? kD2 : 0)) * wcap - lo;
+  WaveRange rM, rI1, rD1, rI2, rD2;
+  rM.init(); rI1.init(); rD1.init(); rI2.init(); rD2.init();
+  int best_ak = 0;
+  for (int kb = lo; kb <= hi; kb += BLOCK) {
+    const int k = kb + tid;
+    bool vM = false, vI1 = false, vD1 = false, vI2 = false, vD2 = false;
+    if (k <= hi) {
+      const int ins1 = max(rd(ring, mo1, k - 1), rd(ring, i1, k - 1)) + 1;
+      const int del1 = max(rd(ring, mo1, k + 1), rd(ring, d1, k + 1));
+      int m = max(del1, max(rd(ring, mx, k) + 1, ins1));
+      int ins2 = kOffsetNull, del2 = kOffsetNull;
+      if constexpr (NC == 5) {
+        ins2 = max(rd(ring, mo2, k - 1), rd(ring, i2, k - 1)) + 1;
+        del2 = max(rd(ring, mo2, k + 1), rd(ring, d2, k + 1));
+        m = max(m, max(ins2, del2));
+      }
+      if (!in_bounds(m, k, plen, tlen)) m = kOffsetNull;
+      if (m != kOffsetNull) {
+        m = lcp_extend<BITS>(sd.P, sd.pb, sd.T, sd.tb, plen, tlen, k, m);
+        best_ak = max(best_ak, 2 * m - k);
+      }
+      outM[k] = m;
+      vM = (m != kOffsetNull);
+      if (has_i1) { outI1[k] = ins1; vI1 = in_bounds(ins1, k, plen, tlen); }
+      if (has_d1) { outD1[k] = del1; vD1 = in_bounds(del1, k, plen, tlen); }
+      if constexpr (NC == 5) {
+        if (has_i2) { outI2[k] = ins2; vI2 = in_bounds(ins2, k, plen, tlen); }
+        if (has_d2) { outD2[k] = del2; vD2 = in_bounds(del2, k, plen, tlen); }
+      }
+    }
+    const int kw = kb + wid * 64;
+    rM.add(__ballot(vM), kw);
+    rI1.add(__ballot(vI1), kw);
+    rD1.add(__ballot(vD1), kw);
+    if constexpr (NC == 5) {
+      rI2.add(__ballot(vI2), kw);
+      rD2.add(__ballot(vD2), kw);
+    }
+  }
+  if (want_max) *max_ak = red.max_all(best_ak);
+  if constexpr (NW > 1) {
+    if (lane == 0) {
+      sh_red[wid][0] = rM.first;  sh_red[wid][1] = rM.last;
+      sh_red[wid][2] = rI1.first; sh_red[wid][3] = rI1.last;
+      sh_red[wid][4] = rD1.first; sh_red[wid][5] = rD1.last;
+      if constexpr (NC == 5) {
+        sh_red[wid][6] = rI2.first; sh_red[wid][7] = rI2.last;
+        sh_red[wid][8] = rD2.first; sh_red[wid][9] = rD2.last;
+      }
+    }
+    __syncthreads();
+    rM.init(); rI1.init(); rD1.init(); rI2.init(); rD2.init();
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      rM.merge(sh_red[w][0], sh_red[w][1]);
+      rI1.merge(sh_red[w][2], sh_red[w][3]);
+      rD1.merge(sh_red[w][4], sh_red[w][5]);
+      if constexpr (NC == 5) {
+        rI2.merge(sh_red[w][6], sh_red[w][7]);
+        rD2.merge(sh_red[w][8], sh_red[w][9]);
+      }
+    }
+  }
+  __syncthreads();  // ring writes of this score visible to the whole workgroup
+  if (lane == cur) {
+    sd.meta.tag = s;
+    sd.meta.alo = lo;
+    unsigned fl = 1u | (rM.null() ? 0x100u : 0u);
+    sd.meta.lo[kM] = rM.tlo(lo);
+    sd.meta.hi[kM] = rM.thi(lo);
+    if (has_i1) { fl |= 2u | (rI1.null() ? 0x200u : 0u); sd.meta.lo[kI1] = rI1.tlo(lo); sd.meta.hi[kI1] = rI1.thi(lo); }
+    if (has_d1) { fl |= 4u | (rD1.null() ? 0x400u : 0u); sd.meta.lo[kD1] = rD1.tlo(lo); sd.meta.hi[kD1] = rD1.thi(lo); }
+    if constexpr (NC == 5) {
+      if (has_i2) { fl |= 8u | (rI2.null() ? 0x800u : 0u); sd.meta.lo[kI2] = rI2.tlo(lo); sd.meta.hi[kI2] = rI2.thi(lo); }
+      if (has_d2) { fl |= 16u | (rD2.null() ? 0x1000u : 0u); sd.meta.lo[kD2] = rD2.tlo(lo); sd.meta.hi[kD2] = rD2.thi(lo); }
+    }
+    sd.meta.flags = fl;
+  }
+  const int kend = tlen - plen;
+  const int ce = sd.ce;
+  bool pce = true;
+  WaveRange rce = rM;
+  if (ce == kI1) { pce = has_i1; rce = rI1; }
+  if (ce == kD1) { pce = has_d1; rce = rD1; }
+  if constexpr (NC == 5) {
+    if (ce == kI2) { pce = has_i2; rce = rI2; }
+    if (ce == kD2) { pce = has_d2; rce = rD2; }
+  }
+  if (pce && !rce.null() && kend >= rce.first && kend <= rce.last) {
+    if (ring[(cur * NC + ce) * wcap + (kend - lo)] >= tlen) {
+      sd.end_score = s;
+      return 1;
+    }
+  }
+  return 0;
+}
+
+// First k0 of the overlap of w0 (side 0) and w1 (side 1, diagonals inverted)
+// with h0 + h1 >= tlen; indel breakpoints also require the forward side's
+// cell inside the matrix.  Sets the breakpoint (uniform) when found.
+template <int NC, int BLOCK>
+__device__ void scan_breakpoint(const Side<NC>& s0d, const Side<NC>& s1d, const WfRef<NC>& w0, const WfRef<NC>& w1,
+                                bool forward, bool indel, int s0, int s1, int gap_open, int comp,
+                                int plen, int tlen, Bp& bp, BlockRed<BLOCK>& red) {
+  const int K = tlen - plen;
+  const int lo_0 = w0.lo, hi_0 = w0.hi;
+  const int lo_1 = K - w1.hi, hi_1 = K - w1.lo;
+  if (hi_1 < lo_0 || hi_0 < lo_1) return;
+  const int min_hi = min(hi_0, hi_1), max_lo = max(lo_0, lo_1);
+  const int sc = indel ? s0 + s1 - gap_open : s0 + s1;
+  if (sc >= bp.score) return;
+  const int* r0 = s0d.ring + w0.base;
+  const int* r1 = s1d.ring + w1.base;
+  int first = INT_MAX;
+  for (int kb = max_lo; kb <= min_hi; kb += BLOCK) {
+    const int k0 = kb + (int)threadIdx.x;
+    if (k0 <= min_hi && first == INT_MAX) {
+      const int k1 = K - k0;
+      const int h0 = r0[k0], h1 = r1[k1];
+      bool ok = h0 + h1 >= tlen;
+      if (ok && indel) {
+        const int fk = forward ? k0 : k1, fh = forward ? h0 : h1;
+        if (fh - fk > plen || fh > tlen) ok = false;
+      }
+      if (ok) first = k0;
+    }
+  }
+  first = red.min_all(first);
+  if (first == INT_MAX) return;
+  const int k0 = first, k1 = K - k0;
+  const int h0 = r0[k0], h1 = r1[k1];
+  if (forward) {
+    bp.score_forward = s0; bp.score_reverse = s1;
+    bp.k_forward = k0; bp.k_reverse = k1;
+    bp.offset_forward = h0; bp.offset_reverse = h1;
+  } else {
+    bp.score_forward = s1; bp.score_reverse = s0;
+    bp.k_forward = k1; bp.k_reverse = k0;
+    bp.offset_forward = h1; bp.offset_reverse = h0;
+  }
+  bp.score = sc;
+  bp.component = comp;
+}
+
+// wavefront_bialign_overlap (wavefront_bialign.c:878-956)
+template <int NC, int BLOCK>
+__device__ void overlap(const Side<NC>& a0, const Side<NC>& a1, int s0, int s1, bool forward, int plen, int tlen,
+                        const Penalties& pen, int wcap, Bp& bp, BlockRed<BLOCK>& red) {
+  const WfRef<NC> m0 = wf_ref(a0, s0, kM, pen, wcap);
+  if (!m0.present) return;
+  const WfRef<NC> d1w0 = wf_ref(a0, s0, kD1, pen, wcap);
+  const WfRef<NC> i1w0 = wf_ref(a0, s0, kI1, pen, wcap);
+  WfRef<NC> d2w0{false, 1, 0, 0}, i2w0{false, 1, 0, 0};
+  if constexpr (NC == 5) {
+    d2w0 = wf_ref(a0, s0, kD2, pen, wcap);
+    i2w0 = wf_ref(a0, s0, kI2, pen, wcap);
+  }
+  for (int i = 0; i < pen.scope; ++i) {
+    const int si = s1 - i;
+    if (si < 0) break;
+    if constexpr (NC == 5) {
+      if (s0 + si - pen.o2 >= bp.score) continue;
+      const WfRef<NC> d2w1 = wf_ref(a1, si, kD2, pen, wcap);
+      if (d2w0.present && d2w1.present)
+        scan_breakpoint<NC, BLOCK>(a0, a1, d2w0, d2w1, forward, true, s0, si, pen.o2, kD2, plen, tlen, bp, red);
+      const WfRef<NC> i2w1 = wf_ref(a1, si, kI2, pen, wcap);
+      if (i2w0.present && i2w1.present)
+        scan_breakpoint<NC, BLOCK>(a0, a1, i2w0, i2w1, forward, true, s0, si, pen.o2, kI2, plen, tlen, bp, red);
+    }
+    if (s0 + si - pen.o1 >= bp.score) continue;
+    const WfRef<NC> d1w1 = wf_ref(a1, si, kD1, pen, wcap);
+    if (d1w0.present && d1w1.present)
+      scan_breakpoint<NC, BLOCK>(a0, a1, d1w0, d1w1, forward, true, s0, si, pen.o1, kD1, plen, tlen, bp, red);
+    const WfRef<NC> i1w1 = wf_ref(a1, si, kI1, pen, wcap);
+    if (i1w0.present && i1w1.present)
+      scan_breakpoint<NC, BLOCK>(a0, a1, i1w0, i1w1, forward, true, s0, si, pen.o1, kI1, plen, tlen, bp, red);
+    if (s0 + si >= bp.score) continue;
+    const WfRef<NC> mw1 = wf_ref(a1, si, kM, pen, wcap);
+    if (mw1.present) scan_breakpoint<NC, BLOCK>(a0, a1, m0, mw1, forward, false, s0, si, 0, kM, plen, tlen, bp, red);
+  }
+}
+
+template <int NC>
+__device__ void side_init(Side<NC>& sd, int cb, int ce, int wcap) {
+  const int lane = threadIdx.x & 63;
+  sd.meta.tag = (lane == 0) ? 0 : -1;
+  sd.meta.alo = 0;
+  sd.meta.flags = (lane == 0) ? (1u << cb) : 0u;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) { sd.meta.lo[c] = 0; sd.meta.hi[c] = 0; }
+  sd.cur = 0;
+  sd.cb = cb;
+  sd.ce = ce;
+  sd.end_score = -1;
+  if (threadIdx.x == 0) sd.ring[(0 * NC + cb) * wcap] = 0;
+}
+
+// Extension of the score-0 wavefront (only M can extend).
+template <int NC, int BLOCK, int BITS>
+__device__ int side_extend0(Side<NC>& sd, int plen, int tlen, int wcap, int* max_ak) {
+  int v0 = 0;
+  *max_ak = 0;
+  if (sd.cb != kM) return 0;  // M absent at score 0: nothing to extend, no termination test
+  v0 = lcp_extend<BITS>(sd.P, sd.pb, sd.T, sd.tb, plen, tlen, 0, 0);
+  __syncthreads();
+  if (threadIdx.x == 0) sd.ring[(0 * NC + kM) * wcap] = v0;
+  __syncthreads();
+  *max_ak = 2 * v0;
+  if (sd.ce == kM && tlen - plen == 0 && v0 >= tlen) {
+    sd.end_score = 0;
+    return 1;
+  }
+  return 0;
+}
+
+template <int NC, int BLOCK, int BITS>
+__global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
+  constexpr int SPW = 64 / BITS;
+  __shared__ int sh_red[BLOCK / 64][2 * NC + 2];
+  __shared__ int sh_min[BLOCK / 64 > 0 ? BLOCK / 64 : 1];
+  __shared__ int sh_q;
+  BlockRed<BLOCK> red{sh_min};
+  const Penalties pen = a.pen;
+  const int wcap = a.wcap;
+  long long wg_dsteps = 0;
+  for (;;) {
+    if (threadIdx.x == 0) sh_q = (int)atomicAdd(a.queue, 1u);
+    __syncthreads();
+    const int q = sh_q;
+    __syncthreads();
+    if (q >= a.n) break;
+    const BiProblem pr = a.probs[q];
+    const int pair = pr.pair;
+    const int plen = pr.pe - pr.pb, tlen = pr.te - pr.tb;
+    const int plen_all = a.plen_total[pair], tlen_all = a.tlen_total[pair];
+    Side<NC> f, r;
+    f.ring = a.ring + (long long)blockIdx.x * a.ring_stride;
+    r.ring = f.ring + (long long)pen.nslots * NC * wcap;
+    {
+      const long long pp = a.fwd_ppos[pair] + pr.pb, tp = a.fwd_tpos[pair] + pr.tb;
+      f.P = a.words + pp / SPW; f.pb = (int)(pp % SPW);
+      f.T = a.words + tp / SPW; f.tb = (int)(tp % SPW);
+      const long long rp = a.rev_ppos[pair] + (plen_all - pr.pe), rt = a.rev_tpos[pair] + (tlen_all - pr.te);
+      r.P = a.rev_words + rp / SPW; r.pb = (int)(rp % SPW);
+      r.T = a.rev_words + rt / SPW; r.tb = (int)(rt % SPW);
+    }
+    __syncthreads();  // the previous sub-problem is done with the rings
+    side_init(f, pr.comp_begin, pr.comp_end, wcap);
+    side_init(r, pr.comp_end, pr.comp_begin, wcap);
+    __syncthreads();
+    Bp bp;
+    bp.score = INT_MAX;
+    bp.score_forward = bp.score_reverse = bp.k_forward = bp.k_reverse = 0;
+    bp.offset_forward = bp.offset_reverse = 0;
+    bp.component = kM;
+    int status = kBiOk;
+    long long dsteps = 0;
+    const int s_cap = (pen.x + pen.o1 + pen.e1 + (NC == 5 ? pen.o2 + pen.e2 : 0) + 2) * (plen + tlen + 2);
+    int sf = 0, sr = 0, fmax = 0, rmax = 0, ak = 0;
+    const int max_ad = plen + tlen - 1;
+    bool last_forward = false;
+    int rc;
+    // phase 1 (wavefront_bialign.c:1003-1044)
+    if (side_extend0<NC, BLOCK, BITS>(f, plen, tlen, wcap, &fmax)) { status = kBiEndReached; goto done; }
+    if (side_extend0<NC, BLOCK, BITS>(r, plen, tlen, wcap, &rmax)) { status = kBiEndReached; goto done; }
+    while (true) {
+      if (fmax + rmax >= max_ad) break;
+      ++sf;
+      rc = side_step<NC, BLOCK, BITS>(f, sf, plen, tlen, pen, wcap, true, &ak, &dsteps, sh_red, red);
+      if (rc < 0) { status = kDevRingOverflow; goto done; }
+      if (fmax < ak) fmax = ak;
+      last_forward = true;
+      if (rc > 0) { status = kBiEndReached; goto done; }
+      if (fmax + rmax >= max_ad) break;
+      ++sr;
+      rc = side_step<NC, BLOCK, BITS>(r, sr, plen, tlen, pen, wcap, true, &ak, &dsteps, sh_red, red);
+      if (rc < 0) { status = kDevRingOverflow; goto done; }
+      if (rmax < ak) rmax = ak;
+      last_forward = false;
+      if (rc > 0) { status = kBiEndReached; goto done; }
+      if (sr + sf >= a.max_steps) { status = kDevMaxSteps; goto done; }
+      if (sf > s_cap) { status = kDevBadTrace; goto done; }
+    }
+    // phase 2 (:1046-1080)
+    {
+      const int scope = pen.scope;
+      const int gap_open = (NC == 5) ? max(pen.o1, pen.o2) : pen.o1;
+      while (true) {
+        if (last_forward) {
+          const int min_sr = (sr > scope - 1) ? sr - (scope - 1) : 0;
+          if (sf + min_sr - gap_open >= bp.score) break;
+          overlap<NC, BLOCK>(f, r, sf, sr, true, plen, tlen, pen, wcap, bp, red);
+          ++sr;
+          rc = side_step<NC, BLOCK, BITS>(r, sr, plen, tlen, pen, wcap, false, &ak, &dsteps, sh_red, red);
+          if (rc < 0) { status = kDevRingOverflow; goto done; }
+          if (rc > 0) { status = kBiEndReached; goto done; }
+        }
+        const int min_sf = (sf > scope - 1) ? sf - (scope - 1) : 0;
+        if (min_sf + sr - gap_open >= bp.score) break;
+        overlap<NC, BLOCK>(r, f, sr, sf, false, plen, tlen, pen, wcap, bp, red);
+        ++sf;
+        rc = side_step<NC, BLOCK, BITS>(f, sf, plen, tlen, pen, wcap, false, &ak, &dsteps, sh_red, red);
+        if (rc < 0) { status = kDevRingOverflow; goto done; }
+        if (rc > 0) { status = kBiEndReached; goto done; }
+        if (sr + sf >= a.max_steps) { status = kDevMaxSteps; goto done; }
+        if (sf > s_cap) { status = kDevBadTrace; goto done; }
+        last_forward = true;
+      }
+    }
+  done:
+    if (threadIdx.x == 0) {
+      BiOutcome o;
+      o.status = status;
+      o.score = bp.score;
+      o.score_forward = bp.score_forward;
+      o.score_reverse = bp.score_reverse;
+      o.k_forward = bp.k_forward;
+      o.k_reverse = bp.k_reverse;
+      o.offset_forward = bp.offset_forward;
+      o.offset_reverse = bp.offset_reverse;
+      o.component = bp.component;
+      o.fwd_end_score = f.end_score;
+      o.rev_end_score = r.end_score;
+      o.pad = 0;
+      o.dsteps = dsteps;
+      a.out[q] = o;
+    }
+    wg_dsteps += dsteps;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && wg_dsteps) atomicAdd(a.dsteps_total, (unsigned long long)wg_dsteps);
+}
+
+template <int NC, int BLOCK, int BITS>
+const void* bp_fn() {
+  return reinterpret_cast<const void*>(&wfa_breakpoint_kernel<NC, BLOCK, BITS>);
+}
+
+const void* select_breakpoint(int two_piece, int bits, int block) {
+#define WFA_BSEL(NC, BL, BI) \
+  if ((two_piece ? 5 : 3) == NC && block == BL && bits == BI) return bp_fn<NC, BL, BI>();
+  WFA_BSEL(3, 64, 2) WFA_BSEL(3, 64, 8) WFA_BSEL(5, 64, 2) WFA_BSEL(5, 64, 8)
+  WFA_BSEL(3, 256, 2) WFA_BSEL(3, 256, 8) WFA_BSEL(5, 256, 2) WFA_BSEL(5, 256, 8)
+  WFA_BSEL(3, 1024, 2) WFA_BSEL(3, 1024, 8) WFA_BSEL(5, 1024, 2) WFA_BSEL(5, 1024, 8)
+#undef WFA_BSEL
+  return nullptr;
+}
+
+// Reversed copies of packed sequences: sequence j (patterns, then texts) of
+// length len[j] at symbol src_pos[j] of `words` is written reversed at symbol
+// dst_pos[j] (word aligned) of `out`, zero padded like the forward packing.
+template <int BITS>
+__global__ void __launch_bounds__(256) reverse_pack_kernel(const uint64_t* words, const int64_t* src_pos,
+                                                            const int64_t* dst_pos, const int* len, int n,
+                                                            uint64_t* out) {
+  constexpr int SPW = 64 / BITS;
+  constexpr uint64_t MASK = (BITS == 64) ? ~0ull : ((1ull << BITS) - 1);
+  const int j = blockIdx.x;
+  if (j >= n) return;
+  const int L = len[j];
+  const long long sp = src_pos[j];
+  uint64_t* dst = out + dst_pos[j] / SPW;
+  const int nw = L / SPW + 2;
+  for (int w = threadIdx.x; w < nw; w += blockDim.x) {
+    uint64_t v = 0;
+    for (int b = 0; b < SPW; ++b) {
+      const int i = w * SPW + b;
+      if (i >= L) break;
+      const long long src = sp + (L - 1 - i);
+      const uint64_t sym = (words[src / SPW] >> ((src % SPW) * BITS)) & MASK;
+      v |= sym << (b * BITS);
+    }
+    dst[w] = v;
+  }
+}
+
+// Copies CIGAR pieces (or writes runs of one operation) to their final place.
+__global__ void __launch_bounds__(256) gather_ops_kernel(const OpsPiece* pieces, int n, const char* src, char* dst) {
+  const int j = blockIdx.x;
+  if (j >= n) return;
+  const OpsPiece pc = pieces[j];
+  for (int i = threadIdx.x; i < pc.len; i += blockDim.x)
+    dst[pc.dst + i] = pc.src >= 0 ? src[pc.src + i] : pc.op;
+}
+
+}  // namespace
+
+hipError_t launch_reverse_pack(const uint64_t* words, const int64_t* src_pos, const int64_t* dst_pos, const int* len,
+                               int n, int bits, uint64_t* out, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  if (bits == 2)
+    hipLaunchKernelGGL(reverse_pack_kernel<2>, dim3(n), dim3(256), 0, stream, words, src_pos, dst_pos, len, n, out);
+  else
+    hipLaunchKernelGGL(reverse_pack_kernel<8>, dim3(n), dim3(256), 0, stream, words, src_pos, dst_pos, len, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_ops(const OpsPiece* pieces, int n, const char* src, char* dst, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_ops_kernel, dim3(n), dim3(256), 0, stream, pieces, n, src, dst);
+  return hipGetLastError();
+}
+
+int breakpoint_max_blocks_per_cu(int two_piece, int bits, int block) {
+  const void* f = select_breakpoint(two_piece, bits, block);
+  if (!f) return 0;
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, 0) != hipSuccess) return 0;
+  return n;
+}
+
+hipError_t launch_breakpoint(const BiArgs& a, int two_piece, int bits, int block, int grid, hipStream_t stream) {
+  const void* f = select_breakpoint(two_piece, bits, block);
+  if (!f) return hipErrorInvalidValue;
+  void* args[] = {const_cast<BiArgs*>(&a)};
+  return hipLaunchKernel(f, dim3(grid), dim3(block), args, 0, stream);
+}
+
+}  // namespace wfa

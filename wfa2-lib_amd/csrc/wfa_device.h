@@ -50,8 +50,10 @@ struct ForwardArgs {
   const int* plen;
   const int* tlen;
   const uint64_t* words;       // packed sequences
-  const int64_t* pat_word;     // first word of each pattern
-  const int64_t* txt_word;     // first word of each text
+  const int64_t* pat_pos;      // first symbol of each pattern (index into the packed words)
+  const int64_t* txt_pos;      // first symbol of each text
+  const signed char* comp_begin_p;  // per-pair begin / end components (null: comp_begin / comp_end)
+  const signed char* comp_end_p;
   int* out_status;
   int* out_score;              // wavefront score at the end (s_end)
   long long* out_dsteps;       // diagonal-steps of the pair (sum of hi-lo+1)
@@ -87,6 +89,7 @@ struct BacktraceArgs {
   const int64_t* ops_off;      // slot of pair i: ops[ops_off[i] .. +plen+tlen)
   int* ops_begin;              // CIGAR = ops[ops_off + ops_begin .. ops_off + plen + tlen)
   unsigned int* fail_count;    // incremented for a pair whose trace is inconsistent
+  const signed char* comp_end_p;  // per-pair end component (null: comp_end)
   int comp_begin, comp_end;
   Penalties pen;
 };
@@ -120,6 +123,61 @@ static_assert(sizeof(Counters) == 64, "Counters layout");
 // Zeroes the queue and the arena pointer; with all=1 also retry; with all=2
 // also the run totals (diagonal-steps, failures).
 hipError_t launch_reset_counters(Counters* c, int all, hipStream_t stream);
+
+// ------------------------------------------------------------------ BiWFA
+// One bidirectional sub-problem: align pattern[pb,pe) with text[tb,te) of a
+// top-level pair, starting in comp_begin and ending in comp_end.
+struct BiProblem {
+  int pair;
+  int pb, pe, tb, te;
+  int comp_begin, comp_end;
+  int pad;
+};
+enum BiStatus : int { kBiOk = 0, kBiEndReached = -2 };
+struct BiOutcome {             // wf_bialign_breakpoint_t plus the search status
+  int status;                  // kBiOk, kBiEndReached, kDevMaxSteps, kDevRingOverflow, kDevBadTrace
+  int score, score_forward, score_reverse;
+  int k_forward, k_reverse, offset_forward, offset_reverse;
+  int component;
+  int fwd_end_score;           // score at which the forward side reached its end, -1 if not
+  int rev_end_score;
+  int pad;
+  long long dsteps;
+};
+struct BiArgs {
+  int n;
+  const BiProblem* probs;
+  const uint64_t* words;       // forward sequences
+  const uint64_t* rev_words;   // reversed sequences
+  const int64_t* fwd_ppos;     // per top-level pair: first symbol of pattern / text ...
+  const int64_t* fwd_tpos;
+  const int64_t* rev_ppos;     // ... and of their reversed copies
+  const int64_t* rev_tpos;
+  const int* plen_total;
+  const int* tlen_total;
+  BiOutcome* out;
+  int* ring;                   // per workgroup: forward ring, then reverse ring
+  long long ring_stride;       // ints per workgroup
+  int wcap;
+  unsigned int* queue;
+  unsigned long long* dsteps_total;
+  int max_steps;
+  Penalties pen;
+};
+hipError_t launch_breakpoint(const BiArgs& a, int two_piece, int bits, int block, int grid, hipStream_t stream);
+// A piece of a final CIGAR: len operations copied from src (or, when src < 0,
+// len copies of op) to dst.
+struct OpsPiece {
+  long long dst;
+  long long src;
+  int len;
+  char op;
+  char pad[3];
+};
+hipError_t launch_gather_ops(const OpsPiece* pieces, int n, const char* src, char* dst, hipStream_t stream);
+hipError_t launch_reverse_pack(const uint64_t* words, const int64_t* src_pos, const int64_t* dst_pos, const int* len,
+                               int n, int bits, uint64_t* out, hipStream_t stream);
+int breakpoint_max_blocks_per_cu(int two_piece, int bits, int block);
 
 // Dynamic-LDS bytes and occupancy for a configuration.
 size_t forward_lds_bytes(int family, int ncomp, int nslots, int wcap, int lmax_words);

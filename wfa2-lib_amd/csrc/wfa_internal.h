@@ -1,0 +1,139 @@
+// Host-side internals shared by the C-ABI layer (wfa_api.cpp) and the BiWFA
+// orchestration (wfa_bialign.cpp).  Not installed.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "wfa_device.h"
+#include "wfa_mi355x.h"
+
+namespace wfa_host {
+
+bool hip_ok(hipError_t e, const char* what);
+
+// Grow-only device allocation.
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  bool ensure(size_t bytes);
+  void release();
+  template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+// Per-aligner device state: one stream, the backtrace-record arena, the HBM
+// wavefront rings, the run counters.
+struct Engine {
+  int device = 0;
+  int num_cus = 0;
+  hipStream_t stream = nullptr;
+  DevBuf rec;
+  DevBuf ring;
+  DevBuf counters;
+  size_t rec_bytes = 0;
+  bool ok = false;
+};
+
+Engine* engine_of(wavefront_aligner_t* wf);
+bool engine_init(Engine* e);
+wfa::Counters* counters_of(Engine* e);
+
+wfa::Penalties make_penalties(const wavefront_penalties_t& p, bool two);
+int classic_score(const wavefront_penalties_t& p, int plen, int tlen, int s);
+
+// A set of alignment jobs on the device: pairs of a batch, or the base-case
+// sub-problems of BiWFA.  Job i aligns plen[i] pattern symbols starting at
+// symbol ppos[i] of `words` with tlen[i] text symbols starting at tpos[i],
+// from component cbeg[i] to cend[i] (M when the arrays are null); results go
+// to status/score/last/dsteps[i] and, with records, the CIGAR into
+// ops[ops_off[i] + ops_begin[i] .. ops_off[i] + plen[i] + tlen[i]).
+struct JobSet {
+  int n = 0;
+  int bits = 2;
+  const int* h_plen = nullptr;
+  const int* h_tlen = nullptr;
+  const uint64_t* words = nullptr;
+  const int* plen = nullptr;
+  const int* tlen = nullptr;
+  const int64_t* ppos = nullptr;
+  const int64_t* tpos = nullptr;
+  const signed char* cbeg = nullptr;
+  const signed char* cend = nullptr;
+  int* status = nullptr;
+  int* score = nullptr;
+  uint32_t* last = nullptr;
+  long long* dsteps = nullptr;
+  char* ops = nullptr;
+  const int64_t* ops_off = nullptr;
+  int* ops_begin = nullptr;
+};
+
+// One forward launch (+ backtrace) over a subset of jobs with one kernel family.
+struct PlanGroup {
+  int family = 0;
+  int block = 64;
+  int wcap = 0;
+  int lmax_words = 0;
+  std::vector<int> ids;
+  DevBuf d_ids;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  void release();
+};
+
+struct RunTotals {
+  double forward_ms = 0;
+  double backtrace_ms = 0;
+  int launches = 0;
+  unsigned long long dsteps = 0;
+  unsigned int fail = 0;
+};
+
+std::vector<PlanGroup> make_plan(const JobSet& js, const std::vector<int>& ids, const wfa::Penalties& pen, int ncomp);
+
+// Aligns every job of the plan (retrying the ones that overflow an LDS ring
+// or the record arena); results stay on the device.  h_status is scratch
+// for the host copy of the statuses.  Returns WF_STATUS_OK or WF_STATUS_OOM.
+int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<PlanGroup>& plan,
+             int records, RunTotals* tot, std::vector<int>& h_status);
+
+}  // namespace wfa_host
+
+// A batch of pairs resident in HBM (the wfa_device_batch_t of the C ABI).
+struct wfa_device_batch_t {
+  int n = 0;
+  int bits = 2;
+  int64_t total_ops = 0;
+  std::vector<int> plen, tlen;
+  std::vector<int64_t> ops_off;
+  std::vector<int64_t> pat_pos, txt_pos;   // symbol positions (host copy)
+  int64_t words_used = 0;                  // words taken by the forward sequences
+  std::vector<int> sorted;                 // pair ids, longest first
+  wfa_host::DevBuf words, d_plen, d_tlen, d_patw, d_txtw, d_status, d_score, d_last, d_dsteps,
+      d_ops_off, d_ops_begin, d_ops;
+  // reversed copies of the sequences (BiWFA), built on first use
+  wfa_host::DevBuf rev_words, d_rev_ppos, d_rev_tpos;
+  bool have_reverse = false;
+  // launch plan, cached for the penalties it was made for
+  std::vector<wfa_host::PlanGroup> plan;
+  std::vector<int> plan_key;
+  // results: on the device after a run, copied to the host on demand
+  std::vector<int> status, score, ops_begin;
+  bool have_results = false;
+  bool host_results = false;
+  bool ops_on_device = false;
+  int first_fail = 0;
+  bool bi_mode = false;                    // last run was BiWFA: scores in bi_score
+  std::vector<int> bi_score;
+  wfa_run_stats_t stats{};
+  int records = 1;
+  wfa_host::JobSet jobs() const;
+  ~wfa_device_batch_t();
+};
+
+namespace wfa_host {
+// BiWFA (memory mode ultralow) over a whole batch; fills the batch's device
+// results like the unidirectional run.  Returns the first failing status.
+int bialign_run(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, RunTotals* tot);
+bool download_results(Engine* e, wfa_device_batch_t* b);
+}  // namespace wfa_host

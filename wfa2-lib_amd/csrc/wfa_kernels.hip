@@ -17,135 +17,10 @@
 // compute that cell, so the winner (and its tie-break M > D2 > D1 > I2 > I1,
 // extension before opening) is decided here once, and stored as 7 bits next
 // to the cell's pre-extension M offset.  The backtrace kernel replays them.
-#include "wfa_device.h"
-#include <climits>
-#include <type_traits>
+#include "wfa_common.h"
 
 namespace wfa {
 namespace {
-
-// Ring metadata lives in registers: lane r of every wave holds the state of
-// ring slot r (score tag, computed-range origin, trimmed [lo,hi] of each
-// component, present/null flags).  A source is read with v_readlane at a
-// uniform slot index, so no memory round trip sits on the per-step critical
-// path; every wave of a workgroup keeps an identical copy.
-template <int NC>
-struct MetaRegs {
-  int tag;             // score held by the slot, -1 if none
-  int alo;             // lo of the computed range: array origin of all components
-  unsigned flags;      // bit c: component present; bit 8+c: component null (trimmed empty)
-  int lo[NC];
-  int hi[NC];
-};
-
-// A compute input.  lim_lo/lim_hi are the limits it contributes to the output
-// range (the shared null wavefront of the reference has lo = 1, hi = -1).
-// Reads at k in [lo, lo+span] return ring[base + k - lo], else NULL; an
-// invalid source gets an empty read range far from any diagonal.
-struct Src {
-  int lim_lo, lim_hi;
-  int lo, span, base;
-  __device__ __forceinline__ bool valid() const { return lim_lo <= lim_hi; }
-};
-constexpr int kFarDiagonal = 0x40000000;
-
-__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ int lane_of(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
-
-template <int NC>
-__device__ __forceinline__ Src get_src(const MetaRegs<NC>& m, int s_src, int slot, int R, int c, int wcap) {
-  Src r{1, -1, kFarDiagonal, 0, 0};
-  if (s_src < 0) return r;
-  if (slot < 0) slot += R;
-  slot = uniform(slot);
-  const int tag = lane_of(m.tag, slot);
-  const unsigned fl = (unsigned)lane_of((int)m.flags, slot);
-  if (tag != s_src) return r;
-  if (!(fl & (1u << c)) || (fl & (0x100u << c))) return r;
-  r.lim_lo = r.lo = lane_of(m.lo[c], slot);
-  r.lim_hi = lane_of(m.hi[c], slot);
-  r.span = r.lim_hi - r.lim_lo;
-  r.base = (slot * NC + c) * wcap + (r.lo - lane_of(m.alo, slot));
-  return r;
-}
-
-// Branch-free predicated read: the load address is always inside the ring.
-template <typename RingPtr>
-__device__ __forceinline__ int rd(RingPtr ring, const Src& s, int k) {
-  const int i = k - s.lo;
-  const bool in = (unsigned)i <= (unsigned)s.span;
-  const int v = (int)ring[s.base + (in ? i : 0)];
-  return in ? v : kOffsetNull;
-}
-
-// Ring element stores.  The 16-bit LDS ring saturates to [-16384, 16383]:
-// values below 0 are all NULL-like and values above the text length are all
-// out of bounds, so every comparison the recurrence makes keeps its outcome
-// (only used when both sequences are shorter than 8192).
-__device__ __forceinline__ int16_t to_ring(int v, int16_t*) {
-  return (int16_t)(v < -16384 ? -16384 : (v > 16383 ? 16383 : v));
-}
-__device__ __forceinline__ int to_ring(int v, int*) { return v; }
-
-__device__ __forceinline__ bool in_bounds(int off, int k, int plen, int tlen) {
-  return (unsigned)off <= (unsigned)tlen && (unsigned)(off - k) <= (unsigned)plen;
-}
-
-// 64-bit window of symbols starting at symbol `pos` (BITS bits per symbol,
-// little-endian inside each word).  Reads words w and w+1: every packed
-// sequence carries at least one padding word.
-template <int BITS, typename WordPtr>
-__device__ __forceinline__ uint64_t window(WordPtr W, int pos) {
-  constexpr unsigned SPW = 64 / BITS;
-  const unsigned w = (unsigned)pos / SPW;
-  const unsigned sh = ((unsigned)pos % SPW) * BITS;
-  const uint64_t lo = W[w];
-  const uint64_t hi = W[w + 1];
-  // (hi << 1) << (63 - sh) is hi << (64 - sh) for sh > 0 and 0 for sh == 0.
-  return (lo >> sh) | ((hi << 1) << (63 - sh));
-}
-
-// Longest common extension of pattern[v..] and text[h..], bounded by both ends.
-template <int BITS, typename WordPtr>
-__device__ __forceinline__ int lcp_extend(WordPtr P, WordPtr T, int plen, int tlen, int k, int h) {
-  constexpr int SPW = 64 / BITS;
-  int v = h - k;
-  for (;;) {
-    const int rem = min(plen - v, tlen - h);
-    if (rem <= 0) return h;
-    const uint64_t x = window<BITS>(P, v) ^ window<BITS>(T, h);
-    const int n = x ? (int)(__builtin_ctzll(x) / BITS) : SPW;
-    if (n >= rem) return h + rem;
-    h += n;
-    v += n;
-    if (n < SPW) return h;
-  }
-}
-
-struct WaveRange {  // first/last valid diagonal seen by one wave (uniform)
-  int first, last;
-  __device__ __forceinline__ void init() { first = INT_MAX; last = INT_MIN; }
-  __device__ __forceinline__ void add(unsigned long long ballot, int k_lane0) {
-    if (ballot) {
-      if (first == INT_MAX) first = k_lane0 + (int)__builtin_ctzll(ballot);
-      last = k_lane0 + 63 - (int)__builtin_clzll(ballot);
-    }
-  }
-  __device__ __forceinline__ void merge(int f, int l) { first = min(first, f); last = max(last, l); }
-  __device__ __forceinline__ bool null() const { return first == INT_MAX; }
-  // Trimmed range as the reference stores it: [first valid, last valid], or (lo, lo-1).
-  __device__ __forceinline__ int tlo(int lo) const { return null() ? lo : first; }
-  __device__ __forceinline__ int thi(int lo) const { return null() ? lo - 1 : last; }
-};
-
-// Orders LDS traffic between the lanes of the (single) wave of a workgroup:
-// a wave executes its LDS instructions in program order, so only the
-// compiler has to be kept from moving accesses across this point.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 template <int NC, int BLOCK, bool SMALL, int BITS>
 __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
@@ -207,29 +82,34 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
     const int plen = a.plen[p];
     const int tlen = a.tlen[p];
     const int kend = tlen - plen;
+    const int cbeg = a.comp_begin_p ? (int)a.comp_begin_p[p] : a.comp_begin;
+    const int cend = a.comp_end_p ? (int)a.comp_end_p[p] : a.comp_end;
     // Generous bound on the end score of any end-to-end alignment; guarantees
     // that every workgroup leaves the loop even on inconsistent input.
     const int s_cap = (pen.x + pen.o1 + pen.e1 + (NC == 5 ? pen.o2 + pen.e2 : 0) + 2) * (plen + tlen + 2);
 
+    // Sequences: word-aligned base pointer plus a symbol offset below SPW.
+    const long long ppos = a.pat_pos[p], tpos = a.txt_pos[p];
+    const int pb = (int)(ppos % SPW), tb = (int)(tpos % SPW);
     const uint64_t* P;
     const uint64_t* T;
     if constexpr (SMALL) {
-      const uint64_t* gP = a.words + a.pat_word[p];
-      const uint64_t* gT = a.words + a.txt_word[p];
-      const int nwp = plen / SPW + 2, nwt = tlen / SPW + 2;
+      const uint64_t* gP = a.words + ppos / SPW;
+      const uint64_t* gT = a.words + tpos / SPW;
+      const int nwp = (pb + plen) / SPW + 2, nwt = (tb + tlen) / SPW + 2;
       block_sync();  // the previous pair is done reading the sequences
       for (int i = tid; i < nwp; i += BLOCK) seqP[i] = gP[i];
       for (int i = tid; i < nwt; i += BLOCK) seqT[i] = gT[i];
       P = seqP;
       T = seqT;
     } else {
-      P = a.words + a.pat_word[p];
-      T = a.words + a.txt_word[p];
+      P = a.words + ppos / SPW;
+      T = a.words + tpos / SPW;
     }
     // Score 0: the begin component holds offset 0 at k = 0 (wavefront_aligner.c:252-380).
     meta.tag = (lane == 0) ? 0 : -1;
     meta.alo = 0;
-    meta.flags = (lane == 0) ? (1u << a.comp_begin) : 0u;
+    meta.flags = (lane == 0) ? (1u << cbeg) : 0u;
 #pragma unroll
     for (int c = 0; c < NC; ++c) { meta.lo[c] = 0; meta.hi[c] = 0; }
     int cur = 0;
@@ -240,10 +120,10 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
     block_sync();  // sequences visible to every lane
     {
       int v0 = 0;
-      if (tid == 0 && a.comp_begin == kM) v0 = lcp_extend<BITS>(P, T, plen, tlen, 0, 0);
+      if (tid == 0 && cbeg == kM) v0 = lcp_extend<BITS>(P, pb, T, tb, plen, tlen, 0, 0);
       v0 = broadcast0(v0, 1);
-      if (tid == 0) ring[(0 * NC + a.comp_begin) * wcap] = to_ring(v0, ring);
-      if (a.comp_begin == kM && a.comp_end == kM && kend == 0 && v0 >= tlen) status = kDevCompleted;
+      if (tid == 0) ring[(0 * NC + cbeg) * wcap] = to_ring(v0, ring);
+      if (cbeg == kM && cend == kM && kend == 0 && v0 >= tlen) status = kDevCompleted;
     }
     block_sync();
 
@@ -356,7 +236,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
           int m = best;
           if (!in_bounds(m, k, plen, tlen)) m = kOffsetNull;  // null-after-max
           if (rec) rec[k - lo] = ((uint32_t)best & kRecOffMask) | (src << kRecSrcShift) | bits;
-          if (m != kOffsetNull) m = lcp_extend<BITS>(P, T, plen, tlen, k, m);
+          if (m != kOffsetNull) m = lcp_extend<BITS>(P, pb, T, tb, plen, tlen, k, m);
           outM[k] = to_ring(m, ring);
           vM = (m != kOffsetNull);
           if (has_i1) { outI1[k] = to_ring(ins1, ring); vI1 = in_bounds(ins1, k, plen, tlen); }
@@ -417,7 +297,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
       // Termination (wavefront_termination.c:37-114): k_end inside the trimmed
       // range of the end component and its offset reaching the text end.
       {
-        const int ce = a.comp_end;
+        const int ce = cend;
         bool pce = true;
         WaveRange rce = rM;
         if (ce == kI1) { pce = has_i1; rce = rI1; }
@@ -458,7 +338,7 @@ __global__ void __launch_bounds__(256) wfa_backtrace_kernel(BacktraceArgs a) {
   char* out = a.ops + a.ops_off[p];
   if (a.status[p] != kDevCompleted) { a.ops_begin[p] = cap; return; }
   const Penalties pen = a.pen;
-  int comp = a.comp_end;
+  int comp = a.comp_end_p ? (int)a.comp_end_p[p] : a.comp_end;
   int s = a.score[p];
   int k = tlen - plen;
   int off = tlen;
