@@ -149,7 +149,8 @@ def main():
     # Plain form: the same kernels, one pair per launch (wavefront_align).
     al_plain = wfa_amd.Aligner(metric="affine", penalties=(0, 4, 6, 2))
     k = min(args.plain_sample, ps.n)
-    al_plain.align(ps.pattern(0), ps.text(0))
+    if k:
+        al_plain.align(ps.pattern(0), ps.text(0))
     t = time.perf_counter()
     for i in range(k):
         al_plain.align(ps.pattern(i), ps.text(i))
