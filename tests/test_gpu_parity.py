@@ -211,3 +211,38 @@ def test_biwfa_long_noisy_pair_vs_oracle():
     got = al.align_batch(ps)
     ref = oracle_run("affine", (0, 4, 6, 2), ps, memory="ultralow")
     assert_same(got, ref, ps)
+
+
+LEAN_CHILD = r"""
+import sys
+sys.path.insert(0, %r); sys.path.insert(0, %r); sys.path.insert(0, %r)
+import numpy as np, oracle as O, wfa_amd
+from wfa_amd import datagen
+ps = datagen.generate(1, 1001, 1500)
+al = wfa_amd.Aligner(metric="affine", penalties=(0, 4, 6, 2))
+db = al.upload(ps)
+assert db.run() == 0
+got = db.download()
+ref = O.oracle_align(O.Conf("affine", (0, 4, 6, 2), "high"), O.Pairs(ps.patterns(), ps.texts()))
+bad = [i for i in range(ps.n) if got.status[i] != ref.status[i] or got.score[i] != ref.score[i] or got.ops(i) != ref.ops(i)]
+assert not bad, bad[:10]
+assert db.stats().diagonal_steps == int(ref.dsteps.sum())
+print("launches", db.stats().forward_launches)
+"""
+
+
+@pytest.mark.parametrize("env", [{"WFA_LEAN_WCAP": "24,24"}, {"WFA_LEAN": "0"}],
+                         ids=["lean-ring-overflow-retries", "generic-lds-family"])
+def test_kernel_families_vs_oracle(env):
+    """The lean family with a tiny ring (every wide pair overflows and is
+    re-run with a wider ring, then on the HBM family), and the generic LDS
+    family with the lean one turned off, both match the oracle."""
+    import os
+    import subprocess
+    from conftest import ROOT
+    say("kernel family %s" % env)
+    code = LEAN_CHILD % (os.path.join(ROOT, "wfa2-lib_amd"), os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests"))
+    r = subprocess.run([sys.executable, "-c", code], env=dict(os.environ, **env), capture_output=True, timeout=240)
+    assert r.returncode == 0, r.stderr.decode()[-2000:]
+    if "WFA_LEAN_WCAP" in env:
+        assert int(r.stdout.split()[-1]) > 1  # the retries ran

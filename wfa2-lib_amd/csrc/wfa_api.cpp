@@ -263,6 +263,34 @@ int small_wcap(const Penalties& pen, int ncomp) {
   return w;
 }
 
+// Lean family (gap-affine, 2-bit sequences up to kLeanMaxLen): ring width per
+// slot for pairs up to short_len and for longer pairs.  A pair whose
+// wavefront outgrows its ring is re-run with twice the width, up to
+// kLeanMaxWcap, then on the HBM-ring family.  WFA_LEAN=0 turns the family off;
+// WFA_LEAN_WCAP="short,long[,short_len]" overrides the widths.
+constexpr int kLeanMaxLen = 1024;
+constexpr int kLeanMaxWcap = 1024;
+struct LeanTuning {
+  bool enabled = true;
+  int wcap_short = 224;
+  int wcap_long = 384;
+  int short_len = 320;
+};
+const LeanTuning& lean_tuning() {
+  static const LeanTuning t = [] {
+    LeanTuning r;
+    const char* on = getenv("WFA_LEAN");
+    if (on && on[0] == '0') r.enabled = false;
+    const char* w = getenv("WFA_LEAN_WCAP");
+    int a = 0, b = 0, c = 0;
+    const int n = w ? sscanf(w, "%d,%d,%d", &a, &b, &c) : 0;
+    if (n >= 2 && a >= 8 && b >= 8 && a <= kLeanMaxWcap && b <= kLeanMaxWcap) { r.wcap_short = a; r.wcap_long = b; }
+    if (n == 3 && c > 0) r.short_len = c;
+    return r;
+  }();
+  return t;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------ planning
@@ -285,8 +313,14 @@ LaunchShape shape_of(Engine* e, const PlanGroup& g, int two, int bits, const Pen
   LaunchShape shape;
   shape.family = g.family;
   shape.block = g.block;
-  shape.lds_bytes = wfa::forward_lds_bytes(g.family, two ? 5 : 3, pen.nslots, g.wcap, g.lmax_words);
-  int per_cu = wfa::forward_max_blocks_per_cu(g.family, g.block, two, bits, shape.lds_bytes);
+  int per_cu = 0;
+  if (g.family == wfa::kLean) {
+    shape.lds_bytes = wfa::lean_lds_bytes(pen, g.wcap, g.lmax_words);
+    per_cu = wfa::lean_max_blocks_per_cu(pen, shape.lds_bytes);
+  } else {
+    shape.lds_bytes = wfa::forward_lds_bytes(g.family, two ? 5 : 3, pen.nslots, g.wcap, g.lmax_words);
+    per_cu = wfa::forward_max_blocks_per_cu(g.family, g.block, two, bits, shape.lds_bytes);
+  }
   if (per_cu <= 0) per_cu = 1;
   shape.grid = (int)std::min<int64_t>((int64_t)g.ids.size(), (int64_t)per_cu * e->num_cus);
   return shape;
@@ -339,7 +373,9 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
   fa.pen = pen;
   if (!hip_ok(wfa::launch_reset_counters(c, 0, st), "reset")) return WF_STATUS_OOM;
   (void)hipEventRecord(grp.ev[0], st);
-  if (!hip_ok(wfa::launch_forward(fa, two, js.bits, shape, st), "forward launch")) return WF_STATUS_OOM;
+  const hipError_t lerr = grp.family == wfa::kLean ? wfa::launch_forward_lean(fa, shape, st)
+                                                   : wfa::launch_forward(fa, two, js.bits, shape, st);
+  if (!hip_ok(lerr, "forward launch")) return WF_STATUS_OOM;
   (void)hipEventRecord(grp.ev[1], st);
   if (records) {
     BacktraceArgs ba{};
@@ -394,8 +430,18 @@ std::vector<PlanGroup> make_plan(const JobSet& js, const std::vector<int>& ids, 
   const int spw = 64 / js.bits;
   std::vector<int> small_ids, cls[3];
   int lmax_small = 0;
+  const LeanTuning& lt = lean_tuning();
+  const bool lean_ok = lt.enabled && ncomp == 3 && js.bits == 2 && !js.cbeg && !js.cend && wfa::lean_supported(pen);
+  std::vector<int> lean_ids[2];
+  int lean_lmax[2] = {0, 0};
   for (int i : ids) {
     const int L = std::max(js.h_plen[i], js.h_tlen[i]);
+    if (lean_ok && L <= kLeanMaxLen) {
+      const int c = L <= lt.short_len ? 0 : 1;
+      lean_ids[c].push_back(i);
+      lean_lmax[c] = std::max<int>(lean_lmax[c], L / spw + 3);
+      continue;
+    }
     if (wcap_small >= 64 && L <= 1024) {
       small_ids.push_back(i);
       lmax_small = std::max<int>(lmax_small, L / spw + 3);
@@ -405,6 +451,16 @@ std::vector<PlanGroup> make_plan(const JobSet& js, const std::vector<int>& ids, 
     }
   }
   std::vector<PlanGroup> plan;
+  for (int c = 0; c < 2; ++c) {
+    if (lean_ids[c].empty()) continue;
+    PlanGroup g;
+    g.family = wfa::kLean;
+    g.block = 64;
+    g.wcap = c == 0 ? lt.wcap_short : lt.wcap_long;
+    g.lmax_words = lean_lmax[c];
+    g.ids = std::move(lean_ids[c]);
+    plan.push_back(std::move(g));
+  }
   if (!small_ids.empty()) {
     PlanGroup g;
     g.family = wfa::kSmall;
@@ -463,7 +519,19 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
         if (h_status[i] == wfa::kDevRingOverflow) ring_over.push_back(i);
         else if (h_status[i] == wfa::kDevArenaFull) arena_full.push_back(i);
       }
-      if (!ring_over.empty()) next.push_back(make_large(js, std::move(ring_over)));
+      if (!ring_over.empty()) {
+        if (g.family == wfa::kLean && g.wcap * 2 <= kLeanMaxWcap) {
+          PlanGroup h;
+          h.family = wfa::kLean;
+          h.block = 64;
+          h.wcap = g.wcap * 2;
+          h.lmax_words = g.lmax_words;
+          h.ids = std::move(ring_over);
+          next.push_back(std::move(h));
+        } else {
+          next.push_back(make_large(js, std::move(ring_over)));
+        }
+      }
       if (!arena_full.empty()) {
         if (arena_full.size() == g.ids.size()) {
           // not a single job of the group fitted: grow the arena

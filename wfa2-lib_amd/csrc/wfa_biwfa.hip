@@ -36,27 +36,29 @@ struct BlockRed {
   // Block-wide min of per-thread values (uniform result).
   __device__ __forceinline__ int min_all(int v) {
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    v = uniform(v);  // every lane holds the minimum
     if constexpr (NW == 1) return v;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wid = uniform(threadIdx.x >> 6);
     __syncthreads();
     if (lane == 0) sh[wid] = v;
     __syncthreads();
     int r = sh[0];
 #pragma unroll
     for (int w = 1; w < NW; ++w) r = min(r, sh[w]);
-    return r;
+    return uniform(r);
   }
   __device__ __forceinline__ int max_all(int v) {
     for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    v = uniform(v);  // every lane holds the maximum
     if constexpr (NW == 1) return v;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wid = uniform(threadIdx.x >> 6);
     __syncthreads();
     if (lane == 0) sh[wid] = v;
     __syncthreads();
     int r = sh[0];
 #pragma unroll
     for (int w = 1; w < NW; ++w) r = max(r, sh[w]);
-    return r;
+    return uniform(r);
   }
 };
 
@@ -104,7 +106,7 @@ __device__ int side_step(Side<NC>& sd, int s, int plen, int tlen, const Penaltie
                          bool want_max, int* max_ak, long long* dsteps, int (*sh_red)[2 * NC + 2],
                          BlockRed<BLOCK>& red) {
   constexpr int NW = BLOCK / 64;
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wid = uniform(tid >> 6);
   const int R = pen.nslots, g = pen.g;
   *max_ak = 0;
   if (s % g) return 0;  // only multiples of g can be non-null
@@ -239,7 +241,7 @@ __device__ int side_step(Side<NC>& sd, int s, int plen, int tlen, const Penaltie
     if (ce == kD2) { pce = has_d2; rce = rD2; }
   }
   if (pce && !rce.null() && kend >= rce.first && kend <= rce.last) {
-    if (ring[(cur * NC + ce) * wcap + (kend - lo)] >= tlen) {
+    if (uniform(ring[(cur * NC + ce) * wcap + (kend - lo)]) >= tlen) {
       sd.end_score = s;
       return 1;
     }
@@ -280,7 +282,7 @@ __device__ void scan_breakpoint(const Side<NC>& s0d, const Side<NC>& s1d, const 
   first = red.min_all(first);
   if (first == INT_MAX) return;
   const int k0 = first, k1 = K - k0;
-  const int h0 = r0[k0], h1 = r1[k1];
+  const int h0 = uniform(r0[k0]), h1 = uniform(r1[k1]);
   if (forward) {
     bp.score_forward = s0; bp.score_reverse = s1;
     bp.k_forward = k0; bp.k_reverse = k1;
@@ -353,7 +355,7 @@ __device__ int side_extend0(Side<NC>& sd, int plen, int tlen, int wcap, int* max
   int v0 = 0;
   *max_ak = 0;
   if (sd.cb != kM) return 0;  // M absent at score 0: nothing to extend, no termination test
-  v0 = lcp_extend<BITS>(sd.P, sd.pb, sd.T, sd.tb, plen, tlen, 0, 0);
+  v0 = uniform(lcp_extend<BITS>(sd.P, sd.pb, sd.T, sd.tb, plen, tlen, 0, 0));
   __syncthreads();
   if (threadIdx.x == 0) sd.ring[(0 * NC + kM) * wcap] = v0;
   __syncthreads();
@@ -378,7 +380,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
   for (;;) {
     if (threadIdx.x == 0) sh_q = (int)atomicAdd(a.queue, 1u);
     __syncthreads();
-    const int q = sh_q;
+    const int q = uniform(sh_q);
     __syncthreads();
     if (q >= a.n) break;
     const BiProblem pr = a.probs[q];

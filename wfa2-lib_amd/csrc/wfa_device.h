@@ -96,7 +96,9 @@ struct BacktraceArgs {
 
 // Kernel families.  kSmall: one 64-lane wavefront per pair, wavefront ring
 // and both sequences in LDS.  kLarge: one workgroup per pair, ring in HBM.
-enum Family : int { kSmall = 0, kLarge = 1 };
+// kLean: like kSmall, gap-affine with 2-bit sequences, specialised on the
+// penalties' slot structure (wfa_lean.hip).
+enum Family : int { kSmall = 0, kLarge = 1, kLean = 2 };
 
 struct LaunchShape {
   int family;
@@ -178,6 +180,14 @@ hipError_t launch_gather_ops(const OpsPiece* pieces, int n, const char* src, cha
 hipError_t launch_reverse_pack(const uint64_t* words, const int64_t* src_pos, const int64_t* dst_pos, const int* len,
                                int n, int bits, uint64_t* out, hipStream_t stream);
 int breakpoint_max_blocks_per_cu(int two_piece, int bits, int block);
+
+// Lean family: whether the penalties' slot structure has an instance, its
+// dynamic LDS, occupancy and launch (ForwardArgs as for launch_forward; the
+// per-pair component arrays must be null and comp_begin = comp_end = M).
+bool lean_supported(const Penalties& pen);
+size_t lean_lds_bytes(const Penalties& pen, int wcap, int lmax_words);
+int lean_max_blocks_per_cu(const Penalties& pen, size_t lds_bytes);
+hipError_t launch_forward_lean(const ForwardArgs& a, const LaunchShape& shape, hipStream_t stream);
 
 // Dynamic-LDS bytes and occupancy for a configuration.
 size_t forward_lds_bytes(int family, int ncomp, int nslots, int wcap, int lmax_words);

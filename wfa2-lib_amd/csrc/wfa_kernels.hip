@@ -36,7 +36,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wid = tid >> 6;
+  const int wid = uniform(tid >> 6);  // wave index: the same in every lane of a wave
   const Penalties pen = a.pen;
   const int R = pen.nslots;  // <= 64 (checked by the host)
   const int wcap = a.wcap;
@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
     } else {
       if (tid == 0) sh_bcast[slot] = v;
       __syncthreads();
-      const int r = sh_bcast[slot];
+      const int r = uniform(sh_bcast[slot]);
       __syncthreads();
       return r;
     }
@@ -307,7 +307,8 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
           if (ce == kD2) { pce = has_d2; rce = rD2; }
         }
         if (pce && !rce.null() && kend >= rce.first && kend <= rce.last) {
-          const int vend = (int)ring[(cur * NC + ce) * wcap + (kend - lo)];
+          // the same LDS word for every lane: keep the loop control scalar
+          const int vend = uniform((int)ring[(cur * NC + ce) * wcap + (kend - lo)]);
           if (vend >= tlen) status = kDevCompleted;
         }
       }
