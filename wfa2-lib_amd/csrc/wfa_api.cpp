@@ -393,8 +393,13 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
     ba.ops_begin = js.ops_begin;
     ba.fail_count = &c->fail;
     ba.comp_end_p = js.cend;
+    ba.comp_begin_p = js.cbeg;
     ba.comp_begin = wfa::kM;
     ba.comp_end = wfa::kM;
+    ba.words = js.words;
+    ba.pat_pos = js.ppos;
+    ba.txt_pos = js.tpos;
+    ba.bits = js.bits;
     ba.pen = pen;
     if (!hip_ok(wfa::launch_backtrace(ba, st), "backtrace launch")) return WF_STATUS_OOM;
   }

@@ -30,16 +30,22 @@ struct Penalties {
   int nslots;              // ring slots = (scope-1)/g + 1
 };
 
-// Record word (one per diagonal of every non-null step):
-//   bits  0..24  M offset before extension (the max of its candidates)
-//   bits 25..27  winning M source: 0 X, 1 D2, 2 D1, 3 I2, 4 I1
-//   bit  28..31  gap-extension flags for I1, D1, I2, D2 at this diagonal
-constexpr uint32_t kRecOffMask = 0x1FFFFFFu;
-constexpr int kRecSrcShift = 25;
-constexpr uint32_t kRecI1Ext = 1u << 28;
-constexpr uint32_t kRecD1Ext = 1u << 29;
-constexpr uint32_t kRecI2Ext = 1u << 30;
-constexpr uint32_t kRecD2Ext = 1u << 31;
+// Record byte (one per diagonal of every non-null step):
+//   bits 0..2  winning M source: 0 X, 1 D2, 2 D1, 3 I2, 4 I1
+//   bits 3..6  gap-extension flags for I1, D1, I2, D2 at this diagonal
+// A step's block is a 16-byte header {s, lo, hi, previous block} followed by
+// hi - lo + 1 record bytes, padded to 16 bytes.  The offsets themselves are
+// not stored: the backtrace walks (score, diagonal, component) with these
+// bits and rebuilds the match runs by extending forward, which reproduces the
+// forward pass's extensions exactly.
+constexpr uint32_t kRecSrcMask = 7u;
+constexpr uint32_t kRecI1Ext = 1u << 3;
+constexpr uint32_t kRecD1Ext = 1u << 4;
+constexpr uint32_t kRecI2Ext = 1u << 5;
+constexpr uint32_t kRecD2Ext = 1u << 6;
+__host__ __device__ inline unsigned long long rec_units_for(int width) {
+  return 1 + (unsigned long long)((width + 15) / 16);
+}
 enum RecSrc : uint32_t { kSrcX = 0, kSrcD2 = 1, kSrcD1 = 2, kSrcI2 = 3, kSrcI1 = 4 };
 constexpr uint32_t kNoBlock = 0xFFFFFFFFu;
 constexpr uint32_t kChunkUnits = 4096;   // record chunk grabbed per workgroup, in 16-byte units
@@ -90,7 +96,12 @@ struct BacktraceArgs {
   int* ops_begin;              // CIGAR = ops[ops_off + ops_begin .. ops_off + plen + tlen)
   unsigned int* fail_count;    // incremented for a pair whose trace is inconsistent
   const signed char* comp_end_p;  // per-pair end component (null: comp_end)
+  const signed char* comp_begin_p; // per-pair begin component (null: comp_begin)
   int comp_begin, comp_end;
+  const uint64_t* words;       // packed sequences (match runs are re-extended)
+  const int64_t* pat_pos;
+  const int64_t* txt_pos;
+  int bits;                    // 2 or 8 bits per symbol
   Penalties pen;
 };
 

@@ -169,9 +169,9 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
       const bool has_i2 = (NC == 5) && (mo2.valid() || i2.valid());
       const bool has_d2 = (NC == 5) && (mo2.valid() || d2.valid());
 
-      uint32_t* rec = nullptr;
+      uint8_t* rec = nullptr;
       if (a.records) {
-        const unsigned long long units = 1 + (unsigned long long)((width + 3) / 4);
+        const unsigned long long units = rec_units_for(width);
         if (chunk_cur + units > chunk_end) {
           const unsigned long long want = units > kChunkUnits ? units : kChunkUnits;
           int base_lo = 0, base_hi = 0;
@@ -194,7 +194,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
         chunk_cur += units;
         if (tid == 0) a.rec_arena[blk] = make_uint4((unsigned)s, (unsigned)lo, (unsigned)hi, prev_blk);
         prev_blk = blk;
-        rec = reinterpret_cast<uint32_t*>(a.rec_arena + blk + 1);
+        rec = reinterpret_cast<uint8_t*>(a.rec_arena + blk + 1);
       }
 
       RT* outM  = ring + (cur * NC + kM) * wcap - lo;
@@ -235,7 +235,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
           if (ins1 > best) { best = ins1; src = kSrcI1; }
           int m = best;
           if (!in_bounds(m, k, plen, tlen)) m = kOffsetNull;  // null-after-max
-          if (rec) rec[k - lo] = ((uint32_t)best & kRecOffMask) | (src << kRecSrcShift) | bits;
+          if (rec) rec[k - lo] = (uint8_t)(src | bits);
           if (m != kOffsetNull) m = lcp_extend<BITS>(P, pb, T, tb, plen, tlen, k, m);
           outM[k] = to_ring(m, ring);
           vM = (m != kOffsetNull);
@@ -327,103 +327,124 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
   if (tid == 0 && wg_dsteps) atomicAdd(a.dsteps_total, wg_dsteps);
 }
 
-// One lane per pair: replays the records from the end cell back to the origin
-// (wavefront_backtrace.c:320-529) and writes the operations backwards into the
-// pair's slot, like cigar->operations[begin_offset..end_offset).
+// One lane per pair, in two passes over the pair's CIGAR slot
+// ops[0 .. plen+tlen):
+//  1. Walk back from the end cell to the origin over (score, diagonal,
+//     component) with the record bytes (wavefront_backtrace.c:320-529: the
+//     recorded source is the maximum of (offset << 4 | type) the reference
+//     takes).  Each mismatch, insertion or deletion is written backwards at
+//     the end of the slot, its top bit set when the cell it leads into is an
+//     M cell (an X, or the last operation of a gap that closes into M).
+//  2. Replay the operations forward from the origin, extending every M cell
+//     maximally (wavefront_extend_kernels.c:68-123, the same extension the
+//     forward pass made there), and write the full CIGAR into the end of the
+//     slot.  The count of matches is known before the replay (2M + 2X + I + D
+//     = plen + tlen), so the CIGAR starts at plen + tlen - (M + X + I + D) and
+//     its writes never pass the operations still to be read.
+// Near the matrix borders only gaps can reach a cell (offsets of valid M cells
+// are never negative), so walking to score 0 emits the same characters as the
+// reference's shortcut at v == 0 or h == 0.
+template <int BITS>
 __global__ void __launch_bounds__(256) wfa_backtrace_kernel(BacktraceArgs a) {
+  constexpr int SPW = 64 / BITS;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= a.n_pairs) return;
   const int p = a.order ? a.order[q] : q;
   const int plen = a.plen[p], tlen = a.tlen[p];
   const int cap = plen + tlen;
-  char* out = a.ops + a.ops_off[p];
+  unsigned char* out = reinterpret_cast<unsigned char*>(a.ops + a.ops_off[p]);
   if (a.status[p] != kDevCompleted) { a.ops_begin[p] = cap; return; }
   const Penalties pen = a.pen;
+  const int cbeg = a.comp_begin_p ? (int)a.comp_begin_p[p] : a.comp_begin;
   int comp = a.comp_end_p ? (int)a.comp_end_p[p] : a.comp_end;
   int s = a.score[p];
   int k = tlen - plen;
-  int off = tlen;
-  int h = off, v = off - k;
   int pos = cap;
+  int nx = 0, nid = 0;
   bool bad = false;
-  if (comp == kM) {
-    for (int i = plen - v; i > 0; --i) out[--pos] = 'D';
-    for (int i = tlen - h; i > 0; --i) out[--pos] = 'I';
-  }
+  bool mark = comp == kM;  // the next operation written leads into an M cell
   uint32_t blk = a.last_block[p];
   uint4 hdr = make_uint4(0, 0, 0, kNoBlock);
   if (blk != kNoBlock && blk < a.rec_units) hdr = a.rec_arena[blk];
-  while (v > 0 && h > 0 && s > 0) {
+  while (s > 0) {
     while (blk != kNoBlock && blk < a.rec_units && (int)hdr.x > s) {
       blk = hdr.w;
       if (blk != kNoBlock && blk < a.rec_units) hdr = a.rec_arena[blk];
     }
-    if (blk == kNoBlock || blk >= a.rec_units || (int)hdr.x != s || k < (int)hdr.y || k > (int)hdr.z) {
+    if (blk == kNoBlock || blk >= a.rec_units || (int)hdr.x != s || k < (int)hdr.y || k > (int)hdr.z || pos <= 0) {
       bad = true;
       break;
     }
-    const uint32_t rec = reinterpret_cast<const uint32_t*>(a.rec_arena + blk + 1)[k - (int)hdr.y];
+    const uint32_t rec = reinterpret_cast<const uint8_t*>(a.rec_arena + blk + 1)[k - (int)hdr.y];
     if (comp == kM) {
-      const int mpre = (int)(rec & kRecOffMask);
-      const int nm = off - mpre;
-      if (nm < 0 || nm >= pos) { bad = true; break; }
-      for (int n = nm; n > 0; --n) out[--pos] = 'M';
-      off = mpre;
-      v = off - k;
-      h = off;
-      if (v <= 0 || h <= 0) break;
-      switch ((rec >> kRecSrcShift) & 7u) {
-        case kSrcX:  s -= pen.x; out[--pos] = 'X'; --off; break;
-        case kSrcD2:
-          if (rec & kRecD2Ext) { s -= pen.e2; comp = kD2; } else { s -= pen.o2 + pen.e2; }
-          out[--pos] = 'D'; ++k; break;
-        case kSrcD1:
-          if (rec & kRecD1Ext) { s -= pen.e1; comp = kD1; } else { s -= pen.o1 + pen.e1; }
-          out[--pos] = 'D'; ++k; break;
-        case kSrcI2:
-          if (rec & kRecI2Ext) { s -= pen.e2; comp = kI2; } else { s -= pen.o2 + pen.e2; }
-          out[--pos] = 'I'; --k; --off; break;
-        default:
-          if (rec & kRecI1Ext) { s -= pen.e1; comp = kI1; } else { s -= pen.o1 + pen.e1; }
-          out[--pos] = 'I'; --k; --off; break;
+      switch (rec & kRecSrcMask) {
+        case kSrcX:
+          out[--pos] = (unsigned char)('X' | (mark ? 0x80 : 0));
+          ++nx;
+          s -= pen.x;
+          mark = true;
+          continue;
+        case kSrcD2: comp = kD2; break;  // closes a gap: same cell of D2
+        case kSrcD1: comp = kD1; break;
+        case kSrcI2: comp = kI2; break;
+        default: comp = kI1; break;
       }
-    } else if (comp == kI1 || comp == kI2) {
-      const bool ext = (comp == kI1) ? (rec & kRecI1Ext) : (rec & kRecI2Ext);
-      const int e = (comp == kI1) ? pen.e1 : pen.e2;
-      const int o = (comp == kI1) ? pen.o1 : pen.o2;
-      if (ext) { s -= e; } else { s -= o + e; comp = kM; }
-      out[--pos] = 'I'; --k; --off;
+    }
+    // gap cell (s, k) of comp: extension from the same gap or opening from M
+    const bool ins = comp == kI1 || comp == kI2;
+    const bool one = comp == kI1 || comp == kD1;
+    const uint32_t ext_bit = comp == kI1 ? kRecI1Ext : comp == kD1 ? kRecD1Ext : comp == kI2 ? kRecI2Ext : kRecD2Ext;
+    out[--pos] = (unsigned char)((ins ? 'I' : 'D') | (mark ? 0x80 : 0));
+    ++nid;
+    k += ins ? -1 : 1;
+    if (rec & ext_bit) {
+      s -= one ? pen.e1 : pen.e2;
+      mark = false;
     } else {
-      const bool ext = (comp == kD1) ? (rec & kRecD1Ext) : (rec & kRecD2Ext);
-      const int e = (comp == kD1) ? pen.e1 : pen.e2;
-      const int o = (comp == kD1) ? pen.o1 : pen.o2;
-      if (ext) { s -= e; } else { s -= o + e; comp = kM; }
-      out[--pos] = 'D'; ++k;
+      s -= one ? pen.o1 + pen.e1 : pen.o2 + pen.e2;
+      comp = kM;
+      mark = true;
     }
-    v = off - k;
-    h = off;
-    if (pos <= 0 && (v > 0 || h > 0)) { bad = true; break; }
   }
-  if (!bad && comp == kM) {
-    if (v > 0 && h > 0) {
-      int n = min(v, h);
-      v -= n;
-      h -= n;
-      if (n > pos) n = pos, bad = true;
-      while (n-- > 0) out[--pos] = 'M';
+  if (!bad && (s != 0 || k != 0 || comp != cbeg)) bad = true;
+  int start = cap;
+  if (!bad) {
+    const int rest = cap - 2 * nx - nid;
+    if (rest < 0 || (rest & 1)) bad = true;
+    const int len = rest / 2 + (cap - pos);
+    start = cap - len;
+    int w = start, v = 0, h = 0;
+    const long long ppos = a.pat_pos[p], tpos = a.txt_pos[p];
+    const uint64_t* P = a.words + ppos / SPW;
+    const uint64_t* T = a.words + tpos / SPW;
+    const int pb = (int)(ppos % SPW), tb = (int)(tpos % SPW);
+    auto extend = [&]() {
+      const int h2 = lcp_extend<BITS>(P, pb, T, tb, plen, tlen, h - v, h);
+      const int m = h2 - h;
+      if (w + m > cap) { bad = true; return; }
+      for (int i = 0; i < m; ++i) out[w++] = 'M';
+      v += m;
+      h += m;
+    };
+    if (!bad && cbeg == kM) extend();
+    for (int i = pos; i < cap && !bad; ++i) {
+      const unsigned char b = out[i];
+      const unsigned char op = b & 0x7F;
+      if (w > i) { bad = true; break; }  // inconsistent records: never overrun unread operations
+      out[w++] = op;
+      if (op == 'X') { ++v; ++h; }
+      else if (op == 'I') ++h;
+      else ++v;
+      if (b & 0x80) extend();
     }
-    if (v + h > pos) bad = true;
-    else {
-      while (v > 0) { out[--pos] = 'D'; --v; }
-      while (h > 0) { out[--pos] = 'I'; --h; }
-    }
+    if (!bad && (v != plen || h != tlen || w != cap)) bad = true;
   }
   if (bad) {
     a.status[p] = kDevBadTrace;
-    pos = cap;
+    start = cap;
     atomicAdd(a.fail_count, 1u);
   }
-  a.ops_begin[p] = pos;
+  a.ops_begin[p] = start;
 }
 
 __global__ void wfa_reset_kernel(Counters* c, int all) {
@@ -495,7 +516,9 @@ hipError_t launch_backtrace(const BacktraceArgs& a, hipStream_t stream) {
   if (a.n_pairs <= 0) return hipSuccess;
   const int block = 256;
   const int grid = (a.n_pairs + block - 1) / block;
-  hipLaunchKernelGGL(wfa_backtrace_kernel, dim3(grid), dim3(block), 0, stream, a);
+  if (a.bits == 2) hipLaunchKernelGGL(wfa_backtrace_kernel<2>, dim3(grid), dim3(block), 0, stream, a);
+  else if (a.bits == 8) hipLaunchKernelGGL(wfa_backtrace_kernel<8>, dim3(grid), dim3(block), 0, stream, a);
+  else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

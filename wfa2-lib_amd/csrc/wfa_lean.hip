@@ -166,9 +166,9 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
         const bool has_i1 = vo || vi;
         const bool has_d1 = vo || vd;
 
-        uint32_t* rec = nullptr;
+        uint8_t* rec = nullptr;
         if (a.records) {
-          const unsigned long long units = 1 + (unsigned long long)((width + 3) / 4);
+          const unsigned long long units = rec_units_for(width);
           if (chunk_cur + units > chunk_end) {
             const unsigned long long want = units > kChunkUnits ? units : kChunkUnits;
             unsigned long long b0 = 0;
@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
           chunk_cur += units;
           if (lane == 0) a.rec_arena[blk] = make_uint4((unsigned)s, (unsigned)lo, (unsigned)hi, prev_blk);
           prev_blk = blk;
-          rec = reinterpret_cast<uint32_t*>(a.rec_arena + blk + 1);
+          rec = reinterpret_cast<uint8_t*>(a.rec_arena + blk + 1);
         }
 
         // Outputs of this step go to slot ph, origin lo - 1.
@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
           if (del1 > best) { best = del1; src = kSrcD1; }
           if (ins1 > best) { best = ins1; src = kSrcI1; }
           const uint32_t bits = (ei1 >= om_l ? kRecI1Ext : 0u) | (ed1 >= om_r ? kRecD1Ext : 0u);
-          if (rec && act) rec[k - lo] = ((uint32_t)best & kRecOffMask) | (src << kRecSrcShift) | bits;
+          if (rec && act) rec[k - lo] = (uint8_t)(src | bits);
           // null-after-max, then the extension of the valid M cells: one
           // 16-symbol window for every lane, more only for lanes whose run
           // fills it (uniform branch).
