@@ -369,6 +369,58 @@ typedef struct {
 } wfa_run_stats_t;
 int wfa_device_batch_stats(wfa_device_batch_t* batch, wfa_run_stats_t* stats);
 
+/* ------------------------------------------------------------ serving
+ * No counterpart in the reference (its wavefront_align is synchronous per
+ * pair).  A request loop for single-pair requests that arrive over time:
+ * requests are queued by length class (short / medium with the memory-high
+ * algorithm, long with BiWFA), and worker lanes -- each an aligner with its
+ * own HIP stream and resident device buffers -- admit the queued requests of
+ * their class into device batches, up to max_batch_pairs and up to the lane's
+ * memory budget on each request's estimated footprint.  Batches of different
+ * lanes run concurrently on the GPU.  Per request, results are identical to
+ * wavefront_align with the same attributes. */
+typedef struct wfa_server_t wfa_server_t;
+typedef struct {
+  int short_lanes;          /* lanes for requests with max(n, m) <= short_max_length */
+  int medium_lanes;         /* lanes for requests between the two limits */
+  int long_lanes;           /* lanes for requests with max(n, m) >= long_min_length (BiWFA) */
+  int short_max_length;
+  int long_min_length;
+  int max_batch_pairs;      /* requests admitted into one device batch */
+  size_t lane_memory_bytes; /* per-lane budget: record arena and admission limit */
+} wfa_server_config_t;
+
+typedef struct {
+  int64_t id;               /* as returned by wfa_server_submit */
+  int status;               /* as wavefront_align's return value */
+  int score;
+  int64_t cigar_offset;     /* operations at cigar_arena[cigar_offset .. +cigar_length) */
+  int cigar_length;
+  double submit_time;       /* seconds on wfa_server_clock(): submitted, */
+  double start_time;        /* its device batch started, */
+  double finish_time;       /* and finished */
+} wfa_server_result_t;
+
+void wfa_server_config_default(wfa_server_config_t* config);
+/* attributes: as for wavefront_aligner_new (memory_mode is chosen per class). */
+wfa_server_t* wfa_server_create(const wavefront_aligner_attr_t* attributes, const wfa_server_config_t* config);
+/* Copies the sequences.  Thread-safe.  Returns the request id (>= 0). */
+int64_t wfa_server_submit(wfa_server_t* server,
+    const char* pattern, int pattern_length, const char* text, int text_length);
+/* Submits pairs[i] at arrival_s[i] seconds after the call, from a thread of
+   the server (the pairs are copied first); returns at once. */
+int wfa_server_replay(wfa_server_t* server, const wfa_pair_t* pairs, const double* arrival_s, int num_pairs);
+/* Waits up to timeout_ms for finished requests, then copies up to
+   max_results of them (in completion order) and their CIGARs into
+   cigar_arena.  Returns the number copied. */
+int wfa_server_collect(wfa_server_t* server, wfa_server_result_t* results, int max_results,
+    char* cigar_arena, size_t cigar_arena_bytes, int timeout_ms);
+/* Requests submitted and not yet collected. */
+int64_t wfa_server_pending(wfa_server_t* server);
+double wfa_server_clock(void);
+/* Finishes the replays and every queued request, then frees the server. */
+void wfa_server_destroy(wfa_server_t* server);
+
 /* ------------------------------------------------------------ CIGAR utils
  * alignment/cigar.c:277-470 */
 bool cigar_is_null(const cigar_t* const cigar);

@@ -62,13 +62,50 @@ void DevBuf::release() {
 
 void PlanGroup::release() {
   d_ids.release();
+  ids_on_device = false;
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
   ev[0] = ev[1] = ev[2] = nullptr;
 }
 
+bool HostBuf::ensure(size_t bytes) {
+  if (bytes <= cap && p) return true;
+  if (p) (void)hipHostFree(p);
+  p = nullptr;
+  cap = 0;
+  const size_t want = std::max<size_t>(bytes, 4096);
+  if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+    p = nullptr;
+    return false;
+  }
+  cap = want;
+  return true;
+}
+
+void HostBuf::release() {
+  if (p) (void)hipHostFree(p);
+  p = nullptr;
+  cap = 0;
+}
+
+void recycle_plan(std::vector<PlanGroup>& old_plan, std::vector<PlanGroup>& new_plan) {
+  size_t i = 0;
+  for (; i < old_plan.size() && i < new_plan.size(); ++i) {
+    PlanGroup& o = old_plan[i];
+    PlanGroup& n = new_plan[i];
+    std::swap(o.d_ids, n.d_ids);
+    for (int j = 0; j < 3; ++j) std::swap(o.ev[j], n.ev[j]);
+    n.ids_on_device = false;
+  }
+  for (auto& g : old_plan) g.release();
+  old_plan.clear();
+}
+
 Engine* engine_of(wavefront_aligner_t* wf) { return static_cast<Engine*>(wf->device); }
 wfa::Counters* counters_of(Engine* e) { return e->counters.as<wfa::Counters>(); }
+
+// Record blocks are addressed by 32-bit indices of 16-byte units.
+constexpr size_t kMaxArenaBytes = 0xFFFFFFF0ull * 16;
 
 bool engine_init(Engine* e) {
   if (e->ok) return true;
@@ -78,12 +115,16 @@ bool engine_init(Engine* e) {
   e->num_cus = prop.multiProcessorCount;
   if (!hip_ok(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "hipStreamCreate")) return false;
   if (!e->counters.ensure(64)) return false;
-  const char* env = getenv("WFA_RECORD_ARENA_MB");
-  size_t mb = env ? strtoull(env, nullptr, 10) : 8192;
+  // Record arena: up to half of the free HBM (block indices address 64 GB),
+  // or WFA_RECORD_ARENA_MB; each run sizes it from its pairs' lengths.
   size_t free_b = 0, total_b = 0;
-  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0)
-    mb = std::min<size_t>(mb, (free_b / 4) >> 20);
-  e->rec_bytes = std::max<size_t>(mb, 64) << 20;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+  size_t cap = std::min<size_t>(free_b / 2, kMaxArenaBytes);
+  const char* env = getenv("WFA_RECORD_ARENA_MB");
+  if (env) cap = std::min<size_t>(strtoull(env, nullptr, 10) << 20, kMaxArenaBytes);
+  if (e->rec_cap) cap = std::min(cap, e->rec_cap);  // a cap set before the first run
+  e->rec_cap = std::max<size_t>(cap, 64ull << 20);
+  e->rec_bytes = std::min<size_t>(8ull << 30, e->rec_cap);
   e->ok = true;
   return true;
 }
@@ -93,6 +134,8 @@ namespace {
 void engine_free(Engine* e) {
   if (!e) return;
   if (e->stream) (void)hipStreamSynchronize(e->stream);
+  delete e->scratch;
+  e->scratch = nullptr;
   e->rec.release();
   e->ring.release();
   e->counters.release();
@@ -333,10 +376,11 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
   if (n == 0) return WF_STATUS_OK;
   hipStream_t st = e->stream;
   const LaunchShape shape = shape_of(e, grp, two, js.bits, pen);
-  if (!grp.d_ids.p) {
+  if (!grp.ids_on_device) {
     if (!grp.d_ids.ensure((size_t)n * 4)) return WF_STATUS_OOM;
     if (!hip_ok(hipMemcpyAsync(grp.d_ids.p, grp.ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, st), "plan upload"))
       return WF_STATUS_OOM;
+    grp.ids_on_device = true;
   }
   for (auto& ev : grp.ev)
     if (!ev && !hip_ok(hipEventCreate(&ev), "hipEventCreate")) return WF_STATUS_OOM;
@@ -370,6 +414,7 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
   fa.records = records;
   fa.comp_begin = wfa::kM;
   fa.comp_end = wfa::kM;
+  fa.defer_shift = 8;  // about L^2/16 bytes of records
   fa.pen = pen;
   if (!hip_ok(wfa::launch_reset_counters(c, 0, st), "reset")) return WF_STATUS_OOM;
   (void)hipEventRecord(grp.ev[0], st);
@@ -489,8 +534,27 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
   std::vector<PlanGroup> retry_groups;
   std::vector<PlanGroup>* pending = &plan;
   if (!hip_ok(wfa::launch_reset_counters(counters_of(e), 2, st), "reset")) return WF_STATUS_OOM;
+  if (records) {
+    // Rough footprint: one byte per diagonal-step, about L^2/16 of them for
+    // the pairs at hand, plus a 64 KB chunk per resident workgroup.
+    double want = 0;
+    for (PlanGroup& g : plan) {
+      if (g.est_rec_bytes < 0) {
+        double w = 0;
+        for (int i : g.ids) {
+          const double L = std::max(js.h_plen[i], js.h_tlen[i]);
+          w += L * L / 16 + 8 * L;
+        }
+        g.est_rec_bytes = w + (double)shape_of(e, g, two, js.bits, pen).grid * wfa::kChunkUnits * 16;
+      }
+      want += g.est_rec_bytes;
+    }
+    const size_t w = (size_t)std::min<double>(want, (double)e->rec_cap);
+    if (w > e->rec_bytes) e->rec_bytes = ((w >> 26) + 1) << 26;  // 64 MB steps
+    e->rec_bytes = std::min(e->rec_bytes, e->rec_cap);
+  }
   for (int round = 0;; ++round) {
-    if (round > 48) {
+    if (round > 256) {
       fprintf(stderr, "[WFA] batch did not converge (record arena too small?)\n");
       return WF_STATUS_OOM;
     }
@@ -542,7 +606,7 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
           // not a single job of the group fitted: grow the arena
           size_t free_b = 0, total_b = 0;
           (void)hipMemGetInfo(&free_b, &total_b);
-          if (e->rec_bytes * 2 > e->rec.cap + free_b / 2) {
+          if (e->rec_bytes * 2 > std::min(e->rec_cap, e->rec.cap + free_b / 2)) {
             fprintf(stderr, "[WFA] record arena exhausted (%zu MB)\n", e->rec_bytes >> 20);
             for (int i : arena_full) h_status[i] = WF_STATUS_OOM;
             if (!hip_ok(hipMemcpy(js.status, h_status.data(), (size_t)js.n * 4, hipMemcpyHostToDevice), "status"))
@@ -618,6 +682,7 @@ wfa_host::JobSet wfa_device_batch_t::jobs() const {
 }
 
 wfa_device_batch_t::~wfa_device_batch_t() {
+  h_words.release();
   for (auto& g : plan) g.release();
   for (wfa_host::DevBuf* x : {&words, &d_plen, &d_tlen, &d_patw, &d_txtw, &d_status, &d_score, &d_last, &d_dsteps,
                               &d_ops_off, &d_ops_begin, &d_ops, &rev_words, &d_rev_ppos, &d_rev_tpos})
@@ -654,6 +719,8 @@ extern "C" void wavefront_aligner_reap(wavefront_aligner_t* const wf) {
   Engine* e = engine_of(wf);
   if (!e) return;
   if (e->stream) (void)hipStreamSynchronize(e->stream);
+  delete e->scratch;
+  e->scratch = nullptr;
   e->rec.release();
   e->ring.release();
 }
@@ -679,78 +746,92 @@ extern "C" size_t wavefront_align_batch_arena_bytes(const wfa_pair_t* pairs, int
   return b;
 }
 
+bool wfa_device_batch_t::load(Engine* e, const wfa_pair_t* pairs, int count) {
+  n = count;
+  bits = all_acgt(pairs, n) ? 2 : 8;
+  plen.resize(n);
+  tlen.resize(n);
+  ops_off.resize(n);
+  pat_pos.resize(n);
+  txt_pos.resize(n);
+  std::vector<int64_t> patw(n), txtw(n);  // word index of each sequence
+  int64_t nw = 0, ops = 0;
+  for (int i = 0; i < n; ++i) {
+    plen[i] = pairs[i].pattern_length;
+    tlen[i] = pairs[i].text_length;
+    patw[i] = nw;
+    nw += nwords(plen[i], bits);
+    txtw[i] = nw;
+    nw += nwords(tlen[i], bits);
+    ops_off[i] = ops;
+    ops += (int64_t)plen[i] + tlen[i];
+  }
+  total_ops = ops;
+  words_used = nw;
+  if (!h_words.ensure(std::max<size_t>(nw, 1) * 8)) {
+    fprintf(stderr, "[WFA] pinned host allocation failed for a batch of %d pairs\n", n);
+    return false;
+  }
+  uint64_t* host_words = h_words.as<uint64_t>();
+  const int64_t per_thread = 1 << 16;  // symbols: small batches are packed inline
+  const int nt = std::max(1, std::min<int>({16, (int)std::thread::hardware_concurrency(),
+                                            (int)std::min<int64_t>(nw * (64 / bits) / per_thread + 1, 16)}));
+  auto pack = [&](int t) {
+    for (int i = t; i < n; i += nt) {
+      pack_seq(host_words + patw[i], pairs[i].pattern, plen[i], bits);
+      pack_seq(host_words + txtw[i], pairs[i].text, tlen[i], bits);
+    }
+  };
+  if (nt == 1) {
+    pack(0);
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t) th.emplace_back(pack, t);
+    for (auto& x : th) x.join();
+  }
+  sorted.resize(n);
+  std::iota(sorted.begin(), sorted.end(), 0);
+  std::stable_sort(sorted.begin(), sorted.end(), [&](int x, int y) {
+    return (int64_t)plen[x] + tlen[x] > (int64_t)plen[y] + tlen[y];
+  });
+  plan_key.clear();  // new pairs: re-plan on the next run
+  have_reverse = false;
+  have_results = host_results = ops_on_device = false;
+  const size_t nn = std::max(n, 1);
+  bool ok = words.ensure(std::max<size_t>(nw, 1) * 8) && d_plen.ensure(nn * 4) && d_tlen.ensure(nn * 4) &&
+            d_patw.ensure(nn * 8) && d_txtw.ensure(nn * 8) && d_status.ensure(nn * 4) && d_score.ensure(nn * 4) &&
+            d_last.ensure(nn * 4) && d_dsteps.ensure(nn * 8) && d_ops_off.ensure(nn * 8) &&
+            d_ops_begin.ensure(nn * 4);
+  if (!ok) {
+    fprintf(stderr, "[WFA] device allocation failed for a batch of %d pairs\n", n);
+    return false;
+  }
+  // Symbol positions (word index times symbols per word) for the kernels.
+  const int64_t spw = 64 / bits;
+  for (int i = 0; i < n; ++i) {
+    pat_pos[i] = patw[i] * spw;
+    txt_pos[i] = txtw[i] * spw;
+  }
+  hipStream_t st = e->stream;
+  ok = n == 0 ||
+       (hip_ok(hipMemcpyAsync(words.p, host_words, nw * 8, hipMemcpyHostToDevice, st), "upload") &&
+        hip_ok(hipMemcpyAsync(d_plen.p, plen.data(), n * 4, hipMemcpyHostToDevice, st), "upload") &&
+        hip_ok(hipMemcpyAsync(d_tlen.p, tlen.data(), n * 4, hipMemcpyHostToDevice, st), "upload") &&
+        hip_ok(hipMemcpyAsync(d_patw.p, pat_pos.data(), n * 8, hipMemcpyHostToDevice, st), "upload") &&
+        hip_ok(hipMemcpyAsync(d_txtw.p, txt_pos.data(), n * 8, hipMemcpyHostToDevice, st), "upload") &&
+        hip_ok(hipMemcpyAsync(d_ops_off.p, ops_off.data(), n * 8, hipMemcpyHostToDevice, st), "upload"));
+  // No wait here: the runs are ordered after the copies on the same stream,
+  // and the host arrays stay untouched until the next load.
+  return ok;
+}
+
 extern "C" wfa_device_batch_t* wfa_device_batch_upload(wavefront_aligner_t* const wf, const wfa_pair_t* pairs,
                                                       int n) {
   Engine* e = engine_of(wf);
   if (!e || !engine_init(e) || n < 0) return nullptr;
   wfa_device_batch_t* b = new (std::nothrow) wfa_device_batch_t();
   if (!b) return nullptr;
-  b->n = n;
-  b->bits = all_acgt(pairs, n) ? 2 : 8;
-  b->plen.resize(n);
-  b->tlen.resize(n);
-  b->ops_off.resize(n);
-  b->pat_pos.resize(n);
-  b->txt_pos.resize(n);
-  std::vector<int64_t> patw(n), txtw(n);  // word index of each sequence
-  int64_t words = 0, ops = 0;
-  for (int i = 0; i < n; ++i) {
-    b->plen[i] = pairs[i].pattern_length;
-    b->tlen[i] = pairs[i].text_length;
-    patw[i] = words;
-    words += nwords(b->plen[i], b->bits);
-    txtw[i] = words;
-    words += nwords(b->tlen[i], b->bits);
-    b->ops_off[i] = ops;
-    ops += (int64_t)b->plen[i] + b->tlen[i];
-  }
-  b->total_ops = ops;
-  b->words_used = words;
-  std::vector<uint64_t> host_words((size_t)words);
-  {
-    const int nt = std::max(1, std::min<int>(16, (int)std::thread::hardware_concurrency()));
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; ++t) {
-      th.emplace_back([&, t]() {
-        for (int i = t; i < n; i += nt) {
-          pack_seq(host_words.data() + patw[i], pairs[i].pattern, b->plen[i], b->bits);
-          pack_seq(host_words.data() + txtw[i], pairs[i].text, b->tlen[i], b->bits);
-        }
-      });
-    }
-    for (auto& x : th) x.join();
-  }
-  b->sorted.resize(n);
-  std::iota(b->sorted.begin(), b->sorted.end(), 0);
-  std::stable_sort(b->sorted.begin(), b->sorted.end(), [&](int x, int y) {
-    return (int64_t)b->plen[x] + b->tlen[x] > (int64_t)b->plen[y] + b->tlen[y];
-  });
-  const size_t nn = std::max(n, 1);
-  bool ok = b->words.ensure(std::max<size_t>(words, 1) * 8) && b->d_plen.ensure(nn * 4) &&
-            b->d_tlen.ensure(nn * 4) && b->d_patw.ensure(nn * 8) && b->d_txtw.ensure(nn * 8) &&
-            b->d_status.ensure(nn * 4) && b->d_score.ensure(nn * 4) && b->d_last.ensure(nn * 4) &&
-            b->d_dsteps.ensure(nn * 8) && b->d_ops_off.ensure(nn * 8) && b->d_ops_begin.ensure(nn * 4);
-  if (!ok) {
-    fprintf(stderr, "[WFA] device allocation failed for a batch of %d pairs\n", n);
-    delete b;
-    return nullptr;
-  }
-  // Symbol positions (word index times symbols per word) for the kernels.
-  const int64_t spw = 64 / b->bits;
-  for (int i = 0; i < n; ++i) {
-    b->pat_pos[i] = patw[i] * spw;
-    b->txt_pos[i] = txtw[i] * spw;
-  }
-  hipStream_t st = e->stream;
-  ok = n == 0 ||
-       (hip_ok(hipMemcpyAsync(b->words.p, host_words.data(), words * 8, hipMemcpyHostToDevice, st), "upload") &&
-        hip_ok(hipMemcpyAsync(b->d_plen.p, b->plen.data(), n * 4, hipMemcpyHostToDevice, st), "upload") &&
-        hip_ok(hipMemcpyAsync(b->d_tlen.p, b->tlen.data(), n * 4, hipMemcpyHostToDevice, st), "upload") &&
-        hip_ok(hipMemcpyAsync(b->d_patw.p, b->pat_pos.data(), n * 8, hipMemcpyHostToDevice, st), "upload") &&
-        hip_ok(hipMemcpyAsync(b->d_txtw.p, b->txt_pos.data(), n * 8, hipMemcpyHostToDevice, st), "upload") &&
-        hip_ok(hipMemcpyAsync(b->d_ops_off.p, b->ops_off.data(), n * 8, hipMemcpyHostToDevice, st), "upload"));
-  ok = ok && hip_ok(hipStreamSynchronize(st), "upload sync");
-  if (!ok) {
+  if (!b->load(e, pairs, n)) {
     delete b;
     return nullptr;
   }
@@ -793,8 +874,9 @@ extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_ba
   } else {
     const std::vector<int> key = {pen.x, pen.o1, pen.e1, pen.o2, pen.e2, two, b->bits};
     if (b->plan_key != key) {
-      for (auto& g : b->plan) g.release();
-      b->plan = make_plan(b->jobs(), b->sorted, pen, ncomp);
+      std::vector<PlanGroup> plan = make_plan(b->jobs(), b->sorted, pen, ncomp);
+      recycle_plan(b->plan, plan);
+      b->plan = std::move(plan);
       b->plan_key = key;
     }
     std::vector<int> scratch;
@@ -887,13 +969,16 @@ extern "C" int wavefront_align_batch(wavefront_aligner_t* const wf, const wfa_pa
             wavefront_align_batch_arena_bytes(pairs, num_pairs));
     return WF_STATUS_OOM;
   }
-  wfa_device_batch_t* b = wfa_device_batch_upload(wf, pairs, num_pairs);
-  if (!b) return WF_STATUS_OOM;
+  // The aligner's resident batch: its device buffers, pinned staging and
+  // plan buffers are reused from call to call (grown when too small).
+  Engine* e = engine_of(wf);
+  if (!e || !engine_init(e)) return WF_STATUS_OOM;
+  if (!e->scratch) e->scratch = new (std::nothrow) wfa_device_batch_t();
+  wfa_device_batch_t* b = e->scratch;
+  if (!b || !b->load(e, pairs, num_pairs)) return WF_STATUS_OOM;
   int rc = wfa_device_batch_run(wf, b);
-  if (rc == WF_STATUS_OOM && !b->have_results) { wfa_device_batch_free(b); return rc; }
-  rc = wfa_device_batch_download(wf, b, results, cigar_arena, cigar_arena_bytes);
-  wfa_device_batch_free(b);
-  return rc;
+  if (rc == WF_STATUS_OOM && !b->have_results) return rc;
+  return wfa_device_batch_download(wf, b, results, cigar_arena, cigar_arena_bytes);
 }
 
 // ------------------------------------------------------------ single pair

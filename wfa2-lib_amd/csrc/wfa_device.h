@@ -78,6 +78,7 @@ struct ForwardArgs {
   int max_steps;               // INT_MAX if unlimited
   int records;                 // 0 = score only
   int comp_begin, comp_end;
+  int defer_shift;             // HBM-ring family: defer a pair when the arena has < L^2 >> shift units left (0: never)
   Penalties pen;
 };
 

@@ -31,8 +31,10 @@ struct Engine {
   DevBuf rec;
   DevBuf ring;
   DevBuf counters;
-  size_t rec_bytes = 0;
+  size_t rec_bytes = 0;     // record arena size for the next run (grows on demand)
+  size_t rec_cap = 0;       // largest record arena this engine may allocate
   bool ok = false;
+  ::wfa_device_batch_t* scratch = nullptr;  // reused by wavefront_align / wavefront_align_batch
 };
 
 Engine* engine_of(wavefront_aligner_t* wf);
@@ -69,6 +71,15 @@ struct JobSet {
   int* ops_begin = nullptr;
 };
 
+// Grow-only pinned host staging buffer (hipHostMalloc).
+struct HostBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  bool ensure(size_t bytes);
+  void release();
+  template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
 // One forward launch (+ backtrace) over a subset of jobs with one kernel family.
 struct PlanGroup {
   int family = 0;
@@ -77,9 +88,15 @@ struct PlanGroup {
   int lmax_words = 0;
   std::vector<int> ids;
   DevBuf d_ids;
+  bool ids_on_device = false;  // d_ids holds ids (uploaded once per plan)
+  double est_rec_bytes = -1;   // rough record footprint of the group (computed once)
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   void release();
 };
+// Moves the device buffers and events of old groups into new ones (same
+// index), releasing what is left: re-planning allocates nothing new when the
+// plan does not grow.
+void recycle_plan(std::vector<PlanGroup>& old_plan, std::vector<PlanGroup>& new_plan);
 
 struct RunTotals {
   double forward_ms = 0;
@@ -127,7 +144,11 @@ struct wfa_device_batch_t {
   std::vector<int> bi_score;
   wfa_run_stats_t stats{};
   int records = 1;
+  wfa_host::HostBuf h_words;               // pinned staging of the packed sequences
   wfa_host::JobSet jobs() const;
+  // (Re)fills the batch with new pairs, reusing every buffer that is large
+  // enough.  Returns false on an allocation or copy failure.
+  bool load(wfa_host::Engine* e, const wfa_pair_t* pairs, int n);
   ~wfa_device_batch_t();
 };
 

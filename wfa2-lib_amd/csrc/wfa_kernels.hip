@@ -117,8 +117,23 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
     int status = 1;  // running
     uint32_t prev_blk = kNoBlock;
     long long dsteps = 0;
+    if constexpr (!SMALL) {
+      // A long pair starts only if the record arena can still hold about
+      // L^2 >> defer_shift units (or is empty); otherwise it is handed back
+      // for a later launch before any work (a pair stopped half way wastes
+      // the records it wrote).
+      if (a.records && a.defer_shift > 0) {
+        int defer = 0;
+        if (tid == 0) {
+          const unsigned long long top = __hip_atomic_load(a.rec_top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          const unsigned long long L = (unsigned long long)max(plen, tlen);
+          defer = top > 0 && top + ((L * L) >> a.defer_shift) > a.rec_units;  // an empty arena always takes it
+        }
+        if (broadcast0(defer, 1)) status = kDevArenaFull;
+      }
+    }
     block_sync();  // sequences visible to every lane
-    {
+    if (status == 1) {
       int v0 = 0;
       if (tid == 0 && cbeg == kM) v0 = lcp_extend<BITS>(P, pb, T, tb, plen, tlen, 0, 0);
       v0 = broadcast0(v0, 1);

@@ -134,6 +134,18 @@ class wfa_result_t(Structure):
                 ("cigar_offset", c_int64), ("cigar_length", c_int)]
 
 
+class wfa_server_config_t(Structure):
+    _fields_ = [("short_lanes", c_int), ("medium_lanes", c_int), ("long_lanes", c_int),
+                ("short_max_length", c_int), ("long_min_length", c_int), ("max_batch_pairs", c_int),
+                ("lane_memory_bytes", ctypes.c_size_t)]
+
+
+class wfa_server_result_t(Structure):
+    _fields_ = [("id", c_int64), ("status", c_int), ("score", c_int), ("cigar_offset", c_int64),
+                ("cigar_length", c_int), ("submit_time", c_double), ("start_time", c_double),
+                ("finish_time", c_double)]
+
+
 class wfa_run_stats_t(Structure):
     _fields_ = [("diagonal_steps", c_int64), ("forward_ms", c_double), ("backtrace_ms", c_double),
                 ("forward_launches", c_int)]
@@ -162,6 +174,17 @@ def lib():
                                                 ctypes.c_size_t]
         L.wfa_device_batch_free.argtypes = [c_void_p]
         L.wfa_device_batch_stats.argtypes = [c_void_p, POINTER(wfa_run_stats_t)]
+        L.wfa_server_config_default.argtypes = [POINTER(wfa_server_config_t)]
+        L.wfa_server_create.restype = c_void_p
+        L.wfa_server_create.argtypes = [POINTER(wavefront_aligner_attr_t), POINTER(wfa_server_config_t)]
+        L.wfa_server_submit.restype = c_int64
+        L.wfa_server_submit.argtypes = [c_void_p, c_char_p, c_int, c_char_p, c_int]
+        L.wfa_server_replay.argtypes = [c_void_p, c_void_p, c_void_p, c_int]
+        L.wfa_server_collect.argtypes = [c_void_p, c_void_p, c_int, c_void_p, ctypes.c_size_t, c_int]
+        L.wfa_server_pending.restype = c_int64
+        L.wfa_server_pending.argtypes = [c_void_p]
+        L.wfa_server_clock.restype = c_double
+        L.wfa_server_destroy.argtypes = [c_void_p]
         L.wavefront_align_strerror.restype = c_char_p
         L.wavefront_align_strerror_short.restype = c_char_p
         _lib = L
@@ -318,6 +341,87 @@ class Aligner:
 
     def upload(self, patterns, texts=None) -> DeviceBatch:
         return DeviceBatch(self, patterns, texts)
+
+
+class ServedResult:
+    __slots__ = ("id", "status", "score", "ops", "submit", "start", "finish")
+
+    def __init__(self, r, ops):
+        self.id, self.status, self.score, self.ops = r.id, r.status, r.score, ops
+        self.submit, self.start, self.finish = r.submit_time, r.start_time, r.finish_time
+
+    @property
+    def latency(self):
+        return self.finish - self.submit
+
+
+class Server:
+    """wfa_server_*: a request loop over single-pair requests.
+
+    Keyword arguments override the fields of wfa_server_config_default; the
+    aligner attributes are built as for Aligner."""
+
+    def __init__(self, metric="affine", penalties=(0, 4, 6, 2), **config):
+        a = default_attributes()
+        a.distance_metric = METRICS[metric]
+        a.alignment_scope = 1
+        if metric == "affine2p":
+            a.affine2p_penalties = affine2p_penalties_t(*tuple(penalties))
+        else:
+            a.affine_penalties = affine_penalties_t(*tuple(penalties)[:4])
+        a.heuristic.strategy = 0
+        cfg = wfa_server_config_t()
+        lib().wfa_server_config_default(ctypes.byref(cfg))
+        for k, v in config.items():
+            setattr(cfg, k, v)
+        self.config = cfg
+        self.attr = a
+        self.h = lib().wfa_server_create(ctypes.byref(a), ctypes.byref(cfg))
+        if not self.h:
+            raise RuntimeError("wfa_server_create failed")
+        self._replays = []
+        self._arena = ctypes.create_string_buffer(1 << 24)
+
+    def submit(self, pattern: bytes, text: bytes) -> int:
+        return lib().wfa_server_submit(self.h, bytes(pattern), len(pattern), bytes(text), len(text))
+
+    def replay(self, pairs, arrival_s):
+        """Submit pair i at arrival_s[i] seconds from now (server thread)."""
+        pr = pairs if isinstance(pairs, _Pairs) else _Pairs(pairs)
+        at = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        assert len(at) == pr.n
+        rc = lib().wfa_server_replay(self.h, pr.address, at.ctypes.data, pr.n)
+        self._replays.append((pr, at))
+        return rc
+
+    def collect(self, max_results=4096, timeout_ms=100):
+        res = (wfa_server_result_t * max_results)()
+        n = lib().wfa_server_collect(self.h, ctypes.addressof(res), max_results, ctypes.addressof(self._arena),
+                                     len(self._arena), int(timeout_ms))
+        out = []
+        for i in range(n):
+            r = res[i]
+            ops = ctypes.string_at(ctypes.addressof(self._arena) + r.cigar_offset, r.cigar_length) if r.cigar_length else b""
+            out.append(ServedResult(r, ops))
+        return out
+
+    def pending(self) -> int:
+        return lib().wfa_server_pending(self.h)
+
+    @staticmethod
+    def clock() -> float:
+        return lib().wfa_server_clock()
+
+    def close(self):
+        if self.h:
+            lib().wfa_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def rle(ops: bytes) -> str:
