@@ -9,7 +9,9 @@ kernel + backtrace kernel, CIGARs left in HBM).
 
 Prints one JSON line (rank 0) with pairs/s as `value`, equivalent GCUPS,
 the roofline of the dominant (forward) kernel, the plain one-pair-per-launch
-GPU form and the reference's OpenMP CPU build timed on this host.
+GPU form against the batched form on the same pairs, the 1 kbp / 5% case, the
+serving loop's p50/p99 latency at a fixed Poisson arrival rate, and the
+reference's OpenMP CPU build timed on this host.
 """
 import argparse
 import json
@@ -74,6 +76,154 @@ def cpu_baseline(ps, threads, res=None):
             "gcups": ps.cells / secs / 1e9}
 
 
+def serve_measurement(duration_s, seed, log, lanes=(1, 1, 1)):
+    """Serving: single-pair requests from the mixture of 60% 1 kbp / 5% pairs,
+    35% 1-20 kbp / 1-15% pairs and 5% 100 kbp / 15% pairs (BiWFA), gap-affine
+    (0,4,6,2), score + CIGAR.  The capacity of the request loop is measured
+    closed loop (a burst of requests at once); then requests arrive as a
+    Poisson process at 70% of that rate for duration_s, and the latency of
+    each (submit to result ready) is recorded."""
+    import numpy as np
+    import wfa_amd
+    from wfa_amd import datagen
+
+    cfg = dict(short_max_length=2048, long_min_length=32768, short_lanes=lanes[0], medium_lanes=lanes[1],
+               long_lanes=lanes[2], max_batch_pairs=4096, lane_memory_bytes=24 << 30)
+
+    srv = wfa_amd.Server(**cfg)
+
+    def run(ps, arrivals):
+        t0 = srv.clock()
+        base = srv.submitted if hasattr(srv, "submitted") else 0
+        srv.replay(ps, arrivals)
+        got = {}
+        while len(got) < ps.n:
+            for r in srv.collect(max_results=4096, timeout_ms=500):
+                got[r.id - base] = r
+            if srv.clock() - t0 > 600:
+                raise RuntimeError("serving run did not finish")
+        srv.submitted = base + ps.n
+        res = [got[i] for i in range(ps.n)]
+        if any(r.status != 0 for r in res):
+            raise RuntimeError("a served request failed")
+        return res, t0
+
+    # The burst also warms the lanes (device buffers, record arenas).
+    burst = datagen.generate(4, seed, 1200)
+    res, t0 = run(burst, np.zeros(burst.n))
+    makespan = max(r.finish for r in res) - min(r.submit for r in res)
+    capacity = burst.n / makespan
+    rate = 0.7 * capacity
+    n = max(200, int(rate * duration_s))
+    log("serving: capacity %.1f requests/s (burst of %d), open loop at %.1f requests/s for %d requests"
+        % (capacity, burst.n, rate, n))
+    ps = datagen.generate(4, seed + 1, n)
+    arrivals = np.cumsum(np.random.default_rng(seed).exponential(1.0 / rate, n))
+    res, t0 = run(ps, arrivals)
+    srv.close()
+    lat = np.array([r.latency for r in res]) * 1e3
+    L = np.maximum(ps.plen, ps.tlen)
+    cls = np.where(L >= cfg["long_min_length"], "long", np.where(L <= cfg["short_max_length"], "short", "medium"))
+    span = max(r.finish for r in res) - min(r.submit for r in res)
+    by = {}
+    for c in ("short", "medium", "long"):
+        m = cls == c
+        if m.any():
+            by[c] = {"requests": int(m.sum()), "p50_ms": float(np.percentile(lat[m], 50)),
+                     "p99_ms": float(np.percentile(lat[m], 99))}
+    return {
+        "workload": "Poisson arrivals of single-pair requests: 60% 1 kbp/5% (1-base indels), 35% "
+                    "1-20 kbp log-uniform/1-15% (geometric indels, mean 3), 5% 100 kbp/15% "
+                    "(geometric indels, mean 2, BiWFA); gap-affine (0,4,6,2), score+CIGAR",
+        "requests": int(n), "rate_req_s": rate, "capacity_req_s": capacity,
+        "throughput_req_s": n / span,
+        "p50_ms": float(np.percentile(lat, 50)), "p90_ms": float(np.percentile(lat, 90)),
+        "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
+        "by_class": by,
+        "lanes": {"short": cfg["short_lanes"], "medium": cfg["medium_lanes"], "long": cfg["long_lanes"]},
+    }
+
+
+def batch_rate(al, ps, steps=3):
+    """pairs/s of wfa_device_batch_run over a resident batch (after one warm-up)."""
+    db = al.upload(ps)
+    if db.run() != 0:
+        raise RuntimeError("batch run failed")
+    t = time.perf_counter()
+    for _ in range(steps):
+        if db.run() != 0:
+            raise RuntimeError("batch run failed")
+    dt = (time.perf_counter() - t) / steps
+    st = db.stats()
+    db.free()
+    return ps.n / dt, dt, st
+
+
+LONG_WORKLOADS = {
+    # name: (datagen shape, metric, penalties, memory mode, default pairs, CPU sample pairs, description)
+    "affine2p-1-20kbp": (2, "affine2p", (0, 4, 6, 2, 24, 1), "high", 3000, 32,
+                         "dual-cost gap-affine (0,4,6,2,24,1) end-to-end pairs, L log-uniform in [1,20] kbp, "
+                         "error ~U[1%,15%], geometric indels (mean 3), score+CIGAR, memory-high"),
+    "biwfa-100kbp": (3, "affine", (0, 4, 6, 2), "ultralow", 256, 8,
+                     "gap-affine (0,4,6,2) end-to-end pairs of 100 kbp, 15% errors, geometric indels "
+                     "(mean 2), score+CIGAR, BiWFA (memory ultralow)"),
+}
+
+
+def long_workload(name, n, no_cpu):
+    """One run over the first n pairs of a long-pair workload (a run takes
+    seconds, so one untimed warm-up on a small slice only), with the
+    reference timed on a small seeded sample of the same pairs and the GPU
+    results of that sample compared with its outputs."""
+    import numpy as np
+    import wfa_amd
+    from wfa_amd import datagen
+    shape, metric, pen, mem, n_def, n_cpu, desc = LONG_WORKLOADS[name]
+    n = n or n_def
+    ps = datagen.generate(shape, 7, n)
+    al = wfa_amd.Aligner(metric=metric, penalties=pen, memory=mem)
+    al.align_batch(ps.subset(list(range(min(4, n)))))
+    db = al.upload(ps)
+    t = time.perf_counter()
+    rc = db.run()
+    dt = time.perf_counter() - t
+    st = db.stats()
+    if rc != 0:
+        raise RuntimeError("run returned %d" % rc)
+    res = db.download()
+    log("%s: %d pairs in %.2f s (%.1f pairs/s, %.3e diagonal-steps/s)" % (name, n, dt, n / dt, st.diagonal_steps / dt))
+    cpu = None
+    if not no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        if O.ref_available():
+            idx = list(range(min(n_cpu, n)))
+            sub = O.Pairs([ps.pattern(i) for i in idx], [ps.text(i) for i in idx])
+            conf = O.Conf(metric, pen, mem)
+            threads = min(16, os.cpu_count() or 1, len(idx))
+            secs, _ = O.ref_time(conf, sub, threads)
+            ref = O.ref_align(conf, sub, threads)
+            bad = sum(1 for j, i in enumerate(idx) if res.status[i] != ref.status[j] or res.score[i] != ref.score[j]
+                      or res.ops(i) != ref.ops(j))
+            if bad:
+                raise RuntimeError("GPU results differ from the reference build on %d pairs" % bad)
+            cells = float(sum(int(ps.plen[i]) * int(ps.tlen[i]) for i in idx))
+            cpu = {"value": len(idx) / secs, "unit": "pairs/s", "cores": threads, "kind": "reference",
+                   "sample": "first %d pairs, wavefront_align of the reference built -O3 with OpenMP, one "
+                             "aligner per thread; %.1f s wall; all %d identical to the GPU results"
+                             % (len(idx), secs, len(idx)), "gcups": cells / secs / 1e9}
+            log("reference: %.2f pairs/s on %d threads; %d pairs identical" % (len(idx) / secs, threads, len(idx)))
+    return {"metric": "pairs/s & equiv GCUPS, batched, exact CIGAR", "value": n / dt, "unit": "pairs/s",
+            "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": dt * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+            "data": "synthetic (seeded SplitMix64 generator)",
+            "config": {"workload": "%s: first %d pairs, %s" % (name, n, desc), "pairs": n,
+                       "sum_nm_cells": ps.cells, "diagonal_steps": int(st.diagonal_steps)},
+            "gcups": ps.cells / dt / 1e9, "diagonal_steps_per_s": st.diagonal_steps / dt,
+            "forward_ms": st.forward_ms, "backtrace_ms": st.backtrace_ms, "launches": st.forward_launches,
+            "cpu_baseline": cpu}
+
+
 def copy_calibration(mb):
     """A plain device copy of known size: reads and writes mb MiB once each.
     rocprofv3 FETCH_SIZE / WRITE_SIZE of this kernel calibrate the counters."""
@@ -95,12 +245,26 @@ def main():
     ap.add_argument("--pairs", type=int, default=1000000)
     ap.add_argument("--plain-sample", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="open-loop serving run length (0: skip)")
+    ap.add_argument("--serve-lanes", default="1,1,1", help="short,medium,long lanes of the request loop")
+    ap.add_argument("--extras", type=int, default=1,
+                    help="0: only the short-read batch (plain form, 1 kbp case and serving skipped)")
+    ap.add_argument("--workload", choices=("short-reads",) + tuple(LONG_WORKLOADS), default="short-reads",
+                    help="short-reads: the headline batch; the others print their own line")
+    ap.add_argument("--long-pairs", type=int, default=0, help="pairs of a long workload (0: its default)")
     ap.add_argument("--mode", choices=("align", "copy"), default="align",
                     help="copy: one device copy of --copy-mb MiB (HBM counter calibration)")
     ap.add_argument("--copy-mb", type=int, default=1024)
     args = ap.parse_args()
     if args.mode == "copy":
         copy_calibration(args.copy_mb)
+        return
+    if args.workload != "short-reads":
+        if args.gpus != 1:
+            print("bench.py measures one GPU; run with --gpus 1.", file=sys.stderr)
+            sys.exit(2)
+        print(json.dumps(long_workload(args.workload, args.long_pairs, args.no_cpu)))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the pairs are independent, so N GPUs would each run "
@@ -146,16 +310,40 @@ def main():
     if (res.status != 0).any():
         raise RuntimeError("some pairs did not complete")
 
-    # Plain form: the same kernels, one pair per launch (wavefront_align).
-    al_plain = wfa_amd.Aligner(metric="affine", penalties=(0, 4, 6, 2))
-    k = min(args.plain_sample, ps.n)
+    # Plain form: the same kernels, one pair per launch (wavefront_align),
+    # against the batched form on 10^5 mixed-length pairs (the first 10^5
+    # short-read pairs, 100-300 bp), timed on a sample of the plain calls.
+    plain = None
+    k = min(args.plain_sample, ps.n) if args.extras else 0
     if k:
+        mixed = ps.subset(list(range(min(100000, ps.n))))
+        al_plain = wfa_amd.Aligner(metric="affine", penalties=(0, 4, 6, 2))
         al_plain.align(ps.pattern(0), ps.text(0))
-    t = time.perf_counter()
-    for i in range(k):
-        al_plain.align(ps.pattern(i), ps.text(i))
-    plain_s = time.perf_counter() - t
-    plain_pairs_s = k / plain_s if k else None
+        t = time.perf_counter()
+        for i in range(k):
+            al_plain.align(mixed.pattern(i), mixed.text(i))
+        plain_pairs_s = k / (time.perf_counter() - t)
+        mixed_rate, _, _ = batch_rate(al, mixed)
+        plain = {"value": plain_pairs_s, "unit": "pairs/s",
+                 "sample": "first %d of %d mixed-length (100-300 bp) pairs, one wavefront_align call "
+                           "(forward + backtrace launches) per pair" % (k, mixed.n),
+                 "batched_same_pairs": mixed_rate, "batched_over_plain": mixed_rate / plain_pairs_s}
+        log("plain form %.0f pairs/s; batched on the same %d pairs %.3e pairs/s (x%.0f)"
+            % (plain_pairs_s, mixed.n, mixed_rate, mixed_rate / plain_pairs_s))
+
+    # 1 kbp / 5% pairs (10,000, seed 0), the reference's own CPU-runnable case.
+    c0_out = None
+    if args.extras:
+        c0 = datagen.generate(0, 0, 10000)
+        c0_rate, c0_dt, _ = batch_rate(al, c0)
+        log("1 kbp/5%%: %d pairs in %.2f ms (%.3e pairs/s)" % (c0.n, c0_dt * 1e3, c0_rate))
+        c0_out = {"pairs": c0.n, "value": c0_rate, "unit": "pairs/s", "gcups": c0.cells * c0_rate / c0.n / 1e9}
+
+    serve = None
+    if args.extras and args.serve_seconds > 0:
+        serve = serve_measurement(args.serve_seconds, 41, log, tuple(int(x) for x in args.serve_lanes.split(",")))
+        log("serving: p50 %.2f ms, p99 %.2f ms at %.1f requests/s"
+            % (serve["p50_ms"], serve["p99_ms"], serve["rate_req_s"]))
 
     fwd_ms = float(np.mean(fwd))
     launches_per_step = float(np.mean(launches))
@@ -195,9 +383,9 @@ def main():
         "backtrace_ms_per_step": float(np.mean(bt)),
         "forward_launches_per_step": launches_per_step,
         "pcie_inclusive_pairs_per_s": ps.n / (total / args.steps + d2h_s),
-        "plain_gpu": {"value": plain_pairs_s, "unit": "pairs/s", "sample": "first %d pairs, one "
-                      "wavefront_align call (one forward + one backtrace launch) per pair" % k,
-                      "batched_over_plain": (pairs_per_s / plain_pairs_s) if plain_pairs_s else None},
+        "plain_gpu": plain,
+        "kbp1_5pct": c0_out,
+        "serve": serve,
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,

@@ -42,10 +42,13 @@ bool hip_ok(hipError_t e, const char* what) {
 
 bool DevBuf::ensure(size_t bytes) {
   if (bytes <= cap && p) return true;
+  // Geometric growth: hipFree waits for all work on the device, so a buffer
+  // that is used again and again (serving lanes, repeated batches) should
+  // reallocate rarely.
+  const size_t want = std::max<size_t>(std::max<size_t>(bytes, 256), p ? std::min(2 * cap, cap + (size_t(4) << 30)) : 0);
   if (p) (void)hipFree(p);
   p = nullptr;
   cap = 0;
-  const size_t want = std::max<size_t>(bytes, 256);
   if (hipMalloc(&p, want) != hipSuccess) {
     p = nullptr;
     return false;
@@ -70,10 +73,10 @@ void PlanGroup::release() {
 
 bool HostBuf::ensure(size_t bytes) {
   if (bytes <= cap && p) return true;
+  const size_t want = std::max<size_t>(std::max<size_t>(bytes, 4096), p ? 2 * cap : 0);
   if (p) (void)hipHostFree(p);
   p = nullptr;
   cap = 0;
-  const size_t want = std::max<size_t>(bytes, 4096);
   if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
     p = nullptr;
     return false;

@@ -152,8 +152,11 @@ void wfa_server_t::lane_main(int cls) {
 }
 
 extern "C" void wfa_server_config_default(wfa_server_config_t* c) {
-  c->short_lanes = 2;
-  c->medium_lanes = 2;
+  // One lane per class: each lane is a HIP stream, and with HIP's default of
+  // four hardware queues per process more streams share queues (kernels of
+  // streams on one queue run in order).
+  c->short_lanes = 1;
+  c->medium_lanes = 1;
   c->long_lanes = 1;
   c->short_max_length = 2048;
   c->long_min_length = 32768;
