@@ -218,7 +218,8 @@ import sys
 sys.path.insert(0, %r); sys.path.insert(0, %r); sys.path.insert(0, %r)
 import numpy as np, oracle as O, wfa_amd
 from wfa_amd import datagen
-ps = datagen.generate(1, 1001, 1500)
+a, b = datagen.generate(1, 1001, 1500), datagen.generate(0, 1002, 150)
+ps = datagen.PairSet.from_lists(a.patterns() + b.patterns(), a.texts() + b.texts())
 al = wfa_amd.Aligner(metric="affine", penalties=(0, 4, 6, 2))
 db = al.upload(ps)
 assert db.run() == 0
@@ -231,11 +232,11 @@ print("launches", db.stats().forward_launches)
 """
 
 
-@pytest.mark.parametrize("env", [{"WFA_LEAN_WCAP": "24,24"}, {"WFA_LEAN": "0"}],
+@pytest.mark.parametrize("env", [{"WFA_LEAN_WCAP": "224,224,4000"}, {"WFA_LEAN": "0"}],
                          ids=["lean-ring-overflow-retries", "generic-lds-family"])
 def test_kernel_families_vs_oracle(env):
-    """The lean family with a tiny ring (every wide pair overflows and is
-    re-run with a wider ring, then on the HBM family), and the generic LDS
+    """The lean family with its narrowest ring for every pair (the 1 kbp
+    pairs overflow it and are re-run with wider rings), and the generic LDS
     family with the lean one turned off, both match the oracle."""
     import os
     import subprocess

@@ -310,12 +310,12 @@ int small_wcap(const Penalties& pen, int ncomp) {
 }
 
 // Lean family (gap-affine, 2-bit sequences up to kLeanMaxLen): ring width per
-// slot for pairs up to short_len and for longer pairs.  A pair whose
-// wavefront outgrows its ring is re-run with twice the width, up to
-// kLeanMaxWcap, then on the HBM-ring family.  WFA_LEAN=0 turns the family off;
-// WFA_LEAN_WCAP="short,long[,short_len]" overrides the widths.
+// slot for pairs up to short_len and for longer pairs, rounded up to a width
+// with a kernel instance (224, 384, 768).  A pair whose wavefront outgrows its
+// ring is re-run with the next width, then on the HBM-ring family.
+// WFA_LEAN=0 turns the family off; WFA_LEAN_WCAP="short,long[,short_len]"
+// overrides the widths.
 constexpr int kLeanMaxLen = 1024;
-constexpr int kLeanMaxWcap = 1024;
 struct LeanTuning {
   bool enabled = true;
   int wcap_short = 224;
@@ -330,7 +330,7 @@ const LeanTuning& lean_tuning() {
     const char* w = getenv("WFA_LEAN_WCAP");
     int a = 0, b = 0, c = 0;
     const int n = w ? sscanf(w, "%d,%d,%d", &a, &b, &c) : 0;
-    if (n >= 2 && a >= 8 && b >= 8 && a <= kLeanMaxWcap && b <= kLeanMaxWcap) { r.wcap_short = a; r.wcap_long = b; }
+    if (n >= 2 && wfa::lean_width(a) > 0 && wfa::lean_width(b) > 0) { r.wcap_short = a; r.wcap_long = b; }
     if (n == 3 && c > 0) r.short_len = c;
     return r;
   }();
@@ -362,7 +362,7 @@ LaunchShape shape_of(Engine* e, const PlanGroup& g, int two, int bits, const Pen
   int per_cu = 0;
   if (g.family == wfa::kLean) {
     shape.lds_bytes = wfa::lean_lds_bytes(pen, g.wcap, g.lmax_words);
-    per_cu = wfa::lean_max_blocks_per_cu(pen, shape.lds_bytes);
+    per_cu = wfa::lean_max_blocks_per_cu(pen, g.wcap, shape.lds_bytes);
   } else {
     shape.lds_bytes = wfa::forward_lds_bytes(g.family, two ? 5 : 3, pen.nslots, g.wcap, g.lmax_words);
     per_cu = wfa::forward_max_blocks_per_cu(g.family, g.block, two, bits, shape.lds_bytes);
@@ -509,7 +509,7 @@ std::vector<PlanGroup> make_plan(const JobSet& js, const std::vector<int>& ids, 
     PlanGroup g;
     g.family = wfa::kLean;
     g.block = 64;
-    g.wcap = c == 0 ? lt.wcap_short : lt.wcap_long;
+    g.wcap = wfa::lean_width(c == 0 ? lt.wcap_short : lt.wcap_long);
     g.lmax_words = lean_lmax[c];
     g.ids = std::move(lean_ids[c]);
     plan.push_back(std::move(g));
@@ -592,11 +592,11 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
         else if (h_status[i] == wfa::kDevArenaFull) arena_full.push_back(i);
       }
       if (!ring_over.empty()) {
-        if (g.family == wfa::kLean && g.wcap * 2 <= kLeanMaxWcap) {
+        if (g.family == wfa::kLean && wfa::lean_width(g.wcap + 1) > 0) {
           PlanGroup h;
           h.family = wfa::kLean;
           h.block = 64;
-          h.wcap = g.wcap * 2;
+          h.wcap = wfa::lean_width(g.wcap + 1);
           h.lmax_words = g.lmax_words;
           h.ids = std::move(ring_over);
           next.push_back(std::move(h));

@@ -197,8 +197,9 @@ int breakpoint_max_blocks_per_cu(int two_piece, int bits, int block);
 // dynamic LDS, occupancy and launch (ForwardArgs as for launch_forward; the
 // per-pair component arrays must be null and comp_begin = comp_end = M).
 bool lean_supported(const Penalties& pen);
+int lean_width(int wcap);  // smallest ring width with an instance >= wcap, 0 if none
 size_t lean_lds_bytes(const Penalties& pen, int wcap, int lmax_words);
-int lean_max_blocks_per_cu(const Penalties& pen, size_t lds_bytes);
+int lean_max_blocks_per_cu(const Penalties& pen, int wcap, size_t lds_bytes);
 hipError_t launch_forward_lean(const ForwardArgs& a, const LaunchShape& shape, hipStream_t stream);
 
 // Dynamic-LDS bytes and occupancy for a configuration.

@@ -71,43 +71,67 @@ __device__ __forceinline__ int lcp16(const uint32_t* P, int pb, const uint32_t* 
   }
 }
 
-template <int R, int DQX, int DQO, int DQE>
+// The kernel arguments, reread from the kernel-argument segment where a
+// value is needed outside the step loop (the asm makes each pointer opaque,
+// so the compiler reloads instead of keeping every field in scalar
+// registers across the loop).
+using KernArgs = const __attribute__((address_space(4))) ForwardArgs*;
+__device__ __forceinline__ KernArgs args_now() {
+#if defined(__HIP_DEVICE_COMPILE__)
+  KernArgs p = (KernArgs)__builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(p));
+  return p;
+#else
+  return nullptr;  // host pass of the single-source compile: never executed
+#endif
+}
+
+// Ring elements before the packed sequences in LDS (16-byte aligned).
+template <int R, int WCAP>
+constexpr int lean_ring_elems() {
+  return ((2 + R * 3 * WCAP) + 7) / 8 * 8;
+}
+
+template <int R, int DQX, int DQO, int DQE, int WCAP, bool REC>
 __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int NC = 3;
   static_assert(DQX > 0 && DQO > 0 && DQE > 0 && DQX < R && DQO < R && DQE < R, "slot distances");
   const int lane = threadIdx.x;
-  const Penalties pen = a.pen;
-  const int g = pen.g;
-  const int wcap = a.wcap;  // ring elements per slot and component (>= width + 2)
+  const int g = a.pen.g;
+  constexpr int wcap = WCAP;  // ring elements per slot and component (>= width + 2)
 
-  const uint32_t* seqP = reinterpret_cast<const uint32_t*>(smem);
-  const uint32_t* seqT = seqP + 2 * a.lmax_words;
-  uint64_t* seqP64 = reinterpret_cast<uint64_t*>(smem);
+  // LDS: the ring (element 0: NULL cell read by absent sources; slots from
+  // element 2), then both sequences.
+  int16_t* ring = reinterpret_cast<int16_t*>(smem);
+  uint64_t* seqP64 = reinterpret_cast<uint64_t*>(smem + 2 * lean_ring_elems<R, WCAP>());
   uint64_t* seqT64 = seqP64 + a.lmax_words;
-  int16_t* ring = reinterpret_cast<int16_t*>(seqT64 + a.lmax_words);
-  // element 0: NULL cell read by absent sources; slots start at element 2
+  const uint32_t* seqP = reinterpret_cast<const uint32_t*>(seqP64);
+  const uint32_t* seqT = reinterpret_cast<const uint32_t*>(seqT64);
   auto base_of = [&](int slot, int c) { return 2 + (slot * NC + c) * wcap; };
   if (lane == 0) ring[0] = kRingNull;
 
-  unsigned long long chunk_cur = 0, chunk_end = 0;
+  uint32_t chunk_cur = 0, chunk_end = 0;  // record chunk of this wave (16-byte units)
   unsigned long long wg_dsteps = 0;
 
   for (;;) {
     int qv = 0;
-    if (lane == 0) qv = (int)atomicAdd(a.queue, 1u);
+    KernArgs A = args_now();
+    if (lane == 0) qv = (int)atomicAdd(A->queue, 1u);
     const int qi = lane_of(qv, 0);
-    if (qi >= a.n_pairs) break;
-    const int p = a.order ? a.order[qi] : qi;
-    const int plen = a.plen[p];
-    const int tlen = a.tlen[p];
+    if (qi >= A->n_pairs) break;
+    const int p = A->order ? A->order[qi] : qi;
+    const int plen = A->plen[p];
+    const int tlen = A->tlen[p];
     const int kend = tlen - plen;
-    const int s_cap = (pen.x + pen.o1 + pen.e1 + 2) * (plen + tlen + 2);
-    const long long ppos = a.pat_pos[p], tpos = a.txt_pos[p];
+    const int s_cap = (A->pen.x + A->pen.o1 + A->pen.e1 + 2) * (plen + tlen + 2);
+    // One bound per step: the step limit or the (never reached) score cap.
+    const int s_stop = min(A->max_steps, s_cap);
+    const long long ppos = A->pat_pos[p], tpos = A->txt_pos[p];
     const int pb = (int)(ppos & 31), tb = (int)(tpos & 31);
     {
-      const uint64_t* gP = a.words + (ppos >> 5);
-      const uint64_t* gT = a.words + (tpos >> 5);
+      const uint64_t* gP = A->words + (ppos >> 5);
+      const uint64_t* gT = A->words + (tpos >> 5);
       const int nwp = (pb + plen) / 32 + 2, nwt = (tb + tlen) / 32 + 2;
       wave_sync();  // the previous pair is done reading the sequences
       for (int i = lane; i < nwp; i += 64) seqP64[i] = gP[i];
@@ -139,12 +163,13 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
       wave_sync();
     }
 
-    while (status == 1) {
+    if (status != 1) goto pair_done;
+    for (;;) {
 #pragma unroll
       for (int j = 1; j <= R; ++j) {
         const int ph = j % R;  // slot of this step (compile-time after unrolling)
         s += g;
-        if (s >= a.max_steps) { status = kDevMaxSteps; break; }
+        if (s >= s_stop) { status = s_stop == s_cap ? kDevBadTrace : kDevMaxSteps; goto pair_done; }
         const int rx = (ph - DQX + R) % R, ro = (ph - DQO + R) % R, re = (ph - DQE + R) % R;
         const LeanSrc mx = lean_src(s_org[rx], s_lo[rx][kM], s_hi[rx][kM], base_of(rx, kM));
         const LeanSrc mo = lean_src(s_org[ro], s_lo[ro][kM], s_hi[ro][kM], base_of(ro, kM));
@@ -154,36 +179,35 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
         if (!(vx || vo || vi || vd)) {  // null step (wavefront_compute_affine.c:236-243)
 #pragma unroll
           for (int c = 0; c < NC; ++c) { s_lo[ph][c] = 1; s_hi[ph][c] = -1; }
-          if (s >= s_cap) { status = kDevBadTrace; break; }
           continue;
         }
         // wavefront_compute_limits_input (wavefront_compute.c:40-86)
         const int lo = min(min(mx.llo, mo.llo - 1), min(i1.llo + 1, d1.llo - 1));
         const int hi = max(max(mx.lhi, mo.lhi + 1), max(i1.lhi + 1, d1.lhi - 1));
         const int width = hi - lo + 1;
-        if (width + 2 > wcap) { status = kDevRingOverflow; break; }
+        if (width + 2 > wcap) { status = kDevRingOverflow; goto pair_done; }
         dsteps += width;
         const bool has_i1 = vo || vi;
         const bool has_d1 = vo || vd;
 
         uint8_t* rec = nullptr;
-        if (a.records) {
-          const unsigned long long units = rec_units_for(width);
+        if constexpr (REC) {
+          const uint32_t units = (uint32_t)rec_units_for(width);
           if (chunk_cur + units > chunk_end) {
-            const unsigned long long want = units > kChunkUnits ? units : kChunkUnits;
+            const uint32_t want = units > kChunkUnits ? units : kChunkUnits;
             unsigned long long b0 = 0;
-            if (lane == 0) b0 = atomicAdd(a.rec_top, want);
-            const int blo = lane_of((int)(unsigned)(b0 & 0xFFFFFFFFull), 0);
-            const int bhi = lane_of((int)(unsigned)(b0 >> 32), 0);
-            chunk_cur = ((unsigned long long)(unsigned)bhi << 32) | (unsigned)blo;
-            chunk_end = chunk_cur + want;
-            if (chunk_end > a.rec_units) {
+            if (lane == 0) b0 = atomicAdd(a.rec_top, (unsigned long long)want);
+            const unsigned blo = (unsigned)lane_of((int)(unsigned)(b0 & 0xFFFFFFFFull), 0);
+            const unsigned bhi = (unsigned)lane_of((int)(unsigned)(b0 >> 32), 0);
+            if (bhi != 0 || (unsigned long long)blo + want > a.rec_units) {
               chunk_cur = chunk_end = 0;
               status = kDevArenaFull;
-              break;
+              goto pair_done;
             }
+            chunk_cur = blo;
+            chunk_end = blo + want;
           }
-          const uint32_t blk = (uint32_t)chunk_cur;
+          const uint32_t blk = chunk_cur;
           chunk_cur += units;
           if (lane == 0) a.rec_arena[blk] = make_uint4((unsigned)s, (unsigned)lo, (unsigned)hi, prev_blk);
           prev_blk = blk;
@@ -195,10 +219,10 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
         int16_t* oM = ring + base_of(ph, kM) - org;
         int16_t* oI = ring + base_of(ph, kI1) - org;
         int16_t* oD = ring + base_of(ph, kD1) - org;
-        WaveRange rM, rI, rD;
-        rM.init(); rI.init(); rD.init();
-        bool fin = false;  // termination (wavefront_termination.c:37-114)
-        for (int kb = lo; kb <= hi; kb += 64) {
+        // One pass over 64 diagonals starting at kb: compute, record, extend,
+        // store; returns the validity ballots and whether k_end reached the
+        // end of the text (termination, wavefront_termination.c:37-114).
+        auto pass = [&](int kb, unsigned long long& bM, unsigned long long& bI, unsigned long long& bD) -> bool {
           const int k = kb + lane;
           const bool act = k <= hi;
           // Reads are clamped, so every lane may read; only active lanes store.
@@ -214,7 +238,7 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
           if (del1 > best) { best = del1; src = kSrcD1; }
           if (ins1 > best) { best = ins1; src = kSrcI1; }
           const uint32_t bits = (ei1 >= om_l ? kRecI1Ext : 0u) | (ed1 >= om_r ? kRecD1Ext : 0u);
-          if (rec && act) rec[k - lo] = (uint8_t)(src | bits);
+          if (REC && act) rec[(unsigned)(k - lo)] = (uint8_t)(src | bits);
           // null-after-max, then the extension of the valid M cells: one
           // 16-symbol window for every lane, more only for lanes whose run
           // fills it (uniform branch).
@@ -239,7 +263,6 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
               }
             }
           }
-          const bool vM = okM;
           const bool vI = act && has_i1 && in_bounds(ins1, k, plen, tlen);
           const bool vD = act && has_d1 && in_bounds(del1, k, plen, tlen);
           if (act) {
@@ -247,15 +270,25 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
             if (has_i1) oI[k] = to_ring(ins1, ring);
             if (has_d1) oD[k] = to_ring(del1, ring);
           }
-          fin = fin || __ballot(okM && k == kend && h >= tlen);
-          rM.add(__ballot(vM), kb);
-          rI.add(__ballot(vI), kb);
-          rD.add(__ballot(vD), kb);
+          bM = __ballot(okM);
+          bI = __ballot(vI);
+          bD = __ballot(vD);
+          return __ballot(okM && k == kend && h >= tlen) != 0;
+        };
+        // Trimmed ranges (wavefront_compute.c:579-613) from the ballots;
+        // absent or empty = (1, -1).  Most steps are at most 64 wide: the
+        // first pass is straight-line code, further passes loop.
+        unsigned long long bM, bI, bD;
+        bool fin = pass(lo, bM, bI, bD);
+        int mlo = bM ? lo + (int)__builtin_ctzll(bM) : 1, mhi = bM ? lo + 63 - (int)__builtin_clzll(bM) : -1;
+        int ilo = bI ? lo + (int)__builtin_ctzll(bI) : 1, ihi = bI ? lo + 63 - (int)__builtin_clzll(bI) : -1;
+        int dlo = bD ? lo + (int)__builtin_ctzll(bD) : 1, dhi = bD ? lo + 63 - (int)__builtin_clzll(bD) : -1;
+        for (int kb = lo + 64; kb <= hi; kb += 64) {
+          fin = pass(kb, bM, bI, bD) || fin;
+          if (bM) { if (mlo > mhi) mlo = kb + (int)__builtin_ctzll(bM); mhi = kb + 63 - (int)__builtin_clzll(bM); }
+          if (bI) { if (ilo > ihi) ilo = kb + (int)__builtin_ctzll(bI); ihi = kb + 63 - (int)__builtin_clzll(bI); }
+          if (bD) { if (dlo > dhi) dlo = kb + (int)__builtin_ctzll(bD); dhi = kb + 63 - (int)__builtin_clzll(bD); }
         }
-        // Trimmed ranges (wavefront_compute.c:579-613); absent or empty = (1, -1).
-        const int mlo = rM.null() ? 1 : rM.first, mhi = rM.null() ? -1 : rM.last;
-        const int ilo = (!has_i1 || rI.null()) ? 1 : rI.first, ihi = (!has_i1 || rI.null()) ? -1 : rI.last;
-        const int dlo = (!has_d1 || rD.null()) ? 1 : rD.first, dhi = (!has_d1 || rD.null()) ? -1 : rD.last;
         // NULL just outside each trimmed range: lanes 0..5 write one cell each.
         if (lane < 6) {
           const int c = lane >> 1;
@@ -273,47 +306,69 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
         wave_sync();  // ring writes of this step before the next step's reads
         // M[k_end] reached the end of the text: a valid cell at k_end lies in
         // the trimmed range, so the lane that computed it decides.
-        if (fin) { status = kDevCompleted; break; }
-        if (s >= s_cap) { status = kDevBadTrace; break; }
+        if (fin) { status = kDevCompleted; goto pair_done; }
       }
     }
-    if (lane == 0) {
-      a.out_status[p] = status;
-      a.out_score[p] = s;
-      a.out_last_block[p] = prev_blk;
-      a.out_dsteps[p] = dsteps;
-      if (status == kDevRingOverflow || status == kDevArenaFull) atomicAdd(a.retry_count, 1u);
-      else if (status != kDevCompleted) atomicAdd(a.fail_count, 1u);
+  pair_done:
+    {
+      KernArgs B = args_now();
+      if (lane == 0) {
+        B->out_status[p] = status;
+        B->out_score[p] = s;
+        B->out_last_block[p] = prev_blk;
+        B->out_dsteps[p] = dsteps;
+        if (status == kDevRingOverflow || status == kDevArenaFull) atomicAdd(B->retry_count, 1u);
+        else if (status != kDevCompleted) atomicAdd(B->fail_count, 1u);
+      }
     }
     if (status == kDevCompleted || status == kDevMaxSteps) wg_dsteps += (unsigned long long)dsteps;
   }
-  if (lane == 0 && wg_dsteps) atomicAdd(a.dsteps_total, wg_dsteps);
+  if (lane == 0 && wg_dsteps) atomicAdd(args_now()->dsteps_total, wg_dsteps);
 }
 
+// Ring widths with an instance (diagonals per slot and component).
+constexpr int kLeanWidths[] = {224, 384, 768};
+
 template <int R, int DQX, int DQO, int DQE>
-const void* lean_fn() {
-  return reinterpret_cast<const void*>(&wfa_forward_lean_kernel<R, DQX, DQO, DQE>);
+const void* lean_fn(int wcap, bool rec) {
+#define WFA_LEAN_W(W)                                                                   \
+  if (wcap == W) return rec ? reinterpret_cast<const void*>(&wfa_forward_lean_kernel<R, DQX, DQO, DQE, W, true>) \
+                            : reinterpret_cast<const void*>(&wfa_forward_lean_kernel<R, DQX, DQO, DQE, W, false>);
+  WFA_LEAN_W(224) WFA_LEAN_W(384) WFA_LEAN_W(768)
+#undef WFA_LEAN_W
+  return nullptr;
 }
 
 // Slot structures with an instance: (R, dq_x, dq_o, dq_e).
-const void* select_lean(const Penalties& pen) {
-  if (pen.g <= 0) return nullptr;
+bool lean_structure(const Penalties& pen) {
+  if (pen.g <= 0) return false;
   const int dx = pen.x / pen.g, dq = (pen.o1 + pen.e1) / pen.g, de = pen.e1 / pen.g, R = pen.nslots;
   // gap-affine (4, 6, 2) and every multiple of (2, 3, 1)
-  if (R == 5 && dx == 2 && dq == 4 && de == 1) return lean_fn<5, 2, 4, 1>();
-  return nullptr;
+  return R == 5 && dx == 2 && dq == 4 && de == 1;
+}
+
+const void* select_lean(const Penalties& pen, int wcap, bool rec) {
+  if (!lean_structure(pen)) return nullptr;
+  return lean_fn<5, 2, 4, 1>(wcap, rec);
 }
 
 }  // namespace
 
-bool lean_supported(const Penalties& pen) { return select_lean(pen) != nullptr; }
+bool lean_supported(const Penalties& pen) { return lean_structure(pen); }
 
-size_t lean_lds_bytes(const Penalties& pen, int wcap, int lmax_words) {
-  return (size_t)2 * lmax_words * sizeof(uint64_t) + (size_t)(2 + pen.nslots * 3 * wcap) * sizeof(int16_t);
+int lean_width(int wcap) {
+  for (int w : kLeanWidths)
+    if (w >= wcap) return w;
+  return 0;
 }
 
-int lean_max_blocks_per_cu(const Penalties& pen, size_t lds_bytes) {
-  const void* f = select_lean(pen);
+size_t lean_lds_bytes(const Penalties& pen, int wcap, int lmax_words) {
+  if (!lean_structure(pen)) return 0;
+  return (size_t)2 * (((2 + pen.nslots * 3 * wcap) + 7) / 8 * 8) + (size_t)2 * lmax_words * sizeof(uint64_t);
+}
+
+int lean_max_blocks_per_cu(const Penalties& pen, int wcap, size_t lds_bytes) {
+  const void* f = select_lean(pen, wcap, true);
   if (!f) return 0;
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 64, lds_bytes) != hipSuccess) return 0;
@@ -321,7 +376,7 @@ int lean_max_blocks_per_cu(const Penalties& pen, size_t lds_bytes) {
 }
 
 hipError_t launch_forward_lean(const ForwardArgs& a, const LaunchShape& shape, hipStream_t stream) {
-  const void* f = select_lean(a.pen);
+  const void* f = select_lean(a.pen, a.wcap, a.records != 0);
   if (!f || shape.block != 64) return hipErrorInvalidValue;
   if (shape.lds_bytes > 48 * 1024) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shape.lds_bytes);
