@@ -28,26 +28,52 @@ namespace {
 constexpr int16_t kRingNull = -16384;  // to_ring(kOffsetNull)
 
 // A compute input: its contribution to the output limits (1, -1 for the null
-// wavefront, wavefront_compute.c:40-86) and its read window.
+// wavefront, wavefront_compute.c:40-86) and its read window.  M inputs clamp
+// k into [lo-1, hi+1] of their computed range: M cells outside the trimmed
+// range hold NULL, and the two cells just outside the computed range are
+// written NULL before each step.  I1/D1 inputs hold raw values outside their
+// trimmed range, so a read checks the range (wavefront_compute_init_ends,
+// wavefront_compute.c:498-575, gives the reference the same NULLs).
 struct LeanSrc {
   int llo, lhi;   // limits contribution; valid iff llo <= lhi
   int clo, chi;   // reads clamp k into [clo, chi]
   int sb;         // ring element of diagonal 0
 };
+constexpr int kFarOff = 1 << 20;  // clamp bounds of an absent gap input (reads land on the NULL cell)
 
-__device__ __forceinline__ LeanSrc lean_src(int org, int lo, int hi, int base) {
+// M input of a slot with computed range [org+1, top-1] and trimmed [lo, hi].
+__device__ __forceinline__ LeanSrc lean_src_m(int org, int top, int lo, int hi, int base) {
   LeanSrc r;
   const bool v = lo <= hi;
   r.llo = v ? lo : 1;
   r.lhi = v ? hi : -1;
-  r.clo = v ? lo - 1 : 0;
-  r.chi = v ? hi + 1 : 0;
+  r.clo = v ? org : 0;
+  r.chi = v ? top : 0;
   r.sb = v ? base - org : 0;  // element 0 of the ring is a NULL cell
+  return r;
+}
+
+// I1 or D1 input with trimmed range [lo, hi].
+__device__ __forceinline__ LeanSrc lean_src_g(int org, int lo, int hi, int base) {
+  LeanSrc r;
+  const bool v = lo <= hi;
+  r.llo = v ? lo : 1;
+  r.lhi = v ? hi : -1;
+  r.clo = v ? lo : kFarOff;
+  r.chi = v ? hi : -kFarOff;
+  r.sb = v ? base - org : kFarOff;  // absent: sb + clamp = 0, the NULL cell
   return r;
 }
 
 __device__ __forceinline__ int lean_rd(const int16_t* ring, const LeanSrc& s, int k) {
   return (int)ring[s.sb + min(max(k, s.clo), s.chi)];
+}
+
+// A gap input read: NULL outside its trimmed range.
+__device__ __forceinline__ int lean_rd_g(const int16_t* ring, const LeanSrc& s, int k) {
+  const int c = min(max(k, s.clo), s.chi);
+  const int v = (int)ring[s.sb + c];
+  return c == k ? v : (int)kRingNull;
 }
 
 // 16 symbols of a 2-bit packed sequence starting at symbol pos (dword view).
@@ -140,10 +166,11 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
     }
 
     // Ring metadata (uniform): origin and trimmed ranges per slot.
-    int s_org[R], s_lo[R][NC], s_hi[R][NC];
+    int s_org[R], s_top[R], s_lo[R][NC], s_hi[R][NC];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       s_org[r] = 0;
+      s_top[r] = 0;
 #pragma unroll
       for (int c = 0; c < NC; ++c) { s_lo[r][c] = 1; s_hi[r][c] = -1; }
     }
@@ -155,6 +182,7 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
       // Score 0: M holds offset 0 at k = 0 (wavefront_aligner.c:252-380), extended.
       const int v0 = uniform(lcp16(seqP, pb, seqT, tb, plen, tlen, 0, 0));
       s_org[0] = -1;
+      s_top[0] = 1;
       s_lo[0][kM] = 0;
       s_hi[0][kM] = 0;
       const int b = base_of(0, kM);
@@ -171,10 +199,10 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
         s += g;
         if (s >= s_stop) { status = s_stop == s_cap ? kDevBadTrace : kDevMaxSteps; goto pair_done; }
         const int rx = (ph - DQX + R) % R, ro = (ph - DQO + R) % R, re = (ph - DQE + R) % R;
-        const LeanSrc mx = lean_src(s_org[rx], s_lo[rx][kM], s_hi[rx][kM], base_of(rx, kM));
-        const LeanSrc mo = lean_src(s_org[ro], s_lo[ro][kM], s_hi[ro][kM], base_of(ro, kM));
-        const LeanSrc i1 = lean_src(s_org[re], s_lo[re][kI1], s_hi[re][kI1], base_of(re, kI1));
-        const LeanSrc d1 = lean_src(s_org[re], s_lo[re][kD1], s_hi[re][kD1], base_of(re, kD1));
+        const LeanSrc mx = lean_src_m(s_org[rx], s_top[rx], s_lo[rx][kM], s_hi[rx][kM], base_of(rx, kM));
+        const LeanSrc mo = lean_src_m(s_org[ro], s_top[ro], s_lo[ro][kM], s_hi[ro][kM], base_of(ro, kM));
+        const LeanSrc i1 = lean_src_g(s_org[re], s_lo[re][kI1], s_hi[re][kI1], base_of(re, kI1));
+        const LeanSrc d1 = lean_src_g(s_org[re], s_lo[re][kD1], s_hi[re][kD1], base_of(re, kD1));
         const bool vx = mx.llo <= mx.lhi, vo = mo.llo <= mo.lhi, vi = i1.llo <= i1.lhi, vd = d1.llo <= d1.lhi;
         if (!(vx || vo || vi || vd)) {  // null step (wavefront_compute_affine.c:236-243)
 #pragma unroll
@@ -219,6 +247,8 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
         int16_t* oM = ring + base_of(ph, kM) - org;
         int16_t* oI = ring + base_of(ph, kI1) - org;
         int16_t* oD = ring + base_of(ph, kD1) - org;
+        // NULL in the two M cells just outside the computed range.
+        if (lane < 2) oM[lane ? hi + 1 : lo - 1] = kRingNull;
         // One pass over 64 diagonals starting at kb: compute, record, extend,
         // store; returns the validity ballots and whether k_end reached the
         // end of the text (termination, wavefront_termination.c:37-114).
@@ -228,8 +258,8 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
           // Reads are clamped, so every lane may read; only active lanes store.
           const int om_l = lean_rd(ring, mo, k - 1);
           const int om_r = lean_rd(ring, mo, k + 1);
-          const int ei1 = lean_rd(ring, i1, k - 1);
-          const int ed1 = lean_rd(ring, d1, k + 1);
+          const int ei1 = lean_rd_g(ring, i1, k - 1);
+          const int ed1 = lean_rd_g(ring, d1, k + 1);
           const int ins1 = max(om_l, ei1) + 1;
           const int del1 = max(om_r, ed1);
           int best = lean_rd(ring, mx, k) + 1;
@@ -289,17 +319,8 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
           if (bI) { if (ilo > ihi) ilo = kb + (int)__builtin_ctzll(bI); ihi = kb + 63 - (int)__builtin_clzll(bI); }
           if (bD) { if (dlo > dhi) dlo = kb + (int)__builtin_ctzll(bD); dhi = kb + 63 - (int)__builtin_clzll(bD); }
         }
-        // NULL just outside each trimmed range: lanes 0..5 write one cell each.
-        if (lane < 6) {
-          const int c = lane >> 1;
-          const int clo = c == 0 ? mlo : (c == 1 ? ilo : dlo);
-          const int chi = c == 0 ? mhi : (c == 1 ? ihi : dhi);
-          if (clo <= chi) {
-            int16_t* o = c == 0 ? oM : (c == 1 ? oI : oD);
-            o[(lane & 1) ? chi + 1 : clo - 1] = kRingNull;
-          }
-        }
         s_org[ph] = org;
+        s_top[ph] = hi + 1;
         s_lo[ph][kM] = mlo; s_hi[ph][kM] = mhi;
         s_lo[ph][kI1] = ilo; s_hi[ph][kI1] = ihi;
         s_lo[ph][kD1] = dlo; s_hi[ph][kD1] = dhi;
