@@ -36,15 +36,15 @@ MEMORY_MODES = {"high": 0, "med": 1, "low": 2, "ultralow": 3}
 class Conf:
     """Aligner configuration (a subset of wavefront_aligner_attr_t)."""
 
-    metric: str = "affine"            # "affine" | "affine2p"
-    penalties: tuple = (0, 4, 6, 2)   # (M,X,O,E) or (M,X,O1,E1,O2,E2)
+    metric: str = "affine"            # "indel" | "edit" | "linear" | "affine" | "affine2p"
+    penalties: tuple = (0, 4, 6, 2)   # (M,X,O,E), (M,X,O1,E1,O2,E2), linear (M,X,I); unused for indel/edit
     memory: str = "high"
     score_only: bool = False
     max_steps: int = 0                # 0 = unlimited
 
     @property
     def metric_id(self) -> int:
-        return 4 if self.metric == "affine2p" else 3
+        return {"indel": 0, "edit": 1, "linear": 2, "affine": 3, "affine2p": 4}[self.metric]
 
     @property
     def pen6(self) -> tuple:

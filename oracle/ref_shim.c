@@ -27,7 +27,16 @@ static void ref_shim_attributes(
   attr->alignment_scope = score_only ? compute_score : compute_alignment;
   attr->memory_mode = (wavefront_memory_t)memory_mode;
   if (max_steps > 0) attr->system.max_alignment_steps = max_steps;
-  if (metric == 4) {
+  if (metric == 0) {
+    attr->distance_metric = indel;
+  } else if (metric == 1) {
+    attr->distance_metric = edit;
+  } else if (metric == 2) {
+    attr->distance_metric = gap_linear;
+    attr->linear_penalties.match = pen[0];
+    attr->linear_penalties.mismatch = pen[1];
+    attr->linear_penalties.indel = pen[2];
+  } else if (metric == 4) {
     attr->distance_metric = gap_affine_2p;
     attr->affine2p_penalties.match = pen[0];
     attr->affine2p_penalties.mismatch = pen[1];
@@ -90,8 +99,9 @@ double ref_align_batch(
     ops_len[i] = (len > 0) ? len : 0;
     if (len > 0) {
       memcpy(ops_out+ops_off[i],cigar->operations+cigar->begin_offset,len);
-      alg_score[i] = (metric == 4) ?
-          cigar_score_gap_affine2p(cigar,&attr.affine2p_penalties) :
+      alg_score[i] = (metric <= 1) ? cigar_score_edit(cigar) :
+          (metric == 2) ? cigar_score_gap_linear(cigar,&attr.linear_penalties) :
+          (metric == 4) ? cigar_score_gap_affine2p(cigar,&attr.affine2p_penalties) :
           cigar_score_gap_affine(cigar,&attr.affine_penalties);
     } else {
       alg_score[i] = (score_only) ? cigar->score : -1;

@@ -30,6 +30,9 @@
 #include <limits.h>
 #include "wfa_oracle.h"
 
+#define M_INDEL 0   /* distance_metric_t, wavefront_penalties.h:41-47 */
+#define M_EDIT 1
+#define M_LINEAR 2
 #define ONULL (INT32_MIN/2)                 /* WAVEFRONT_OFFSET_NULL, wavefront_offset.h:46 */
 
 /* Status codes, wfa.h:45-55 */
@@ -172,6 +175,25 @@ static inline int32_t wf_rd(const owf_t* w, int k) {
 
 static void aln_set_penalties(oaln_t* a, const oracle_conf_t* conf) {
   a->metric = conf->metric;
+  a->max_steps = (conf->max_steps > 0) ? conf->max_steps : INT_MAX;
+  if (a->metric <= M_LINEAR) {
+    /* wavefront_penalties_set_{indel,edit,linear}, wavefront_penalties.c:39-93;
+       scopes from wavefront_components.c:45-80 */
+    a->match = 0;
+    a->e1 = a->o2 = a->e2 = -1;
+    if (a->metric == M_INDEL) { a->x = -1; a->o1 = 1; a->scope = 2; return; }
+    if (a->metric == M_EDIT) { a->x = 1; a->o1 = 1; a->scope = 2; return; }
+    if (conf->match < 0) {  /* Eizenga's transform */
+      a->match = conf->match;
+      a->x = 2 * conf->mismatch - 2 * conf->match;
+      a->o1 = 2 * conf->gap_opening1 - conf->match;
+    } else {
+      a->x = conf->mismatch;
+      a->o1 = conf->gap_opening1;
+    }
+    a->scope = imax(a->x, a->o1) + 1;
+    return;
+  }
   if (conf->match < 0) { /* Eizenga's transform, wavefront_penalties.c:114-119,152-159 */
     a->match = conf->match;
     a->x = 2 * conf->mismatch - 2 * conf->match;
@@ -194,6 +216,7 @@ static void aln_set_penalties(oaln_t* a, const oracle_conf_t* conf) {
 
 /* wavefront_compute_classic_score, wavefront_compute.c:108-120 */
 static int classic_score(const oaln_t* a, int plen, int tlen, int s) {
+  if (a->metric <= M_EDIT) return s;
   const int swg_match = -a->match;
   if (swg_match == 0) return -s;
   return (swg_match * (plen + tlen) - s) / 2;
@@ -220,9 +243,93 @@ static void trim_ends(const oaln_t* a, owf_t* w) {
 #define WLO(w) ((w) ? (w)->lo : 1)
 #define WHI(w) ((w) ? (w)->hi : -1)
 
+/* wf_compute_edit_{best,worst}_score, wavefront_compute_edit.c:199-218
+   (plain int arithmetic, as there: a NULL offset cancels out of best) */
+static int edit_best(const oaln_t* a, int k, int32_t off) {
+  const int left_v = a->plen - (off - k), left_h = a->tlen - off;
+  return (left_v >= left_h) ? left_v - left_h : left_h - left_v;
+}
+static int edit_worst(const oaln_t* a, int k, int32_t off) {
+  return imax(a->plen - (off - k), a->tlen - off);
+}
+
+/* wavefront_compute_edit_exact_prune, wavefront_compute_edit.c:219-278 */
+static void edit_exact_prune(const oaln_t* a, owf_t* w) {
+  const int lo = w->lo, hi = w->hi;
+  if (hi - lo + 1 < 1000) return;
+  const int sample_k = lo + (hi - lo) / 2;
+  const int32_t sample = wf_raw(w, sample_k);
+  if (sample < 0) return;
+  const int smax_sample = edit_worst(a, sample_k, sample);
+  const int smin_lo = edit_best(a, lo, wf_raw(w, lo));
+  const int smin_hi = edit_best(a, hi, wf_raw(w, hi));
+  if (smin_lo <= smax_sample && smin_hi <= smax_sample) return;
+  int min_worst = INT_MAX, k;
+  for (k = lo; k <= hi; ++k) {
+    const int32_t off = wf_raw(w, k);
+    if (off < 0) continue;
+    const int sw = edit_worst(a, k, off);
+    if (sw < min_worst) min_worst = sw;
+  }
+  int lo_r = lo;
+  for (k = lo; k <= hi; ++k) {
+    if (edit_best(a, k, wf_raw(w, k)) <= min_worst) break;
+    ++lo_r;
+  }
+  w->lo = lo_r;
+  int hi_r = hi;
+  for (k = hi; k > lo_r; --k) {
+    if (edit_best(a, k, wf_raw(w, k)) <= min_worst) break;
+    --hi_r;
+  }
+  w->hi = hi_r;
+}
+
+/* wavefront_compute_edit (compute_edit.c:329-373: indel and edit, from the
+   previous wavefront only) and wavefront_compute_linear (compute_linear.c:
+   166-195); kernels compute_edit.c:44-102, compute_linear.c:44-76 */
+static void compute_linear(oaln_t* a, int s) {
+  aln_grow(a, slot_of(a, s) + 1);
+  owf_t* out = wf_claim(a, C_M, s);
+  int lo, hi, k;
+  if (a->metric <= M_EDIT) {
+    owf_t* prev = wf_at(a, C_M, s - 1);
+    if (!prev) return;
+    lo = prev->lo - 1;
+    hi = prev->hi + 1;
+    wf_alloc(out, lo, hi);
+    for (k = lo; k <= hi; ++k) {
+      const int32_t ins = wf_rd(prev, k - 1), del = wf_rd(prev, k + 1);
+      int32_t m = (a->metric == M_INDEL) ? vmax(del, ins + 1) : vmax(del, vmax(ins, wf_rd(prev, k)) + 1);
+      if ((uint32_t)m > (uint32_t)a->tlen) m = ONULL;
+      if ((uint32_t)(m - k) > (uint32_t)a->plen) m = ONULL;
+      out->v[k - lo] = m;
+    }
+    a->dsteps += hi - lo + 1;
+    trim_ends(a, out);
+    if (a->metric == M_EDIT) edit_exact_prune(a, out);
+    return;
+  }
+  owf_t* mx = wf_src(a, C_M, s - a->x);
+  owf_t* mo = wf_src(a, C_M, s - a->o1);
+  if (!mx && !mo) return; /* null step */
+  lo = imin(WLO(mx), WLO(mo) - 1);
+  hi = imax(WHI(mx), WHI(mo) + 1);
+  wf_alloc(out, lo, hi);
+  for (k = lo; k <= hi; ++k) {
+    int32_t m = vmax(wf_rd(mo, k + 1), vmax(wf_rd(mx, k), wf_rd(mo, k - 1)) + 1);
+    if ((uint32_t)m > (uint32_t)a->tlen) m = ONULL;
+    if ((uint32_t)(m - k) > (uint32_t)a->plen) m = ONULL;
+    out->v[k - lo] = m;
+  }
+  a->dsteps += hi - lo + 1;
+  trim_ends(a, out);
+}
+
 static void compute(oaln_t* a, int s) {
   /* wavefront_compute_affine (compute_affine.c:229-260) and
      wavefront_compute_affine2p (compute_affine2p.c:334-368) */
+  if (a->metric <= M_LINEAR) { compute_linear(a, s); return; }
   const bool two = (a->metric == 4);
   aln_grow(a, slot_of(a, s) + 1);  /* before taking pointers into the slot arrays */
   owf_t* mx  = wf_src(a, C_M, s - a->x);
@@ -353,9 +460,50 @@ static int64_t probe(oaln_t* a, int c, int s, int k, int add, int type) {
   return (int64_t)(wf_raw(w, k) + add) * 16 + type;
 }
 
+/* wavefront_backtrace_linear, wavefront_backtrace.c:223-319 (indel, edit,
+   gap-linear): M only, ties M > D > I. */
+static void backtrace_linear(oaln_t* a, int s_end, int k_end, int off_end) {
+  aln_ops_reserve(a, 2 * (a->plen + a->tlen) + 8);
+  int pos = a->ops_cap;
+  int s = s_end, k = k_end, off = off_end;
+  int h = off, v = off - k, i;
+  for (i = a->plen - v; i > 0; --i) a->ops[--pos] = 'D';
+  for (i = a->tlen - h; i > 0; --i) a->ops[--pos] = 'I';
+  while (v > 0 && h > 0 && s > 0) {
+    const int sx = s - a->x, so = s - a->o1;
+    const int64_t misms = (a->metric != M_INDEL) ? probe(a, C_M, sx, k, 1, BT_M) : ONULL;
+    const int64_t ins = probe(a, C_M, so, k - 1, 1, BT_I1O);
+    const int64_t del = probe(a, C_M, so, k + 1, 0, BT_D1O);
+    const int64_t best = lmax(misms, lmax(ins, del));
+    if (best < 0) break;  /* no source */
+    const int max_off = (int)(best >> 4);
+    int n = off - max_off;
+    while (n-- > 0) a->ops[--pos] = 'M';
+    off = max_off;
+    v = off - k; h = off;
+    if (v <= 0 || h <= 0) break;
+    switch ((int)(best & 15)) {
+      case BT_M:   s = sx; a->ops[--pos] = 'X'; --off; break;
+      case BT_I1O: s = so; a->ops[--pos] = 'I'; --k; --off; break;
+      default:     s = so; a->ops[--pos] = 'D'; ++k; break;
+    }
+    v = off - k; h = off;
+  }
+  if (v > 0 && h > 0) {
+    int n = imin(v, h);
+    v -= n; h -= n;
+    while (n-- > 0) a->ops[--pos] = 'M';
+  }
+  while (v > 0) { a->ops[--pos] = 'D'; --v; }
+  while (h > 0) { a->ops[--pos] = 'I'; --h; }
+  a->ops_begin = pos;
+  a->ops_end = a->ops_cap;
+}
+
 /* wavefront_backtrace_affine, wavefront_backtrace.c:320-529.  Ops are written
    backwards so that a->ops[ops_begin..ops_end) is the CIGAR. */
 static void backtrace(oaln_t* a, int s_end, int k_end, int off_end) {
+  if (a->metric <= M_LINEAR) { backtrace_linear(a, s_end, k_end, off_end); return; }
   const bool two = (a->metric == 4);
   aln_ops_reserve(a, 2 * (a->plen + a->tlen) + 8);
   int pos = a->ops_cap;               /* next op goes to ops[--pos] */
@@ -564,11 +712,13 @@ static void bi_overlap(oaln_t* a0, oaln_t* a1, int s0, int s1, bool forward, obp
       owf_t* i2w1 = wf_at(a1, C_I2, si);
       if (i2w0 && i2w1) bp_indel2indel(a0, forward, s0, si, i2w0, i2w1, C_I2, a0->o2, bp);
     }
-    if (s0 + si - a0->o1 >= bp->score) continue;
-    owf_t* d1w1 = wf_at(a1, C_D1, si);
-    if (d1w0 && d1w1) bp_indel2indel(a0, forward, s0, si, d1w0, d1w1, C_D1, a0->o1, bp);
-    owf_t* i1w1 = wf_at(a1, C_I1, si);
-    if (i1w0 && i1w1) bp_indel2indel(a0, forward, s0, si, i1w0, i1w1, C_I1, a0->o1, bp);
+    if (a0->metric >= 3) {  /* I1/D1 breakpoints: gap-affine metrics only */
+      if (s0 + si - a0->o1 >= bp->score) continue;
+      owf_t* d1w1 = wf_at(a1, C_D1, si);
+      if (d1w0 && d1w1) bp_indel2indel(a0, forward, s0, si, d1w0, d1w1, C_D1, a0->o1, bp);
+      owf_t* i1w1 = wf_at(a1, C_I1, si);
+      if (i1w0 && i1w1) bp_indel2indel(a0, forward, s0, si, i1w0, i1w1, C_I1, a0->o1, bp);
+    }
     if (s0 + si >= bp->score) continue;
     owf_t* mw1 = wf_at(a1, C_M, si);
     if (mw1) bp_m2m(a0, forward, s0, si, m0, mw1, bp);
@@ -613,7 +763,8 @@ static int bi_find_breakpoint(obi_t* b, int cb, int ce, obp_t* bp) {
     if (sr + sf >= max_steps) return ST_MAX_STEPS;
   }
   const int scope = f->scope;
-  const int gap_open = (f->metric == 4) ? imax(f->o1, f->o2) : f->o1;
+  /* wavefront_bialign_overlap_gopen_adjust, wavefront_bialign.c:960-973 */
+  const int gap_open = (f->metric == 4) ? imax(f->o1, f->o2) : (f->metric == 3) ? f->o1 : 0;
   while (1) {
     if (last_forward) {
       const int min_sr = (sr > scope - 1) ? sr - (scope - 1) : 0;
@@ -772,6 +923,24 @@ void oracle_align_batch(const oracle_conf_t* conf, int n, const char* buf,
 /* cigar_score_gap_affine / _affine2p, alignment/cigar.c:277-342 */
 int oracle_cigar_score(const oracle_conf_t* conf, const char* ops, int len) {
   int score = 0, i;
+  if (conf->metric <= M_EDIT) {  /* cigar_score_edit, alignment/cigar.c:244-259 (also for indel) */
+    for (i = 0; i < len; ++i) {
+      if (ops[i] == 'X' || ops[i] == 'I' || ops[i] == 'D') ++score;
+      else if (ops[i] != 'M') return INT_MIN;
+    }
+    return score;
+  }
+  if (conf->metric == M_LINEAR) {  /* cigar_score_gap_linear, alignment/cigar.c:260-276 */
+    for (i = 0; i < len; ++i) {
+      switch (ops[i]) {
+        case 'M': score -= conf->match; break;
+        case 'X': score -= conf->mismatch; break;
+        case 'I': case 'D': score -= conf->gap_opening1; break;
+        default: return INT_MIN;
+      }
+    }
+    return score;
+  }
   if (conf->metric != 4) {
     char last = 0;
     for (i = 0; i < len; ++i) {

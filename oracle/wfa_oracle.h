@@ -1,8 +1,9 @@
 /*
  * CPU oracle for the exact WFA hot path (TEST INFRASTRUCTURE ONLY).
  *
- * A plain-C restatement of WFA2-lib v2.3's wavefront_align for gap-affine and
- * dual-cost gap-affine, end-to-end, in memory mode high (and med/low, whose
+ * A plain-C restatement of WFA2-lib v2.3's wavefront_align for indel, edit,
+ * gap-linear, gap-affine and dual-cost gap-affine, end-to-end, in memory mode
+ * high (and med/low, whose
  * CIGARs the reference asserts equal memory-high: tests/wfa.utest.sh:67-78)
  * and ultralow (BiWFA).  It is the checker for the HIP path; only tests/,
  * __graft_entry__.smoke() and the cpu_baseline leg of bench.py may load it.
@@ -18,8 +19,10 @@
 #include <stdint.h>
 
 typedef struct {
-  int metric;          /* 3 = gap-affine, 4 = gap-affine-2p (wavefront_penalties.h:41-47) */
-  int match;           /* user penalties, as in affine{,2p}_penalties_t */
+  int metric;          /* 0 indel, 1 edit, 2 gap-linear, 3 gap-affine, 4 gap-affine-2p
+                          (wavefront_penalties.h:41-47) */
+  int match;           /* user penalties, as in affine{,2p}_penalties_t; gap-linear:
+                          match, mismatch, gap_opening1 = indel (linear_penalties_t) */
   int mismatch;
   int gap_opening1;
   int gap_extension1;
@@ -50,7 +53,7 @@ void oracle_align_batch(
     int* status, int* score, char* ops_out, const int64_t* ops_off, int* ops_len,
     int64_t* dsteps, int* end_score);
 
-/* cigar_score_gap_affine / cigar_score_gap_affine2p (alignment/cigar.c:277-342). */
+/* cigar_score_edit / _gap_linear / _gap_affine / _gap_affine2p (alignment/cigar.c:244-342). */
 int oracle_cigar_score(const oracle_conf_t* conf, const char* ops, int len);
 
 #endif

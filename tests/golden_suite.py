@@ -43,11 +43,16 @@ def golden_lines(name):
         return [l.rstrip("\n") for l in f]
 
 
-def cases():
+def cases(metrics=("affine", "affine2p", "edit", "indel")):
     """(file stem, metric, penalties, memory, score_only) for every golden file."""
     out = []
     for pre, (mem, so) in MODES.items():
-        for suf, pen in PENALTIES.items():
-            out.append(("%s.affine%s" % (pre, suf), "affine", pen, mem, so))
-        out.append(("%s.affine2p" % pre, "affine2p", AFFINE2P, mem, so))
+        if "affine" in metrics:
+            for suf, pen in PENALTIES.items():
+                out.append(("%s.affine%s" % (pre, suf), "affine", pen, mem, so))
+        if "affine2p" in metrics:
+            out.append(("%s.affine2p" % pre, "affine2p", AFFINE2P, mem, so))
+        for m in ("edit", "indel"):
+            if m in metrics:
+                out.append(("%s.%s" % (pre, m), m, (0, 1, 1), mem, so))
     return out

@@ -3,7 +3,8 @@
 1. Golden vectors: the reference's own regression outputs for the 305-pair
    suite (tests/wfa.utest.check/*.alg) in every exact mode the device path
    covers: memory high / med / score-only / BiWFA / BiWFA score-only,
-   gap-affine with seven penalty sets (match < 0 included) and gap-affine-2p.
+   gap-affine with seven penalty sets (match < 0 included), gap-affine-2p,
+   edit and indel.  (Gap-linear has no golden file: it is pinned by 2 and 3.)
 2. The reference itself, compiled from its sources into oracle/_ref/, on
    seeded random pairs of the benchmark shapes (skipped if not built).
 3. Mathematics: an independent Gotoh DP (tests/dp.py) for the optimal score,
@@ -95,6 +96,34 @@ def test_oracle_score_is_dp_optimum(metric, pen):
                 assert r.score[i] == best
 
 
+# (metric, oracle penalties, Gotoh-equivalent (match, mismatch, open, extend), sign of the score)
+LINEAR_SETS = [
+    ("edit", (0, 1, 1), (0, 1, 0, 1), -1),   # edit/indel scores are distances (positive)
+    ("indel", (0, 1, 1), (0, 2, 0, 1), -1),  # a mismatch costs as much as I+D: same optimum
+    ("linear", (0, 4, 2), (0, 4, 0, 2), 1),
+    ("linear", (-1, 3, 2), (-1, 3, 0, 2), 1),
+]
+
+
+@pytest.mark.parametrize("metric,pen,gpen,sign", LINEAR_SETS)
+def test_oracle_linear_metrics_are_dp_optimum(metric, pen, gpen, sign):
+    """Edit, indel and gap-linear: the score is the optimum of an independent
+    DP with the equivalent costs, the CIGAR is valid and re-scores to it."""
+    pats, texts = _random_pairs(hash((metric, pen)) & 0xFFFF, 60, 0, 36, 8, max_indel=4)
+    for memory in ("high", "ultralow"):
+        r = O.oracle_align(O.Conf(metric, pen, memory), _pairs(pats, texts))
+        for i, (p, t) in enumerate(zip(pats, texts)):
+            assert r.status[i] == 0
+            best = dp.gotoh(p, t, *gpen)
+            ops = r.ops(i)
+            assert dp.cigar_valid(p, t, ops), (p, t, ops)
+            assert dp.cigar_score(ops, *gpen) == best, (p, t, ops)
+            if metric == "indel":
+                assert b"X" not in ops
+            if memory == "high" or (len(p) and len(t)):
+                assert r.score[i] == sign * best
+
+
 def test_oracle_closed_forms():
     conf = O.Conf("affine", (0, 4, 6, 2))
     t = b"ACGTTGCAACGTACGGATCCATGACCAGTACGATCG"
@@ -135,6 +164,14 @@ def test_oracle_max_steps():
     ("affine", (0, 4, 6, 2), "ultralow", "kbp"),
     ("affine2p", (0, 4, 6, 2, 24, 1), "ultralow", "kbp"),
     ("affine", (-3, 5, 3, 2), "high", "short"),
+    ("edit", (0, 1, 1), "high", "short"),
+    ("indel", (0, 1, 1), "high", "short"),
+    ("linear", (0, 4, 2), "high", "short"),
+    ("linear", (-1, 3, 2), "high", "short"),
+    ("edit", (0, 1, 1), "ultralow", "kbp"),
+    ("indel", (0, 1, 1), "ultralow", "kbp"),
+    ("linear", (0, 4, 2), "ultralow", "kbp"),
+    ("edit", (0, 1, 1), "high", "kbp"),
 ])
 def test_oracle_matches_reference_build(metric, pen, memory, shape):
     from wfa_amd import datagen
