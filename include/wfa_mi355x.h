@@ -10,12 +10,12 @@
  * unchanged.  Alignment runs on one AMD Instinct MI355X (gfx950) through HIP.
  *
  * Exact path (bit-identical scores and CIGARs to the reference):
- *   distance_metric gap_affine / gap_affine_2p, alignment_end2end,
- *   heuristic wf_heuristic_none, memory modes high / med / low / ultralow,
- *   compute_score or compute_alignment.
- * Configurations outside it (edit/indel/linear metrics, ends-free/extension,
- * heuristics, lambda sequences) are rejected with WF_STATUS_UNSUPPORTED and a
- * "[WFA]" message on stderr; they never silently produce different results.
+ *   distance_metric indel / edit / gap_linear / gap_affine / gap_affine_2p,
+ *   alignment_end2end, heuristic wf_heuristic_none, memory modes high / med /
+ *   low / ultralow, compute_score or compute_alignment.
+ * Configurations outside it (ends-free/extension, heuristics, lambda
+ * sequences) are rejected with WF_STATUS_UNSUPPORTED and a "[WFA]" message on
+ * stderr; they never silently produce different results.
  *
  * Not thread-safe per aligner (as the reference, wavefront_aligner.c); one
  * aligner per host thread, any number of aligners per device.
@@ -422,8 +422,10 @@ double wfa_server_clock(void);
 void wfa_server_destroy(wfa_server_t* server);
 
 /* ------------------------------------------------------------ CIGAR utils
- * alignment/cigar.c:277-470 */
+ * alignment/cigar.c:244-470 */
 bool cigar_is_null(const cigar_t* const cigar);
+int cigar_score_edit(const cigar_t* const cigar);                                          /* cigar.c:244-259 */
+int cigar_score_gap_linear(const cigar_t* const cigar, const linear_penalties_t* const penalties); /* :260-276 */
 int cigar_score_gap_affine(const cigar_t* const cigar, const affine_penalties_t* const penalties);
 int cigar_score_gap_affine2p(const cigar_t* const cigar, const affine2p_penalties_t* const penalties);
 bool cigar_check_alignment(FILE* const stream,

@@ -37,7 +37,7 @@ def test_default_attributes_match_reference():
 
 
 @pytest.mark.parametrize("kw", [
-    dict(metric="edit"),
+    dict(metric="edit", ends_free=(0, 0, 5, 5)),
     dict(metric="affine", heuristic_none=False),
 ])
 def test_unsupported_configurations_are_refused(kw):
@@ -61,3 +61,31 @@ def test_generator_is_seeded_and_independent_per_pair():
     assert a.text(0) != c.text(0)
     assert all(100 <= a.tlen[i] <= 300 for i in range(a.n))
     assert set(a.buf) <= set(b"ACGT")
+
+
+def test_cigar_scores_of_every_metric_match_oracle():
+    """cigar_score_edit / _gap_linear / _gap_affine / _gap_affine2p of the C
+    ABI (alignment/cigar.c:244-342) against the oracle's scorer on random
+    operation strings."""
+    import ctypes
+    import random
+
+    import oracle as O
+    L = wfa_amd.lib()
+    rng = random.Random(7)
+    for _ in range(200):
+        # at least one operation: the affine2p scorer exits on an empty CIGAR, as the reference
+        ops = bytes(rng.choice(b"MMMXID") for _ in range(rng.randint(1, 40)))
+        buf = ctypes.create_string_buffer(ops, len(ops) + 1)
+        c = wfa_amd.cigar_t()
+        c.operations = ctypes.cast(buf, ctypes.c_void_p)
+        c.begin_offset, c.end_offset = 0, len(ops)
+        ref = ctypes.byref(c)
+        assert L.cigar_score_edit(ref) == O.cigar_score(O.Conf("edit", (0, 1, 1)), ops)
+        lp = wfa_amd.linear_penalties_t(-1, 3, 2)
+        assert L.cigar_score_gap_linear(ref, ctypes.byref(lp)) == O.cigar_score(O.Conf("linear", (-1, 3, 2)), ops)
+        ap = wfa_amd.affine_penalties_t(0, 4, 6, 2)
+        assert L.cigar_score_gap_affine(ref, ctypes.byref(ap)) == O.cigar_score(O.Conf("affine", (0, 4, 6, 2)), ops)
+        a2 = wfa_amd.affine2p_penalties_t(0, 4, 6, 2, 24, 1)
+        assert L.cigar_score_gap_affine2p(ref, ctypes.byref(a2)) == O.cigar_score(
+            O.Conf("affine2p", (0, 4, 6, 2, 24, 1)), ops)

@@ -38,7 +38,7 @@ def assert_same(got, ref, ps, check_dsteps=None):
         assert np.array_equal(check_dsteps, ref.dsteps)
 
 
-GOLDEN_GPU = G.cases()
+GOLDEN_GPU = G.cases()  # every golden file: affine, affine2p, edit, indel
 
 
 @pytest.fixture(scope="module")

@@ -172,10 +172,19 @@ def test_oracle_max_steps():
     ("indel", (0, 1, 1), "ultralow", "kbp"),
     ("linear", (0, 4, 2), "ultralow", "kbp"),
     ("edit", (0, 1, 1), "high", "kbp"),
+    ("edit", (0, 1, 1), "high", "skew"),
+    ("edit", (0, 1, 1), "ultralow", "skew"),
+    ("indel", (0, 1, 1), "high", "skew"),
+    ("linear", (-1, 3, 2), "ultralow", "skew"),
 ])
 def test_oracle_matches_reference_build(metric, pen, memory, shape):
+    """The skewed pairs (one long insertion) make edit distance prune its
+    wavefronts (wavefront_compute_edit.c:219-278)."""
     from wfa_amd import datagen
-    ps = datagen.generate(1, 11, 600) if shape == "short" else datagen.generate(2, 12, 6)
+    if shape == "skew":
+        ps = datagen.skewed(5, 8)
+    else:
+        ps = datagen.generate(1, 11, 600) if shape == "short" else datagen.generate(2, 12, 6)
     pairs = _pairs(ps.patterns(), ps.texts())
     conf = O.Conf(metric, pen, memory)
     a = O.oracle_align(conf, pairs)
@@ -184,6 +193,17 @@ def test_oracle_matches_reference_build(metric, pen, memory, shape):
     assert list(a.score) == list(b.score)
     for i in range(pairs.n):
         assert a.ops(i) == b.ops(i), i
+
+
+def test_oracle_edit_prune_cuts_diagonal_steps():
+    # Edit and indel share the recurrence except for the mismatch source;
+    # on skewed pairs only edit prunes, and its wavefronts stay far narrower.
+    from wfa_amd import datagen
+    ps = datagen.skewed(5, 8)
+    pairs = _pairs(ps.patterns(), ps.texts())
+    e = O.oracle_align(O.Conf("edit", (0, 1, 1)), pairs)
+    i = O.oracle_align(O.Conf("indel", (0, 1, 1)), pairs)
+    assert e.dsteps.sum() * 2 < i.dsteps.sum()
 
 
 def test_oracle_counts_diagonal_steps():

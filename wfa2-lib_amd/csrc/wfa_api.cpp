@@ -159,16 +159,33 @@ int gcd(int a, int b) {
 }
 }  // namespace
 
-wfa::Penalties make_penalties(const wavefront_penalties_t& p, bool two) {
+// Device penalties and ring geometry; scopes as wavefront_components.c:45-124.
+wfa::Penalties make_penalties(const wavefront_penalties_t& p) {
   wfa::Penalties r{};
+  r.metric = (int)p.distance_metric;
+  if (p.distance_metric <= gap_linear) {
+    // One component (M): gaps are read from M at s - o1 (e1 = 0).
+    r.ncomp = 1;
+    r.x = p.distance_metric == indel ? 1 : p.mismatch;  // indel has no mismatch source
+    r.o1 = p.distance_metric == gap_linear ? p.gap_opening1 : 1;
+    r.e1 = 0;
+    r.o2 = 0;
+    r.e2 = 1;
+    r.scope = p.distance_metric == gap_linear ? std::max(r.x, r.o1) + 1 : 2;
+    r.g = std::max(gcd(r.x, r.o1), 1);
+    r.nslots = (r.scope - 1) / r.g + 1;
+    return r;
+  }
+  const bool two = p.distance_metric == gap_affine_2p;
+  r.ncomp = two ? 5 : 3;
   r.x = p.mismatch;
   r.o1 = p.gap_opening1;
   r.e1 = p.gap_extension1;
   r.o2 = two ? p.gap_opening2 : 0;
   r.e2 = two ? p.gap_extension2 : 1;
-  int indel = r.o1 + r.e1;
-  if (two) indel = std::max(indel, r.o2 + r.e2);
-  r.scope = std::max(indel, r.x) + 1;
+  int indel_scope = r.o1 + r.e1;
+  if (two) indel_scope = std::max(indel_scope, r.o2 + r.e2);
+  r.scope = std::max(indel_scope, r.x) + 1;
   int g = gcd(r.x, gcd(r.o1 + r.e1, r.e1));
   if (two) g = gcd(g, gcd(r.o2 + r.e2, r.e2));
   r.g = std::max(g, 1);
@@ -176,11 +193,44 @@ wfa::Penalties make_penalties(const wavefront_penalties_t& p, bool two) {
   return r;
 }
 
-// wavefront_penalties_set_affine{,2p} (wavefront_penalties.c:95-173).
+// wavefront_penalties_set_{indel,edit,linear,affine,affine2p}
+// (wavefront_penalties.c:39-173).
 void set_penalties(wavefront_penalties_t* wp, const wavefront_aligner_attr_t* attr) {
   memset(wp, 0, sizeof(*wp));
   wp->distance_metric = attr->distance_metric;
-  if (attr->distance_metric == gap_affine_2p) {
+  if (attr->distance_metric == indel || attr->distance_metric == edit) {
+    wp->match = 0;
+    wp->mismatch = attr->distance_metric == indel ? -1 : 1;
+    wp->gap_opening1 = 1;
+    wp->gap_extension1 = -1;
+    wp->gap_opening2 = -1;
+    wp->gap_extension2 = -1;
+  } else if (attr->distance_metric == gap_linear) {
+    const linear_penalties_t& a = attr->linear_penalties;
+    if (a.match > 0) {
+      fprintf(stderr, "[WFA::Penalties] Match score must be negative or zero (M=%d)\n", a.match);
+      exit(1);
+    }
+    if (a.mismatch <= 0 || a.indel <= 0) {
+      fprintf(stderr, "[WFA::Penalties] Penalties (X=%d,D=%d,I=%d) must be (X>0,D>0,I>0)\n", a.mismatch, a.indel,
+              a.indel);
+      exit(1);
+    }
+    if (a.match < 0) {
+      wp->match = a.match;
+      wp->mismatch = 2 * a.mismatch - 2 * a.match;
+      wp->gap_opening1 = 2 * a.indel - a.match;
+    } else {
+      wp->match = 0;
+      wp->mismatch = a.mismatch;
+      wp->gap_opening1 = a.indel;
+    }
+    wp->gap_extension1 = -1;
+    wp->gap_opening2 = -1;
+    wp->gap_extension2 = -1;
+    wp->linear_penalties = a;
+    wp->internal_gap_e = a.indel;
+  } else if (attr->distance_metric == gap_affine_2p) {
     const affine2p_penalties_t& a = attr->affine2p_penalties;
     if (a.match > 0) {
       fprintf(stderr, "[WFA::Penalties] Match score must be negative or zero (M=%d)\n", a.match);
@@ -235,13 +285,12 @@ void set_penalties(wavefront_penalties_t* wp, const wavefront_aligner_attr_t* at
     wp->gap_extension2 = -1;
     wp->affine_penalties = a;
     wp->internal_gap_e = a.gap_extension;
-  } else {
-    wp->linear_penalties = attr->linear_penalties;
   }
 }
 
 // wavefront_compute_classic_score (wavefront_compute.c:108-120).
 int classic_score(const wavefront_penalties_t& p, int plen, int tlen, int s) {
+  if (p.distance_metric <= edit) return s;
   const int swg_match = -p.match;
   if (swg_match == 0) return -s;
   return (swg_match * (plen + tlen) - s) / 2;
@@ -254,11 +303,11 @@ namespace {
 int check_supported(wavefront_aligner_t* wf) {
   const distance_metric_t m = wf->penalties.distance_metric;
   const char* why = nullptr;
-  if (m != gap_affine && m != gap_affine_2p) why = "only gap-affine and gap-affine-2p metrics";
+  if (m < indel || m > gap_affine_2p) why = "an unknown distance metric";
   else if (wf->alignment_form.span != alignment_end2end || wf->alignment_form.extension)
     why = "only end-to-end alignment";
   else if (wf->heuristic.strategy != wf_heuristic_none) why = "only heuristic none (exact alignment)";
-  else if (make_penalties(wf->penalties, m == gap_affine_2p).nslots > 64)
+  else if (make_penalties(wf->penalties).nslots > 64)
     why = "penalties whose score scope needs more than 64 wavefront slots";
   if (!why) return WF_STATUS_OK;
   if (wf->system.verbose >= 0)
@@ -355,7 +404,7 @@ PlanGroup make_large(const JobSet& js, std::vector<int> ids) {
   return g;
 }
 
-LaunchShape shape_of(Engine* e, const PlanGroup& g, int two, int bits, const Penalties& pen) {
+LaunchShape shape_of(Engine* e, const PlanGroup& g, int bits, const Penalties& pen) {
   LaunchShape shape;
   shape.family = g.family;
   shape.block = g.block;
@@ -364,8 +413,8 @@ LaunchShape shape_of(Engine* e, const PlanGroup& g, int two, int bits, const Pen
     shape.lds_bytes = wfa::lean_lds_bytes(pen, g.wcap, g.lmax_words);
     per_cu = wfa::lean_max_blocks_per_cu(pen, g.wcap, shape.lds_bytes);
   } else {
-    shape.lds_bytes = wfa::forward_lds_bytes(g.family, two ? 5 : 3, pen.nslots, g.wcap, g.lmax_words);
-    per_cu = wfa::forward_max_blocks_per_cu(g.family, g.block, two, bits, shape.lds_bytes);
+    shape.lds_bytes = wfa::forward_lds_bytes(g.family, pen.ncomp, pen.nslots, g.wcap, g.lmax_words);
+    per_cu = wfa::forward_max_blocks_per_cu(g.family, g.block, pen.ncomp, bits, shape.lds_bytes);
   }
   if (per_cu <= 0) per_cu = 1;
   shape.grid = (int)std::min<int64_t>((int64_t)g.ids.size(), (int64_t)per_cu * e->num_cus);
@@ -373,12 +422,12 @@ LaunchShape shape_of(Engine* e, const PlanGroup& g, int two, int bits, const Pen
 }
 
 // Enqueues forward + backtrace of one group on the aligner's stream.
-int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGroup& grp, int two,
-                  const Penalties& pen, int records) {
+int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGroup& grp, const Penalties& pen,
+                  int records) {
   const int n = (int)grp.ids.size();
   if (n == 0) return WF_STATUS_OK;
   hipStream_t st = e->stream;
-  const LaunchShape shape = shape_of(e, grp, two, js.bits, pen);
+  const LaunchShape shape = shape_of(e, grp, js.bits, pen);
   if (!grp.ids_on_device) {
     if (!grp.d_ids.ensure((size_t)n * 4)) return WF_STATUS_OOM;
     if (!hip_ok(hipMemcpyAsync(grp.d_ids.p, grp.ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, st), "plan upload"))
@@ -406,7 +455,7 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
   fa.rec_units = records ? std::min<unsigned long long>(e->rec.cap / 16, 0xFFFFFFF0ull) : 0;
   fa.rec_top = &c->rec_top;
   fa.ring_global = grp.family == wfa::kLarge ? e->ring.as<int>() : nullptr;
-  fa.ring_stride = grp.family == wfa::kLarge ? (long long)pen.nslots * (two ? 5 : 3) * grp.wcap : 0;
+  fa.ring_stride = grp.family == wfa::kLarge ? (long long)pen.nslots * pen.ncomp * grp.wcap : 0;
   fa.queue = &c->queue;
   fa.dsteps_total = &c->dsteps;
   fa.retry_count = &c->retry;
@@ -422,7 +471,7 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
   if (!hip_ok(wfa::launch_reset_counters(c, 0, st), "reset")) return WF_STATUS_OOM;
   (void)hipEventRecord(grp.ev[0], st);
   const hipError_t lerr = grp.family == wfa::kLean ? wfa::launch_forward_lean(fa, shape, st)
-                                                   : wfa::launch_forward(fa, two, js.bits, shape, st);
+                                                   : wfa::launch_forward(fa, pen.ncomp, js.bits, shape, st);
   if (!hip_ok(lerr, "forward launch")) return WF_STATUS_OOM;
   (void)hipEventRecord(grp.ev[1], st);
   if (records) {
@@ -456,7 +505,7 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
 }
 
 // Device memory every group of the plan needs, allocated before any launch.
-bool reserve_for(Engine* e, std::vector<PlanGroup>& plan, int two, int bits, const Penalties& pen, int records) {
+bool reserve_for(Engine* e, std::vector<PlanGroup>& plan, int bits, const Penalties& pen, int records) {
   if (records && !e->rec.ensure(e->rec_bytes)) {
     fprintf(stderr, "[WFA] cannot allocate the %zu MB record arena\n", e->rec_bytes >> 20);
     return false;
@@ -464,8 +513,8 @@ bool reserve_for(Engine* e, std::vector<PlanGroup>& plan, int two, int bits, con
   size_t ring = 0;
   for (const PlanGroup& g : plan) {
     if (g.family != wfa::kLarge) continue;
-    const LaunchShape sh = shape_of(e, g, two, bits, pen);
-    ring = std::max(ring, (size_t)pen.nslots * (two ? 5 : 3) * g.wcap * sh.grid * sizeof(int));
+    const LaunchShape sh = shape_of(e, g, bits, pen);
+    ring = std::max(ring, (size_t)pen.nslots * pen.ncomp * g.wcap * sh.grid * sizeof(int));
   }
   if (ring && !e->ring.ensure(ring)) {
     fprintf(stderr, "[WFA] cannot allocate %zu MB of wavefront rings\n", ring >> 20);
@@ -478,7 +527,8 @@ bool reserve_for(Engine* e, std::vector<PlanGroup>& plan, int two, int bits, con
 
 // Splits the jobs over kernel families: short jobs on the LDS family, the
 // rest on the HBM-ring family by workgroup size.
-std::vector<PlanGroup> make_plan(const JobSet& js, const std::vector<int>& ids, const Penalties& pen, int ncomp) {
+std::vector<PlanGroup> make_plan(const JobSet& js, const std::vector<int>& ids, const Penalties& pen) {
+  const int ncomp = pen.ncomp;
   const int wcap_small = small_wcap(pen, ncomp);
   const int spw = 64 / js.bits;
   std::vector<int> small_ids, cls[3];
@@ -530,8 +580,7 @@ std::vector<PlanGroup> make_plan(const JobSet& js, const std::vector<int>& ids, 
 
 int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<PlanGroup>& plan, int records,
              RunTotals* tot, std::vector<int>& h_status) {
-  const int two = wf->penalties.distance_metric == gap_affine_2p;
-  const Penalties pen = make_penalties(wf->penalties, two);
+  const Penalties pen = make_penalties(wf->penalties);
   hipStream_t st = e->stream;
   wfa::Counters host_c{};
   std::vector<PlanGroup> retry_groups;
@@ -548,7 +597,7 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
           const double L = std::max(js.h_plen[i], js.h_tlen[i]);
           w += L * L / 16 + 8 * L;
         }
-        g.est_rec_bytes = w + (double)shape_of(e, g, two, js.bits, pen).grid * wfa::kChunkUnits * 16;
+        g.est_rec_bytes = w + (double)shape_of(e, g, js.bits, pen).grid * wfa::kChunkUnits * 16;
       }
       want += g.est_rec_bytes;
     }
@@ -561,10 +610,10 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
       fprintf(stderr, "[WFA] batch did not converge (record arena too small?)\n");
       return WF_STATUS_OOM;
     }
-    if (!reserve_for(e, *pending, two, js.bits, pen, records)) return WF_STATUS_OOM;
+    if (!reserve_for(e, *pending, js.bits, pen, records)) return WF_STATUS_OOM;
     if (!hip_ok(wfa::launch_reset_counters(counters_of(e), 1, st), "reset")) return WF_STATUS_OOM;
     for (PlanGroup& g : *pending) {
-      const int rc = enqueue_group(wf, e, js, g, two, pen, records);
+      const int rc = enqueue_group(wf, e, js, g, pen, records);
       if (rc != WF_STATUS_OK) return rc;
       ++tot->launches;
     }
@@ -849,9 +898,7 @@ extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_ba
   if (sup != WF_STATUS_OK) return sup;
   Engine* e = engine_of(wf);
   if (!e || !engine_init(e)) return WF_STATUS_OOM;
-  const int two = wf->penalties.distance_metric == gap_affine_2p;
-  const Penalties pen = make_penalties(wf->penalties, two);
-  const int ncomp = two ? 5 : 3;
+  const Penalties pen = make_penalties(wf->penalties);
   const int records = (wf->alignment_scope == compute_alignment) ? 1 : 0;
   b->records = records;
   b->stats = wfa_run_stats_t{};
@@ -875,9 +922,9 @@ extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_ba
     rc = bialign_run(wf, e, b, &tot);
     if (rc == WF_STATUS_OOM) return rc;
   } else {
-    const std::vector<int> key = {pen.x, pen.o1, pen.e1, pen.o2, pen.e2, two, b->bits};
+    const std::vector<int> key = {pen.metric, pen.x, pen.o1, pen.e1, pen.o2, pen.e2, b->bits};
     if (b->plan_key != key) {
-      std::vector<PlanGroup> plan = make_plan(b->jobs(), b->sorted, pen, ncomp);
+      std::vector<PlanGroup> plan = make_plan(b->jobs(), b->sorted, pen);
       recycle_plan(b->plan, plan);
       b->plan = std::move(plan);
       b->plan_key = key;

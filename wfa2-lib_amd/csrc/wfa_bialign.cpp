@@ -84,9 +84,8 @@ int bp_block(int size) { return size <= 512 ? 64 : (size <= 8192 ? 256 : 1024); 
 // Runs the breakpoint search of `nodes` (grouped by size) and returns the outcomes.
 bool run_breakpoints(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, const std::vector<Node>& nodes,
                      std::vector<wfa::BiOutcome>& out, RunTotals* tot) {
-  const int two = wf->penalties.distance_metric == gap_affine_2p;
-  const wfa::Penalties pen = make_penalties(wf->penalties, two);
-  const int ncomp = two ? 5 : 3;
+  const wfa::Penalties pen = make_penalties(wf->penalties);
+  const int ncomp = pen.ncomp;
   out.assign(nodes.size(), wfa::BiOutcome{});
   if (nodes.empty()) return true;
   // groups by workgroup size
@@ -109,7 +108,7 @@ bool run_breakpoints(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, 
       wmax = std::max(wmax, (probs[j].pe - probs[j].pb) + (probs[j].te - probs[j].tb));
     }
     const int wcap = wmax + 8;
-    int per_cu = wfa::breakpoint_max_blocks_per_cu(two, b->bits, block);
+    int per_cu = wfa::breakpoint_max_blocks_per_cu(ncomp, b->bits, block);
     if (per_cu <= 0) per_cu = 1;
     const int grid = (int)std::min<int64_t>((int64_t)ids.size(), (int64_t)per_cu * e->num_cus);
     const long long stride = 2LL * pen.nslots * ncomp * wcap;
@@ -148,7 +147,7 @@ bool run_breakpoints(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, 
                                     hipMemcpyHostToDevice, st), "bi upload") &&
               hip_ok(wfa::launch_reset_counters(c, 2, st), "reset");
     (void)hipEventRecord(e0, st);
-    ok = ok && hip_ok(wfa::launch_breakpoint(a, two, b->bits, block, grid, st), "breakpoint launch");
+    ok = ok && hip_ok(wfa::launch_breakpoint(a, ncomp, b->bits, block, grid, st), "breakpoint launch");
     (void)hipEventRecord(e1, st);
     ok = ok && hip_ok(hipMemcpyAsync(res.data(), d_out.p, res.size() * sizeof(wfa::BiOutcome), hipMemcpyDeviceToHost, st),
                       "bi download") &&
@@ -305,11 +304,10 @@ int bialign_run(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, RunTo
       js.ops = d_ops.as<char>();
       js.ops_off = d_off.as<int64_t>();
       js.ops_begin = d_beg.as<int>();
-      const int two = wf->penalties.distance_metric == gap_affine_2p;
       std::vector<int> ids(nb);
       for (int j = 0; j < nb; ++j) ids[j] = j;
       std::sort(ids.begin(), ids.end(), [&](int x, int y) { return bpl[x] + btl[x] > bpl[y] + btl[y]; });
-      std::vector<PlanGroup> plan = make_plan(js, ids, make_penalties(wf->penalties, two), two ? 5 : 3);
+      std::vector<PlanGroup> plan = make_plan(js, ids, make_penalties(wf->penalties));
       RunTotals bt;
       const int rc = run_jobs(wf, e, js, plan, 1, &bt, bstatus);
       for (auto& g : plan) g.release();

@@ -113,11 +113,13 @@ __device__ int side_step(Side<NC>& sd, int s, int plen, int tlen, const Penaltie
   sd.cur = (sd.cur + 1 == R) ? 0 : sd.cur + 1;
   const int cur = sd.cur;
   const int dq_x = pen.x / g, dq_o1 = (pen.o1 + pen.e1) / g, dq_e1 = pen.e1 / g;
-  const Src mx  = get_src(sd.meta, s - pen.x, cur - dq_x, R, kM, wcap);
+  Src mx{1, -1, kFarDiagonal, 0, 0}, i1 = mx, d1 = mx, mo2 = mx, i2 = mx, d2 = mx;
+  if (NC > 1 || pen.metric != kIndel) mx = get_src(sd.meta, s - pen.x, cur - dq_x, R, kM, wcap);
   const Src mo1 = get_src(sd.meta, s - pen.o1 - pen.e1, cur - dq_o1, R, kM, wcap);
-  const Src i1  = get_src(sd.meta, s - pen.e1, cur - dq_e1, R, kI1, wcap);
-  const Src d1  = get_src(sd.meta, s - pen.e1, cur - dq_e1, R, kD1, wcap);
-  Src mo2{1, -1, kFarDiagonal, 0, 0}, i2 = mo2, d2 = mo2;
+  if constexpr (NC > 1) {
+    i1 = get_src(sd.meta, s - pen.e1, cur - dq_e1, R, kI1, wcap);
+    d1 = get_src(sd.meta, s - pen.e1, cur - dq_e1, R, kD1, wcap);
+  }
   if constexpr (NC == 5) {
     const int dq_o2 = (pen.o2 + pen.e2) / g, dq_e2 = pen.e2 / g;
     mo2 = get_src(sd.meta, s - pen.o2 - pen.e2, cur - dq_o2, R, kM, wcap);
@@ -125,14 +127,24 @@ __device__ int side_step(Side<NC>& sd, int s, int plen, int tlen, const Penaltie
     d2  = get_src(sd.meta, s - pen.e2, cur - dq_e2, R, kD2, wcap);
   }
   const bool any = mx.valid() || mo1.valid() || i1.valid() || d1.valid() || mo2.valid() || i2.valid() || d2.valid();
+  // indel / edit: the previous wavefront is never empty end to end
+  if (NC == 1 && pen.metric <= kEdit && !mo1.valid()) return -1;
   if (!any) {
     if (lane == cur) { sd.meta.tag = s; sd.meta.flags = 0u; }
     return 0;
   }
-  int lo = mx.lim_lo, hi = mx.lim_hi;
-  lo = min(lo, mo1.lim_lo - 1); hi = max(hi, mo1.lim_hi + 1);
-  lo = min(lo, i1.lim_lo + 1);  hi = max(hi, i1.lim_hi + 1);
-  lo = min(lo, d1.lim_lo - 1);  hi = max(hi, d1.lim_hi - 1);
+  int lo, hi;
+  if (NC == 1 && pen.metric <= kEdit) {
+    lo = mo1.lim_lo - 1;
+    hi = mo1.lim_hi + 1;
+  } else {
+    lo = min(mx.lim_lo, mo1.lim_lo - 1);
+    hi = max(mx.lim_hi, mo1.lim_hi + 1);
+  }
+  if constexpr (NC > 1) {
+    lo = min(lo, i1.lim_lo + 1);  hi = max(hi, i1.lim_hi + 1);
+    lo = min(lo, d1.lim_lo - 1);  hi = max(hi, d1.lim_hi - 1);
+  }
   if constexpr (NC == 5) {
     lo = min(lo, mo2.lim_lo - 1); hi = max(hi, mo2.lim_hi + 1);
     lo = min(lo, i2.lim_lo + 1);  hi = max(hi, i2.lim_hi + 1);
@@ -142,14 +154,16 @@ __device__ int side_step(Side<NC>& sd, int s, int plen, int tlen, const Penaltie
   hi = uniform(hi);
   if (hi - lo + 1 > wcap) return -1;
   *dsteps += hi - lo + 1;
-  const bool has_i1 = mo1.valid() || i1.valid();
-  const bool has_d1 = mo1.valid() || d1.valid();
+  const bool prune_try = NC == 1 && pen.metric == kEdit && hi - lo + 1 >= kEditPruneWidth;
+  int wmin = INT_MAX;
+  const bool has_i1 = (NC > 1) && (mo1.valid() || i1.valid());
+  const bool has_d1 = (NC > 1) && (mo1.valid() || d1.valid());
   const bool has_i2 = (NC == 5) && (mo2.valid() || i2.valid());
   const bool has_d2 = (NC == 5) && (mo2.valid() || d2.valid());
   int* ring = sd.ring;
   int* outM  = ring + (cur * NC + kM) * wcap - lo;
-  int* outI1 = ring + (cur * NC + kI1) * wcap - lo;
-  int* outD1 = ring + (cur * NC + kD1) * wcap - lo;
+  int* outI1 = ring + (cur * NC + (NC > 1 ? kI1 : 0)) * wcap - lo;
+  int* outD1 = ring + (cur * NC + (NC > 1 ? kD1 : 0)) * wcap - lo;
   int* outI2 = ring + (cur * NC + (NC == 5 ? kI2 : 0)) * wcap - lo;
   int* outD2 = ring + (cur * NC + (NC == 5 ? kD2 : 0)) * wcap - lo;
   WaveRange rM, rI1, rD1, rI2, rD2;
@@ -158,7 +172,21 @@ __device__ int side_step(Side<NC>& sd, int s, int plen, int tlen, const Penaltie
   for (int kb = lo; kb <= hi; kb += BLOCK) {
     const int k = kb + tid;
     bool vM = false, vI1 = false, vD1 = false, vI2 = false, vD2 = false;
-    if (k <= hi) {
+    if constexpr (NC == 1) {
+      if (k <= hi) {
+        // indel / edit / gap-linear (wavefront_compute_edit.c:44-102,
+        // wavefront_compute_linear.c:44-76)
+        int m = max(rd(ring, mo1, k + 1), max(rd(ring, mo1, k - 1), rd(ring, mx, k)) + 1);
+        if (!in_bounds(m, k, plen, tlen)) m = kOffsetNull;
+        if (m != kOffsetNull) {
+          if (prune_try) wmin = min(wmin, max(plen - (m - k), tlen - m));
+          m = lcp_extend<BITS>(sd.P, sd.pb, sd.T, sd.tb, plen, tlen, k, m);
+          best_ak = max(best_ak, 2 * m - k);
+        }
+        outM[k] = m;
+        vM = (m != kOffsetNull);
+      }
+    } else if (k <= hi) {
       const int ins1 = max(rd(ring, mo1, k - 1), rd(ring, i1, k - 1)) + 1;
       const int del1 = max(rd(ring, mo1, k + 1), rd(ring, d1, k + 1));
       int m = max(del1, max(rd(ring, mx, k) + 1, ins1));
@@ -184,19 +212,24 @@ __device__ int side_step(Side<NC>& sd, int s, int plen, int tlen, const Penaltie
     }
     const int kw = kb + wid * 64;
     rM.add(__ballot(vM), kw);
-    rI1.add(__ballot(vI1), kw);
-    rD1.add(__ballot(vD1), kw);
+    if constexpr (NC > 1) {
+      rI1.add(__ballot(vI1), kw);
+      rD1.add(__ballot(vD1), kw);
+    }
     if constexpr (NC == 5) {
       rI2.add(__ballot(vI2), kw);
       rD2.add(__ballot(vD2), kw);
     }
   }
   if (want_max) *max_ak = red.max_all(best_ak);
+  if (prune_try) wmin = red.min_all(wmin);
   if constexpr (NW > 1) {
     if (lane == 0) {
       sh_red[wid][0] = rM.first;  sh_red[wid][1] = rM.last;
-      sh_red[wid][2] = rI1.first; sh_red[wid][3] = rI1.last;
-      sh_red[wid][4] = rD1.first; sh_red[wid][5] = rD1.last;
+      if constexpr (NC > 1) {
+        sh_red[wid][2] = rI1.first; sh_red[wid][3] = rI1.last;
+        sh_red[wid][4] = rD1.first; sh_red[wid][5] = rD1.last;
+      }
       if constexpr (NC == 5) {
         sh_red[wid][6] = rI2.first; sh_red[wid][7] = rI2.last;
         sh_red[wid][8] = rD2.first; sh_red[wid][9] = rD2.last;
@@ -207,8 +240,10 @@ __device__ int side_step(Side<NC>& sd, int s, int plen, int tlen, const Penaltie
 #pragma unroll
     for (int w = 0; w < NW; ++w) {
       rM.merge(sh_red[w][0], sh_red[w][1]);
-      rI1.merge(sh_red[w][2], sh_red[w][3]);
-      rD1.merge(sh_red[w][4], sh_red[w][5]);
+      if constexpr (NC > 1) {
+        rI1.merge(sh_red[w][2], sh_red[w][3]);
+        rD1.merge(sh_red[w][4], sh_red[w][5]);
+      }
       if constexpr (NC == 5) {
         rI2.merge(sh_red[w][6], sh_red[w][7]);
         rD2.merge(sh_red[w][8], sh_red[w][9]);
@@ -216,14 +251,33 @@ __device__ int side_step(Side<NC>& sd, int s, int plen, int tlen, const Penaltie
     }
   }
   __syncthreads();  // ring writes of this score visible to the whole workgroup
+  if constexpr (NC == 1) {
+    if (prune_try && !rM.null()) {
+      const int first0 = rM.first, last0 = rM.last;
+      edit_prune(rM, ring, mo1, wmin, plen, tlen);
+      // The reference extends only the kept range (the prune runs in the
+      // compute step, before the extension), so the furthest antidiagonal
+      // is taken over that range (wavefront_extend.c:173-211).
+      if (want_max && (rM.first != first0 || rM.last != last0)) {
+        int b = 0;
+        for (int k = rM.first + tid; k <= rM.last; k += BLOCK) {
+          const int m = outM[k];
+          if (m != kOffsetNull) b = max(b, 2 * m - k);
+        }
+        *max_ak = red.max_all(b);
+      }
+    }
+  }
   if (lane == cur) {
     sd.meta.tag = s;
     sd.meta.alo = lo;
     unsigned fl = 1u | (rM.null() ? 0x100u : 0u);
     sd.meta.lo[kM] = rM.tlo(lo);
     sd.meta.hi[kM] = rM.thi(lo);
-    if (has_i1) { fl |= 2u | (rI1.null() ? 0x200u : 0u); sd.meta.lo[kI1] = rI1.tlo(lo); sd.meta.hi[kI1] = rI1.thi(lo); }
-    if (has_d1) { fl |= 4u | (rD1.null() ? 0x400u : 0u); sd.meta.lo[kD1] = rD1.tlo(lo); sd.meta.hi[kD1] = rD1.thi(lo); }
+    if constexpr (NC > 1) {
+      if (has_i1) { fl |= 2u | (rI1.null() ? 0x200u : 0u); sd.meta.lo[kI1] = rI1.tlo(lo); sd.meta.hi[kI1] = rI1.thi(lo); }
+      if (has_d1) { fl |= 4u | (rD1.null() ? 0x400u : 0u); sd.meta.lo[kD1] = rD1.tlo(lo); sd.meta.hi[kD1] = rD1.thi(lo); }
+    }
     if constexpr (NC == 5) {
       if (has_i2) { fl |= 8u | (rI2.null() ? 0x800u : 0u); sd.meta.lo[kI2] = rI2.tlo(lo); sd.meta.hi[kI2] = rI2.thi(lo); }
       if (has_d2) { fl |= 16u | (rD2.null() ? 0x1000u : 0u); sd.meta.lo[kD2] = rD2.tlo(lo); sd.meta.hi[kD2] = rD2.thi(lo); }
@@ -231,7 +285,7 @@ __device__ int side_step(Side<NC>& sd, int s, int plen, int tlen, const Penaltie
     sd.meta.flags = fl;
   }
   const int kend = tlen - plen;
-  const int ce = sd.ce;
+  const int ce = NC == 1 ? (int)kM : sd.ce;
   bool pce = true;
   WaveRange rce = rM;
   if (ce == kI1) { pce = has_i1; rce = rI1; }
@@ -302,8 +356,11 @@ __device__ void overlap(const Side<NC>& a0, const Side<NC>& a1, int s0, int s1, 
                         const Penalties& pen, int wcap, Bp& bp, BlockRed<BLOCK>& red) {
   const WfRef<NC> m0 = wf_ref(a0, s0, kM, pen, wcap);
   if (!m0.present) return;
-  const WfRef<NC> d1w0 = wf_ref(a0, s0, kD1, pen, wcap);
-  const WfRef<NC> i1w0 = wf_ref(a0, s0, kI1, pen, wcap);
+  WfRef<NC> d1w0{false, 1, 0, 0}, i1w0{false, 1, 0, 0};
+  if constexpr (NC > 1) {
+    d1w0 = wf_ref(a0, s0, kD1, pen, wcap);
+    i1w0 = wf_ref(a0, s0, kI1, pen, wcap);
+  }
   WfRef<NC> d2w0{false, 1, 0, 0}, i2w0{false, 1, 0, 0};
   if constexpr (NC == 5) {
     d2w0 = wf_ref(a0, s0, kD2, pen, wcap);
@@ -321,13 +378,15 @@ __device__ void overlap(const Side<NC>& a0, const Side<NC>& a1, int s0, int s1, 
       if (i2w0.present && i2w1.present)
         scan_breakpoint<NC, BLOCK>(a0, a1, i2w0, i2w1, forward, true, s0, si, pen.o2, kI2, plen, tlen, bp, red);
     }
-    if (s0 + si - pen.o1 >= bp.score) continue;
-    const WfRef<NC> d1w1 = wf_ref(a1, si, kD1, pen, wcap);
-    if (d1w0.present && d1w1.present)
-      scan_breakpoint<NC, BLOCK>(a0, a1, d1w0, d1w1, forward, true, s0, si, pen.o1, kD1, plen, tlen, bp, red);
-    const WfRef<NC> i1w1 = wf_ref(a1, si, kI1, pen, wcap);
-    if (i1w0.present && i1w1.present)
-      scan_breakpoint<NC, BLOCK>(a0, a1, i1w0, i1w1, forward, true, s0, si, pen.o1, kI1, plen, tlen, bp, red);
+    if constexpr (NC > 1) {  // I1/D1 breakpoints: gap-affine metrics only
+      if (s0 + si - pen.o1 >= bp.score) continue;
+      const WfRef<NC> d1w1 = wf_ref(a1, si, kD1, pen, wcap);
+      if (d1w0.present && d1w1.present)
+        scan_breakpoint<NC, BLOCK>(a0, a1, d1w0, d1w1, forward, true, s0, si, pen.o1, kD1, plen, tlen, bp, red);
+      const WfRef<NC> i1w1 = wf_ref(a1, si, kI1, pen, wcap);
+      if (i1w0.present && i1w1.present)
+        scan_breakpoint<NC, BLOCK>(a0, a1, i1w0, i1w1, forward, true, s0, si, pen.o1, kI1, plen, tlen, bp, red);
+    }
     if (s0 + si >= bp.score) continue;
     const WfRef<NC> mw1 = wf_ref(a1, si, kM, pen, wcap);
     if (mw1.present) scan_breakpoint<NC, BLOCK>(a0, a1, m0, mw1, forward, false, s0, si, 0, kM, plen, tlen, bp, red);
@@ -438,7 +497,8 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
     // phase 2 (:1046-1080)
     {
       const int scope = pen.scope;
-      const int gap_open = (NC == 5) ? max(pen.o1, pen.o2) : pen.o1;
+      // wavefront_bialign_overlap_gopen_adjust (wavefront_bialign.c:960-973)
+      const int gap_open = (NC == 5) ? max(pen.o1, pen.o2) : (NC == 3 ? pen.o1 : 0);
       while (true) {
         if (last_forward) {
           const int min_sr = (sr > scope - 1) ? sr - (scope - 1) : 0;
@@ -490,12 +550,15 @@ const void* bp_fn() {
   return reinterpret_cast<const void*>(&wfa_breakpoint_kernel<NC, BLOCK, BITS>);
 }
 
-const void* select_breakpoint(int two_piece, int bits, int block) {
+const void* select_breakpoint(int ncomp, int bits, int block) {
 #define WFA_BSEL(NC, BL, BI) \
-  if ((two_piece ? 5 : 3) == NC && block == BL && bits == BI) return bp_fn<NC, BL, BI>();
-  WFA_BSEL(3, 64, 2) WFA_BSEL(3, 64, 8) WFA_BSEL(5, 64, 2) WFA_BSEL(5, 64, 8)
-  WFA_BSEL(3, 256, 2) WFA_BSEL(3, 256, 8) WFA_BSEL(5, 256, 2) WFA_BSEL(5, 256, 8)
-  WFA_BSEL(3, 1024, 2) WFA_BSEL(3, 1024, 8) WFA_BSEL(5, 1024, 2) WFA_BSEL(5, 1024, 8)
+  if (ncomp == NC && block == BL && bits == BI) return bp_fn<NC, BL, BI>();
+#define WFA_BSEL_NC(BL) \
+  WFA_BSEL(1, BL, 2) WFA_BSEL(1, BL, 8) WFA_BSEL(3, BL, 2) WFA_BSEL(3, BL, 8) WFA_BSEL(5, BL, 2) WFA_BSEL(5, BL, 8)
+  WFA_BSEL_NC(64)
+  WFA_BSEL_NC(256)
+  WFA_BSEL_NC(1024)
+#undef WFA_BSEL_NC
 #undef WFA_BSEL
   return nullptr;
 }
@@ -555,16 +618,16 @@ hipError_t launch_gather_ops(const OpsPiece* pieces, int n, const char* src, cha
   return hipGetLastError();
 }
 
-int breakpoint_max_blocks_per_cu(int two_piece, int bits, int block) {
-  const void* f = select_breakpoint(two_piece, bits, block);
+int breakpoint_max_blocks_per_cu(int ncomp, int bits, int block) {
+  const void* f = select_breakpoint(ncomp, bits, block);
   if (!f) return 0;
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, 0) != hipSuccess) return 0;
   return n;
 }
 
-hipError_t launch_breakpoint(const BiArgs& a, int two_piece, int bits, int block, int grid, hipStream_t stream) {
-  const void* f = select_breakpoint(two_piece, bits, block);
+hipError_t launch_breakpoint(const BiArgs& a, int ncomp, int bits, int block, int grid, hipStream_t stream) {
+  const void* f = select_breakpoint(ncomp, bits, block);
   if (!f) return hipErrorInvalidValue;
   void* args[] = {const_cast<BiArgs*>(&a)};
   return hipLaunchKernel(f, dim3(grid), dim3(block), args, 0, stream);

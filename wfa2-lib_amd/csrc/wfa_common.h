@@ -122,6 +122,32 @@ struct WaveRange {  // first/last valid diagonal seen by one wave (uniform)
   __device__ __forceinline__ int thi(int lo) const { return null() ? lo - 1 : last; }
 };
 
+// wavefront_compute_edit_exact_prune (wavefront_compute_edit.c:219-278) on
+// the trimmed M range r of an edit-distance wavefront.  The reference's best
+// case |(plen - v) - (tlen - h)| of a diagonal does not depend on the offset:
+// it is |k - kend|.  Its two scans therefore keep [max(lo, kend - W),
+// min(hi, kend + W)], W being the least worst case max(plen - v, tlen - h)
+// over the valid pre-extension cells (wmin, uniform).  The prune is tried when
+// the range holds at least kEditPruneWidth diagonals and the middle diagonal's
+// pre-extension offset, recomputed here from the previous wavefront, is valid
+// and bounds both ends' best cases from below.
+template <typename RingPtr>
+__device__ __forceinline__ void edit_prune(WaveRange& r, RingPtr ring, const Src& prev, int wmin, int plen,
+                                           int tlen) {
+  const int lo = r.first, hi = r.last;
+  if (hi - lo + 1 < kEditPruneWidth) return;
+  const int ks = lo + (hi - lo) / 2;
+  int m = max(rd(ring, prev, ks + 1), max(rd(ring, prev, ks - 1), rd(ring, prev, ks)) + 1);
+  if (!in_bounds(m, ks, plen, tlen)) m = kOffsetNull;
+  m = uniform(m);
+  if (m < 0) return;
+  const int kend = tlen - plen;
+  const int smax = max(plen - (m - ks), tlen - m);
+  if (abs(lo - kend) <= smax && abs(hi - kend) <= smax) return;
+  r.first = max(lo, kend - wmin);
+  r.last = min(hi, kend + wmin);
+}
+
 // Orders LDS traffic between the lanes of the (single) wave of a workgroup:
 // a wave executes its LDS instructions in program order, so only the
 // compiler has to be kept from moving accesses across this point.

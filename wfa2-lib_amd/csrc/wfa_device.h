@@ -22,13 +22,25 @@ enum DevStatus : int {
   kDevBadTrace = -2003,       // backtrace found an inconsistent record (bug guard)
 };
 
-// Internal (Eizenga-transformed) penalties plus derived scalars.
+// Distance metrics, numbered as distance_metric_t (wavefront_penalties.h:41-47).
+enum Metric : int { kIndel = 0, kEdit = 1, kLinear = 2, kAffine = 3, kAffine2p = 4 };
+
+// Internal (Eizenga-transformed) penalties plus derived scalars.  The
+// single-component metrics (indel, edit, gap-linear) keep only M: o1 is the
+// cost of one gap symbol and e1 is 0, so an M cell's gap sources sit at
+// s - o1 (indel: no mismatch source; edit: x = o1 = 1).
 struct Penalties {
   int x, o1, e1, o2, e2;   // o2/e2 unused for gap-affine
   int scope;               // max_score_scope
   int g;                   // gcd of all score increments: only multiples of g can be non-null
   int nslots;              // ring slots = (scope-1)/g + 1
+  int metric;              // Metric
+  int ncomp;               // wavefront components: 1 (M), 3 (M, I1, D1) or 5
 };
+
+// Edit distance prunes a wavefront of at least this many diagonals
+// (wavefront_compute_edit.c:219-278).
+constexpr int kEditPruneWidth = 1000;
 
 // Record byte (one per diagonal of every non-null step):
 //   bits 0..2  winning M source: 0 X, 1 D2, 2 D1, 3 I2, 4 I1
@@ -120,7 +132,7 @@ struct LaunchShape {
 };
 
 // Returns hipSuccess or the launch error.  bits: 2 (ACGT packed) or 8 (bytes).
-hipError_t launch_forward(const ForwardArgs& a, int two_piece, int bits,
+hipError_t launch_forward(const ForwardArgs& a, int ncomp, int bits,
                           const LaunchShape& shape, hipStream_t stream);
 hipError_t launch_backtrace(const BacktraceArgs& a, hipStream_t stream);
 // Device counters of one run (one 64-byte block).
@@ -178,7 +190,7 @@ struct BiArgs {
   int max_steps;
   Penalties pen;
 };
-hipError_t launch_breakpoint(const BiArgs& a, int two_piece, int bits, int block, int grid, hipStream_t stream);
+hipError_t launch_breakpoint(const BiArgs& a, int ncomp, int bits, int block, int grid, hipStream_t stream);
 // A piece of a final CIGAR: len operations copied from src (or, when src < 0,
 // len copies of op) to dst.
 struct OpsPiece {
@@ -191,7 +203,7 @@ struct OpsPiece {
 hipError_t launch_gather_ops(const OpsPiece* pieces, int n, const char* src, char* dst, hipStream_t stream);
 hipError_t launch_reverse_pack(const uint64_t* words, const int64_t* src_pos, const int64_t* dst_pos, const int* len,
                                int n, int bits, uint64_t* out, hipStream_t stream);
-int breakpoint_max_blocks_per_cu(int two_piece, int bits, int block);
+int breakpoint_max_blocks_per_cu(int ncomp, int bits, int block);
 
 // Lean family: whether the penalties' slot structure has an instance, its
 // dynamic LDS, occupancy and launch (ForwardArgs as for launch_forward; the
@@ -204,7 +216,7 @@ hipError_t launch_forward_lean(const ForwardArgs& a, const LaunchShape& shape, h
 
 // Dynamic-LDS bytes and occupancy for a configuration.
 size_t forward_lds_bytes(int family, int ncomp, int nslots, int wcap, int lmax_words);
-int forward_max_blocks_per_cu(int family, int block, int two_piece, int bits, size_t lds_bytes);
+int forward_max_blocks_per_cu(int family, int block, int ncomp, int bits, size_t lds_bytes);
 int pick_block(int family, int max_width);
 
 }  // namespace wfa

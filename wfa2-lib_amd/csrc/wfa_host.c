@@ -173,9 +173,39 @@ char* wavefront_align_strerror_short(const int error_code) {
 }
 
 /* ----------------------------------------------------------- CIGAR
- * Scoring follows alignment/cigar.c:277-342; printing :436-470. */
+ * Scoring follows alignment/cigar.c:244-342; printing :436-470. */
 bool cigar_is_null(const cigar_t* const cigar) {
   return cigar->begin_offset >= cigar->end_offset;
+}
+
+static void cigar_unknown_op(void) {
+  fprintf(stderr, "[CIGAR] Computing CIGAR score: Unknown operation\n");
+  exit(1);
+}
+
+/* Edit (and indel) distance: one per non-match operation. */
+int cigar_score_edit(const cigar_t* const cigar) {
+  int score = 0;
+  for (int i = cigar->begin_offset; i < cigar->end_offset; ++i) {
+    const char op = cigar->operations[i];
+    if (op == 'X' || op == 'I' || op == 'D') ++score;
+    else if (op != 'M') cigar_unknown_op();
+  }
+  return score;
+}
+
+int cigar_score_gap_linear(const cigar_t* const cigar, const linear_penalties_t* const pen) {
+  int score = 0;
+  for (int i = cigar->begin_offset; i < cigar->end_offset; ++i) {
+    switch (cigar->operations[i]) {
+      case 'M': score -= pen->match; break;
+      case 'X': score -= pen->mismatch; break;
+      case 'I':
+      case 'D': score -= pen->indel; break;
+      default: cigar_unknown_op();
+    }
+  }
+  return score;
 }
 
 int cigar_score_gap_affine(const cigar_t* const cigar, const affine_penalties_t* const pen) {

@@ -41,7 +41,7 @@ Engine* engine_of(wavefront_aligner_t* wf);
 bool engine_init(Engine* e);
 wfa::Counters* counters_of(Engine* e);
 
-wfa::Penalties make_penalties(const wavefront_penalties_t& p, bool two);
+wfa::Penalties make_penalties(const wavefront_penalties_t& p);
 int classic_score(const wavefront_penalties_t& p, int plen, int tlen, int s);
 
 // A set of alignment jobs on the device: pairs of a batch, or the base-case
@@ -106,7 +106,7 @@ struct RunTotals {
   unsigned int fail = 0;
 };
 
-std::vector<PlanGroup> make_plan(const JobSet& js, const std::vector<int>& ids, const wfa::Penalties& pen, int ncomp);
+std::vector<PlanGroup> make_plan(const JobSet& js, const std::vector<int>& ids, const wfa::Penalties& pen);
 
 // Aligns every job of the plan (retrying the ones that overflow an LDS ring
 // or the record arena); results stay on the device.  h_status is scratch

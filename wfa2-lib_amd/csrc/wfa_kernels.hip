@@ -32,7 +32,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
   // One wave and the ring in LDS: no workgroup barrier is ever needed.
   constexpr bool WAVE_ONLY = SMALL && NW == 1;
   __shared__ int sh_bcast[2];
-  __shared__ int sh_red[NW > 1 ? NW : 1][2 * NC];
+  __shared__ int sh_red[NW > 1 ? NW : 1][2 * NC + 1];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -146,11 +146,13 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
       s += g;
       if (s >= a.max_steps) { status = kDevMaxSteps; break; }
       cur = (cur + 1 == R) ? 0 : cur + 1;
-      const Src mx  = get_src(meta, s - pen.x, cur - dq_x, R, kM, wcap);
+      Src mx{1, -1, kFarDiagonal, 0, 0}, i1 = mx, d1 = mx, mo2 = mx, i2 = mx, d2 = mx;
+      if (NC > 1 || pen.metric != kIndel) mx = get_src(meta, s - pen.x, cur - dq_x, R, kM, wcap);
       const Src mo1 = get_src(meta, s - pen.o1 - pen.e1, cur - dq_o1, R, kM, wcap);
-      const Src i1  = get_src(meta, s - pen.e1, cur - dq_e1, R, kI1, wcap);
-      const Src d1  = get_src(meta, s - pen.e1, cur - dq_e1, R, kD1, wcap);
-      Src mo2{1, -1, kFarDiagonal, 0, 0}, i2 = mo2, d2 = mo2;
+      if constexpr (NC > 1) {
+        i1 = get_src(meta, s - pen.e1, cur - dq_e1, R, kI1, wcap);
+        d1 = get_src(meta, s - pen.e1, cur - dq_e1, R, kD1, wcap);
+      }
       if constexpr (NC == 5) {
         mo2 = get_src(meta, s - pen.o2 - pen.e2, cur - dq_o2, R, kM, wcap);
         i2  = get_src(meta, s - pen.e2, cur - dq_e2, R, kI2, wcap);
@@ -158,16 +160,27 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
       }
       const bool any = mx.valid() || mo1.valid() || i1.valid() || d1.valid() ||
                        mo2.valid() || i2.valid() || d2.valid();
-      if (!any) {  // null step (wavefront_compute_affine.c:236-243)
+      // Indel and edit always extend the previous wavefront, which is never
+      // empty for an end-to-end alignment (wavefront_compute_edit.c:329-373).
+      if (NC == 1 && pen.metric <= kEdit && !mo1.valid()) { status = kDevBadTrace; break; }
+      if (!any) {  // null step (wavefront_compute_affine.c:236-243, wavefront_compute_linear.c:170-176)
         if (s >= s_cap) { status = kDevBadTrace; break; }
         if (lane == cur) { meta.tag = s; meta.flags = 0u; }
         continue;
       }
       // wavefront_compute_limits_input (wavefront_compute.c:40-86)
-      int lo = mx.lim_lo, hi = mx.lim_hi;
-      lo = min(lo, mo1.lim_lo - 1); hi = max(hi, mo1.lim_hi + 1);
-      lo = min(lo, i1.lim_lo + 1);  hi = max(hi, i1.lim_hi + 1);
-      lo = min(lo, d1.lim_lo - 1);  hi = max(hi, d1.lim_hi - 1);
+      int lo, hi;
+      if (NC == 1 && pen.metric <= kEdit) {
+        lo = mo1.lim_lo - 1;
+        hi = mo1.lim_hi + 1;
+      } else {
+        lo = min(mx.lim_lo, mo1.lim_lo - 1);
+        hi = max(mx.lim_hi, mo1.lim_hi + 1);
+      }
+      if constexpr (NC > 1) {
+        lo = min(lo, i1.lim_lo + 1);  hi = max(hi, i1.lim_hi + 1);
+        lo = min(lo, d1.lim_lo - 1);  hi = max(hi, d1.lim_hi - 1);
+      }
       if constexpr (NC == 5) {
         lo = min(lo, mo2.lim_lo - 1); hi = max(hi, mo2.lim_hi + 1);
         lo = min(lo, i2.lim_lo + 1);  hi = max(hi, i2.lim_hi + 1);
@@ -178,9 +191,13 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
       const int width = hi - lo + 1;
       if (width > wcap) { status = kDevRingOverflow; break; }
       dsteps += width;
+      // Edit distance may prune this wavefront: collect the least worst-case
+      // score over its valid pre-extension cells.
+      const bool prune_try = NC == 1 && pen.metric == kEdit && width >= kEditPruneWidth;
+      int wmin = INT_MAX;
       // outputs present (wavefront_compute.c:446-493)
-      const bool has_i1 = mo1.valid() || i1.valid();
-      const bool has_d1 = mo1.valid() || d1.valid();
+      const bool has_i1 = (NC > 1) && (mo1.valid() || i1.valid());
+      const bool has_d1 = (NC > 1) && (mo1.valid() || d1.valid());
       const bool has_i2 = (NC == 5) && (mo2.valid() || i2.valid());
       const bool has_d2 = (NC == 5) && (mo2.valid() || d2.valid());
 
@@ -213,8 +230,8 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
       }
 
       RT* outM  = ring + (cur * NC + kM) * wcap - lo;
-      RT* outI1 = ring + (cur * NC + kI1) * wcap - lo;
-      RT* outD1 = ring + (cur * NC + kD1) * wcap - lo;
+      RT* outI1 = ring + (cur * NC + (NC > 1 ? kI1 : 0)) * wcap - lo;
+      RT* outD1 = ring + (cur * NC + (NC > 1 ? kD1 : 0)) * wcap - lo;
       RT* outI2 = ring + (cur * NC + (NC == 5 ? kI2 : 0)) * wcap - lo;
       RT* outD2 = ring + (cur * NC + (NC == 5 ? kD2 : 0)) * wcap - lo;
 
@@ -223,7 +240,28 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
       for (int kb = lo; kb <= hi; kb += BLOCK) {
         const int k = kb + tid;
         bool vM = false, vI1 = false, vD1 = false, vI2 = false, vD2 = false;
-        if (k <= hi) {
+        if constexpr (NC == 1) {
+          if (k <= hi) {
+            // wavefront_compute_{indel,edit}_idm (wavefront_compute_edit.c:44-102)
+            // and wavefront_compute_linear_idm (wavefront_compute_linear.c:44-76);
+            // argmax with ties to the higher backtrace type: X > D > I
+            // (wavefront_backtrace.c:223-319)
+            const int del = rd(ring, mo1, k + 1);
+            const int ins = rd(ring, mo1, k - 1) + 1;
+            int m = rd(ring, mx, k) + 1;  // NULL + 1 for indel (mx absent)
+            uint32_t src = kSrcX;
+            if (del > m) { m = del; src = kSrcD1; }
+            if (ins > m) { m = ins; src = kSrcI1; }
+            if (!in_bounds(m, k, plen, tlen)) m = kOffsetNull;
+            if (rec) rec[k - lo] = (uint8_t)src;
+            if (m != kOffsetNull) {
+              if (prune_try) wmin = min(wmin, max(plen - (m - k), tlen - m));
+              m = lcp_extend<BITS>(P, pb, T, tb, plen, tlen, k, m);
+            }
+            outM[k] = to_ring(m, ring);
+            vM = (m != kOffsetNull);
+          }
+        } else if (k <= hi) {
           const int om_l = rd(ring, mo1, k - 1);
           const int om_r = rd(ring, mo1, k + 1);
           const int ei1 = rd(ring, i1, k - 1);
@@ -263,18 +301,27 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
         }
         const int kw = kb + wid * 64;
         rM.add(__ballot(vM), kw);
-        rI1.add(__ballot(vI1), kw);
-        rD1.add(__ballot(vD1), kw);
+        if constexpr (NC > 1) {
+          rI1.add(__ballot(vI1), kw);
+          rD1.add(__ballot(vD1), kw);
+        }
         if constexpr (NC == 5) {
           rI2.add(__ballot(vI2), kw);
           rD2.add(__ballot(vD2), kw);
         }
       }
+      if (prune_try) {
+        for (int o = 32; o > 0; o >>= 1) wmin = min(wmin, __shfl_xor(wmin, o, 64));
+        wmin = uniform(wmin);
+      }
       if constexpr (NW > 1) {
         if (lane == 0) {
           sh_red[wid][0] = rM.first;  sh_red[wid][1] = rM.last;
-          sh_red[wid][2] = rI1.first; sh_red[wid][3] = rI1.last;
-          sh_red[wid][4] = rD1.first; sh_red[wid][5] = rD1.last;
+          if constexpr (NC == 1) sh_red[wid][2] = wmin;
+          if constexpr (NC > 1) {
+            sh_red[wid][2] = rI1.first; sh_red[wid][3] = rI1.last;
+            sh_red[wid][4] = rD1.first; sh_red[wid][5] = rD1.last;
+          }
           if constexpr (NC == 5) {
             sh_red[wid][6] = rI2.first; sh_red[wid][7] = rI2.last;
             sh_red[wid][8] = rD2.first; sh_red[wid][9] = rD2.last;
@@ -285,8 +332,11 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
           rM.merge(sh_red[w][0], sh_red[w][1]);
-          rI1.merge(sh_red[w][2], sh_red[w][3]);
-          rD1.merge(sh_red[w][4], sh_red[w][5]);
+          if constexpr (NC == 1) wmin = min(wmin, sh_red[w][2]);
+          if constexpr (NC > 1) {
+            rI1.merge(sh_red[w][2], sh_red[w][3]);
+            rD1.merge(sh_red[w][4], sh_red[w][5]);
+          }
           if constexpr (NC == 5) {
             rI2.merge(sh_red[w][6], sh_red[w][7]);
             rD2.merge(sh_red[w][8], sh_red[w][9]);
@@ -294,6 +344,9 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
         }
       }
       block_sync();  // ring values of this step visible to every lane; sh_red free again
+      if constexpr (NC == 1) {
+        if (prune_try && !rM.null()) edit_prune(rM, ring, mo1, wmin, plen, tlen);
+      }
       // Record the step in the slot registers (every wave keeps a copy).
       if (lane == cur) {
         meta.tag = s;
@@ -301,8 +354,10 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
         unsigned fl = 1u | (rM.null() ? 0x100u : 0u);
         meta.lo[kM] = rM.tlo(lo);
         meta.hi[kM] = rM.thi(lo);
-        if (has_i1) { fl |= 2u | (rI1.null() ? 0x200u : 0u); meta.lo[kI1] = rI1.tlo(lo); meta.hi[kI1] = rI1.thi(lo); }
-        if (has_d1) { fl |= 4u | (rD1.null() ? 0x400u : 0u); meta.lo[kD1] = rD1.tlo(lo); meta.hi[kD1] = rD1.thi(lo); }
+        if constexpr (NC > 1) {
+          if (has_i1) { fl |= 2u | (rI1.null() ? 0x200u : 0u); meta.lo[kI1] = rI1.tlo(lo); meta.hi[kI1] = rI1.thi(lo); }
+          if (has_d1) { fl |= 4u | (rD1.null() ? 0x400u : 0u); meta.lo[kD1] = rD1.tlo(lo); meta.hi[kD1] = rD1.thi(lo); }
+        }
         if constexpr (NC == 5) {
           if (has_i2) { fl |= 8u | (rI2.null() ? 0x800u : 0u); meta.lo[kI2] = rI2.tlo(lo); meta.hi[kI2] = rI2.thi(lo); }
           if (has_d2) { fl |= 16u | (rD2.null() ? 0x1000u : 0u); meta.lo[kD2] = rD2.tlo(lo); meta.hi[kD2] = rD2.thi(lo); }
@@ -312,7 +367,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
       // Termination (wavefront_termination.c:37-114): k_end inside the trimmed
       // range of the end component and its offset reaching the text end.
       {
-        const int ce = cend;
+        const int ce = NC == 1 ? (int)kM : cend;
         bool pce = true;
         WaveRange rce = rM;
         if (ce == kI1) { pce = has_i1; rce = rI1; }
@@ -477,14 +532,17 @@ const void* forward_fn() {
 }
 
 // Every instance the host can select.
-const void* select_forward(int family, int two_piece, int bits, int block) {
-#define WFA_SEL(NC, BL, SM, BI)                                                        \
-  if ((two_piece ? 5 : 3) == NC && block == BL && (family == kSmall) == SM && bits == BI) \
-    return forward_fn<NC, BL, SM, BI>();
-  WFA_SEL(3, 64, true, 2) WFA_SEL(3, 64, true, 8) WFA_SEL(5, 64, true, 2) WFA_SEL(5, 64, true, 8)
-  WFA_SEL(3, 64, false, 2) WFA_SEL(3, 64, false, 8) WFA_SEL(5, 64, false, 2) WFA_SEL(5, 64, false, 8)
-  WFA_SEL(3, 256, false, 2) WFA_SEL(3, 256, false, 8) WFA_SEL(5, 256, false, 2) WFA_SEL(5, 256, false, 8)
-  WFA_SEL(3, 1024, false, 2) WFA_SEL(3, 1024, false, 8) WFA_SEL(5, 1024, false, 2) WFA_SEL(5, 1024, false, 8)
+const void* select_forward(int family, int ncomp, int bits, int block) {
+#define WFA_SEL(NC, BL, SM, BI) \
+  if (ncomp == NC && block == BL && (family == kSmall) == SM && bits == BI) return forward_fn<NC, BL, SM, BI>();
+#define WFA_SEL_NC(BL, SM) \
+  WFA_SEL(1, BL, SM, 2) WFA_SEL(1, BL, SM, 8) WFA_SEL(3, BL, SM, 2) WFA_SEL(3, BL, SM, 8) \
+  WFA_SEL(5, BL, SM, 2) WFA_SEL(5, BL, SM, 8)
+  WFA_SEL_NC(64, true)
+  WFA_SEL_NC(64, false)
+  WFA_SEL_NC(256, false)
+  WFA_SEL_NC(1024, false)
+#undef WFA_SEL_NC
 #undef WFA_SEL
   return nullptr;
 }
@@ -507,17 +565,17 @@ int pick_block(int family, int max_width) {
   return 1024;
 }
 
-int forward_max_blocks_per_cu(int family, int block, int two_piece, int bits, size_t lds_bytes) {
-  const void* f = select_forward(family, two_piece, bits, block);
+int forward_max_blocks_per_cu(int family, int block, int ncomp, int bits, size_t lds_bytes) {
+  const void* f = select_forward(family, ncomp, bits, block);
   if (!f) return 0;
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, lds_bytes) != hipSuccess) return 0;
   return n;
 }
 
-hipError_t launch_forward(const ForwardArgs& a, int two_piece, int bits, const LaunchShape& shape,
+hipError_t launch_forward(const ForwardArgs& a, int ncomp, int bits, const LaunchShape& shape,
                           hipStream_t stream) {
-  const void* f = select_forward(shape.family, two_piece, bits, shape.block);
+  const void* f = select_forward(shape.family, ncomp, bits, shape.block);
   if (!f) return hipErrorInvalidValue;
   if (shape.lds_bytes > 48 * 1024) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shape.lds_bytes);

@@ -284,20 +284,34 @@ class DeviceBatch:
             pass
 
 
+def set_metric(a, metric, penalties):
+    """Distance metric and penalties of attributes `a` (wavefront_aligner_attr_t).
+
+    penalties: affine (match, mismatch, gap_opening, gap_extension); affine2p
+    (match, mismatch, o1, e1, o2, e2); linear (match, mismatch, indel); indel
+    and edit take none (their costs are fixed)."""
+    a.distance_metric = METRICS[metric]
+    if metric == "affine2p":
+        a.affine2p_penalties = affine2p_penalties_t(*tuple(penalties))
+    elif metric == "affine":
+        a.affine_penalties = affine_penalties_t(*tuple(penalties)[:4])
+    elif metric == "linear":
+        a.linear_penalties = linear_penalties_t(*tuple(penalties)[:3])
+
+
 class Aligner:
     """wavefront_aligner_new with attributes built from keyword arguments."""
 
     def __init__(self, metric="affine", penalties=(0, 4, 6, 2), memory="high", score_only=False,
-                 max_steps=0, heuristic_none=True):
+                 max_steps=0, heuristic_none=True, ends_free=None):
         a = default_attributes()
-        a.distance_metric = METRICS[metric]
+        set_metric(a, metric, penalties)
         a.alignment_scope = 0 if score_only else 1
         a.memory_mode = MEMORY[memory]
-        if metric == "affine2p":
-            p = tuple(penalties)
-            a.affine2p_penalties = affine2p_penalties_t(*p)
-        elif metric == "affine":
-            a.affine_penalties = affine_penalties_t(*tuple(penalties)[:4])
+        if ends_free is not None:  # (pattern_begin, pattern_end, text_begin, text_end)
+            a.alignment_form.span = 1  # alignment_endsfree
+            (a.alignment_form.pattern_begin_free, a.alignment_form.pattern_end_free,
+             a.alignment_form.text_begin_free, a.alignment_form.text_end_free) = ends_free
         if heuristic_none:
             a.heuristic.strategy = 0
         if max_steps:
@@ -363,12 +377,8 @@ class Server:
 
     def __init__(self, metric="affine", penalties=(0, 4, 6, 2), **config):
         a = default_attributes()
-        a.distance_metric = METRICS[metric]
+        set_metric(a, metric, penalties)
         a.alignment_scope = 1
-        if metric == "affine2p":
-            a.affine2p_penalties = affine2p_penalties_t(*tuple(penalties))
-        else:
-            a.affine_penalties = affine_penalties_t(*tuple(penalties)[:4])
         a.heuristic.strategy = 0
         cfg = wfa_server_config_t()
         lib().wfa_server_config_default(ctypes.byref(cfg))

@@ -107,3 +107,27 @@ def generate(shape: int, seed: int, n: int, first: int = 0) -> PairSet:
     finally:
         _L().wfa_gen_free(g)
     return PairSet(buf, p_off, plen, t_off, tlen, err, shp)
+
+
+def skewed(seed: int, n: int, tlen=(600, 1200), insert=(1200, 2400), sub=0.03, unrelated_every=4) -> PairSet:
+    """Pairs of very different lengths: the pattern is the text with one long
+    random insertion and `sub` substitutions; every `unrelated_every`-th pair
+    is two unrelated random sequences of the same lengths.  Their wavefronts
+    grow wider than 1000 diagonals far from the end diagonal, which is where
+    edit distance prunes (wavefront_compute_edit.c:219-278)."""
+    rng = np.random.default_rng(seed)
+    alphabet = np.frombuffer(b"ACGT", dtype=np.uint8)
+    pats, texts = [], []
+    for i in range(n):
+        t = alphabet[rng.integers(0, 4, int(rng.integers(tlen[0], tlen[1] + 1)))]
+        ins = alphabet[rng.integers(0, 4, int(rng.integers(insert[0], insert[1] + 1)))]
+        at = int(rng.integers(0, len(t) + 1))
+        p = np.concatenate([t[:at], ins, t[at:]])
+        if unrelated_every and i % unrelated_every == unrelated_every - 1:
+            p = alphabet[rng.integers(0, 4, len(p))]
+        else:
+            hit = rng.random(len(p)) < sub
+            p = np.where(hit, alphabet[(np.searchsorted(alphabet, p) + rng.integers(1, 4, len(p))) % 4], p)
+        pats.append(p.tobytes())
+        texts.append(t.tobytes())
+    return PairSet.from_lists(pats, texts)
