@@ -5,9 +5,10 @@ Two checkers live under oracle/:
 * ``liboracle.so``: the plain-C restatement in ``wfa_oracle.c`` of WFA2-lib
   v2.3 ``wavefront_align`` (gap-affine / gap-affine-2p, memory high and
   BiWFA), which also counts diagonal-steps.
-* ``_ref/libwfa_ref_v{3,4}.so``: the reference itself, compiled from its own
+* ``_ref/libwfa_ref_v{2,3,4}.so``: the reference itself, compiled from its own
   sources by ``oracle/Makefile`` together with ``ref_shim.c`` (one aligner per
-  OpenMP thread, as tools/align_benchmark/align_benchmark.c:504-586).
+  OpenMP thread, as tools/align_benchmark/align_benchmark.c:504-586), for
+  x86-64-v2 (scalar kernels), -v3 (AVX2) and -v4 (AVX-512).
 
 Only tests/, ``__graft_entry__.smoke()`` and the cpu_baseline leg of bench.py
 import this module.  The product library never calls into it.
@@ -26,8 +27,10 @@ REF_DIR = os.path.join(HERE, "_ref")
 
 # Status codes, wavefront/wfa.h:45-55
 WF_STATUS_ALG_COMPLETED = 0
+WF_STATUS_ALG_PARTIAL = 1
 WF_STATUS_MAX_STEPS_REACHED = -100
 WF_STATUS_UNATTAINABLE = -300
+ORACLE_UNSUPPORTED = -400
 
 MEMORY_MODES = {"high": 0, "med": 1, "low": 2, "ultralow": 3}
 
@@ -41,6 +44,17 @@ class Conf:
     memory: str = "high"
     score_only: bool = False
     max_steps: int = 0                # 0 = unlimited
+    ends_free: tuple = None           # (pattern_begin, pattern_end, text_begin, text_end) or None = end-to-end
+    extension: bool = False           # alignment_form.extension (ends-free to both sequence ends + max-trim)
+    heuristic: tuple = None           # (strategy, min_wavefront_length, max_distance_threshold, steps_between_cutoffs)
+
+    @property
+    def form6(self):
+        """{span, pattern_begin, pattern_end, text_begin, text_end, extension} or None."""
+        if self.ends_free is None and not self.extension:
+            return None
+        ef = tuple(self.ends_free) if self.ends_free is not None else (0, 0, 0, 0)
+        return (1,) + ef + (int(self.extension),)
 
     @property
     def metric_id(self) -> int:
@@ -55,13 +69,16 @@ class Conf:
 class _CConf(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
         "metric", "match", "mismatch", "gap_opening1", "gap_extension1",
-        "gap_opening2", "gap_extension2", "memory_mode", "score_only", "max_steps")]
+        "gap_opening2", "gap_extension2", "memory_mode", "score_only", "max_steps",
+        "ends_free", "pattern_begin_free", "pattern_end_free", "text_begin_free", "text_end_free",
+        "extension")]
 
 
 def _cconf(conf: Conf) -> _CConf:
     p = conf.pen6
+    f = conf.form6 or (0, 0, 0, 0, 0, 0)
     return _CConf(conf.metric_id, p[0], p[1], p[2], p[3], p[4], p[5],
-                  MEMORY_MODES[conf.memory], int(conf.score_only), int(conf.max_steps))
+                  MEMORY_MODES[conf.memory], int(conf.score_only), int(conf.max_steps), *f)
 
 
 class Pairs:
@@ -186,33 +203,34 @@ def _cpu_has_avx512() -> bool:
     return " avx512cd" in flags and " avx512vl" in flags
 
 
-_ref_lib = None
+_ref_libs = {}
 
 
-def ref_lib_path() -> str:
-    v = "v4" if _cpu_has_avx512() else "v3"
+def ref_lib_path(isa=None) -> str:
+    """isa: "v2" (scalar kernels), "v3" (AVX2), "v4" (AVX-512); default: the
+    widest this host runs."""
+    v = isa or ("v4" if _cpu_has_avx512() else "v3")
     return os.path.join(REF_DIR, "libwfa_ref_%s.so" % v)
 
 
-def ref_available() -> bool:
-    return os.path.exists(ref_lib_path())
+def ref_available(isa=None) -> bool:
+    return os.path.exists(ref_lib_path(isa))
 
 
-def ref_lib():
-    global _ref_lib
-    if _ref_lib is None:
-        path = ref_lib_path()
+def ref_lib(isa=None):
+    path = ref_lib_path(isa)
+    if path not in _ref_libs:
         if not os.path.exists(path):
             raise RuntimeError("%s is not built (make -C oracle ref)" % path)
         lib = ctypes.CDLL(path)
-        lib.ref_align_batch.restype = ctypes.c_double
+        lib.ref_align_batch_ex.restype = ctypes.c_double
         lib.ref_align_time.restype = ctypes.c_double
-        _ref_lib = lib
-    return _ref_lib
+        _ref_libs[path] = lib
+    return _ref_libs[path]
 
 
-def ref_align(conf: Conf, pairs: Pairs, threads: int = 4) -> Result:
-    lib = ref_lib()
+def ref_align(conf: Conf, pairs: Pairs, threads: int = 4, isa=None) -> Result:
+    lib = ref_lib(isa)
     n = pairs.n
     status = np.zeros(n, np.int32)
     score = np.zeros(n, np.int32)
@@ -220,10 +238,13 @@ def ref_align(conf: Conf, pairs: Pairs, threads: int = 4) -> Result:
     ops_len = np.zeros(n, np.int32)
     ops = ctypes.create_string_buffer(pairs.ops_cap)
     pen = (ctypes.c_int * 6)(*conf.pen6)
-    secs = lib.ref_align_batch(conf.metric_id, pen, MEMORY_MODES[conf.memory], int(conf.score_only),
-                               int(conf.max_steps), int(threads), n, pairs.buf,
-                               _ptr(pairs.p_off), _ptr(pairs.plen), _ptr(pairs.t_off), _ptr(pairs.tlen),
-                               _ptr(status), _ptr(score), _ptr(alg), ops, _ptr(pairs.ops_off), _ptr(ops_len))
+    f6 = conf.form6
+    form = (ctypes.c_int * 6)(*f6) if f6 is not None else None
+    heur = (ctypes.c_int * 4)(*conf.heuristic) if conf.heuristic is not None else None
+    secs = lib.ref_align_batch_ex(conf.metric_id, pen, MEMORY_MODES[conf.memory], int(conf.score_only),
+                                  int(conf.max_steps), form, heur, int(threads), n, pairs.buf,
+                                  _ptr(pairs.p_off), _ptr(pairs.plen), _ptr(pairs.t_off), _ptr(pairs.tlen),
+                                  _ptr(status), _ptr(score), _ptr(alg), ops, _ptr(pairs.ops_off), _ptr(ops_len))
     return Result(n, status, score, ops.raw, pairs.ops_off, ops_len, alg_score=alg, seconds=secs)
 
 

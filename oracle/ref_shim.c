@@ -14,16 +14,36 @@
 #include <omp.h>
 #include "wavefront/wavefront_align.h"
 
-/* Penalty vector layout: {match, mismatch, gap_open1, gap_ext1, gap_open2, gap_ext2}. */
+/* Penalty vector layout: {match, mismatch, gap_open1, gap_ext1, gap_open2, gap_ext2}.
+ * form (may be NULL = end-to-end): {span, pattern_begin_free, pattern_end_free,
+ * text_begin_free, text_end_free, extension}.
+ * heur (may be NULL = none): {strategy, min_wavefront_length,
+ * max_distance_threshold, steps_between_cutoffs}. */
 static void ref_shim_attributes(
     wavefront_aligner_attr_t* const attr,
     const int metric,
     const int* const pen,
     const int memory_mode,
     const int score_only,
-    const int max_steps) {
+    const int max_steps,
+    const int* const form,
+    const int* const heur) {
   *attr = wavefront_aligner_attr_default;
   attr->heuristic.strategy = wf_heuristic_none;
+  if (form != NULL) {
+    attr->alignment_form.span = form[0] ? alignment_endsfree : alignment_end2end;
+    attr->alignment_form.pattern_begin_free = form[1];
+    attr->alignment_form.pattern_end_free = form[2];
+    attr->alignment_form.text_begin_free = form[3];
+    attr->alignment_form.text_end_free = form[4];
+    attr->alignment_form.extension = form[5] != 0;
+  }
+  if (heur != NULL && heur[0] != 0) {
+    attr->heuristic.strategy = (wf_heuristic_strategy)heur[0];
+    attr->heuristic.min_wavefront_length = heur[1];
+    attr->heuristic.max_distance_threshold = heur[2];
+    attr->heuristic.steps_between_cutoffs = heur[3];
+  }
   attr->alignment_scope = score_only ? compute_score : compute_alignment;
   attr->memory_mode = (wavefront_memory_t)memory_mode;
   if (max_steps > 0) attr->system.max_alignment_steps = max_steps;
@@ -62,12 +82,14 @@ static void ref_shim_attributes(
  * ops_off[i] must leave room for plen[i]+tlen[i] bytes.  Returns the wall
  * time in seconds of the parallel alignment loop.
  */
-double ref_align_batch(
+double ref_align_batch_ex(
     const int metric,
     const int* const pen,
     const int memory_mode,
     const int score_only,
     const int max_steps,
+    const int* const form,
+    const int* const heur,
     const int num_threads,
     const int n,
     const char* const buf,
@@ -82,7 +104,7 @@ double ref_align_batch(
     const int64_t* const ops_off,
     int* const ops_len) {
   wavefront_aligner_attr_t attr;
-  ref_shim_attributes(&attr,metric,pen,memory_mode,score_only,max_steps);
+  ref_shim_attributes(&attr,metric,pen,memory_mode,score_only,max_steps,form,heur);
   const int nt = (num_threads > 0) ? num_threads : 1;
   wavefront_aligner_t** aligners = calloc(nt,sizeof(wavefront_aligner_t*));
   int t;
@@ -113,6 +135,30 @@ double ref_align_batch(
   return t1 - t0;
 }
 
+/* End-to-end, heuristic none. */
+double ref_align_batch(
+    const int metric,
+    const int* const pen,
+    const int memory_mode,
+    const int score_only,
+    const int max_steps,
+    const int num_threads,
+    const int n,
+    const char* const buf,
+    const int64_t* const p_off,
+    const int* const plen,
+    const int64_t* const t_off,
+    const int* const tlen,
+    int* const status,
+    int* const score,
+    int* const alg_score,
+    char* const ops_out,
+    const int64_t* const ops_off,
+    int* const ops_len) {
+  return ref_align_batch_ex(metric,pen,memory_mode,score_only,max_steps,NULL,NULL,num_threads,
+      n,buf,p_off,plen,t_off,tlen,status,score,alg_score,ops_out,ops_off,ops_len);
+}
+
 /* Time-only variant for the CPU baseline: aligns and discards the results. */
 double ref_align_time(
     const int metric,
@@ -127,7 +173,7 @@ double ref_align_time(
     const int* const tlen,
     int64_t* const checksum) {
   wavefront_aligner_attr_t attr;
-  ref_shim_attributes(&attr,metric,pen,memory_mode,0,0);
+  ref_shim_attributes(&attr,metric,pen,memory_mode,0,0,NULL,NULL);
   const int nt = (num_threads > 0) ? num_threads : 1;
   wavefront_aligner_t** aligners = calloc(nt,sizeof(wavefront_aligner_t*));
   int t;

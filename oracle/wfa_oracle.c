@@ -84,6 +84,11 @@ typedef struct {
   int64_t dsteps;
   int status;         /* ST_OK while running, ST_END_REACHED at the end */
   int end_score;
+  /* alignment form (wavefront_attributes.h:63-72); extension sets the ends
+     free up to both sequence ends (wavefront_align.c:57-63) */
+  bool endsfree;
+  int pbf, pef, tbf, tef;
+  int end_k, end_off; /* end cell (wf_aligner->alignment_end_pos) */
   /* CIGAR, written backwards by the backtrace into ops[begin..end) */
   char* ops; int ops_cap; int ops_begin; int ops_end;
   int cigar_score;
@@ -307,7 +312,7 @@ static void compute_linear(oaln_t* a, int s) {
     }
     a->dsteps += hi - lo + 1;
     trim_ends(a, out);
-    if (a->metric == M_EDIT) edit_exact_prune(a, out);
+    if (a->metric == M_EDIT && !a->endsfree) edit_exact_prune(a, out);  /* end-to-end only, :370-373 */
     return;
   }
   owf_t* mx = wf_src(a, C_M, s - a->x);
@@ -412,9 +417,40 @@ static bool termination(oaln_t* a, owf_t* m, int s) {
   return true;
 }
 
+/* wavefront_termination_endsfree, wavefront_termination.c:115-162 */
+static bool termination_endsfree(oaln_t* a, int s, int k, int32_t off) {
+  const int h = off, v = off - k;
+  const bool done = (h >= a->tlen && a->plen - v <= a->pef) || (v >= a->plen && a->tlen - h <= a->tef);
+  if (!done) return false;
+  a->status = ST_END_REACHED;
+  a->end_score = s;
+  a->end_k = k;
+  a->end_off = off;
+  return true;
+}
+
+/* wavefront_extend_endsfree, wavefront_extend.c:212-330, with the scalar
+   kernel wavefront_extend_matches_packed_endsfree (wavefront_extend_kernels.c:
+   154-196): diagonals in ascending order, each extended and then tested; the
+   first that ends the alignment stops the extension. */
+static int extend_endsfree(oaln_t* a, int s) {
+  owf_t* m = wf_at(a, C_M, s);
+  if (!m) return 0;
+  int k;
+  for (k = m->lo; k <= m->hi; ++k) {
+    const int32_t off = wf_raw(m, k);
+    if (off == ONULL) continue;
+    const int32_t ext = lcp_extend(a, k, off);
+    m->v[k - m->alo] = ext;
+    if (termination_endsfree(a, s, k, ext)) return 1;
+  }
+  return 0;
+}
+
 /* wavefront_extend_end2end{,_max}, wavefront_extend.c:86-211.  Returns 1 when
    the end is reached.  max_ak (if given) receives the max antidiagonal. */
 static int extend(oaln_t* a, int s, int* max_ak) {
+  if (a->endsfree) return extend_endsfree(a, s);
   if (max_ak) *max_ak = 0;
   owf_t* m = wf_at(a, C_M, s);
   if (!m) return 0;
@@ -428,7 +464,11 @@ static int extend(oaln_t* a, int s, int* max_ak) {
     const int32_t ak = 2 * ext - k;
     if (best < ak) best = ak;
   }
-  if (termination(a, m, s)) return 1;
+  if (termination(a, m, s)) {
+    a->end_k = a->tlen - a->plen;
+    a->end_off = a->tlen;
+    return 1;
+  }
   if (max_ak) *max_ak = best;
   return 0;
 }
@@ -445,6 +485,15 @@ static void aln_init(oaln_t* a, int comp_begin, int comp_end) {
   owf_t* w[NCOMP];
   int c;
   for (c = 0; c < NCOMP; ++c) w[c] = wf_claim(a, c, 0);
+  if (a->endsfree && a->match == 0 && comp_begin == C_M) {
+    /* free beginnings: the score-0 wavefront holds (v=0, h) for h <= text
+       begin-free and (v, h=0) for v <= pattern begin-free, untrimmed
+       (wavefront_aligner_init_wf_m, wavefront_aligner.c:252-307) */
+    int k;
+    wf_alloc(w[C_M], -a->pbf, a->tbf);
+    for (k = -a->pbf; k <= a->tbf; ++k) w[C_M]->v[k + a->pbf] = (k > 0) ? k : 0;
+    return;
+  }
   wf_alloc(w[comp_begin], 0, 0);
   w[comp_begin]->v[0] = 0;
 }
@@ -584,8 +633,14 @@ static int unialign(oaln_t* a, bool score_only) {
   a->cigar_score = INT32_MIN;
   while (1) {
     if (extend(a, s, NULL)) {
-      if (!score_only) backtrace(a, s, a->tlen - a->plen, a->tlen);
-      a->cigar_score = classic_score(a, a->plen, a->tlen, s);
+      if (score_only) {
+        /* compute_score: the score is taken at the full lengths (:159-166) */
+        a->cigar_score = classic_score(a, a->plen, a->tlen, s);
+        return ST_ALG_COMPLETED;
+      }
+      backtrace(a, s, a->end_k, a->end_off);
+      const int end_v = a->end_off - a->end_k, end_h = a->end_off;
+      a->cigar_score = classic_score(a, end_v, end_h, s);
       return ST_ALG_COMPLETED;
     }
     ++s;
@@ -879,24 +934,110 @@ static int bialign(const oracle_conf_t* conf, const char* P, int plen, const cha
   return status;
 }
 
+/* ---------------------------------------------------------------- maxtrim */
+
+/* cigar_maxtrim_gap_{linear,affine,affine2p} (alignment/cigar_utils.c:111-278):
+   keeps the prefix of the CIGAR with the highest score, a match scoring
+   -match (1 when match is 0).  Returns whether the CIGAR was shortened; *keep
+   receives the kept length and *best the kept prefix's score (INT32_MIN when
+   the best score is 0 and the CIGAR is cleared); both are left as they are
+   when the CIGAR is not examined: indel and edit are never trimmed
+   (wavefront_aligner.c:664-676), and affine2p skips an empty CIGAR. */
+static bool maxtrim(const oracle_conf_t* c, const char* ops, int n, int* keep, int* best) {
+  *keep = n;
+  if (c->metric <= M_EDIT) return false;
+  const int mscore = (c->match != 0) ? c->match : -1;
+  int max_score = 0, max_i = 0, score = 0, i;
+  if (c->metric == 4) {
+    if (n <= 0) return false;
+    char last = 0;
+    int run = 0;
+    for (i = 0; i <= n; ++i) {
+      const char op = (i < n) ? ops[i] : 0;
+      if (i == n || (op != last && last != 0)) {
+        int cost;
+        switch (last) {
+          case 'M': cost = mscore * run; break;
+          case 'X': cost = c->mismatch * run; break;
+          default: cost = imin(c->gap_opening1 + c->gap_extension1 * run, c->gap_opening2 + c->gap_extension2 * run);
+        }
+        score -= cost;
+        if (max_score < score) { max_score = score; max_i = i - 1; }
+        run = 0;
+      }
+      last = op;
+      ++run;
+    }
+  } else {
+    char last = 0;
+    for (i = 0; i < n; ++i) {
+      switch (ops[i]) {
+        case 'M': score -= mscore; break;
+        case 'X': score -= c->mismatch; break;
+        default:
+          if (c->metric == M_LINEAR) score -= c->gap_opening1;
+          else score -= c->gap_extension1 + ((last == ops[i]) ? 0 : c->gap_opening1);
+      }
+      last = ops[i];
+      if (max_score < score) { max_score = score; max_i = i; }
+    }
+  }
+  const bool trimmed = (max_i != n - 1);
+  if (max_score == 0) {
+    *keep = 0;
+    *best = INT32_MIN;
+  } else {
+    *keep = max_i + 1;
+    *best = max_score;
+  }
+  return trimmed;
+}
+
 /* ------------------------------------------------------------------ API */
 
 int oracle_align(const oracle_conf_t* conf, const char* P, int plen, const char* T, int tlen,
                  int* score, char* ops, int* ops_len, int64_t* dsteps, int* end_score) {
   if (end_score) *end_score = -1;
-  if (conf->memory_mode == 3)
+  if (ops_len) *ops_len = 0;
+  const bool endsfree = conf->ends_free || conf->extension;
+  if (conf->memory_mode == 3) {
+    if (endsfree) return ORACLE_UNSUPPORTED;
     return bialign(conf, P, plen, T, tlen, score, ops, ops_len, dsteps, end_score);
+  }
   oaln_t a;
   memset(&a, 0, sizeof(a));
   aln_set_penalties(&a, conf);
   a.modular = conf->score_only ? true : false;
   a.P = P; a.plen = plen; a.T = T; a.tlen = tlen;
+  a.endsfree = endsfree;
+  if (conf->extension) {
+    a.pbf = 0; a.pef = plen; a.tbf = 0; a.tef = tlen;
+  } else if (endsfree) {
+    a.pbf = conf->pattern_begin_free; a.pef = conf->pattern_end_free;
+    a.tbf = conf->text_begin_free; a.tef = conf->text_end_free;
+    if (a.pbf > plen || a.pef > plen || a.tbf > tlen || a.tef > tlen) return ORACLE_UNSUPPORTED;
+    if (a.match != 0 && (a.pbf > 0 || a.tbf > 0)) return ORACLE_UNSUPPORTED;
+    /* indel / edit with a free beginning: the reference's backtrace sees the
+       score-0 wavefront only at k = 0 (its initialised range stays [0,0]:
+       wavefront.c:89-109, wavefront_aligner.c:252-307, and
+       wavefront_compute_edit.c:329-373 never widens it), and its CIGARs then
+       hold 'M' on mismatching bases; not restated */
+    if (a.metric <= M_EDIT && (a.pbf > 0 || a.tbf > 0)) return ORACLE_UNSUPPORTED;
+  }
   aln_init(&a, C_M, C_M);
-  const int st = unialign(&a, conf->score_only != 0);
+  int st = unialign(&a, conf->score_only != 0);
   *score = a.cigar_score;
-  if (ops_len) *ops_len = 0;
   if (st == ST_ALG_COMPLETED && !conf->score_only && ops && ops_len) {
-    const int n = a.ops_end - a.ops_begin;
+    int n = a.ops_end - a.ops_begin;
+    if (conf->extension) {
+      /* wavefront_unialign_terminate, wavefront_unialign.c:188-236: the
+         backtrace leaves the raw score; max-trim replaces it */
+      int keep = n, best = a.end_score;
+      const bool trimmed = maxtrim(conf, a.ops + a.ops_begin, n, &keep, &best);
+      *score = best;
+      n = keep;
+      if (trimmed) st = 1;  /* WF_STATUS_ALG_PARTIAL */
+    }
     memcpy(ops, a.ops + a.ops_begin, n);
     *ops_len = n;
   }

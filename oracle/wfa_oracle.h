@@ -2,10 +2,12 @@
  * CPU oracle for the exact WFA hot path (TEST INFRASTRUCTURE ONLY).
  *
  * A plain-C restatement of WFA2-lib v2.3's wavefront_align for indel, edit,
- * gap-linear, gap-affine and dual-cost gap-affine, end-to-end, in memory mode
- * high (and med/low, whose
+ * gap-linear, gap-affine and dual-cost gap-affine, end-to-end and ends-free
+ * (with the extension form), in memory mode high (and med/low, whose
  * CIGARs the reference asserts equal memory-high: tests/wfa.utest.sh:67-78)
- * and ultralow (BiWFA).  It is the checker for the HIP path; only tests/,
+ * and ultralow (BiWFA, end-to-end).  Ends-free termination takes the first
+ * diagonal in ascending order, as the reference's scalar extend kernel
+ * (wavefront_extend_kernels.c:154-196).  It is the checker for the HIP path; only tests/,
  * __graft_entry__.smoke() and the cpu_baseline leg of bench.py may load it.
  *
  * Pinning: every function here is checked against the reference's golden
@@ -31,7 +33,17 @@ typedef struct {
   int memory_mode;     /* 0 high, 1 med, 2 low, 3 ultralow (wavefront_attributes.h:104-109) */
   int score_only;      /* alignment_scope == compute_score */
   int max_steps;       /* <= 0 means unlimited (INT_MAX) */
+  int ends_free;       /* alignment_form.span == alignment_endsfree */
+  int pattern_begin_free, pattern_end_free, text_begin_free, text_end_free;
+  int extension;       /* alignment_form.extension (wavefront_align.c:57-63) */
 } oracle_conf_t;
+
+/* Status returned for a configuration this restatement does not cover:
+   ends-free with a free beginning and match < 0 (wavefront_compute.c:124-262),
+   indel / edit with a free beginning (see oracle_align), or ends-free /
+   extension with BiWFA, which the reference rejects with exit(1)
+   (wavefront_align.c:66-80). */
+#define ORACLE_UNSUPPORTED (-400)
 
 /*
  * Align one pair.  Returns the reference's status code (wfa.h:45-55).

@@ -214,3 +214,139 @@ def test_oracle_counts_diagonal_steps():
     r = O.oracle_align(conf, _pairs([t, t[:50] + b"T" + t[51:]], [t, t]))
     assert r.dsteps[0] == 0
     assert r.dsteps[1] > 0 and r.end_score[1] == 4
+
+
+# ------------------------------------------------------------- ends-free
+
+def _semiglobal_pairs(seed, n, flank=30):
+    """Reads with random flanks on one side or the other (seeded), so that
+    free ends of either sequence matter."""
+    from wfa_amd import datagen
+    import numpy as np
+    ps = datagen.generate(1, seed, n)
+    rng = np.random.default_rng(seed)
+    pats, texts = [], []
+    for i in range(ps.n):
+        p, t = ps.pattern(i), ps.text(i)
+        pre = bytes(rng.choice(list(b"ACGT"), int(rng.integers(0, flank))))
+        suf = bytes(rng.choice(list(b"ACGT"), int(rng.integers(0, flank))))
+        if i % 2:
+            pats.append(p); texts.append(pre + t + suf)
+        else:
+            pats.append(pre + p + suf); texts.append(t)
+    return pats, texts
+
+
+# (metric, penalties, form): every ends-free form the device path accepts
+ENDSFREE_SETS = [
+    ("affine", (0, 4, 6, 2), dict(ends_free=(0, 0, 30, 30))),
+    ("affine", (0, 4, 6, 2), dict(ends_free=(30, 30, 0, 0))),
+    ("affine", (0, 4, 6, 2), dict(ends_free=(10, 10, 10, 10))),
+    ("affine", (-2, 4, 6, 2), dict(ends_free=(0, 30, 0, 30))),
+    ("affine", (0, 4, 6, 2), dict(extension=True)),
+    ("affine", (-1, 3, 5, 2), dict(extension=True)),
+    ("affine2p", (0, 4, 6, 2, 24, 1), dict(ends_free=(20, 20, 20, 20))),
+    ("affine2p", (0, 4, 6, 2, 24, 1), dict(extension=True)),
+    ("linear", (0, 4, 2), dict(ends_free=(30, 30, 30, 30))),
+    ("linear", (-2, 3, 4), dict(extension=True)),
+    ("edit", (0, 1, 1), dict(ends_free=(0, 30, 0, 30))),
+    ("edit", (0, 1, 1), dict(extension=True)),
+    ("indel", (0, 1, 1), dict(ends_free=(0, 0, 0, 30))),
+]
+
+
+@pytest.mark.skipif(not O.ref_available("v2"), reason="scalar reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("metric,pen,form", ENDSFREE_SETS)
+def test_oracle_endsfree_matches_reference_build(metric, pen, form):
+    """Against the reference's scalar build (x86-64-v2): its ends-free
+    extension tests diagonals in ascending order (wavefront_extend_kernels.c:
+    154-196); the AVX2 / AVX-512 kernels test the diagonals whose first four
+    symbols match before the others (wavefront_extend_kernels_avx.c:296-392),
+    so they can end on another diagonal of the same score."""
+    pats, texts = _semiglobal_pairs(31, 1500)
+    pairs = _pairs(pats, texts)
+    # indel / edit extension in memory med reports score INT32_MIN (its
+    # piggyback backtrace leaves cigar->score unset); the device path refuses it
+    mems = ("high",) if metric in ("edit", "indel") and form.get("extension") else ("high", "med")
+    for mem in mems:
+        for so in (False, True):
+            conf = O.Conf(metric, pen, mem, score_only=so, **form)
+            a = O.oracle_align(conf, pairs)
+            b = O.ref_align(conf, pairs, isa="v2")
+            assert list(a.status) == list(b.status), (mem, so)
+            assert list(a.score) == list(b.score), (mem, so)
+            for i in range(pairs.n):
+                assert a.ops(i) == b.ops(i), (mem, so, i)
+
+
+@pytest.mark.parametrize("metric,pen,gpen,form", [
+    ("affine", (0, 4, 6, 2), (0, 4, 6, 2), (0, 0, 5, 5)),
+    ("affine", (0, 4, 6, 2), (0, 4, 6, 2), (5, 5, 0, 0)),
+    ("affine", (0, 4, 6, 2), (0, 4, 6, 2), (3, 4, 2, 6)),
+    ("affine2p", (0, 4, 6, 2, 24, 1), (0, 4, 6, 2, 24, 1), (4, 4, 4, 4)),
+    ("linear", (0, 4, 2), (0, 4, 0, 2), (4, 4, 4, 4)),
+    ("edit", (0, 1, 1), (0, 1, 0, 1), (0, 5, 0, 5)),
+    ("indel", (0, 1, 1), (0, 2, 0, 1), (0, 4, 0, 6)),
+])
+def test_oracle_endsfree_score_is_dp_optimum(metric, pen, gpen, form):
+    """Ends-free scores equal a semi-global DP's optimum; the CIGAR is valid
+    and, without the gaps its free ends absorb, re-scores to it."""
+    pats, texts = _random_pairs(hash((metric, form)) & 0xFFFF, 80, 0, 30, 6, max_indel=4)
+    r = O.oracle_align(O.Conf(metric, pen, "high", ends_free=form), _pairs(pats, texts))
+    sign = -1 if metric in ("edit", "indel") else 1
+    for i, (p, t) in enumerate(zip(pats, texts)):
+        if r.status[i] == O.ORACLE_UNSUPPORTED:
+            assert form[0] > len(p) or form[1] > len(p) or form[2] > len(t) or form[3] > len(t)
+            continue
+        assert r.status[i] == 0
+        best = dp.gotoh_endsfree(p, t, *gpen, pbf=form[0], pef=form[1], tbf=form[2], tef=form[3])
+        assert r.score[i] == sign * best, (p, t, r.ops(i))
+        ops = r.ops(i)
+        assert dp.cigar_valid(p, t, ops), (p, t, ops)
+        assert dp.cigar_score(dp.strip_free_ends(ops, *form), *gpen) == best, (p, t, ops)
+
+
+@pytest.mark.parametrize("metric,pen", [("affine", (0, 4, 6, 2)), ("affine", (-1, 3, 5, 2)),
+                                        ("affine2p", (0, 4, 6, 2, 24, 1)), ("linear", (-2, 3, 4)),
+                                        ("linear", (0, 4, 2))])
+def test_oracle_extension_is_maxtrimmed_endsfree(metric, pen):
+    """Extension = free ends up to both sequence ends (wavefront_align.c:57-63)
+    then the best-scoring CIGAR prefix (alignment/cigar_utils.c:111-278),
+    status partial when it was cut."""
+    pats, texts = _semiglobal_pairs(32, 400)
+    pairs = _pairs(pats, texts)
+    ext = O.oracle_align(O.Conf(metric, pen, "high", extension=True), pairs)
+    trimmed_any = False
+    for i in range(pairs.n):
+        full = O.oracle_align(O.Conf(metric, pen, "high",
+                                     ends_free=(0, len(pats[i]), 0, len(texts[i]))), _pairs([pats[i]], [texts[i]]))
+        want_ops, want_score, trimmed = dp.maxtrim(full.ops(0), metric, *pen)
+        assert ext.ops(i) == want_ops, i
+        assert ext.status[i] == (O.WF_STATUS_ALG_PARTIAL if trimmed else 0), i
+        assert ext.score[i] == (want_score if want_score is not None else -2**31), i
+        trimmed_any |= trimmed
+    assert trimmed_any
+
+
+def test_oracle_endsfree_closed_forms():
+    t = b"GATTACAGATTACACCGGTTAACC"
+    inner = t[5:19]
+    conf = O.Conf("affine", (0, 4, 6, 2), ends_free=(0, 0, 5, 5))
+    r = O.oracle_align(conf, _pairs([inner], [t]))
+    assert r.status[0] == 0 and r.score[0] == 0
+    assert r.ops(0) == b"I" * 5 + b"M" * len(inner) + b"I" * 5
+    # one free symbol short at the end: the last insertion is charged o + e
+    r = O.oracle_align(O.Conf("affine", (0, 4, 6, 2), ends_free=(0, 0, 5, 4)), _pairs([inner], [t]))
+    assert r.score[0] == -(6 + 2)
+    # the pattern's free ends
+    r = O.oracle_align(O.Conf("edit", (0, 1, 1), ends_free=(0, 5, 0, 0)), _pairs([t], [t[:-5]]))
+    assert r.score[0] == 0 and r.ops(0) == b"M" * (len(t) - 5) + b"D" * 5
+
+
+def test_oracle_refuses_what_it_does_not_restate():
+    pairs = _pairs([b"ACGTACGT"], [b"TTACGTACGTTT"])
+    for conf in (O.Conf("edit", (0, 1, 1), ends_free=(0, 0, 2, 2)),          # free beginning, edit
+                 O.Conf("affine", (-1, 4, 6, 2), ends_free=(0, 0, 2, 2)),    # free beginning, match < 0
+                 O.Conf("affine", (0, 4, 6, 2), "ultralow", ends_free=(0, 0, 2, 2)),
+                 O.Conf("affine", (0, 4, 6, 2), "ultralow", extension=True)):
+        assert O.oracle_align(conf, pairs).status[0] == O.ORACLE_UNSUPPORTED
