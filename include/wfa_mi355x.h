@@ -11,11 +11,18 @@
  *
  * Exact path (bit-identical scores and CIGARs to the reference):
  *   distance_metric indel / edit / gap_linear / gap_affine / gap_affine_2p,
- *   alignment_end2end, heuristic wf_heuristic_none, memory modes high / med /
- *   low / ultralow, compute_score or compute_alignment.
- * Configurations outside it (ends-free/extension, heuristics, lambda
- * sequences) are rejected with WF_STATUS_UNSUPPORTED and a "[WFA]" message on
- * stderr; they never silently produce different results.
+ *   heuristic wf_heuristic_none, compute_score or compute_alignment;
+ *   alignment_end2end in memory modes high / med / low / ultralow;
+ *   alignment_endsfree and the extension form in memory modes high / med /
+ *   low (the reference rejects them in ultralow with exit(1), and so does
+ *   this library), ending on the first diagonal in ascending order as the
+ *   reference's scalar extend kernel does.
+ * Configurations outside it are rejected with WF_STATUS_UNSUPPORTED and a
+ * "[WFA]" message on stderr; they never silently produce different results:
+ * heuristics, lambda sequences, free beginnings with a match bonus or with
+ * indel/edit distances, indel/edit extension outside memory mode high.
+ * Configuration errors (invalid penalties, ends-free lengths larger than the
+ * sequences) print the reference's message and exit(1) as it does.
  *
  * Not thread-safe per aligner (as the reference, wavefront_aligner.c); one
  * aligner per host thread, any number of aligners per device.

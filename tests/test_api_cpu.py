@@ -37,7 +37,10 @@ def test_default_attributes_match_reference():
 
 
 @pytest.mark.parametrize("kw", [
-    dict(metric="edit", ends_free=(0, 0, 5, 5)),
+    dict(metric="edit", ends_free=(0, 0, 2, 2)),                       # free beginning, edit
+    dict(metric="affine", penalties=(-1, 4, 6, 2), ends_free=(2, 0, 0, 0)),  # free beginning, match < 0
+    dict(metric="affine", memory="ultralow", ends_free=(0, 0, 0, 0)),  # ends-free span in BiWFA
+    dict(metric="edit", memory="med", extension=True),                 # indel/edit extension outside high
     dict(metric="affine", heuristic_none=False),
 ])
 def test_unsupported_configurations_are_refused(kw):
@@ -89,3 +92,30 @@ def test_cigar_scores_of_every_metric_match_oracle():
         a2 = wfa_amd.affine2p_penalties_t(0, 4, 6, 2, 24, 1)
         assert L.cigar_score_gap_affine2p(ref, ctypes.byref(a2)) == O.cigar_score(
             O.Conf("affine2p", (0, 4, 6, 2, 24, 1)), ops)
+
+
+EXIT_CHILD = r"""
+import sys
+sys.path.insert(0, %r)
+import wfa_amd
+al = wfa_amd.Aligner(**%r)
+al.align(b"ACGTACGT", b"ACGTTACGT")
+print("returned")
+"""
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(metric="affine", memory="ultralow", ends_free=(0, 2, 0, 2)), b"BiWFA ends-free has not been tested"),
+    (dict(metric="affine", memory="ultralow", extension=True), b"BiWFA ends-free has not been tested"),
+    (dict(metric="affine", ends_free=(0, 0, 20, 0)), b"Ends-free parameters must be not larger"),
+])
+def test_configuration_errors_exit_like_the_reference(kw, msg):
+    """wavefront_align_presets__checks (wavefront_align.c:46-106) ends the
+    process with exit(1) and a message; so does the drop-in, before any
+    device work."""
+    import subprocess
+    import sys
+    code = EXIT_CHILD % (os.path.join(ROOT, "wfa2-lib_amd"), kw)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, timeout=120)
+    assert r.returncode == 1 and b"returned" not in r.stdout
+    assert msg in r.stderr

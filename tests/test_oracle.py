@@ -219,22 +219,9 @@ def test_oracle_counts_diagonal_steps():
 # ------------------------------------------------------------- ends-free
 
 def _semiglobal_pairs(seed, n, flank=30):
-    """Reads with random flanks on one side or the other (seeded), so that
-    free ends of either sequence matter."""
     from wfa_amd import datagen
-    import numpy as np
-    ps = datagen.generate(1, seed, n)
-    rng = np.random.default_rng(seed)
-    pats, texts = [], []
-    for i in range(ps.n):
-        p, t = ps.pattern(i), ps.text(i)
-        pre = bytes(rng.choice(list(b"ACGT"), int(rng.integers(0, flank))))
-        suf = bytes(rng.choice(list(b"ACGT"), int(rng.integers(0, flank))))
-        if i % 2:
-            pats.append(p); texts.append(pre + t + suf)
-        else:
-            pats.append(pre + p + suf); texts.append(t)
-    return pats, texts
+    ps = datagen.semiglobal(1, seed, n, flank)
+    return ps.patterns(), ps.texts()
 
 
 # (metric, penalties, form): every ends-free form the device path accepts

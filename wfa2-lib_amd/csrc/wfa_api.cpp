@@ -302,10 +302,27 @@ namespace {
 // Configurations the exact device path covers; everything else is refused.
 int check_supported(wavefront_aligner_t* wf) {
   const distance_metric_t m = wf->penalties.distance_metric;
+  const alignment_form_t& f = wf->alignment_form;
+  const bool endsfree = f.span == alignment_endsfree;
+  const bool begin_free = endsfree && !f.extension && (f.pattern_begin_free > 0 || f.text_begin_free > 0);
   const char* why = nullptr;
   if (m < indel || m > gap_affine_2p) why = "an unknown distance metric";
-  else if (wf->alignment_form.span != alignment_end2end || wf->alignment_form.extension)
-    why = "only end-to-end alignment";
+  else if (endsfree && !f.extension &&
+           (f.pattern_begin_free < 0 || f.pattern_end_free < 0 || f.text_begin_free < 0 || f.text_end_free < 0))
+    why = "negative ends-free lengths";
+  else if (endsfree && wf->memory_mode == wavefront_memory_ultralow)
+    why = "the ends-free form in memory mode ultralow";
+  // A free beginning with a match bonus starts wavefronts at later scores
+  // (wavefront_compute.c:124-262), which the device does not model.
+  else if (begin_free && wf->penalties.match != 0) why = "free beginnings with a match bonus (match < 0)";
+  // Indel / edit with a free beginning: the reference's backtrace sees only
+  // k = 0 of the score-0 wavefront (its initialised range stays [0,0],
+  // wavefront.c:89-109) and writes 'M' over mismatching bases.
+  else if (begin_free && m <= edit) why = "free beginnings with indel/edit distances";
+  // Indel / edit extension in memory med/low leaves the score unset in the
+  // reference (piggyback backtrace); only memory high is defined.
+  else if (f.extension && m <= edit && wf->memory_mode != wavefront_memory_high)
+    why = "indel/edit extension outside memory mode high";
   else if (wf->heuristic.strategy != wf_heuristic_none) why = "only heuristic none (exact alignment)";
   else if (make_penalties(wf->penalties).nslots > 64)
     why = "penalties whose score scope needs more than 64 wavefront slots";
@@ -313,6 +330,129 @@ int check_supported(wavefront_aligner_t* wf) {
   if (wf->system.verbose >= 0)
     fprintf(stderr, "[WFA] Unsupported configuration on the device path: %s\n", why);
   return WF_STATUS_UNSUPPORTED;
+}
+
+// wavefront_align_presets__checks (wavefront_align.c:46-106): configuration
+// errors the reference reports with exit(1), checked for every pair.
+void presets_check(wavefront_aligner_t* wf, const int* plens, const int* tlens, int n) {
+  const alignment_form_t& f = wf->alignment_form;
+  for (int i = 0; i < n; ++i) {
+    const int plen = plens[i], tlen = tlens[i];
+    int pbf = f.pattern_begin_free, pef = f.pattern_end_free, tbf = f.text_begin_free, tef = f.text_end_free;
+    if (f.span == alignment_endsfree && f.extension) { pbf = 0; pef = plen; tbf = 0; tef = tlen; }
+    if (wf->memory_mode == wavefront_memory_ultralow) {
+      if (pbf > 0 || pef > 0 || tbf > 0 || tef > 0) {
+        fprintf(stderr, "[WFA] BiWFA ends-free has not been tested properly yet (let me know and I'll do it)\n");
+        exit(1);
+      }
+      if (f.extension) {
+        fprintf(stderr, "[WFA] BiWFA extension is not implemented yet (let me know and I'll add it)\n");
+        exit(1);
+      }
+    }
+    if (f.span == alignment_endsfree && (pbf > plen || pef > plen || tbf > tlen || tef > tlen)) {
+      fprintf(stderr, "[WFA] Ends-free parameters must be not larger than the sequences "
+              "(P0=%d,Pf=%d,T0=%d,Tf=%d). Must be (P0<=|P|,Pf<=|P|,T0<=|T|,Tf<=|T|) where (|P|,|T|)=(%d,%d)\n",
+              pbf, pef, tbf, tef, plen, tlen);
+      exit(1);
+    }
+  }
+}
+
+void presets_check(wavefront_aligner_t* wf, const wfa_pair_t* pairs, int n) {
+  std::vector<int> pl(n), tl(n);
+  for (int i = 0; i < n; ++i) { pl[i] = pairs[i].pattern_length; tl[i] = pairs[i].text_length; }
+  presets_check(wf, pl.data(), tl.data(), n);
+}
+
+// Ends-free form of a run; extension frees both ends entirely (wavefront_align.c:57-63).
+EndsFree ends_free_of(const wavefront_aligner_t* wf) {
+  EndsFree e;
+  const alignment_form_t& f = wf->alignment_form;
+  if (f.span != alignment_endsfree) return e;
+  e.on = 1;
+  if (f.extension) {
+    e.pef = e.tef = wfa::kFreeAll;
+  } else {
+    e.pbf = f.pattern_begin_free;
+    e.pef = f.pattern_end_free;
+    e.tbf = f.text_begin_free;
+    e.tef = f.text_end_free;
+  }
+  return e;
+}
+
+// cigar_maxtrim_gap_{linear,affine,affine2p} (alignment/cigar_utils.c:111-278):
+// the extension form keeps the prefix of the CIGAR with the best score, a
+// match scoring -match (1 when match is 0).  `examined` is false where the
+// reference leaves the CIGAR as it is (indel / edit, an empty affine2p CIGAR).
+struct Trim {
+  bool examined = false, trimmed = false;
+  int keep = 0, score = INT32_MIN, end_v = -1, end_h = -1;
+};
+Trim maxtrim(const wavefront_penalties_t& p, const char* ops, int n) {
+  Trim t;
+  t.keep = n;
+  const distance_metric_t m = p.distance_metric;
+  if (m <= edit) return t;
+  if (m == gap_affine_2p && n <= 0) return t;
+  t.examined = true;
+  int match = 0, mismatch = 0, o1 = 0, e1 = 0, o2 = 0, e2 = 0;
+  if (m == gap_linear) {
+    match = p.linear_penalties.match; mismatch = p.linear_penalties.mismatch; o1 = p.linear_penalties.indel;
+  } else if (m == gap_affine) {
+    match = p.affine_penalties.match; mismatch = p.affine_penalties.mismatch;
+    o1 = p.affine_penalties.gap_opening; e1 = p.affine_penalties.gap_extension;
+  } else {
+    const affine2p_penalties_t& a = p.affine2p_penalties;
+    match = a.match; mismatch = a.mismatch; o1 = a.gap_opening1; e1 = a.gap_extension1;
+    o2 = a.gap_opening2; e2 = a.gap_extension2;
+  }
+  const int mscore = match != 0 ? match : -1;
+  int best = 0, best_i = 0, best_v = 0, best_h = 0, score = 0, v = 0, h = 0;
+  if (m == gap_affine_2p) {
+    // scored per run of equal operations, compared at run ends
+    int i = 0;
+    while (i < n) {
+      int j = i;
+      while (j < n && ops[j] == ops[i]) ++j;
+      const int L = j - i;
+      switch (ops[i]) {
+        case 'M': score -= mscore * L; v += L; h += L; break;
+        case 'X': score -= mismatch * L; v += L; h += L; break;
+        case 'D': score -= std::min(o1 + e1 * L, o2 + e2 * L); v += L; break;
+        default: score -= std::min(o1 + e1 * L, o2 + e2 * L); h += L; break;
+      }
+      if (best < score) { best = score; best_i = j - 1; best_v = v; best_h = h; }
+      i = j;
+    }
+  } else {
+    char last = 0;
+    for (int i = 0; i < n; ++i) {
+      const char op = ops[i];
+      switch (op) {
+        case 'M': score -= mscore; ++v; ++h; break;
+        case 'X': score -= mismatch; ++v; ++h; break;
+        default: {
+          const int cost = m == gap_linear ? o1 : e1 + (last == op ? 0 : o1);
+          score -= cost;
+          if (op == 'D') ++v; else ++h;
+        }
+      }
+      last = op;
+      if (best < score) { best = score; best_i = i; best_v = v; best_h = h; }
+    }
+  }
+  t.trimmed = best_i != n - 1;
+  if (best == 0) {
+    t.keep = 0;  // cigar_clear: score INT32_MIN, end position (-1, -1)
+  } else {
+    t.keep = best_i + 1;
+    t.score = best;
+    t.end_v = best_v;
+    t.end_h = best_h;
+  }
+  return t;
 }
 
 bool all_acgt(const wfa_pair_t* pairs, int n) {
@@ -467,6 +607,13 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
   fa.comp_begin = wfa::kM;
   fa.comp_end = wfa::kM;
   fa.defer_shift = 8;  // about L^2/16 bytes of records
+  fa.endsfree = js.ef.on;
+  fa.pbf = js.ef.pbf;
+  fa.pef = js.ef.pef;
+  fa.tbf = js.ef.tbf;
+  fa.tef = js.ef.tef;
+  fa.out_end_k = js.end_k;
+  fa.out_end_off = js.end_off;
   fa.pen = pen;
   if (!hip_ok(wfa::launch_reset_counters(c, 0, st), "reset")) return WF_STATUS_OOM;
   (void)hipEventRecord(grp.ev[0], st);
@@ -497,6 +644,9 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
     ba.pat_pos = js.ppos;
     ba.txt_pos = js.tpos;
     ba.bits = js.bits;
+    ba.endsfree = js.ef.on;
+    ba.end_k = js.end_k;
+    ba.end_off = js.end_off;
     ba.pen = pen;
     if (!hip_ok(wfa::launch_backtrace(ba, st), "backtrace launch")) return WF_STATUS_OOM;
   }
@@ -534,7 +684,8 @@ std::vector<PlanGroup> make_plan(const JobSet& js, const std::vector<int>& ids, 
   std::vector<int> small_ids, cls[3];
   int lmax_small = 0;
   const LeanTuning& lt = lean_tuning();
-  const bool lean_ok = lt.enabled && ncomp == 3 && js.bits == 2 && !js.cbeg && !js.cend && wfa::lean_supported(pen);
+  const bool lean_ok = lt.enabled && ncomp == 3 && js.bits == 2 && !js.cbeg && !js.cend && !js.ef.on &&
+                       wfa::lean_supported(pen);
   std::vector<int> lean_ids[2];
   int lean_lmax[2] = {0, 0};
   for (int i : ids) {
@@ -699,10 +850,15 @@ bool download_results(Engine* e, wfa_device_batch_t* b) {
   b->status.resize(n);
   b->score.resize(n);
   b->ops_begin.resize(n);
+  b->end_k.resize(n);
+  b->end_off.resize(n);
   bool ok = n == 0 ||
             (hip_ok(hipMemcpyAsync(b->status.data(), b->d_status.p, n * 4, hipMemcpyDeviceToHost, st), "download") &&
              hip_ok(hipMemcpyAsync(b->score.data(), b->d_score.p, n * 4, hipMemcpyDeviceToHost, st), "download") &&
              hip_ok(hipMemcpyAsync(b->ops_begin.data(), b->d_ops_begin.p, n * 4, hipMemcpyDeviceToHost, st), "download"));
+  if (ok && n > 0 && b->ef.on && !b->bi_mode)
+    ok = hip_ok(hipMemcpyAsync(b->end_k.data(), b->d_end_k.p, n * 4, hipMemcpyDeviceToHost, st), "download") &&
+         hip_ok(hipMemcpyAsync(b->end_off.data(), b->d_end_off.p, n * 4, hipMemcpyDeviceToHost, st), "download");
   ok = ok && hip_ok(hipStreamSynchronize(st), "download sync");
   b->host_results = ok;
   return ok;
@@ -730,6 +886,9 @@ wfa_host::JobSet wfa_device_batch_t::jobs() const {
   js.ops = d_ops.as<char>();
   js.ops_off = d_ops_off.as<int64_t>();
   js.ops_begin = d_ops_begin.as<int>();
+  js.end_k = d_end_k.as<int>();
+  js.end_off = d_end_off.as<int>();
+  js.ef = ef;
   return js;
 }
 
@@ -737,7 +896,8 @@ wfa_device_batch_t::~wfa_device_batch_t() {
   h_words.release();
   for (auto& g : plan) g.release();
   for (wfa_host::DevBuf* x : {&words, &d_plen, &d_tlen, &d_patw, &d_txtw, &d_status, &d_score, &d_last, &d_dsteps,
-                              &d_ops_off, &d_ops_begin, &d_ops, &rev_words, &d_rev_ppos, &d_rev_tpos})
+                              &d_ops_off, &d_ops_begin, &d_ops, &d_end_k, &d_end_off, &rev_words, &d_rev_ppos,
+                              &d_rev_tpos})
     x->release();
 }
 
@@ -853,7 +1013,7 @@ bool wfa_device_batch_t::load(Engine* e, const wfa_pair_t* pairs, int count) {
   bool ok = words.ensure(std::max<size_t>(nw, 1) * 8) && d_plen.ensure(nn * 4) && d_tlen.ensure(nn * 4) &&
             d_patw.ensure(nn * 8) && d_txtw.ensure(nn * 8) && d_status.ensure(nn * 4) && d_score.ensure(nn * 4) &&
             d_last.ensure(nn * 4) && d_dsteps.ensure(nn * 8) && d_ops_off.ensure(nn * 8) &&
-            d_ops_begin.ensure(nn * 4);
+            d_ops_begin.ensure(nn * 4) && d_end_k.ensure(nn * 4) && d_end_off.ensure(nn * 4);
   if (!ok) {
     fprintf(stderr, "[WFA] device allocation failed for a batch of %d pairs\n", n);
     return false;
@@ -894,6 +1054,7 @@ extern "C" void wfa_device_batch_free(wfa_device_batch_t* b) { delete b; }
 
 extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_batch_t* b) {
   if (!b) return WF_STATUS_OOM;
+  presets_check(wf, b->plen.data(), b->tlen.data(), b->n);
   const int sup = check_supported(wf);
   if (sup != WF_STATUS_OK) return sup;
   Engine* e = engine_of(wf);
@@ -901,6 +1062,7 @@ extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_ba
   const Penalties pen = make_penalties(wf->penalties);
   const int records = (wf->alignment_scope == compute_alignment) ? 1 : 0;
   b->records = records;
+  b->ef = ends_free_of(wf);
   b->stats = wfa_run_stats_t{};
   b->have_results = false;
   b->host_results = false;
@@ -922,7 +1084,7 @@ extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_ba
     rc = bialign_run(wf, e, b, &tot);
     if (rc == WF_STATUS_OOM) return rc;
   } else {
-    const std::vector<int> key = {pen.metric, pen.x, pen.o1, pen.e1, pen.o2, pen.e2, b->bits};
+    const std::vector<int> key = {pen.metric, pen.x, pen.o1, pen.e1, pen.o2, pen.e2, b->bits, b->ef.on};
     if (b->plan_key != key) {
       std::vector<PlanGroup> plan = make_plan(b->jobs(), b->sorted, pen);
       recycle_plan(b->plan, plan);
@@ -977,6 +1139,11 @@ extern "C" int wfa_device_batch_download(wavefront_aligner_t* const wf, wfa_devi
       return WF_STATUS_OOM;
     if (!hip_ok(hipStreamSynchronize(e->stream), "cigar download")) return WF_STATUS_OOM;
   }
+  const bool extension = wf->alignment_form.extension && b->records && !b->bi_mode;
+  if (extension && !want_ops) {
+    fprintf(stderr, "[WFA] the extension form needs the CIGAR arena (it trims the CIGAR)\n");
+    return WF_STATUS_OOM;
+  }
   int first_fail = WF_STATUS_ALG_COMPLETED;
   for (int i = 0; i < b->n; ++i) {
     wfa_result_t& r = results[i];
@@ -988,23 +1155,52 @@ extern "C" int wfa_device_batch_download(wavefront_aligner_t* const wf, wfa_devi
     r.cigar_offset = b->ops_off[i];
     r.cigar_length = 0;
     if (st == WF_STATUS_ALG_COMPLETED) {
-      r.score = b->bi_mode ? b->bi_score[i] : classic_score(wf->penalties, b->plen[i], b->tlen[i], b->score[i]);
       if (b->records) {
         r.cigar_offset = b->ops_off[i] + b->ops_begin[i];
         r.cigar_length = b->plen[i] + b->tlen[i] - b->ops_begin[i];
+      }
+      if (b->bi_mode) {
+        r.score = b->bi_score[i];
+      } else if (!b->records) {
+        // compute_score: the score at the full lengths (wavefront_unialign.c:159-166)
+        r.score = classic_score(wf->penalties, b->plen[i], b->tlen[i], b->score[i]);
+      } else {
+        // wavefront_unialign_terminate (wavefront_unialign.c:167-236): the
+        // end cell gives the end position and the score's lengths; the
+        // extension form then keeps the best-scoring CIGAR prefix.
+        const int k = b->ef.on ? b->end_k[i] : b->tlen[i] - b->plen[i];
+        const int off = b->ef.on ? b->end_off[i] : b->tlen[i];
+        r.end_v = off - k;
+        r.end_h = off;
+        if (!extension) {
+          r.score = classic_score(wf->penalties, r.end_v, r.end_h, b->score[i]);
+        } else {
+          const Trim t = maxtrim(wf->penalties, cigar_arena + r.cigar_offset, r.cigar_length);
+          r.score = b->score[i];  // the backtrace's raw score (wavefront_backtrace.c:96, :527)
+          r.end_v = r.end_h = -1;
+          if (t.examined) {
+            r.cigar_length = t.keep;
+            r.score = t.score;
+            r.end_v = t.end_v;
+            r.end_h = t.end_h;
+          }
+          if (t.trimmed) r.status = st = WF_STATUS_ALG_PARTIAL;
+        }
       }
     } else if (st == WF_STATUS_MAX_STEPS_REACHED && !b->bi_mode) {
       r.score = -wf->system.max_alignment_steps;  // wavefront_unialign.c:102-107
     } else {
       r.score = INT32_MIN;
     }
-    if (st != WF_STATUS_ALG_COMPLETED && first_fail == WF_STATUS_ALG_COMPLETED) first_fail = st;
+    if (st != WF_STATUS_ALG_COMPLETED && st != WF_STATUS_ALG_PARTIAL && first_fail == WF_STATUS_ALG_COMPLETED)
+      first_fail = st;
   }
   return first_fail;
 }
 
 extern "C" int wavefront_align_batch(wavefront_aligner_t* const wf, const wfa_pair_t* pairs, int num_pairs,
                                      wfa_result_t* results, char* cigar_arena, size_t cigar_arena_bytes) {
+  if (num_pairs > 0) presets_check(wf, pairs, num_pairs);
   const int sup = check_supported(wf);
   if (sup != WF_STATUS_OK) {
     for (int i = 0; i < num_pairs; ++i) {
@@ -1055,6 +1251,7 @@ int align_one(wavefront_aligner_t* wf) {
   cigar->score = INT32_MIN;
   cigar->end_v = cigar->end_h = -1;
   wf->align_status.dropped = false;
+  presets_check(wf, &plen, &tlen, 1);
   const int sup = check_supported(wf);
   if (sup != WF_STATUS_OK) {
     wf->align_status.status = sup;
@@ -1073,10 +1270,11 @@ int align_one(wavefront_aligner_t* wf) {
   (void)rc;
   wf->align_status.status = r.status;
   wf->align_status.score = r.score;
-  if (r.status == WF_STATUS_ALG_COMPLETED || r.status == WF_STATUS_MAX_STEPS_REACHED) {
+  const bool done = r.status == WF_STATUS_ALG_COMPLETED || r.status == WF_STATUS_ALG_PARTIAL;
+  if (done || r.status == WF_STATUS_MAX_STEPS_REACHED) {
     cigar->score = r.score;
   }
-  if (r.status == WF_STATUS_ALG_COMPLETED) {
+  if (done) {
     cigar->end_v = r.end_v;
     cigar->end_h = r.end_h;
     if (cig) {

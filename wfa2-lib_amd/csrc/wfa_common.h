@@ -106,6 +106,13 @@ __device__ __forceinline__ int lcp_extend(WordPtr P, int pb, WordPtr T, int tb, 
   }
 }
 
+// wavefront_termination_endsfree (wavefront_termination.c:115-162) for the
+// valid, extended cell at offset off of diagonal k.
+__device__ __forceinline__ bool endsfree_done(int off, int k, int plen, int tlen, int pef, int tef) {
+  const int v = off - k;
+  return (off >= tlen && plen - v <= pef) || (v >= plen && tlen - off <= tef);
+}
+
 struct WaveRange {  // first/last valid diagonal seen by one wave (uniform)
   int first, last;
   __device__ __forceinline__ void init() { first = INT_MAX; last = INT_MIN; }

@@ -91,8 +91,19 @@ struct ForwardArgs {
   int records;                 // 0 = score only
   int comp_begin, comp_end;
   int defer_shift;             // HBM-ring family: defer a pair when the arena has < L^2 >> shift units left (0: never)
+  // Ends-free form (alignment_endsfree): the score-0 wavefront spans
+  // [-pbf, tbf] and the alignment ends at the first diagonal whose extended
+  // cell reaches the text end with at most pef pattern symbols left, or the
+  // pattern end with at most tef text symbols left.  0: end-to-end.
+  int endsfree;
+  int pbf, pef, tbf, tef;
+  int* out_end_k;              // end cell of each completed pair: diagonal and offset
+  int* out_end_off;
   Penalties pen;
 };
+
+// Free-end value that covers any sequence (the extension form frees both ends entirely).
+constexpr int kFreeAll = 0x3FFFFFFF;
 
 struct BacktraceArgs {
   int n_pairs;
@@ -115,6 +126,9 @@ struct BacktraceArgs {
   const int64_t* pat_pos;
   const int64_t* txt_pos;
   int bits;                    // 2 or 8 bits per symbol
+  int endsfree;                // trace from (end_k, end_off), free leading / trailing gaps
+  const int* end_k;
+  const int* end_off;
   Penalties pen;
 };
 

@@ -44,6 +44,13 @@ wfa::Counters* counters_of(Engine* e);
 wfa::Penalties make_penalties(const wavefront_penalties_t& p);
 int classic_score(const wavefront_penalties_t& p, int plen, int tlen, int s);
 
+// Ends-free form of a run (alignment_endsfree); the extension form frees
+// both ends entirely (pef = tef = wfa::kFreeAll).
+struct EndsFree {
+  int on = 0;
+  int pbf = 0, pef = 0, tbf = 0, tef = 0;
+};
+
 // A set of alignment jobs on the device: pairs of a batch, or the base-case
 // sub-problems of BiWFA.  Job i aligns plen[i] pattern symbols starting at
 // symbol ppos[i] of `words` with tlen[i] text symbols starting at tpos[i],
@@ -69,6 +76,9 @@ struct JobSet {
   char* ops = nullptr;
   const int64_t* ops_off = nullptr;
   int* ops_begin = nullptr;
+  int* end_k = nullptr;        // end cell (diagonal, offset) of each completed job
+  int* end_off = nullptr;
+  EndsFree ef;
 };
 
 // Grow-only pinned host staging buffer (hipHostMalloc).
@@ -127,7 +137,7 @@ struct wfa_device_batch_t {
   int64_t words_used = 0;                  // words taken by the forward sequences
   std::vector<int> sorted;                 // pair ids, longest first
   wfa_host::DevBuf words, d_plen, d_tlen, d_patw, d_txtw, d_status, d_score, d_last, d_dsteps,
-      d_ops_off, d_ops_begin, d_ops;
+      d_ops_off, d_ops_begin, d_ops, d_end_k, d_end_off;
   // reversed copies of the sequences (BiWFA), built on first use
   wfa_host::DevBuf rev_words, d_rev_ppos, d_rev_tpos;
   bool have_reverse = false;
@@ -135,7 +145,8 @@ struct wfa_device_batch_t {
   std::vector<wfa_host::PlanGroup> plan;
   std::vector<int> plan_key;
   // results: on the device after a run, copied to the host on demand
-  std::vector<int> status, score, ops_begin;
+  std::vector<int> status, score, ops_begin, end_k, end_off;
+  wfa_host::EndsFree ef;                   // form of the last run
   bool have_results = false;
   bool host_results = false;
   bool ops_on_device = false;

@@ -33,6 +33,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
   constexpr bool WAVE_ONLY = SMALL && NW == 1;
   __shared__ int sh_bcast[2];
   __shared__ int sh_red[NW > 1 ? NW : 1][2 * NC + 1];
+  __shared__ int sh_hit[NW > 1 ? NW : 1];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -66,6 +67,21 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
       const int r = uniform(sh_bcast[slot]);
       __syncthreads();
       return r;
+    }
+  };
+
+  // Least of the per-wave values v (uniform in each wave) over the workgroup.
+  auto block_min = [&](int v) -> int {
+    if constexpr (NW == 1) {
+      return v;
+    } else {
+      __syncthreads();
+      if (lane == 0) sh_hit[wid] = v;
+      __syncthreads();
+      int r = sh_hit[0];
+#pragma unroll
+      for (int w = 1; w < NW; ++w) r = min(r, sh_hit[w]);
+      return uniform(r);
     }
   };
 
@@ -133,7 +149,38 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
       }
     }
     block_sync();  // sequences visible to every lane
-    if (status == 1) {
+    int end_k = kend, end_off = tlen;
+    if (status == 1 && a.endsfree) {
+      // Ends-free score 0: (v=0, h=k) for 0 < k <= tbf, (v=-k, h=0) for
+      // -pbf <= k < 0 (wavefront_aligner.c:252-307), untrimmed; each cell is
+      // extended and the first diagonal that ends the alignment wins
+      // (wavefront_extend_kernels.c:154-196).
+      const int lo0 = -a.pbf, hi0 = a.tbf;
+      if (hi0 - lo0 + 1 > wcap) {
+        status = kDevRingOverflow;
+      } else {
+        int hitk = INT_MAX;
+        for (int kb = lo0; kb <= hi0; kb += BLOCK) {
+          const int k = kb + tid;
+          bool hit = false;
+          if (k <= hi0) {
+            const int m = lcp_extend<BITS>(P, pb, T, tb, plen, tlen, k, k > 0 ? k : 0);
+            ring[(0 * NC + kM) * wcap + (k - lo0)] = to_ring(m, ring);
+            hit = endsfree_done(m, k, plen, tlen, a.pef, a.tef);
+          }
+          const unsigned long long bal = __ballot(hit);
+          if (bal && hitk == INT_MAX) hitk = kb + wid * 64 + (int)__builtin_ctzll(bal);
+        }
+        if (lane == 0) { meta.alo = lo0; meta.lo[kM] = lo0; meta.hi[kM] = hi0; }
+        hitk = block_min(hitk);
+        block_sync();
+        if (hitk != INT_MAX) {
+          status = kDevCompleted;
+          end_k = hitk;
+          end_off = uniform((int)ring[(0 * NC + kM) * wcap + (hitk - lo0)]);
+        }
+      }
+    } else if (status == 1) {
       int v0 = 0;
       if (tid == 0 && cbeg == kM) v0 = lcp_extend<BITS>(P, pb, T, tb, plen, tlen, 0, 0);
       v0 = broadcast0(v0, 1);
@@ -193,7 +240,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
       dsteps += width;
       // Edit distance may prune this wavefront: collect the least worst-case
       // score over its valid pre-extension cells.
-      const bool prune_try = NC == 1 && pen.metric == kEdit && width >= kEditPruneWidth;
+      const bool prune_try = NC == 1 && pen.metric == kEdit && !a.endsfree && width >= kEditPruneWidth;
       int wmin = INT_MAX;
       // outputs present (wavefront_compute.c:446-493)
       const bool has_i1 = (NC > 1) && (mo1.valid() || i1.valid());
@@ -237,9 +284,11 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
 
       WaveRange rM, rI1, rD1, rI2, rD2;
       rM.init(); rI1.init(); rD1.init(); rI2.init(); rD2.init();
+      int hitk = INT_MAX;  // ends-free: first diagonal that ends the alignment
       for (int kb = lo; kb <= hi; kb += BLOCK) {
         const int k = kb + tid;
         bool vM = false, vI1 = false, vD1 = false, vI2 = false, vD2 = false;
+        bool hit = false;
         if constexpr (NC == 1) {
           if (k <= hi) {
             // wavefront_compute_{indel,edit}_idm (wavefront_compute_edit.c:44-102)
@@ -257,6 +306,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
             if (m != kOffsetNull) {
               if (prune_try) wmin = min(wmin, max(plen - (m - k), tlen - m));
               m = lcp_extend<BITS>(P, pb, T, tb, plen, tlen, k, m);
+              hit = a.endsfree && endsfree_done(m, k, plen, tlen, a.pef, a.tef);
             }
             outM[k] = to_ring(m, ring);
             vM = (m != kOffsetNull);
@@ -289,7 +339,10 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
           int m = best;
           if (!in_bounds(m, k, plen, tlen)) m = kOffsetNull;  // null-after-max
           if (rec) rec[k - lo] = (uint8_t)(src | bits);
-          if (m != kOffsetNull) m = lcp_extend<BITS>(P, pb, T, tb, plen, tlen, k, m);
+          if (m != kOffsetNull) {
+            m = lcp_extend<BITS>(P, pb, T, tb, plen, tlen, k, m);
+            hit = a.endsfree && endsfree_done(m, k, plen, tlen, a.pef, a.tef);
+          }
           outM[k] = to_ring(m, ring);
           vM = (m != kOffsetNull);
           if (has_i1) { outI1[k] = to_ring(ins1, ring); vI1 = in_bounds(ins1, k, plen, tlen); }
@@ -301,6 +354,10 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
         }
         const int kw = kb + wid * 64;
         rM.add(__ballot(vM), kw);
+        if (a.endsfree) {
+          const unsigned long long bal = __ballot(hit);
+          if (bal && hitk == INT_MAX) hitk = kw + (int)__builtin_ctzll(bal);
+        }
         if constexpr (NC > 1) {
           rI1.add(__ballot(vI1), kw);
           rD1.add(__ballot(vD1), kw);
@@ -364,9 +421,16 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
         }
         meta.flags = fl;
       }
-      // Termination (wavefront_termination.c:37-114): k_end inside the trimmed
-      // range of the end component and its offset reaching the text end.
-      {
+      if (a.endsfree) {
+        hitk = block_min(hitk);
+        if (hitk != INT_MAX) {
+          status = kDevCompleted;
+          end_k = hitk;
+          end_off = uniform((int)ring[(cur * NC + kM) * wcap + (hitk - lo)]);
+        }
+      } else {
+        // Termination (wavefront_termination.c:37-114): k_end inside the trimmed
+        // range of the end component and its offset reaching the text end.
         const int ce = NC == 1 ? (int)kM : cend;
         bool pce = true;
         WaveRange rce = rM;
@@ -387,6 +451,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
     if (tid == 0) {
       a.out_status[p] = (status == 1) ? kDevCompleted : status;
       a.out_score[p] = s;
+      if (a.out_end_k) { a.out_end_k[p] = end_k; a.out_end_off[p] = end_off; }
       a.out_last_block[p] = prev_blk;
       a.out_dsteps[p] = dsteps;
       if (status == kDevRingOverflow || status == kDevArenaFull) atomicAdd(a.retry_count, 1u);
@@ -432,11 +497,25 @@ __global__ void __launch_bounds__(256) wfa_backtrace_kernel(BacktraceArgs a) {
   int pos = cap;
   int nx = 0, nid = 0;
   bool bad = false;
+  if (a.endsfree) {
+    // Free trailing gaps after the end cell (wavefront_backtrace.c:345-355):
+    // the pattern's rest as deletions, or the text's rest as insertions.
+    k = a.end_k[p];
+    const int off = a.end_off[p];
+    const int nd = plen - (off - k), ni = tlen - off;
+    if (nd < 0 || ni < 0 || nd + ni > cap) {
+      bad = true;
+    } else {
+      for (int i = 0; i < nd; ++i) out[--pos] = 'D';
+      for (int i = 0; i < ni; ++i) out[--pos] = 'I';
+    }
+  }
+  const int ntrail = cap - pos;
   bool mark = comp == kM;  // the next operation written leads into an M cell
   uint32_t blk = a.last_block[p];
   uint4 hdr = make_uint4(0, 0, 0, kNoBlock);
   if (blk != kNoBlock && blk < a.rec_units) hdr = a.rec_arena[blk];
-  while (s > 0) {
+  while (s > 0 && !bad) {
     while (blk != kNoBlock && blk < a.rec_units && (int)hdr.x > s) {
       blk = hdr.w;
       if (blk != kNoBlock && blk < a.rec_units) hdr = a.rec_arena[blk];
@@ -476,14 +555,23 @@ __global__ void __launch_bounds__(256) wfa_backtrace_kernel(BacktraceArgs a) {
       mark = true;
     }
   }
-  if (!bad && (s != 0 || k != 0 || comp != cbeg)) bad = true;
+  // The walk ends at score 0: at the origin, or (ends-free) at a free
+  // beginning cell (v=0, h=k) or (v=-k, h=0) of the score-0 wavefront.
+  if (!bad && (s != 0 || comp != cbeg || (k != 0 && !a.endsfree))) bad = true;
+  const int nlead = k < 0 ? -k : k;
   int start = cap;
   if (!bad) {
-    const int rest = cap - 2 * nx - nid;
+    // 2M + 2X + I + D = plen + tlen: the match count follows, and with it
+    // the CIGAR's length and start (rest >= 0 keeps start in [0, pos - nlead]).
+    const int rest = cap - 2 * nx - nid - ntrail - nlead;
     if (rest < 0 || (rest & 1)) bad = true;
-    const int len = rest / 2 + (cap - pos);
+    const int len = bad ? 0 : rest / 2 + (cap - pos) + nlead;
     start = cap - len;
     int w = start, v = 0, h = 0;
+    if (!bad) {
+      for (int i = 0; i < nlead; ++i) out[w++] = k > 0 ? 'I' : 'D';  // free leading gaps
+      if (k > 0) h = k; else v = -k;
+    }
     const long long ppos = a.pat_pos[p], tpos = a.txt_pos[p];
     const uint64_t* P = a.words + ppos / SPW;
     const uint64_t* T = a.words + tpos / SPW;

@@ -303,7 +303,7 @@ class Aligner:
     """wavefront_aligner_new with attributes built from keyword arguments."""
 
     def __init__(self, metric="affine", penalties=(0, 4, 6, 2), memory="high", score_only=False,
-                 max_steps=0, heuristic_none=True, ends_free=None):
+                 max_steps=0, heuristic_none=True, ends_free=None, extension=False):
         a = default_attributes()
         set_metric(a, metric, penalties)
         a.alignment_scope = 0 if score_only else 1
@@ -312,6 +312,9 @@ class Aligner:
             a.alignment_form.span = 1  # alignment_endsfree
             (a.alignment_form.pattern_begin_free, a.alignment_form.pattern_end_free,
              a.alignment_form.text_begin_free, a.alignment_form.text_end_free) = ends_free
+        if extension:  # wavefront_aligner_set_alignment_extension
+            a.alignment_form.span = 1
+            a.alignment_form.extension = True
         if heuristic_none:
             a.heuristic.strategy = 0
         if max_steps:

@@ -131,3 +131,24 @@ def skewed(seed: int, n: int, tlen=(600, 1200), insert=(1200, 2400), sub=0.03, u
         pats.append(p.tobytes())
         texts.append(t.tobytes())
     return PairSet.from_lists(pats, texts)
+
+
+def semiglobal(shape: int, seed: int, n: int, flank: int = 30) -> PairSet:
+    """Pairs of the given shape with random flanks of up to `flank` bases on
+    both sides of the text (odd pairs) or of the pattern (even pairs), the
+    workload of ends-free alignment (free ends of either sequence)."""
+    ps = generate(shape, seed, n)
+    rng = np.random.default_rng(seed)
+    alphabet = np.frombuffer(b"ACGT", dtype=np.uint8)
+    pats, texts = [], []
+    for i in range(ps.n):
+        p, t = ps.pattern(i), ps.text(i)
+        pre = alphabet[rng.integers(0, 4, int(rng.integers(0, flank)))].tobytes()
+        suf = alphabet[rng.integers(0, 4, int(rng.integers(0, flank)))].tobytes()
+        if i % 2:
+            pats.append(p)
+            texts.append(pre + t + suf)
+        else:
+            pats.append(pre + p + suf)
+            texts.append(t)
+    return PairSet.from_lists(pats, texts)
