@@ -71,14 +71,15 @@ class _CConf(ctypes.Structure):
         "metric", "match", "mismatch", "gap_opening1", "gap_extension1",
         "gap_opening2", "gap_extension2", "memory_mode", "score_only", "max_steps",
         "ends_free", "pattern_begin_free", "pattern_end_free", "text_begin_free", "text_end_free",
-        "extension")]
+        "extension", "heuristic", "min_wavefront_length", "max_distance_threshold", "steps_between_cutoffs")]
 
 
 def _cconf(conf: Conf) -> _CConf:
     p = conf.pen6
     f = conf.form6 or (0, 0, 0, 0, 0, 0)
+    h = conf.heuristic or (0, 0, 0, 0)
     return _CConf(conf.metric_id, p[0], p[1], p[2], p[3], p[4], p[5],
-                  MEMORY_MODES[conf.memory], int(conf.score_only), int(conf.max_steps), *f)
+                  MEMORY_MODES[conf.memory], int(conf.score_only), int(conf.max_steps), *f, *h)
 
 
 class Pairs:

@@ -64,7 +64,9 @@ typedef struct {
   int score;          /* score held by this slot, -1 if none */
   bool present;
   bool null;
-  int lo, hi;         /* trimmed range */
+  int lo, hi;         /* trimmed range (narrowed further by a heuristic cut-off) */
+  int rlo, rhi;       /* readable range: the trimmed range before any cut-off
+                         (wf_elements_init_min/max, wavefront_compute.c:579-613) */
   int alo, ahi;       /* computed range */
   int32_t* v;         /* values, v[k-alo] */
   int cap;
@@ -89,6 +91,9 @@ typedef struct {
   bool endsfree;
   int pbf, pef, tbf, tef;
   int end_k, end_off; /* end cell (wf_aligner->alignment_end_pos) */
+  /* wf-adaptive heuristic (wavefront_heuristic.c:229-290) */
+  bool wfadaptive;
+  int h_min_len, h_max_thr, h_steps, steps_wait;
   /* CIGAR, written backwards by the backtrace into ops[begin..end) */
   char* ops; int ops_cap; int ops_begin; int ops_end;
   int cigar_score;
@@ -164,15 +169,18 @@ static void wf_alloc(owf_t* w, int lo, int hi) {
   }
   w->present = true;
   w->null = false;
-  w->lo = w->alo = lo;
-  w->hi = w->ahi = hi;
+  w->lo = w->alo = w->rlo = lo;
+  w->hi = w->ahi = w->rhi = hi;
 }
 
 static inline int32_t wf_raw(const owf_t* w, int k) { return w->v[k - w->alo]; }
 
-/* Compute-kernel read: NULL outside [lo,hi] or for the null wavefront. */
+/* Compute-kernel read: NULL outside the readable range or for the null
+   wavefront.  A heuristic cut-off narrows [lo,hi] but leaves the cells it
+   cut readable: wavefront_compute_init_ends (wavefront_compute.c:498-575)
+   fills NULLs only beyond the initialised range. */
 static inline int32_t wf_rd(const owf_t* w, int k) {
-  if (!w || k < w->lo || k > w->hi) return ONULL;
+  if (!w || k < w->rlo || k > w->rhi) return ONULL;
   return wf_raw(w, k);
 }
 
@@ -180,6 +188,10 @@ static inline int32_t wf_rd(const owf_t* w, int k) {
 
 static void aln_set_penalties(oaln_t* a, const oracle_conf_t* conf) {
   a->metric = conf->metric;
+  a->wfadaptive = (conf->heuristic & 4) != 0;
+  a->h_min_len = conf->min_wavefront_length;
+  a->h_max_thr = conf->max_distance_threshold;
+  a->h_steps = conf->steps_between_cutoffs;
   a->max_steps = (conf->max_steps > 0) ? conf->max_steps : INT_MAX;
   if (a->metric <= M_LINEAR) {
     /* wavefront_penalties_set_{indel,edit,linear}, wavefront_penalties.c:39-93;
@@ -243,6 +255,8 @@ static void trim_ends(const oaln_t* a, owf_t* w) {
   }
   w->lo = k;
   w->null = (w->lo > w->hi);
+  w->rlo = w->lo;
+  w->rhi = w->hi;
 }
 
 #define WLO(w) ((w) ? (w)->lo : 1)
@@ -288,6 +302,8 @@ static void edit_exact_prune(const oaln_t* a, owf_t* w) {
     --hi_r;
   }
   w->hi = hi_r;
+  w->rlo = w->lo;  /* wavefront_compute_edit NULLs the two cells beyond each end (:355-358) */
+  w->rhi = w->hi;
 }
 
 /* wavefront_compute_edit (compute_edit.c:329-373: indel and edit, from the
@@ -417,6 +433,58 @@ static bool termination(oaln_t* a, owf_t* m, int s) {
   return true;
 }
 
+/* wavefront_heuristic_cufoff with wf-adaptive (wavefront_heuristic.c:229-290,
+   506-561), after the extension of score s: every steps_between_cutoffs
+   non-empty M wavefronts of at least min_wavefront_length diagonals, keep the
+   diagonals whose distance to the end max(plen - v, tlen - h) is within
+   max_distance_threshold of the least one, scanning in from both ends but
+   never past the end diagonal; then narrow I1/D1/I2/D2 to M's range. */
+static void cutoff(oaln_t* a, int s) {
+  owf_t* m = wf_at(a, C_M, s);
+  if (!m || m->lo > m->hi) return;
+  --a->steps_wait;
+  if (a->steps_wait > 0) return;
+  const int lo = m->lo, hi = m->hi;
+  if (hi - lo + 1 < a->h_min_len) return;
+  int32_t* d = malloc(sizeof(int32_t) * (hi - lo + 1));
+  int min_d = imax(a->plen, a->tlen), k;
+  for (k = lo; k <= hi; ++k) {
+    const int32_t off = wf_raw(m, k);
+    const int dist = (off >= 0) ? imax(a->plen - (off - k), a->tlen - off) : -ONULL;
+    d[k - lo] = dist;
+    min_d = imin(min_d, dist);
+  }
+  const int k_end = a->tlen - a->plen;
+  const int top = imin(k_end, hi);
+  int lo_r = lo;
+  for (k = lo; k < top; ++k) {
+    if (d[k - lo] - min_d <= a->h_max_thr) break;
+    ++lo_r;
+  }
+  m->lo = lo_r;
+  const int bottom = imax(k_end, m->lo);
+  int hi_r = hi;
+  for (k = hi; k > bottom; --k) {
+    if (d[k - lo] - min_d <= a->h_max_thr) break;
+    --hi_r;
+  }
+  m->hi = hi_r;
+  free(d);
+  a->steps_wait = a->h_steps;
+  if (m->lo == lo && m->hi == hi) return;
+  if (m->lo > m->hi) m->null = true;
+  if (a->metric <= M_EDIT) { m->rlo = m->lo; m->rhi = m->hi; }  /* NULLs beyond the ends, as the prune */
+  if (a->metric <= M_LINEAR) return;
+  int c;
+  for (c = C_I1; c < NCOMP; ++c) {  /* wf_heuristic_equate (:163-171) */
+    owf_t* w = wf_at(a, c, s);
+    if (!w) continue;
+    if (m->lo > w->lo) w->lo = m->lo;
+    if (m->hi < w->hi) w->hi = m->hi;
+    if (w->lo > w->hi) w->null = true;
+  }
+}
+
 /* wavefront_termination_endsfree, wavefront_termination.c:115-162 */
 static bool termination_endsfree(oaln_t* a, int s, int k, int32_t off) {
   const int h = off, v = off - k;
@@ -444,6 +512,7 @@ static int extend_endsfree(oaln_t* a, int s) {
     m->v[k - m->alo] = ext;
     if (termination_endsfree(a, s, k, ext)) return 1;
   }
+  if (a->wfadaptive) cutoff(a, s);
   return 0;
 }
 
@@ -469,6 +538,7 @@ static int extend(oaln_t* a, int s, int* max_ak) {
     a->end_off = a->tlen;
     return 1;
   }
+  if (a->wfadaptive) cutoff(a, s);
   if (max_ak) *max_ak = best;
   return 0;
 }
@@ -482,6 +552,7 @@ static void aln_init(oaln_t* a, int comp_begin, int comp_end) {
   a->comp_end = comp_end;
   a->status = ST_OK;
   a->end_score = -1;
+  a->steps_wait = a->h_steps;  /* wavefront_heuristic_clear (wavefront_heuristic.c:114-121) */
   owf_t* w[NCOMP];
   int c;
   for (c = 0; c < NCOMP; ++c) w[c] = wf_claim(a, c, 0);
@@ -505,7 +576,8 @@ static void aln_ops_reserve(oaln_t* a, int n) {
 static int64_t probe(oaln_t* a, int c, int s, int k, int add, int type) {
   if (s < 0) return ONULL;
   owf_t* w = wf_at(a, c, s);
-  if (!w || w->null || k < w->lo || k > w->hi) return ONULL;
+  /* the backtrace probes the initialised range (wavefront_backtrace.c:64-219) */
+  if (!w || k < w->rlo || k > w->rhi) return ONULL;
   return (int64_t)(wf_raw(w, k) + add) * 16 + type;
 }
 
@@ -896,6 +968,7 @@ static int bialign(const oracle_conf_t* conf, const char* P, int plen, const cha
   aln_set_penalties(&b.fwd, conf); b.fwd.modular = true;
   aln_set_penalties(&b.rev, conf); b.rev.modular = true;
   aln_set_penalties(&b.base, conf); b.base.modular = false;
+  b.base.wfadaptive = false;  /* the base aligner runs without heuristic (wavefront_bialigner.c:64-67) */
   b.P = P; b.T = T; b.plen = plen; b.tlen = tlen;
   b.Prev = reversed(P, plen); b.Trev = reversed(T, tlen);
   b.cigar_score = INT32_MIN;

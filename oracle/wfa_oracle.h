@@ -5,7 +5,8 @@
  * gap-linear, gap-affine and dual-cost gap-affine, end-to-end and ends-free
  * (with the extension form), in memory mode high (and med/low, whose
  * CIGARs the reference asserts equal memory-high: tests/wfa.utest.sh:67-78)
- * and ultralow (BiWFA, end-to-end).  Ends-free termination takes the first
+ * and ultralow (BiWFA, end-to-end), exact or with the wf-adaptive heuristic.
+ * Ends-free termination takes the first
  * diagonal in ascending order, as the reference's scalar extend kernel
  * (wavefront_extend_kernels.c:154-196).  It is the checker for the HIP path; only tests/,
  * __graft_entry__.smoke() and the cpu_baseline leg of bench.py may load it.
@@ -36,6 +37,9 @@ typedef struct {
   int ends_free;       /* alignment_form.span == alignment_endsfree */
   int pattern_begin_free, pattern_end_free, text_begin_free, text_end_free;
   int extension;       /* alignment_form.extension (wavefront_align.c:57-63) */
+  int heuristic;       /* wavefront_heuristic_t.strategy: 0 none, 4 wf-adaptive
+                          (wavefront_heuristic.h:41-49) */
+  int min_wavefront_length, max_distance_threshold, steps_between_cutoffs;
 } oracle_conf_t;
 
 /* Status returned for a configuration this restatement does not cover:

@@ -43,16 +43,28 @@ def golden_lines(name):
         return [l.rstrip("\n") for l in f]
 
 
-def cases(metrics=("affine", "affine2p", "edit", "indel")):
-    """(file stem, metric, penalties, memory, score_only) for every golden file."""
+# wf-adaptive (strategy 4) with (min_wavefront_length, max_distance_threshold,
+# steps_between_cutoffs), tests/wfa.utest.sh:57-58
+HEURISTICS = {
+    ".wfapt0": (4, 10, 50, 1),
+    ".wfapt1": (4, 10, 50, 10),
+}
+
+
+def cases(metrics=("affine", "affine2p", "edit", "indel", "wfapt")):
+    """(file stem, metric, penalties, memory, score_only, heuristic) for
+    every golden file; heuristic is None (exact) or the wf-adaptive tuple."""
     out = []
     for pre, (mem, so) in MODES.items():
         if "affine" in metrics:
             for suf, pen in PENALTIES.items():
-                out.append(("%s.affine%s" % (pre, suf), "affine", pen, mem, so))
+                out.append(("%s.affine%s" % (pre, suf), "affine", pen, mem, so, None))
         if "affine2p" in metrics:
-            out.append(("%s.affine2p" % pre, "affine2p", AFFINE2P, mem, so))
+            out.append(("%s.affine2p" % pre, "affine2p", AFFINE2P, mem, so, None))
         for m in ("edit", "indel"):
             if m in metrics:
-                out.append(("%s.%s" % (pre, m), m, (0, 1, 1), mem, so))
+                out.append(("%s.%s" % (pre, m), m, (0, 1, 1), mem, so, None))
+        if "wfapt" in metrics:
+            for suf, h in HEURISTICS.items():
+                out.append(("%s.affine%s" % (pre, suf), "affine", (0, 4, 6, 2), mem, so, h))
     return out

@@ -38,7 +38,7 @@ def assert_same(got, ref, ps, check_dsteps=None):
         assert np.array_equal(check_dsteps, ref.dsteps)
 
 
-GOLDEN_GPU = G.cases()  # every golden file: affine, affine2p, edit, indel
+GOLDEN_GPU = G.cases(("affine", "affine2p", "edit", "indel"))
 
 
 @pytest.fixture(scope="module")
@@ -49,7 +49,7 @@ def suite():
 
 @pytest.mark.parametrize("case", GOLDEN_GPU, ids=[c[0] for c in GOLDEN_GPU])
 def test_golden_suite(suite, case):
-    name, metric, pen, mem, so = case
+    name, metric, pen, mem, so, heur = case
     say("golden %s on GPU" % name)
     al = wfa_amd.Aligner(metric=metric, penalties=pen, memory=mem, score_only=so)
     res = al.align_batch(suite)

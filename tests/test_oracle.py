@@ -31,9 +31,9 @@ def suite():
 
 @pytest.mark.parametrize("case", G.cases(), ids=[c[0] for c in G.cases()])
 def test_oracle_matches_golden(suite, case):
-    name, metric, pen, mem, so = case
+    name, metric, pen, mem, so, heur = case
     pats, texts = suite
-    conf = O.Conf(metric, pen, mem, so)
+    conf = O.Conf(metric, pen, mem, so, heuristic=heur)
     r = O.oracle_align(conf, _pairs(pats, texts))
     assert O.alg_lines(r, so) == G.golden_lines(name)
 
@@ -337,3 +337,71 @@ def test_oracle_refuses_what_it_does_not_restate():
                  O.Conf("affine", (0, 4, 6, 2), "ultralow", ends_free=(0, 0, 2, 2)),
                  O.Conf("affine", (0, 4, 6, 2), "ultralow", extension=True)):
         assert O.oracle_align(conf, pairs).status[0] == O.ORACLE_UNSUPPORTED
+
+
+# ------------------------------------------------------------- heuristics
+
+WFAPT_SETS = [
+    ("affine", (0, 4, 6, 2), (4, 10, 50, 1)),
+    ("affine", (-3, 5, 3, 2), (4, 10, 50, 10)),
+    ("affine2p", (0, 4, 6, 2, 24, 1), (4, 10, 50, 1)),
+    ("linear", (0, 4, 2), (4, 50, 10, 3)),
+    ("edit", (0, 1, 1), (4, 10, 50, 1)),
+    ("indel", (0, 1, 1), (4, 10, 50, 10)),
+]
+
+
+@pytest.mark.skipif(not O.ref_available("v2"), reason="scalar reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("metric,pen,heur", WFAPT_SETS)
+def test_oracle_wfadaptive_matches_reference_build(metric, pen, heur):
+    """wf-adaptive (wavefront_heuristic.c:229-290) in every memory mode, with
+    CIGARs and score-only, short reads and 1-10 kbp pairs."""
+    from wfa_amd import datagen
+    a, b = datagen.generate(1, 41, 600), datagen.generate(2, 42, 6)
+    keep = [i for i in range(b.n) if b.plen[i] <= 10000]
+    pats = a.patterns() + [b.pattern(i) for i in keep]
+    texts = a.texts() + [b.text(i) for i in keep]
+    pairs = _pairs(pats, texts)
+    for mem in ("high", "ultralow"):
+        for so in (False, True):
+            conf = O.Conf(metric, pen, mem, so, heuristic=heur)
+            x = O.oracle_align(conf, pairs)
+            y = O.ref_align(conf, pairs, isa="v2")
+            assert list(x.status) == list(y.status), (mem, so)
+            assert list(x.score) == list(y.score), (mem, so)
+            if not so:
+                for i in range(pairs.n):
+                    assert x.ops(i) == y.ops(i), (mem, i)
+
+
+@pytest.mark.skipif(not O.ref_available("v2"), reason="scalar reference build (oracle/_ref) not present")
+def test_oracle_wfadaptive_endsfree_matches_reference_build():
+    from wfa_amd import datagen
+    ps = datagen.semiglobal(1, 43, 800)
+    pairs = _pairs(ps.patterns(), ps.texts())
+    for metric, pen, form in (("affine", (0, 4, 6, 2), dict(ends_free=(0, 0, 30, 30))),
+                              ("affine", (0, 4, 6, 2), dict(extension=True)),
+                              ("edit", (0, 1, 1), dict(ends_free=(0, 30, 0, 30)))):
+        conf = O.Conf(metric, pen, "high", heuristic=(4, 10, 50, 1), **form)
+        x = O.oracle_align(conf, pairs)
+        y = O.ref_align(conf, pairs, isa="v2")
+        assert list(x.status) == list(y.status) and list(x.score) == list(y.score)
+        assert all(x.ops(i) == y.ops(i) for i in range(pairs.n))
+
+
+def test_oracle_wfadaptive_is_a_valid_alignment_no_better_than_optimal():
+    """What a heuristic may change is which alignment is found, never its
+    validity: the CIGAR transforms pattern into text, re-scores to the
+    reported score, and that score never beats the DP optimum; with a
+    threshold nothing can exceed it equals the exact result."""
+    pats, texts = _random_pairs(77, 60, 40, 90, 12, max_indel=3)
+    pairs = _pairs(pats, texts)
+    r = O.oracle_align(O.Conf("affine", (0, 4, 6, 2), heuristic=(4, 10, 5, 1)), pairs)
+    exact = O.oracle_align(O.Conf("affine", (0, 4, 6, 2)), pairs)
+    loose = O.oracle_align(O.Conf("affine", (0, 4, 6, 2), heuristic=(4, 10, 100000, 1)), pairs)
+    for i, (p, t) in enumerate(zip(pats, texts)):
+        assert r.status[i] == 0
+        assert dp.cigar_valid(p, t, r.ops(i))
+        assert dp.cigar_score(r.ops(i), 0, 4, 6, 2) == r.score[i]
+        assert r.score[i] <= dp.gotoh(p, t, 0, 4, 6, 2)
+        assert (loose.score[i], loose.ops(i)) == (exact.score[i], exact.ops(i))
