@@ -11,7 +11,8 @@
  *
  * Exact path (bit-identical scores and CIGARs to the reference):
  *   distance_metric indel / edit / gap_linear / gap_affine / gap_affine_2p,
- *   heuristic wf_heuristic_none, compute_score or compute_alignment;
+ *   heuristic wf_heuristic_none or wf_heuristic_wfadaptive (the default
+ *   attributes), compute_score or compute_alignment;
  *   alignment_end2end in memory modes high / med / low / ultralow;
  *   alignment_endsfree and the extension form in memory modes high / med /
  *   low (the reference rejects them in ultralow with exit(1), and so does
@@ -19,7 +20,8 @@
  *   reference's scalar extend kernel does.
  * Configurations outside it are rejected with WF_STATUS_UNSUPPORTED and a
  * "[WFA]" message on stderr; they never silently produce different results:
- * heuristics, lambda sequences, free beginnings with a match bonus or with
+ * the other heuristics (x-drop, z-drop, banded, wfmash), lambda sequences,
+ * free beginnings with a match bonus or with
  * indel/edit distances, indel/edit extension outside memory mode high.
  * Configuration errors (invalid penalties, ends-free lengths larger than the
  * sequences) print the reference's message and exit(1) as it does.

@@ -41,7 +41,8 @@ def test_default_attributes_match_reference():
     dict(metric="affine", penalties=(-1, 4, 6, 2), ends_free=(2, 0, 0, 0)),  # free beginning, match < 0
     dict(metric="affine", memory="ultralow", ends_free=(0, 0, 0, 0)),  # ends-free span in BiWFA
     dict(metric="edit", memory="med", extension=True),                 # indel/edit extension outside high
-    dict(metric="affine", heuristic_none=False),
+    dict(metric="affine", heuristic=(0x10, 0, 0, 1)),                  # x-drop
+    dict(metric="affine", heuristic=(0x04 | 0x10, 10, 50, 1)),         # wf-adaptive + x-drop
 ])
 def test_unsupported_configurations_are_refused(kw):
     al = wfa_amd.Aligner(**kw)

@@ -38,7 +38,7 @@ def assert_same(got, ref, ps, check_dsteps=None):
         assert np.array_equal(check_dsteps, ref.dsteps)
 
 
-GOLDEN_GPU = G.cases(("affine", "affine2p", "edit", "indel"))
+GOLDEN_GPU = G.cases()  # every golden file, the wf-adaptive ones included
 
 
 @pytest.fixture(scope="module")
@@ -51,10 +51,10 @@ def suite():
 def test_golden_suite(suite, case):
     name, metric, pen, mem, so, heur = case
     say("golden %s on GPU" % name)
-    al = wfa_amd.Aligner(metric=metric, penalties=pen, memory=mem, score_only=so)
+    al = wfa_amd.Aligner(metric=metric, penalties=pen, memory=mem, score_only=so, heuristic=heur)
     res = al.align_batch(suite)
     lines = []
-    conf = O.Conf(metric, pen, mem, so)
+    conf = O.Conf(metric, pen, mem, so, heuristic=heur)
     for i in range(suite.n):
         assert res.status[i] == 0, (i, res.status[i])
         if so:

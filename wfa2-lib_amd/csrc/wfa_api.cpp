@@ -323,7 +323,8 @@ int check_supported(wavefront_aligner_t* wf) {
   // reference (piggyback backtrace); only memory high is defined.
   else if (f.extension && m <= edit && wf->memory_mode != wavefront_memory_high)
     why = "indel/edit extension outside memory mode high";
-  else if (wf->heuristic.strategy != wf_heuristic_none) why = "only heuristic none (exact alignment)";
+  else if (wf->heuristic.strategy != wf_heuristic_none && wf->heuristic.strategy != wf_heuristic_wfadaptive)
+    why = "heuristics other than wf-adaptive (x-drop, z-drop, banded, wfmash)";
   else if (make_penalties(wf->penalties).nslots > 64)
     why = "penalties whose score scope needs more than 64 wavefront slots";
   if (!why) return WF_STATUS_OK;
@@ -364,6 +365,20 @@ void presets_check(wavefront_aligner_t* wf, const wfa_pair_t* pairs, int n) {
   for (int i = 0; i < n; ++i) { pl[i] = pairs[i].pattern_length; tl[i] = pairs[i].text_length; }
   presets_check(wf, pl.data(), tl.data(), n);
 }
+
+}  // namespace
+
+wfa::HeurArgs heuristic_of(const wavefront_aligner_t* wf) {
+  wfa::HeurArgs h{};
+  if (wf->heuristic.strategy != wf_heuristic_wfadaptive) return h;
+  h.wfadaptive = 1;
+  h.min_len = wf->heuristic.min_wavefront_length;
+  h.thr = wf->heuristic.max_distance_threshold;
+  h.steps = wf->heuristic.steps_between_cutoffs;
+  return h;
+}
+
+namespace {
 
 // Ends-free form of a run; extension frees both ends entirely (wavefront_align.c:57-63).
 EndsFree ends_free_of(const wavefront_aligner_t* wf) {
@@ -544,7 +559,7 @@ PlanGroup make_large(const JobSet& js, std::vector<int> ids) {
   return g;
 }
 
-LaunchShape shape_of(Engine* e, const PlanGroup& g, int bits, const Penalties& pen) {
+LaunchShape shape_of(Engine* e, const PlanGroup& g, int bits, const Penalties& pen, bool cut) {
   LaunchShape shape;
   shape.family = g.family;
   shape.block = g.block;
@@ -554,7 +569,7 @@ LaunchShape shape_of(Engine* e, const PlanGroup& g, int bits, const Penalties& p
     per_cu = wfa::lean_max_blocks_per_cu(pen, g.wcap, shape.lds_bytes);
   } else {
     shape.lds_bytes = wfa::forward_lds_bytes(g.family, pen.ncomp, pen.nslots, g.wcap, g.lmax_words);
-    per_cu = wfa::forward_max_blocks_per_cu(g.family, g.block, pen.ncomp, bits, shape.lds_bytes);
+    per_cu = wfa::forward_max_blocks_per_cu(g.family, g.block, pen.ncomp, bits, cut, shape.lds_bytes);
   }
   if (per_cu <= 0) per_cu = 1;
   shape.grid = (int)std::min<int64_t>((int64_t)g.ids.size(), (int64_t)per_cu * e->num_cus);
@@ -567,7 +582,7 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
   const int n = (int)grp.ids.size();
   if (n == 0) return WF_STATUS_OK;
   hipStream_t st = e->stream;
-  const LaunchShape shape = shape_of(e, grp, js.bits, pen);
+  const LaunchShape shape = shape_of(e, grp, js.bits, pen, js.heur.wfadaptive != 0);
   if (!grp.ids_on_device) {
     if (!grp.d_ids.ensure((size_t)n * 4)) return WF_STATUS_OOM;
     if (!hip_ok(hipMemcpyAsync(grp.d_ids.p, grp.ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, st), "plan upload"))
@@ -614,6 +629,10 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
   fa.tef = js.ef.tef;
   fa.out_end_k = js.end_k;
   fa.out_end_off = js.end_off;
+  fa.wfadaptive = js.heur.wfadaptive;
+  fa.h_min_len = js.heur.min_len;
+  fa.h_thr = js.heur.thr;
+  fa.h_steps = js.heur.steps;
   fa.pen = pen;
   if (!hip_ok(wfa::launch_reset_counters(c, 0, st), "reset")) return WF_STATUS_OOM;
   (void)hipEventRecord(grp.ev[0], st);
@@ -655,7 +674,7 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
 }
 
 // Device memory every group of the plan needs, allocated before any launch.
-bool reserve_for(Engine* e, std::vector<PlanGroup>& plan, int bits, const Penalties& pen, int records) {
+bool reserve_for(Engine* e, std::vector<PlanGroup>& plan, int bits, const Penalties& pen, bool cut, int records) {
   if (records && !e->rec.ensure(e->rec_bytes)) {
     fprintf(stderr, "[WFA] cannot allocate the %zu MB record arena\n", e->rec_bytes >> 20);
     return false;
@@ -663,7 +682,7 @@ bool reserve_for(Engine* e, std::vector<PlanGroup>& plan, int bits, const Penalt
   size_t ring = 0;
   for (const PlanGroup& g : plan) {
     if (g.family != wfa::kLarge) continue;
-    const LaunchShape sh = shape_of(e, g, bits, pen);
+    const LaunchShape sh = shape_of(e, g, bits, pen, cut);
     ring = std::max(ring, (size_t)pen.nslots * pen.ncomp * g.wcap * sh.grid * sizeof(int));
   }
   if (ring && !e->ring.ensure(ring)) {
@@ -685,7 +704,7 @@ std::vector<PlanGroup> make_plan(const JobSet& js, const std::vector<int>& ids, 
   int lmax_small = 0;
   const LeanTuning& lt = lean_tuning();
   const bool lean_ok = lt.enabled && ncomp == 3 && js.bits == 2 && !js.cbeg && !js.cend && !js.ef.on &&
-                       wfa::lean_supported(pen);
+                       !js.heur.wfadaptive && wfa::lean_supported(pen);
   std::vector<int> lean_ids[2];
   int lean_lmax[2] = {0, 0};
   for (int i : ids) {
@@ -748,7 +767,7 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
           const double L = std::max(js.h_plen[i], js.h_tlen[i]);
           w += L * L / 16 + 8 * L;
         }
-        g.est_rec_bytes = w + (double)shape_of(e, g, js.bits, pen).grid * wfa::kChunkUnits * 16;
+        g.est_rec_bytes = w + (double)shape_of(e, g, js.bits, pen, js.heur.wfadaptive != 0).grid * wfa::kChunkUnits * 16;
       }
       want += g.est_rec_bytes;
     }
@@ -761,7 +780,7 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
       fprintf(stderr, "[WFA] batch did not converge (record arena too small?)\n");
       return WF_STATUS_OOM;
     }
-    if (!reserve_for(e, *pending, js.bits, pen, records)) return WF_STATUS_OOM;
+    if (!reserve_for(e, *pending, js.bits, pen, js.heur.wfadaptive != 0, records)) return WF_STATUS_OOM;
     if (!hip_ok(wfa::launch_reset_counters(counters_of(e), 1, st), "reset")) return WF_STATUS_OOM;
     for (PlanGroup& g : *pending) {
       const int rc = enqueue_group(wf, e, js, g, pen, records);
@@ -889,6 +908,7 @@ wfa_host::JobSet wfa_device_batch_t::jobs() const {
   js.end_k = d_end_k.as<int>();
   js.end_off = d_end_off.as<int>();
   js.ef = ef;
+  js.heur = heur;
   return js;
 }
 
@@ -1063,6 +1083,7 @@ extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_ba
   const int records = (wf->alignment_scope == compute_alignment) ? 1 : 0;
   b->records = records;
   b->ef = ends_free_of(wf);
+  b->heur = heuristic_of(wf);
   b->stats = wfa_run_stats_t{};
   b->have_results = false;
   b->host_results = false;
@@ -1084,7 +1105,8 @@ extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_ba
     rc = bialign_run(wf, e, b, &tot);
     if (rc == WF_STATUS_OOM) return rc;
   } else {
-    const std::vector<int> key = {pen.metric, pen.x, pen.o1, pen.e1, pen.o2, pen.e2, b->bits, b->ef.on};
+    const std::vector<int> key = {pen.metric, pen.x, pen.o1, pen.e1, pen.o2, pen.e2, b->bits, b->ef.on,
+                                  b->heur.wfadaptive};
     if (b->plan_key != key) {
       std::vector<PlanGroup> plan = make_plan(b->jobs(), b->sorted, pen);
       recycle_plan(b->plan, plan);

@@ -108,7 +108,8 @@ bool run_breakpoints(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, 
       wmax = std::max(wmax, (probs[j].pe - probs[j].pb) + (probs[j].te - probs[j].tb));
     }
     const int wcap = wmax + 8;
-    int per_cu = wfa::breakpoint_max_blocks_per_cu(ncomp, b->bits, block);
+    const wfa::HeurArgs heur = heuristic_of(wf);
+    int per_cu = wfa::breakpoint_max_blocks_per_cu(ncomp, b->bits, block, heur.wfadaptive != 0);
     if (per_cu <= 0) per_cu = 1;
     const int grid = (int)std::min<int64_t>((int64_t)ids.size(), (int64_t)per_cu * e->num_cus);
     const long long stride = 2LL * pen.nslots * ncomp * wcap;
@@ -137,6 +138,7 @@ bool run_breakpoints(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, 
     a.queue = &c->queue;
     a.dsteps_total = &c->dsteps;
     a.max_steps = wf->system.max_alignment_steps > 0 ? wf->system.max_alignment_steps : INT_MAX;
+    a.heur = heur;  // the breakpoint aligners inherit the heuristic (wavefront_bialigner.c:53)
     a.pen = pen;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     (void)hipEventCreate(&e0);

@@ -16,9 +16,10 @@
 namespace wfa {
 namespace {
 
-template <int NC>
+template <int NC, bool CUT>
 struct Side {
-  MetaRegs<NC> meta;
+  MetaRegs<NC, CUT> meta;
+  int steps_wait;          // wf-adaptive countdown (wavefront_heuristic.c:506-521)
   int cur;                 // ring slot of the latest multiple-of-g score
   int* ring;
   const uint64_t* P;
@@ -27,39 +28,6 @@ struct Side {
   int tb;
   int cb, ce;              // begin / end component of this direction
   int end_score;           // score at which this side reached its end, -1 if not
-};
-
-template <int BLOCK>
-struct BlockRed {
-  static constexpr int NW = BLOCK / 64;
-  int* sh;                 // NW ints of LDS
-  // Block-wide min of per-thread values (uniform result).
-  __device__ __forceinline__ int min_all(int v) {
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    v = uniform(v);  // every lane holds the minimum
-    if constexpr (NW == 1) return v;
-    const int lane = threadIdx.x & 63, wid = uniform(threadIdx.x >> 6);
-    __syncthreads();
-    if (lane == 0) sh[wid] = v;
-    __syncthreads();
-    int r = sh[0];
-#pragma unroll
-    for (int w = 1; w < NW; ++w) r = min(r, sh[w]);
-    return uniform(r);
-  }
-  __device__ __forceinline__ int max_all(int v) {
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    v = uniform(v);  // every lane holds the maximum
-    if constexpr (NW == 1) return v;
-    const int lane = threadIdx.x & 63, wid = uniform(threadIdx.x >> 6);
-    __syncthreads();
-    if (lane == 0) sh[wid] = v;
-    __syncthreads();
-    int r = sh[0];
-#pragma unroll
-    for (int w = 1; w < NW; ++w) r = max(r, sh[w]);
-    return uniform(r);
-  }
 };
 
 struct Bp {  // wf_bialign_breakpoint_t
@@ -76,8 +44,8 @@ struct WfRef {
   int lo, hi, base;  // value at k is ring[base + k]
 };
 
-template <int NC>
-__device__ __forceinline__ WfRef<NC> wf_ref(const Side<NC>& sd, int s, int c, const Penalties& pen, int wcap) {
+template <int NC, bool CUT>
+__device__ __forceinline__ WfRef<NC> wf_ref(const Side<NC, CUT>& sd, int s, int c, const Penalties& pen, int wcap) {
   WfRef<NC> r{false, 1, 0, 0};
   if (s < 0 || (s % pen.g) != 0) return r;
   // the slot of score s, counted back from the latest multiple of g (slot cur)
@@ -91,18 +59,60 @@ __device__ __forceinline__ WfRef<NC> wf_ref(const Side<NC>& sd, int s, int c, co
   const unsigned fl = (unsigned)lane_of((int)sd.meta.flags, slot);
   if (!(fl & (1u << c))) return r;
   r.present = true;
-  r.lo = lane_of(sd.meta.lo[c], slot);
-  r.hi = lane_of(sd.meta.hi[c], slot);
+  // the overlap scans the range a cut-off left (wavefront_bialign.c:477-573)
+  if constexpr (CUT) {
+    r.lo = lane_of(sd.meta.cut.lo[c], slot);
+    r.hi = lane_of(sd.meta.cut.hi[c], slot);
+  } else {
+    r.lo = lane_of(sd.meta.lo[c], slot);
+    r.hi = lane_of(sd.meta.hi[c], slot);
+  }
   r.base = (slot * NC + c) * wcap - lane_of(sd.meta.alo, slot);
   return r;
+}
+
+// wf-adaptive cut-off of the side's M wavefront in ring slot `slot`, trimmed
+// to [lo, hi] with array origin alo (wavefront_heuristic_cufoff,
+// wavefront_heuristic.c:506-561), then I1/D1/I2/D2 narrowed to M's range;
+// indel / edit then read M only inside the cut range (their compute step
+// NULLs the two cells beyond each end, wavefront_compute_edit.c:355-358).
+template <int NC, int BLOCK, bool CUT>
+__device__ void side_cutoff(Side<NC, CUT>& sd, const HeurArgs& heur, BlockRed<BLOCK>& red, int slot, int alo, int lo,
+                            int hi, int plen, int tlen, int wcap, int metric) {
+  if constexpr (CUT) {
+    --sd.steps_wait;
+    if (sd.steps_wait > 0 || hi - lo + 1 < heur.min_len) return;
+    const int* m = sd.ring + (slot * NC + kM) * wcap - alo;
+    int clo, chi;
+    wfadaptive_cut<BLOCK>(m, lo, hi, plen, tlen, heur.thr, red, &clo, &chi);
+    sd.steps_wait = heur.steps;
+    // unchanged M: nothing else is narrowed either (wavefront_heuristic.c:540-541)
+    if (clo == lo && chi == hi) return;
+    if ((threadIdx.x & 63) == slot) {
+      unsigned fl = sd.meta.flags;
+      sd.meta.cut.lo[kM] = clo;
+      sd.meta.cut.hi[kM] = chi;
+      if (clo > chi) fl |= 0x100u;
+      if (metric <= kEdit) { sd.meta.lo[kM] = clo; sd.meta.hi[kM] = chi; }
+#pragma unroll
+      for (int c = 1; c < NC; ++c) {
+        if ((fl & (1u << c)) && !(fl & (0x100u << c))) {
+          sd.meta.cut.lo[c] = max(sd.meta.cut.lo[c], clo);
+          sd.meta.cut.hi[c] = min(sd.meta.cut.hi[c], chi);
+          if (sd.meta.cut.lo[c] > sd.meta.cut.hi[c]) fl |= 0x100u << c;
+        }
+      }
+      sd.meta.flags = fl;
+    }
+  }
 }
 
 // Compute score s of one side (no backtrace records) fused with its
 // extension; returns 1 when the end is reached (wavefront_termination.c),
 // -1 on an internal error, 0 otherwise.  *max_ak gets the largest
 // antidiagonal 2*offset-k of the extended M wavefront (0 when absent).
-template <int NC, int BLOCK, int BITS>
-__device__ int side_step(Side<NC>& sd, int s, int plen, int tlen, const Penalties& pen, int wcap,
+template <int NC, int BLOCK, int BITS, bool CUT>
+__device__ int side_step(Side<NC, CUT>& sd, int s, int plen, int tlen, const Penalties& pen, const HeurArgs& heur, int wcap,
                          bool want_max, int* max_ak, long long* dsteps, int (*sh_red)[2 * NC + 2],
                          BlockRed<BLOCK>& red) {
   constexpr int NW = BLOCK / 64;
@@ -282,6 +292,10 @@ __device__ int side_step(Side<NC>& sd, int s, int plen, int tlen, const Penaltie
       if (has_i2) { fl |= 8u | (rI2.null() ? 0x800u : 0u); sd.meta.lo[kI2] = rI2.tlo(lo); sd.meta.hi[kI2] = rI2.thi(lo); }
       if (has_d2) { fl |= 16u | (rD2.null() ? 0x1000u : 0u); sd.meta.lo[kD2] = rD2.tlo(lo); sd.meta.hi[kD2] = rD2.thi(lo); }
     }
+    if constexpr (CUT) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) { sd.meta.cut.lo[c] = sd.meta.lo[c]; sd.meta.cut.hi[c] = sd.meta.hi[c]; }
+    }
     sd.meta.flags = fl;
   }
   const int kend = tlen - plen;
@@ -300,14 +314,15 @@ __device__ int side_step(Side<NC>& sd, int s, int plen, int tlen, const Penaltie
       return 1;
     }
   }
+  if (!rM.null()) side_cutoff<NC, BLOCK>(sd, heur, red, cur, lo, rM.first, rM.last, plen, tlen, wcap, pen.metric);
   return 0;
 }
 
 // First k0 of the overlap of w0 (side 0) and w1 (side 1, diagonals inverted)
 // with h0 + h1 >= tlen; indel breakpoints also require the forward side's
 // cell inside the matrix.  Sets the breakpoint (uniform) when found.
-template <int NC, int BLOCK>
-__device__ void scan_breakpoint(const Side<NC>& s0d, const Side<NC>& s1d, const WfRef<NC>& w0, const WfRef<NC>& w1,
+template <int NC, int BLOCK, bool CUT>
+__device__ void scan_breakpoint(const Side<NC, CUT>& s0d, const Side<NC, CUT>& s1d, const WfRef<NC>& w0, const WfRef<NC>& w1,
                                 bool forward, bool indel, int s0, int s1, int gap_open, int comp,
                                 int plen, int tlen, Bp& bp, BlockRed<BLOCK>& red) {
   const int K = tlen - plen;
@@ -351,8 +366,8 @@ __device__ void scan_breakpoint(const Side<NC>& s0d, const Side<NC>& s1d, const 
 }
 
 // wavefront_bialign_overlap (wavefront_bialign.c:878-956)
-template <int NC, int BLOCK>
-__device__ void overlap(const Side<NC>& a0, const Side<NC>& a1, int s0, int s1, bool forward, int plen, int tlen,
+template <int NC, int BLOCK, bool CUT>
+__device__ void overlap(const Side<NC, CUT>& a0, const Side<NC, CUT>& a1, int s0, int s1, bool forward, int plen, int tlen,
                         const Penalties& pen, int wcap, Bp& bp, BlockRed<BLOCK>& red) {
   const WfRef<NC> m0 = wf_ref(a0, s0, kM, pen, wcap);
   if (!m0.present) return;
@@ -373,34 +388,39 @@ __device__ void overlap(const Side<NC>& a0, const Side<NC>& a1, int s0, int s1, 
       if (s0 + si - pen.o2 >= bp.score) continue;
       const WfRef<NC> d2w1 = wf_ref(a1, si, kD2, pen, wcap);
       if (d2w0.present && d2w1.present)
-        scan_breakpoint<NC, BLOCK>(a0, a1, d2w0, d2w1, forward, true, s0, si, pen.o2, kD2, plen, tlen, bp, red);
+        scan_breakpoint<NC, BLOCK, CUT>(a0, a1, d2w0, d2w1, forward, true, s0, si, pen.o2, kD2, plen, tlen, bp, red);
       const WfRef<NC> i2w1 = wf_ref(a1, si, kI2, pen, wcap);
       if (i2w0.present && i2w1.present)
-        scan_breakpoint<NC, BLOCK>(a0, a1, i2w0, i2w1, forward, true, s0, si, pen.o2, kI2, plen, tlen, bp, red);
+        scan_breakpoint<NC, BLOCK, CUT>(a0, a1, i2w0, i2w1, forward, true, s0, si, pen.o2, kI2, plen, tlen, bp, red);
     }
     if constexpr (NC > 1) {  // I1/D1 breakpoints: gap-affine metrics only
       if (s0 + si - pen.o1 >= bp.score) continue;
       const WfRef<NC> d1w1 = wf_ref(a1, si, kD1, pen, wcap);
       if (d1w0.present && d1w1.present)
-        scan_breakpoint<NC, BLOCK>(a0, a1, d1w0, d1w1, forward, true, s0, si, pen.o1, kD1, plen, tlen, bp, red);
+        scan_breakpoint<NC, BLOCK, CUT>(a0, a1, d1w0, d1w1, forward, true, s0, si, pen.o1, kD1, plen, tlen, bp, red);
       const WfRef<NC> i1w1 = wf_ref(a1, si, kI1, pen, wcap);
       if (i1w0.present && i1w1.present)
-        scan_breakpoint<NC, BLOCK>(a0, a1, i1w0, i1w1, forward, true, s0, si, pen.o1, kI1, plen, tlen, bp, red);
+        scan_breakpoint<NC, BLOCK, CUT>(a0, a1, i1w0, i1w1, forward, true, s0, si, pen.o1, kI1, plen, tlen, bp, red);
     }
     if (s0 + si >= bp.score) continue;
     const WfRef<NC> mw1 = wf_ref(a1, si, kM, pen, wcap);
-    if (mw1.present) scan_breakpoint<NC, BLOCK>(a0, a1, m0, mw1, forward, false, s0, si, 0, kM, plen, tlen, bp, red);
+    if (mw1.present) scan_breakpoint<NC, BLOCK, CUT>(a0, a1, m0, mw1, forward, false, s0, si, 0, kM, plen, tlen, bp, red);
   }
 }
 
-template <int NC>
-__device__ void side_init(Side<NC>& sd, int cb, int ce, int wcap) {
+template <int NC, bool CUT>
+__device__ void side_init(Side<NC, CUT>& sd, int cb, int ce, int wcap, int steps) {
   const int lane = threadIdx.x & 63;
   sd.meta.tag = (lane == 0) ? 0 : -1;
   sd.meta.alo = 0;
   sd.meta.flags = (lane == 0) ? (1u << cb) : 0u;
 #pragma unroll
   for (int c = 0; c < NC; ++c) { sd.meta.lo[c] = 0; sd.meta.hi[c] = 0; }
+  if constexpr (CUT) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) { sd.meta.cut.lo[c] = 0; sd.meta.cut.hi[c] = 0; }
+  }
+  sd.steps_wait = steps;  // wavefront_heuristic_clear (wavefront_heuristic.c:114-121)
   sd.cur = 0;
   sd.cb = cb;
   sd.ce = ce;
@@ -409,8 +429,9 @@ __device__ void side_init(Side<NC>& sd, int cb, int ce, int wcap) {
 }
 
 // Extension of the score-0 wavefront (only M can extend).
-template <int NC, int BLOCK, int BITS>
-__device__ int side_extend0(Side<NC>& sd, int plen, int tlen, int wcap, int* max_ak) {
+template <int NC, int BLOCK, int BITS, bool CUT>
+__device__ int side_extend0(Side<NC, CUT>& sd, int plen, int tlen, int wcap, const Penalties& pen,
+                            const HeurArgs& heur, BlockRed<BLOCK>& red, int* max_ak) {
   int v0 = 0;
   *max_ak = 0;
   if (sd.cb != kM) return 0;  // M absent at score 0: nothing to extend, no termination test
@@ -423,10 +444,11 @@ __device__ int side_extend0(Side<NC>& sd, int plen, int tlen, int wcap, int* max
     sd.end_score = 0;
     return 1;
   }
+  side_cutoff<NC, BLOCK>(sd, heur, red, 0, 0, 0, 0, plen, tlen, wcap, pen.metric);
   return 0;
 }
 
-template <int NC, int BLOCK, int BITS>
+template <int NC, int BLOCK, int BITS, bool CUT>
 __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
   constexpr int SPW = 64 / BITS;
   __shared__ int sh_red[BLOCK / 64][2 * NC + 2];
@@ -446,7 +468,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
     const int pair = pr.pair;
     const int plen = pr.pe - pr.pb, tlen = pr.te - pr.tb;
     const int plen_all = a.plen_total[pair], tlen_all = a.tlen_total[pair];
-    Side<NC> f, r;
+    Side<NC, CUT> f, r;
     f.ring = a.ring + (long long)blockIdx.x * a.ring_stride;
     r.ring = f.ring + (long long)pen.nslots * NC * wcap;
     {
@@ -458,8 +480,8 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
       r.T = a.rev_words + rt / SPW; r.tb = (int)(rt % SPW);
     }
     __syncthreads();  // the previous sub-problem is done with the rings
-    side_init(f, pr.comp_begin, pr.comp_end, wcap);
-    side_init(r, pr.comp_end, pr.comp_begin, wcap);
+    side_init(f, pr.comp_begin, pr.comp_end, wcap, a.heur.steps);
+    side_init(r, pr.comp_end, pr.comp_begin, wcap, a.heur.steps);
     __syncthreads();
     Bp bp;
     bp.score = INT_MAX;
@@ -474,19 +496,19 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
     bool last_forward = false;
     int rc;
     // phase 1 (wavefront_bialign.c:1003-1044)
-    if (side_extend0<NC, BLOCK, BITS>(f, plen, tlen, wcap, &fmax)) { status = kBiEndReached; goto done; }
-    if (side_extend0<NC, BLOCK, BITS>(r, plen, tlen, wcap, &rmax)) { status = kBiEndReached; goto done; }
+    if (side_extend0<NC, BLOCK, BITS>(f, plen, tlen, wcap, pen, a.heur, red, &fmax)) { status = kBiEndReached; goto done; }
+    if (side_extend0<NC, BLOCK, BITS>(r, plen, tlen, wcap, pen, a.heur, red, &rmax)) { status = kBiEndReached; goto done; }
     while (true) {
       if (fmax + rmax >= max_ad) break;
       ++sf;
-      rc = side_step<NC, BLOCK, BITS>(f, sf, plen, tlen, pen, wcap, true, &ak, &dsteps, sh_red, red);
+      rc = side_step<NC, BLOCK, BITS>(f, sf, plen, tlen, pen, a.heur, wcap, true, &ak, &dsteps, sh_red, red);
       if (rc < 0) { status = kDevRingOverflow; goto done; }
       if (fmax < ak) fmax = ak;
       last_forward = true;
       if (rc > 0) { status = kBiEndReached; goto done; }
       if (fmax + rmax >= max_ad) break;
       ++sr;
-      rc = side_step<NC, BLOCK, BITS>(r, sr, plen, tlen, pen, wcap, true, &ak, &dsteps, sh_red, red);
+      rc = side_step<NC, BLOCK, BITS>(r, sr, plen, tlen, pen, a.heur, wcap, true, &ak, &dsteps, sh_red, red);
       if (rc < 0) { status = kDevRingOverflow; goto done; }
       if (rmax < ak) rmax = ak;
       last_forward = false;
@@ -503,17 +525,17 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
         if (last_forward) {
           const int min_sr = (sr > scope - 1) ? sr - (scope - 1) : 0;
           if (sf + min_sr - gap_open >= bp.score) break;
-          overlap<NC, BLOCK>(f, r, sf, sr, true, plen, tlen, pen, wcap, bp, red);
+          overlap<NC, BLOCK, CUT>(f, r, sf, sr, true, plen, tlen, pen, wcap, bp, red);
           ++sr;
-          rc = side_step<NC, BLOCK, BITS>(r, sr, plen, tlen, pen, wcap, false, &ak, &dsteps, sh_red, red);
+          rc = side_step<NC, BLOCK, BITS>(r, sr, plen, tlen, pen, a.heur, wcap, false, &ak, &dsteps, sh_red, red);
           if (rc < 0) { status = kDevRingOverflow; goto done; }
           if (rc > 0) { status = kBiEndReached; goto done; }
         }
         const int min_sf = (sf > scope - 1) ? sf - (scope - 1) : 0;
         if (min_sf + sr - gap_open >= bp.score) break;
-        overlap<NC, BLOCK>(r, f, sr, sf, false, plen, tlen, pen, wcap, bp, red);
+        overlap<NC, BLOCK, CUT>(r, f, sr, sf, false, plen, tlen, pen, wcap, bp, red);
         ++sf;
-        rc = side_step<NC, BLOCK, BITS>(f, sf, plen, tlen, pen, wcap, false, &ak, &dsteps, sh_red, red);
+        rc = side_step<NC, BLOCK, BITS>(f, sf, plen, tlen, pen, a.heur, wcap, false, &ak, &dsteps, sh_red, red);
         if (rc < 0) { status = kDevRingOverflow; goto done; }
         if (rc > 0) { status = kBiEndReached; goto done; }
         if (sr + sf >= a.max_steps) { status = kDevMaxSteps; goto done; }
@@ -545,19 +567,23 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
   if (threadIdx.x == 0 && wg_dsteps) atomicAdd(a.dsteps_total, (unsigned long long)wg_dsteps);
 }
 
-template <int NC, int BLOCK, int BITS>
+template <int NC, int BLOCK, int BITS, bool CUT>
 const void* bp_fn() {
-  return reinterpret_cast<const void*>(&wfa_breakpoint_kernel<NC, BLOCK, BITS>);
+  return reinterpret_cast<const void*>(&wfa_breakpoint_kernel<NC, BLOCK, BITS, CUT>);
 }
 
-const void* select_breakpoint(int ncomp, int bits, int block) {
-#define WFA_BSEL(NC, BL, BI) \
-  if (ncomp == NC && block == BL && bits == BI) return bp_fn<NC, BL, BI>();
-#define WFA_BSEL_NC(BL) \
-  WFA_BSEL(1, BL, 2) WFA_BSEL(1, BL, 8) WFA_BSEL(3, BL, 2) WFA_BSEL(3, BL, 8) WFA_BSEL(5, BL, 2) WFA_BSEL(5, BL, 8)
-  WFA_BSEL_NC(64)
-  WFA_BSEL_NC(256)
-  WFA_BSEL_NC(1024)
+const void* select_breakpoint(int ncomp, int bits, int block, bool cut) {
+#define WFA_BSEL(NC, BL, BI, CU) \
+  if (ncomp == NC && block == BL && bits == BI && cut == CU) return bp_fn<NC, BL, BI, CU>();
+#define WFA_BSEL_NC(BL, CU) \
+  WFA_BSEL(1, BL, 2, CU) WFA_BSEL(1, BL, 8, CU) WFA_BSEL(3, BL, 2, CU) WFA_BSEL(3, BL, 8, CU) \
+  WFA_BSEL(5, BL, 2, CU) WFA_BSEL(5, BL, 8, CU)
+  WFA_BSEL_NC(64, false)
+  WFA_BSEL_NC(256, false)
+  WFA_BSEL_NC(1024, false)
+  WFA_BSEL_NC(64, true)
+  WFA_BSEL_NC(256, true)
+  WFA_BSEL_NC(1024, true)
 #undef WFA_BSEL_NC
 #undef WFA_BSEL
   return nullptr;
@@ -618,8 +644,8 @@ hipError_t launch_gather_ops(const OpsPiece* pieces, int n, const char* src, cha
   return hipGetLastError();
 }
 
-int breakpoint_max_blocks_per_cu(int ncomp, int bits, int block) {
-  const void* f = select_breakpoint(ncomp, bits, block);
+int breakpoint_max_blocks_per_cu(int ncomp, int bits, int block, bool cut) {
+  const void* f = select_breakpoint(ncomp, bits, block, cut);
   if (!f) return 0;
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, 0) != hipSuccess) return 0;
@@ -627,7 +653,7 @@ int breakpoint_max_blocks_per_cu(int ncomp, int bits, int block) {
 }
 
 hipError_t launch_breakpoint(const BiArgs& a, int ncomp, int bits, int block, int grid, hipStream_t stream) {
-  const void* f = select_breakpoint(ncomp, bits, block);
+  const void* f = select_breakpoint(ncomp, bits, block, a.heur.wfadaptive != 0);
   if (!f) return hipErrorInvalidValue;
   void* args[] = {const_cast<BiArgs*>(&a)};
   return hipLaunchKernel(f, dim3(grid), dim3(block), args, 0, stream);

@@ -12,13 +12,27 @@ namespace wfa {
 // component, present/null flags).  A source is read with v_readlane at a
 // uniform slot index, so no memory round trip sits on the per-step critical
 // path; every wave of a workgroup keeps an identical copy.
+// With a heuristic cut-off (CUT) a slot also keeps, per component, the range
+// the cut-off left: it sets the limits of the wavefronts computed from it,
+// while reads still see the whole trimmed range (the reference narrows
+// lo/hi but not the initialised range that wavefront_compute_init_ends,
+// wavefront_compute.c:498-575, fills with NULLs).
+template <int NC, bool CUT>
+struct CutRange {
+  int lo[NC];
+  int hi[NC];
+};
 template <int NC>
+struct CutRange<NC, false> {};
+
+template <int NC, bool CUT = false>
 struct MetaRegs {
   int tag;             // score held by the slot, -1 if none
   int alo;             // lo of the computed range: array origin of all components
-  unsigned flags;      // bit c: component present; bit 8+c: component null (trimmed empty)
-  int lo[NC];
+  unsigned flags;      // bit c: component present; bit 8+c: component null (empty after trim / cut-off)
+  int lo[NC];          // trimmed range (readable)
   int hi[NC];
+  CutRange<NC, CUT> cut;
 };
 
 // A compute input.  lim_lo/lim_hi are the limits it contributes to the output
@@ -35,8 +49,8 @@ constexpr int kFarDiagonal = 0x40000000;
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ int lane_of(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 
-template <int NC>
-__device__ __forceinline__ Src get_src(const MetaRegs<NC>& m, int s_src, int slot, int R, int c, int wcap) {
+template <int NC, bool CUT>
+__device__ __forceinline__ Src get_src(const MetaRegs<NC, CUT>& m, int s_src, int slot, int R, int c, int wcap) {
   Src r{1, -1, kFarDiagonal, 0, 0};
   if (s_src < 0) return r;
   if (slot < 0) slot += R;
@@ -45,9 +59,15 @@ __device__ __forceinline__ Src get_src(const MetaRegs<NC>& m, int s_src, int slo
   const unsigned fl = (unsigned)lane_of((int)m.flags, slot);
   if (tag != s_src) return r;
   if (!(fl & (1u << c)) || (fl & (0x100u << c))) return r;
-  r.lim_lo = r.lo = lane_of(m.lo[c], slot);
-  r.lim_hi = lane_of(m.hi[c], slot);
-  r.span = r.lim_hi - r.lim_lo;
+  r.lo = lane_of(m.lo[c], slot);
+  r.span = lane_of(m.hi[c], slot) - r.lo;
+  if constexpr (CUT) {
+    r.lim_lo = lane_of(m.cut.lo[c], slot);
+    r.lim_hi = lane_of(m.cut.hi[c], slot);
+  } else {
+    r.lim_lo = r.lo;
+    r.lim_hi = r.lo + r.span;
+  }
   r.base = (slot * NC + c) * wcap + (r.lo - lane_of(m.alo, slot));
   return r;
 }
@@ -153,6 +173,75 @@ __device__ __forceinline__ void edit_prune(WaveRange& r, RingPtr ring, const Src
   if (abs(lo - kend) <= smax && abs(hi - kend) <= smax) return;
   r.first = max(lo, kend - wmin);
   r.last = min(hi, kend + wmin);
+}
+
+template <int BLOCK>
+struct BlockRed {
+  static constexpr int NW = BLOCK / 64;
+  int* sh;                 // NW ints of LDS
+  // Block-wide min of per-thread values (uniform result).
+  __device__ __forceinline__ int min_all(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    v = uniform(v);  // every lane holds the minimum
+    if constexpr (NW == 1) return v;
+    const int lane = threadIdx.x & 63, wid = uniform(threadIdx.x >> 6);
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    int r = sh[0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) r = min(r, sh[w]);
+    return uniform(r);
+  }
+  __device__ __forceinline__ int max_all(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    v = uniform(v);  // every lane holds the maximum
+    if constexpr (NW == 1) return v;
+    const int lane = threadIdx.x & 63, wid = uniform(threadIdx.x >> 6);
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    int r = sh[0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) r = max(r, sh[w]);
+    return uniform(r);
+  }
+};
+
+// Distance of a NULL cell (-WAVEFRONT_OFFSET_NULL, wavefront_heuristic.c:125-133).
+constexpr int kNullDistance = 1 << 30;
+
+// wf-adaptive cut-off (wavefront_heuristic_wfadaptive and its reduce step,
+// wavefront_heuristic.c:229-290) of the extended M wavefront [lo, hi], whose
+// diagonal k holds m[k]: keeps the diagonals whose distance to the end,
+// max(plen - v, tlen - h), is within thr of the least one, scanning in from
+// both ends but never past the end diagonal.  Two block passes: the least
+// distance, then the first and last diagonals within thr of it.  The kept
+// range (uniform) goes to *clo, *chi.
+template <int BLOCK, typename RingPtr>
+__device__ void wfadaptive_cut(RingPtr m, int lo, int hi, int plen, int tlen, int thr, BlockRed<BLOCK>& red,
+                               int* clo, int* chi) {
+  const int tid = threadIdx.x;
+  int dmin = max(plen, tlen);
+  for (int k = lo + tid; k <= hi; k += BLOCK) {
+    const int off = (int)m[k];
+    dmin = min(dmin, off >= 0 ? max(plen - (off - k), tlen - off) : kNullDistance);
+  }
+  dmin = red.min_all(dmin);
+  int first = INT_MAX, last = INT_MIN;
+  for (int k = lo + tid; k <= hi; k += BLOCK) {
+    const int off = (int)m[k];
+    const int d = off >= 0 ? max(plen - (off - k), tlen - off) : kNullDistance;
+    if (d - dmin <= thr) { first = min(first, k); last = max(last, k); }
+  }
+  first = red.min_all(first);
+  last = red.max_all(last);
+  const int kend = tlen - plen;
+  const int top = min(kend, hi);
+  const int lo_r = lo >= top ? lo : min(first, top);
+  const int bottom = max(kend, lo_r);
+  *clo = lo_r;
+  *chi = hi <= bottom ? hi : max(last, bottom);
 }
 
 // Orders LDS traffic between the lanes of the (single) wave of a workgroup:

@@ -99,6 +99,10 @@ struct ForwardArgs {
   int pbf, pef, tbf, tef;
   int* out_end_k;              // end cell of each completed pair: diagonal and offset
   int* out_end_off;
+  // wf-adaptive heuristic (wavefront_heuristic.c:229-290); the kernel
+  // instance with the cut-off is launched when wfadaptive is set.
+  int wfadaptive;
+  int h_min_len, h_thr, h_steps;
   Penalties pen;
 };
 
@@ -184,6 +188,13 @@ struct BiOutcome {             // wf_bialign_breakpoint_t plus the search status
   int pad;
   long long dsteps;
 };
+// wf-adaptive parameters (wavefront_heuristic.c:229-290): the cut-off kernel
+// instance runs when wfadaptive is set.
+struct HeurArgs {
+  int wfadaptive;
+  int min_len, thr, steps;
+};
+
 struct BiArgs {
   int n;
   const BiProblem* probs;
@@ -202,6 +213,7 @@ struct BiArgs {
   unsigned int* queue;
   unsigned long long* dsteps_total;
   int max_steps;
+  HeurArgs heur;
   Penalties pen;
 };
 hipError_t launch_breakpoint(const BiArgs& a, int ncomp, int bits, int block, int grid, hipStream_t stream);
@@ -217,7 +229,7 @@ struct OpsPiece {
 hipError_t launch_gather_ops(const OpsPiece* pieces, int n, const char* src, char* dst, hipStream_t stream);
 hipError_t launch_reverse_pack(const uint64_t* words, const int64_t* src_pos, const int64_t* dst_pos, const int* len,
                                int n, int bits, uint64_t* out, hipStream_t stream);
-int breakpoint_max_blocks_per_cu(int ncomp, int bits, int block);
+int breakpoint_max_blocks_per_cu(int ncomp, int bits, int block, bool cut);
 
 // Lean family: whether the penalties' slot structure has an instance, its
 // dynamic LDS, occupancy and launch (ForwardArgs as for launch_forward; the
@@ -230,7 +242,7 @@ hipError_t launch_forward_lean(const ForwardArgs& a, const LaunchShape& shape, h
 
 // Dynamic-LDS bytes and occupancy for a configuration.
 size_t forward_lds_bytes(int family, int ncomp, int nslots, int wcap, int lmax_words);
-int forward_max_blocks_per_cu(int family, int block, int ncomp, int bits, size_t lds_bytes);
+int forward_max_blocks_per_cu(int family, int block, int ncomp, int bits, bool cut, size_t lds_bytes);
 int pick_block(int family, int max_width);
 
 }  // namespace wfa

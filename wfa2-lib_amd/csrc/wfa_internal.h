@@ -79,7 +79,11 @@ struct JobSet {
   int* end_k = nullptr;        // end cell (diagonal, offset) of each completed job
   int* end_off = nullptr;
   EndsFree ef;
+  wfa::HeurArgs heur{};        // wf-adaptive (none for BiWFA base jobs)
 };
+
+// The aligner's wf-adaptive parameters (none unless the strategy is wf-adaptive).
+wfa::HeurArgs heuristic_of(const wavefront_aligner_t* wf);
 
 // Grow-only pinned host staging buffer (hipHostMalloc).
 struct HostBuf {
@@ -147,6 +151,7 @@ struct wfa_device_batch_t {
   // results: on the device after a run, copied to the host on demand
   std::vector<int> status, score, ops_begin, end_k, end_off;
   wfa_host::EndsFree ef;                   // form of the last run
+  wfa::HeurArgs heur{};                    // heuristic of the last run
   bool have_results = false;
   bool host_results = false;
   bool ops_on_device = false;

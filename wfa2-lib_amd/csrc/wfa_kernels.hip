@@ -22,7 +22,7 @@
 namespace wfa {
 namespace {
 
-template <int NC, int BLOCK, bool SMALL, int BITS>
+template <int NC, int BLOCK, bool SMALL, int BITS, bool CUT>
 __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
   // The LDS family stores the ring as int16, the HBM family as int32.
   using RT = typename std::conditional<SMALL, int16_t, int>::type;
@@ -34,6 +34,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
   __shared__ int sh_bcast[2];
   __shared__ int sh_red[NW > 1 ? NW : 1][2 * NC + 1];
   __shared__ int sh_hit[NW > 1 ? NW : 1];
+  BlockRed<BLOCK> red{sh_hit};
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -70,24 +71,44 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
     }
   };
 
-  // Least of the per-wave values v (uniform in each wave) over the workgroup.
-  auto block_min = [&](int v) -> int {
-    if constexpr (NW == 1) {
-      return v;
-    } else {
-      __syncthreads();
-      if (lane == 0) sh_hit[wid] = v;
-      __syncthreads();
-      int r = sh_hit[0];
+  // wf-adaptive cut-off of the M wavefront of ring slot `slot`, trimmed to
+  // [lo, hi] (wavefront_heuristic_cufoff, wavefront_heuristic.c:506-561):
+  // every h_steps-th non-empty M wavefront of at least h_min_len diagonals
+  // is cut, then I1/D1/I2/D2 are narrowed to M's range (:163-171).  Indel
+  // and edit then read M only inside the cut range: their compute step NULLs
+  // the two cells beyond each end (wavefront_compute_edit.c:355-358).
+  auto cutoff = [&](MetaRegs<NC, CUT>& mt, int& steps_wait, int slot, int alo, int lo, int hi, int plen_, int tlen_) {
+    if constexpr (CUT) {
+      --steps_wait;
+      if (steps_wait > 0 || hi - lo + 1 < a.h_min_len) return;
+      const RT* m = ring + (slot * NC + kM) * wcap - alo;
+      int clo, chi;
+      wfadaptive_cut<BLOCK>(m, lo, hi, plen_, tlen_, a.h_thr, red, &clo, &chi);
+      steps_wait = a.h_steps;
+      // unchanged M: nothing else is narrowed either (wavefront_heuristic.c:540-541)
+      if (clo == lo && chi == hi) return;
+      if (lane == slot) {
+        unsigned fl = mt.flags;
+        mt.cut.lo[kM] = clo;
+        mt.cut.hi[kM] = chi;
+        if (clo > chi) fl |= 0x100u;
+        if (pen.metric <= kEdit) { mt.lo[kM] = clo; mt.hi[kM] = chi; }
 #pragma unroll
-      for (int w = 1; w < NW; ++w) r = min(r, sh_hit[w]);
-      return uniform(r);
+        for (int c = 1; c < NC; ++c) {
+          if ((fl & (1u << c)) && !(fl & (0x100u << c))) {
+            mt.cut.lo[c] = max(mt.cut.lo[c], clo);
+            mt.cut.hi[c] = min(mt.cut.hi[c], chi);
+            if (mt.cut.lo[c] > mt.cut.hi[c]) fl |= 0x100u << c;
+          }
+        }
+        mt.flags = fl;
+      }
     }
   };
 
   unsigned long long chunk_cur = 0, chunk_end = 0;  // record chunk of this workgroup (16-byte units)
   unsigned long long wg_dsteps = 0;                  // diagonal-steps of this workgroup
-  MetaRegs<NC> meta;
+  MetaRegs<NC, CUT> meta;
 
   for (;;) {
     int qv = 0;
@@ -128,6 +149,11 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
     meta.flags = (lane == 0) ? (1u << cbeg) : 0u;
 #pragma unroll
     for (int c = 0; c < NC; ++c) { meta.lo[c] = 0; meta.hi[c] = 0; }
+    if constexpr (CUT) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) { meta.cut.lo[c] = 0; meta.cut.hi[c] = 0; }
+    }
+    int steps_wait = a.h_steps;  // wavefront_heuristic_clear (wavefront_heuristic.c:114-121)
     int cur = 0;
     int s = 0;
     int status = 1;  // running
@@ -171,8 +197,13 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
           const unsigned long long bal = __ballot(hit);
           if (bal && hitk == INT_MAX) hitk = kb + wid * 64 + (int)__builtin_ctzll(bal);
         }
-        if (lane == 0) { meta.alo = lo0; meta.lo[kM] = lo0; meta.hi[kM] = hi0; }
-        hitk = block_min(hitk);
+        if (lane == 0) {
+          meta.alo = lo0;
+          meta.lo[kM] = lo0;
+          meta.hi[kM] = hi0;
+          if constexpr (CUT) { meta.cut.lo[kM] = lo0; meta.cut.hi[kM] = hi0; }
+        }
+        hitk = red.min_all(hitk);
         block_sync();
         if (hitk != INT_MAX) {
           status = kDevCompleted;
@@ -188,6 +219,11 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
       if (cbeg == kM && cend == kM && kend == 0 && v0 >= tlen) status = kDevCompleted;
     }
     block_sync();
+    // the heuristic also runs after the score-0 extension (wavefront_extend.c:112-118)
+    if (CUT && status == 1 && cbeg == kM) {
+      const int lo0 = a.endsfree ? -a.pbf : 0, hi0 = a.endsfree ? a.tbf : 0;
+      cutoff(meta, steps_wait, 0, lo0, lo0, hi0, plen, tlen);
+    }
 
     while (status == 1) {
       s += g;
@@ -419,10 +455,14 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
           if (has_i2) { fl |= 8u | (rI2.null() ? 0x800u : 0u); meta.lo[kI2] = rI2.tlo(lo); meta.hi[kI2] = rI2.thi(lo); }
           if (has_d2) { fl |= 16u | (rD2.null() ? 0x1000u : 0u); meta.lo[kD2] = rD2.tlo(lo); meta.hi[kD2] = rD2.thi(lo); }
         }
+        if constexpr (CUT) {
+#pragma unroll
+          for (int c = 0; c < NC; ++c) { meta.cut.lo[c] = meta.lo[c]; meta.cut.hi[c] = meta.hi[c]; }
+        }
         meta.flags = fl;
       }
       if (a.endsfree) {
-        hitk = block_min(hitk);
+        hitk = red.min_all(hitk);
         if (hitk != INT_MAX) {
           status = kDevCompleted;
           end_k = hitk;
@@ -446,6 +486,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
           if (vend >= tlen) status = kDevCompleted;
         }
       }
+      if (CUT && status == 1 && !rM.null()) cutoff(meta, steps_wait, cur, lo, rM.first, rM.last, plen, tlen);
       if (status == 1 && s >= s_cap) status = kDevBadTrace;  // unreachable for valid input
     }
     if (tid == 0) {
@@ -614,22 +655,27 @@ __global__ void wfa_reset_kernel(Counters* c, int all) {
   }
 }
 
-template <int NC, int BLOCK, bool SMALL, int BITS>
+template <int NC, int BLOCK, bool SMALL, int BITS, bool CUT>
 const void* forward_fn() {
-  return reinterpret_cast<const void*>(&wfa_forward_kernel<NC, BLOCK, SMALL, BITS>);
+  return reinterpret_cast<const void*>(&wfa_forward_kernel<NC, BLOCK, SMALL, BITS, CUT>);
 }
 
 // Every instance the host can select.
-const void* select_forward(int family, int ncomp, int bits, int block) {
-#define WFA_SEL(NC, BL, SM, BI) \
-  if (ncomp == NC && block == BL && (family == kSmall) == SM && bits == BI) return forward_fn<NC, BL, SM, BI>();
-#define WFA_SEL_NC(BL, SM) \
-  WFA_SEL(1, BL, SM, 2) WFA_SEL(1, BL, SM, 8) WFA_SEL(3, BL, SM, 2) WFA_SEL(3, BL, SM, 8) \
-  WFA_SEL(5, BL, SM, 2) WFA_SEL(5, BL, SM, 8)
-  WFA_SEL_NC(64, true)
-  WFA_SEL_NC(64, false)
-  WFA_SEL_NC(256, false)
-  WFA_SEL_NC(1024, false)
+const void* select_forward(int family, int ncomp, int bits, int block, bool cut) {
+#define WFA_SEL(NC, BL, SM, BI, CU) \
+  if (ncomp == NC && block == BL && (family == kSmall) == SM && bits == BI && cut == CU) \
+    return forward_fn<NC, BL, SM, BI, CU>();
+#define WFA_SEL_NC(BL, SM, CU) \
+  WFA_SEL(1, BL, SM, 2, CU) WFA_SEL(1, BL, SM, 8, CU) WFA_SEL(3, BL, SM, 2, CU) WFA_SEL(3, BL, SM, 8, CU) \
+  WFA_SEL(5, BL, SM, 2, CU) WFA_SEL(5, BL, SM, 8, CU)
+  WFA_SEL_NC(64, true, false)
+  WFA_SEL_NC(64, false, false)
+  WFA_SEL_NC(256, false, false)
+  WFA_SEL_NC(1024, false, false)
+  WFA_SEL_NC(64, true, true)
+  WFA_SEL_NC(64, false, true)
+  WFA_SEL_NC(256, false, true)
+  WFA_SEL_NC(1024, false, true)
 #undef WFA_SEL_NC
 #undef WFA_SEL
   return nullptr;
@@ -653,8 +699,8 @@ int pick_block(int family, int max_width) {
   return 1024;
 }
 
-int forward_max_blocks_per_cu(int family, int block, int ncomp, int bits, size_t lds_bytes) {
-  const void* f = select_forward(family, ncomp, bits, block);
+int forward_max_blocks_per_cu(int family, int block, int ncomp, int bits, bool cut, size_t lds_bytes) {
+  const void* f = select_forward(family, ncomp, bits, block, cut);
   if (!f) return 0;
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, lds_bytes) != hipSuccess) return 0;
@@ -663,7 +709,7 @@ int forward_max_blocks_per_cu(int family, int block, int ncomp, int bits, size_t
 
 hipError_t launch_forward(const ForwardArgs& a, int ncomp, int bits, const LaunchShape& shape,
                           hipStream_t stream) {
-  const void* f = select_forward(shape.family, ncomp, bits, shape.block);
+  const void* f = select_forward(shape.family, ncomp, bits, shape.block, a.wfadaptive != 0);
   if (!f) return hipErrorInvalidValue;
   if (shape.lds_bytes > 48 * 1024) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shape.lds_bytes);

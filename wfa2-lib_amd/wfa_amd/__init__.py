@@ -303,7 +303,7 @@ class Aligner:
     """wavefront_aligner_new with attributes built from keyword arguments."""
 
     def __init__(self, metric="affine", penalties=(0, 4, 6, 2), memory="high", score_only=False,
-                 max_steps=0, heuristic_none=True, ends_free=None, extension=False):
+                 max_steps=0, heuristic_none=True, ends_free=None, extension=False, heuristic=None):
         a = default_attributes()
         set_metric(a, metric, penalties)
         a.alignment_scope = 0 if score_only else 1
@@ -317,6 +317,10 @@ class Aligner:
             a.alignment_form.extension = True
         if heuristic_none:
             a.heuristic.strategy = 0
+        if heuristic is not None:  # (strategy, min_wavefront_length, max_distance_threshold, steps_between_cutoffs)
+            (a.heuristic.strategy, a.heuristic.min_wavefront_length, a.heuristic.max_distance_threshold,
+             a.heuristic.steps_between_cutoffs) = heuristic
+            a.heuristic.steps_wait = heuristic[3]
         if max_steps:
             a.system.max_alignment_steps = int(max_steps)
         self.attr = a
