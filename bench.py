@@ -76,6 +76,68 @@ def cpu_baseline(ps, threads, res=None):
             "gcups": ps.cells / secs / 1e9}
 
 
+# Configurations beyond the headline one (indel / edit / gap-linear, ends-free,
+# extension, the wf-adaptive heuristic): (name, workload, metric, penalties,
+# aligner keywords, reference-build ISA for the spot check).  The ends-free
+# forms are checked against the reference's scalar build, whose diagonal
+# order the device follows.
+VARIANTS = [
+    ("edit end-to-end", "short", "edit", (0, 1, 1), {}, None),
+    ("indel end-to-end", "short", "indel", (0, 1, 1), {}, None),
+    ("gap-linear (0,4,2) end-to-end", "short", "linear", (0, 4, 2), {}, None),
+    ("gap-affine (0,4,6,2) wf-adaptive (10,50,1), the default attributes", "short", "affine", (0, 4, 6, 2),
+     {"heuristic": (4, 10, 50, 1)}, None),
+    ("gap-affine (0,4,6,2) ends-free, text ends free up to 30", "semiglobal", "affine", (0, 4, 6, 2),
+     {"ends_free": (0, 0, 30, 30)}, "v2"),
+    ("gap-affine (0,4,6,2) extension", "semiglobal", "affine", (0, 4, 6, 2), {"extension": True}, "v2"),
+    ("gap-affine-2p (0,4,6,2,24,1) end-to-end", "short", "affine2p", (0, 4, 6, 2, 24, 1), {}, None),
+]
+
+
+def variant_measurements(n, cpu_sample, threads, no_cpu):
+    """pairs/s of each configuration of VARIANTS over n resident pairs (short
+    reads, or short reads with random flanks of up to 30 bases for the
+    ends-free forms), beside the reference build on a sample of the same pairs
+    (one aligner per thread), with a parity spot check of that sample."""
+    import numpy as np
+    import wfa_amd
+    from wfa_amd import datagen
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    O = None
+    if not no_cpu:
+        try:
+            import oracle as O
+        except Exception as exc:  # pragma: no cover
+            log("reference build unavailable: %s" % exc)
+    sets = {"short": datagen.generate(1, 7, n), "semiglobal": datagen.semiglobal(1, 7, n)}
+    out = []
+    for name, wl, metric, pen, kw, isa in VARIANTS:
+        ps = sets[wl]
+        al = wfa_amd.Aligner(metric=metric, penalties=pen, **kw)
+        rate, dt, st = batch_rate(al, ps)
+        row = {"config": name, "pairs": ps.n, "value": rate, "unit": "pairs/s",
+               "gcups": ps.cells / dt / 1e9, "diagonal_steps": int(st.diagonal_steps)}
+        if O is not None and O.ref_available(isa):
+            k = min(cpu_sample, ps.n)
+            sub = ps.subset(list(range(k)))
+            pairs = O.Pairs(sub.patterns(), sub.texts())
+            conf = O.Conf(metric, pen, "high", **kw)
+            ref = O.ref_align(conf, pairs, threads, isa=isa)
+            got = al.align_batch(sub)
+            differ = sum(1 for i in range(k) if got.status[i] != ref.status[i] or got.score[i] != ref.score[i]
+                         or got.ops(i) != ref.ops(i))
+            row["cpu_reference"] = {"value": k / ref.seconds, "unit": "pairs/s", "cores": threads,
+                                    "sample": "first %d pairs, %s" % (k, os.path.basename(O.ref_lib_path(isa)))}
+            row["parity_spot_check"] = {"pairs": k, "differ": differ}
+            if differ:
+                raise RuntimeError("%s: GPU results differ from the reference build" % name)
+        log("%s: %.3e pairs/s%s" % (name, rate, (" (reference %.3e pairs/s on %d threads)"
+                                                  % (row["cpu_reference"]["value"], threads))
+                                    if "cpu_reference" in row else ""))
+        out.append(row)
+    return out
+
+
 def serve_measurement(duration_s, seed, log, lanes=(1, 1, 1)):
     """Serving: single-pair requests from the mixture of 60% 1 kbp / 5% pairs,
     35% 1-20 kbp / 1-15% pairs and 5% 100 kbp / 15% pairs (BiWFA), gap-affine
@@ -250,6 +312,8 @@ def main():
     ap.add_argument("--serve-lanes", default="1,1,1", help="short,medium,long lanes of the request loop")
     ap.add_argument("--extras", type=int, default=1,
                     help="0: only the short-read batch (plain form, 1 kbp case and serving skipped)")
+    ap.add_argument("--variant-pairs", type=int, default=100000,
+                    help="pairs per configuration of the variant table (0: skip)")
     ap.add_argument("--workload", choices=("short-reads",) + tuple(LONG_WORKLOADS), default="short-reads",
                     help="short-reads: the headline batch; the others print their own line")
     ap.add_argument("--long-pairs", type=int, default=0, help="pairs of a long workload (0: its default)")
@@ -339,6 +403,10 @@ def main():
         log("1 kbp/5%%: %d pairs in %.2f ms (%.3e pairs/s)" % (c0.n, c0_dt * 1e3, c0_rate))
         c0_out = {"pairs": c0.n, "value": c0_rate, "unit": "pairs/s", "gcups": c0.cells * c0_rate / c0.n / 1e9}
 
+    variants = None
+    if args.extras and args.variant_pairs > 0:
+        variants = variant_measurements(args.variant_pairs, 4000, min(16, os.cpu_count() or 1), args.no_cpu)
+
     serve = None
     if args.extras and args.serve_seconds > 0:
         serve = serve_measurement(args.serve_seconds, 41, log, tuple(int(x) for x in args.serve_lanes.split(",")))
@@ -385,6 +453,7 @@ def main():
         "pcie_inclusive_pairs_per_s": ps.n / (total / args.steps + d2h_s),
         "plain_gpu": plain,
         "kbp1_5pct": c0_out,
+        "variants": variants,
         "serve": serve,
         "roofline": {
             "bound": "hbm",
