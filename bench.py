@@ -138,6 +138,37 @@ def variant_measurements(n, cpu_sample, threads, no_cpu):
     return out
 
 
+def seq_file_measurement(n):
+    """End-to-end .seq -> .alg throughput of wfa_align_seq_file (parse, device
+    batches, format and write, overlapped) on n short-read pairs written to a
+    local temporary file first (outside the timing)."""
+    import shutil
+    import tempfile
+    import wfa_amd
+    from wfa_amd import datagen
+    ps = datagen.generate(1, 9, n)
+    tmp = tempfile.mkdtemp(prefix="wfa_seq_")
+    try:
+        src, dst = os.path.join(tmp, "in.seq"), os.path.join(tmp, "out.alg")
+        with open(src, "wb") as f:
+            for i in range(ps.n):
+                f.write(b">" + ps.pattern(i) + b"\n<" + ps.text(i) + b"\n")
+        al = wfa_amd.Aligner(metric="affine", penalties=(0, 4, 6, 2))
+        rc, st = al.align_seq_file(src, dst)
+        if rc != 0 or st.pairs != ps.n:
+            raise RuntimeError("seq file run failed (%d, %d pairs)" % (rc, st.pairs))
+        out = {"pairs": int(st.pairs), "value": st.pairs / st.seconds, "unit": "pairs/s",
+               "input_mb_s": st.bytes_in / st.seconds / 1e6, "seconds": st.seconds,
+               "read_s": st.read_seconds, "align_s": st.align_seconds, "write_s": st.write_seconds,
+               "sample": "%d short-read pairs (gap-affine 0,4,6,2, score+CIGAR) from a local .seq file "
+                         "of %.0f MB to a .alg file of %.0f MB" % (ps.n, st.bytes_in / 1e6, st.bytes_out / 1e6)}
+        log(".seq -> .alg: %.3e pairs/s (%.0f MB/s in; read %.2f s, align %.2f s, write %.2f s, total %.2f s)"
+            % (out["value"], out["input_mb_s"], st.read_seconds, st.align_seconds, st.write_seconds, st.seconds))
+        return out
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def serve_measurement(duration_s, seed, log, lanes=(1, 1, 1)):
     """Serving: single-pair requests from the mixture of 60% 1 kbp / 5% pairs,
     35% 1-20 kbp / 1-15% pairs and 5% 100 kbp / 15% pairs (BiWFA), gap-affine
@@ -312,6 +343,7 @@ def main():
     ap.add_argument("--serve-lanes", default="1,1,1", help="short,medium,long lanes of the request loop")
     ap.add_argument("--extras", type=int, default=1,
                     help="0: only the short-read batch (plain form, 1 kbp case and serving skipped)")
+    ap.add_argument("--seq-pairs", type=int, default=500000, help="pairs of the .seq -> .alg run (0: skip)")
     ap.add_argument("--variant-pairs", type=int, default=100000,
                     help="pairs per configuration of the variant table (0: skip)")
     ap.add_argument("--workload", choices=("short-reads",) + tuple(LONG_WORKLOADS), default="short-reads",
@@ -407,6 +439,8 @@ def main():
     if args.extras and args.variant_pairs > 0:
         variants = variant_measurements(args.variant_pairs, 4000, min(16, os.cpu_count() or 1), args.no_cpu)
 
+    seq_file = seq_file_measurement(args.seq_pairs) if args.extras and args.seq_pairs > 0 else None
+
     serve = None
     if args.extras and args.serve_seconds > 0:
         serve = serve_measurement(args.serve_seconds, 41, log, tuple(int(x) for x in args.serve_lanes.split(",")))
@@ -454,6 +488,7 @@ def main():
         "plain_gpu": plain,
         "kbp1_5pct": c0_out,
         "variants": variants,
+        "seq_file": seq_file,
         "serve": serve,
         "roofline": {
             "bound": "hbm",

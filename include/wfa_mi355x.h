@@ -213,9 +213,10 @@ typedef struct {
   alignment_system_t system;
 } wavefront_aligner_attr_t;
 
-/* Identical defaults to wavefront_attributes.c:38-100 (note: heuristic
-   wf-adaptive(10,50,1); set heuristic.strategy = wf_heuristic_none, or call
-   wavefront_aligner_set_heuristic_none, for the exact device path). */
+/* Identical defaults to wavefront_attributes.c:38-100, including the
+   heuristic wf-adaptive(10,50,1), which the device runs with the reference's
+   results; set heuristic.strategy = wf_heuristic_none (or call
+   wavefront_aligner_set_heuristic_none) for exact alignment. */
 extern wavefront_aligner_attr_t wavefront_aligner_attr_default;
 
 /* ---------------------------------------------------------------- aligner
@@ -313,7 +314,7 @@ char* wavefront_align_strerror_short(const int error_code);
 int wavefront_align(wavefront_aligner_t* const wf_aligner,
     const char* const pattern, const int pattern_length,
     const char* const text, const int text_length);
-typedef bool (*alignment_match_funct_t)(int, int, void*);
+typedef int (*alignment_match_funct_t)(int, int, void*);  /* wavefront_sequences.h:55 */
 int wavefront_align_lambda(wavefront_aligner_t* const wf_aligner,
     alignment_match_funct_t const match_funct, void* match_funct_arguments,
     const int pattern_length, const int text_length);
@@ -430,6 +431,27 @@ double wfa_server_clock(void);
 /* Finishes the replays and every queued request, then frees the server. */
 void wfa_server_destroy(wfa_server_t* server);
 
+/* ------------------------------------------------------------ file I/O (new)
+ * The input and output formats of the reference's align_benchmark tool,
+ * streamed through the device: a .seq file of line pairs (">pattern",
+ * "<text"; tools/align_benchmark/align_benchmark.c:313-339) is aligned in
+ * batches of batch_pairs pairs (0: 262144) with the aligner's configuration,
+ * and one .alg line per pair is written (benchmark_utils.c:103-179):
+ * "score\tCIGAR", or with full_output "plen\ttlen\tscore\tpattern\ttext\tCIGAR".
+ * Parsing, device work and formatting of consecutive batches overlap.
+ * Returns 0 or the first failing per-pair status. */
+typedef struct {
+  int64_t pairs;
+  int64_t bytes_in;
+  int64_t bytes_out;
+  double seconds;         /* whole call */
+  double read_seconds;    /* reader thread: parsing */
+  double align_seconds;   /* calling thread: wavefront_align_batch */
+  double write_seconds;   /* writer thread: formatting and writing */
+} wfa_io_stats_t;
+int wfa_align_seq_file(wavefront_aligner_t* const wf_aligner, const char* input_path, const char* output_path,
+    int batch_pairs, int full_output, wfa_io_stats_t* stats);
+
 /* ------------------------------------------------------------ CIGAR utils
  * alignment/cigar.c:244-470 */
 bool cigar_is_null(const cigar_t* const cigar);
@@ -442,6 +464,11 @@ bool cigar_check_alignment(FILE* const stream,
     const char* const text, const int text_length,
     const cigar_t* const cigar, const bool verbose);
 int cigar_sprint(char* const buffer, const cigar_t* const cigar, const bool print_matches);
+/* SAM CIGAR runs (length << 4 | BAM code) and their text (alignment/cigar.c:180-239, :470-502) */
+void cigar_get_CIGAR(cigar_t* const cigar, const bool show_mismatches, uint32_t** const cigar_buffer,
+    int* const cigar_length);
+int cigar_sprint_SAM_CIGAR(char* const buffer, cigar_t* const cigar, const bool show_mismatches);
+void cigar_print_SAM_CIGAR(FILE* const stream, cigar_t* const cigar, const bool show_mismatches);
 void cigar_print(FILE* const stream, const cigar_t* const cigar, const bool print_matches);
 void cigar_print_pretty(FILE* const stream, const cigar_t* const cigar,
     const char* const pattern, const int pattern_length,

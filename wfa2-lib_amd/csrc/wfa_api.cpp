@@ -941,6 +941,7 @@ extern "C" wavefront_aligner_t* wavefront_aligner_new(wavefront_aligner_attr_t* 
   wf->cigar = static_cast<cigar_t*>(calloc(1, sizeof(cigar_t)));
   wf->cigar->max_operations = 2 * (1000 + 1000);
   wf->cigar->operations = static_cast<char*>(malloc(wf->cigar->max_operations));
+  wf->cigar->cigar_buffer = static_cast<uint32_t*>(malloc(sizeof(uint32_t) * wf->cigar->max_operations));
   wf->cigar->score = INT32_MIN;
   wf->cigar->end_v = wf->cigar->end_h = -1;
   wf->device = new (std::nothrow) Engine();
@@ -1272,6 +1273,7 @@ int align_one(wavefront_aligner_t* wf) {
   cigar->begin_offset = cigar->end_offset = 0;
   cigar->score = INT32_MIN;
   cigar->end_v = cigar->end_h = -1;
+  cigar->cigar_length = 0;  // drop the cached SAM CIGAR
   wf->align_status.dropped = false;
   presets_check(wf, &plen, &tlen, 1);
   const int sup = check_supported(wf);
@@ -1285,6 +1287,8 @@ int align_one(wavefront_aligner_t* wf) {
     free(cigar->operations);
     cigar->max_operations = 2 * (plen + tlen) + 1;
     cigar->operations = static_cast<char*>(malloc(cigar->max_operations));
+    free(cigar->cigar_buffer);
+    cigar->cigar_buffer = static_cast<uint32_t*>(malloc(sizeof(uint32_t) * cigar->max_operations));
   }
   wfa_result_t r;
   const int rc = wavefront_align_batch(wf, &pr, 1, &r, cig ? cigar->operations : nullptr,

@@ -313,6 +313,75 @@ int cigar_sprint(char* const buffer, const cigar_t* const cigar, const bool prin
   return cursor;
 }
 
+/* SAM / BAM operation codes (alignment/cigar.c:38-56). */
+static uint32_t sam_op_code(char op) {
+  switch (op) {
+    case 'M': return 0;
+    case 'I': return 1;
+    case 'D': return 2;
+    case 'N': return 3;
+    case '=': return 7;
+    case 'X': return 8;
+    default: return 15;
+  }
+}
+
+/* Run-length SAM CIGAR, (length << 4 | code) per run (alignment/cigar.c:
+   180-239): mismatches fold into 'M' unless show_mismatches, which writes
+   matches as '='.  As the reference, the first operation is taken as it is
+   (a leading 'X' stays 'X'), and the result is cached in the cigar until
+   the next alignment or a different show_mismatches. */
+void cigar_get_CIGAR(cigar_t* const cigar, const bool show_mismatches, uint32_t** const cigar_buffer,
+    int* const cigar_length) {
+  if (cigar->cigar_length == 0 || cigar->has_misms != show_mismatches) {
+    const int n = cigar->end_offset - cigar->begin_offset;
+    cigar->cigar_length = 0;
+    if (n > 0) {
+      uint32_t* const out = cigar->cigar_buffer;
+      const char* const ops = cigar->operations + cigar->begin_offset;
+      int len = 0;
+      char last = ops[0];
+      uint32_t run = 1;
+      for (int i = 1; i <= n; ++i) {
+        char op = 0;
+        if (i < n) {
+          op = ops[i];
+          if (!show_mismatches && op == 'X') op = 'M';
+          if (op == last) { ++run; continue; }
+        }
+        out[len++] = (run << 4) | ((show_mismatches && last == 'M') ? 7u : sam_op_code(last));
+        last = op;
+        run = 1;
+      }
+      cigar->has_misms = show_mismatches;
+      cigar->cigar_length = len;
+    }
+  }
+  *cigar_buffer = cigar->cigar_buffer;
+  *cigar_length = cigar->cigar_length;
+}
+
+int cigar_sprint_SAM_CIGAR(char* const buffer, cigar_t* const cigar, const bool show_mismatches) {
+  uint32_t* runs;
+  int n;
+  cigar_get_CIGAR(cigar, show_mismatches, &runs, &n);
+  int cursor = 0;
+  for (int i = 0; i < n; ++i) {
+    const uint32_t code = runs[i] & 0xf;
+    cursor += sprintf(buffer + cursor, "%u%c", runs[i] >> 4, code <= 8 ? "MIDN---=X"[code] : '?');
+  }
+  buffer[cursor] = '\0';
+  return cursor;
+}
+
+void cigar_print_SAM_CIGAR(FILE* const stream, cigar_t* const cigar, const bool show_mismatches) {
+  if (cigar_is_null(cigar)) return;
+  char* const buffer = malloc(4 * (size_t)(cigar->end_offset - cigar->begin_offset) + 20);
+  cigar_sprint_SAM_CIGAR(buffer, cigar, show_mismatches);
+  fprintf(stream, "%s", buffer);
+  free(buffer);
+}
+
 void cigar_print(FILE* const stream, const cigar_t* const cigar, const bool print_matches) {
   if (cigar_is_null(cigar)) return;
   char* const buffer = malloc(2 * (cigar->end_offset - cigar->begin_offset) + 20);

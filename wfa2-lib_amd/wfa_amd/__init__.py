@@ -165,6 +165,7 @@ def lib():
         L.wavefront_aligner_new.argtypes = [POINTER(wavefront_aligner_attr_t)]
         L.wavefront_aligner_delete.argtypes = [POINTER(wavefront_aligner_t)]
         L.wavefront_align.argtypes = [POINTER(wavefront_aligner_t), c_char_p, c_int, c_char_p, c_int]
+        L.wfa_align_seq_file.argtypes = [POINTER(wavefront_aligner_t), c_char_p, c_char_p, c_int, c_int, c_void_p]
         L.wavefront_align_batch.argtypes = [POINTER(wavefront_aligner_t), c_void_p, c_int, c_void_p,
                                             c_void_p, ctypes.c_size_t]
         L.wfa_device_batch_upload.restype = c_void_p
@@ -362,6 +363,19 @@ class Aligner:
 
     def upload(self, patterns, texts=None) -> DeviceBatch:
         return DeviceBatch(self, patterns, texts)
+
+    def align_seq_file(self, input_path, output_path, batch_pairs=0, full=False):
+        """wfa_align_seq_file: .seq in, .alg out; returns (status, wfa_io_stats_t)."""
+        st = wfa_io_stats_t()
+        rc = lib().wfa_align_seq_file(self.wf, os.fsencode(input_path), os.fsencode(output_path),
+                                      int(batch_pairs), int(bool(full)), ctypes.byref(st))
+        return rc, st
+
+
+class wfa_io_stats_t(Structure):
+    _fields_ = [("pairs", c_int64), ("bytes_in", c_int64), ("bytes_out", c_int64), ("seconds", ctypes.c_double),
+                ("read_seconds", ctypes.c_double), ("align_seconds", ctypes.c_double),
+                ("write_seconds", ctypes.c_double)]
 
 
 class ServedResult:
