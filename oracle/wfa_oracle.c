@@ -42,6 +42,7 @@
 #define ST_OK                 -1
 #define ST_END_REACHED        -2
 #define ST_END_UNREACHABLE    -3
+#define ST_ALG_PARTIAL         1
 
 /* BiWFA constants, wavefront_bialign.c:52-54 */
 #define BI_FALLBACK_MIN_SCORE  250
@@ -85,15 +86,17 @@ typedef struct {
   int max_steps;
   int64_t dsteps;
   int status;         /* ST_OK while running, ST_END_REACHED at the end */
+  int null_steps;     /* consecutive compute steps with every input null
+                         (align_status.num_null_steps, wavefront_compute_affine.c:236-244) */
   int end_score;
   /* alignment form (wavefront_attributes.h:63-72); extension sets the ends
      free up to both sequence ends (wavefront_align.c:57-63) */
   bool endsfree;
   int pbf, pef, tbf, tef;
   int end_k, end_off; /* end cell (wf_aligner->alignment_end_pos) */
-  /* wf-adaptive heuristic (wavefront_heuristic.c:229-290) */
-  bool wfadaptive;
-  int h_min_len, h_max_thr, h_steps, steps_wait;
+  /* heuristics (wavefront_heuristic.c:229-290, 467-500) */
+  bool wfadaptive, banded_static, banded_adaptive;
+  int h_min_len, h_max_thr, h_steps, steps_wait, band_min_k, band_max_k;
   /* CIGAR, written backwards by the backtrace into ops[begin..end) */
   char* ops; int ops_cap; int ops_begin; int ops_end;
   int cigar_score;
@@ -189,6 +192,10 @@ static inline int32_t wf_rd(const owf_t* w, int k) {
 static void aln_set_penalties(oaln_t* a, const oracle_conf_t* conf) {
   a->metric = conf->metric;
   a->wfadaptive = (conf->heuristic & 4) != 0;
+  a->banded_static = (conf->heuristic & 1) != 0;
+  a->banded_adaptive = !a->banded_static && (conf->heuristic & 2) != 0;
+  a->band_min_k = conf->min_k;
+  a->band_max_k = conf->max_k;
   a->h_min_len = conf->min_wavefront_length;
   a->h_max_thr = conf->max_distance_threshold;
   a->h_steps = conf->steps_between_cutoffs;
@@ -333,7 +340,8 @@ static void compute_linear(oaln_t* a, int s) {
   }
   owf_t* mx = wf_src(a, C_M, s - a->x);
   owf_t* mo = wf_src(a, C_M, s - a->o1);
-  if (!mx && !mo) return; /* null step */
+  if (!mx && !mo) { ++a->null_steps; return; } /* null step (compute_linear.c:170-177) */
+  a->null_steps = 0;
   lo = imin(WLO(mx), WLO(mo) - 1);
   hi = imax(WHI(mx), WHI(mo) + 1);
   wf_alloc(out, lo, hi);
@@ -366,7 +374,8 @@ static void compute(oaln_t* a, int s) {
   owf_t* out[NCOMP];
   int c;
   for (c = 0; c < NCOMP; ++c) out[c] = wf_claim(a, c, s);
-  if (!mx && !mo1 && !i1 && !d1 && !mo2 && !i2 && !d2) return; /* null step */
+  if (!mx && !mo1 && !i1 && !d1 && !mo2 && !i2 && !d2) { ++a->null_steps; return; } /* null step */
+  a->null_steps = 0;
   /* wavefront_compute_limits_input, wavefront_compute.c:40-86 */
   int lo = WLO(mx), hi = WHI(mx);
   lo = imin(lo, WLO(mo1) - 1); hi = imax(hi, WHI(mo1) + 1);
@@ -433,26 +442,25 @@ static bool termination(oaln_t* a, owf_t* m, int s) {
   return true;
 }
 
-/* wavefront_heuristic_cufoff with wf-adaptive (wavefront_heuristic.c:229-290,
-   506-561), after the extension of score s: every steps_between_cutoffs
-   non-empty M wavefronts of at least min_wavefront_length diagonals, keep the
-   diagonals whose distance to the end max(plen - v, tlen - h) is within
-   max_distance_threshold of the least one, scanning in from both ends but
-   never past the end diagonal; then narrow I1/D1/I2/D2 to M's range. */
-static void cutoff(oaln_t* a, int s) {
-  owf_t* m = wf_at(a, C_M, s);
-  if (!m || m->lo > m->hi) return;
-  --a->steps_wait;
+/* wf_distance_end2end (wavefront_heuristic.c:125-133): the larger of what is
+   left of the pattern and of the text, or -NULL for a NULL cell. */
+static int distance_end2end(const oaln_t* a, int k, int32_t off) {
+  return (off >= 0) ? imax(a->plen - (off - k), a->tlen - off) : -ONULL;
+}
+
+/* wf-adaptive (wavefront_heuristic_wfadaptive and its reduce step,
+   wavefront_heuristic.c:229-290): keep the diagonals whose distance to the
+   end is within max_distance_threshold of the least one, scanning in from
+   both ends but never past the end diagonal. */
+static void wfadaptive_cut(oaln_t* a, owf_t* m) {
   if (a->steps_wait > 0) return;
   const int lo = m->lo, hi = m->hi;
   if (hi - lo + 1 < a->h_min_len) return;
   int32_t* d = malloc(sizeof(int32_t) * (hi - lo + 1));
   int min_d = imax(a->plen, a->tlen), k;
   for (k = lo; k <= hi; ++k) {
-    const int32_t off = wf_raw(m, k);
-    const int dist = (off >= 0) ? imax(a->plen - (off - k), a->tlen - off) : -ONULL;
-    d[k - lo] = dist;
-    min_d = imin(min_d, dist);
+    d[k - lo] = distance_end2end(a, k, wf_raw(m, k));
+    min_d = imin(min_d, d[k - lo]);
   }
   const int k_end = a->tlen - a->plen;
   const int top = imin(k_end, hi);
@@ -471,9 +479,53 @@ static void cutoff(oaln_t* a, int s) {
   m->hi = hi_r;
   free(d);
   a->steps_wait = a->h_steps;
+}
+
+/* wavefront_heuristic_banded_adaptive (wavefront_heuristic.c:477-513): a
+   wavefront wider than the band keeps a band-wide window placed by the
+   distances to the end sampled at its ends and quarters. */
+static void banded_adaptive(oaln_t* a, owf_t* m) {
+  if (a->steps_wait > 0) return;
+  const int lo = m->lo, hi = m->hi;
+  const int len = hi - lo + 1;
+  if (len < 4) return;
+  const int max_len = a->band_max_k - a->band_min_k + 1;
+  if (len > max_len) {
+    const int leeway = (len - max_len) / 2, quarter = len / 4;
+    const int d0 = distance_end2end(a, lo, wf_raw(m, lo));
+    const int d1 = distance_end2end(a, lo + quarter, wf_raw(m, lo + quarter));
+    const int d2 = distance_end2end(a, lo + 2 * quarter, wf_raw(m, lo + 2 * quarter));
+    const int d3 = distance_end2end(a, hi, wf_raw(m, hi));
+    int new_lo = lo;
+    if (d0 > d3) new_lo += leeway;
+    if (d1 > d2) new_lo += leeway;
+    m->lo = imax(new_lo, lo);
+    m->hi = imin(new_lo + max_len - 1, hi);
+  }
+  a->steps_wait = a->h_steps;
+}
+
+/* wavefront_heuristic_cufoff (wavefront_heuristic.c:506-561) after the
+   extension of score s: one countdown for all strategies, wf-adaptive then the
+   band (static: clamp to [min_k, max_k] every time), and when M changed,
+   I1/D1/I2/D2 narrowed to M's range.  The cut narrows [lo,hi] only: the cut
+   cells stay readable (rlo/rhi) except for indel / edit, whose compute step
+   NULLs the two cells beyond each end (wavefront_compute_edit.c:355-358). */
+static void cutoff(oaln_t* a, int s) {
+  owf_t* m = wf_at(a, C_M, s);
+  if (!m || m->lo > m->hi) return;
+  --a->steps_wait;
+  const int lo = m->lo, hi = m->hi;
+  if (a->wfadaptive) wfadaptive_cut(a, m);
+  if (a->banded_static) {
+    if (m->lo < a->band_min_k) m->lo = a->band_min_k;
+    if (m->hi > a->band_max_k) m->hi = a->band_max_k;
+  } else if (a->banded_adaptive) {
+    banded_adaptive(a, m);
+  }
   if (m->lo == lo && m->hi == hi) return;
   if (m->lo > m->hi) m->null = true;
-  if (a->metric <= M_EDIT) { m->rlo = m->lo; m->rhi = m->hi; }  /* NULLs beyond the ends, as the prune */
+  if (a->metric <= M_EDIT) { m->rlo = m->lo; m->rhi = m->hi; }
   if (a->metric <= M_LINEAR) return;
   int c;
   for (c = C_I1; c < NCOMP; ++c) {  /* wf_heuristic_equate (:163-171) */
@@ -497,13 +549,24 @@ static bool termination_endsfree(oaln_t* a, int s, int k, int32_t off) {
   return true;
 }
 
+/* A null step leaves no M wavefront; more of them in a row than the score
+   scope spans means no later wavefront can have an input: the end is
+   unreachable, which only a heuristic cut-off can bring about
+   (wavefront_extend.c:94-101, 184-191, 268-275). */
+static int unreachable(oaln_t* a, int s) {
+  if (a->null_steps <= a->scope) return 0;
+  a->status = ST_END_UNREACHABLE;
+  a->end_score = s;
+  return 1;
+}
+
 /* wavefront_extend_endsfree, wavefront_extend.c:212-330, with the scalar
    kernel wavefront_extend_matches_packed_endsfree (wavefront_extend_kernels.c:
    154-196): diagonals in ascending order, each extended and then tested; the
    first that ends the alignment stops the extension. */
 static int extend_endsfree(oaln_t* a, int s) {
   owf_t* m = wf_at(a, C_M, s);
-  if (!m) return 0;
+  if (!m) return unreachable(a, s);
   int k;
   for (k = m->lo; k <= m->hi; ++k) {
     const int32_t off = wf_raw(m, k);
@@ -512,7 +575,7 @@ static int extend_endsfree(oaln_t* a, int s) {
     m->v[k - m->alo] = ext;
     if (termination_endsfree(a, s, k, ext)) return 1;
   }
-  if (a->wfadaptive) cutoff(a, s);
+  if (a->wfadaptive || a->banded_static || a->banded_adaptive) cutoff(a, s);
   return 0;
 }
 
@@ -522,7 +585,7 @@ static int extend(oaln_t* a, int s, int* max_ak) {
   if (a->endsfree) return extend_endsfree(a, s);
   if (max_ak) *max_ak = 0;
   owf_t* m = wf_at(a, C_M, s);
-  if (!m) return 0;
+  if (!m) return unreachable(a, s);
   int32_t best = 0;
   int k;
   for (k = m->lo; k <= m->hi; ++k) {
@@ -538,7 +601,7 @@ static int extend(oaln_t* a, int s, int* max_ak) {
     a->end_off = a->tlen;
     return 1;
   }
-  if (a->wfadaptive) cutoff(a, s);
+  if (a->wfadaptive || a->banded_static || a->banded_adaptive) cutoff(a, s);
   if (max_ak) *max_ak = best;
   return 0;
 }
@@ -552,6 +615,7 @@ static void aln_init(oaln_t* a, int comp_begin, int comp_end) {
   a->comp_end = comp_end;
   a->status = ST_OK;
   a->end_score = -1;
+  a->null_steps = 0;  /* wavefront_aligner.c:411 */
   a->steps_wait = a->h_steps;  /* wavefront_heuristic_clear (wavefront_heuristic.c:114-121) */
   owf_t* w[NCOMP];
   int c;
@@ -581,6 +645,15 @@ static int64_t probe(oaln_t* a, int c, int s, int k, int add, int type) {
   return (int64_t)(wf_raw(w, k) + add) * 16 + type;
 }
 
+/* wavefront_backtrace_matches (wavefront_backtrace.c:77-98): n matches go
+   before the CIGAR written so far.  The write position moves by n whatever
+   its sign: a source offset past the current one, which a heuristic cut-off
+   can leave readable, takes back the last -n operations. */
+static void bt_matches(oaln_t* a, int* pos, int n) {
+  if (n < 0) { *pos = imin(*pos - n, a->ops_cap); return; }
+  while (n-- > 0) a->ops[--*pos] = 'M';
+}
+
 /* wavefront_backtrace_linear, wavefront_backtrace.c:223-319 (indel, edit,
    gap-linear): M only, ties M > D > I. */
 static void backtrace_linear(oaln_t* a, int s_end, int k_end, int off_end) {
@@ -598,8 +671,7 @@ static void backtrace_linear(oaln_t* a, int s_end, int k_end, int off_end) {
     const int64_t best = lmax(misms, lmax(ins, del));
     if (best < 0) break;  /* no source */
     const int max_off = (int)(best >> 4);
-    int n = off - max_off;
-    while (n-- > 0) a->ops[--pos] = 'M';
+    bt_matches(a, &pos, off - max_off);
     off = max_off;
     v = off - k; h = off;
     if (v <= 0 || h <= 0) break;
@@ -659,8 +731,7 @@ static void backtrace(oaln_t* a, int s_end, int k_end, int off_end) {
     if (best < 0) break;  /* no source */
     if (comp == C_M) {
       const int max_off = (int)(best >> 4);
-      int n = off - max_off;
-      while (n-- > 0) a->ops[--pos] = 'M';
+      bt_matches(a, &pos, off - max_off);
       off = max_off;
       v = off - k; h = off;
       if (v <= 0 || h <= 0) break;
@@ -705,6 +776,24 @@ static int unialign(oaln_t* a, bool score_only) {
   a->cigar_score = INT32_MIN;
   while (1) {
     if (extend(a, s, NULL)) {
+      if (a->status == ST_END_UNREACHABLE) {
+        /* wavefront_unialign_terminate (wavefront_unialign.c:149-236): no
+           end cell was recorded (alignment_end_pos keeps k = INT_MAX and
+           offset = NULL, wavefront_aligner.c:411-414), so no backtrace; a
+           score-only run takes the classic score at that NULL end, with the
+           wrap-around of 32-bit int arithmetic */
+        if (score_only) {
+          const int32_t off = ONULL;
+          const int32_t v = (int32_t)((uint32_t)off - (uint32_t)INT_MAX), h = off;
+          if (a->metric <= M_EDIT || a->match == 0) {
+            a->cigar_score = classic_score(a, 0, 0, s);
+          } else {
+            const uint32_t sum = (uint32_t)(-a->match) * ((uint32_t)v + (uint32_t)h) - (uint32_t)s;
+            a->cigar_score = (int32_t)sum / 2;
+          }
+        }
+        return ST_ALG_PARTIAL;
+      }
       if (score_only) {
         /* compute_score: the score is taken at the full lengths (:159-166) */
         a->cigar_score = classic_score(a, a->plen, a->tlen, s);
@@ -720,6 +809,16 @@ static int unialign(oaln_t* a, bool score_only) {
     if (s >= a->max_steps) {
       a->cigar_score = -a->max_steps;
       return ST_MAX_STEPS;
+    }
+    if (a->metric <= M_EDIT) {
+      /* A null indel / edit wavefront stays null (the next one spans its
+         NULLed ends, wavefront_compute_edit.c:344-352) and the reference
+         computes on to the step limit; its result is taken here at once. */
+      owf_t* m = wf_at(a, C_M, s);
+      if (m && m->null) {
+        a->cigar_score = -a->max_steps;
+        return ST_MAX_STEPS;
+      }
     }
   }
 }
@@ -968,7 +1067,8 @@ static int bialign(const oracle_conf_t* conf, const char* P, int plen, const cha
   aln_set_penalties(&b.fwd, conf); b.fwd.modular = true;
   aln_set_penalties(&b.rev, conf); b.rev.modular = true;
   aln_set_penalties(&b.base, conf); b.base.modular = false;
-  b.base.wfadaptive = false;  /* the base aligner runs without heuristic (wavefront_bialigner.c:64-67) */
+  /* the base aligner runs without heuristic (wavefront_bialigner.c:64-67) */
+  b.base.wfadaptive = b.base.banded_static = b.base.banded_adaptive = false;
   b.P = P; b.T = T; b.plen = plen; b.tlen = tlen;
   b.Prev = reversed(P, plen); b.Trev = reversed(T, tlen);
   b.cigar_score = INT32_MIN;
@@ -1100,16 +1200,19 @@ int oracle_align(const oracle_conf_t* conf, const char* P, int plen, const char*
   aln_init(&a, C_M, C_M);
   int st = unialign(&a, conf->score_only != 0);
   *score = a.cigar_score;
-  if (st == ST_ALG_COMPLETED && !conf->score_only && ops && ops_len) {
+  const bool dropped = (a.status == ST_END_UNREACHABLE);
+  if ((st == ST_ALG_COMPLETED || dropped) && !conf->score_only && ops && ops_len) {
     int n = a.ops_end - a.ops_begin;
-    if (conf->extension) {
+    if (conf->extension || dropped) {
       /* wavefront_unialign_terminate, wavefront_unialign.c:188-236: the
-         backtrace leaves the raw score; max-trim replaces it */
-      int keep = n, best = a.end_score;
+         backtrace leaves the raw score; max-trim replaces it.  A dropped
+         alignment has no backtrace and keeps the cleared CIGAR's score
+         (INT32_MIN, cigar.c:79-90) unless max-trim replaces it. */
+      int keep = n, best = dropped ? INT32_MIN : a.end_score;
       const bool trimmed = maxtrim(conf, a.ops + a.ops_begin, n, &keep, &best);
       *score = best;
       n = keep;
-      if (trimmed) st = 1;  /* WF_STATUS_ALG_PARTIAL */
+      if (trimmed) st = ST_ALG_PARTIAL;
     }
     memcpy(ops, a.ops + a.ops_begin, n);
     *ops_len = n;

@@ -37,9 +37,10 @@ typedef struct {
   int ends_free;       /* alignment_form.span == alignment_endsfree */
   int pattern_begin_free, pattern_end_free, text_begin_free, text_end_free;
   int extension;       /* alignment_form.extension (wavefront_align.c:57-63) */
-  int heuristic;       /* wavefront_heuristic_t.strategy: 0 none, 4 wf-adaptive
-                          (wavefront_heuristic.h:41-49) */
+  int heuristic;       /* wavefront_heuristic_t.strategy bits (wavefront_heuristic.h:41-49):
+                          1 banded static, 2 banded adaptive, 4 wf-adaptive */
   int min_wavefront_length, max_distance_threshold, steps_between_cutoffs;
+  int min_k, max_k;    /* band of the banded strategies */
 } oracle_conf_t;
 
 /* Status returned for a configuration this restatement does not cover:

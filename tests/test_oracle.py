@@ -389,6 +389,87 @@ def test_oracle_wfadaptive_endsfree_matches_reference_build():
         assert all(x.ops(i) == y.ops(i) for i in range(pairs.n))
 
 
+BANDED = [
+    (1, 0, 0, 0, -10, 10),      # banded static
+    (1, 0, 0, 0, -3, 40),       # banded static, lopsided band
+    (1, 0, 0, 0, 5, 10),        # banded static without diagonal 0: nothing is reachable
+    (2, 0, 0, 1, -10, 10),      # banded adaptive
+    (2, 0, 0, 3, -5, 5),        # banded adaptive, every third step
+    (5, 10, 50, 1, -20, 20),    # wf-adaptive then banded static
+    (6, 10, 50, 2, -8, 8),      # wf-adaptive then banded adaptive
+]
+
+
+def _banded_pairs():
+    from wfa_amd import datagen
+    a, b = datagen.generate(1, 41, 300), datagen.generate(2, 42, 4)
+    ps = datagen.semiglobal(1, 43, 100)
+    keep = [i for i in range(b.n) if b.plen[i] <= 10000]
+    return (a.patterns() + [b.pattern(i) for i in keep] + ps.patterns(),
+            a.texts() + [b.text(i) for i in keep] + ps.texts())
+
+
+@pytest.mark.skipif(not O.ref_available("v2"), reason="scalar reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("metric,pen", [("affine", (0, 4, 6, 2)), ("affine", (-3, 5, 3, 2)),
+                                        ("affine2p", (0, 4, 6, 2, 24, 1)), ("linear", (0, 4, 2)),
+                                        ("edit", (0, 1, 1)), ("indel", (0, 1, 1))])
+def test_oracle_banded_matches_reference_build(metric, pen):
+    """Banded static and banded adaptive (wavefront_heuristic.c:451-500),
+    alone and after wf-adaptive, in memory modes high and ultralow.  Bands
+    that keep the end out of reach end in WF_STATUS_ALG_PARTIAL with an
+    empty CIGAR for the gap metrics (END_UNREACHABLE after more null steps
+    than the score scope, wavefront_extend.c:94-101) and WF_STATUS_UNATTAINABLE
+    in BiWFA.  For indel and edit the reference's high-memory form crashes
+    when a band empties the wavefront, so only pairs whose end diagonal lies
+    in the band are compared there."""
+    pats, texts = _banded_pairs()
+    for h in BANDED:
+        sel = list(range(len(pats)))
+        if metric in ("edit", "indel"):
+            # the oracle stops at once where the band empties the wavefront
+            # (a step-limit result); those pairs crash the reference
+            full = _pairs(pats, texts)
+            r = O.oracle_align(O.Conf(metric, pen, "high", heuristic=h, max_steps=30000), full)
+            sel = [i for i in sel if r.status[i] != O.WF_STATUS_MAX_STEPS_REACHED]
+        if not sel:
+            continue
+        pairs = _pairs([pats[i] for i in sel], [texts[i] for i in sel])
+        for mem in ("high", "ultralow"):
+            for so in (False, True):
+                conf = O.Conf(metric, pen, mem, so, heuristic=h, max_steps=30000)
+                x = O.oracle_align(conf, pairs)
+                y = O.ref_align(conf, pairs, isa="v2")
+                assert list(x.status) == list(y.status), (h, mem, so)
+                assert list(x.score) == list(y.score), (h, mem, so)
+                if not so:
+                    for i in range(pairs.n):
+                        assert x.ops(i) == y.ops(i), (h, mem, i)
+
+
+def test_oracle_banded_static_is_banded_dynamic_programming():
+    """A static band [min_k, max_k] that holds diagonals 0 and k_end limits
+    the alignment to cells with min_k - 1 <= h - v <= max_k + 1 at most; the
+    score equals the best over alignments inside [min_k, max_k] or improves
+    on it only through the one-diagonal overhang, and never beats the DP
+    optimum; a band wider than both sequences gives the exact result."""
+    pats, texts = _random_pairs(91, 40, 30, 70, 10, max_indel=3)
+    pairs = _pairs(pats, texts)
+    exact = O.oracle_align(O.Conf("affine", (0, 4, 6, 2)), pairs)
+    wide = O.oracle_align(O.Conf("affine", (0, 4, 6, 2), heuristic=(1, 0, 0, 0, -200, 200)), pairs)
+    tight = O.oracle_align(O.Conf("affine", (0, 4, 6, 2), heuristic=(1, 0, 0, 0, -2, 2)), pairs)
+    for i, (p, t) in enumerate(zip(pats, texts)):
+        assert (wide.status[i], wide.score[i], wide.ops(i)) == (exact.status[i], exact.score[i], exact.ops(i))
+        if not -2 <= len(t) - len(p) <= 2:
+            continue
+        assert tight.status[i] == 0 and dp.cigar_valid(p, t, tight.ops(i))
+        assert dp.cigar_score(tight.ops(i), 0, 4, 6, 2) == tight.score[i] <= dp.gotoh(p, t, 0, 4, 6, 2)
+        k, v, h = 0, 0, 0
+        for op in tight.ops(i):
+            v += op in b"MXD"
+            h += op in b"MXI"
+            assert -3 <= h - v <= 3
+
+
 def test_oracle_wfadaptive_is_a_valid_alignment_no_better_than_optimal():
     """What a heuristic may change is which alignment is found, never its
     validity: the CIGAR transforms pattern into text, re-scores to the
