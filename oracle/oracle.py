@@ -144,7 +144,7 @@ def oracle_lib():
 
 class Result:
     def __init__(self, n, status, score, ops_raw, ops_off, ops_len, dsteps=None, end_score=None,
-                 alg_score=None, seconds=None):
+                 alg_score=None, seconds=None, stale=None):
         self.n = n
         self.status = status
         self.score = score
@@ -155,6 +155,7 @@ class Result:
         self.end_score = end_score
         self.alg_score = alg_score
         self.seconds = seconds
+        self.stale = stale  # oracle only: the backtrace met cells a cut-off left readable
 
     def ops(self, i) -> bytes:
         o = int(self.ops_off[i])
@@ -169,13 +170,14 @@ def oracle_align(conf: Conf, pairs: Pairs) -> Result:
     ops_len = np.zeros(n, np.int32)
     dsteps = np.zeros(n, np.int64)
     end_score = np.zeros(n, np.int32)
+    stale = np.zeros(n, np.int32)
     ops = ctypes.create_string_buffer(pairs.ops_cap)
     cc = _cconf(conf)
     lib.oracle_align_batch(ctypes.byref(cc), ctypes.c_int(n), pairs.buf,
                            _ptr(pairs.p_off), _ptr(pairs.plen), _ptr(pairs.t_off), _ptr(pairs.tlen),
                            _ptr(status), _ptr(score), ops, _ptr(pairs.ops_off), _ptr(ops_len),
-                           _ptr(dsteps), _ptr(end_score))
-    r = Result(n, status, score, ops.raw, pairs.ops_off, ops_len, dsteps, end_score)
+                           _ptr(dsteps), _ptr(end_score), _ptr(stale))
+    r = Result(n, status, score, ops.raw, pairs.ops_off, ops_len, dsteps, end_score, stale=stale)
     r.alg_score = alg_scores(conf, r)
     return r
 

@@ -97,6 +97,8 @@ typedef struct {
   /* heuristics (wavefront_heuristic.c:229-290, 467-500) */
   bool wfadaptive, banded_static, banded_adaptive;
   int h_min_len, h_max_thr, h_steps, steps_wait, band_min_k, band_max_k;
+  bool stale_hit;     /* the backtrace met a cell whose choices change when
+                         cut-away wavefronts are read (see cell_stale) */
   /* CIGAR, written backwards by the backtrace into ops[begin..end) */
   char* ops; int ops_cap; int ops_begin; int ops_end;
   int cigar_score;
@@ -616,6 +618,7 @@ static void aln_init(oaln_t* a, int comp_begin, int comp_end) {
   a->status = ST_OK;
   a->end_score = -1;
   a->null_steps = 0;  /* wavefront_aligner.c:411 */
+  a->stale_hit = false;
   a->steps_wait = a->h_steps;  /* wavefront_heuristic_clear (wavefront_heuristic.c:114-121) */
   owf_t* w[NCOMP];
   int c;
@@ -635,6 +638,67 @@ static void aln_init(oaln_t* a, int comp_begin, int comp_end) {
 
 static void aln_ops_reserve(oaln_t* a, int n) {
   if (a->ops_cap < n) { free(a->ops); a->ops_cap = n + 16; a->ops = malloc(a->ops_cap); }
+}
+
+/* A source of cell (s, k) as the compute step reads it (a null wavefront is
+   no input) or, with probe_view, as the backtrace probes it (a wavefront a
+   cut-off emptied keeps its readable cells); below 0 reads as NULL. */
+static int32_t rd_view(oaln_t* a, int c, int s, int k, bool probe_view) {
+  if (s < 0) return ONULL;
+  owf_t* w = wf_at(a, c, s);
+  if (!w || (!probe_view && w->null) || k < w->rlo || k > w->rhi) return ONULL;
+  const int32_t v = wf_raw(w, k);
+  return v < 0 ? ONULL : v;
+}
+
+/* The choices at cell (s, k): best M offset, winning M source and the gap
+   extension flags, ties broken M > D2 > D1 > I2 > I1 and extension before
+   opening (as the backtrace's maximum of offset * 16 + type does). */
+static int64_t cell_choices(oaln_t* a, int s, int k, bool pv) {
+  const bool two = (a->metric == 4);
+  int32_t best = ((a->metric == M_INDEL) ? ONULL : rd_view(a, C_M, s - a->x, k, pv)) + 1;
+  int src = 0, bits = 0;
+  if (a->metric <= M_LINEAR) {
+    const int32_t del = rd_view(a, C_M, s - a->o1, k + 1, pv), ins = rd_view(a, C_M, s - a->o1, k - 1, pv) + 1;
+    if (del > best) { best = del; src = 2; }
+    if (ins > best) { best = ins; src = 4; }
+  } else {
+    const int so1 = s - a->o1 - a->e1, se1 = s - a->e1;
+    const int32_t oml = rd_view(a, C_M, so1, k - 1, pv), omr = rd_view(a, C_M, so1, k + 1, pv);
+    const int32_t eil = rd_view(a, C_I1, se1, k - 1, pv), edr = rd_view(a, C_D1, se1, k + 1, pv);
+    bits = (eil >= oml ? 8 : 0) | (edr >= omr ? 16 : 0);
+    int32_t ins2 = ONULL;
+    if (two) {
+      const int so2 = s - a->o2 - a->e2, se2 = s - a->e2;
+      const int32_t o2l = rd_view(a, C_M, so2, k - 1, pv), o2r = rd_view(a, C_M, so2, k + 1, pv);
+      const int32_t e2l = rd_view(a, C_I2, se2, k - 1, pv), e2r = rd_view(a, C_D2, se2, k + 1, pv);
+      bits |= (e2l >= o2l ? 32 : 0) | (e2r >= o2r ? 64 : 0);
+      ins2 = vmax(o2l, e2l) + 1;
+      const int32_t del2 = vmax(o2r, e2r);
+      if (del2 > best) { best = del2; src = 1; }
+    }
+    const int32_t del1 = vmax(omr, edr), ins1 = vmax(oml, eil) + 1;
+    if (del1 > best) { best = del1; src = 2; }
+    if (two && ins2 > best) { best = ins2; src = 3; }
+    if (ins1 > best) { best = ins1; src = 4; }
+  }
+  return ((int64_t)best << 8) | src | bits;
+}
+
+/* A backtrace through cell (s, k) probes, where a cut-off emptied a source
+   wavefront, cells the compute step did not read (wavefront_backtrace.c:
+   64-219 probes any stored wavefront over its initialised range).  When the
+   best probed offset passes the cell's extended offset, the match count
+   before the cell is negative and operations are taken back (bt_matches).
+   A NULL cell is compared by its computed best instead.  Recorded for the
+   tests of a device path that declines such traces. */
+static void cell_stale(oaln_t* a, int s, int k) {
+  if (!(a->wfadaptive || a->banded_static || a->banded_adaptive)) return;
+  const int64_t q = cell_choices(a, s, k, true) >> 8;
+  const int64_t p = cell_choices(a, s, k, false) >> 8;
+  owf_t* w = wf_at(a, C_M, s);
+  const int64_t post = (w && k >= w->alo && k <= w->ahi) ? wf_raw(w, k) : ONULL;
+  if (q >= 0 && q > (post > p ? post : p)) a->stale_hit = true;
 }
 
 static int64_t probe(oaln_t* a, int c, int s, int k, int add, int type) {
@@ -664,6 +728,7 @@ static void backtrace_linear(oaln_t* a, int s_end, int k_end, int off_end) {
   for (i = a->plen - v; i > 0; --i) a->ops[--pos] = 'D';
   for (i = a->tlen - h; i > 0; --i) a->ops[--pos] = 'I';
   while (v > 0 && h > 0 && s > 0) {
+    cell_stale(a, s, k);
     const int sx = s - a->x, so = s - a->o1;
     const int64_t misms = (a->metric != M_INDEL) ? probe(a, C_M, sx, k, 1, BT_M) : ONULL;
     const int64_t ins = probe(a, C_M, so, k - 1, 1, BT_I1O);
@@ -709,6 +774,7 @@ static void backtrace(oaln_t* a, int s_end, int k_end, int off_end) {
     for (i = a->tlen - h; i > 0; --i) a->ops[--pos] = 'I';
   }
   while (v > 0 && h > 0 && s > 0) {
+    cell_stale(a, s, k);
     const int sx = s - a->x, so1 = s - a->o1 - a->e1, se1 = s - a->e1;
     const int so2 = s - a->o2 - a->e2, se2 = s - a->e2;
     int64_t best;
@@ -1169,8 +1235,9 @@ static bool maxtrim(const oracle_conf_t* c, const char* ops, int n, int* keep, i
 /* ------------------------------------------------------------------ API */
 
 int oracle_align(const oracle_conf_t* conf, const char* P, int plen, const char* T, int tlen,
-                 int* score, char* ops, int* ops_len, int64_t* dsteps, int* end_score) {
+                 int* score, char* ops, int* ops_len, int64_t* dsteps, int* end_score, int* stale) {
   if (end_score) *end_score = -1;
+  if (stale) *stale = 0;
   if (ops_len) *ops_len = 0;
   const bool endsfree = conf->ends_free || conf->extension;
   if (conf->memory_mode == 3) {
@@ -1218,6 +1285,7 @@ int oracle_align(const oracle_conf_t* conf, const char* P, int plen, const char*
     *ops_len = n;
   }
   if (dsteps) *dsteps = a.dsteps;
+  if (stale) *stale = a.stale_hit;
   if (end_score) *end_score = (st == ST_ALG_COMPLETED) ? a.end_score : -1;
   aln_free(&a);
   return st;
@@ -1226,14 +1294,15 @@ int oracle_align(const oracle_conf_t* conf, const char* P, int plen, const char*
 void oracle_align_batch(const oracle_conf_t* conf, int n, const char* buf,
                         const int64_t* p_off, const int* plen, const int64_t* t_off, const int* tlen,
                         int* status, int* score, char* ops_out, const int64_t* ops_off, int* ops_len,
-                        int64_t* dsteps, int* end_score) {
+                        int64_t* dsteps, int* end_score, int* stale) {
   int i;
   #pragma omp parallel for schedule(dynamic, 4)
   for (i = 0; i < n; ++i) {
     status[i] = oracle_align(conf, buf + p_off[i], plen[i], buf + t_off[i], tlen[i],
                              &score[i], ops_out ? ops_out + ops_off[i] : NULL,
                              ops_len ? &ops_len[i] : NULL,
-                             dsteps ? &dsteps[i] : NULL, end_score ? &end_score[i] : NULL);
+                             dsteps ? &dsteps[i] : NULL, end_score ? &end_score[i] : NULL,
+                             stale ? &stale[i] : NULL);
   }
 }
 

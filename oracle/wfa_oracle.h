@@ -56,19 +56,22 @@ typedef struct {
  * ops        <- CIGAR operations (M/X/I/D, one per column), capacity plen+tlen
  * ops_len    <- number of operations written
  * dsteps     <- diagonal-steps: sum of (hi-lo+1) over every non-null compute
+ * stale      <- 1 when the backtrace passed a cell whose choices change once
+ *               the cells of wavefronts a heuristic cut-off emptied are read
+ *               (the reference's backtrace reads them; the compute does not)
  */
 int oracle_align(
     const oracle_conf_t* conf,
     const char* pattern, int plen,
     const char* text, int tlen,
-    int* score, char* ops, int* ops_len, int64_t* dsteps, int* end_score);
+    int* score, char* ops, int* ops_len, int64_t* dsteps, int* end_score, int* stale);
 
 /* Batch form over a shared byte buffer (see oracle/oracle.py). */
 void oracle_align_batch(
     const oracle_conf_t* conf, int n, const char* buf,
     const int64_t* p_off, const int* plen, const int64_t* t_off, const int* tlen,
     int* status, int* score, char* ops_out, const int64_t* ops_off, int* ops_len,
-    int64_t* dsteps, int* end_score);
+    int64_t* dsteps, int* end_score, int* stale);
 
 /* cigar_score_edit / _gap_linear / _gap_affine / _gap_affine2p (alignment/cigar.c:244-342). */
 int oracle_cigar_score(const oracle_conf_t* conf, const char* ops, int len);

@@ -42,7 +42,10 @@ def test_default_attributes_match_reference():
     dict(metric="affine", memory="ultralow", ends_free=(0, 0, 0, 0)),  # ends-free span in BiWFA
     dict(metric="edit", memory="med", extension=True),                 # indel/edit extension outside high
     dict(metric="affine", heuristic=(0x10, 0, 0, 1)),                  # x-drop
+    dict(metric="affine", heuristic=(0x20, 0, 0, 1)),                  # z-drop
+    dict(metric="affine", heuristic=(0x40, 10, 50, 1)),                # wfmash
     dict(metric="affine", heuristic=(0x04 | 0x10, 10, 50, 1)),         # wf-adaptive + x-drop
+    dict(metric="affine", heuristic=(0x01 | 0x20, 0, 0, 1, -5, 5)),    # banded static + z-drop
 ])
 def test_unsupported_configurations_are_refused(kw):
     al = wfa_amd.Aligner(**kw)
@@ -120,3 +123,27 @@ def test_configuration_errors_exit_like_the_reference(kw, msg):
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, timeout=120)
     assert r.returncode == 1 and b"returned" not in r.stdout
     assert msg in r.stderr
+
+
+def test_heuristic_setters_accumulate_like_the_reference():
+    """wavefront_aligner_set_heuristic_* add their strategy to those already
+    set and overwrite their own parameters (wavefront_aligner.c:534-600,
+    wavefront_heuristic.c:40-113); set_none clears the strategy only."""
+    import ctypes
+    L = wfa_amd.lib()
+    attr = wfa_amd.default_attributes()
+    wf = L.wavefront_aligner_new(ctypes.byref(attr))
+    try:
+        h = wf.contents.heuristic
+        assert h.strategy == 4
+        L.wavefront_aligner_set_heuristic_banded_static(wf, -7, 9)
+        h = wf.contents.heuristic
+        assert (h.strategy, h.min_k, h.max_k, h.min_wavefront_length) == (5, -7, 9, 10)
+        L.wavefront_aligner_set_heuristic_banded_adaptive(wf, -3, 3, 7)
+        h = wf.contents.heuristic
+        assert (h.strategy, h.min_k, h.max_k, h.steps_between_cutoffs) == (7, -3, 3, 7)
+        L.wavefront_aligner_set_heuristic_none(wf)
+        h = wf.contents.heuristic
+        assert (h.strategy, h.min_k, h.max_k) == (0, -3, 3)
+    finally:
+        L.wavefront_aligner_delete(wf)

@@ -323,8 +323,12 @@ int check_supported(wavefront_aligner_t* wf) {
   // reference (piggyback backtrace); only memory high is defined.
   else if (f.extension && m <= edit && wf->memory_mode != wavefront_memory_high)
     why = "indel/edit extension outside memory mode high";
-  else if (wf->heuristic.strategy != wf_heuristic_none && wf->heuristic.strategy != wf_heuristic_wfadaptive)
-    why = "heuristics other than wf-adaptive (x-drop, z-drop, banded, wfmash)";
+  // The strategies take effect as wavefront_aligner_init_heuristic selects
+  // them (wavefront_aligner.c:189-228): wf-adaptive before wfmash, x-drop
+  // before z-drop, banded static before banded adaptive.
+  else if ((wf->heuristic.strategy & (wf_heuristic_xdrop | wf_heuristic_zdrop)) ||
+           ((wf->heuristic.strategy & wf_heuristic_wfmash) && !(wf->heuristic.strategy & wf_heuristic_wfadaptive)))
+    why = "the x-drop, z-drop and wfmash heuristics";
   else if (make_penalties(wf->penalties).nslots > 64)
     why = "penalties whose score scope needs more than 64 wavefront slots";
   if (!why) return WF_STATUS_OK;
@@ -370,11 +374,15 @@ void presets_check(wavefront_aligner_t* wf, const wfa_pair_t* pairs, int n) {
 
 wfa::HeurArgs heuristic_of(const wavefront_aligner_t* wf) {
   wfa::HeurArgs h{};
-  if (wf->heuristic.strategy != wf_heuristic_wfadaptive) return h;
-  h.wfadaptive = 1;
+  const int st = (int)wf->heuristic.strategy;
+  h.wfadaptive = (st & wf_heuristic_wfadaptive) ? 1 : 0;
+  h.banded = (st & wf_heuristic_banded_static) ? 1 : (st & wf_heuristic_banded_adaptive) ? 2 : 0;
+  if (!h.cut()) return h;
   h.min_len = wf->heuristic.min_wavefront_length;
   h.thr = wf->heuristic.max_distance_threshold;
   h.steps = wf->heuristic.steps_between_cutoffs;
+  h.band_lo = wf->heuristic.min_k;
+  h.band_hi = wf->heuristic.max_k;
   return h;
 }
 
@@ -582,7 +590,7 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
   const int n = (int)grp.ids.size();
   if (n == 0) return WF_STATUS_OK;
   hipStream_t st = e->stream;
-  const LaunchShape shape = shape_of(e, grp, js.bits, pen, js.heur.wfadaptive != 0);
+  const LaunchShape shape = shape_of(e, grp, js.bits, pen, js.heur.cut());
   if (!grp.ids_on_device) {
     if (!grp.d_ids.ensure((size_t)n * 4)) return WF_STATUS_OOM;
     if (!hip_ok(hipMemcpyAsync(grp.d_ids.p, grp.ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, st), "plan upload"))
@@ -633,6 +641,9 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
   fa.h_min_len = js.heur.min_len;
   fa.h_thr = js.heur.thr;
   fa.h_steps = js.heur.steps;
+  fa.banded = js.heur.banded;
+  fa.band_lo = js.heur.band_lo;
+  fa.band_hi = js.heur.band_hi;
   fa.pen = pen;
   if (!hip_ok(wfa::launch_reset_counters(c, 0, st), "reset")) return WF_STATUS_OOM;
   (void)hipEventRecord(grp.ev[0], st);
@@ -704,7 +715,7 @@ std::vector<PlanGroup> make_plan(const JobSet& js, const std::vector<int>& ids, 
   int lmax_small = 0;
   const LeanTuning& lt = lean_tuning();
   const bool lean_ok = lt.enabled && ncomp == 3 && js.bits == 2 && !js.cbeg && !js.cend && !js.ef.on &&
-                       !js.heur.wfadaptive && wfa::lean_supported(pen);
+                       !js.heur.cut() && wfa::lean_supported(pen);
   std::vector<int> lean_ids[2];
   int lean_lmax[2] = {0, 0};
   for (int i : ids) {
@@ -767,7 +778,7 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
           const double L = std::max(js.h_plen[i], js.h_tlen[i]);
           w += L * L / 16 + 8 * L;
         }
-        g.est_rec_bytes = w + (double)shape_of(e, g, js.bits, pen, js.heur.wfadaptive != 0).grid * wfa::kChunkUnits * 16;
+        g.est_rec_bytes = w + (double)shape_of(e, g, js.bits, pen, js.heur.cut()).grid * wfa::kChunkUnits * 16;
       }
       want += g.est_rec_bytes;
     }
@@ -780,7 +791,7 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
       fprintf(stderr, "[WFA] batch did not converge (record arena too small?)\n");
       return WF_STATUS_OOM;
     }
-    if (!reserve_for(e, *pending, js.bits, pen, js.heur.wfadaptive != 0, records)) return WF_STATUS_OOM;
+    if (!reserve_for(e, *pending, js.bits, pen, js.heur.cut(), records)) return WF_STATUS_OOM;
     if (!hip_ok(wfa::launch_reset_counters(counters_of(e), 1, st), "reset")) return WF_STATUS_OOM;
     for (PlanGroup& g : *pending) {
       const int rc = enqueue_group(wf, e, js, g, pen, records);
@@ -1107,7 +1118,7 @@ extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_ba
     if (rc == WF_STATUS_OOM) return rc;
   } else {
     const std::vector<int> key = {pen.metric, pen.x, pen.o1, pen.e1, pen.o2, pen.e2, b->bits, b->ef.on,
-                                  b->heur.wfadaptive};
+                                  b->heur.cut() ? 1 : 0};
     if (b->plan_key != key) {
       std::vector<PlanGroup> plan = make_plan(b->jobs(), b->sorted, pen);
       recycle_plan(b->plan, plan);
@@ -1132,6 +1143,17 @@ extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_ba
       fprintf(stderr, "[WFA] internal error: inconsistent backtrace for pair %d\n", i);
       b->status[i] = WF_STATUS_UNATTAINABLE;
     }
+    if (b->status[i] == wfa::kDevStaleTrace) {
+      // The reference's backtrace (wavefront_backtrace.c:64-219) would read
+      // cells of a wavefront the cut-off emptied, which the compute step
+      // ignored; that trace is not replayed.
+      if (wf->system.verbose >= 0)
+        fprintf(stderr, "[WFA] pair %d: backtrace through a wavefront emptied by the heuristic cut-off "
+                "is not supported\n", i);
+      b->status[i] = WF_STATUS_UNSUPPORTED;
+    }
+    // an end made unreachable by the cut-off is a partial result, not a failure
+    if (b->status[i] == wfa::kDevUnreachable) continue;
     if (b->status[i] != WF_STATUS_ALG_COMPLETED && first_fail == WF_STATUS_ALG_COMPLETED) first_fail = b->status[i];
   }
   b->first_fail = first_fail;
@@ -1212,6 +1234,29 @@ extern "C" int wfa_device_batch_download(wavefront_aligner_t* const wf, wfa_devi
       }
     } else if (st == WF_STATUS_MAX_STEPS_REACHED && !b->bi_mode) {
       r.score = -wf->system.max_alignment_steps;  // wavefront_unialign.c:102-107
+    } else if (st == wfa::kDevUnreachable) {
+      // WF_STATUS_END_UNREACHABLE (wavefront_unialign.c:147-236): no end cell
+      // was recorded (alignment_end_pos keeps k = INT_MAX and offset = NULL,
+      // wavefront_aligner.c:411-414), so there is no backtrace.  Score-only
+      // takes the classic score at that NULL end with the wrap-around of
+      // 32-bit arithmetic; with a CIGAR, max-trim of the empty CIGAR leaves
+      // the cleared score and end (cigar.c:79-90).
+      r.status = st = WF_STATUS_ALG_PARTIAL;
+      if (!b->records) {
+        const uint32_t off = (uint32_t)wfa::kOffsetNull;
+        r.end_v = (int)(off - (uint32_t)INT_MAX);
+        r.end_h = (int)off;
+        const int swg_match = -wf->penalties.match;
+        if (wf->penalties.distance_metric <= edit || swg_match == 0) {
+          r.score = classic_score(wf->penalties, 0, 0, b->score[i]);
+        } else {
+          const uint32_t num = (uint32_t)swg_match * ((uint32_t)r.end_v + (uint32_t)r.end_h) - (uint32_t)b->score[i];
+          r.score = (int)num / 2;
+        }
+      } else {
+        r.score = INT32_MIN;
+        r.end_v = r.end_h = -1;
+      }
     } else {
       r.score = INT32_MIN;
     }

@@ -109,7 +109,7 @@ bool run_breakpoints(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, 
     }
     const int wcap = wmax + 8;
     const wfa::HeurArgs heur = heuristic_of(wf);
-    int per_cu = wfa::breakpoint_max_blocks_per_cu(ncomp, b->bits, block, heur.wfadaptive != 0);
+    int per_cu = wfa::breakpoint_max_blocks_per_cu(ncomp, b->bits, block, heur.cut());
     if (per_cu <= 0) per_cu = 1;
     const int grid = (int)std::min<int64_t>((int64_t)ids.size(), (int64_t)per_cu * e->num_cus);
     const long long stride = 2LL * pen.nslots * ncomp * wcap;
@@ -245,6 +245,9 @@ int bialign_run(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, RunTo
           else fail_pair(st, WF_STATUS_UNATTAINABLE, p.pb);
         } else if (o.status == wfa::kDevMaxSteps) {
           fail_pair(st, WF_STATUS_MAX_STEPS_REACHED, p.pb);
+        } else if (o.status == wfa::kDevUnreachable) {
+          // a side the cut-off left unreachable (wavefront_bialign.c:1084-1111, 1286-1294)
+          fail_pair(st, WF_STATUS_UNATTAINABLE, p.pb);
         } else {
           if (o.status != wfa::kDevMaxSteps)
             fprintf(stderr, "[WFA] BiWFA breakpoint search failed (status %d) for pair %d\n", o.status, p.pair);

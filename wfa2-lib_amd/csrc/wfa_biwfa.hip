@@ -19,7 +19,9 @@ namespace {
 template <int NC, bool CUT>
 struct Side {
   MetaRegs<NC, CUT> meta;
-  int steps_wait;          // wf-adaptive countdown (wavefront_heuristic.c:506-521)
+  int steps_wait;          // cut-off countdown (wavefront_heuristic.c:506-521)
+  int nulls;               // null steps in a row (align_status.num_null_steps)
+  bool dead;               // indel / edit: the cut-off emptied the wavefront
   int cur;                 // ring slot of the latest multiple-of-g score
   int* ring;
   const uint64_t* P;
@@ -71,21 +73,30 @@ __device__ __forceinline__ WfRef<NC> wf_ref(const Side<NC, CUT>& sd, int s, int 
   return r;
 }
 
-// wf-adaptive cut-off of the side's M wavefront in ring slot `slot`, trimmed
+// Heuristic cut-off of the side's M wavefront in ring slot `slot`, trimmed
 // to [lo, hi] with array origin alo (wavefront_heuristic_cufoff,
-// wavefront_heuristic.c:506-561), then I1/D1/I2/D2 narrowed to M's range;
-// indel / edit then read M only inside the cut range (their compute step
-// NULLs the two cells beyond each end, wavefront_compute_edit.c:355-358).
+// wavefront_heuristic.c:506-561): wf-adaptive, then the band (static clamp
+// or adaptive window), one countdown for both; when M changed, I1/D1/I2/D2
+// are narrowed to M's range; indel / edit then read M only inside the cut
+// range (their compute step NULLs the two cells beyond each end,
+// wavefront_compute_edit.c:355-358).
 template <int NC, int BLOCK, bool CUT>
 __device__ void side_cutoff(Side<NC, CUT>& sd, const HeurArgs& heur, BlockRed<BLOCK>& red, int slot, int alo, int lo,
                             int hi, int plen, int tlen, int wcap, int metric) {
   if constexpr (CUT) {
     --sd.steps_wait;
-    if (sd.steps_wait > 0 || hi - lo + 1 < heur.min_len) return;
     const int* m = sd.ring + (slot * NC + kM) * wcap - alo;
-    int clo, chi;
-    wfadaptive_cut<BLOCK>(m, lo, hi, plen, tlen, heur.thr, red, &clo, &chi);
-    sd.steps_wait = heur.steps;
+    int clo = lo, chi = hi;
+    if (heur.wfadaptive && sd.steps_wait <= 0 && hi - lo + 1 >= heur.min_len) {
+      wfadaptive_cut<BLOCK>(m, lo, hi, plen, tlen, heur.thr, red, &clo, &chi);
+      sd.steps_wait = heur.steps;
+    }
+    if (heur.banded == 1) {
+      clo = max(clo, heur.band_lo);
+      chi = min(chi, heur.band_hi);
+    } else if (heur.banded == 2) {
+      banded_adaptive_cut(m, heur.band_lo, heur.band_hi, plen, tlen, sd.steps_wait, heur.steps, &clo, &chi);
+    }
     // unchanged M: nothing else is narrowed either (wavefront_heuristic.c:540-541)
     if (clo == lo && chi == hi) return;
     if ((threadIdx.x & 63) == slot) {
@@ -109,8 +120,10 @@ __device__ void side_cutoff(Side<NC, CUT>& sd, const HeurArgs& heur, BlockRed<BL
 
 // Compute score s of one side (no backtrace records) fused with its
 // extension; returns 1 when the end is reached (wavefront_termination.c),
-// -1 on an internal error, 0 otherwise.  *max_ak gets the largest
-// antidiagonal 2*offset-k of the extended M wavefront (0 when absent).
+// 2 when the cut-off left it unreachable (more null steps in a row than the
+// score scope, wavefront_extend.c:184-191), -1 on an internal error, 0
+// otherwise.  *max_ak gets the largest antidiagonal 2*offset-k of the
+// extended M wavefront (0 when absent).
 template <int NC, int BLOCK, int BITS, bool CUT>
 __device__ int side_step(Side<NC, CUT>& sd, int s, int plen, int tlen, const Penalties& pen, const HeurArgs& heur, int wcap,
                          bool want_max, int* max_ak, long long* dsteps, int (*sh_red)[2 * NC + 2],
@@ -119,7 +132,12 @@ __device__ int side_step(Side<NC, CUT>& sd, int s, int plen, int tlen, const Pen
   const int tid = threadIdx.x, lane = tid & 63, wid = uniform(tid >> 6);
   const int R = pen.nslots, g = pen.g;
   *max_ak = 0;
-  if (s % g) return 0;  // only multiples of g can be non-null
+  if (s % g) {  // only multiples of g can be non-null
+    if constexpr (CUT) {
+      if (++sd.nulls > pen.scope) return 2;
+    }
+    return 0;
+  }
   sd.cur = (sd.cur + 1 == R) ? 0 : sd.cur + 1;
   const int cur = sd.cur;
   const int dq_x = pen.x / g, dq_o1 = (pen.o1 + pen.e1) / g, dq_e1 = pen.e1 / g;
@@ -137,12 +155,22 @@ __device__ int side_step(Side<NC, CUT>& sd, int s, int plen, int tlen, const Pen
     d2  = get_src(sd.meta, s - pen.e2, cur - dq_e2, R, kD2, wcap);
   }
   const bool any = mx.valid() || mo1.valid() || i1.valid() || d1.valid() || mo2.valid() || i2.valid() || d2.valid();
-  // indel / edit: the previous wavefront is never empty end to end
-  if (NC == 1 && pen.metric <= kEdit && !mo1.valid()) return -1;
-  if (!any) {
+  // indel / edit: only a cut-off empties the previous wavefront, and every
+  // later one is then empty too (wavefront_compute_edit.c:344-352)
+  if (NC == 1 && pen.metric <= kEdit && !mo1.valid()) {
+    if constexpr (!CUT) return -1;
+    sd.dead = true;
     if (lane == cur) { sd.meta.tag = s; sd.meta.flags = 0u; }
     return 0;
   }
+  if (!any) {
+    if constexpr (CUT) {
+      if (++sd.nulls > pen.scope) return 2;
+    }
+    if (lane == cur) { sd.meta.tag = s; sd.meta.flags = 0u; }
+    return 0;
+  }
+  if constexpr (CUT) sd.nulls = 0;
   int lo, hi;
   if (NC == 1 && pen.metric <= kEdit) {
     lo = mo1.lim_lo - 1;
@@ -421,6 +449,8 @@ __device__ void side_init(Side<NC, CUT>& sd, int cb, int ce, int wcap, int steps
     for (int c = 0; c < NC; ++c) { sd.meta.cut.lo[c] = 0; sd.meta.cut.hi[c] = 0; }
   }
   sd.steps_wait = steps;  // wavefront_heuristic_clear (wavefront_heuristic.c:114-121)
+  sd.nulls = 0;
+  sd.dead = false;
   sd.cur = 0;
   sd.cb = cb;
   sd.ce = ce;
@@ -503,6 +533,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
       ++sf;
       rc = side_step<NC, BLOCK, BITS>(f, sf, plen, tlen, pen, a.heur, wcap, true, &ak, &dsteps, sh_red, red);
       if (rc < 0) { status = kDevRingOverflow; goto done; }
+      if (rc == 2) { status = kDevUnreachable; goto done; }
       if (fmax < ak) fmax = ak;
       last_forward = true;
       if (rc > 0) { status = kBiEndReached; goto done; }
@@ -510,10 +541,13 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
       ++sr;
       rc = side_step<NC, BLOCK, BITS>(r, sr, plen, tlen, pen, a.heur, wcap, true, &ak, &dsteps, sh_red, red);
       if (rc < 0) { status = kDevRingOverflow; goto done; }
+      if (rc == 2) { status = kDevUnreachable; goto done; }
       if (rmax < ak) rmax = ak;
       last_forward = false;
       if (rc > 0) { status = kBiEndReached; goto done; }
       if (sr + sf >= a.max_steps) { status = kDevMaxSteps; goto done; }
+      // both sides emptied: nothing changes until the step limit
+      if (f.dead && r.dead) { status = kDevMaxSteps; goto done; }
       if (sf > s_cap) { status = kDevBadTrace; goto done; }
     }
     // phase 2 (:1046-1080)
@@ -529,6 +563,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
           ++sr;
           rc = side_step<NC, BLOCK, BITS>(r, sr, plen, tlen, pen, a.heur, wcap, false, &ak, &dsteps, sh_red, red);
           if (rc < 0) { status = kDevRingOverflow; goto done; }
+          if (rc == 2) { status = kDevUnreachable; goto done; }
           if (rc > 0) { status = kBiEndReached; goto done; }
         }
         const int min_sf = (sf > scope - 1) ? sf - (scope - 1) : 0;
@@ -537,8 +572,11 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
         ++sf;
         rc = side_step<NC, BLOCK, BITS>(f, sf, plen, tlen, pen, a.heur, wcap, false, &ak, &dsteps, sh_red, red);
         if (rc < 0) { status = kDevRingOverflow; goto done; }
+        if (rc == 2) { status = kDevUnreachable; goto done; }
         if (rc > 0) { status = kBiEndReached; goto done; }
         if (sr + sf >= a.max_steps) { status = kDevMaxSteps; goto done; }
+        // both sides emptied and no breakpoint yet: nothing changes until the step limit
+        if (f.dead && r.dead && bp.score == INT_MAX) { status = kDevMaxSteps; goto done; }
         if (sf > s_cap) { status = kDevBadTrace; goto done; }
         last_forward = true;
       }
@@ -653,7 +691,7 @@ int breakpoint_max_blocks_per_cu(int ncomp, int bits, int block, bool cut) {
 }
 
 hipError_t launch_breakpoint(const BiArgs& a, int ncomp, int bits, int block, int grid, hipStream_t stream) {
-  const void* f = select_breakpoint(ncomp, bits, block, a.heur.wfadaptive != 0);
+  const void* f = select_breakpoint(ncomp, bits, block, a.heur.cut());
   if (!f) return hipErrorInvalidValue;
   void* args[] = {const_cast<BiArgs*>(&a)};
   return hipLaunchKernel(f, dim3(grid), dim3(block), args, 0, stream);

@@ -35,6 +35,8 @@ struct MetaRegs {
   CutRange<NC, CUT> cut;
 };
 
+constexpr int kFarDiagonal = 0x40000000;
+
 // A compute input.  lim_lo/lim_hi are the limits it contributes to the output
 // range (the shared null wavefront of the reference has lo = 1, hi = -1).
 // Reads at k in [lo, lo+span] return ring[base + k - lo], else NULL; an
@@ -43,14 +45,19 @@ struct Src {
   int lim_lo, lim_hi;
   int lo, span, base;
   __device__ __forceinline__ bool valid() const { return lim_lo <= lim_hi; }
+  __device__ __forceinline__ bool readable() const { return lo != kFarDiagonal && span >= 0; }
 };
-constexpr int kFarDiagonal = 0x40000000;
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ int lane_of(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 
+// The input wavefront of component c at score s_src, held in ring slot
+// `slot`.  A null wavefront is no input (invalid); with keep_null, one that
+// the cut-off emptied still gives its readable cells (the reference's
+// backtrace probes them, wavefront_backtrace.c:64-219).
 template <int NC, bool CUT>
-__device__ __forceinline__ Src get_src(const MetaRegs<NC, CUT>& m, int s_src, int slot, int R, int c, int wcap) {
+__device__ __forceinline__ Src get_src(const MetaRegs<NC, CUT>& m, int s_src, int slot, int R, int c, int wcap,
+                                       bool keep_null = false) {
   Src r{1, -1, kFarDiagonal, 0, 0};
   if (s_src < 0) return r;
   if (slot < 0) slot += R;
@@ -58,9 +65,12 @@ __device__ __forceinline__ Src get_src(const MetaRegs<NC, CUT>& m, int s_src, in
   const int tag = lane_of(m.tag, slot);
   const unsigned fl = (unsigned)lane_of((int)m.flags, slot);
   if (tag != s_src) return r;
-  if (!(fl & (1u << c)) || (fl & (0x100u << c))) return r;
-  r.lo = lane_of(m.lo[c], slot);
-  r.span = lane_of(m.hi[c], slot) - r.lo;
+  if (!(fl & (1u << c)) || (!keep_null && (fl & (0x100u << c)))) return r;
+  const int lo = lane_of(m.lo[c], slot), span = lane_of(m.hi[c], slot) - lo;
+  // a trimmed-empty wavefront has nothing to read (rd needs span >= 0)
+  if (span < 0) return r;
+  r.lo = lo;
+  r.span = span;
   if constexpr (CUT) {
     r.lim_lo = lane_of(m.cut.lo[c], slot);
     r.lim_hi = lane_of(m.cut.hi[c], slot);
@@ -79,6 +89,62 @@ __device__ __forceinline__ int rd(RingPtr ring, const Src& s, int k) {
   const bool in = (unsigned)i <= (unsigned)s.span;
   const int v = (int)ring[s.base + (in ? i : 0)];
   return in ? v : kOffsetNull;
+}
+
+// The choices of cell k over its sources, as one key: the winning M offset,
+// the winning M source and the gap-extension flags (the record byte's bits
+// 0..6), ties as the compute step breaks them.  Reads below 0 count as the
+// one NULL offset, so that two ways of holding NULL compare equal.
+template <int NC, typename RingPtr>
+__device__ __forceinline__ unsigned long long choice_key(RingPtr ring, const Src& mx, const Src& mo1, const Src& i1,
+                                                         const Src& d1, const Src& mo2, const Src& i2, const Src& d2,
+                                                         int k) {
+  auto rdn = [&](const Src& s, int kk) {
+    const int v = rd(ring, s, kk);
+    return v < 0 ? kOffsetNull : v;
+  };
+  int best = rdn(mx, k) + 1;
+  uint32_t src = kSrcX, bits = 0;
+  if constexpr (NC == 1) {
+    const int del = rdn(mo1, k + 1), ins = rdn(mo1, k - 1) + 1;
+    if (del > best) { best = del; src = kSrcD1; }
+    if (ins > best) { best = ins; src = kSrcI1; }
+  } else {
+    const int oml = rdn(mo1, k - 1), omr = rdn(mo1, k + 1), eil = rdn(i1, k - 1), edr = rdn(d1, k + 1);
+    bits = (eil >= oml ? kRecI1Ext : 0u) | (edr >= omr ? kRecD1Ext : 0u);
+    int ins2 = kOffsetNull;
+    if constexpr (NC == 5) {
+      const int o2l = rdn(mo2, k - 1), o2r = rdn(mo2, k + 1), e2l = rdn(i2, k - 1), e2r = rdn(d2, k + 1);
+      bits |= (e2l >= o2l ? kRecI2Ext : 0u) | (e2r >= o2r ? kRecD2Ext : 0u);
+      ins2 = max(o2l, e2l) + 1;
+      const int del2 = max(o2r, e2r);
+      if (del2 > best) { best = del2; src = kSrcD2; }
+    }
+    const int del1 = max(omr, edr), ins1 = max(oml, eil) + 1;
+    if (del1 > best) { best = del1; src = kSrcD1; }
+    if constexpr (NC == 5) { if (ins2 > best) { best = ins2; src = kSrcI2; } }
+    if (ins1 > best) { best = ins1; src = kSrcI1; }
+  }
+  return ((unsigned long long)(unsigned)best << 32) | (src | bits);
+}
+
+// Record byte of cell k for a step with an input the cut-off emptied: the
+// reference's backtrace probes that input's readable cells too
+// (wavefront_backtrace.c:64-219), so the byte holds the choices over the
+// probed sources.  The match run it then writes before the cell is the
+// extended offset m minus the winning probed offset; a winning offset past m
+// (possible only through such a cell) takes operations back instead, which
+// the replay does not do: kRecStale marks the cell.  A NULL cell (m) is
+// compared by its computed best (best) instead.
+template <int NC, typename RingPtr>
+__device__ __forceinline__ uint32_t probed_choice(RingPtr ring, const Src& mx, const Src& mo1, const Src& i1,
+                                                  const Src& d1, const Src& mo2, const Src& i2, const Src& d2, int k,
+                                                  int m, int best) {
+  const unsigned long long key = choice_key<NC>(ring, mx, mo1, i1, d1, mo2, i2, d2, k);
+  const int q = (int)(unsigned)(key >> 32);
+  uint32_t r = (uint32_t)key & 0x7Fu;
+  if (q >= 0 && q > max(m, best)) r |= kRecStale;
+  return r;
 }
 
 // Ring element stores.  The 16-bit LDS ring saturates to [-16384, 16383]:
@@ -242,6 +308,37 @@ __device__ void wfadaptive_cut(RingPtr m, int lo, int hi, int plen, int tlen, in
   const int bottom = max(kend, lo_r);
   *clo = lo_r;
   *chi = hi <= bottom ? hi : max(last, bottom);
+}
+
+// Banded adaptive cut-off (wavefront_heuristic_banded_adaptive,
+// wavefront_heuristic.c:460-500) of the extended M wavefront [*clo, *chi],
+// whose diagonal k holds m[k]: once the step countdown has run out, a
+// wavefront of at least 4 diagonals that is wider than the band keeps a
+// band-wide window, moved up by half the excess when the low end is farther
+// from the end than the high end, and again when the first quarter is
+// farther than the middle.  Every lane reads the same four cells.
+template <typename RingPtr>
+__device__ __forceinline__ void banded_adaptive_cut(RingPtr m, int band_lo, int band_hi, int plen, int tlen,
+                                                    int& steps_wait, int steps, int* clo, int* chi) {
+  if (steps_wait > 0) return;
+  const int lo = *clo, hi = *chi;
+  const int len = hi - lo + 1;
+  if (len < 4) return;
+  const int max_len = band_hi - band_lo + 1;
+  if (len > max_len) {
+    auto dist = [&](int k) {
+      const int off = (int)m[k];
+      return off >= 0 ? max(plen - (off - k), tlen - off) : kNullDistance;
+    };
+    const int leeway = (len - max_len) / 2, quarter = len / 4;
+    const int d0 = dist(lo), d1 = dist(lo + quarter), d2 = dist(lo + 2 * quarter), d3 = dist(hi);
+    int new_lo = lo;
+    if (d0 > d3) new_lo += leeway;
+    if (d1 > d2) new_lo += leeway;
+    *clo = max(new_lo, lo);
+    *chi = min(new_lo + max_len - 1, hi);
+  }
+  steps_wait = steps;
 }
 
 // Orders LDS traffic between the lanes of the (single) wave of a workgroup:

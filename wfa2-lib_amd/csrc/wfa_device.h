@@ -20,6 +20,11 @@ enum DevStatus : int {
   kDevRingOverflow = -2001,   // a wavefront wider than the launch's ring capacity
   kDevArenaFull = -2002,      // the backtrace-record arena ran out
   kDevBadTrace = -2003,       // backtrace found an inconsistent record (bug guard)
+  // Final statuses of the heuristic cut-off instances:
+  kDevUnreachable = -2004,    // more null steps in a row than the score scope
+                              // (WF_STATUS_END_UNREACHABLE, wavefront_extend.c:94-101)
+  kDevStaleTrace = -2005,     // the reference's backtrace would read a cell of a
+                              // wavefront the cut-off emptied (not replayed)
 };
 
 // Distance metrics, numbered as distance_metric_t (wavefront_penalties.h:41-47).
@@ -45,6 +50,10 @@ constexpr int kEditPruneWidth = 1000;
 // Record byte (one per diagonal of every non-null step):
 //   bits 0..2  winning M source: 0 X, 1 D2, 2 D1, 3 I2, 4 I1
 //   bits 3..6  gap-extension flags for I1, D1, I2, D2 at this diagonal
+//   bit  7     (cut-off instances) a source the reference's backtrace probes
+//              here lies in a wavefront the cut-off emptied, and changes a
+//              choice or a value of this diagonal: a backtrace through it is
+//              not replayed
 // A step's block is a 16-byte header {s, lo, hi, previous block} followed by
 // hi - lo + 1 record bytes, padded to 16 bytes.  The offsets themselves are
 // not stored: the backtrace walks (score, diagonal, component) with these
@@ -55,6 +64,7 @@ constexpr uint32_t kRecI1Ext = 1u << 3;
 constexpr uint32_t kRecD1Ext = 1u << 4;
 constexpr uint32_t kRecI2Ext = 1u << 5;
 constexpr uint32_t kRecD2Ext = 1u << 6;
+constexpr uint32_t kRecStale = 1u << 7;
 __host__ __device__ inline unsigned long long rec_units_for(int width) {
   return 1 + (unsigned long long)((width + 15) / 16);
 }
@@ -99,10 +109,13 @@ struct ForwardArgs {
   int pbf, pef, tbf, tef;
   int* out_end_k;              // end cell of each completed pair: diagonal and offset
   int* out_end_off;
-  // wf-adaptive heuristic (wavefront_heuristic.c:229-290); the kernel
-  // instance with the cut-off is launched when wfadaptive is set.
+  // Heuristic cut-off (wavefront_heuristic.c:506-561): wf-adaptive
+  // (:229-290) and a band (:451-500; banded 1 static, 2 adaptive, over
+  // diagonals [band_lo, band_hi]).  The kernel instance with the cut-off is
+  // launched when either is set.
   int wfadaptive;
   int h_min_len, h_thr, h_steps;
+  int banded, band_lo, band_hi;
   Penalties pen;
 };
 
@@ -188,11 +201,13 @@ struct BiOutcome {             // wf_bialign_breakpoint_t plus the search status
   int pad;
   long long dsteps;
 };
-// wf-adaptive parameters (wavefront_heuristic.c:229-290): the cut-off kernel
-// instance runs when wfadaptive is set.
+// Heuristic parameters (wavefront_heuristic.c:229-290, 451-500): the
+// cut-off kernel instance runs when wfadaptive or banded is set.
 struct HeurArgs {
   int wfadaptive;
   int min_len, thr, steps;
+  int banded, band_lo, band_hi;  // banded: 0 none, 1 static, 2 adaptive
+  bool cut() const { return wfadaptive || banded; }
 };
 
 struct BiArgs {

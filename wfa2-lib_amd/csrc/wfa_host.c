@@ -72,57 +72,56 @@ void wavefront_aligner_set_alignment_extension(wavefront_aligner_t* const wf) {
   wf->alignment_form.span = alignment_endsfree;
   wf->alignment_form.extension = true;
 }
-static void heuristic_reset(wavefront_heuristic_t* h) {
-  memset(h, 0, sizeof(*h));
-}
+/* The heuristic setters (wavefront_aligner.c:534-600 over
+   wavefront_heuristic.c:40-113) add their strategy to the ones already set
+   and overwrite the parameters they take (steps_between_cutoffs is shared);
+   set_none clears the strategy only. */
 void wavefront_aligner_set_heuristic_none(wavefront_aligner_t* const wf) {
-  heuristic_reset(&wf->heuristic);
   wf->heuristic.strategy = wf_heuristic_none;
 }
 void wavefront_aligner_set_heuristic_wfadaptive(wavefront_aligner_t* const wf,
     const int min_wavefront_length, const int max_distance_threshold, const int score_steps) {
-  heuristic_reset(&wf->heuristic);
-  wf->heuristic.strategy = wf_heuristic_wfadaptive;
+  wf->heuristic.strategy |= wf_heuristic_wfadaptive;
   wf->heuristic.min_wavefront_length = min_wavefront_length;
   wf->heuristic.max_distance_threshold = max_distance_threshold;
   wf->heuristic.steps_between_cutoffs = score_steps;
+  wf->heuristic.steps_wait = score_steps;
 }
 void wavefront_aligner_set_heuristic_wfmash(wavefront_aligner_t* const wf,
     const int min_wavefront_length, const int max_distance_threshold, const int score_steps) {
-  heuristic_reset(&wf->heuristic);
-  wf->heuristic.strategy = wf_heuristic_wfmash;
+  wf->heuristic.strategy |= wf_heuristic_wfmash;
   wf->heuristic.min_wavefront_length = min_wavefront_length;
   wf->heuristic.max_distance_threshold = max_distance_threshold;
   wf->heuristic.steps_between_cutoffs = score_steps;
+  wf->heuristic.steps_wait = score_steps;
 }
 void wavefront_aligner_set_heuristic_xdrop(wavefront_aligner_t* const wf,
     const int xdrop, const int score_steps) {
-  heuristic_reset(&wf->heuristic);
-  wf->heuristic.strategy = wf_heuristic_xdrop;
+  wf->heuristic.strategy |= wf_heuristic_xdrop;
   wf->heuristic.xdrop = xdrop;
   wf->heuristic.steps_between_cutoffs = score_steps;
+  wf->heuristic.steps_wait = score_steps;
 }
 void wavefront_aligner_set_heuristic_zdrop(wavefront_aligner_t* const wf,
     const int ydrop, const int score_steps) {
-  heuristic_reset(&wf->heuristic);
-  wf->heuristic.strategy = wf_heuristic_zdrop;
+  wf->heuristic.strategy |= wf_heuristic_zdrop;
   wf->heuristic.zdrop = ydrop;
   wf->heuristic.steps_between_cutoffs = score_steps;
+  wf->heuristic.steps_wait = score_steps;
 }
 void wavefront_aligner_set_heuristic_banded_static(wavefront_aligner_t* const wf,
     const int band_min_k, const int band_max_k) {
-  heuristic_reset(&wf->heuristic);
-  wf->heuristic.strategy = wf_heuristic_banded_static;
+  wf->heuristic.strategy |= wf_heuristic_banded_static;
   wf->heuristic.min_k = band_min_k;
   wf->heuristic.max_k = band_max_k;
 }
 void wavefront_aligner_set_heuristic_banded_adaptive(wavefront_aligner_t* const wf,
     const int band_min_k, const int band_max_k, const int score_steps) {
-  heuristic_reset(&wf->heuristic);
-  wf->heuristic.strategy = wf_heuristic_banded_adaptive;
+  wf->heuristic.strategy |= wf_heuristic_banded_adaptive;
   wf->heuristic.min_k = band_min_k;
   wf->heuristic.max_k = band_max_k;
   wf->heuristic.steps_between_cutoffs = score_steps;
+  wf->heuristic.steps_wait = score_steps;
 }
 void wavefront_aligner_set_max_alignment_steps(wavefront_aligner_t* const wf,
     const int max_alignment_steps) {

@@ -71,20 +71,30 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
     }
   };
 
-  // wf-adaptive cut-off of the M wavefront of ring slot `slot`, trimmed to
-  // [lo, hi] (wavefront_heuristic_cufoff, wavefront_heuristic.c:506-561):
-  // every h_steps-th non-empty M wavefront of at least h_min_len diagonals
-  // is cut, then I1/D1/I2/D2 are narrowed to M's range (:163-171).  Indel
-  // and edit then read M only inside the cut range: their compute step NULLs
-  // the two cells beyond each end (wavefront_compute_edit.c:355-358).
+  // Heuristic cut-off of the M wavefront of ring slot `slot`, trimmed to
+  // [lo, hi] (wavefront_heuristic_cufoff, wavefront_heuristic.c:506-561),
+  // with one step countdown for every strategy: wf-adaptive cuts every
+  // h_steps-th non-empty M wavefront of at least h_min_len diagonals; then
+  // the band: static clamps to [band_lo, band_hi] every time, adaptive moves
+  // a band-wide window.  When M changed, I1/D1/I2/D2 are narrowed to M's
+  // range (:163-171).  Indel and edit then read M only inside the cut range:
+  // their compute step NULLs the two cells beyond each end
+  // (wavefront_compute_edit.c:355-358).
   auto cutoff = [&](MetaRegs<NC, CUT>& mt, int& steps_wait, int slot, int alo, int lo, int hi, int plen_, int tlen_) {
     if constexpr (CUT) {
       --steps_wait;
-      if (steps_wait > 0 || hi - lo + 1 < a.h_min_len) return;
       const RT* m = ring + (slot * NC + kM) * wcap - alo;
-      int clo, chi;
-      wfadaptive_cut<BLOCK>(m, lo, hi, plen_, tlen_, a.h_thr, red, &clo, &chi);
-      steps_wait = a.h_steps;
+      int clo = lo, chi = hi;
+      if (a.wfadaptive && steps_wait <= 0 && hi - lo + 1 >= a.h_min_len) {
+        wfadaptive_cut<BLOCK>(m, lo, hi, plen_, tlen_, a.h_thr, red, &clo, &chi);
+        steps_wait = a.h_steps;
+      }
+      if (a.banded == 1) {
+        clo = max(clo, a.band_lo);
+        chi = min(chi, a.band_hi);
+      } else if (a.banded == 2) {
+        banded_adaptive_cut(m, a.band_lo, a.band_hi, plen_, tlen_, steps_wait, a.h_steps, &clo, &chi);
+      }
       // unchanged M: nothing else is narrowed either (wavefront_heuristic.c:540-541)
       if (clo == lo && chi == hi) return;
       if (lane == slot) {
@@ -225,8 +235,21 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
       cutoff(meta, steps_wait, 0, lo0, lo0, hi0, plen, tlen);
     }
 
+    int nulls = 0;  // null steps in a row, every score counted (align_status.num_null_steps)
     while (status == 1) {
       s += g;
+      if constexpr (CUT) {
+        // The scores between two multiples of g hold no wavefront: each is a
+        // null step of the reference, whose extension ends the alignment as
+        // unreachable once the run exceeds the score scope
+        // (wavefront_extend.c:94-101) unless the step limit comes first.
+        if (nulls + g - 1 > pen.scope) {
+          s = s - g + pen.scope + 1 - nulls;
+          status = (s >= a.max_steps) ? kDevMaxSteps : kDevUnreachable;
+          break;
+        }
+        nulls += g - 1;
+      }
       if (s >= a.max_steps) { status = kDevMaxSteps; break; }
       cur = (cur + 1 == R) ? 0 : cur + 1;
       Src mx{1, -1, kFarDiagonal, 0, 0}, i1 = mx, d1 = mx, mo2 = mx, i2 = mx, d2 = mx;
@@ -243,13 +266,42 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
       }
       const bool any = mx.valid() || mo1.valid() || i1.valid() || d1.valid() ||
                        mo2.valid() || i2.valid() || d2.valid();
-      // Indel and edit always extend the previous wavefront, which is never
-      // empty for an end-to-end alignment (wavefront_compute_edit.c:329-373).
-      if (NC == 1 && pen.metric <= kEdit && !mo1.valid()) { status = kDevBadTrace; break; }
+      // Indel and edit always extend the previous wavefront, which only a
+      // cut-off can empty: every later one is then empty as well
+      // (wavefront_compute_edit.c:344-352 spans the NULLed ends) and the
+      // reference computes on to the step limit.
+      if (NC == 1 && pen.metric <= kEdit && !mo1.valid()) { status = CUT ? kDevMaxSteps : kDevBadTrace; break; }
       if (!any) {  // null step (wavefront_compute_affine.c:236-243, wavefront_compute_linear.c:170-176)
+        if constexpr (CUT) {
+          if (++nulls > pen.scope) { status = kDevUnreachable; break; }
+        }
         if (s >= s_cap) { status = kDevBadTrace; break; }
         if (lane == cur) { meta.tag = s; meta.flags = 0u; }
         continue;
+      }
+      if constexpr (CUT) nulls = 0;
+      // Sources as the reference's backtrace probes them: also the readable
+      // cells of an input the cut-off emptied (a null wavefront to compute).
+      Src bmx = mx, bmo1 = mo1, bi1 = i1, bd1 = d1, bmo2 = mo2, bi2 = i2, bd2 = d2;
+      bool stale = false;
+      if constexpr (CUT) {
+        auto probe_src = [&](const Src& in, int s_src, int slot, int c) -> Src {
+          if (in.valid()) return in;
+          const Src b = get_src(meta, s_src, slot, R, c, wcap, true);
+          stale = stale || b.readable();
+          return b;
+        };
+        if (NC > 1 || pen.metric != kIndel) bmx = probe_src(mx, s - pen.x, cur - dq_x, kM);
+        bmo1 = probe_src(mo1, s - pen.o1 - pen.e1, cur - dq_o1, kM);
+        if constexpr (NC > 1) {
+          bi1 = probe_src(i1, s - pen.e1, cur - dq_e1, kI1);
+          bd1 = probe_src(d1, s - pen.e1, cur - dq_e1, kD1);
+        }
+        if constexpr (NC == 5) {
+          bmo2 = probe_src(mo2, s - pen.o2 - pen.e2, cur - dq_o2, kM);
+          bi2 = probe_src(i2, s - pen.e2, cur - dq_e2, kI2);
+          bd2 = probe_src(d2, s - pen.e2, cur - dq_e2, kD2);
+        }
       }
       // wavefront_compute_limits_input (wavefront_compute.c:40-86)
       int lo, hi;
@@ -337,13 +389,16 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
             uint32_t src = kSrcX;
             if (del > m) { m = del; src = kSrcD1; }
             if (ins > m) { m = ins; src = kSrcI1; }
+            const int best = m;
             if (!in_bounds(m, k, plen, tlen)) m = kOffsetNull;
-            if (rec) rec[k - lo] = (uint8_t)src;
             if (m != kOffsetNull) {
               if (prune_try) wmin = min(wmin, max(plen - (m - k), tlen - m));
               m = lcp_extend<BITS>(P, pb, T, tb, plen, tlen, k, m);
               hit = a.endsfree && endsfree_done(m, k, plen, tlen, a.pef, a.tef);
             }
+            if (rec)
+              rec[k - lo] = (uint8_t)(CUT && stale ? probed_choice<NC>(ring, bmx, bmo1, bi1, bd1, bmo2, bi2, bd2, k, m, best)
+                                                   : src);
             outM[k] = to_ring(m, ring);
             vM = (m != kOffsetNull);
           }
@@ -374,11 +429,13 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
           if (ins1 > best) { best = ins1; src = kSrcI1; }
           int m = best;
           if (!in_bounds(m, k, plen, tlen)) m = kOffsetNull;  // null-after-max
-          if (rec) rec[k - lo] = (uint8_t)(src | bits);
           if (m != kOffsetNull) {
             m = lcp_extend<BITS>(P, pb, T, tb, plen, tlen, k, m);
             hit = a.endsfree && endsfree_done(m, k, plen, tlen, a.pef, a.tef);
           }
+          if (rec)
+            rec[k - lo] = (uint8_t)(CUT && stale ? probed_choice<NC>(ring, bmx, bmo1, bi1, bd1, bmo2, bi2, bd2, k, m, best)
+                                                 : (src | bits));
           outM[k] = to_ring(m, ring);
           vM = (m != kOffsetNull);
           if (has_i1) { outI1[k] = to_ring(ins1, ring); vI1 = in_bounds(ins1, k, plen, tlen); }
@@ -537,7 +594,7 @@ __global__ void __launch_bounds__(256) wfa_backtrace_kernel(BacktraceArgs a) {
   int k = tlen - plen;
   int pos = cap;
   int nx = 0, nid = 0;
-  bool bad = false;
+  bool bad = false, stale = false;
   if (a.endsfree) {
     // Free trailing gaps after the end cell (wavefront_backtrace.c:345-355):
     // the pattern's rest as deletions, or the text's rest as insertions.
@@ -566,6 +623,10 @@ __global__ void __launch_bounds__(256) wfa_backtrace_kernel(BacktraceArgs a) {
       break;
     }
     const uint32_t rec = reinterpret_cast<const uint8_t*>(a.rec_arena + blk + 1)[k - (int)hdr.y];
+    if (rec & kRecStale) {
+      stale = bad = true;
+      break;
+    }
     if (comp == kM) {
       switch (rec & kRecSrcMask) {
         case kSrcX:
@@ -638,7 +699,11 @@ __global__ void __launch_bounds__(256) wfa_backtrace_kernel(BacktraceArgs a) {
     }
     if (!bad && (v != plen || h != tlen || w != cap)) bad = true;
   }
-  if (bad) {
+  if (stale) {
+    a.status[p] = kDevStaleTrace;
+    start = cap;
+    atomicAdd(a.fail_count, 1u);
+  } else if (bad) {
     a.status[p] = kDevBadTrace;
     start = cap;
     atomicAdd(a.fail_count, 1u);
@@ -709,7 +774,7 @@ int forward_max_blocks_per_cu(int family, int block, int ncomp, int bits, bool c
 
 hipError_t launch_forward(const ForwardArgs& a, int ncomp, int bits, const LaunchShape& shape,
                           hipStream_t stream) {
-  const void* f = select_forward(shape.family, ncomp, bits, shape.block, a.wfadaptive != 0);
+  const void* f = select_forward(shape.family, ncomp, bits, shape.block, a.wfadaptive || a.banded);
   if (!f) return hipErrorInvalidValue;
   if (shape.lds_bytes > 48 * 1024) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shape.lds_bytes);
