@@ -12,8 +12,9 @@
  * Exact path (bit-identical scores and CIGARs to the reference):
  *   distance_metric indel / edit / gap_linear / gap_affine / gap_affine_2p,
  *   heuristic wf_heuristic_none, wf_heuristic_wfadaptive (the default
- *   attributes), wf_heuristic_banded_static / _banded_adaptive, alone or
- *   after wf-adaptive, compute_score or compute_alignment;
+ *   attributes), wf_heuristic_wfmash, wf_heuristic_xdrop,
+ *   wf_heuristic_banded_static / _banded_adaptive, alone or combined,
+ *   compute_score or compute_alignment;
  *   alignment_end2end in memory modes high / med / low / ultralow;
  *   alignment_endsfree and the extension form in memory modes high / med /
  *   low (the reference rejects them in ultralow with exit(1), and so does
@@ -21,13 +22,14 @@
  *   reference's scalar extend kernel does.
  * Configurations outside it are rejected with WF_STATUS_UNSUPPORTED and a
  * "[WFA]" message on stderr; they never silently produce different results:
- * the other heuristics (x-drop, z-drop, wfmash), lambda sequences,
+ * the z-drop heuristic, lambda sequences,
  * free beginnings with a match bonus or with
  * indel/edit distances, indel/edit extension outside memory mode high, and
  * (per pair) a banded alignment whose reference backtrace would take
  * operations back through a wavefront the cut-off emptied.
  * Configuration errors (invalid penalties, ends-free lengths larger than the
- * sequences) print the reference's message and exit(1) as it does.
+ * sequences, x-drop / z-drop with indel or edit) print the reference's
+ * message and exit(1) as it does.
  *
  * Not thread-safe per aligner (as the reference, wavefront_aligner.c); one
  * aligner per host thread, any number of aligners per device.

@@ -95,8 +95,9 @@ typedef struct {
   int pbf, pef, tbf, tef;
   int end_k, end_off; /* end cell (wf_aligner->alignment_end_pos) */
   /* heuristics (wavefront_heuristic.c:229-290, 467-500) */
-  bool wfadaptive, banded_static, banded_adaptive;
-  int h_min_len, h_max_thr, h_steps, steps_wait, band_min_k, band_max_k;
+  bool wfadaptive, wfmash, xdrop, banded_static, banded_adaptive;
+  int h_min_len, h_max_thr, h_steps, steps_wait, band_min_k, band_max_k, h_xdrop;
+  int max_sw, max_sw_k;  /* x-drop: best Smith-Waterman score seen, its diagonal (INT_MAX: none) */
   bool stale_hit;     /* the backtrace met a cell whose choices change when
                          cut-away wavefronts are read (see cell_stale) */
   /* CIGAR, written backwards by the backtrace into ops[begin..end) */
@@ -193,7 +194,13 @@ static inline int32_t wf_rd(const owf_t* w, int k) {
 
 static void aln_set_penalties(oaln_t* a, const oracle_conf_t* conf) {
   a->metric = conf->metric;
+  /* the strategies as wavefront_aligner_init_heuristic selects them
+     (wavefront_aligner.c:189-228): wf-adaptive before wfmash, static band
+     before adaptive band */
   a->wfadaptive = (conf->heuristic & 4) != 0;
+  a->wfmash = !a->wfadaptive && (conf->heuristic & 0x40) != 0;
+  a->xdrop = (conf->heuristic & 0x10) != 0;
+  a->h_xdrop = conf->xdrop;
   a->banded_static = (conf->heuristic & 1) != 0;
   a->banded_adaptive = !a->banded_static && (conf->heuristic & 2) != 0;
   a->band_min_k = conf->min_k;
@@ -450,18 +457,30 @@ static int distance_end2end(const oaln_t* a, int k, int32_t off) {
   return (off >= 0) ? imax(a->plen - (off - k), a->tlen - off) : -ONULL;
 }
 
-/* wf-adaptive (wavefront_heuristic_wfadaptive and its reduce step,
-   wavefront_heuristic.c:229-290): keep the diagonals whose distance to the
-   end is within max_distance_threshold of the least one, scanning in from
-   both ends but never past the end diagonal. */
-static void wfadaptive_cut(oaln_t* a, owf_t* m) {
+/* wf_distance_end2end_weighted (wavefront_heuristic.c:134-145): what is left
+   of each sequence as a fraction of its length, scaled to the mean length
+   mfactor, in single-precision float and truncated. */
+static int distance_weighted(const oaln_t* a, int k, int32_t off, int mfactor) {
+  const int v = off - k, h = off;
+  const int left_v = (int)((float)(a->plen - v) / a->plen * mfactor);
+  const int left_h = (int)((float)(a->tlen - h) / a->tlen * mfactor);
+  return (off >= 0) ? imax(left_v, left_h) : -ONULL;
+}
+
+/* wf-adaptive and wfmash (wavefront_heuristic_wfadaptive and its reduce
+   step, wavefront_heuristic.c:190-290): keep the diagonals whose distance to
+   the end (plain, or weighted for wfmash) is within max_distance_threshold of
+   the least one, scanning in from both ends but never past the end
+   diagonal. */
+static void wfadaptive_cut(oaln_t* a, owf_t* m, bool weighted) {
   if (a->steps_wait > 0) return;
   const int lo = m->lo, hi = m->hi;
   if (hi - lo + 1 < a->h_min_len) return;
   int32_t* d = malloc(sizeof(int32_t) * (hi - lo + 1));
   int min_d = imax(a->plen, a->tlen), k;
+  const int mfactor = (int)((float)(a->plen + a->tlen) / 2);
   for (k = lo; k <= hi; ++k) {
-    d[k - lo] = distance_end2end(a, k, wf_raw(m, k));
+    d[k - lo] = weighted ? distance_weighted(a, k, wf_raw(m, k), mfactor) : distance_end2end(a, k, wf_raw(m, k));
     min_d = imin(min_d, d[k - lo]);
   }
   const int k_end = a->tlen - a->plen;
@@ -480,6 +499,49 @@ static void wfadaptive_cut(oaln_t* a, owf_t* m) {
   }
   m->hi = hi_r;
   free(d);
+  a->steps_wait = a->h_steps;
+}
+
+/* wavefront_heuristic_xdrop (wavefront_heuristic.c:294-370): Smith-Waterman
+   scores (swg_match * (v + h) - s) / 2 of the valid cells, swg_match being
+   -match, or -1 when match is 0; once a best score has been seen, the
+   diagonals scoring xdrop or more below it are cut from both ends (cells
+   with a negative offset are passed over), then the best is raised to this
+   wavefront's best (first diagonal in ascending order). */
+static void xdrop_cut(oaln_t* a, owf_t* m, int s) {
+  if (a->steps_wait > 0) return;
+  const int lo = m->lo, hi = m->hi;
+  const int swg_match = (a->match != 0) ? -a->match : -1;
+  int32_t* sw = malloc(sizeof(int32_t) * (hi - lo + 1));
+  int cmax = INT_MIN, cmax_k = 0, k;
+  for (k = lo; k <= hi; ++k) {
+    const int32_t off = wf_raw(m, k);
+    if (off < 0) continue;
+    const int v = off - k, h = off;
+    sw[k - lo] = (swg_match * (v + h) - s) / 2;
+    if (cmax < sw[k - lo]) { cmax = sw[k - lo]; cmax_k = k; }
+  }
+  if (a->max_sw_k != INT_MAX) {
+    const int best = a->max_sw;
+    /* best - sw in 32-bit wrap-around arithmetic */
+#define XDROP_KEEP(kk) ((int32_t)((uint32_t)best - (uint32_t)sw[(kk) - lo]) < a->h_xdrop)
+    for (k = lo; k <= hi; ++k) {
+      if (wf_raw(m, k) < 0) continue;
+      if (XDROP_KEEP(k)) break;
+    }
+    m->lo = k;
+    for (k = hi; k >= m->lo; --k) {
+      if (wf_raw(m, k) < 0) continue;
+      if (XDROP_KEEP(k)) break;
+    }
+    m->hi = k;
+#undef XDROP_KEEP
+    if (cmax > a->max_sw) { a->max_sw = cmax; a->max_sw_k = cmax_k; }
+  } else {
+    a->max_sw = cmax;
+    a->max_sw_k = cmax_k;
+  }
+  free(sw);
   a->steps_wait = a->h_steps;
 }
 
@@ -507,6 +569,10 @@ static void banded_adaptive(oaln_t* a, owf_t* m) {
   a->steps_wait = a->h_steps;
 }
 
+static bool has_cutoff(const oaln_t* a) {
+  return a->wfadaptive || a->wfmash || a->xdrop || a->banded_static || a->banded_adaptive;
+}
+
 /* wavefront_heuristic_cufoff (wavefront_heuristic.c:506-561) after the
    extension of score s: one countdown for all strategies, wf-adaptive then the
    band (static: clamp to [min_k, max_k] every time), and when M changed,
@@ -518,7 +584,9 @@ static void cutoff(oaln_t* a, int s) {
   if (!m || m->lo > m->hi) return;
   --a->steps_wait;
   const int lo = m->lo, hi = m->hi;
-  if (a->wfadaptive) wfadaptive_cut(a, m);
+  if (a->wfadaptive) wfadaptive_cut(a, m, false);
+  else if (a->wfmash) wfadaptive_cut(a, m, true);
+  if (a->xdrop) xdrop_cut(a, m, s);
   if (a->banded_static) {
     if (m->lo < a->band_min_k) m->lo = a->band_min_k;
     if (m->hi > a->band_max_k) m->hi = a->band_max_k;
@@ -577,7 +645,7 @@ static int extend_endsfree(oaln_t* a, int s) {
     m->v[k - m->alo] = ext;
     if (termination_endsfree(a, s, k, ext)) return 1;
   }
-  if (a->wfadaptive || a->banded_static || a->banded_adaptive) cutoff(a, s);
+  if (has_cutoff(a)) cutoff(a, s);
   return 0;
 }
 
@@ -603,7 +671,7 @@ static int extend(oaln_t* a, int s, int* max_ak) {
     a->end_off = a->tlen;
     return 1;
   }
-  if (a->wfadaptive || a->banded_static || a->banded_adaptive) cutoff(a, s);
+  if (has_cutoff(a)) cutoff(a, s);
   if (max_ak) *max_ak = best;
   return 0;
 }
@@ -618,6 +686,8 @@ static void aln_init(oaln_t* a, int comp_begin, int comp_end) {
   a->status = ST_OK;
   a->end_score = -1;
   a->null_steps = 0;  /* wavefront_aligner.c:411 */
+  a->max_sw = 0;      /* wavefront_heuristic_clear (wavefront_heuristic.c:114-121) */
+  a->max_sw_k = INT_MAX;
   a->stale_hit = false;
   a->steps_wait = a->h_steps;  /* wavefront_heuristic_clear (wavefront_heuristic.c:114-121) */
   owf_t* w[NCOMP];
@@ -693,7 +763,7 @@ static int64_t cell_choices(oaln_t* a, int s, int k, bool pv) {
    A NULL cell is compared by its computed best instead.  Recorded for the
    tests of a device path that declines such traces. */
 static void cell_stale(oaln_t* a, int s, int k) {
-  if (!(a->wfadaptive || a->banded_static || a->banded_adaptive)) return;
+  if (!(has_cutoff(a))) return;
   const int64_t q = cell_choices(a, s, k, true) >> 8;
   const int64_t p = cell_choices(a, s, k, false) >> 8;
   owf_t* w = wf_at(a, C_M, s);
@@ -1134,7 +1204,8 @@ static int bialign(const oracle_conf_t* conf, const char* P, int plen, const cha
   aln_set_penalties(&b.rev, conf); b.rev.modular = true;
   aln_set_penalties(&b.base, conf); b.base.modular = false;
   /* the base aligner runs without heuristic (wavefront_bialigner.c:64-67) */
-  b.base.wfadaptive = b.base.banded_static = b.base.banded_adaptive = false;
+  b.base.wfadaptive = b.base.wfmash = b.base.xdrop = false;
+  b.base.banded_static = b.base.banded_adaptive = false;
   b.P = P; b.T = T; b.plen = plen; b.tlen = tlen;
   b.Prev = reversed(P, plen); b.Trev = reversed(T, tlen);
   b.cigar_score = INT32_MIN;
@@ -1240,6 +1311,7 @@ int oracle_align(const oracle_conf_t* conf, const char* P, int plen, const char*
   if (stale) *stale = 0;
   if (ops_len) *ops_len = 0;
   const bool endsfree = conf->ends_free || conf->extension;
+  if ((conf->heuristic & 0x10) && conf->metric <= M_EDIT) return ORACLE_UNSUPPORTED;
   if (conf->memory_mode == 3) {
     if (endsfree) return ORACLE_UNSUPPORTED;
     return bialign(conf, P, plen, T, tlen, score, ops, ops_len, dsteps, end_score);

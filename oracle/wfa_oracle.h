@@ -38,16 +38,18 @@ typedef struct {
   int pattern_begin_free, pattern_end_free, text_begin_free, text_end_free;
   int extension;       /* alignment_form.extension (wavefront_align.c:57-63) */
   int heuristic;       /* wavefront_heuristic_t.strategy bits (wavefront_heuristic.h:41-49):
-                          1 banded static, 2 banded adaptive, 4 wf-adaptive */
+                          1 banded static, 2 banded adaptive, 4 wf-adaptive,
+                          0x10 x-drop, 0x40 wfmash */
   int min_wavefront_length, max_distance_threshold, steps_between_cutoffs;
   int min_k, max_k;    /* band of the banded strategies */
+  int xdrop;           /* x-drop threshold */
 } oracle_conf_t;
 
 /* Status returned for a configuration this restatement does not cover:
    ends-free with a free beginning and match < 0 (wavefront_compute.c:124-262),
-   indel / edit with a free beginning (see oracle_align), or ends-free /
-   extension with BiWFA, which the reference rejects with exit(1)
-   (wavefront_align.c:66-80). */
+   indel / edit with a free beginning (see oracle_align), or what the
+   reference rejects with exit(1): ends-free / extension with BiWFA
+   (wavefront_align.c:66-80), x-drop with indel / edit (:82-89). */
 #define ORACLE_UNSUPPORTED (-400)
 
 /*

@@ -41,10 +41,8 @@ def test_default_attributes_match_reference():
     dict(metric="affine", penalties=(-1, 4, 6, 2), ends_free=(2, 0, 0, 0)),  # free beginning, match < 0
     dict(metric="affine", memory="ultralow", ends_free=(0, 0, 0, 0)),  # ends-free span in BiWFA
     dict(metric="edit", memory="med", extension=True),                 # indel/edit extension outside high
-    dict(metric="affine", heuristic=(0x10, 0, 0, 1)),                  # x-drop
     dict(metric="affine", heuristic=(0x20, 0, 0, 1)),                  # z-drop
-    dict(metric="affine", heuristic=(0x40, 10, 50, 1)),                # wfmash
-    dict(metric="affine", heuristic=(0x04 | 0x10, 10, 50, 1)),         # wf-adaptive + x-drop
+    dict(metric="affine", heuristic=(0x04 | 0x20, 10, 50, 1)),         # wf-adaptive + z-drop
     dict(metric="affine", heuristic=(0x01 | 0x20, 0, 0, 1, -5, 5)),    # banded static + z-drop
 ])
 def test_unsupported_configurations_are_refused(kw):
@@ -112,6 +110,8 @@ print("returned")
     (dict(metric="affine", memory="ultralow", ends_free=(0, 2, 0, 2)), b"BiWFA ends-free has not been tested"),
     (dict(metric="affine", memory="ultralow", extension=True), b"BiWFA ends-free has not been tested"),
     (dict(metric="affine", ends_free=(0, 0, 20, 0)), b"Ends-free parameters must be not larger"),
+    (dict(metric="edit", heuristic=(0x10, 0, 0, 1, 0, 0, 20)), b"Heuristics drops are not compatible"),
+    (dict(metric="indel", heuristic=(0x20, 0, 0, 1)), b"Heuristics drops are not compatible"),
 ])
 def test_configuration_errors_exit_like_the_reference(kw, msg):
     """wavefront_align_presets__checks (wavefront_align.c:46-106) ends the

@@ -446,6 +446,50 @@ def test_oracle_banded_matches_reference_build(metric, pen):
                         assert x.ops(i) == y.ops(i), (h, mem, i)
 
 
+DROPS = [
+    (0x40, 10, 50, 1),                   # wfmash
+    (0x40, 10, 20, 3),
+    (0x40 | 1, 10, 50, 1, -20, 20),      # wfmash then banded static
+    (0x10, 0, 0, 1, 0, 0, 30),           # x-drop
+    (0x10, 0, 0, 5, 0, 0, 100),
+    (0x14, 10, 50, 1, 0, 0, 40),         # wf-adaptive then x-drop
+    (0x11, 0, 0, 1, -10, 10, 40),        # x-drop then banded static
+]
+
+
+@pytest.mark.skipif(not O.ref_available("v2"), reason="scalar reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("metric,pen", [("affine", (-3, 5, 3, 2)), ("affine", (-1, 4, 6, 2)), ("affine", (0, 4, 6, 2)),
+                                        ("affine2p", (-1, 4, 6, 2, 24, 1)), ("linear", (-2, 4, 2)),
+                                        ("edit", (0, 1, 1))])
+def test_oracle_wfmash_and_xdrop_match_reference_build(metric, pen):
+    """wfmash (weighted distances in single-precision float,
+    wavefront_heuristic.c:134-145, 190-208) and x-drop on Smith-Waterman
+    scores (:294-370), alone and combined, in memory modes high and ultralow.
+    x-drop with indel / edit is a configuration error in the reference
+    (exit(1)), so edit runs wfmash only; an edit wavefront a band empties
+    crashes the reference's high-memory form, so those pairs are left out
+    there as in the banded test."""
+    pats, texts = _banded_pairs()
+    for h in DROPS:
+        if metric == "edit" and h[0] & 0x10:
+            continue
+        sel = list(range(len(pats)))
+        if metric == "edit":
+            r = O.oracle_align(O.Conf(metric, pen, "high", heuristic=h, max_steps=30000), _pairs(pats, texts))
+            sel = [i for i in sel if r.status[i] != O.WF_STATUS_MAX_STEPS_REACHED]
+        pairs = _pairs([pats[i] for i in sel], [texts[i] for i in sel])
+        for mem in ("high", "ultralow"):
+            for so in (False, True):
+                conf = O.Conf(metric, pen, mem, so, heuristic=h, max_steps=30000)
+                x = O.oracle_align(conf, pairs)
+                y = O.ref_align(conf, pairs, isa="v2")
+                assert list(x.status) == list(y.status), (h, mem, so)
+                assert list(x.score) == list(y.score), (h, mem, so)
+                if not so:
+                    for i in range(pairs.n):
+                        assert x.ops(i) == y.ops(i), (h, mem, i)
+
+
 def test_oracle_banded_static_is_banded_dynamic_programming():
     """A static band [min_k, max_k] that holds diagonals 0 and k_end limits
     the alignment to cells with min_k - 1 <= h - v <= max_k + 1 at most; the

@@ -326,9 +326,8 @@ int check_supported(wavefront_aligner_t* wf) {
   // The strategies take effect as wavefront_aligner_init_heuristic selects
   // them (wavefront_aligner.c:189-228): wf-adaptive before wfmash, x-drop
   // before z-drop, banded static before banded adaptive.
-  else if ((wf->heuristic.strategy & (wf_heuristic_xdrop | wf_heuristic_zdrop)) ||
-           ((wf->heuristic.strategy & wf_heuristic_wfmash) && !(wf->heuristic.strategy & wf_heuristic_wfadaptive)))
-    why = "the x-drop, z-drop and wfmash heuristics";
+  else if ((wf->heuristic.strategy & wf_heuristic_zdrop) && !(wf->heuristic.strategy & wf_heuristic_xdrop))
+    why = "the z-drop heuristic";
   else if (make_penalties(wf->penalties).nslots > 64)
     why = "penalties whose score scope needs more than 64 wavefront slots";
   if (!why) return WF_STATUS_OK;
@@ -341,6 +340,11 @@ int check_supported(wavefront_aligner_t* wf) {
 // errors the reference reports with exit(1), checked for every pair.
 void presets_check(wavefront_aligner_t* wf, const int* plens, const int* tlens, int n) {
   const alignment_form_t& f = wf->alignment_form;
+  if ((wf->heuristic.strategy & (wf_heuristic_xdrop | wf_heuristic_zdrop)) &&
+      (wf->penalties.distance_metric == edit || wf->penalties.distance_metric == indel)) {
+    fprintf(stderr, "[WFA] Heuristics drops are not compatible with 'edit'/'indel' distance metrics\n");
+    exit(1);
+  }
   for (int i = 0; i < n; ++i) {
     const int plen = plens[i], tlen = tlens[i];
     int pbf = f.pattern_begin_free, pef = f.pattern_end_free, tbf = f.text_begin_free, tef = f.text_end_free;
@@ -375,12 +379,15 @@ void presets_check(wavefront_aligner_t* wf, const wfa_pair_t* pairs, int n) {
 wfa::HeurArgs heuristic_of(const wavefront_aligner_t* wf) {
   wfa::HeurArgs h{};
   const int st = (int)wf->heuristic.strategy;
-  h.wfadaptive = (st & wf_heuristic_wfadaptive) ? 1 : 0;
+  h.wfadaptive = (st & wf_heuristic_wfadaptive) ? 1 : (st & wf_heuristic_wfmash) ? 2 : 0;
+  h.xdrop = (st & wf_heuristic_xdrop) ? 1 : 0;
   h.banded = (st & wf_heuristic_banded_static) ? 1 : (st & wf_heuristic_banded_adaptive) ? 2 : 0;
   if (!h.cut()) return h;
   h.min_len = wf->heuristic.min_wavefront_length;
   h.thr = wf->heuristic.max_distance_threshold;
   h.steps = wf->heuristic.steps_between_cutoffs;
+  h.xdrop_thr = wf->heuristic.xdrop;
+  h.swg_match = wf->penalties.match != 0 ? -wf->penalties.match : -1;  // wavefront_heuristic.c:303-304
   h.band_lo = wf->heuristic.min_k;
   h.band_hi = wf->heuristic.max_k;
   return h;
@@ -637,13 +644,7 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
   fa.tef = js.ef.tef;
   fa.out_end_k = js.end_k;
   fa.out_end_off = js.end_off;
-  fa.wfadaptive = js.heur.wfadaptive;
-  fa.h_min_len = js.heur.min_len;
-  fa.h_thr = js.heur.thr;
-  fa.h_steps = js.heur.steps;
-  fa.banded = js.heur.banded;
-  fa.band_lo = js.heur.band_lo;
-  fa.band_hi = js.heur.band_hi;
+  fa.heur = js.heur;
   fa.pen = pen;
   if (!hip_ok(wfa::launch_reset_counters(c, 0, st), "reset")) return WF_STATUS_OOM;
   (void)hipEventRecord(grp.ev[0], st);

@@ -21,6 +21,7 @@ struct Side {
   MetaRegs<NC, CUT> meta;
   int steps_wait;          // cut-off countdown (wavefront_heuristic.c:506-521)
   int nulls;               // null steps in a row (align_status.num_null_steps)
+  XdropState xs;           // x-drop's best score (wavefront_heuristic.c:114-121)
   bool dead;               // indel / edit: the cut-off emptied the wavefront
   int cur;                 // ring slot of the latest multiple-of-g score
   int* ring;
@@ -73,22 +74,26 @@ __device__ __forceinline__ WfRef<NC> wf_ref(const Side<NC, CUT>& sd, int s, int 
   return r;
 }
 
-// Heuristic cut-off of the side's M wavefront in ring slot `slot`, trimmed
-// to [lo, hi] with array origin alo (wavefront_heuristic_cufoff,
-// wavefront_heuristic.c:506-561): wf-adaptive, then the band (static clamp
-// or adaptive window), one countdown for both; when M changed, I1/D1/I2/D2
-// are narrowed to M's range; indel / edit then read M only inside the cut
-// range (their compute step NULLs the two cells beyond each end,
-// wavefront_compute_edit.c:355-358).
+// Heuristic cut-off of the side's M wavefront of score s in ring slot
+// `slot`, trimmed to [lo, hi] with array origin alo
+// (wavefront_heuristic_cufoff, wavefront_heuristic.c:506-561): wf-adaptive
+// or wfmash, x-drop, then the band (static clamp or adaptive window), one
+// countdown for all; when M changed, I1/D1/I2/D2 are narrowed to M's range;
+// indel / edit then read M only inside the cut range (their compute step
+// NULLs the two cells beyond each end, wavefront_compute_edit.c:355-358).
 template <int NC, int BLOCK, bool CUT>
 __device__ void side_cutoff(Side<NC, CUT>& sd, const HeurArgs& heur, BlockRed<BLOCK>& red, int slot, int alo, int lo,
-                            int hi, int plen, int tlen, int wcap, int metric) {
+                            int hi, int s, int plen, int tlen, int wcap, int metric) {
   if constexpr (CUT) {
     --sd.steps_wait;
     const int* m = sd.ring + (slot * NC + kM) * wcap - alo;
     int clo = lo, chi = hi;
     if (heur.wfadaptive && sd.steps_wait <= 0 && hi - lo + 1 >= heur.min_len) {
-      wfadaptive_cut<BLOCK>(m, lo, hi, plen, tlen, heur.thr, red, &clo, &chi);
+      wfadaptive_cut<BLOCK>(m, lo, hi, plen, tlen, heur.thr, heur.wfadaptive == 2, red, &clo, &chi);
+      sd.steps_wait = heur.steps;
+    }
+    if (heur.xdrop && sd.steps_wait <= 0) {
+      xdrop_cut<BLOCK>(m, s, heur.swg_match, heur.xdrop_thr, red, sd.xs, &clo, &chi);
       sd.steps_wait = heur.steps;
     }
     if (heur.banded == 1) {
@@ -342,7 +347,7 @@ __device__ int side_step(Side<NC, CUT>& sd, int s, int plen, int tlen, const Pen
       return 1;
     }
   }
-  if (!rM.null()) side_cutoff<NC, BLOCK>(sd, heur, red, cur, lo, rM.first, rM.last, plen, tlen, wcap, pen.metric);
+  if (!rM.null()) side_cutoff<NC, BLOCK>(sd, heur, red, cur, lo, rM.first, rM.last, s, plen, tlen, wcap, pen.metric);
   return 0;
 }
 
@@ -450,6 +455,7 @@ __device__ void side_init(Side<NC, CUT>& sd, int cb, int ce, int wcap, int steps
   }
   sd.steps_wait = steps;  // wavefront_heuristic_clear (wavefront_heuristic.c:114-121)
   sd.nulls = 0;
+  sd.xs.clear();
   sd.dead = false;
   sd.cur = 0;
   sd.cb = cb;
@@ -474,7 +480,7 @@ __device__ int side_extend0(Side<NC, CUT>& sd, int plen, int tlen, int wcap, con
     sd.end_score = 0;
     return 1;
   }
-  side_cutoff<NC, BLOCK>(sd, heur, red, 0, 0, 0, 0, plen, tlen, wcap, pen.metric);
+  side_cutoff<NC, BLOCK>(sd, heur, red, 0, 0, 0, 0, 0, plen, tlen, wcap, pen.metric);
   return 0;
 }
 

@@ -277,27 +277,44 @@ struct BlockRed {
 // Distance of a NULL cell (-WAVEFRONT_OFFSET_NULL, wavefront_heuristic.c:125-133).
 constexpr int kNullDistance = 1 << 30;
 
+// Float to int conversion as x86's cvttss2si does it (the reference's
+// host): truncation, and INT_MIN for NaN and out-of-range values.
+__device__ __forceinline__ int trunc_to_int(float f) {
+  return (f >= -2147483648.0f && f < 2147483648.0f) ? (int)f : INT_MIN;
+}
+
+// Distance of cell (k, off) to the end: max(plen - v, tlen - h)
+// (wf_distance_end2end, wavefront_heuristic.c:125-133) or, weighted (wfmash,
+// :134-145), what is left of each sequence as a fraction of its length
+// scaled to the mean length mfactor, in IEEE single precision and truncated;
+// a NULL cell is kNullDistance.
+__device__ __forceinline__ int end_distance(int off, int k, int plen, int tlen, bool weighted, int mfactor) {
+  if (off < 0) return kNullDistance;
+  const int v = off - k, h = off;
+  if (!weighted) return max(plen - v, tlen - h);
+  const int left_v = trunc_to_int((float)(plen - v) / (float)plen * (float)mfactor);
+  const int left_h = trunc_to_int((float)(tlen - h) / (float)tlen * (float)mfactor);
+  return max(left_v, left_h);
+}
+
 // wf-adaptive cut-off (wavefront_heuristic_wfadaptive and its reduce step,
-// wavefront_heuristic.c:229-290) of the extended M wavefront [lo, hi], whose
-// diagonal k holds m[k]: keeps the diagonals whose distance to the end,
-// max(plen - v, tlen - h), is within thr of the least one, scanning in from
+// wavefront_heuristic.c:190-290) of the extended M wavefront [lo, hi], whose
+// diagonal k holds m[k]: keeps the diagonals whose distance to the end (plain,
+// or weighted for wfmash) is within thr of the least one, scanning in from
 // both ends but never past the end diagonal.  Two block passes: the least
 // distance, then the first and last diagonals within thr of it.  The kept
 // range (uniform) goes to *clo, *chi.
 template <int BLOCK, typename RingPtr>
-__device__ void wfadaptive_cut(RingPtr m, int lo, int hi, int plen, int tlen, int thr, BlockRed<BLOCK>& red,
-                               int* clo, int* chi) {
+__device__ void wfadaptive_cut(RingPtr m, int lo, int hi, int plen, int tlen, int thr, bool weighted,
+                               BlockRed<BLOCK>& red, int* clo, int* chi) {
   const int tid = threadIdx.x;
+  const int mfactor = weighted ? trunc_to_int((float)(plen + tlen) / 2.0f) : 0;
   int dmin = max(plen, tlen);
-  for (int k = lo + tid; k <= hi; k += BLOCK) {
-    const int off = (int)m[k];
-    dmin = min(dmin, off >= 0 ? max(plen - (off - k), tlen - off) : kNullDistance);
-  }
+  for (int k = lo + tid; k <= hi; k += BLOCK) dmin = min(dmin, end_distance((int)m[k], k, plen, tlen, weighted, mfactor));
   dmin = red.min_all(dmin);
   int first = INT_MAX, last = INT_MIN;
   for (int k = lo + tid; k <= hi; k += BLOCK) {
-    const int off = (int)m[k];
-    const int d = off >= 0 ? max(plen - (off - k), tlen - off) : kNullDistance;
+    const int d = end_distance((int)m[k], k, plen, tlen, weighted, mfactor);
     if (d - dmin <= thr) { first = min(first, k); last = max(last, k); }
   }
   first = red.min_all(first);
@@ -308,6 +325,59 @@ __device__ void wfadaptive_cut(RingPtr m, int lo, int hi, int plen, int tlen, in
   const int bottom = max(kend, lo_r);
   *clo = lo_r;
   *chi = hi <= bottom ? hi : max(last, bottom);
+}
+
+// x-drop state of one alignment (wavefront_heuristic_clear,
+// wavefront_heuristic.c:114-121): the best Smith-Waterman score seen, once
+// there is one.
+struct XdropState {
+  int best;
+  bool has;
+  __device__ __forceinline__ void clear() { best = 0; has = false; }
+};
+
+// x-drop cut-off (wavefront_heuristic_xdrop, wavefront_heuristic.c:294-370)
+// of the extended M wavefront [*clo, *chi] of score s: Smith-Waterman scores
+// (swg_match * (v + h) - s) / 2 of the valid cells; once a best score has
+// been seen, the range keeps from the first to the last valid diagonal
+// scoring less than xdrop below it (none: an empty range [hi + 1, hi]), and
+// the best rises to this wavefront's best.  Differences wrap around in 32
+// bits as the reference's int arithmetic does.
+template <int BLOCK, typename RingPtr>
+__device__ void xdrop_cut(RingPtr m, int s, int swg_match, int xdrop, BlockRed<BLOCK>& red, XdropState& xs,
+                          int* clo, int* chi) {
+  const int tid = threadIdx.x;
+  const int lo = *clo, hi = *chi;
+  auto sw_of = [&](int off, int k) { return (swg_match * ((off - k) + off) - s) / 2; };
+  int cmax = INT_MIN;
+  for (int k = lo + tid; k <= hi; k += BLOCK) {
+    const int off = (int)m[k];
+    if (off >= 0) cmax = max(cmax, sw_of(off, k));
+  }
+  cmax = red.max_all(cmax);
+  if (xs.has) {
+    const int best = xs.best;
+    int first = INT_MAX, last = INT_MIN;
+    for (int k = lo + tid; k <= hi; k += BLOCK) {
+      const int off = (int)m[k];
+      if (off >= 0 && (int)((unsigned)best - (unsigned)sw_of(off, k)) < xdrop) {
+        first = min(first, k);
+        last = max(last, k);
+      }
+    }
+    first = red.min_all(first);
+    last = red.max_all(last);
+    if (first == INT_MAX) {
+      *clo = hi + 1;  // both scans pass every diagonal
+    } else {
+      *clo = first;
+      *chi = last;
+    }
+    if (cmax > xs.best) xs.best = cmax;
+  } else {
+    xs.best = cmax;
+    xs.has = true;
+  }
 }
 
 // Banded adaptive cut-off (wavefront_heuristic_banded_adaptive,

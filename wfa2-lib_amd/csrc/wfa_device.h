@@ -72,6 +72,18 @@ enum RecSrc : uint32_t { kSrcX = 0, kSrcD2 = 1, kSrcD1 = 2, kSrcI2 = 3, kSrcI1 =
 constexpr uint32_t kNoBlock = 0xFFFFFFFFu;
 constexpr uint32_t kChunkUnits = 4096;   // record chunk grabbed per workgroup, in 16-byte units
 
+// Heuristic cut-off parameters (wavefront_heuristic.c:506-561), as
+// wavefront_aligner_init_heuristic selects the strategies: the kernel
+// instances with the cut-off run when any is set.
+struct HeurArgs {
+  int wfadaptive;                // 0 none, 1 wf-adaptive, 2 wfmash (weighted distances, :134-145)
+  int min_len, thr, steps;       // and the countdown shared by every strategy
+  int xdrop, xdrop_thr;          // x-drop (:294-370) on Smith-Waterman scores
+  int swg_match;                 // (swg_match * (v + h) - s) / 2: -match, or -1 when match is 0
+  int banded, band_lo, band_hi;  // banded: 0 none, 1 static, 2 adaptive (:451-500)
+  __host__ __device__ bool cut() const { return wfadaptive || xdrop || banded; }
+};
+
 struct ForwardArgs {
   int n_pairs;
   const int* order;            // processing order (pair ids), longest first
@@ -109,13 +121,7 @@ struct ForwardArgs {
   int pbf, pef, tbf, tef;
   int* out_end_k;              // end cell of each completed pair: diagonal and offset
   int* out_end_off;
-  // Heuristic cut-off (wavefront_heuristic.c:506-561): wf-adaptive
-  // (:229-290) and a band (:451-500; banded 1 static, 2 adaptive, over
-  // diagonals [band_lo, band_hi]).  The kernel instance with the cut-off is
-  // launched when either is set.
-  int wfadaptive;
-  int h_min_len, h_thr, h_steps;
-  int banded, band_lo, band_hi;
+  HeurArgs heur;                // heuristic cut-off (none: the exact instances)
   Penalties pen;
 };
 
@@ -200,14 +206,6 @@ struct BiOutcome {             // wf_bialign_breakpoint_t plus the search status
   int rev_end_score;
   int pad;
   long long dsteps;
-};
-// Heuristic parameters (wavefront_heuristic.c:229-290, 451-500): the
-// cut-off kernel instance runs when wfadaptive or banded is set.
-struct HeurArgs {
-  int wfadaptive;
-  int min_len, thr, steps;
-  int banded, band_lo, band_hi;  // banded: 0 none, 1 static, 2 adaptive
-  bool cut() const { return wfadaptive || banded; }
 };
 
 struct BiArgs {

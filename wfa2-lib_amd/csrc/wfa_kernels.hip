@@ -71,29 +71,36 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
     }
   };
 
-  // Heuristic cut-off of the M wavefront of ring slot `slot`, trimmed to
-  // [lo, hi] (wavefront_heuristic_cufoff, wavefront_heuristic.c:506-561),
-  // with one step countdown for every strategy: wf-adaptive cuts every
-  // h_steps-th non-empty M wavefront of at least h_min_len diagonals; then
-  // the band: static clamps to [band_lo, band_hi] every time, adaptive moves
-  // a band-wide window.  When M changed, I1/D1/I2/D2 are narrowed to M's
-  // range (:163-171).  Indel and edit then read M only inside the cut range:
-  // their compute step NULLs the two cells beyond each end
-  // (wavefront_compute_edit.c:355-358).
-  auto cutoff = [&](MetaRegs<NC, CUT>& mt, int& steps_wait, int slot, int alo, int lo, int hi, int plen_, int tlen_) {
+  // Heuristic cut-off of the M wavefront of score s in ring slot `slot`,
+  // trimmed to [lo, hi] (wavefront_heuristic_cufoff,
+  // wavefront_heuristic.c:506-561), with one step countdown for every
+  // strategy, in the reference's order: wf-adaptive / wfmash cuts every
+  // steps-th non-empty M wavefront of at least min_len diagonals; x-drop cuts
+  // on Smith-Waterman scores; then the band: static clamps to [band_lo,
+  // band_hi] every time, adaptive moves a band-wide window.  When M changed,
+  // I1/D1/I2/D2 are narrowed to M's range (:163-171).  Indel and edit then
+  // read M only inside the cut range: their compute step NULLs the two cells
+  // beyond each end (wavefront_compute_edit.c:355-358).
+  auto cutoff = [&](MetaRegs<NC, CUT>& mt, int& steps_wait, XdropState& xs, int slot, int alo, int lo, int hi,
+                    int s_, int plen_, int tlen_) {
     if constexpr (CUT) {
+      const HeurArgs& hr = a.heur;
       --steps_wait;
       const RT* m = ring + (slot * NC + kM) * wcap - alo;
       int clo = lo, chi = hi;
-      if (a.wfadaptive && steps_wait <= 0 && hi - lo + 1 >= a.h_min_len) {
-        wfadaptive_cut<BLOCK>(m, lo, hi, plen_, tlen_, a.h_thr, red, &clo, &chi);
-        steps_wait = a.h_steps;
+      if (hr.wfadaptive && steps_wait <= 0 && hi - lo + 1 >= hr.min_len) {
+        wfadaptive_cut<BLOCK>(m, lo, hi, plen_, tlen_, hr.thr, hr.wfadaptive == 2, red, &clo, &chi);
+        steps_wait = hr.steps;
       }
-      if (a.banded == 1) {
-        clo = max(clo, a.band_lo);
-        chi = min(chi, a.band_hi);
-      } else if (a.banded == 2) {
-        banded_adaptive_cut(m, a.band_lo, a.band_hi, plen_, tlen_, steps_wait, a.h_steps, &clo, &chi);
+      if (hr.xdrop && steps_wait <= 0) {
+        xdrop_cut<BLOCK>(m, s_, hr.swg_match, hr.xdrop_thr, red, xs, &clo, &chi);
+        steps_wait = hr.steps;
+      }
+      if (hr.banded == 1) {
+        clo = max(clo, hr.band_lo);
+        chi = min(chi, hr.band_hi);
+      } else if (hr.banded == 2) {
+        banded_adaptive_cut(m, hr.band_lo, hr.band_hi, plen_, tlen_, steps_wait, hr.steps, &clo, &chi);
       }
       // unchanged M: nothing else is narrowed either (wavefront_heuristic.c:540-541)
       if (clo == lo && chi == hi) return;
@@ -163,7 +170,9 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
 #pragma unroll
       for (int c = 0; c < NC; ++c) { meta.cut.lo[c] = 0; meta.cut.hi[c] = 0; }
     }
-    int steps_wait = a.h_steps;  // wavefront_heuristic_clear (wavefront_heuristic.c:114-121)
+    int steps_wait = a.heur.steps;  // wavefront_heuristic_clear (wavefront_heuristic.c:114-121)
+    XdropState xs;
+    xs.clear();
     int cur = 0;
     int s = 0;
     int status = 1;  // running
@@ -232,7 +241,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
     // the heuristic also runs after the score-0 extension (wavefront_extend.c:112-118)
     if (CUT && status == 1 && cbeg == kM) {
       const int lo0 = a.endsfree ? -a.pbf : 0, hi0 = a.endsfree ? a.tbf : 0;
-      cutoff(meta, steps_wait, 0, lo0, lo0, hi0, plen, tlen);
+      cutoff(meta, steps_wait, xs, 0, lo0, lo0, hi0, 0, plen, tlen);
     }
 
     int nulls = 0;  // null steps in a row, every score counted (align_status.num_null_steps)
@@ -543,7 +552,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
           if (vend >= tlen) status = kDevCompleted;
         }
       }
-      if (CUT && status == 1 && !rM.null()) cutoff(meta, steps_wait, cur, lo, rM.first, rM.last, plen, tlen);
+      if (CUT && status == 1 && !rM.null()) cutoff(meta, steps_wait, xs, cur, lo, rM.first, rM.last, s, plen, tlen);
       if (status == 1 && s >= s_cap) status = kDevBadTrace;  // unreachable for valid input
     }
     if (tid == 0) {
@@ -774,7 +783,7 @@ int forward_max_blocks_per_cu(int family, int block, int ncomp, int bits, bool c
 
 hipError_t launch_forward(const ForwardArgs& a, int ncomp, int bits, const LaunchShape& shape,
                           hipStream_t stream) {
-  const void* f = select_forward(shape.family, ncomp, bits, shape.block, a.wfadaptive || a.banded);
+  const void* f = select_forward(shape.family, ncomp, bits, shape.block, a.heur.cut());
   if (!f) return hipErrorInvalidValue;
   if (shape.lds_bytes > 48 * 1024) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shape.lds_bytes);
