@@ -12,7 +12,7 @@
  * Exact path (bit-identical scores and CIGARs to the reference):
  *   distance_metric indel / edit / gap_linear / gap_affine / gap_affine_2p,
  *   heuristic wf_heuristic_none, wf_heuristic_wfadaptive (the default
- *   attributes), wf_heuristic_wfmash, wf_heuristic_xdrop,
+ *   attributes), wf_heuristic_wfmash, wf_heuristic_xdrop / _zdrop,
  *   wf_heuristic_banded_static / _banded_adaptive, alone or combined,
  *   compute_score or compute_alignment;
  *   alignment_end2end in memory modes high / med / low / ultralow;
@@ -22,7 +22,7 @@
  *   reference's scalar extend kernel does.
  * Configurations outside it are rejected with WF_STATUS_UNSUPPORTED and a
  * "[WFA]" message on stderr; they never silently produce different results:
- * the z-drop heuristic, lambda sequences,
+ * z-drop alignments with a CIGAR in memory modes med / low, lambda sequences,
  * free beginnings with a match bonus or with
  * indel/edit distances, indel/edit extension outside memory mode high, and
  * (per pair) a banded alignment whose reference backtrace would take

@@ -47,8 +47,8 @@ class Conf:
     ends_free: tuple = None           # (pattern_begin, pattern_end, text_begin, text_end) or None = end-to-end
     extension: bool = False           # alignment_form.extension (ends-free to both sequence ends + max-trim)
     heuristic: tuple = None           # (strategy, min_wavefront_length, max_distance_threshold,
-                                      #  steps_between_cutoffs[, min_k, max_k[, xdrop]]): 4 wf-adaptive,
-                                      #  0x40 wfmash, 0x10 x-drop, 1 banded static, 2 banded adaptive
+                                      #  steps_between_cutoffs[, min_k, max_k[, xdrop[, zdrop]]]): 4 wf-adaptive,
+                                      #  0x40 wfmash, 0x10 x-drop, 0x20 z-drop, 1 banded static, 2 banded adaptive
                                       #  (bits may combine)
 
     @property
@@ -75,14 +75,14 @@ class _CConf(ctypes.Structure):
         "gap_opening2", "gap_extension2", "memory_mode", "score_only", "max_steps",
         "ends_free", "pattern_begin_free", "pattern_end_free", "text_begin_free", "text_end_free",
         "extension", "heuristic", "min_wavefront_length", "max_distance_threshold", "steps_between_cutoffs",
-        "min_k", "max_k", "xdrop")]
+        "min_k", "max_k", "xdrop", "zdrop")]
 
 
 def _cconf(conf: Conf) -> _CConf:
     p = conf.pen6
     f = conf.form6 or (0, 0, 0, 0, 0, 0)
     h = tuple(conf.heuristic or (0, 0, 0, 0))
-    h = h + (0,) * (7 - len(h))
+    h = h + (0,) * (8 - len(h))
     return _CConf(conf.metric_id, p[0], p[1], p[2], p[3], p[4], p[5],
                   MEMORY_MODES[conf.memory], int(conf.score_only), int(conf.max_steps), *f, *h)
 
@@ -248,7 +248,7 @@ def ref_align(conf: Conf, pairs: Pairs, threads: int = 4, isa=None) -> Result:
     pen = (ctypes.c_int * 6)(*conf.pen6)
     f6 = conf.form6
     form = (ctypes.c_int * 6)(*f6) if f6 is not None else None
-    heur = (ctypes.c_int * 7)(*(tuple(conf.heuristic) + (0,) * (7 - len(conf.heuristic)))) \
+    heur = (ctypes.c_int * 8)(*(tuple(conf.heuristic) + (0,) * (8 - len(conf.heuristic)))) \
         if conf.heuristic is not None else None
     secs = lib.ref_align_batch_ex(conf.metric_id, pen, MEMORY_MODES[conf.memory], int(conf.score_only),
                                   int(conf.max_steps), form, heur, int(threads), n, pairs.buf,

@@ -18,7 +18,7 @@
  * form (may be NULL = end-to-end): {span, pattern_begin_free, pattern_end_free,
  * text_begin_free, text_end_free, extension}.
  * heur (may be NULL = none): {strategy, min_wavefront_length,
- * max_distance_threshold, steps_between_cutoffs, min_k, max_k, xdrop}. */
+ * max_distance_threshold, steps_between_cutoffs, min_k, max_k, xdrop, zdrop}. */
 static void ref_shim_attributes(
     wavefront_aligner_attr_t* const attr,
     const int metric,
@@ -46,6 +46,7 @@ static void ref_shim_attributes(
     attr->heuristic.min_k = heur[4];
     attr->heuristic.max_k = heur[5];
     attr->heuristic.xdrop = heur[6];
+    attr->heuristic.zdrop = heur[7];
   }
   attr->alignment_scope = score_only ? compute_score : compute_alignment;
   attr->memory_mode = (wavefront_memory_t)memory_mode;

@@ -95,9 +95,11 @@ typedef struct {
   int pbf, pef, tbf, tef;
   int end_k, end_off; /* end cell (wf_aligner->alignment_end_pos) */
   /* heuristics (wavefront_heuristic.c:229-290, 467-500) */
-  bool wfadaptive, wfmash, xdrop, banded_static, banded_adaptive;
-  int h_min_len, h_max_thr, h_steps, steps_wait, band_min_k, band_max_k, h_xdrop;
-  int max_sw, max_sw_k;  /* x-drop: best Smith-Waterman score seen, its diagonal (INT_MAX: none) */
+  bool wfadaptive, wfmash, xdrop, zdrop, banded_static, banded_adaptive;
+  int h_min_len, h_max_thr, h_steps, steps_wait, band_min_k, band_max_k, h_xdrop, h_zdrop;
+  /* x-drop / z-drop: best Smith-Waterman score seen, its diagonal (INT_MAX:
+     none), offset and score */
+  int max_sw, max_sw_k, max_sw_off, max_wf_score;
   bool stale_hit;     /* the backtrace met a cell whose choices change when
                          cut-away wavefronts are read (see cell_stale) */
   /* CIGAR, written backwards by the backtrace into ops[begin..end) */
@@ -200,7 +202,9 @@ static void aln_set_penalties(oaln_t* a, const oracle_conf_t* conf) {
   a->wfadaptive = (conf->heuristic & 4) != 0;
   a->wfmash = !a->wfadaptive && (conf->heuristic & 0x40) != 0;
   a->xdrop = (conf->heuristic & 0x10) != 0;
+  a->zdrop = !a->xdrop && (conf->heuristic & 0x20) != 0;
   a->h_xdrop = conf->xdrop;
+  a->h_zdrop = conf->zdrop;
   a->banded_static = (conf->heuristic & 1) != 0;
   a->banded_adaptive = !a->banded_static && (conf->heuristic & 2) != 0;
   a->band_min_k = conf->min_k;
@@ -545,6 +549,36 @@ static void xdrop_cut(oaln_t* a, owf_t* m, int s) {
   a->steps_wait = a->h_steps;
 }
 
+/* wavefront_heuristic_zdrop (wavefront_heuristic.c:371-431): the same
+   Smith-Waterman scores; a wavefront whose best exceeds the best seen
+   replaces it (with its diagonal, offset and score); one whose best falls
+   more than zdrop below it drops the alignment, its end set to the best
+   cell.  Returns whether the alignment dropped (the countdown then stays). */
+static bool zdrop_check(oaln_t* a, owf_t* m, int s) {
+  if (a->steps_wait > 0) return false;
+  const int swg_match = (a->match != 0) ? -a->match : -1;
+  int cmax = INT_MIN, cmax_k = 0, cmax_off = 0, k;
+  for (k = m->lo; k <= m->hi; ++k) {
+    const int32_t off = wf_raw(m, k);
+    if (off < 0) continue;
+    const int sw = (swg_match * ((off - k) + off) - s) / 2;
+    if (cmax < sw) { cmax = sw; cmax_k = k; cmax_off = off; }
+  }
+  if (a->max_sw_k != INT_MAX) {
+    if (cmax > a->max_sw) {
+      a->max_sw = cmax; a->max_wf_score = s; a->max_sw_k = cmax_k; a->max_sw_off = cmax_off;
+    } else if ((int32_t)((uint32_t)a->max_sw - (uint32_t)cmax) > a->h_zdrop) {
+      a->end_k = a->max_sw_k;
+      a->end_off = a->max_sw_off;
+      return true;
+    }
+  } else {
+    a->max_sw = cmax; a->max_wf_score = s; a->max_sw_k = cmax_k; a->max_sw_off = cmax_off;
+  }
+  a->steps_wait = a->h_steps;
+  return false;
+}
+
 /* wavefront_heuristic_banded_adaptive (wavefront_heuristic.c:477-513): a
    wavefront wider than the band keeps a band-wide window placed by the
    distances to the end sampled at its ends and quarters. */
@@ -570,7 +604,7 @@ static void banded_adaptive(oaln_t* a, owf_t* m) {
 }
 
 static bool has_cutoff(const oaln_t* a) {
-  return a->wfadaptive || a->wfmash || a->xdrop || a->banded_static || a->banded_adaptive;
+  return a->wfadaptive || a->wfmash || a->xdrop || a->zdrop || a->banded_static || a->banded_adaptive;
 }
 
 /* wavefront_heuristic_cufoff (wavefront_heuristic.c:506-561) after the
@@ -579,24 +613,25 @@ static bool has_cutoff(const oaln_t* a) {
    I1/D1/I2/D2 narrowed to M's range.  The cut narrows [lo,hi] only: the cut
    cells stay readable (rlo/rhi) except for indel / edit, whose compute step
    NULLs the two cells beyond each end (wavefront_compute_edit.c:355-358). */
-static void cutoff(oaln_t* a, int s) {
+static bool cutoff(oaln_t* a, int s) {
   owf_t* m = wf_at(a, C_M, s);
-  if (!m || m->lo > m->hi) return;
+  if (!m || m->lo > m->hi) return false;
   --a->steps_wait;
   const int lo = m->lo, hi = m->hi;
   if (a->wfadaptive) wfadaptive_cut(a, m, false);
   else if (a->wfmash) wfadaptive_cut(a, m, true);
   if (a->xdrop) xdrop_cut(a, m, s);
+  else if (a->zdrop && zdrop_check(a, m, s)) return true;
   if (a->banded_static) {
     if (m->lo < a->band_min_k) m->lo = a->band_min_k;
     if (m->hi > a->band_max_k) m->hi = a->band_max_k;
   } else if (a->banded_adaptive) {
     banded_adaptive(a, m);
   }
-  if (m->lo == lo && m->hi == hi) return;
+  if (m->lo == lo && m->hi == hi) return false;
   if (m->lo > m->hi) m->null = true;
   if (a->metric <= M_EDIT) { m->rlo = m->lo; m->rhi = m->hi; }
-  if (a->metric <= M_LINEAR) return;
+  if (a->metric <= M_LINEAR) return false;
   int c;
   for (c = C_I1; c < NCOMP; ++c) {  /* wf_heuristic_equate (:163-171) */
     owf_t* w = wf_at(a, c, s);
@@ -605,6 +640,15 @@ static void cutoff(oaln_t* a, int s) {
     if (m->hi < w->hi) w->hi = m->hi;
     if (w->lo > w->hi) w->null = true;
   }
+  return false;
+}
+
+/* A z-drop ends the alignment as unreachable at score s, its end cell the
+   best one seen (wavefront_extend.c:112-118, 201-207, 284-290). */
+static int dropped(oaln_t* a, int s) {
+  a->status = ST_END_UNREACHABLE;
+  a->end_score = s;
+  return 1;
 }
 
 /* wavefront_termination_endsfree, wavefront_termination.c:115-162 */
@@ -645,7 +689,7 @@ static int extend_endsfree(oaln_t* a, int s) {
     m->v[k - m->alo] = ext;
     if (termination_endsfree(a, s, k, ext)) return 1;
   }
-  if (has_cutoff(a)) cutoff(a, s);
+  if (has_cutoff(a) && cutoff(a, s)) return dropped(a, s);
   return 0;
 }
 
@@ -671,7 +715,7 @@ static int extend(oaln_t* a, int s, int* max_ak) {
     a->end_off = a->tlen;
     return 1;
   }
-  if (has_cutoff(a)) cutoff(a, s);
+  if (has_cutoff(a) && cutoff(a, s)) return dropped(a, s);
   if (max_ak) *max_ak = best;
   return 0;
 }
@@ -688,6 +732,10 @@ static void aln_init(oaln_t* a, int comp_begin, int comp_end) {
   a->null_steps = 0;  /* wavefront_aligner.c:411 */
   a->max_sw = 0;      /* wavefront_heuristic_clear (wavefront_heuristic.c:114-121) */
   a->max_sw_k = INT_MAX;
+  a->max_sw_off = ONULL;
+  a->max_wf_score = 0;
+  a->end_k = INT_MAX;  /* alignment_end_pos cleared (wavefront_aligner.c:411-414) */
+  a->end_off = ONULL;
   a->stale_hit = false;
   a->steps_wait = a->h_steps;  /* wavefront_heuristic_clear (wavefront_heuristic.c:114-121) */
   owf_t* w[NCOMP];
@@ -913,20 +961,22 @@ static int unialign(oaln_t* a, bool score_only) {
   while (1) {
     if (extend(a, s, NULL)) {
       if (a->status == ST_END_UNREACHABLE) {
-        /* wavefront_unialign_terminate (wavefront_unialign.c:149-236): no
-           end cell was recorded (alignment_end_pos keeps k = INT_MAX and
-           offset = NULL, wavefront_aligner.c:411-414), so no backtrace; a
-           score-only run takes the classic score at that NULL end, with the
-           wrap-around of 32-bit int arithmetic */
+        /* wavefront_unialign_terminate (wavefront_unialign.c:149-236) at the
+           end cell a z-drop recorded, or at none (alignment_end_pos keeps
+           k = INT_MAX and offset = NULL, wavefront_aligner.c:411-414): a
+           score-only run takes the classic score at that end, with the
+           wrap-around of 32-bit int arithmetic; a backtrace runs from it
+           at the current score when it is not NULL */
         if (score_only) {
-          const int32_t off = ONULL;
-          const int32_t v = (int32_t)((uint32_t)off - (uint32_t)INT_MAX), h = off;
+          const int32_t v = (int32_t)((uint32_t)a->end_off - (uint32_t)a->end_k), h = a->end_off;
           if (a->metric <= M_EDIT || a->match == 0) {
             a->cigar_score = classic_score(a, 0, 0, s);
           } else {
             const uint32_t sum = (uint32_t)(-a->match) * ((uint32_t)v + (uint32_t)h) - (uint32_t)s;
             a->cigar_score = (int32_t)sum / 2;
           }
+        } else if (a->end_off != ONULL) {
+          backtrace(a, s, a->end_k, a->end_off);
         }
         return ST_ALG_PARTIAL;
       }
@@ -1204,7 +1254,7 @@ static int bialign(const oracle_conf_t* conf, const char* P, int plen, const cha
   aln_set_penalties(&b.rev, conf); b.rev.modular = true;
   aln_set_penalties(&b.base, conf); b.base.modular = false;
   /* the base aligner runs without heuristic (wavefront_bialigner.c:64-67) */
-  b.base.wfadaptive = b.base.wfmash = b.base.xdrop = false;
+  b.base.wfadaptive = b.base.wfmash = b.base.xdrop = b.base.zdrop = false;
   b.base.banded_static = b.base.banded_adaptive = false;
   b.P = P; b.T = T; b.plen = plen; b.tlen = tlen;
   b.Prev = reversed(P, plen); b.Trev = reversed(T, tlen);
@@ -1311,7 +1361,11 @@ int oracle_align(const oracle_conf_t* conf, const char* P, int plen, const char*
   if (stale) *stale = 0;
   if (ops_len) *ops_len = 0;
   const bool endsfree = conf->ends_free || conf->extension;
-  if ((conf->heuristic & 0x10) && conf->metric <= M_EDIT) return ORACLE_UNSUPPORTED;
+  if ((conf->heuristic & 0x30) && conf->metric <= M_EDIT) return ORACLE_UNSUPPORTED;
+  /* z-drop alignments in memory modes med / low trace the piggybacked CIGAR
+     of another cell (wavefront_unialign.c:177-189); not restated */
+  if ((conf->heuristic & 0x30) == 0x20 && !conf->score_only && (conf->memory_mode == 1 || conf->memory_mode == 2))
+    return ORACLE_UNSUPPORTED;
   if (conf->memory_mode == 3) {
     if (endsfree) return ORACLE_UNSUPPORTED;
     return bialign(conf, P, plen, T, tlen, score, ops, ops_len, dsteps, end_score);
@@ -1347,7 +1401,7 @@ int oracle_align(const oracle_conf_t* conf, const char* P, int plen, const char*
          backtrace leaves the raw score; max-trim replaces it.  A dropped
          alignment has no backtrace and keeps the cleared CIGAR's score
          (INT32_MIN, cigar.c:79-90) unless max-trim replaces it. */
-      int keep = n, best = dropped ? INT32_MIN : a.end_score;
+      int keep = n, best = (dropped && a.end_off == ONULL) ? INT32_MIN : a.end_score;
       const bool trimmed = maxtrim(conf, a.ops + a.ops_begin, n, &keep, &best);
       *score = best;
       n = keep;

@@ -43,13 +43,16 @@ typedef struct {
   int min_wavefront_length, max_distance_threshold, steps_between_cutoffs;
   int min_k, max_k;    /* band of the banded strategies */
   int xdrop;           /* x-drop threshold */
+  int zdrop;           /* z-drop threshold (0x20) */
 } oracle_conf_t;
 
 /* Status returned for a configuration this restatement does not cover:
    ends-free with a free beginning and match < 0 (wavefront_compute.c:124-262),
    indel / edit with a free beginning (see oracle_align), or what the
    reference rejects with exit(1): ends-free / extension with BiWFA
-   (wavefront_align.c:66-80), x-drop with indel / edit (:82-89). */
+   (wavefront_align.c:66-80), x-drop / z-drop with indel / edit (:82-89);
+   and a z-drop alignment in memory mode med / low, which traces a
+   piggybacked CIGAR (wavefront_unialign.c:177-189). */
 #define ORACLE_UNSUPPORTED (-400)
 
 /*

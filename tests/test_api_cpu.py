@@ -41,9 +41,8 @@ def test_default_attributes_match_reference():
     dict(metric="affine", penalties=(-1, 4, 6, 2), ends_free=(2, 0, 0, 0)),  # free beginning, match < 0
     dict(metric="affine", memory="ultralow", ends_free=(0, 0, 0, 0)),  # ends-free span in BiWFA
     dict(metric="edit", memory="med", extension=True),                 # indel/edit extension outside high
-    dict(metric="affine", heuristic=(0x20, 0, 0, 1)),                  # z-drop
-    dict(metric="affine", heuristic=(0x04 | 0x20, 10, 50, 1)),         # wf-adaptive + z-drop
-    dict(metric="affine", heuristic=(0x01 | 0x20, 0, 0, 1, -5, 5)),    # banded static + z-drop
+    dict(metric="affine", memory="med", heuristic=(0x20, 0, 0, 1, 0, 0, 0, 30)),   # z-drop, CIGAR, med
+    dict(metric="affine", memory="low", heuristic=(0x24, 10, 50, 1, 0, 0, 0, 30)),  # wf-adaptive + z-drop, low
 ])
 def test_unsupported_configurations_are_refused(kw):
     al = wfa_amd.Aligner(**kw)

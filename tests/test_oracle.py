@@ -454,6 +454,10 @@ DROPS = [
     (0x10, 0, 0, 5, 0, 0, 100),
     (0x14, 10, 50, 1, 0, 0, 40),         # wf-adaptive then x-drop
     (0x11, 0, 0, 1, -10, 10, 40),        # x-drop then banded static
+    (0x20, 0, 0, 1, 0, 0, 0, 30),        # z-drop
+    (0x20, 0, 0, 3, 0, 0, 0, 100),
+    (0x24, 10, 50, 1, 0, 0, 0, 40),      # wf-adaptive then z-drop
+    (0x21, 0, 0, 1, -10, 10, 0, 40),     # z-drop then banded static
 ]
 
 
@@ -463,15 +467,16 @@ DROPS = [
                                         ("edit", (0, 1, 1))])
 def test_oracle_wfmash_and_xdrop_match_reference_build(metric, pen):
     """wfmash (weighted distances in single-precision float,
-    wavefront_heuristic.c:134-145, 190-208) and x-drop on Smith-Waterman
-    scores (:294-370), alone and combined, in memory modes high and ultralow.
-    x-drop with indel / edit is a configuration error in the reference
-    (exit(1)), so edit runs wfmash only; an edit wavefront a band empties
-    crashes the reference's high-memory form, so those pairs are left out
-    there as in the banded test."""
+    wavefront_heuristic.c:134-145, 190-208), x-drop and z-drop on
+    Smith-Waterman scores (:294-431), alone and combined, in memory modes
+    high and ultralow; a z-dropped alignment is traced from the best cell at
+    the score it dropped at.  Drops with indel / edit are a configuration
+    error in the reference (exit(1)), so edit runs wfmash only; an edit
+    wavefront a band empties crashes the reference's high-memory form, so
+    those pairs are left out there as in the banded test."""
     pats, texts = _banded_pairs()
     for h in DROPS:
-        if metric == "edit" and h[0] & 0x10:
+        if metric == "edit" and h[0] & 0x30:
             continue
         sel = list(range(len(pats)))
         if metric == "edit":

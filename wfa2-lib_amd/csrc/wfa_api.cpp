@@ -326,8 +326,12 @@ int check_supported(wavefront_aligner_t* wf) {
   // The strategies take effect as wavefront_aligner_init_heuristic selects
   // them (wavefront_aligner.c:189-228): wf-adaptive before wfmash, x-drop
   // before z-drop, banded static before banded adaptive.
-  else if ((wf->heuristic.strategy & wf_heuristic_zdrop) && !(wf->heuristic.strategy & wf_heuristic_xdrop))
-    why = "the z-drop heuristic";
+  // A z-dropped alignment in memory mode med / low traces the piggybacked
+  // CIGAR of another cell (wavefront_unialign.c:177-189).
+  else if ((wf->heuristic.strategy & wf_heuristic_zdrop) && !(wf->heuristic.strategy & wf_heuristic_xdrop) &&
+           wf->alignment_scope == compute_alignment &&
+           (wf->memory_mode == wavefront_memory_med || wf->memory_mode == wavefront_memory_low))
+    why = "the z-drop heuristic with alignment in memory mode med or low";
   else if (make_penalties(wf->penalties).nslots > 64)
     why = "penalties whose score scope needs more than 64 wavefront slots";
   if (!why) return WF_STATUS_OK;
@@ -381,12 +385,14 @@ wfa::HeurArgs heuristic_of(const wavefront_aligner_t* wf) {
   const int st = (int)wf->heuristic.strategy;
   h.wfadaptive = (st & wf_heuristic_wfadaptive) ? 1 : (st & wf_heuristic_wfmash) ? 2 : 0;
   h.xdrop = (st & wf_heuristic_xdrop) ? 1 : 0;
+  h.zdrop = (!h.xdrop && (st & wf_heuristic_zdrop)) ? 1 : 0;
   h.banded = (st & wf_heuristic_banded_static) ? 1 : (st & wf_heuristic_banded_adaptive) ? 2 : 0;
   if (!h.cut()) return h;
   h.min_len = wf->heuristic.min_wavefront_length;
   h.thr = wf->heuristic.max_distance_threshold;
   h.steps = wf->heuristic.steps_between_cutoffs;
   h.xdrop_thr = wf->heuristic.xdrop;
+  h.zdrop_thr = wf->heuristic.zdrop;
   h.swg_match = wf->penalties.match != 0 ? -wf->penalties.match : -1;  // wavefront_heuristic.c:303-304
   h.band_lo = wf->heuristic.min_k;
   h.band_hi = wf->heuristic.max_k;
@@ -644,6 +650,7 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
   fa.tef = js.ef.tef;
   fa.out_end_k = js.end_k;
   fa.out_end_off = js.end_off;
+  fa.out_end_rec = js.end_rec;
   fa.heur = js.heur;
   fa.pen = pen;
   if (!hip_ok(wfa::launch_reset_counters(c, 0, st), "reset")) return WF_STATUS_OOM;
@@ -678,6 +685,7 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
     ba.endsfree = js.ef.on;
     ba.end_k = js.end_k;
     ba.end_off = js.end_off;
+    ba.end_rec = js.end_rec;
     ba.pen = pen;
     if (!hip_ok(wfa::launch_backtrace(ba, st), "backtrace launch")) return WF_STATUS_OOM;
   }
@@ -887,7 +895,7 @@ bool download_results(Engine* e, wfa_device_batch_t* b) {
             (hip_ok(hipMemcpyAsync(b->status.data(), b->d_status.p, n * 4, hipMemcpyDeviceToHost, st), "download") &&
              hip_ok(hipMemcpyAsync(b->score.data(), b->d_score.p, n * 4, hipMemcpyDeviceToHost, st), "download") &&
              hip_ok(hipMemcpyAsync(b->ops_begin.data(), b->d_ops_begin.p, n * 4, hipMemcpyDeviceToHost, st), "download"));
-  if (ok && n > 0 && b->ef.on && !b->bi_mode)
+  if (ok && n > 0 && (b->ef.on || b->heur.zdrop) && !b->bi_mode)
     ok = hip_ok(hipMemcpyAsync(b->end_k.data(), b->d_end_k.p, n * 4, hipMemcpyDeviceToHost, st), "download") &&
          hip_ok(hipMemcpyAsync(b->end_off.data(), b->d_end_off.p, n * 4, hipMemcpyDeviceToHost, st), "download");
   ok = ok && hip_ok(hipStreamSynchronize(st), "download sync");
@@ -919,6 +927,7 @@ wfa_host::JobSet wfa_device_batch_t::jobs() const {
   js.ops_begin = d_ops_begin.as<int>();
   js.end_k = d_end_k.as<int>();
   js.end_off = d_end_off.as<int>();
+  js.end_rec = d_end_rec.as<int>();
   js.ef = ef;
   js.heur = heur;
   return js;
@@ -928,7 +937,8 @@ wfa_device_batch_t::~wfa_device_batch_t() {
   h_words.release();
   for (auto& g : plan) g.release();
   for (wfa_host::DevBuf* x : {&words, &d_plen, &d_tlen, &d_patw, &d_txtw, &d_status, &d_score, &d_last, &d_dsteps,
-                              &d_ops_off, &d_ops_begin, &d_ops, &d_end_k, &d_end_off, &rev_words, &d_rev_ppos,
+                              &d_ops_off, &d_ops_begin, &d_ops, &d_end_k, &d_end_off, &d_end_rec, &rev_words,
+                              &d_rev_ppos,
                               &d_rev_tpos})
     x->release();
 }
@@ -1046,7 +1056,8 @@ bool wfa_device_batch_t::load(Engine* e, const wfa_pair_t* pairs, int count) {
   bool ok = words.ensure(std::max<size_t>(nw, 1) * 8) && d_plen.ensure(nn * 4) && d_tlen.ensure(nn * 4) &&
             d_patw.ensure(nn * 8) && d_txtw.ensure(nn * 8) && d_status.ensure(nn * 4) && d_score.ensure(nn * 4) &&
             d_last.ensure(nn * 4) && d_dsteps.ensure(nn * 8) && d_ops_off.ensure(nn * 8) &&
-            d_ops_begin.ensure(nn * 4) && d_end_k.ensure(nn * 4) && d_end_off.ensure(nn * 4);
+            d_ops_begin.ensure(nn * 4) && d_end_k.ensure(nn * 4) && d_end_off.ensure(nn * 4) &&
+            d_end_rec.ensure(nn * 4);
   if (!ok) {
     fprintf(stderr, "[WFA] device allocation failed for a batch of %d pairs\n", n);
     return false;
@@ -1153,8 +1164,9 @@ extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_ba
                 "is not supported\n", i);
       b->status[i] = WF_STATUS_UNSUPPORTED;
     }
-    // an end made unreachable by the cut-off is a partial result, not a failure
-    if (b->status[i] == wfa::kDevUnreachable) continue;
+    // an end made unreachable (or z-dropped) by the cut-off is a partial
+    // result, not a failure
+    if (b->status[i] == wfa::kDevUnreachable || b->status[i] == wfa::kDevZdropped) continue;
     if (b->status[i] != WF_STATUS_ALG_COMPLETED && first_fail == WF_STATUS_ALG_COMPLETED) first_fail = b->status[i];
   }
   b->first_fail = first_fail;
@@ -1186,8 +1198,8 @@ extern "C" int wfa_device_batch_download(wavefront_aligner_t* const wf, wfa_devi
     if (!hip_ok(hipStreamSynchronize(e->stream), "cigar download")) return WF_STATUS_OOM;
   }
   const bool extension = wf->alignment_form.extension && b->records && !b->bi_mode;
-  if (extension && !want_ops) {
-    fprintf(stderr, "[WFA] the extension form needs the CIGAR arena (it trims the CIGAR)\n");
+  if ((extension || (b->heur.zdrop && b->records && !b->bi_mode)) && !want_ops) {
+    fprintf(stderr, "[WFA] the extension form and z-drop need the CIGAR arena (they trim the CIGAR)\n");
     return WF_STATUS_OOM;
   }
   int first_fail = WF_STATUS_ALG_COMPLETED;
@@ -1235,17 +1247,29 @@ extern "C" int wfa_device_batch_download(wavefront_aligner_t* const wf, wfa_devi
       }
     } else if (st == WF_STATUS_MAX_STEPS_REACHED && !b->bi_mode) {
       r.score = -wf->system.max_alignment_steps;  // wavefront_unialign.c:102-107
-    } else if (st == wfa::kDevUnreachable) {
-      // WF_STATUS_END_UNREACHABLE (wavefront_unialign.c:147-236): no end cell
-      // was recorded (alignment_end_pos keeps k = INT_MAX and offset = NULL,
-      // wavefront_aligner.c:411-414), so there is no backtrace.  Score-only
-      // takes the classic score at that NULL end with the wrap-around of
-      // 32-bit arithmetic; with a CIGAR, max-trim of the empty CIGAR leaves
-      // the cleared score and end (cigar.c:79-90).
+    } else if (b->bi_mode) {
+      // what the BiWFA recursion left in cigar->score, and the CIGAR it had
+      // appended before the failure
+      r.score = b->bi_score[i];
+      if (b->records) {
+        r.cigar_offset = b->ops_off[i] + b->ops_begin[i];
+        r.cigar_length = b->plen[i] + b->tlen[i] - b->ops_begin[i];
+      }
+    } else if (st == wfa::kDevUnreachable || st == wfa::kDevZdropped) {
+      // WF_STATUS_END_UNREACHABLE (wavefront_unialign.c:147-236) at the end
+      // cell a z-drop recorded, or at none (alignment_end_pos keeps
+      // k = INT_MAX and offset = NULL, wavefront_aligner.c:411-414).
+      // Score-only takes the classic score at that end with the wrap-around
+      // of 32-bit arithmetic.  With a CIGAR, a z-drop's backtrace (traced on
+      // the device) is max-trimmed, keeping the backtrace's raw score when
+      // max-trim does not examine it; without an end cell, max-trim of the
+      // empty CIGAR leaves the cleared score and end (cigar.c:79-90).
+      const bool zdrop = st == wfa::kDevZdropped;
       r.status = st = WF_STATUS_ALG_PARTIAL;
+      const uint32_t off = zdrop ? (uint32_t)b->end_off[i] : (uint32_t)wfa::kOffsetNull;
+      const uint32_t kk = zdrop ? (uint32_t)b->end_k[i] : (uint32_t)INT_MAX;
       if (!b->records) {
-        const uint32_t off = (uint32_t)wfa::kOffsetNull;
-        r.end_v = (int)(off - (uint32_t)INT_MAX);
+        r.end_v = (int)(off - kk);
         r.end_h = (int)off;
         const int swg_match = -wf->penalties.match;
         if (wf->penalties.distance_metric <= edit || swg_match == 0) {
@@ -1254,9 +1278,21 @@ extern "C" int wfa_device_batch_download(wavefront_aligner_t* const wf, wfa_devi
           const uint32_t num = (uint32_t)swg_match * ((uint32_t)r.end_v + (uint32_t)r.end_h) - (uint32_t)b->score[i];
           r.score = (int)num / 2;
         }
-      } else {
+      } else if (!zdrop) {
         r.score = INT32_MIN;
         r.end_v = r.end_h = -1;
+      } else {
+        r.cigar_offset = b->ops_off[i] + b->ops_begin[i];
+        r.cigar_length = b->plen[i] + b->tlen[i] - b->ops_begin[i];
+        const Trim t = maxtrim(wf->penalties, cigar_arena + r.cigar_offset, r.cigar_length);
+        r.score = b->score[i];  // the backtrace's raw score (wavefront_backtrace.c:317, :527)
+        r.end_v = r.end_h = -1;
+        if (t.examined) {
+          r.cigar_length = t.keep;
+          r.score = t.score;
+          r.end_v = t.end_v;
+          r.end_h = t.end_h;
+        }
       }
     } else {
       r.score = INT32_MIN;

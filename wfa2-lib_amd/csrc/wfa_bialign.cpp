@@ -38,18 +38,47 @@ struct Leaf {
   int base;        // index of the base-case job, or -1
 };
 
-struct PairState {
-  int status = WF_STATUS_ALG_COMPLETED;
-  int fail_pb = INT_MAX;   // leftmost failing sub-problem (depth-first order)
-  int score = INT32_MIN;   // cigar->score
-  std::vector<Leaf> leaves;
+struct BaseScore {
+  int pb, tb, score;
 };
 
-void fail_pair(PairState& ps, int status, int pb) {
-  if (pb < ps.fail_pb) {
+struct PairState {
+  int status = WF_STATUS_ALG_COMPLETED;
+  int fail_pb = INT_MAX, fail_tb = INT_MAX;  // leftmost failing sub-problem (depth-first order)
+  bool fail_base = false;  // ... and whether it is a base alignment, with its score
+  int fail_score = INT32_MIN;
+  int score = INT32_MIN;   // cigar->score
+  std::vector<Leaf> leaves;
+  std::vector<BaseScore> bases;  // completed base alignments
+};
+
+// The reference's recursion stops at the first failing sub-problem in
+// depth-first order, which is the leftmost one.
+void fail_pair(PairState& ps, int status, int pb, int tb, bool base = false, int score = INT32_MIN) {
+  if (pb < ps.fail_pb || (pb == ps.fail_pb && tb < ps.fail_tb)) {
     ps.fail_pb = pb;
+    ps.fail_tb = tb;
     ps.status = status;
+    ps.fail_base = base;
+    ps.fail_score = score;
   }
+}
+
+// cigar->score of a failed pair: every base alignment sets it
+// (wavefront_bialign.c:181-182), so it is the failing base's own score, or
+// that of the last base alignment run before the failing search, if any.
+int failed_score(const PairState& ps) {
+  if (ps.fail_base) return ps.fail_score;
+  int best = INT32_MIN, bpb = INT_MIN, btb = INT_MIN;
+  for (const BaseScore& b : ps.bases) {
+    const bool before = b.pb < ps.fail_pb || (b.pb == ps.fail_pb && b.tb < ps.fail_tb);
+    if (before && (b.pb > bpb || (b.pb == bpb && b.tb > btb))) {
+      bpb = b.pb;
+      btb = b.tb;
+      best = b.score;
+    }
+  }
+  return best;
 }
 
 bool ensure_reverse(Engine* e, wfa_device_batch_t* b) {
@@ -206,7 +235,11 @@ int bialign_run(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, RunTo
       std::vector<Node> bp_nodes;
       for (const Node& nd : nodes) {
         PairState& st = ps[nd.prob.pair];
-        if (st.status != WF_STATUS_ALG_COMPLETED) continue;
+        // After a failure the reference's depth-first recursion has still
+        // run everything to its left (whose base alignments set the score).
+        if (st.status != WF_STATUS_ALG_COMPLETED &&
+            !(nd.prob.pb < st.fail_pb || (nd.prob.pb == st.fail_pb && nd.prob.tb < st.fail_tb)))
+          continue;
         const wfa::BiProblem& p = nd.prob;
         const int pl = p.pe - p.pb, tl = p.te - p.tb;
         if (tl == 0) {
@@ -242,16 +275,16 @@ int bialign_run(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, RunTo
           // wavefront_bialign_find_breakpoint_exception (:1084-1111)
           const int reached = o.fwd_end_score >= 0 ? o.fwd_end_score : o.rev_end_score;
           if (reached <= kRecoveryMinScore) base_nodes.push_back(nd);
-          else fail_pair(st, WF_STATUS_UNATTAINABLE, p.pb);
+          else fail_pair(st, WF_STATUS_UNATTAINABLE, p.pb, p.tb);
         } else if (o.status == wfa::kDevMaxSteps) {
-          fail_pair(st, WF_STATUS_MAX_STEPS_REACHED, p.pb);
+          fail_pair(st, WF_STATUS_MAX_STEPS_REACHED, p.pb, p.tb);
         } else if (o.status == wfa::kDevUnreachable) {
           // a side the cut-off left unreachable (wavefront_bialign.c:1084-1111, 1286-1294)
-          fail_pair(st, WF_STATUS_UNATTAINABLE, p.pb);
+          fail_pair(st, WF_STATUS_UNATTAINABLE, p.pb, p.tb);
         } else {
           if (o.status != wfa::kDevMaxSteps)
             fprintf(stderr, "[WFA] BiWFA breakpoint search failed (status %d) for pair %d\n", o.status, p.pair);
-          fail_pair(st, WF_STATUS_UNATTAINABLE, p.pb);
+          fail_pair(st, WF_STATUS_UNATTAINABLE, p.pb, p.tb);
         }
       }
       nodes.swap(next);
@@ -332,24 +365,32 @@ int bialign_run(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, RunTo
         const wfa::BiProblem& p = nd.prob;
         PairState& s = ps[p.pair];
         if (bstatus[j] != WF_STATUS_ALG_COMPLETED) {
-          // wavefront_bialign_base (:159-190): anything but completed is unattainable
-          fail_pair(s, WF_STATUS_UNATTAINABLE, p.pb);
+          // wavefront_bialign_base (:159-190): anything but completed is
+          // unattainable; the base's own score stays (the step limit's)
+          fail_pair(s, WF_STATUS_UNATTAINABLE, p.pb, p.tb, true,
+                    bstatus[j] == WF_STATUS_MAX_STEPS_REACHED ? -wf->system.max_alignment_steps : INT32_MIN);
           continue;
         }
         s.leaves.push_back(Leaf{p.pb, p.tb, 0, bpl[j] + btl[j] - bbegin[j], j});
+        s.bases.push_back(BaseScore{p.pb, p.tb, classic_score(wf->penalties, bpl[j], btl[j], bscore[j])});
         // the base call sets cigar->score; at level 0 nothing overrides it
         if (nd.level == 0) s.score = classic_score(wf->penalties, bpl[j], btl[j], bscore[j]);
       }
     }
-    // Assemble every completed pair right-aligned in its slot of the batch's operations.
+    // Assemble every pair right-aligned in its slot of the batch's operations:
+    // all its pieces, or for a failed pair the ones the reference's recursion
+    // appended before the failure (the cigar it leaves, wavefront_bialign.c:
+    // 159-190, 1145-1222).
     std::vector<wfa::OpsPiece> pieces;
     std::vector<int> begin(n, 0);
     for (int i = 0; i < n; ++i) {
       PairState& s = ps[i];
       const int cap = b->plen[i] + b->tlen[i];
       if (s.status != WF_STATUS_ALG_COMPLETED) {
-        begin[i] = cap;
-        continue;
+        std::vector<Leaf> kept;
+        for (const Leaf& l : s.leaves)
+          if (l.pb < s.fail_pb || (l.pb == s.fail_pb && l.tb < s.fail_tb)) kept.push_back(l);
+        s.leaves.swap(kept);
       }
       std::sort(s.leaves.begin(), s.leaves.end(),
                 [](const Leaf& x, const Leaf& y) { return x.pb != y.pb ? x.pb < y.pb : x.tb < y.tb; });
@@ -400,7 +441,7 @@ int bialign_run(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, RunTo
   b->bi_score.assign(n, INT32_MIN);
   for (int i = 0; i < n; ++i) {
     status[i] = ps[i].status;
-    b->bi_score[i] = ps[i].score;
+    b->bi_score[i] = (records && status[i] != WF_STATUS_ALG_COMPLETED) ? failed_score(ps[i]) : ps[i].score;
     if (status[i] != WF_STATUS_ALG_COMPLETED) tot->fail++;
   }
   if (n > 0 && !hip_ok(hipMemcpy(b->d_status.p, status.data(), (size_t)n * 4, hipMemcpyHostToDevice), "results"))

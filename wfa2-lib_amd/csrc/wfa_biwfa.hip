@@ -21,7 +21,7 @@ struct Side {
   MetaRegs<NC, CUT> meta;
   int steps_wait;          // cut-off countdown (wavefront_heuristic.c:506-521)
   int nulls;               // null steps in a row (align_status.num_null_steps)
-  XdropState xs;           // x-drop's best score (wavefront_heuristic.c:114-121)
+  DropState xs;           // x-drop's best score (wavefront_heuristic.c:114-121)
   bool dead;               // indel / edit: the cut-off emptied the wavefront
   int cur;                 // ring slot of the latest multiple-of-g score
   int* ring;
@@ -77,12 +77,13 @@ __device__ __forceinline__ WfRef<NC> wf_ref(const Side<NC, CUT>& sd, int s, int 
 // Heuristic cut-off of the side's M wavefront of score s in ring slot
 // `slot`, trimmed to [lo, hi] with array origin alo
 // (wavefront_heuristic_cufoff, wavefront_heuristic.c:506-561): wf-adaptive
-// or wfmash, x-drop, then the band (static clamp or adaptive window), one
-// countdown for all; when M changed, I1/D1/I2/D2 are narrowed to M's range;
+// or wfmash, x-drop or z-drop, then the band (static clamp or adaptive
+// window), one countdown for all; when M changed, I1/D1/I2/D2 are narrowed
+// to M's range; a z-drop returns true (the side's end is then unreachable);
 // indel / edit then read M only inside the cut range (their compute step
 // NULLs the two cells beyond each end, wavefront_compute_edit.c:355-358).
 template <int NC, int BLOCK, bool CUT>
-__device__ void side_cutoff(Side<NC, CUT>& sd, const HeurArgs& heur, BlockRed<BLOCK>& red, int slot, int alo, int lo,
+__device__ bool side_cutoff(Side<NC, CUT>& sd, const HeurArgs& heur, BlockRed<BLOCK>& red, int slot, int alo, int lo,
                             int hi, int s, int plen, int tlen, int wcap, int metric) {
   if constexpr (CUT) {
     --sd.steps_wait;
@@ -95,6 +96,9 @@ __device__ void side_cutoff(Side<NC, CUT>& sd, const HeurArgs& heur, BlockRed<BL
     if (heur.xdrop && sd.steps_wait <= 0) {
       xdrop_cut<BLOCK>(m, s, heur.swg_match, heur.xdrop_thr, red, sd.xs, &clo, &chi);
       sd.steps_wait = heur.steps;
+    } else if (heur.zdrop && sd.steps_wait <= 0) {
+      if (zdrop_check<BLOCK>(m, clo, chi, s, heur.swg_match, heur.zdrop_thr, red, sd.xs)) return true;
+      sd.steps_wait = heur.steps;
     }
     if (heur.banded == 1) {
       clo = max(clo, heur.band_lo);
@@ -103,7 +107,7 @@ __device__ void side_cutoff(Side<NC, CUT>& sd, const HeurArgs& heur, BlockRed<BL
       banded_adaptive_cut(m, heur.band_lo, heur.band_hi, plen, tlen, sd.steps_wait, heur.steps, &clo, &chi);
     }
     // unchanged M: nothing else is narrowed either (wavefront_heuristic.c:540-541)
-    if (clo == lo && chi == hi) return;
+    if (clo == lo && chi == hi) return false;
     if ((threadIdx.x & 63) == slot) {
       unsigned fl = sd.meta.flags;
       sd.meta.cut.lo[kM] = clo;
@@ -121,6 +125,7 @@ __device__ void side_cutoff(Side<NC, CUT>& sd, const HeurArgs& heur, BlockRed<BL
       sd.meta.flags = fl;
     }
   }
+  return false;
 }
 
 // Compute score s of one side (no backtrace records) fused with its
@@ -347,7 +352,8 @@ __device__ int side_step(Side<NC, CUT>& sd, int s, int plen, int tlen, const Pen
       return 1;
     }
   }
-  if (!rM.null()) side_cutoff<NC, BLOCK>(sd, heur, red, cur, lo, rM.first, rM.last, s, plen, tlen, wcap, pen.metric);
+  if (!rM.null() && side_cutoff<NC, BLOCK>(sd, heur, red, cur, lo, rM.first, rM.last, s, plen, tlen, wcap, pen.metric))
+    return 2;
   return 0;
 }
 
@@ -480,6 +486,7 @@ __device__ int side_extend0(Side<NC, CUT>& sd, int plen, int tlen, int wcap, con
     sd.end_score = 0;
     return 1;
   }
+  // (the first z-drop test only records the best score: no drop at score 0)
   side_cutoff<NC, BLOCK>(sd, heur, red, 0, 0, 0, 0, 0, plen, tlen, wcap, pen.metric);
   return 0;
 }

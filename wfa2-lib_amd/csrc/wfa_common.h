@@ -131,20 +131,81 @@ __device__ __forceinline__ unsigned long long choice_key(RingPtr ring, const Src
 // Record byte of cell k for a step with an input the cut-off emptied: the
 // reference's backtrace probes that input's readable cells too
 // (wavefront_backtrace.c:64-219), so the byte holds the choices over the
-// probed sources.  The match run it then writes before the cell is the
+// probed sources (b*).  The match run it then writes before the cell is the
 // extended offset m minus the winning probed offset; a winning offset past m
 // (possible only through such a cell) takes operations back instead, which
-// the replay does not do: kRecStale marks the cell.  A NULL cell (m) is
-// compared by its computed best (best) instead.
+// the replay does not do: kRecStale marks the cell.  A NULL cell (m < 0) is
+// compared by its computed best (over the compute sources c*) instead.
 template <int NC, typename RingPtr>
-__device__ __forceinline__ uint32_t probed_choice(RingPtr ring, const Src& mx, const Src& mo1, const Src& i1,
-                                                  const Src& d1, const Src& mo2, const Src& i2, const Src& d2, int k,
-                                                  int m, int best) {
-  const unsigned long long key = choice_key<NC>(ring, mx, mo1, i1, d1, mo2, i2, d2, k);
-  const int q = (int)(unsigned)(key >> 32);
-  uint32_t r = (uint32_t)key & 0x7Fu;
+__device__ __forceinline__ uint32_t probed_record(RingPtr ring, const Src& bmx, const Src& bmo1, const Src& bi1,
+                                                  const Src& bd1, const Src& bmo2, const Src& bi2, const Src& bd2,
+                                                  const Src& cmx, const Src& cmo1, const Src& ci1, const Src& cd1,
+                                                  const Src& cmo2, const Src& ci2, const Src& cd2, int k, int m) {
+  const unsigned long long kb = choice_key<NC>(ring, bmx, bmo1, bi1, bd1, bmo2, bi2, bd2, k);
+  const unsigned long long kc = choice_key<NC>(ring, cmx, cmo1, ci1, cd1, cmo2, ci2, cd2, k);
+  const int q = (int)(unsigned)(kb >> 32), best = (int)(unsigned)(kc >> 32);
+  uint32_t r = (uint32_t)kb & 0x7Fu;
   if (q >= 0 && q > max(m, best)) r |= kRecStale;
   return r;
+}
+
+// The sources of score s (ring slot cur) as the compute step reads them (c:
+// mx, mo1, i1, d1, mo2, i2, d2) and as the reference's backtrace probes them
+// (b: also the readable cells of a wavefront the cut-off emptied,
+// wavefront_backtrace.c:64-219).  Returns whether the two differ.
+template <int NC, bool CUT>
+__device__ __forceinline__ bool probe_sources(const MetaRegs<NC, CUT>& meta, int s, int cur, const Penalties& pen,
+                                              int wcap, Src* c, Src* b) {
+  const int R = pen.nslots, g = pen.g;
+  bool stale = false;
+  auto both = [&](int i, int s_src, int slot, int comp) {
+    c[i] = get_src(meta, s_src, slot, R, comp, wcap);
+    b[i] = c[i];
+    if (!c[i].valid()) {
+      b[i] = get_src(meta, s_src, slot, R, comp, wcap, true);
+      stale = stale || b[i].readable();
+    }
+  };
+  const Src none{1, -1, kFarDiagonal, 0, 0};
+  for (int i = 0; i < 7; ++i) c[i] = b[i] = none;
+  if (NC > 1 || pen.metric != kIndel) both(0, s - pen.x, cur - pen.x / g, kM);
+  both(1, s - pen.o1 - pen.e1, cur - (pen.o1 + pen.e1) / g, kM);
+  if constexpr (NC > 1) {
+    both(2, s - pen.e1, cur - pen.e1 / g, kI1);
+    both(3, s - pen.e1, cur - pen.e1 / g, kD1);
+  }
+  if constexpr (NC == 5) {
+    both(4, s - pen.o2 - pen.e2, cur - (pen.o2 + pen.e2) / g, kM);
+    both(5, s - pen.e2, cur - pen.e2 / g, kI2);
+    both(6, s - pen.e2, cur - pen.e2 / g, kD2);
+  }
+  return stale;
+}
+
+// Record bytes of step s (ring slot cur, diagonals [lo, hi]) when an input
+// is a wavefront the cut-off emptied: each byte becomes probed_record's.
+template <int NC, int BLOCK, bool CUT, typename RingPtr>
+__device__ __forceinline__ void probed_records(RingPtr ring, const MetaRegs<NC, CUT>& meta, int s, int cur,
+                                               const Penalties& pen, int wcap, int lo, int hi, uint8_t* rec) {
+  Src c[7], b[7];
+  if (!probe_sources<NC, CUT>(meta, s, cur, pen, wcap, c, b)) return;
+  const auto* outM = ring + (cur * NC + kM) * wcap - lo;
+  for (int k = lo + (int)threadIdx.x; k <= hi; k += BLOCK)
+    rec[k - lo] = (uint8_t)probed_record<NC>(ring, b[0], b[1], b[2], b[3], b[4], b[5], b[6], c[0], c[1], c[2], c[3],
+                                             c[4], c[5], c[6], k, (int)outM[k]);
+}
+
+// Record byte of the end cell (s, k) of a z-drop, whose diagonal may lie
+// outside the computed range of score s: the choices over the sources the
+// reference's backtrace probes from it, or 0xFF when none is valid (the
+// backtrace then takes no step, wavefront_backtrace.c:265-266, 470-471).
+template <int NC, bool CUT, typename RingPtr>
+__device__ __forceinline__ int end_record(RingPtr ring, const MetaRegs<NC, CUT>& meta, int s, int cur,
+                                          const Penalties& pen, int wcap, int k) {
+  Src c[7], b[7];
+  probe_sources<NC, CUT>(meta, s, cur, pen, wcap, c, b);
+  const unsigned long long key = choice_key<NC>(ring, b[0], b[1], b[2], b[3], b[4], b[5], b[6], k);
+  return (int)(unsigned)(key >> 32) >= 0 ? (int)(key & 0x7Fu) : 0xFF;
 }
 
 // Ring element stores.  The 16-bit LDS ring saturates to [-16384, 16383]:
@@ -327,13 +388,13 @@ __device__ void wfadaptive_cut(RingPtr m, int lo, int hi, int plen, int tlen, in
   *chi = hi <= bottom ? hi : max(last, bottom);
 }
 
-// x-drop state of one alignment (wavefront_heuristic_clear,
+// x-drop / z-drop state of one alignment (wavefront_heuristic_clear,
 // wavefront_heuristic.c:114-121): the best Smith-Waterman score seen, once
-// there is one.
-struct XdropState {
-  int best;
+// there is one, with its cell (z-drop).
+struct DropState {
+  int best, k, off;
   bool has;
-  __device__ __forceinline__ void clear() { best = 0; has = false; }
+  __device__ __forceinline__ void clear() { best = 0; k = 0; off = kOffsetNull; has = false; }
 };
 
 // x-drop cut-off (wavefront_heuristic_xdrop, wavefront_heuristic.c:294-370)
@@ -344,7 +405,7 @@ struct XdropState {
 // the best rises to this wavefront's best.  Differences wrap around in 32
 // bits as the reference's int arithmetic does.
 template <int BLOCK, typename RingPtr>
-__device__ void xdrop_cut(RingPtr m, int s, int swg_match, int xdrop, BlockRed<BLOCK>& red, XdropState& xs,
+__device__ void xdrop_cut(RingPtr m, int s, int swg_match, int xdrop, BlockRed<BLOCK>& red, DropState& xs,
                           int* clo, int* chi) {
   const int tid = threadIdx.x;
   const int lo = *clo, hi = *chi;
@@ -378,6 +439,47 @@ __device__ void xdrop_cut(RingPtr m, int s, int swg_match, int xdrop, BlockRed<B
     xs.best = cmax;
     xs.has = true;
   }
+}
+
+// z-drop test (wavefront_heuristic_zdrop, wavefront_heuristic.c:371-431) of
+// the extended M wavefront [lo, hi] of score s: its best Smith-Waterman score
+// (first diagonal in ascending order) replaces a lower best seen, with its
+// cell; one more than zdrop below the best drops the alignment (true), its
+// end the best cell.  Differences wrap around in 32 bits.
+template <int BLOCK, typename RingPtr>
+__device__ bool zdrop_check(RingPtr m, int lo, int hi, int s, int swg_match, int zdrop, BlockRed<BLOCK>& red,
+                            DropState& ds) {
+  const int tid = threadIdx.x;
+  auto sw_of = [&](int off, int k) { return (swg_match * ((off - k) + off) - s) / 2; };
+  int cmax = INT_MIN;
+  for (int k = lo + tid; k <= hi; k += BLOCK) {
+    const int off = (int)m[k];
+    if (off >= 0) cmax = max(cmax, sw_of(off, k));
+  }
+  cmax = red.max_all(cmax);
+  int cmax_k = INT_MAX;
+  for (int k = lo + tid; k <= hi; k += BLOCK) {
+    const int off = (int)m[k];
+    if (off >= 0 && sw_of(off, k) == cmax) cmax_k = min(cmax_k, k);
+  }
+  cmax_k = red.min_all(cmax_k);
+  const int cmax_off = cmax_k == INT_MAX ? 0 : uniform((int)m[cmax_k]);
+  if (cmax_k == INT_MAX) cmax_k = 0;  // no valid cell: the reference's (0, 0)
+  if (ds.has) {
+    if (cmax > ds.best) {
+      ds.best = cmax;
+      ds.k = cmax_k;
+      ds.off = cmax_off;
+    } else if ((int)((unsigned)ds.best - (unsigned)cmax) > zdrop) {
+      return true;
+    }
+  } else {
+    ds.best = cmax;
+    ds.k = cmax_k;
+    ds.off = cmax_off;
+    ds.has = true;
+  }
+  return false;
 }
 
 // Banded adaptive cut-off (wavefront_heuristic_banded_adaptive,

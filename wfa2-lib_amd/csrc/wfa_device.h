@@ -25,6 +25,7 @@ enum DevStatus : int {
                               // (WF_STATUS_END_UNREACHABLE, wavefront_extend.c:94-101)
   kDevStaleTrace = -2005,     // the reference's backtrace would read a cell of a
                               // wavefront the cut-off emptied (not replayed)
+  kDevZdropped = -2006,       // z-drop: partial alignment ending at the best cell seen
 };
 
 // Distance metrics, numbered as distance_metric_t (wavefront_penalties.h:41-47).
@@ -79,9 +80,10 @@ struct HeurArgs {
   int wfadaptive;                // 0 none, 1 wf-adaptive, 2 wfmash (weighted distances, :134-145)
   int min_len, thr, steps;       // and the countdown shared by every strategy
   int xdrop, xdrop_thr;          // x-drop (:294-370) on Smith-Waterman scores
+  int zdrop, zdrop_thr;          // z-drop (:371-431), when x-drop is not set
   int swg_match;                 // (swg_match * (v + h) - s) / 2: -match, or -1 when match is 0
   int banded, band_lo, band_hi;  // banded: 0 none, 1 static, 2 adaptive (:451-500)
-  __host__ __device__ bool cut() const { return wfadaptive || xdrop || banded; }
+  __host__ __device__ bool cut() const { return wfadaptive || xdrop || zdrop || banded; }
 };
 
 struct ForwardArgs {
@@ -121,6 +123,7 @@ struct ForwardArgs {
   int pbf, pef, tbf, tef;
   int* out_end_k;              // end cell of each completed pair: diagonal and offset
   int* out_end_off;
+  int* out_end_rec;            // z-drop: the end cell's record byte (0xFF: no source), or null
   HeurArgs heur;                // heuristic cut-off (none: the exact instances)
   Penalties pen;
 };
@@ -152,6 +155,7 @@ struct BacktraceArgs {
   int endsfree;                // trace from (end_k, end_off), free leading / trailing gaps
   const int* end_k;
   const int* end_off;
+  const int* end_rec;          // z-drop: the end cell's record byte (0xFF: no source)
   Penalties pen;
 };
 

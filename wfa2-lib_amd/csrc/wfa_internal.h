@@ -78,6 +78,7 @@ struct JobSet {
   int* ops_begin = nullptr;
   int* end_k = nullptr;        // end cell (diagonal, offset) of each completed job
   int* end_off = nullptr;
+  int* end_rec = nullptr;      // z-drop: record byte of the end cell (0xFF: no source)
   EndsFree ef;
   wfa::HeurArgs heur{};        // wf-adaptive (none for BiWFA base jobs)
 };
@@ -141,7 +142,7 @@ struct wfa_device_batch_t {
   int64_t words_used = 0;                  // words taken by the forward sequences
   std::vector<int> sorted;                 // pair ids, longest first
   wfa_host::DevBuf words, d_plen, d_tlen, d_patw, d_txtw, d_status, d_score, d_last, d_dsteps,
-      d_ops_off, d_ops_begin, d_ops, d_end_k, d_end_off;
+      d_ops_off, d_ops_begin, d_ops, d_end_k, d_end_off, d_end_rec;
   // reversed copies of the sequences (BiWFA), built on first use
   wfa_host::DevBuf rev_words, d_rev_ppos, d_rev_tpos;
   bool have_reverse = false;

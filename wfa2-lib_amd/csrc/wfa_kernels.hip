@@ -81,8 +81,8 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
   // I1/D1/I2/D2 are narrowed to M's range (:163-171).  Indel and edit then
   // read M only inside the cut range: their compute step NULLs the two cells
   // beyond each end (wavefront_compute_edit.c:355-358).
-  auto cutoff = [&](MetaRegs<NC, CUT>& mt, int& steps_wait, XdropState& xs, int slot, int alo, int lo, int hi,
-                    int s_, int plen_, int tlen_) {
+  auto cutoff = [&](MetaRegs<NC, CUT>& mt, int& steps_wait, DropState& xs, int slot, int alo, int lo, int hi,
+                    int s_, int plen_, int tlen_) -> bool {
     if constexpr (CUT) {
       const HeurArgs& hr = a.heur;
       --steps_wait;
@@ -95,6 +95,10 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
       if (hr.xdrop && steps_wait <= 0) {
         xdrop_cut<BLOCK>(m, s_, hr.swg_match, hr.xdrop_thr, red, xs, &clo, &chi);
         steps_wait = hr.steps;
+      } else if (hr.zdrop && steps_wait <= 0) {
+        // a z-drop ends the alignment here, its countdown untouched
+        if (zdrop_check<BLOCK>(m, clo, chi, s_, hr.swg_match, hr.zdrop_thr, red, xs)) return true;
+        steps_wait = hr.steps;
       }
       if (hr.banded == 1) {
         clo = max(clo, hr.band_lo);
@@ -103,7 +107,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
         banded_adaptive_cut(m, hr.band_lo, hr.band_hi, plen_, tlen_, steps_wait, hr.steps, &clo, &chi);
       }
       // unchanged M: nothing else is narrowed either (wavefront_heuristic.c:540-541)
-      if (clo == lo && chi == hi) return;
+      if (clo == lo && chi == hi) return false;
       if (lane == slot) {
         unsigned fl = mt.flags;
         mt.cut.lo[kM] = clo;
@@ -121,6 +125,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
         mt.flags = fl;
       }
     }
+    return false;
   };
 
   unsigned long long chunk_cur = 0, chunk_end = 0;  // record chunk of this workgroup (16-byte units)
@@ -171,7 +176,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
       for (int c = 0; c < NC; ++c) { meta.cut.lo[c] = 0; meta.cut.hi[c] = 0; }
     }
     int steps_wait = a.heur.steps;  // wavefront_heuristic_clear (wavefront_heuristic.c:114-121)
-    XdropState xs;
+    DropState xs;
     xs.clear();
     int cur = 0;
     int s = 0;
@@ -241,7 +246,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
     // the heuristic also runs after the score-0 extension (wavefront_extend.c:112-118)
     if (CUT && status == 1 && cbeg == kM) {
       const int lo0 = a.endsfree ? -a.pbf : 0, hi0 = a.endsfree ? a.tbf : 0;
-      cutoff(meta, steps_wait, xs, 0, lo0, lo0, hi0, 0, plen, tlen);
+      if (cutoff(meta, steps_wait, xs, 0, lo0, lo0, hi0, 0, plen, tlen)) status = kDevZdropped;
     }
 
     int nulls = 0;  // null steps in a row, every score counted (align_status.num_null_steps)
@@ -289,29 +294,6 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
         continue;
       }
       if constexpr (CUT) nulls = 0;
-      // Sources as the reference's backtrace probes them: also the readable
-      // cells of an input the cut-off emptied (a null wavefront to compute).
-      Src bmx = mx, bmo1 = mo1, bi1 = i1, bd1 = d1, bmo2 = mo2, bi2 = i2, bd2 = d2;
-      bool stale = false;
-      if constexpr (CUT) {
-        auto probe_src = [&](const Src& in, int s_src, int slot, int c) -> Src {
-          if (in.valid()) return in;
-          const Src b = get_src(meta, s_src, slot, R, c, wcap, true);
-          stale = stale || b.readable();
-          return b;
-        };
-        if (NC > 1 || pen.metric != kIndel) bmx = probe_src(mx, s - pen.x, cur - dq_x, kM);
-        bmo1 = probe_src(mo1, s - pen.o1 - pen.e1, cur - dq_o1, kM);
-        if constexpr (NC > 1) {
-          bi1 = probe_src(i1, s - pen.e1, cur - dq_e1, kI1);
-          bd1 = probe_src(d1, s - pen.e1, cur - dq_e1, kD1);
-        }
-        if constexpr (NC == 5) {
-          bmo2 = probe_src(mo2, s - pen.o2 - pen.e2, cur - dq_o2, kM);
-          bi2 = probe_src(i2, s - pen.e2, cur - dq_e2, kI2);
-          bd2 = probe_src(d2, s - pen.e2, cur - dq_e2, kD2);
-        }
-      }
       // wavefront_compute_limits_input (wavefront_compute.c:40-86)
       int lo, hi;
       if (NC == 1 && pen.metric <= kEdit) {
@@ -398,16 +380,13 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
             uint32_t src = kSrcX;
             if (del > m) { m = del; src = kSrcD1; }
             if (ins > m) { m = ins; src = kSrcI1; }
-            const int best = m;
             if (!in_bounds(m, k, plen, tlen)) m = kOffsetNull;
             if (m != kOffsetNull) {
               if (prune_try) wmin = min(wmin, max(plen - (m - k), tlen - m));
               m = lcp_extend<BITS>(P, pb, T, tb, plen, tlen, k, m);
               hit = a.endsfree && endsfree_done(m, k, plen, tlen, a.pef, a.tef);
             }
-            if (rec)
-              rec[k - lo] = (uint8_t)(CUT && stale ? probed_choice<NC>(ring, bmx, bmo1, bi1, bd1, bmo2, bi2, bd2, k, m, best)
-                                                   : src);
+            if (rec) rec[k - lo] = (uint8_t)src;
             outM[k] = to_ring(m, ring);
             vM = (m != kOffsetNull);
           }
@@ -442,9 +421,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
             m = lcp_extend<BITS>(P, pb, T, tb, plen, tlen, k, m);
             hit = a.endsfree && endsfree_done(m, k, plen, tlen, a.pef, a.tef);
           }
-          if (rec)
-            rec[k - lo] = (uint8_t)(CUT && stale ? probed_choice<NC>(ring, bmx, bmo1, bi1, bd1, bmo2, bi2, bd2, k, m, best)
-                                                 : (src | bits));
+          if (rec) rec[k - lo] = (uint8_t)(src | bits);
           outM[k] = to_ring(m, ring);
           vM = (m != kOffsetNull);
           if (has_i1) { outI1[k] = to_ring(ins1, ring); vI1 = in_bounds(ins1, k, plen, tlen); }
@@ -503,6 +480,14 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
         }
       }
       block_sync();  // ring values of this step visible to every lane; sh_red free again
+      if constexpr (CUT) {
+        // The reference's backtrace also probes the readable cells of an
+        // input the cut-off emptied (a null wavefront to compute,
+        // wavefront_backtrace.c:64-219).  Where one exists, the record bytes
+        // take the choices over the probed sources (probed_record); each lane
+        // rewrites the bytes it wrote above.
+        if (rec) probed_records<NC, BLOCK>(ring, meta, s, cur, pen, wcap, lo, hi, rec);
+      }
       if constexpr (NC == 1) {
         if (prune_try && !rM.null()) edit_prune(rM, ring, mo1, wmin, plen, tlen);
       }
@@ -552,8 +537,16 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
           if (vend >= tlen) status = kDevCompleted;
         }
       }
-      if (CUT && status == 1 && !rM.null()) cutoff(meta, steps_wait, xs, cur, lo, rM.first, rM.last, s, plen, tlen);
+      if (CUT && status == 1 && !rM.null() && cutoff(meta, steps_wait, xs, cur, lo, rM.first, rM.last, s, plen, tlen)) {
+        status = kDevZdropped;
+        const int er = end_record<NC, CUT>(ring, meta, s, cur, pen, wcap, xs.k);
+        if (tid == 0 && a.out_end_rec) a.out_end_rec[p] = er;
+      }
       if (status == 1 && s >= s_cap) status = kDevBadTrace;  // unreachable for valid input
+    }
+    if (CUT && status == kDevZdropped) {  // alignment_end_pos of the z-drop (wavefront_heuristic.c:424-431)
+      end_k = xs.k;
+      end_off = xs.off;
     }
     if (tid == 0) {
       a.out_status[p] = (status == 1) ? kDevCompleted : status;
@@ -564,7 +557,9 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
       if (status == kDevRingOverflow || status == kDevArenaFull) atomicAdd(a.retry_count, 1u);
       else if (status != 1 && status != kDevCompleted) atomicAdd(a.fail_count, 1u);
     }
-    if (status == 1 || status == kDevCompleted || status == kDevMaxSteps) wg_dsteps += (unsigned long long)dsteps;
+    if (status == 1 || status == kDevCompleted || status == kDevMaxSteps || status == kDevUnreachable ||
+        status == kDevZdropped)
+      wg_dsteps += (unsigned long long)dsteps;
   }
   if (tid == 0 && wg_dsteps) atomicAdd(a.dsteps_total, wg_dsteps);
 }
@@ -595,7 +590,10 @@ __global__ void __launch_bounds__(256) wfa_backtrace_kernel(BacktraceArgs a) {
   const int plen = a.plen[p], tlen = a.tlen[p];
   const int cap = plen + tlen;
   unsigned char* out = reinterpret_cast<unsigned char*>(a.ops + a.ops_off[p]);
-  if (a.status[p] != kDevCompleted) { a.ops_begin[p] = cap; return; }
+  // A z-dropped pair is traced from the best cell it saw, at the score it
+  // dropped at (wavefront_unialign.c:175-200), like an ends-free end cell.
+  const bool zd = a.status[p] == kDevZdropped;
+  if (a.status[p] != kDevCompleted && !zd) { a.ops_begin[p] = cap; return; }
   const Penalties pen = a.pen;
   const int cbeg = a.comp_begin_p ? (int)a.comp_begin_p[p] : a.comp_begin;
   int comp = a.comp_end_p ? (int)a.comp_end_p[p] : a.comp_end;
@@ -604,34 +602,66 @@ __global__ void __launch_bounds__(256) wfa_backtrace_kernel(BacktraceArgs a) {
   int pos = cap;
   int nx = 0, nid = 0;
   bool bad = false, stale = false;
-  if (a.endsfree) {
-    // Free trailing gaps after the end cell (wavefront_backtrace.c:345-355):
-    // the pattern's rest as deletions, or the text's rest as insertions.
+  // A z-drop's first step uses the end cell's own record byte (its diagonal
+  // may lie outside the computed range of its score); with no valid source,
+  // or on a border, the backtrace takes no step (wavefront_backtrace.c:
+  // 265-266, 296-316): matches along the diagonal, then gaps to the origin.
+  const int endrec = zd && a.end_rec ? a.end_rec[p] : 0;
+  const int vE = zd ? a.end_off[p] - a.end_k[p] : 0, hE = zd ? a.end_off[p] : 0;
+  const bool direct = zd && (endrec == 0xFF || vE <= 0 || hE <= 0 || s <= 0);
+  int ntrail_all = 0;
+  if (a.endsfree || zd) {
+    // Trailing gaps after the end cell (wavefront_backtrace.c:345-355): the
+    // pattern's rest as deletions, then the text's rest as insertions.  A
+    // z-dropped CIGAR that takes steps keeps none of them: max-trim never
+    // keeps a gap at the end, and the taken-back ones are among them.
     k = a.end_k[p];
     const int off = a.end_off[p];
     const int nd = plen - (off - k), ni = tlen - off;
+    ntrail_all = nd + ni;
     if (nd < 0 || ni < 0 || nd + ni > cap) {
       bad = true;
-    } else {
+    } else if (!zd || direct) {
       for (int i = 0; i < nd; ++i) out[--pos] = 'D';
       for (int i = 0; i < ni; ++i) out[--pos] = 'I';
     }
   }
   const int ntrail = cap - pos;
+  bool first = zd;
+  if (direct && !bad) {
+    const int m = (vE > 0 && hE > 0) ? min(vE, hE) : 0;
+    if (vE < 0 || hE < 0 || vE + hE - m > pos) {
+      bad = true;
+    } else {
+      for (int i = 0; i < m; ++i) out[--pos] = 'M';
+      for (int i = 0; i < vE - m; ++i) out[--pos] = 'D';
+      for (int i = 0; i < hE - m; ++i) out[--pos] = 'I';
+    }
+  }
   bool mark = comp == kM;  // the next operation written leads into an M cell
   uint32_t blk = a.last_block[p];
   uint4 hdr = make_uint4(0, 0, 0, kNoBlock);
   if (blk != kNoBlock && blk < a.rec_units) hdr = a.rec_arena[blk];
-  while (s > 0 && !bad) {
-    while (blk != kNoBlock && blk < a.rec_units && (int)hdr.x > s) {
-      blk = hdr.w;
-      if (blk != kNoBlock && blk < a.rec_units) hdr = a.rec_arena[blk];
-    }
-    if (blk == kNoBlock || blk >= a.rec_units || (int)hdr.x != s || k < (int)hdr.y || k > (int)hdr.z || pos <= 0) {
+  while (s > 0 && !bad && !direct) {
+    if (pos <= 0) {
       bad = true;
       break;
     }
-    const uint32_t rec = reinterpret_cast<const uint8_t*>(a.rec_arena + blk + 1)[k - (int)hdr.y];
+    uint32_t rec;
+    if (first) {
+      first = false;
+      rec = (uint32_t)endrec;
+    } else {
+      while (blk != kNoBlock && blk < a.rec_units && (int)hdr.x > s) {
+        blk = hdr.w;
+        if (blk != kNoBlock && blk < a.rec_units) hdr = a.rec_arena[blk];
+      }
+      if (blk == kNoBlock || blk >= a.rec_units || (int)hdr.x != s || k < (int)hdr.y || k > (int)hdr.z) {
+        bad = true;
+        break;
+      }
+      rec = reinterpret_cast<const uint8_t*>(a.rec_arena + blk + 1)[k - (int)hdr.y];
+    }
     if (rec & kRecStale) {
       stale = bad = true;
       break;
@@ -668,25 +698,65 @@ __global__ void __launch_bounds__(256) wfa_backtrace_kernel(BacktraceArgs a) {
   }
   // The walk ends at score 0: at the origin, or (ends-free) at a free
   // beginning cell (v=0, h=k) or (v=-k, h=0) of the score-0 wavefront.
-  if (!bad && (s != 0 || comp != cbeg || (k != 0 && !a.endsfree))) bad = true;
+  if (!bad && !direct && (s != 0 || comp != cbeg || (k != 0 && !a.endsfree))) bad = true;
   const int nlead = k < 0 ? -k : k;
-  int start = cap;
-  if (!bad) {
+  int start = direct && !bad ? pos : cap;
+  const long long ppos = a.pat_pos[p], tpos = a.txt_pos[p];
+  const uint64_t* P = a.words + ppos / SPW;
+  const uint64_t* T = a.words + tpos / SPW;
+  const int pb = (int)(ppos % SPW), tb = (int)(tpos % SPW);
+  const int last_walk = cap - ntrail - 1;  // the operation leading into the end cell
+  // A z-drop's end cell: the reference writes the matches from the cell's
+  // computed offset up to the best cell's offset whatever the bases
+  // (wavefront_backtrace.c:77-98); a best offset behind the computed one
+  // takes back that many of the trailing gaps written before it.  A dry run
+  // of the replay finds the computed offset and the matches before it.
+  int zrun = 0, zback = 0, zmatches = 0;
+  if (!bad && zd && !direct) {
+    int v = k < 0 ? -k : 0, h = k > 0 ? k : 0, q = h;
+    auto dry = [&]() {
+      const int m = lcp_extend<BITS>(P, pb, T, tb, plen, tlen, h - v, h) - h;
+      zmatches += m;
+      v += m;
+      h += m;
+    };
+    if (cbeg == kM) {
+      if (pos == cap - ntrail) q = h; else dry();
+    }
+    for (int i = pos; i < cap - ntrail; ++i) {
+      const unsigned char b = out[i];
+      const unsigned char op = b & 0x7F;
+      if (op == 'X') { ++v; ++h; }
+      else if (op == 'I') ++h;
+      else ++v;
+      if (b & 0x80) {
+        if (i == last_walk) q = h; else dry();
+      }
+    }
+    const int n = a.end_off[p] - q;
+    zrun = n > 0 ? n : 0;
+    zback = n < 0 ? -n : 0;
+    if (zback > ntrail_all) stale = bad = true;  // would take back past the trailing gaps
+  }
+  if (!bad && !direct) {
     // 2M + 2X + I + D = plen + tlen: the match count follows, and with it
     // the CIGAR's length and start (rest >= 0 keeps start in [0, pos - nlead]).
-    const int rest = cap - 2 * nx - nid - ntrail - nlead;
-    if (rest < 0 || (rest & 1)) bad = true;
-    const int len = bad ? 0 : rest / 2 + (cap - pos) + nlead;
-    start = cap - len;
+    // A z-drop's CIGAR is counted from the dry run instead.
+    int len;
+    if (zd) {
+      len = zmatches + zrun + (cap - pos) + nlead;
+      if (len > cap) bad = true;
+    } else {
+      const int rest = cap - 2 * nx - nid - ntrail - nlead;
+      if (rest < 0 || (rest & 1)) bad = true;
+      len = bad ? 0 : rest / 2 + (cap - pos) + nlead;
+    }
+    start = bad ? cap : cap - len;
     int w = start, v = 0, h = 0;
     if (!bad) {
       for (int i = 0; i < nlead; ++i) out[w++] = k > 0 ? 'I' : 'D';  // free leading gaps
       if (k > 0) h = k; else v = -k;
     }
-    const long long ppos = a.pat_pos[p], tpos = a.txt_pos[p];
-    const uint64_t* P = a.words + ppos / SPW;
-    const uint64_t* T = a.words + tpos / SPW;
-    const int pb = (int)(ppos % SPW), tb = (int)(tpos % SPW);
     auto extend = [&]() {
       const int h2 = lcp_extend<BITS>(P, pb, T, tb, plen, tlen, h - v, h);
       const int m = h2 - h;
@@ -695,7 +765,15 @@ __global__ void __launch_bounds__(256) wfa_backtrace_kernel(BacktraceArgs a) {
       v += m;
       h += m;
     };
-    if (!bad && cbeg == kM) extend();
+    auto run_to_end = [&]() {
+      if (w + zrun > cap) { bad = true; return; }
+      for (int i = 0; i < zrun; ++i) out[w++] = 'M';
+      v += zrun;
+      h += zrun;
+    };
+    if (!bad && cbeg == kM) {
+      if (zd && pos == cap - ntrail) run_to_end(); else extend();
+    }
     for (int i = pos; i < cap && !bad; ++i) {
       const unsigned char b = out[i];
       const unsigned char op = b & 0x7F;
@@ -704,9 +782,12 @@ __global__ void __launch_bounds__(256) wfa_backtrace_kernel(BacktraceArgs a) {
       if (op == 'X') { ++v; ++h; }
       else if (op == 'I') ++h;
       else ++v;
-      if (b & 0x80) extend();
+      if (b & 0x80) {
+        if (zd && i == last_walk) run_to_end(); else extend();
+      }
     }
-    if (!bad && (v != plen || h != tlen || w != cap)) bad = true;
+    // the end: (plen, tlen), or a z-drop's end cell moved by what it takes back
+    if (!bad && (v != (zd ? vE + zback : plen) || h != (zd ? hE + zback : tlen) || w != cap)) bad = true;
   }
   if (stale) {
     a.status[p] = kDevStaleTrace;

@@ -320,11 +320,12 @@ class Aligner:
             a.heuristic.strategy = 0
         if heuristic is not None:
             # (strategy, min_wavefront_length, max_distance_threshold, steps_between_cutoffs
-            #  [, min_k, max_k[, xdrop]]); strategy bits as wf_heuristic_strategy: 1 banded static,
-            # 2 banded adaptive, 4 wf-adaptive, 0x10 x-drop, 0x40 wfmash
-            h = tuple(heuristic) + (0,) * (7 - len(heuristic))
+            #  [, min_k, max_k[, xdrop[, zdrop]]]); strategy bits as wf_heuristic_strategy: 1 banded
+            # static, 2 banded adaptive, 4 wf-adaptive, 0x10 x-drop, 0x20 z-drop, 0x40 wfmash
+            h = tuple(heuristic) + (0,) * (8 - len(heuristic))
             (a.heuristic.strategy, a.heuristic.min_wavefront_length, a.heuristic.max_distance_threshold,
-             a.heuristic.steps_between_cutoffs, a.heuristic.min_k, a.heuristic.max_k, a.heuristic.xdrop) = h
+             a.heuristic.steps_between_cutoffs, a.heuristic.min_k, a.heuristic.max_k, a.heuristic.xdrop,
+             a.heuristic.zdrop) = h
             a.heuristic.steps_wait = h[3]
         if max_steps:
             a.system.max_alignment_steps = int(max_steps)
