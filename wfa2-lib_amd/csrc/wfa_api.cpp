@@ -580,7 +580,7 @@ PlanGroup make_large(const JobSet& js, std::vector<int> ids) {
   return g;
 }
 
-LaunchShape shape_of(Engine* e, const PlanGroup& g, int bits, const Penalties& pen, bool cut) {
+LaunchShape shape_of(Engine* e, const PlanGroup& g, int bits, const Penalties& pen, int cut) {
   LaunchShape shape;
   shape.family = g.family;
   shape.block = g.block;
@@ -694,7 +694,7 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
 }
 
 // Device memory every group of the plan needs, allocated before any launch.
-bool reserve_for(Engine* e, std::vector<PlanGroup>& plan, int bits, const Penalties& pen, bool cut, int records) {
+bool reserve_for(Engine* e, std::vector<PlanGroup>& plan, int bits, const Penalties& pen, int cut, int records) {
   if (records && !e->rec.ensure(e->rec_bytes)) {
     fprintf(stderr, "[WFA] cannot allocate the %zu MB record arena\n", e->rec_bytes >> 20);
     return false;
@@ -1130,7 +1130,7 @@ extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_ba
     if (rc == WF_STATUS_OOM) return rc;
   } else {
     const std::vector<int> key = {pen.metric, pen.x, pen.o1, pen.e1, pen.o2, pen.e2, b->bits, b->ef.on,
-                                  b->heur.cut() ? 1 : 0};
+                                  b->heur.cut()};
     if (b->plan_key != key) {
       std::vector<PlanGroup> plan = make_plan(b->jobs(), b->sorted, pen);
       recycle_plan(b->plan, plan);

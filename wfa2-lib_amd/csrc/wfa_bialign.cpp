@@ -138,7 +138,7 @@ bool run_breakpoints(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, 
     }
     const int wcap = wmax + 8;
     const wfa::HeurArgs heur = heuristic_of(wf);
-    int per_cu = wfa::breakpoint_max_blocks_per_cu(ncomp, b->bits, block, heur.cut());
+    int per_cu = wfa::breakpoint_max_blocks_per_cu(ncomp, b->bits, block, heur.cut() != 0);
     if (per_cu <= 0) per_cu = 1;
     const int grid = (int)std::min<int64_t>((int64_t)ids.size(), (int64_t)per_cu * e->num_cus);
     const long long stride = 2LL * pen.nslots * ncomp * wcap;

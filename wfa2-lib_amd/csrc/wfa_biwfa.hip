@@ -704,7 +704,7 @@ int breakpoint_max_blocks_per_cu(int ncomp, int bits, int block, bool cut) {
 }
 
 hipError_t launch_breakpoint(const BiArgs& a, int ncomp, int bits, int block, int grid, hipStream_t stream) {
-  const void* f = select_breakpoint(ncomp, bits, block, a.heur.cut());
+  const void* f = select_breakpoint(ncomp, bits, block, a.heur.cut() != 0);
   if (!f) return hipErrorInvalidValue;
   void* args[] = {const_cast<BiArgs*>(&a)};
   return hipLaunchKernel(f, dim3(grid), dim3(block), args, 0, stream);

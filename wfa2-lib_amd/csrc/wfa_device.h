@@ -83,7 +83,8 @@ struct HeurArgs {
   int zdrop, zdrop_thr;          // z-drop (:371-431), when x-drop is not set
   int swg_match;                 // (swg_match * (v + h) - s) / 2: -match, or -1 when match is 0
   int banded, band_lo, band_hi;  // banded: 0 none, 1 static, 2 adaptive (:451-500)
-  __host__ __device__ bool cut() const { return wfadaptive || xdrop || zdrop || banded; }
+  // kernel instance: 0 exact, 1 wf-adaptive / wfmash / banded, 2 with an x-drop or z-drop
+  __host__ __device__ int cut() const { return (xdrop || zdrop) ? 2 : ((wfadaptive || banded) ? 1 : 0); }
 };
 
 struct ForwardArgs {
@@ -259,7 +260,7 @@ hipError_t launch_forward_lean(const ForwardArgs& a, const LaunchShape& shape, h
 
 // Dynamic-LDS bytes and occupancy for a configuration.
 size_t forward_lds_bytes(int family, int ncomp, int nslots, int wcap, int lmax_words);
-int forward_max_blocks_per_cu(int family, int block, int ncomp, int bits, bool cut, size_t lds_bytes);
+int forward_max_blocks_per_cu(int family, int block, int ncomp, int bits, int cut, size_t lds_bytes);
 int pick_block(int family, int max_width);
 
 }  // namespace wfa

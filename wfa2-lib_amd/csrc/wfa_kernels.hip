@@ -22,8 +22,10 @@
 namespace wfa {
 namespace {
 
-template <int NC, int BLOCK, bool SMALL, int BITS, bool CUT>
+// CUT: 0 exact; 1 wf-adaptive, wfmash and the bands; 2 also x-drop / z-drop.
+template <int NC, int BLOCK, bool SMALL, int BITS, int CUT>
 __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
+  constexpr bool HC = CUT != 0;
   // The LDS family stores the ring as int16, the HBM family as int32.
   using RT = typename std::conditional<SMALL, int16_t, int>::type;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -81,7 +83,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
   // I1/D1/I2/D2 are narrowed to M's range (:163-171).  Indel and edit then
   // read M only inside the cut range: their compute step NULLs the two cells
   // beyond each end (wavefront_compute_edit.c:355-358).
-  auto cutoff = [&](MetaRegs<NC, CUT>& mt, int& steps_wait, DropState& xs, int slot, int alo, int lo, int hi,
+  auto cutoff = [&](MetaRegs<NC, HC>& mt, int& steps_wait, DropState& xs, int slot, int alo, int lo, int hi,
                     int s_, int plen_, int tlen_) -> bool {
     if constexpr (CUT) {
       const HeurArgs& hr = a.heur;
@@ -92,10 +94,10 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
         wfadaptive_cut<BLOCK>(m, lo, hi, plen_, tlen_, hr.thr, hr.wfadaptive == 2, red, &clo, &chi);
         steps_wait = hr.steps;
       }
-      if (hr.xdrop && steps_wait <= 0) {
+      if (CUT == 2 && hr.xdrop && steps_wait <= 0) {
         xdrop_cut<BLOCK>(m, s_, hr.swg_match, hr.xdrop_thr, red, xs, &clo, &chi);
         steps_wait = hr.steps;
-      } else if (hr.zdrop && steps_wait <= 0) {
+      } else if (CUT == 2 && hr.zdrop && steps_wait <= 0) {
         // a z-drop ends the alignment here, its countdown untouched
         if (zdrop_check<BLOCK>(m, clo, chi, s_, hr.swg_match, hr.zdrop_thr, red, xs)) return true;
         steps_wait = hr.steps;
@@ -130,7 +132,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
 
   unsigned long long chunk_cur = 0, chunk_end = 0;  // record chunk of this workgroup (16-byte units)
   unsigned long long wg_dsteps = 0;                  // diagonal-steps of this workgroup
-  MetaRegs<NC, CUT> meta;
+  MetaRegs<NC, HC> meta;
 
   for (;;) {
     int qv = 0;
@@ -539,12 +541,12 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
       }
       if (CUT && status == 1 && !rM.null() && cutoff(meta, steps_wait, xs, cur, lo, rM.first, rM.last, s, plen, tlen)) {
         status = kDevZdropped;
-        const int er = end_record<NC, CUT>(ring, meta, s, cur, pen, wcap, xs.k);
+        const int er = end_record<NC, HC>(ring, meta, s, cur, pen, wcap, xs.k);
         if (tid == 0 && a.out_end_rec) a.out_end_rec[p] = er;
       }
       if (status == 1 && s >= s_cap) status = kDevBadTrace;  // unreachable for valid input
     }
-    if (CUT && status == kDevZdropped) {  // alignment_end_pos of the z-drop (wavefront_heuristic.c:424-431)
+    if (CUT == 2 && status == kDevZdropped) {  // alignment_end_pos of the z-drop (wavefront_heuristic.c:424-431)
       end_k = xs.k;
       end_off = xs.off;
     }
@@ -810,27 +812,31 @@ __global__ void wfa_reset_kernel(Counters* c, int all) {
   }
 }
 
-template <int NC, int BLOCK, bool SMALL, int BITS, bool CUT>
+template <int NC, int BLOCK, bool SMALL, int BITS, int CUT>
 const void* forward_fn() {
   return reinterpret_cast<const void*>(&wfa_forward_kernel<NC, BLOCK, SMALL, BITS, CUT>);
 }
 
 // Every instance the host can select.
-const void* select_forward(int family, int ncomp, int bits, int block, bool cut) {
+const void* select_forward(int family, int ncomp, int bits, int block, int cut) {
 #define WFA_SEL(NC, BL, SM, BI, CU) \
   if (ncomp == NC && block == BL && (family == kSmall) == SM && bits == BI && cut == CU) \
     return forward_fn<NC, BL, SM, BI, CU>();
 #define WFA_SEL_NC(BL, SM, CU) \
   WFA_SEL(1, BL, SM, 2, CU) WFA_SEL(1, BL, SM, 8, CU) WFA_SEL(3, BL, SM, 2, CU) WFA_SEL(3, BL, SM, 8, CU) \
   WFA_SEL(5, BL, SM, 2, CU) WFA_SEL(5, BL, SM, 8, CU)
-  WFA_SEL_NC(64, true, false)
-  WFA_SEL_NC(64, false, false)
-  WFA_SEL_NC(256, false, false)
-  WFA_SEL_NC(1024, false, false)
-  WFA_SEL_NC(64, true, true)
-  WFA_SEL_NC(64, false, true)
-  WFA_SEL_NC(256, false, true)
-  WFA_SEL_NC(1024, false, true)
+  WFA_SEL_NC(64, true, 0)
+  WFA_SEL_NC(64, false, 0)
+  WFA_SEL_NC(256, false, 0)
+  WFA_SEL_NC(1024, false, 0)
+  WFA_SEL_NC(64, true, 1)
+  WFA_SEL_NC(64, false, 1)
+  WFA_SEL_NC(256, false, 1)
+  WFA_SEL_NC(1024, false, 1)
+  WFA_SEL_NC(64, true, 2)
+  WFA_SEL_NC(64, false, 2)
+  WFA_SEL_NC(256, false, 2)
+  WFA_SEL_NC(1024, false, 2)
 #undef WFA_SEL_NC
 #undef WFA_SEL
   return nullptr;
@@ -854,7 +860,7 @@ int pick_block(int family, int max_width) {
   return 1024;
 }
 
-int forward_max_blocks_per_cu(int family, int block, int ncomp, int bits, bool cut, size_t lds_bytes) {
+int forward_max_blocks_per_cu(int family, int block, int ncomp, int bits, int cut, size_t lds_bytes) {
   const void* f = select_forward(family, ncomp, bits, block, cut);
   if (!f) return 0;
   int n = 0;
