@@ -566,6 +566,45 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
   if (tid == 0 && wg_dsteps) atomicAdd(a.dsteps_total, wg_dsteps);
 }
 
+// Forward writer of the replay: consecutive bytes from position w0 of a
+// slot are combined into aligned 4-byte stores (one memory transaction per
+// four operations instead of one per operation).  A dword that reaches
+// below w0 is written byte by byte, and so is the partial dword at the end
+// (finish), so no byte outside [w0, w) is touched: the neighbouring slots
+// belong to other lanes.  Positions below w may sit in the combiner until
+// their dword fills; the replay only reads positions at or above w.
+struct OpWriter {
+  unsigned char* out;
+  int w, w0;
+  uint32_t acc;
+  __device__ __forceinline__ unsigned phase() const { return (unsigned)(reinterpret_cast<uintptr_t>(out + w) & 3u); }
+  __device__ __forceinline__ void put(unsigned c) {
+    const unsigned d = phase();
+    acc |= (c & 0xFFu) << (8u * d);
+    ++w;
+    if (d == 3u) {
+      if (w - 4 >= w0) {
+        *reinterpret_cast<uint32_t*>(out + w - 4) = acc;
+      } else {
+        for (int i = w0; i < w; ++i) out[i] = (unsigned char)(acc >> (8u * (3u - (unsigned)(w - 1 - i))));
+      }
+      acc = 0;
+    }
+  }
+  // m copies of c: bytes up to the next dword boundary, then whole dwords.
+  __device__ __forceinline__ void run(unsigned c, int m) {
+    while (m > 0 && phase() != 0u) { put(c); --m; }
+    const uint32_t c4 = (c & 0xFFu) * 0x01010101u;
+    for (; m >= 4; m -= 4, w += 4) *reinterpret_cast<uint32_t*>(out + w) = c4;
+    for (; m > 0; --m) put(c);
+  }
+  __device__ __forceinline__ void finish() {
+    const int d = (int)phase();
+    for (int i = max(w - d, w0); i < w; ++i) out[i] = (unsigned char)(acc >> (8 * (d - (w - i))));
+    acc = 0;
+  }
+};
+
 // One lane per pair, in two passes over the pair's CIGAR slot
 // ops[0 .. plen+tlen):
 //  1. Walk back from the end cell to the origin over (score, diagonal,
@@ -754,22 +793,23 @@ __global__ void __launch_bounds__(256) wfa_backtrace_kernel(BacktraceArgs a) {
       len = bad ? 0 : rest / 2 + (cap - pos) + nlead;
     }
     start = bad ? cap : cap - len;
-    int w = start, v = 0, h = 0;
+    OpWriter ow{out, start, start, 0u};
+    int v = 0, h = 0;
     if (!bad) {
-      for (int i = 0; i < nlead; ++i) out[w++] = k > 0 ? 'I' : 'D';  // free leading gaps
+      ow.run(k > 0 ? 'I' : 'D', nlead);  // free leading gaps
       if (k > 0) h = k; else v = -k;
     }
     auto extend = [&]() {
       const int h2 = lcp_extend<BITS>(P, pb, T, tb, plen, tlen, h - v, h);
       const int m = h2 - h;
-      if (w + m > cap) { bad = true; return; }
-      for (int i = 0; i < m; ++i) out[w++] = 'M';
+      if (ow.w + m > cap) { bad = true; return; }
+      ow.run('M', m);
       v += m;
       h += m;
     };
     auto run_to_end = [&]() {
-      if (w + zrun > cap) { bad = true; return; }
-      for (int i = 0; i < zrun; ++i) out[w++] = 'M';
+      if (ow.w + zrun > cap) { bad = true; return; }
+      ow.run('M', zrun);
       v += zrun;
       h += zrun;
     };
@@ -779,8 +819,8 @@ __global__ void __launch_bounds__(256) wfa_backtrace_kernel(BacktraceArgs a) {
     for (int i = pos; i < cap && !bad; ++i) {
       const unsigned char b = out[i];
       const unsigned char op = b & 0x7F;
-      if (w > i) { bad = true; break; }  // inconsistent records: never overrun unread operations
-      out[w++] = op;
+      if (ow.w > i) { bad = true; break; }  // inconsistent records: never overrun unread operations
+      ow.put(op);
       if (op == 'X') { ++v; ++h; }
       else if (op == 'I') ++h;
       else ++v;
@@ -788,8 +828,9 @@ __global__ void __launch_bounds__(256) wfa_backtrace_kernel(BacktraceArgs a) {
         if (zd && i == last_walk) run_to_end(); else extend();
       }
     }
+    ow.finish();
     // the end: (plen, tlen), or a z-drop's end cell moved by what it takes back
-    if (!bad && (v != (zd ? vE + zback : plen) || h != (zd ? hE + zback : tlen) || w != cap)) bad = true;
+    if (!bad && (v != (zd ? vE + zback : plen) || h != (zd ? hE + zback : tlen) || ow.w != cap)) bad = true;
   }
   if (stale) {
     a.status[p] = kDevStaleTrace;
