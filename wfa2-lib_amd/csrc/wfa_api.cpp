@@ -723,7 +723,12 @@ std::vector<PlanGroup> make_plan(const JobSet& js, const std::vector<int>& ids, 
   std::vector<int> small_ids, cls[3];
   int lmax_small = 0;
   const LeanTuning& lt = lean_tuning();
-  const bool lean_ok = lt.enabled && ncomp == 3 && js.bits == 2 && !js.cbeg && !js.cend && !js.ef.on &&
+  // Ends-free pairs take the wider ring in both length classes (their
+  // wavefronts grow over the free flanks: measured on 100-300 bp reads with
+  // flanks of up to 30 bases, the narrow ring re-ran enough pairs to cost
+  // 20%); the score-0 wavefront spans pbf + tbf + 1 diagonals and must fit.
+  const bool ef_fits = !js.ef.on || (int64_t)js.ef.pbf + js.ef.tbf + 3 <= wfa::lean_width(lt.wcap_long);
+  const bool lean_ok = lt.enabled && ncomp == 3 && js.bits == 2 && !js.cbeg && !js.cend && ef_fits &&
                        !js.heur.cut() && wfa::lean_supported(pen);
   std::vector<int> lean_ids[2];
   int lean_lmax[2] = {0, 0};
@@ -749,7 +754,7 @@ std::vector<PlanGroup> make_plan(const JobSet& js, const std::vector<int>& ids, 
     PlanGroup g;
     g.family = wfa::kLean;
     g.block = 64;
-    g.wcap = wfa::lean_width(c == 0 ? lt.wcap_short : lt.wcap_long);
+    g.wcap = wfa::lean_width(c == 0 && !js.ef.on ? lt.wcap_short : lt.wcap_long);
     g.lmax_words = lean_lmax[c];
     g.ids = std::move(lean_ids[c]);
     plan.push_back(std::move(g));

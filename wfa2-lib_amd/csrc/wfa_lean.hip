@@ -118,7 +118,9 @@ constexpr int lean_ring_elems() {
   return ((2 + R * 3 * WCAP) + 7) / 8 * 8;
 }
 
-template <int R, int DQX, int DQO, int DQE, int WCAP, bool REC>
+// EF: ends-free form (score-0 wavefront over [-pbf, tbf], the alignment ends
+// at the first diagonal whose extended cell satisfies the free ends).
+template <int R, int DQX, int DQO, int DQE, int WCAP, bool REC, bool EF>
 __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int NC = 3;
@@ -178,7 +180,46 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
     int status = 1;
     uint32_t prev_blk = kNoBlock;
     long long dsteps = 0;
-    {
+    int end_k = kend, end_off = tlen;
+    if constexpr (EF) {
+      // Ends-free score 0: (v=0, h=k) for 0 < k <= tbf and (v=-k, h=0) for
+      // -pbf <= k < 0 (wavefront_aligner.c:252-307), each extended, the range
+      // untrimmed; the first diagonal that ends the alignment wins
+      // (wavefront_extend_kernels.c:154-196).
+      const int lo0 = -a.pbf, hi0 = a.tbf;
+      if (hi0 - lo0 + 3 > wcap) {
+        status = kDevRingOverflow;
+      } else {
+        const int b = base_of(0, kM) - (lo0 - 1);  // ring element of diagonal 0
+        int hitk = INT_MAX, hitoff = 0;
+        for (int kb = lo0; kb <= hi0; kb += 64) {
+          const int k = kb + lane;
+          const bool act = k <= hi0;
+          int m = 0;
+          if (act) {
+            m = lcp16(seqP, pb, seqT, tb, plen, tlen, k, k > 0 ? k : 0);
+            ring[b + k] = to_ring(m, ring);
+          }
+          const unsigned long long bal = __ballot(act && endsfree_done(m, k, plen, tlen, a.pef, a.tef));
+          if (bal && hitk == INT_MAX) {
+            const int l = (int)__builtin_ctzll(bal);
+            hitk = kb + l;
+            hitoff = __builtin_amdgcn_readlane(m, l);
+          }
+        }
+        if (lane < 2) ring[b + (lane ? hi0 + 1 : lo0 - 1)] = kRingNull;
+        s_org[0] = lo0 - 1;
+        s_top[0] = hi0 + 1;
+        s_lo[0][kM] = lo0;
+        s_hi[0][kM] = hi0;
+        if (hitk != INT_MAX) {
+          status = kDevCompleted;
+          end_k = hitk;
+          end_off = hitoff;
+        }
+      }
+      wave_sync();
+    } else {
       // Score 0: M holds offset 0 at k = 0 (wavefront_aligner.c:252-380), extended.
       const int v0 = uniform(lcp16(seqP, pb, seqT, tb, plen, tlen, 0, 0));
       s_org[0] = -1;
@@ -250,9 +291,12 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
         // NULL in the two M cells just outside the computed range.
         if (lane < 2) oM[lane ? hi + 1 : lo - 1] = kRingNull;
         // One pass over 64 diagonals starting at kb: compute, record, extend,
-        // store; returns the validity ballots and whether k_end reached the
-        // end of the text (termination, wavefront_termination.c:37-114).
-        auto pass = [&](int kb, unsigned long long& bM, unsigned long long& bI, unsigned long long& bD) -> bool {
+        // store; returns the validity ballots and the ballot of the lanes
+        // that end the alignment: k_end reaching the end of the text
+        // (wavefront_termination.c:37-114), or, ends-free, the free ends
+        // (fin_off: the offset of the first such lane).
+        auto pass = [&](int kb, unsigned long long& bM, unsigned long long& bI, unsigned long long& bD,
+                        int& fin_off) -> unsigned long long {
           const int k = kb + lane;
           const bool act = k <= hi;
           // Reads are clamped, so every lane may read; only active lanes store.
@@ -303,18 +347,34 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
           bM = __ballot(okM);
           bI = __ballot(vI);
           bD = __ballot(vD);
-          return __ballot(okM && k == kend && h >= tlen) != 0;
+          if constexpr (EF) {
+            const unsigned long long fb = __ballot(okM && endsfree_done(h, k, plen, tlen, a.pef, a.tef));
+            if (fb) fin_off = __builtin_amdgcn_readlane(h, (int)__builtin_ctzll(fb));
+            return fb;
+          } else {
+            return __ballot(okM && k == kend && h >= tlen);
+          }
         };
         // Trimmed ranges (wavefront_compute.c:579-613) from the ballots;
         // absent or empty = (1, -1).  Most steps are at most 64 wide: the
         // first pass is straight-line code, further passes loop.
         unsigned long long bM, bI, bD;
-        bool fin = pass(lo, bM, bI, bD);
+        int fin_off = 0;
+        unsigned long long fin = pass(lo, bM, bI, bD, fin_off);
+        int fin_k = (EF && fin) ? lo + (int)__builtin_ctzll(fin) : INT_MAX;
         int mlo = bM ? lo + (int)__builtin_ctzll(bM) : 1, mhi = bM ? lo + 63 - (int)__builtin_clzll(bM) : -1;
         int ilo = bI ? lo + (int)__builtin_ctzll(bI) : 1, ihi = bI ? lo + 63 - (int)__builtin_clzll(bI) : -1;
         int dlo = bD ? lo + (int)__builtin_ctzll(bD) : 1, dhi = bD ? lo + 63 - (int)__builtin_clzll(bD) : -1;
         for (int kb = lo + 64; kb <= hi; kb += 64) {
-          fin = pass(kb, bM, bI, bD) || fin;
+          int off2 = 0;
+          const unsigned long long f2 = pass(kb, bM, bI, bD, off2);
+          if constexpr (EF) {
+            if (f2 && fin_k == INT_MAX) {
+              fin_k = kb + (int)__builtin_ctzll(f2);
+              fin_off = off2;
+            }
+          }
+          fin |= f2;
           if (bM) { if (mlo > mhi) mlo = kb + (int)__builtin_ctzll(bM); mhi = kb + 63 - (int)__builtin_clzll(bM); }
           if (bI) { if (ilo > ihi) ilo = kb + (int)__builtin_ctzll(bI); ihi = kb + 63 - (int)__builtin_clzll(bI); }
           if (bD) { if (dlo > dhi) dlo = kb + (int)__builtin_ctzll(bD); dhi = kb + 63 - (int)__builtin_clzll(bD); }
@@ -327,7 +387,14 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
         wave_sync();  // ring writes of this step before the next step's reads
         // M[k_end] reached the end of the text: a valid cell at k_end lies in
         // the trimmed range, so the lane that computed it decides.
-        if (fin) { status = kDevCompleted; goto pair_done; }
+        if (fin) {
+          status = kDevCompleted;
+          if constexpr (EF) {
+            end_k = fin_k;
+            end_off = fin_off;
+          }
+          goto pair_done;
+        }
       }
     }
   pair_done:
@@ -338,6 +405,10 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
         B->out_score[p] = s;
         B->out_last_block[p] = prev_blk;
         B->out_dsteps[p] = dsteps;
+        if (EF && B->out_end_k) {
+          B->out_end_k[p] = end_k;
+          B->out_end_off[p] = end_off;
+        }
         if (status == kDevRingOverflow || status == kDevArenaFull) atomicAdd(B->retry_count, 1u);
         else if (status != kDevCompleted) atomicAdd(B->fail_count, 1u);
       }
@@ -350,11 +421,12 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
 // Ring widths with an instance (diagonals per slot and component).
 constexpr int kLeanWidths[] = {224, 384, 768};
 
-template <int R, int DQX, int DQO, int DQE>
+template <int R, int DQX, int DQO, int DQE, bool EF>
 const void* lean_fn(int wcap, bool rec) {
-#define WFA_LEAN_W(W)                                                                   \
-  if (wcap == W) return rec ? reinterpret_cast<const void*>(&wfa_forward_lean_kernel<R, DQX, DQO, DQE, W, true>) \
-                            : reinterpret_cast<const void*>(&wfa_forward_lean_kernel<R, DQX, DQO, DQE, W, false>);
+#define WFA_LEAN_W(W)                                                                                 \
+  if (wcap == W)                                                                                      \
+    return rec ? reinterpret_cast<const void*>(&wfa_forward_lean_kernel<R, DQX, DQO, DQE, W, true, EF>) \
+               : reinterpret_cast<const void*>(&wfa_forward_lean_kernel<R, DQX, DQO, DQE, W, false, EF>);
   WFA_LEAN_W(224) WFA_LEAN_W(384) WFA_LEAN_W(768)
 #undef WFA_LEAN_W
   return nullptr;
@@ -368,9 +440,9 @@ bool lean_structure(const Penalties& pen) {
   return R == 5 && dx == 2 && dq == 4 && de == 1;
 }
 
-const void* select_lean(const Penalties& pen, int wcap, bool rec) {
+const void* select_lean(const Penalties& pen, int wcap, bool rec, bool ef) {
   if (!lean_structure(pen)) return nullptr;
-  return lean_fn<5, 2, 4, 1>(wcap, rec);
+  return ef ? lean_fn<5, 2, 4, 1, true>(wcap, rec) : lean_fn<5, 2, 4, 1, false>(wcap, rec);
 }
 
 }  // namespace
@@ -389,7 +461,7 @@ size_t lean_lds_bytes(const Penalties& pen, int wcap, int lmax_words) {
 }
 
 int lean_max_blocks_per_cu(const Penalties& pen, int wcap, size_t lds_bytes) {
-  const void* f = select_lean(pen, wcap, true);
+  const void* f = select_lean(pen, wcap, true, false);
   if (!f) return 0;
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 64, lds_bytes) != hipSuccess) return 0;
@@ -397,7 +469,7 @@ int lean_max_blocks_per_cu(const Penalties& pen, int wcap, size_t lds_bytes) {
 }
 
 hipError_t launch_forward_lean(const ForwardArgs& a, const LaunchShape& shape, hipStream_t stream) {
-  const void* f = select_lean(a.pen, a.wcap, a.records != 0);
+  const void* f = select_lean(a.pen, a.wcap, a.records != 0, a.endsfree != 0);
   if (!f || shape.block != 64) return hipErrorInvalidValue;
   if (shape.lds_bytes > 48 * 1024) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shape.lds_bytes);
