@@ -76,6 +76,59 @@ __device__ __forceinline__ int lean_rd_g(const int16_t* ring, const LeanSrc& s, 
   return c == k ? v : (int)kRingNull;
 }
 
+// The choices of cell k over clamp-read sources (winning M offset, its
+// source, the gap-extension flags), ties as the compute step breaks them;
+// reads below 0 count as the one NULL offset (the generic kernel's
+// choice_key over the lean ring).
+__device__ __forceinline__ unsigned long long lean_choice_key(const int16_t* ring, const LeanSrc& mx, const LeanSrc& mo,
+                                                              const LeanSrc& i1, const LeanSrc& d1, int k) {
+  auto rdn = [&](const LeanSrc& sr, int kk) {
+    const int v = lean_rd(ring, sr, kk);
+    return v < 0 ? kOffsetNull : v;
+  };
+  int best = rdn(mx, k) + 1;
+  uint32_t src = kSrcX;
+  const int oml = rdn(mo, k - 1), omr = rdn(mo, k + 1), eil = rdn(i1, k - 1), edr = rdn(d1, k + 1);
+  const uint32_t bits = (eil >= oml ? kRecI1Ext : 0u) | (edr >= omr ? kRecD1Ext : 0u);
+  const int del1 = max(omr, edr), ins1 = max(oml, eil) + 1;
+  if (del1 > best) { best = del1; src = kSrcD1; }
+  if (ins1 > best) { best = ins1; src = kSrcI1; }
+  return ((unsigned long long)(unsigned)best << 32) | (src | bits);
+}
+
+// wf-adaptive cut-off (wavefront_heuristic_wfadaptive, wavefront_heuristic.c:
+// 190-290) of the extended M wavefront [lo, hi] of one wave (m[k] its ring
+// cells): the least distance to the end over the wave, then the first and
+// last diagonals within thr of it by ballot, never scanning past the end
+// diagonal; the same result as the generic kernel's wfadaptive_cut.
+__device__ __forceinline__ void lean_wfadaptive(const int16_t* m, int lo, int hi, int plen, int tlen, int thr,
+                                                int* clo, int* chi) {
+  const int lane = threadIdx.x;
+  int dmin = max(plen, tlen);
+  for (int kb = lo; kb <= hi; kb += 64) {
+    const int k = kb + lane;
+    if (k <= hi) dmin = min(dmin, end_distance((int)m[k], k, plen, tlen, false, 0));
+  }
+  for (int o = 32; o > 0; o >>= 1) dmin = min(dmin, __shfl_xor(dmin, o, 64));
+  dmin = uniform(dmin);
+  int first = INT_MAX, last = INT_MIN;
+  for (int kb = lo; kb <= hi; kb += 64) {
+    const int k = kb + lane;
+    const bool keep = k <= hi && end_distance((int)m[k], k, plen, tlen, false, 0) - dmin <= thr;
+    const unsigned long long bal = __ballot(keep);
+    if (bal) {
+      if (first == INT_MAX) first = kb + (int)__builtin_ctzll(bal);
+      last = kb + 63 - (int)__builtin_clzll(bal);
+    }
+  }
+  const int kend = tlen - plen;
+  const int top = min(kend, hi);
+  const int lo_r = lo >= top ? lo : min(first, top);
+  const int bottom = max(kend, lo_r);
+  *clo = lo_r;
+  *chi = hi <= bottom ? hi : max(last, bottom);
+}
+
 // 16 symbols of a 2-bit packed sequence starting at symbol pos (dword view).
 __device__ __forceinline__ uint32_t win16(const uint32_t* W, int pos) {
   const unsigned w = (unsigned)pos >> 4;
@@ -120,10 +173,21 @@ constexpr int lean_ring_elems() {
 
 // EF: ends-free form (score-0 wavefront over [-pbf, tbf], the alignment ends
 // at the first diagonal whose extended cell satisfies the free ends).
-template <int R, int DQX, int DQO, int DQE, int WCAP, bool REC, bool EF>
+// CUT: the wf-adaptive heuristic (wavefront_heuristic.c:190-290, 506-561).
+// The slot ranges then hold what the cut-off left (the limits of the
+// wavefronts computed from them), while reads still see every cell of the
+// trimmed range: gap cells outside it are overwritten with NULL after each
+// step, so every input is read with one clamp, like M.  `emptied` marks the
+// wavefronts a cut-off emptied that hold readable cells: the reference's
+// backtrace probes those (wavefront_backtrace.c:64-219), so a step reading
+// one rewrites its record bytes with the probed choices, as the generic
+// kernel's probed_records does.
+template <int R, int DQX, int DQO, int DQE, int WCAP, bool REC, bool EF, bool CUT>
 __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int NC = 3;
+  static_assert(!(EF && CUT), "ends-free with a heuristic runs on the generic kernel");
+  static_assert(R * NC <= 32, "emptied mask");
   static_assert(DQX > 0 && DQO > 0 && DQE > 0 && DQX < R && DQO < R && DQE < R, "slot distances");
   const int lane = threadIdx.x;
   const int g = a.pen.g;
@@ -181,6 +245,9 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
     uint32_t prev_blk = kNoBlock;
     long long dsteps = 0;
     int end_k = kend, end_off = tlen;
+    int steps_wait = a.heur.steps;  // wavefront_heuristic_clear (wavefront_heuristic.c:114-121)
+    int nulls = 0;                  // null steps in a row, every score counted (align_status.num_null_steps)
+    uint32_t emptied = 0;           // bit slot*3+c: emptied by a cut-off, readable by the backtrace
     if constexpr (EF) {
       // Ends-free score 0: (v=0, h=k) for 0 < k <= tbf and (v=-k, h=0) for
       // -pbf <= k < 0 (wavefront_aligner.c:252-307), each extended, the range
@@ -230,6 +297,13 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
       if (lane < 3) ring[b + lane] = (lane == 1) ? to_ring(v0, ring) : kRingNull;
       if (kend == 0 && v0 >= tlen) status = kDevCompleted;
       wave_sync();
+      // the heuristic also runs after the score-0 extension
+      // (wavefront_extend.c:112-118); a one-diagonal range at k = 0 keeps
+      // its range under the cut-off, so only the countdown moves
+      if (CUT && status == 1) {
+        KernArgs H = args_now();
+        if (--steps_wait <= 0 && 1 >= H->heur.min_len) steps_wait = H->heur.steps;
+      }
     }
 
     if (status != 1) goto pair_done;
@@ -238,18 +312,39 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
       for (int j = 1; j <= R; ++j) {
         const int ph = j % R;  // slot of this step (compile-time after unrolling)
         s += g;
+        if constexpr (CUT) {
+          // The scores between two multiples of g hold no wavefront: each is
+          // a null step of the reference, whose extension ends the alignment
+          // as unreachable once the run exceeds the score scope
+          // (wavefront_extend.c:94-101) unless the step limit comes first.
+          KernArgs H = args_now();
+          const int scope = H->pen.scope;
+          if (nulls + g - 1 > scope) {
+            s = s - g + scope + 1 - nulls;
+            status = (s >= H->max_steps) ? kDevMaxSteps : kDevUnreachable;
+            goto pair_done;
+          }
+          nulls += g - 1;
+        }
         if (s >= s_stop) { status = s_stop == s_cap ? kDevBadTrace : kDevMaxSteps; goto pair_done; }
         const int rx = (ph - DQX + R) % R, ro = (ph - DQO + R) % R, re = (ph - DQE + R) % R;
         const LeanSrc mx = lean_src_m(s_org[rx], s_top[rx], s_lo[rx][kM], s_hi[rx][kM], base_of(rx, kM));
         const LeanSrc mo = lean_src_m(s_org[ro], s_top[ro], s_lo[ro][kM], s_hi[ro][kM], base_of(ro, kM));
-        const LeanSrc i1 = lean_src_g(s_org[re], s_lo[re][kI1], s_hi[re][kI1], base_of(re, kI1));
-        const LeanSrc d1 = lean_src_g(s_org[re], s_lo[re][kD1], s_hi[re][kD1], base_of(re, kD1));
+        const LeanSrc i1 = CUT ? lean_src_m(s_org[re], s_top[re], s_lo[re][kI1], s_hi[re][kI1], base_of(re, kI1))
+                               : lean_src_g(s_org[re], s_lo[re][kI1], s_hi[re][kI1], base_of(re, kI1));
+        const LeanSrc d1 = CUT ? lean_src_m(s_org[re], s_top[re], s_lo[re][kD1], s_hi[re][kD1], base_of(re, kD1))
+                               : lean_src_g(s_org[re], s_lo[re][kD1], s_hi[re][kD1], base_of(re, kD1));
         const bool vx = mx.llo <= mx.lhi, vo = mo.llo <= mo.lhi, vi = i1.llo <= i1.lhi, vd = d1.llo <= d1.lhi;
         if (!(vx || vo || vi || vd)) {  // null step (wavefront_compute_affine.c:236-243)
+          if constexpr (CUT) {
+            if (++nulls > args_now()->pen.scope) { status = kDevUnreachable; goto pair_done; }
+            emptied &= ~(7u << (3 * ph));
+          }
 #pragma unroll
           for (int c = 0; c < NC; ++c) { s_lo[ph][c] = 1; s_hi[ph][c] = -1; }
           continue;
         }
+        if constexpr (CUT) nulls = 0;
         // wavefront_compute_limits_input (wavefront_compute.c:40-86)
         const int lo = min(min(mx.llo, mo.llo - 1), min(i1.llo + 1, d1.llo - 1));
         const int hi = max(max(mx.lhi, mo.lhi + 1), max(i1.lhi + 1, d1.lhi - 1));
@@ -288,8 +383,16 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
         int16_t* oM = ring + base_of(ph, kM) - org;
         int16_t* oI = ring + base_of(ph, kI1) - org;
         int16_t* oD = ring + base_of(ph, kD1) - org;
-        // NULL in the two M cells just outside the computed range.
-        if (lane < 2) oM[lane ? hi + 1 : lo - 1] = kRingNull;
+        // NULL in the two M cells just outside the computed range (and in
+        // the gap cells there when every input is clamp-read).
+        if (lane < 2) {
+          const int kk = lane ? hi + 1 : lo - 1;
+          oM[kk] = kRingNull;
+          if constexpr (CUT) {
+            oI[kk] = kRingNull;
+            oD[kk] = kRingNull;
+          }
+        }
         // One pass over 64 diagonals starting at kb: compute, record, extend,
         // store; returns the validity ballots and the ballot of the lanes
         // that end the alignment: k_end reaching the end of the text
@@ -302,8 +405,8 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
           // Reads are clamped, so every lane may read; only active lanes store.
           const int om_l = lean_rd(ring, mo, k - 1);
           const int om_r = lean_rd(ring, mo, k + 1);
-          const int ei1 = lean_rd_g(ring, i1, k - 1);
-          const int ed1 = lean_rd_g(ring, d1, k + 1);
+          const int ei1 = CUT ? lean_rd(ring, i1, k - 1) : lean_rd_g(ring, i1, k - 1);
+          const int ed1 = CUT ? lean_rd(ring, d1, k + 1) : lean_rd_g(ring, d1, k + 1);
           const int ins1 = max(om_l, ei1) + 1;
           const int del1 = max(om_r, ed1);
           int best = lean_rd(ring, mx, k) + 1;
@@ -379,6 +482,50 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
           if (bI) { if (ilo > ihi) ilo = kb + (int)__builtin_ctzll(bI); ihi = kb + 63 - (int)__builtin_clzll(bI); }
           if (bD) { if (dlo > dhi) dlo = kb + (int)__builtin_ctzll(bD); dhi = kb + 63 - (int)__builtin_clzll(bD); }
         }
+        if constexpr (CUT) {
+          // gap cells outside their trimmed range read NULL from now on
+          for (int kb = lo; kb <= hi; kb += 64) {
+            const int k = kb + lane;
+            if (k <= hi) {
+              if (has_i1 && (k < ilo || k > ihi)) oI[k] = kRingNull;
+              if (has_d1 && (k < dlo || k > dhi)) oD[k] = kRingNull;
+            }
+          }
+          if constexpr (REC) {
+            // An input a cut-off emptied still has cells the reference's
+            // backtrace probes: the record bytes take the probed choices and
+            // flag the cells whose trace would take operations back
+            // (probed_record of the generic kernel).
+            const uint32_t smask = (1u << (3 * rx + kM)) | (1u << (3 * ro + kM)) | (1u << (3 * re + kI1)) |
+                                   (1u << (3 * re + kD1));
+            if (emptied & smask) {
+              wave_sync();  // this step's M cells
+              auto bview = [&](const LeanSrc& c, int slot, int comp) {
+                LeanSrc b = c;
+                if ((emptied >> (3 * slot + comp)) & 1u) {
+                  b.clo = s_org[slot];
+                  b.chi = s_top[slot];
+                  b.sb = base_of(slot, comp) - s_org[slot];
+                }
+                return b;
+              };
+              const LeanSrc bx = bview(mx, rx, kM), bo = bview(mo, ro, kM), bi = bview(i1, re, kI1),
+                            bd = bview(d1, re, kD1);
+              for (int kb = lo; kb <= hi; kb += 64) {
+                const int k = kb + lane;
+                if (k <= hi) {
+                  const unsigned long long kbk = lean_choice_key(ring, bx, bo, bi, bd, k);
+                  const unsigned long long kck = lean_choice_key(ring, mx, mo, i1, d1, k);
+                  const int q = (int)(unsigned)(kbk >> 32), best = (int)(unsigned)(kck >> 32);
+                  uint32_t r = (uint32_t)kbk & 0x7Fu;
+                  if (q >= 0 && q > max((int)oM[k], best)) r |= kRecStale;
+                  rec[(unsigned)(k - lo)] = (uint8_t)r;
+                }
+              }
+            }
+          }
+          emptied &= ~(7u << (3 * ph));
+        }
         s_org[ph] = org;
         s_top[ph] = hi + 1;
         s_lo[ph][kM] = mlo; s_hi[ph][kM] = mhi;
@@ -394,6 +541,41 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
             end_off = fin_off;
           }
           goto pair_done;
+        }
+        if constexpr (CUT) {
+          // wavefront_heuristic_cufoff (wavefront_heuristic.c:506-561) on the
+          // trimmed M range: every steps-th non-empty wavefront of at least
+          // min_len diagonals; when M changed, I1/D1 are narrowed to M's
+          // range (:163-171).
+          if (mlo <= mhi) {
+            KernArgs H = args_now();
+            --steps_wait;
+            if (steps_wait <= 0 && mhi - mlo + 1 >= H->heur.min_len) {
+              steps_wait = H->heur.steps;
+              int clo, chi;
+              lean_wfadaptive(oM, mlo, mhi, plen, tlen, H->heur.thr, &clo, &chi);
+              if (clo != mlo || chi != mhi) {
+                if (clo > chi) {
+                  s_lo[ph][kM] = 1; s_hi[ph][kM] = -1;
+                  emptied |= 1u << (3 * ph + kM);
+                } else {
+                  s_lo[ph][kM] = clo; s_hi[ph][kM] = chi;
+                }
+#pragma unroll
+                for (int c = 1; c < NC; ++c) {
+                  if (s_lo[ph][c] <= s_hi[ph][c]) {
+                    const int nlo = max(s_lo[ph][c], clo), nhi = min(s_hi[ph][c], chi);
+                    if (nlo > nhi) {
+                      s_lo[ph][c] = 1; s_hi[ph][c] = -1;
+                      emptied |= 1u << (3 * ph + c);
+                    } else {
+                      s_lo[ph][c] = nlo; s_hi[ph][c] = nhi;
+                    }
+                  }
+                }
+              }
+            }
+          }
         }
       }
     }
@@ -413,7 +595,8 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
         else if (status != kDevCompleted) atomicAdd(B->fail_count, 1u);
       }
     }
-    if (status == kDevCompleted || status == kDevMaxSteps) wg_dsteps += (unsigned long long)dsteps;
+    if (status == kDevCompleted || status == kDevMaxSteps || status == kDevUnreachable)
+      wg_dsteps += (unsigned long long)dsteps;
   }
   if (lane == 0 && wg_dsteps) atomicAdd(args_now()->dsteps_total, wg_dsteps);
 }
@@ -421,12 +604,12 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
 // Ring widths with an instance (diagonals per slot and component).
 constexpr int kLeanWidths[] = {224, 384, 768};
 
-template <int R, int DQX, int DQO, int DQE, bool EF>
+template <int R, int DQX, int DQO, int DQE, bool EF, bool CUT>
 const void* lean_fn(int wcap, bool rec) {
-#define WFA_LEAN_W(W)                                                                                 \
-  if (wcap == W)                                                                                      \
-    return rec ? reinterpret_cast<const void*>(&wfa_forward_lean_kernel<R, DQX, DQO, DQE, W, true, EF>) \
-               : reinterpret_cast<const void*>(&wfa_forward_lean_kernel<R, DQX, DQO, DQE, W, false, EF>);
+#define WFA_LEAN_W(W)                                                                                      \
+  if (wcap == W)                                                                                           \
+    return rec ? reinterpret_cast<const void*>(&wfa_forward_lean_kernel<R, DQX, DQO, DQE, W, true, EF, CUT>) \
+               : reinterpret_cast<const void*>(&wfa_forward_lean_kernel<R, DQX, DQO, DQE, W, false, EF, CUT>);
   WFA_LEAN_W(224) WFA_LEAN_W(384) WFA_LEAN_W(768)
 #undef WFA_LEAN_W
   return nullptr;
@@ -440,9 +623,10 @@ bool lean_structure(const Penalties& pen) {
   return R == 5 && dx == 2 && dq == 4 && de == 1;
 }
 
-const void* select_lean(const Penalties& pen, int wcap, bool rec, bool ef) {
-  if (!lean_structure(pen)) return nullptr;
-  return ef ? lean_fn<5, 2, 4, 1, true>(wcap, rec) : lean_fn<5, 2, 4, 1, false>(wcap, rec);
+const void* select_lean(const Penalties& pen, int wcap, bool rec, bool ef, bool cut) {
+  if (!lean_structure(pen) || (ef && cut)) return nullptr;
+  if (cut) return lean_fn<5, 2, 4, 1, false, true>(wcap, rec);
+  return ef ? lean_fn<5, 2, 4, 1, true, false>(wcap, rec) : lean_fn<5, 2, 4, 1, false, false>(wcap, rec);
 }
 
 }  // namespace
@@ -461,7 +645,7 @@ size_t lean_lds_bytes(const Penalties& pen, int wcap, int lmax_words) {
 }
 
 int lean_max_blocks_per_cu(const Penalties& pen, int wcap, size_t lds_bytes) {
-  const void* f = select_lean(pen, wcap, true, false);
+  const void* f = select_lean(pen, wcap, true, false, false);
   if (!f) return 0;
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 64, lds_bytes) != hipSuccess) return 0;
@@ -469,7 +653,7 @@ int lean_max_blocks_per_cu(const Penalties& pen, int wcap, size_t lds_bytes) {
 }
 
 hipError_t launch_forward_lean(const ForwardArgs& a, const LaunchShape& shape, hipStream_t stream) {
-  const void* f = select_lean(a.pen, a.wcap, a.records != 0, a.endsfree != 0);
+  const void* f = select_lean(a.pen, a.wcap, a.records != 0, a.endsfree != 0, a.heur.cut() != 0);
   if (!f || shape.block != 64) return hipErrorInvalidValue;
   if (shape.lds_bytes > 48 * 1024) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shape.lds_bytes);
