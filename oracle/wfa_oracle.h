@@ -5,7 +5,8 @@
  * gap-linear, gap-affine and dual-cost gap-affine, end-to-end and ends-free
  * (with the extension form), in memory mode high (and med/low, whose
  * CIGARs the reference asserts equal memory-high: tests/wfa.utest.sh:67-78)
- * and ultralow (BiWFA, end-to-end), exact or with the wf-adaptive heuristic.
+ * and ultralow (BiWFA, end-to-end), exact or with the heuristics (wf-adaptive,
+ * wfmash, banded static / adaptive, x-drop, z-drop; wavefront_heuristic.c).
  * Ends-free termination takes the first
  * diagonal in ascending order, as the reference's scalar extend kernel
  * (wavefront_extend_kernels.c:154-196).  It is the checker for the HIP path; only tests/,
