@@ -41,14 +41,15 @@ struct LeanSrc {
 };
 constexpr int kFarOff = 1 << 20;  // clamp bounds of an absent gap input (reads land on the NULL cell)
 
-// M input of a slot with computed range [org+1, top-1] and trimmed [lo, hi].
-__device__ __forceinline__ LeanSrc lean_src_m(int org, int top, int lo, int hi, int base) {
+// M input of a slot whose ring origin is org, with limits [lo, hi], read
+// through a clamp to [cl, ch] (cells holding NULL at both bounds).
+__device__ __forceinline__ LeanSrc lean_src_m(int cl, int ch, int lo, int hi, int base, int org) {
   LeanSrc r;
   const bool v = lo <= hi;
   r.llo = v ? lo : 1;
   r.lhi = v ? hi : -1;
-  r.clo = v ? org : 0;
-  r.chi = v ? top : 0;
+  r.clo = v ? cl : 0;
+  r.chi = v ? ch : 0;
   r.sb = v ? base - org : 0;  // element 0 of the ring is a NULL cell
   return r;
 }
@@ -243,7 +244,7 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
     int s = 0;
     int status = 1;
     uint32_t prev_blk = kNoBlock;
-    long long dsteps = 0;
+    int dsteps = 0;  // below 2^31 for pairs of at most 1,024 bases
     int end_k = kend, end_off = tlen;
     int steps_wait = a.heur.steps;  // wavefront_heuristic_clear (wavefront_heuristic.c:114-121)
     int nulls = 0;                  // null steps in a row, every score counted (align_status.num_null_steps)
@@ -328,11 +329,18 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
         }
         if (s >= s_stop) { status = s_stop == s_cap ? kDevBadTrace : kDevMaxSteps; goto pair_done; }
         const int rx = (ph - DQX + R) % R, ro = (ph - DQO + R) % R, re = (ph - DQE + R) % R;
-        const LeanSrc mx = lean_src_m(s_org[rx], s_top[rx], s_lo[rx][kM], s_hi[rx][kM], base_of(rx, kM));
-        const LeanSrc mo = lean_src_m(s_org[ro], s_top[ro], s_lo[ro][kM], s_hi[ro][kM], base_of(ro, kM));
-        const LeanSrc i1 = CUT ? lean_src_m(s_org[re], s_top[re], s_lo[re][kI1], s_hi[re][kI1], base_of(re, kI1))
+        // Without a cut-off the M ranges are the trimmed ranges, whose
+        // neighbouring cells hold NULL (a pad or a cell trimmed away), so the
+        // clamp may use them in place of the computed range.
+        const LeanSrc mx = lean_src_m(CUT ? s_org[rx] : s_lo[rx][kM] - 1, CUT ? s_top[rx] : s_hi[rx][kM] + 1,
+                                      s_lo[rx][kM], s_hi[rx][kM], base_of(rx, kM), s_org[rx]);
+        const LeanSrc mo = lean_src_m(CUT ? s_org[ro] : s_lo[ro][kM] - 1, CUT ? s_top[ro] : s_hi[ro][kM] + 1,
+                                      s_lo[ro][kM], s_hi[ro][kM], base_of(ro, kM), s_org[ro]);
+        const LeanSrc i1 = CUT ? lean_src_m(s_org[re], s_top[re], s_lo[re][kI1], s_hi[re][kI1], base_of(re, kI1),
+                                            s_org[re])
                                : lean_src_g(s_org[re], s_lo[re][kI1], s_hi[re][kI1], base_of(re, kI1));
-        const LeanSrc d1 = CUT ? lean_src_m(s_org[re], s_top[re], s_lo[re][kD1], s_hi[re][kD1], base_of(re, kD1))
+        const LeanSrc d1 = CUT ? lean_src_m(s_org[re], s_top[re], s_lo[re][kD1], s_hi[re][kD1], base_of(re, kD1),
+                                            s_org[re])
                                : lean_src_g(s_org[re], s_lo[re][kD1], s_hi[re][kD1], base_of(re, kD1));
         const bool vx = mx.llo <= mx.lhi, vo = mo.llo <= mo.lhi, vi = i1.llo <= i1.lhi, vd = d1.llo <= d1.lhi;
         if (!(vx || vo || vi || vd)) {  // null step (wavefront_compute_affine.c:236-243)
@@ -360,10 +368,11 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
           if (chunk_cur + units > chunk_end) {
             const uint32_t want = units > kChunkUnits ? units : kChunkUnits;
             unsigned long long b0 = 0;
-            if (lane == 0) b0 = atomicAdd(a.rec_top, (unsigned long long)want);
+            KernArgs C = args_now();  // the arena's fields only on this rare path
+            if (lane == 0) b0 = atomicAdd(C->rec_top, (unsigned long long)want);
             const unsigned blo = (unsigned)lane_of((int)(unsigned)(b0 & 0xFFFFFFFFull), 0);
             const unsigned bhi = (unsigned)lane_of((int)(unsigned)(b0 >> 32), 0);
-            if (bhi != 0 || (unsigned long long)blo + want > a.rec_units) {
+            if (bhi != 0 || (unsigned long long)blo + want > C->rec_units) {
               chunk_cur = chunk_end = 0;
               status = kDevArenaFull;
               goto pair_done;
@@ -586,7 +595,7 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
         B->out_status[p] = status;
         B->out_score[p] = s;
         B->out_last_block[p] = prev_blk;
-        B->out_dsteps[p] = dsteps;
+        B->out_dsteps[p] = (long long)dsteps;
         if (EF && B->out_end_k) {
           B->out_end_k[p] = end_k;
           B->out_end_off[p] = end_off;
