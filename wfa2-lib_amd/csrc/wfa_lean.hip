@@ -98,24 +98,26 @@ __device__ __forceinline__ unsigned long long lean_choice_key(const int16_t* rin
 }
 
 // wf-adaptive cut-off (wavefront_heuristic_wfadaptive, wavefront_heuristic.c:
-// 190-290) of the extended M wavefront [lo, hi] of one wave (m[k] its ring
-// cells): the least distance to the end over the wave, then the first and
-// last diagonals within thr of it by ballot, never scanning past the end
-// diagonal; the same result as the generic kernel's wfadaptive_cut.
+// 190-290; weighted: wfmash, :134-145) of the extended M wavefront [lo, hi]
+// of one wave (m[k] its ring cells): the least distance to the end over the
+// wave, then the first and last diagonals within thr of it by ballot, never
+// scanning past the end diagonal; the same result as the generic kernel's
+// wfadaptive_cut.
 __device__ __forceinline__ void lean_wfadaptive(const int16_t* m, int lo, int hi, int plen, int tlen, int thr,
-                                                int* clo, int* chi) {
+                                                bool weighted, int* clo, int* chi) {
   const int lane = threadIdx.x;
+  const int mfactor = weighted ? trunc_to_int((float)(plen + tlen) / 2.0f) : 0;
   int dmin = max(plen, tlen);
   for (int kb = lo; kb <= hi; kb += 64) {
     const int k = kb + lane;
-    if (k <= hi) dmin = min(dmin, end_distance((int)m[k], k, plen, tlen, false, 0));
+    if (k <= hi) dmin = min(dmin, end_distance((int)m[k], k, plen, tlen, weighted, mfactor));
   }
   for (int o = 32; o > 0; o >>= 1) dmin = min(dmin, __shfl_xor(dmin, o, 64));
   dmin = uniform(dmin);
   int first = INT_MAX, last = INT_MIN;
   for (int kb = lo; kb <= hi; kb += 64) {
     const int k = kb + lane;
-    const bool keep = k <= hi && end_distance((int)m[k], k, plen, tlen, false, 0) - dmin <= thr;
+    const bool keep = k <= hi && end_distance((int)m[k], k, plen, tlen, weighted, mfactor) - dmin <= thr;
     const unsigned long long bal = __ballot(keep);
     if (bal) {
       if (first == INT_MAX) first = kb + (int)__builtin_ctzll(bal);
@@ -174,7 +176,8 @@ constexpr int lean_ring_elems() {
 
 // EF: ends-free form (score-0 wavefront over [-pbf, tbf], the alignment ends
 // at the first diagonal whose extended cell satisfies the free ends).
-// CUT: the wf-adaptive heuristic (wavefront_heuristic.c:190-290, 506-561).
+// CUT: the heuristic cut-offs other than the drops: wf-adaptive or wfmash,
+// then a static or adaptive band (wavefront_heuristic.c:190-290, 451-561).
 // The slot ranges then hold what the cut-off left (the limits of the
 // wavefronts computed from them), while reads still see every cell of the
 // trimmed range: gap cells outside it are overwritten with NULL after each
@@ -249,6 +252,29 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
     int steps_wait = a.heur.steps;  // wavefront_heuristic_clear (wavefront_heuristic.c:114-121)
     int nulls = 0;                  // null steps in a row, every score counted (align_status.num_null_steps)
     uint32_t emptied = 0;           // bit slot*3+c: emptied by a cut-off, readable by the backtrace
+    // wavefront_heuristic_cufoff (wavefront_heuristic.c:506-561) on the
+    // trimmed M range [mlo, mhi] (m[k]: its cells), one step countdown for
+    // every strategy: wf-adaptive / wfmash cut every steps-th wavefront of at
+    // least min_len diagonals; then the band: static clamps every time,
+    // adaptive moves a band-wide window.  The kept range goes to clo, chi.
+    auto cut_range = [&](const int16_t* m, int mlo, int mhi, int& clo, int& chi) {
+      KernArgs H = args_now();
+      --steps_wait;
+      clo = mlo;
+      chi = mhi;
+      const int wa = H->heur.wfadaptive;
+      if (wa && steps_wait <= 0 && mhi - mlo + 1 >= H->heur.min_len) {
+        lean_wfadaptive(m, mlo, mhi, plen, tlen, H->heur.thr, wa == 2, &clo, &chi);
+        steps_wait = H->heur.steps;
+      }
+      const int band = H->heur.banded;
+      if (band == 1) {
+        clo = max(clo, H->heur.band_lo);
+        chi = min(chi, H->heur.band_hi);
+      } else if (band == 2) {
+        banded_adaptive_cut(m, H->heur.band_lo, H->heur.band_hi, plen, tlen, steps_wait, H->heur.steps, &clo, &chi);
+      }
+    };
     if constexpr (EF) {
       // Ends-free score 0: (v=0, h=k) for 0 < k <= tbf and (v=-k, h=0) for
       // -pbf <= k < 0 (wavefront_aligner.c:252-307), each extended, the range
@@ -299,11 +325,18 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
       if (kend == 0 && v0 >= tlen) status = kDevCompleted;
       wave_sync();
       // the heuristic also runs after the score-0 extension
-      // (wavefront_extend.c:112-118); a one-diagonal range at k = 0 keeps
-      // its range under the cut-off, so only the countdown moves
+      // (wavefront_extend.c:112-118)
       if (CUT && status == 1) {
-        KernArgs H = args_now();
-        if (--steps_wait <= 0 && 1 >= H->heur.min_len) steps_wait = H->heur.steps;
+        int clo, chi;
+        cut_range(ring + b + 1, 0, 0, clo, chi);
+        if (clo != 0 || chi != 0) {
+          if (clo > chi) {
+            s_lo[0][kM] = 1; s_hi[0][kM] = -1;
+            emptied |= 1u << kM;
+          } else {
+            s_lo[0][kM] = clo; s_hi[0][kM] = chi;
+          }
+        }
       }
     }
 
@@ -552,17 +585,12 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
           goto pair_done;
         }
         if constexpr (CUT) {
-          // wavefront_heuristic_cufoff (wavefront_heuristic.c:506-561) on the
-          // trimmed M range: every steps-th non-empty wavefront of at least
-          // min_len diagonals; when M changed, I1/D1 are narrowed to M's
-          // range (:163-171).
+          // The cut-off of a non-empty M wavefront; when M changed, I1/D1
+          // are narrowed to M's range (wavefront_heuristic.c:163-171).
           if (mlo <= mhi) {
-            KernArgs H = args_now();
-            --steps_wait;
-            if (steps_wait <= 0 && mhi - mlo + 1 >= H->heur.min_len) {
-              steps_wait = H->heur.steps;
+            {
               int clo, chi;
-              lean_wfadaptive(oM, mlo, mhi, plen, tlen, H->heur.thr, &clo, &chi);
+              cut_range(oM, mlo, mhi, clo, chi);
               if (clo != mlo || chi != mhi) {
                 if (clo > chi) {
                   s_lo[ph][kM] = 1; s_hi[ph][kM] = -1;
