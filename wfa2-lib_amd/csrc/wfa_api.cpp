@@ -597,14 +597,18 @@ LaunchShape shape_of(Engine* e, const PlanGroup& g, int bits, const Penalties& p
   return shape;
 }
 
-// Enqueues forward + backtrace of one group on the aligner's stream.
+// Enqueues forward + backtrace of one group on the aligner's stream; with
+// fresh, the run counters were just cleared and need no reset of their own.
+// A group that takes jobs 0..n-1 in order passes no order array.
 int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGroup& grp, const Penalties& pen,
-                  int records) {
+                  int records, bool fresh) {
   const int n = (int)grp.ids.size();
   if (n == 0) return WF_STATUS_OK;
   hipStream_t st = e->stream;
   const LaunchShape shape = shape_of(e, grp, js.bits, pen, js.heur.cut());
-  if (!grp.ids_on_device) {
+  bool ident = true;
+  for (int i = 0; i < n && ident; ++i) ident = grp.ids[i] == i;
+  if (!ident && !grp.ids_on_device) {
     if (!grp.d_ids.ensure((size_t)n * 4)) return WF_STATUS_OOM;
     if (!hip_ok(hipMemcpyAsync(grp.d_ids.p, grp.ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, st), "plan upload"))
       return WF_STATUS_OOM;
@@ -615,7 +619,7 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
   wfa::Counters* c = counters_of(e);
   ForwardArgs fa{};
   fa.n_pairs = n;
-  fa.order = grp.d_ids.as<int>();
+  fa.order = ident ? nullptr : grp.d_ids.as<int>();
   fa.plen = js.plen;
   fa.tlen = js.tlen;
   fa.words = js.words;
@@ -653,7 +657,7 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
   fa.out_end_rec = js.end_rec;
   fa.heur = js.heur;
   fa.pen = pen;
-  if (!hip_ok(wfa::launch_reset_counters(c, 0, st), "reset")) return WF_STATUS_OOM;
+  if (!fresh && !hip_ok(wfa::launch_reset_counters(c, 0, st), "reset")) return WF_STATUS_OOM;
   (void)hipEventRecord(grp.ev[0], st);
   const hipError_t lerr = grp.family == wfa::kLean ? wfa::launch_forward_lean(fa, shape, st)
                                                    : wfa::launch_forward(fa, pen.ncomp, js.bits, shape, st);
@@ -662,7 +666,7 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
   if (records) {
     BacktraceArgs ba{};
     ba.n_pairs = n;
-    ba.order = grp.d_ids.as<int>();
+    ba.order = ident ? nullptr : grp.d_ids.as<int>();
     ba.plen = js.plen;
     ba.tlen = js.tlen;
     ba.status = js.status;
@@ -808,10 +812,13 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
       return WF_STATUS_OOM;
     }
     if (!reserve_for(e, *pending, js.bits, pen, js.heur.cut(), records)) return WF_STATUS_OOM;
-    if (!hip_ok(wfa::launch_reset_counters(counters_of(e), 1, st), "reset")) return WF_STATUS_OOM;
+    // the reset before the first round clears every counter
+    if (round > 0 && !hip_ok(wfa::launch_reset_counters(counters_of(e), 1, st), "reset")) return WF_STATUS_OOM;
+    bool fresh = true;
     for (PlanGroup& g : *pending) {
-      const int rc = enqueue_group(wf, e, js, g, pen, records);
+      const int rc = enqueue_group(wf, e, js, g, pen, records, fresh);
       if (rc != WF_STATUS_OK) return rc;
+      if (!g.ids.empty()) fresh = false;
       ++tot->launches;
     }
     if (!hip_ok(hipMemcpyAsync(&host_c, e->counters.p, sizeof(host_c), hipMemcpyDeviceToHost, st), "counters") ||
@@ -889,24 +896,40 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
   return WF_STATUS_OK;
 }
 
-bool download_results(Engine* e, wfa_device_batch_t* b) {
-  if (b->host_results) return true;
+bool download_results(Engine* e, wfa_device_batch_t* b, char* cigar_dst) {
   hipStream_t st = e->stream;
-  const size_t n = (size_t)b->n;
-  b->status.resize(n);
-  b->score.resize(n);
-  b->ops_begin.resize(n);
-  b->end_k.resize(n);
-  b->end_off.resize(n);
-  bool ok = n == 0 ||
-            (hip_ok(hipMemcpyAsync(b->status.data(), b->d_status.p, n * 4, hipMemcpyDeviceToHost, st), "download") &&
-             hip_ok(hipMemcpyAsync(b->score.data(), b->d_score.p, n * 4, hipMemcpyDeviceToHost, st), "download") &&
-             hip_ok(hipMemcpyAsync(b->ops_begin.data(), b->d_ops_begin.p, n * 4, hipMemcpyDeviceToHost, st), "download"));
-  if (ok && n > 0 && (b->ef.on || b->heur.zdrop) && !b->bi_mode)
-    ok = hip_ok(hipMemcpyAsync(b->end_k.data(), b->d_end_k.p, n * 4, hipMemcpyDeviceToHost, st), "download") &&
-         hip_ok(hipMemcpyAsync(b->end_off.data(), b->d_end_off.p, n * 4, hipMemcpyDeviceToHost, st), "download");
+  const size_t n = (size_t)b->n, nn = std::max<size_t>(n, 1);
+  const bool res = !b->host_results && n > 0;
+  // status, score, ops_begin, and end_k, end_off when the end cell varies
+  const bool ends = (b->ef.on || b->heur.zdrop) && !b->bi_mode;
+  const size_t bytes = nn * 4 * (ends ? 5 : 3);
+  bool ok = true;
+  if (res) {
+    ok = b->h_res.ensure(bytes) &&
+         hip_ok(hipMemcpyAsync(b->h_res.p, b->d_res.p, bytes, hipMemcpyDeviceToHost, st), "download");
+  }
+  if (ok && cigar_dst && b->total_ops > 0)
+    ok = hip_ok(hipMemcpyAsync(cigar_dst, b->d_ops.p, b->total_ops, hipMemcpyDeviceToHost, st), "cigar download");
+  if (!res && !cigar_dst) return ok;
   ok = ok && hip_ok(hipStreamSynchronize(st), "download sync");
-  b->host_results = ok;
+  if (ok && !b->host_results) {
+    b->status.resize(n);
+    b->score.resize(n);
+    b->ops_begin.resize(n);
+    b->end_k.resize(n);
+    b->end_off.resize(n);
+    if (n > 0) {
+      const char* h = b->h_res.as<char>();
+      memcpy(b->status.data(), h, n * 4);
+      memcpy(b->score.data(), h + nn * 4, n * 4);
+      memcpy(b->ops_begin.data(), h + 2 * nn * 4, n * 4);
+      if (ends) {
+        memcpy(b->end_k.data(), h + 3 * nn * 4, n * 4);
+        memcpy(b->end_off.data(), h + 4 * nn * 4, n * 4);
+      }
+    }
+    b->host_results = true;
+  }
   return ok;
 }
 
@@ -942,10 +965,9 @@ wfa_host::JobSet wfa_device_batch_t::jobs() const {
 
 wfa_device_batch_t::~wfa_device_batch_t() {
   h_words.release();
+  h_res.release();
   for (auto& g : plan) g.release();
-  for (wfa_host::DevBuf* x : {&words, &d_plen, &d_tlen, &d_patw, &d_txtw, &d_status, &d_score, &d_last, &d_dsteps,
-                              &d_ops_off, &d_ops_begin, &d_ops, &d_end_k, &d_end_off, &d_end_rec, &rev_words,
-                              &d_rev_ppos,
+  for (wfa_host::DevBuf* x : {&words, &d_res, &d_last, &d_dsteps, &d_ops, &d_end_rec, &rev_words, &d_rev_ppos,
                               &d_rev_tpos})
     x->release();
 }
@@ -1030,7 +1052,11 @@ bool wfa_device_batch_t::load(Engine* e, const wfa_pair_t* pairs, int count) {
   }
   total_ops = ops;
   words_used = nw;
-  if (!h_words.ensure(std::max<size_t>(nw, 1) * 8)) {
+  // per-pair inputs after the words: pat_pos, txt_pos, ops_off (int64), plen, tlen (int32)
+  const size_t nn = std::max(n, 1);
+  const size_t meta_off = (size_t)std::max<int64_t>(nw, 1) * 8;
+  const size_t meta_bytes = nn * (3 * 8 + 2 * 4);
+  if (!h_words.ensure(meta_off + meta_bytes)) {
     fprintf(stderr, "[WFA] pinned host allocation failed for a batch of %d pairs\n", n);
     return false;
   }
@@ -1059,12 +1085,8 @@ bool wfa_device_batch_t::load(Engine* e, const wfa_pair_t* pairs, int count) {
   plan_key.clear();  // new pairs: re-plan on the next run
   have_reverse = false;
   have_results = host_results = ops_on_device = false;
-  const size_t nn = std::max(n, 1);
-  bool ok = words.ensure(std::max<size_t>(nw, 1) * 8) && d_plen.ensure(nn * 4) && d_tlen.ensure(nn * 4) &&
-            d_patw.ensure(nn * 8) && d_txtw.ensure(nn * 8) && d_status.ensure(nn * 4) && d_score.ensure(nn * 4) &&
-            d_last.ensure(nn * 4) && d_dsteps.ensure(nn * 8) && d_ops_off.ensure(nn * 8) &&
-            d_ops_begin.ensure(nn * 4) && d_end_k.ensure(nn * 4) && d_end_off.ensure(nn * 4) &&
-            d_end_rec.ensure(nn * 4);
+  bool ok = words.ensure(meta_off + meta_bytes) && d_res.ensure(nn * 5 * 4) && d_last.ensure(nn * 4) &&
+            d_dsteps.ensure(nn * 8) && d_end_rec.ensure(nn * 4);
   if (!ok) {
     fprintf(stderr, "[WFA] device allocation failed for a batch of %d pairs\n", n);
     return false;
@@ -1075,14 +1097,29 @@ bool wfa_device_batch_t::load(Engine* e, const wfa_pair_t* pairs, int count) {
     pat_pos[i] = patw[i] * spw;
     txt_pos[i] = txtw[i] * spw;
   }
+  char* dm = words.as<char>() + meta_off;
+  d_patw.p = dm;
+  d_txtw.p = dm + nn * 8;
+  d_ops_off.p = dm + 2 * nn * 8;
+  d_plen.p = dm + 3 * nn * 8;
+  d_tlen.p = dm + 3 * nn * 8 + nn * 4;
+  char* dr = d_res.as<char>();
+  d_status.p = dr;
+  d_score.p = dr + nn * 4;
+  d_ops_begin.p = dr + 2 * nn * 4;
+  d_end_k.p = dr + 3 * nn * 4;
+  d_end_off.p = dr + 4 * nn * 4;
+  char* hm = h_words.as<char>() + meta_off;
+  if (n > 0) {
+    memcpy(hm, pat_pos.data(), (size_t)n * 8);
+    memcpy(hm + nn * 8, txt_pos.data(), (size_t)n * 8);
+    memcpy(hm + 2 * nn * 8, ops_off.data(), (size_t)n * 8);
+    memcpy(hm + 3 * nn * 8, plen.data(), (size_t)n * 4);
+    memcpy(hm + 3 * nn * 8 + nn * 4, tlen.data(), (size_t)n * 4);
+  }
   hipStream_t st = e->stream;
-  ok = n == 0 ||
-       (hip_ok(hipMemcpyAsync(words.p, host_words, nw * 8, hipMemcpyHostToDevice, st), "upload") &&
-        hip_ok(hipMemcpyAsync(d_plen.p, plen.data(), n * 4, hipMemcpyHostToDevice, st), "upload") &&
-        hip_ok(hipMemcpyAsync(d_tlen.p, tlen.data(), n * 4, hipMemcpyHostToDevice, st), "upload") &&
-        hip_ok(hipMemcpyAsync(d_patw.p, pat_pos.data(), n * 8, hipMemcpyHostToDevice, st), "upload") &&
-        hip_ok(hipMemcpyAsync(d_txtw.p, txt_pos.data(), n * 8, hipMemcpyHostToDevice, st), "upload") &&
-        hip_ok(hipMemcpyAsync(d_ops_off.p, ops_off.data(), n * 8, hipMemcpyHostToDevice, st), "upload"));
+  ok = n == 0 || hip_ok(hipMemcpyAsync(words.p, host_words, meta_off + meta_bytes, hipMemcpyHostToDevice, st),
+                        "upload");
   // No wait here: the runs are ordered after the copies on the same stream,
   // and the host arrays stay untouched until the next load.
   return ok;
@@ -1191,19 +1228,13 @@ extern "C" int wfa_device_batch_download(wavefront_aligner_t* const wf, wfa_devi
                                          size_t cigar_arena_bytes) {
   if (!b || !b->have_results) return WF_STATUS_UNATTAINABLE;
   Engine* e = engine_of(wf);
-  if (!download_results(e, b)) return WF_STATUS_OOM;
   const bool want_ops = cigar_arena != nullptr && b->records && b->ops_on_device;
-  if (want_ops) {
-    if (cigar_arena_bytes < (size_t)b->total_ops) {
-      fprintf(stderr, "[WFA] cigar arena too small (%zu < %lld bytes)\n", cigar_arena_bytes,
-              (long long)b->total_ops);
-      return WF_STATUS_OOM;
-    }
-    if (b->total_ops > 0 &&
-        !hip_ok(hipMemcpyAsync(cigar_arena, b->d_ops.p, b->total_ops, hipMemcpyDeviceToHost, e->stream), "cigar download"))
-      return WF_STATUS_OOM;
-    if (!hip_ok(hipStreamSynchronize(e->stream), "cigar download")) return WF_STATUS_OOM;
+  if (want_ops && cigar_arena_bytes < (size_t)b->total_ops) {
+    fprintf(stderr, "[WFA] cigar arena too small (%zu < %lld bytes)\n", cigar_arena_bytes,
+            (long long)b->total_ops);
+    return WF_STATUS_OOM;
   }
+  if (!download_results(e, b, want_ops ? cigar_arena : nullptr)) return WF_STATUS_OOM;
   const bool extension = wf->alignment_form.extension && b->records && !b->bi_mode;
   if ((extension || (b->heur.zdrop && b->records && !b->bi_mode)) && !want_ops) {
     fprintf(stderr, "[WFA] the extension form and z-drop need the CIGAR arena (they trim the CIGAR)\n");

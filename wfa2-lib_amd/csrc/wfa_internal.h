@@ -22,6 +22,12 @@ struct DevBuf {
   template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
 };
 
+// A typed view into a DevBuf owned elsewhere (never freed through the view).
+struct DevView {
+  void* p = nullptr;
+  template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
 // Per-aligner device state: one stream, the backtrace-record arena, the HBM
 // wavefront rings, the run counters.
 struct Engine {
@@ -141,8 +147,12 @@ struct wfa_device_batch_t {
   std::vector<int64_t> pat_pos, txt_pos;   // symbol positions (host copy)
   int64_t words_used = 0;                  // words taken by the forward sequences
   std::vector<int> sorted;                 // pair ids, longest first
-  wfa_host::DevBuf words, d_plen, d_tlen, d_patw, d_txtw, d_status, d_score, d_last, d_dsteps,
-      d_ops_off, d_ops_begin, d_ops, d_end_k, d_end_off, d_end_rec;
+  // `words` holds the packed sequences and, after them, the per-pair inputs
+  // (one host-to-device copy); `d_res` holds the per-pair results (one copy
+  // back).  The d_* views point into them.
+  wfa_host::DevBuf words, d_res, d_last, d_dsteps, d_ops, d_end_rec;
+  wfa_host::DevView d_patw, d_txtw, d_ops_off, d_plen, d_tlen;
+  wfa_host::DevView d_status, d_score, d_ops_begin, d_end_k, d_end_off;
   // reversed copies of the sequences (BiWFA), built on first use
   wfa_host::DevBuf rev_words, d_rev_ppos, d_rev_tpos;
   bool have_reverse = false;
@@ -161,7 +171,8 @@ struct wfa_device_batch_t {
   std::vector<int> bi_score;
   wfa_run_stats_t stats{};
   int records = 1;
-  wfa_host::HostBuf h_words;               // pinned staging of the packed sequences
+  wfa_host::HostBuf h_words;               // pinned staging of the packed sequences and per-pair inputs
+  wfa_host::HostBuf h_res;                 // pinned staging of the results
   wfa_host::JobSet jobs() const;
   // (Re)fills the batch with new pairs, reusing every buffer that is large
   // enough.  Returns false on an allocation or copy failure.
@@ -173,5 +184,7 @@ namespace wfa_host {
 // BiWFA (memory mode ultralow) over a whole batch; fills the batch's device
 // results like the unidirectional run.  Returns the first failing status.
 int bialign_run(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, RunTotals* tot);
-bool download_results(Engine* e, wfa_device_batch_t* b);
+// Copies the results to the host (once per run) and, when cigar_dst is
+// given, the operations arena with them, under one wait.
+bool download_results(Engine* e, wfa_device_batch_t* b, char* cigar_dst = nullptr);
 }  // namespace wfa_host
