@@ -77,6 +77,20 @@ __device__ __forceinline__ int lean_rd_g(const int16_t* ring, const LeanSrc& s, 
   return c == k ? v : (int)kRingNull;
 }
 
+// Minimum over the 64 lanes, uniform: prefix minima within each row of 16
+// lanes by row shifts of 1, 2, 4 and 8 (a lane without a source keeps
+// INT_MAX), then the rows joined by the row broadcasts of lanes 15 and 31;
+// lane 63 ends with the minimum of all.
+__device__ __forceinline__ int wave_min(int v) {
+  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x111, 0xf, 0xf, false));  // row_shr:1
+  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x112, 0xf, 0xf, false));  // row_shr:2
+  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x114, 0xf, 0xf, false));  // row_shr:4
+  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x118, 0xf, 0xf, false));  // row_shr:8
+  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x142, 0xa, 0xf, false));  // row_bcast:15 into rows 1, 3
+  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x143, 0xc, 0xf, false));  // row_bcast:31 into rows 2, 3
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
 // The choices of cell k over clamp-read sources (winning M offset, its
 // source, the gap-extension flags), ties as the compute step breaks them;
 // reads below 0 count as the one NULL offset (the generic kernel's
@@ -112,8 +126,7 @@ __device__ __forceinline__ void lean_wfadaptive(const int16_t* m, int lo, int hi
     const int k = kb + lane;
     if (k <= hi) dmin = min(dmin, end_distance((int)m[k], k, plen, tlen, weighted, mfactor));
   }
-  for (int o = 32; o > 0; o >>= 1) dmin = min(dmin, __shfl_xor(dmin, o, 64));
-  dmin = uniform(dmin);
+  dmin = wave_min(dmin);
   int first = INT_MAX, last = INT_MIN;
   for (int kb = lo; kb <= hi; kb += 64) {
     const int k = kb + lane;
@@ -130,6 +143,25 @@ __device__ __forceinline__ void lean_wfadaptive(const int16_t* m, int lo, int hi
   const int bottom = max(kend, lo_r);
   *clo = lo_r;
   *chi = hi <= bottom ? hi : max(last, bottom);
+}
+
+// lean_wfadaptive for a step of at most 64 diagonals from registers: lane l
+// holds d, the distance of diagonal lo + l computed in the pass (NULL and
+// absent cells: kNullDistance); only [mlo, mhi] takes part, as there.
+__device__ __forceinline__ void lean_wfadaptive_regs(int d, int lo, int mlo, int mhi, int plen, int tlen, int thr,
+                                                     int* clo, int* chi) {
+  const int k = lo + (int)threadIdx.x;
+  const bool in = k >= mlo && k <= mhi;
+  const int dmin = min(max(plen, tlen), wave_min(in ? d : INT_MAX));
+  const unsigned long long bal = __ballot(in && d - dmin <= thr);
+  const int first = bal ? lo + (int)__builtin_ctzll(bal) : INT_MAX;
+  const int last = bal ? lo + 63 - (int)__builtin_clzll(bal) : INT_MIN;
+  const int kend = tlen - plen;
+  const int top = min(kend, mhi);
+  const int lo_r = mlo >= top ? mlo : min(first, top);
+  const int bottom = max(kend, lo_r);
+  *clo = lo_r;
+  *chi = mhi <= bottom ? mhi : max(last, bottom);
 }
 
 // 16 symbols of a 2-bit packed sequence starting at symbol pos (dword view).
@@ -250,6 +282,8 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
     int dsteps = 0;  // below 2^31 for pairs of at most 1,024 bases
     int end_k = kend, end_off = tlen;
     int steps_wait = a.heur.steps;  // wavefront_heuristic_clear (wavefront_heuristic.c:114-121)
+    const bool wmash = CUT && a.heur.wfadaptive == 2;  // weighted distances (wfmash)
+    const int mfactor = wmash ? trunc_to_int((float)(plen + tlen) / 2.0f) : 0;
     int nulls = 0;                  // null steps in a row, every score counted (align_status.num_null_steps)
     uint32_t emptied = 0;           // bit slot*3+c: emptied by a cut-off, readable by the backtrace
     // wavefront_heuristic_cufoff (wavefront_heuristic.c:506-561) on the
@@ -257,14 +291,17 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
     // every strategy: wf-adaptive / wfmash cut every steps-th wavefront of at
     // least min_len diagonals; then the band: static clamps every time,
     // adaptive moves a band-wide window.  The kept range goes to clo, chi.
-    auto cut_range = [&](const int16_t* m, int mlo, int mhi, int& clo, int& chi) {
+    // A step of at most 64 diagonals (one pass, first diagonal plo) passes
+    // the distances its lanes computed (dl); otherwise the ring is read.
+    auto cut_range = [&](const int16_t* m, int mlo, int mhi, int& clo, int& chi, bool one_pass, int plo, int dl) {
       KernArgs H = args_now();
       --steps_wait;
       clo = mlo;
       chi = mhi;
       const int wa = H->heur.wfadaptive;
       if (wa && steps_wait <= 0 && mhi - mlo + 1 >= H->heur.min_len) {
-        lean_wfadaptive(m, mlo, mhi, plen, tlen, H->heur.thr, wa == 2, &clo, &chi);
+        if (one_pass) lean_wfadaptive_regs(dl, plo, mlo, mhi, plen, tlen, H->heur.thr, &clo, &chi);
+        else lean_wfadaptive(m, mlo, mhi, plen, tlen, H->heur.thr, wa == 2, &clo, &chi);
         steps_wait = H->heur.steps;
       }
       const int band = H->heur.banded;
@@ -328,7 +365,7 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
       // (wavefront_extend.c:112-118)
       if (CUT && status == 1) {
         int clo, chi;
-        cut_range(ring + b + 1, 0, 0, clo, chi);
+        cut_range(ring + b + 1, 0, 0, clo, chi, false, 0, 0);
         if (clo != 0 || chi != 0) {
           if (clo > chi) {
             s_lo[0][kM] = 1; s_hi[0][kM] = -1;
@@ -435,6 +472,7 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
             oD[kk] = kRingNull;
           }
         }
+        int dl = kNullDistance;  // CUT: this lane's distance to the end in the last pass
         // One pass over 64 diagonals starting at kb: compute, record, extend,
         // store; returns the validity ballots and the ballot of the lanes
         // that end the alignment: k_end reaching the end of the text
@@ -482,6 +520,7 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
               }
             }
           }
+          if constexpr (CUT) dl = okM ? end_distance(h, k, plen, tlen, wmash, mfactor) : kNullDistance;
           const bool vI = act && has_i1 && in_bounds(ins1, k, plen, tlen);
           const bool vD = act && has_d1 && in_bounds(del1, k, plen, tlen);
           if (act) {
@@ -590,7 +629,7 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
           if (mlo <= mhi) {
             {
               int clo, chi;
-              cut_range(oM, mlo, mhi, clo, chi);
+              cut_range(oM, mlo, mhi, clo, chi, hi - lo < 64, lo, dl);
               if (clo != mlo || chi != mhi) {
                 if (clo > chi) {
                   s_lo[ph][kM] = 1; s_hi[ph][kM] = -1;
