@@ -528,8 +528,20 @@ void pack_seq(uint64_t* dst, const char* src, int len, int bits) {
 // elements): ~8.5 KB with the sequences keeps ~18 wavefronts per CU resident.
 constexpr int kSmallLdsBudget = 8 * 1024;
 
+// Ring width of the generic LDS family: the width the budget allows, and
+// when that is under 256 diagonals, up to twice the budget for a ring of at
+// most 256 (measured on 100-300 bp reads, gap-affine (0,5,7,3), ten slots:
+// 11.1 M pairs/s at 128 diagonals, 15.3 M at 256, 9.0 M at 512: wider rings
+// re-run fewer pairs, larger ones cost occupancy).  WFA_SMALL_LDS (bytes)
+// replaces the budget for tuning.
 int small_wcap(const Penalties& pen, int ncomp) {
-  int w = kSmallLdsBudget / (pen.nslots * ncomp * 2);
+  static const int budget = [] {
+    const char* v = getenv("WFA_SMALL_LDS");
+    return v && atoi(v) > 0 ? atoi(v) : kSmallLdsBudget;
+  }();
+  const int per = pen.nslots * ncomp * 2;  // bytes per diagonal
+  int w = budget / per;
+  if (w < 256) w = std::min(256, 2 * budget / per);
   w = (w / 32) * 32;
   return w;
 }
