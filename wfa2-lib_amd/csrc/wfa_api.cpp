@@ -190,6 +190,8 @@ wfa::Penalties make_penalties(const wavefront_penalties_t& p) {
   if (two) g = gcd(g, gcd(r.o2 + r.e2, r.e2));
   r.g = std::max(g, 1);
   r.nslots = (r.scope - 1) / r.g + 1;
+  r.r1 = r.e1 / r.g + 1;
+  r.r2 = two ? r.e2 / r.g + 1 : 0;
   return r;
 }
 
@@ -539,7 +541,8 @@ int small_wcap(const Penalties& pen, int ncomp) {
     const char* v = getenv("WFA_SMALL_LDS");
     return v && atoi(v) > 0 ? atoi(v) : kSmallLdsBudget;
   }();
-  const int per = pen.nslots * ncomp * 2;  // bytes per diagonal
+  (void)ncomp;
+  const int per = wfa::ring_slots(pen) * 2;  // bytes per diagonal
   int w = budget / per;
   if (w < 256) w = std::min(256, 2 * budget / per);
   w = (w / 32) * 32;
@@ -601,7 +604,7 @@ LaunchShape shape_of(Engine* e, const PlanGroup& g, int bits, const Penalties& p
     shape.lds_bytes = wfa::lean_lds_bytes(pen, g.wcap, g.lmax_words);
     per_cu = wfa::lean_max_blocks_per_cu(pen, g.wcap, shape.lds_bytes);
   } else {
-    shape.lds_bytes = wfa::forward_lds_bytes(g.family, pen.ncomp, pen.nslots, g.wcap, g.lmax_words);
+    shape.lds_bytes = wfa::forward_lds_bytes(g.family, wfa::ring_slots(pen), g.wcap, g.lmax_words);
     per_cu = wfa::forward_max_blocks_per_cu(g.family, g.block, pen.ncomp, bits, cut, shape.lds_bytes);
   }
   if (per_cu <= 0) per_cu = 1;
@@ -647,7 +650,7 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
   fa.rec_units = records ? std::min<unsigned long long>(e->rec.cap / 16, 0xFFFFFFF0ull) : 0;
   fa.rec_top = &c->rec_top;
   fa.ring_global = grp.family == wfa::kLarge ? e->ring.as<int>() : nullptr;
-  fa.ring_stride = grp.family == wfa::kLarge ? (long long)pen.nslots * pen.ncomp * grp.wcap : 0;
+  fa.ring_stride = grp.family == wfa::kLarge ? (long long)wfa::ring_slots(pen) * grp.wcap : 0;
   fa.queue = &c->queue;
   fa.dsteps_total = &c->dsteps;
   fa.retry_count = &c->retry;
@@ -719,7 +722,7 @@ bool reserve_for(Engine* e, std::vector<PlanGroup>& plan, int bits, const Penalt
   for (const PlanGroup& g : plan) {
     if (g.family != wfa::kLarge) continue;
     const LaunchShape sh = shape_of(e, g, bits, pen, cut);
-    ring = std::max(ring, (size_t)pen.nslots * pen.ncomp * g.wcap * sh.grid * sizeof(int));
+    ring = std::max(ring, (size_t)wfa::ring_slots(pen) * g.wcap * sh.grid * sizeof(int));
   }
   if (ring && !e->ring.ensure(ring)) {
     fprintf(stderr, "[WFA] cannot allocate %zu MB of wavefront rings\n", ring >> 20);

@@ -82,6 +82,62 @@ __device__ __forceinline__ Src get_src(const MetaRegs<NC, CUT>& m, int s_src, in
   return r;
 }
 
+// Ring layout of the generic forward kernel: M in nslots slots, then I1 and
+// D1 in r1 slots each, then I2 and D2 in r2 slots each (a gap wavefront is
+// only read e/g steps later).  RingPos: the current step's slot in each.
+struct RingPos {
+  int cur, cur1, cur2;
+  __device__ __forceinline__ void advance(const Penalties& pen) {
+    cur = (cur + 1 == pen.nslots) ? 0 : cur + 1;
+    cur1 = (cur1 + 1 >= pen.r1) ? 0 : cur1 + 1;
+    cur2 = (cur2 + 1 >= pen.r2) ? 0 : cur2 + 1;
+  }
+};
+
+// Element offset of component c's ring slot `back` steps before the current
+// one (back < the component's ring depth).
+__device__ __forceinline__ int ring_base(const Penalties& pen, int wcap, int c, const RingPos& p, int back) {
+  if (c == kM) {
+    int x = p.cur - back;
+    if (x < 0) x += pen.nslots;
+    return x * wcap;
+  }
+  const bool one = c == kI1 || c == kD1;
+  const int rc = one ? pen.r1 : pen.r2;
+  int x = (one ? p.cur1 : p.cur2) - back;
+  if (x < 0) x += rc;
+  const int first = pen.nslots + (c == kD1 ? pen.r1 : 0) + (c == kI2 ? 2 * pen.r1 : 0) +
+                    (c == kD2 ? 2 * pen.r1 + pen.r2 : 0);
+  return (first + x) * wcap;
+}
+
+// get_src over that layout: rbase is ring_base of the source.
+template <int NC, bool CUT>
+__device__ __forceinline__ Src get_src_at(const MetaRegs<NC, CUT>& m, int s_src, int slot, int R, int c, int rbase,
+                                          bool keep_null = false) {
+  Src r{1, -1, kFarDiagonal, 0, 0};
+  if (s_src < 0) return r;
+  if (slot < 0) slot += R;
+  slot = uniform(slot);
+  const int tag = lane_of(m.tag, slot);
+  const unsigned fl = (unsigned)lane_of((int)m.flags, slot);
+  if (tag != s_src) return r;
+  if (!(fl & (1u << c)) || (!keep_null && (fl & (0x100u << c)))) return r;
+  const int lo = lane_of(m.lo[c], slot), span = lane_of(m.hi[c], slot) - lo;
+  if (span < 0) return r;
+  r.lo = lo;
+  r.span = span;
+  if constexpr (CUT) {
+    r.lim_lo = lane_of(m.cut.lo[c], slot);
+    r.lim_hi = lane_of(m.cut.hi[c], slot);
+  } else {
+    r.lim_lo = r.lo;
+    r.lim_hi = r.lo + r.span;
+  }
+  r.base = rbase + (r.lo - lane_of(m.alo, slot));
+  return r;
+}
+
 // Branch-free predicated read: the load address is always inside the ring.
 template <typename RingPtr>
 __device__ __forceinline__ int rd(RingPtr ring, const Src& s, int k) {
@@ -154,30 +210,31 @@ __device__ __forceinline__ uint32_t probed_record(RingPtr ring, const Src& bmx, 
 // (b: also the readable cells of a wavefront the cut-off emptied,
 // wavefront_backtrace.c:64-219).  Returns whether the two differ.
 template <int NC, bool CUT>
-__device__ __forceinline__ bool probe_sources(const MetaRegs<NC, CUT>& meta, int s, int cur, const Penalties& pen,
-                                              int wcap, Src* c, Src* b) {
-  const int R = pen.nslots, g = pen.g;
+__device__ __forceinline__ bool probe_sources(const MetaRegs<NC, CUT>& meta, int s, const RingPos& rp,
+                                              const Penalties& pen, int wcap, Src* c, Src* b) {
+  const int R = pen.nslots, g = pen.g, cur = rp.cur;
   bool stale = false;
-  auto both = [&](int i, int s_src, int slot, int comp) {
-    c[i] = get_src(meta, s_src, slot, R, comp, wcap);
+  auto both = [&](int i, int s_src, int back, int comp) {
+    const int rb = ring_base(pen, wcap, comp, rp, back);
+    c[i] = get_src_at(meta, s_src, cur - back, R, comp, rb);
     b[i] = c[i];
     if (!c[i].valid()) {
-      b[i] = get_src(meta, s_src, slot, R, comp, wcap, true);
+      b[i] = get_src_at(meta, s_src, cur - back, R, comp, rb, true);
       stale = stale || b[i].readable();
     }
   };
   const Src none{1, -1, kFarDiagonal, 0, 0};
   for (int i = 0; i < 7; ++i) c[i] = b[i] = none;
-  if (NC > 1 || pen.metric != kIndel) both(0, s - pen.x, cur - pen.x / g, kM);
-  both(1, s - pen.o1 - pen.e1, cur - (pen.o1 + pen.e1) / g, kM);
+  if (NC > 1 || pen.metric != kIndel) both(0, s - pen.x, pen.x / g, kM);
+  both(1, s - pen.o1 - pen.e1, (pen.o1 + pen.e1) / g, kM);
   if constexpr (NC > 1) {
-    both(2, s - pen.e1, cur - pen.e1 / g, kI1);
-    both(3, s - pen.e1, cur - pen.e1 / g, kD1);
+    both(2, s - pen.e1, pen.e1 / g, kI1);
+    both(3, s - pen.e1, pen.e1 / g, kD1);
   }
   if constexpr (NC == 5) {
-    both(4, s - pen.o2 - pen.e2, cur - (pen.o2 + pen.e2) / g, kM);
-    both(5, s - pen.e2, cur - pen.e2 / g, kI2);
-    both(6, s - pen.e2, cur - pen.e2 / g, kD2);
+    both(4, s - pen.o2 - pen.e2, (pen.o2 + pen.e2) / g, kM);
+    both(5, s - pen.e2, pen.e2 / g, kI2);
+    both(6, s - pen.e2, pen.e2 / g, kD2);
   }
   return stale;
 }
@@ -185,11 +242,11 @@ __device__ __forceinline__ bool probe_sources(const MetaRegs<NC, CUT>& meta, int
 // Record bytes of step s (ring slot cur, diagonals [lo, hi]) when an input
 // is a wavefront the cut-off emptied: each byte becomes probed_record's.
 template <int NC, int BLOCK, bool CUT, typename RingPtr>
-__device__ __forceinline__ void probed_records(RingPtr ring, const MetaRegs<NC, CUT>& meta, int s, int cur,
+__device__ __forceinline__ void probed_records(RingPtr ring, const MetaRegs<NC, CUT>& meta, int s, const RingPos& rp,
                                                const Penalties& pen, int wcap, int lo, int hi, uint8_t* rec) {
   Src c[7], b[7];
-  if (!probe_sources<NC, CUT>(meta, s, cur, pen, wcap, c, b)) return;
-  const auto* outM = ring + (cur * NC + kM) * wcap - lo;
+  if (!probe_sources<NC, CUT>(meta, s, rp, pen, wcap, c, b)) return;
+  const auto* outM = ring + ring_base(pen, wcap, kM, rp, 0) - lo;
   for (int k = lo + (int)threadIdx.x; k <= hi; k += BLOCK)
     rec[k - lo] = (uint8_t)probed_record<NC>(ring, b[0], b[1], b[2], b[3], b[4], b[5], b[6], c[0], c[1], c[2], c[3],
                                              c[4], c[5], c[6], k, (int)outM[k]);
@@ -200,10 +257,10 @@ __device__ __forceinline__ void probed_records(RingPtr ring, const MetaRegs<NC, 
 // reference's backtrace probes from it, or 0xFF when none is valid (the
 // backtrace then takes no step, wavefront_backtrace.c:265-266, 470-471).
 template <int NC, bool CUT, typename RingPtr>
-__device__ __forceinline__ int end_record(RingPtr ring, const MetaRegs<NC, CUT>& meta, int s, int cur,
+__device__ __forceinline__ int end_record(RingPtr ring, const MetaRegs<NC, CUT>& meta, int s, const RingPos& rp,
                                           const Penalties& pen, int wcap, int k) {
   Src c[7], b[7];
-  probe_sources<NC, CUT>(meta, s, cur, pen, wcap, c, b);
+  probe_sources<NC, CUT>(meta, s, rp, pen, wcap, c, b);
   const unsigned long long key = choice_key<NC>(ring, b[0], b[1], b[2], b[3], b[4], b[5], b[6], k);
   return (int)(unsigned)(key >> 32) >= 0 ? (int)(key & 0x7Fu) : 0xFF;
 }

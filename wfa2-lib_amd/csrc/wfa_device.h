@@ -42,7 +42,14 @@ struct Penalties {
   int nslots;              // ring slots = (scope-1)/g + 1
   int metric;              // Metric
   int ncomp;               // wavefront components: 1 (M), 3 (M, I1, D1) or 5
+  // A gap wavefront is read only e/g steps after it is computed, so the
+  // generic forward kernel keeps I1/D1 in rings of r1 = e1/g + 1 slots and
+  // I2/D2 in rings of r2 = e2/g + 1 (0 without them); M keeps nslots.
+  int r1, r2;
 };
+
+// Slots of every component's ring in the generic forward kernel.
+__host__ __device__ inline int ring_slots(const Penalties& p) { return p.nslots + 2 * p.r1 + 2 * p.r2; }
 
 // Edit distance prunes a wavefront of at least this many diagonals
 // (wavefront_compute_edit.c:219-278).
@@ -259,7 +266,7 @@ int lean_max_blocks_per_cu(const Penalties& pen, int wcap, size_t lds_bytes);
 hipError_t launch_forward_lean(const ForwardArgs& a, const LaunchShape& shape, hipStream_t stream);
 
 // Dynamic-LDS bytes and occupancy for a configuration.
-size_t forward_lds_bytes(int family, int ncomp, int nslots, int wcap, int lmax_words);
+size_t forward_lds_bytes(int family, int ring_slots, int wcap, int lmax_words);
 int forward_max_blocks_per_cu(int family, int block, int ncomp, int bits, int cut, size_t lds_bytes);
 int pick_block(int family, int max_width);
 

@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
     if constexpr (CUT) {
       const HeurArgs& hr = a.heur;
       --steps_wait;
-      const RT* m = ring + (slot * NC + kM) * wcap - alo;
+      const RT* m = ring + slot * wcap - alo;  // the M ring shares the metadata slots
       int clo = lo, chi = hi;
       if (hr.wfadaptive && steps_wait <= 0 && hi - lo + 1 >= hr.min_len) {
         wfadaptive_cut<BLOCK>(m, lo, hi, plen_, tlen_, hr.thr, hr.wfadaptive == 2, red, &clo, &chi);
@@ -181,6 +181,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
     DropState xs;
     xs.clear();
     int cur = 0;
+    RingPos rp{0, 0, 0};  // rp.cur == cur: the M ring shares the metadata slots
     int s = 0;
     int status = 1;  // running
     uint32_t prev_blk = kNoBlock;
@@ -241,7 +242,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
       int v0 = 0;
       if (tid == 0 && cbeg == kM) v0 = lcp_extend<BITS>(P, pb, T, tb, plen, tlen, 0, 0);
       v0 = broadcast0(v0, 1);
-      if (tid == 0) ring[(0 * NC + cbeg) * wcap] = to_ring(v0, ring);
+      if (tid == 0) ring[ring_base(pen, wcap, cbeg, rp, 0)] = to_ring(v0, ring);
       if (cbeg == kM && cend == kM && kend == 0 && v0 >= tlen) status = kDevCompleted;
     }
     block_sync();
@@ -268,17 +269,21 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
       }
       if (s >= a.max_steps) { status = kDevMaxSteps; break; }
       cur = (cur + 1 == R) ? 0 : cur + 1;
+      rp.advance(pen);
+      auto src = [&](int s_src, int back, int c) {
+        return get_src_at(meta, s_src, cur - back, R, c, ring_base(pen, wcap, c, rp, back));
+      };
       Src mx{1, -1, kFarDiagonal, 0, 0}, i1 = mx, d1 = mx, mo2 = mx, i2 = mx, d2 = mx;
-      if (NC > 1 || pen.metric != kIndel) mx = get_src(meta, s - pen.x, cur - dq_x, R, kM, wcap);
-      const Src mo1 = get_src(meta, s - pen.o1 - pen.e1, cur - dq_o1, R, kM, wcap);
+      if (NC > 1 || pen.metric != kIndel) mx = src(s - pen.x, dq_x, kM);
+      const Src mo1 = src(s - pen.o1 - pen.e1, dq_o1, kM);
       if constexpr (NC > 1) {
-        i1 = get_src(meta, s - pen.e1, cur - dq_e1, R, kI1, wcap);
-        d1 = get_src(meta, s - pen.e1, cur - dq_e1, R, kD1, wcap);
+        i1 = src(s - pen.e1, dq_e1, kI1);
+        d1 = src(s - pen.e1, dq_e1, kD1);
       }
       if constexpr (NC == 5) {
-        mo2 = get_src(meta, s - pen.o2 - pen.e2, cur - dq_o2, R, kM, wcap);
-        i2  = get_src(meta, s - pen.e2, cur - dq_e2, R, kI2, wcap);
-        d2  = get_src(meta, s - pen.e2, cur - dq_e2, R, kD2, wcap);
+        mo2 = src(s - pen.o2 - pen.e2, dq_o2, kM);
+        i2  = src(s - pen.e2, dq_e2, kI2);
+        d2  = src(s - pen.e2, dq_e2, kD2);
       }
       const bool any = mx.valid() || mo1.valid() || i1.valid() || d1.valid() ||
                        mo2.valid() || i2.valid() || d2.valid();
@@ -357,11 +362,11 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
         rec = reinterpret_cast<uint8_t*>(a.rec_arena + blk + 1);
       }
 
-      RT* outM  = ring + (cur * NC + kM) * wcap - lo;
-      RT* outI1 = ring + (cur * NC + (NC > 1 ? kI1 : 0)) * wcap - lo;
-      RT* outD1 = ring + (cur * NC + (NC > 1 ? kD1 : 0)) * wcap - lo;
-      RT* outI2 = ring + (cur * NC + (NC == 5 ? kI2 : 0)) * wcap - lo;
-      RT* outD2 = ring + (cur * NC + (NC == 5 ? kD2 : 0)) * wcap - lo;
+      RT* outM  = ring + ring_base(pen, wcap, kM, rp, 0) - lo;
+      RT* outI1 = ring + ring_base(pen, wcap, NC > 1 ? kI1 : kM, rp, 0) - lo;
+      RT* outD1 = ring + ring_base(pen, wcap, NC > 1 ? kD1 : kM, rp, 0) - lo;
+      RT* outI2 = ring + ring_base(pen, wcap, NC == 5 ? kI2 : kM, rp, 0) - lo;
+      RT* outD2 = ring + ring_base(pen, wcap, NC == 5 ? kD2 : kM, rp, 0) - lo;
 
       WaveRange rM, rI1, rD1, rI2, rD2;
       rM.init(); rI1.init(); rD1.init(); rI2.init(); rD2.init();
@@ -488,7 +493,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
         // wavefront_backtrace.c:64-219).  Where one exists, the record bytes
         // take the choices over the probed sources (probed_record); each lane
         // rewrites the bytes it wrote above.
-        if (rec) probed_records<NC, BLOCK>(ring, meta, s, cur, pen, wcap, lo, hi, rec);
+        if (rec) probed_records<NC, BLOCK>(ring, meta, s, rp, pen, wcap, lo, hi, rec);
       }
       if constexpr (NC == 1) {
         if (prune_try && !rM.null()) edit_prune(rM, ring, mo1, wmin, plen, tlen);
@@ -519,7 +524,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
         if (hitk != INT_MAX) {
           status = kDevCompleted;
           end_k = hitk;
-          end_off = uniform((int)ring[(cur * NC + kM) * wcap + (hitk - lo)]);
+          end_off = uniform((int)ring[ring_base(pen, wcap, kM, rp, 0) + (hitk - lo)]);
         }
       } else {
         // Termination (wavefront_termination.c:37-114): k_end inside the trimmed
@@ -535,13 +540,13 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
         }
         if (pce && !rce.null() && kend >= rce.first && kend <= rce.last) {
           // the same LDS word for every lane: keep the loop control scalar
-          const int vend = uniform((int)ring[(cur * NC + ce) * wcap + (kend - lo)]);
+          const int vend = uniform((int)ring[ring_base(pen, wcap, ce, rp, 0) + (kend - lo)]);
           if (vend >= tlen) status = kDevCompleted;
         }
       }
       if (CUT && status == 1 && !rM.null() && cutoff(meta, steps_wait, xs, cur, lo, rM.first, rM.last, s, plen, tlen)) {
         status = kDevZdropped;
-        const int er = end_record<NC, HC>(ring, meta, s, cur, pen, wcap, xs.k);
+        const int er = end_record<NC, HC>(ring, meta, s, rp, pen, wcap, xs.k);
         if (tid == 0 && a.out_end_rec) a.out_end_rec[p] = er;
       }
       if (status == 1 && s >= s_cap) status = kDevBadTrace;  // unreachable for valid input
@@ -885,11 +890,11 @@ const void* select_forward(int family, int ncomp, int bits, int block, int cut) 
 
 }  // namespace
 
-size_t forward_lds_bytes(int family, int ncomp, int nslots, int wcap, int lmax_words) {
+size_t forward_lds_bytes(int family, int ring_slots, int wcap, int lmax_words) {
   size_t b = 0;
   if (family == kSmall) {
     b += (size_t)2 * lmax_words * sizeof(uint64_t);
-    b += (size_t)nslots * ncomp * wcap * sizeof(int16_t);
+    b += (size_t)ring_slots * wcap * sizeof(int16_t);
   }
   return b;
 }
