@@ -88,6 +88,29 @@ def test_wfadaptive_endsfree_vs_oracle():
         assert st.diagonal_steps == int(ref.dsteps.sum())
 
 
+def test_cutoffs_on_endsfree_wavefronts_vs_oracle():
+    """Cut-offs on ends-free wavefronts, including the score-0 wavefront over
+    wide free beginnings: the tightest wf-adaptive cut, wfmash, a static band
+    that narrows the score-0 range and an adaptive band.  A pair whose
+    reference trace would take operations back through a wavefront a cut-off
+    emptied (the oracle's stale flag) is declined, and only such a pair."""
+    say("cut-offs on ends-free wavefronts")
+    ps = datagen.semiglobal(1, 4105, 600)
+    for heur, form in (((4, 1, 0, 1), dict(ends_free=(20, 20, 20, 20))),
+                       ((4, 10, 50, 1), dict(ends_free=(30, 30, 0, 0))),
+                       ((0x40, 10, 50, 1), dict(extension=True)),
+                       ((1, 0, 0, 0, -5, 5), dict(ends_free=(10, 10, 10, 10))),
+                       ((2, 0, 0, 1, -8, 8), dict(ends_free=(0, 0, 30, 30)))):
+        for so in (False, True):
+            got, ref, st = run_both("affine", (0, 4, 6, 2), heur, ps, "high", so, **form)
+            declined = [i for i in range(ps.n) if got.status[i] == wfa_amd.WF_STATUS_UNSUPPORTED]
+            assert declined == [i for i in range(ps.n) if (not so) and ref.stale[i]], (heur, form, so)
+            bad = [i for i in range(ps.n) if i not in declined and
+                   (got.status[i] != ref.status[i] or got.score[i] != ref.score[i] or
+                    (not so and got.ops(i) != ref.ops(i)))]
+            assert not bad, (heur, form, so, bad[:10])
+
+
 def test_default_attributes_are_wfadaptive():
     """An aligner made from wavefront_aligner_attr_default (wf-adaptive
     10, 50, 1: wavefront_attributes.c:38-100) aligns like the oracle with

@@ -747,8 +747,8 @@ std::vector<PlanGroup> make_plan(const JobSet& js, const std::vector<int>& ids, 
   // flanks of up to 30 bases, the narrow ring re-ran enough pairs to cost
   // 20%); the score-0 wavefront spans pbf + tbf + 1 diagonals and must fit.
   const bool ef_fits = !js.ef.on || (int64_t)js.ef.pbf + js.ef.tbf + 3 <= wfa::lean_width(lt.wcap_long);
-  // of the heuristics, the lean family runs all but the drops, end-to-end
-  const bool heur_ok = js.heur.cut() == 0 || (js.heur.cut() == 1 && !js.ef.on);
+  // of the heuristics, the lean family runs all but the drops
+  const bool heur_ok = js.heur.cut() <= 1;
   const bool lean_ok = lt.enabled && ncomp == 3 && js.bits == 2 && !js.cbeg && !js.cend && ef_fits && heur_ok &&
                        wfa::lean_supported(pen);
   std::vector<int> lean_ids[2];

@@ -222,7 +222,6 @@ template <int R, int DQX, int DQO, int DQE, int WCAP, bool REC, bool EF, bool CU
 __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int NC = 3;
-  static_assert(!(EF && CUT), "ends-free with a heuristic runs on the generic kernel");
   static_assert(R * NC <= 32, "emptied mask");
   static_assert(DQX > 0 && DQO > 0 && DQE > 0 && DQX < R && DQO < R && DQE < R, "slot distances");
   const int lane = threadIdx.x;
@@ -347,6 +346,21 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
           status = kDevCompleted;
           end_k = hitk;
           end_off = hitoff;
+        }
+        wave_sync();
+        // the heuristic also runs after the score-0 extension
+        // (wavefront_extend.c:112-118), on the untrimmed range
+        if (CUT && status == 1) {
+          int clo, chi;
+          cut_range(ring + b, lo0, hi0, clo, chi, false, 0, 0);
+          if (clo != lo0 || chi != hi0) {
+            if (clo > chi) {
+              s_lo[0][kM] = 1; s_hi[0][kM] = -1;
+              emptied |= 1u << kM;
+            } else {
+              s_lo[0][kM] = clo; s_hi[0][kM] = chi;
+            }
+          }
         }
       }
       wave_sync();
@@ -700,8 +714,8 @@ bool lean_structure(const Penalties& pen) {
 }
 
 const void* select_lean(const Penalties& pen, int wcap, bool rec, bool ef, bool cut) {
-  if (!lean_structure(pen) || (ef && cut)) return nullptr;
-  if (cut) return lean_fn<5, 2, 4, 1, false, true>(wcap, rec);
+  if (!lean_structure(pen)) return nullptr;
+  if (cut) return ef ? lean_fn<5, 2, 4, 1, true, true>(wcap, rec) : lean_fn<5, 2, 4, 1, false, true>(wcap, rec);
   return ef ? lean_fn<5, 2, 4, 1, true, false>(wcap, rec) : lean_fn<5, 2, 4, 1, false, false>(wcap, rec);
 }
 
