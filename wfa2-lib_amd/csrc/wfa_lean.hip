@@ -200,10 +200,11 @@ __device__ __forceinline__ KernArgs args_now() {
 #endif
 }
 
-// Ring elements before the packed sequences in LDS (16-byte aligned).
+// Ring elements before the packed sequences in LDS (16-byte aligned): M in
+// R slots, I1 and D1 in two slots each (they are read one step later only).
 template <int R, int WCAP>
 constexpr int lean_ring_elems() {
-  return ((2 + R * 3 * WCAP) + 7) / 8 * 8;
+  return ((2 + (R + 4) * WCAP) + 7) / 8 * 8;
 }
 
 // EF: ends-free form (score-0 wavefront over [-pbf, tbf], the alignment ends
@@ -235,7 +236,12 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
   uint64_t* seqT64 = seqP64 + a.lmax_words;
   const uint32_t* seqP = reinterpret_cast<const uint32_t*>(seqP64);
   const uint32_t* seqT = reinterpret_cast<const uint32_t*>(seqT64);
-  auto base_of = [&](int slot, int c) { return 2 + (slot * NC + c) * wcap; };
+  static_assert(DQE == 1, "the gap rings hold two steps");
+  auto mbase = [&](int slot) { return 2 + slot * wcap; };
+  // I1/D1 ring slot of this step (cur) or of the previous one: the step
+  // index's parity, kept in par.
+  int par = 0;
+  auto gbase = [&](int c, bool cur) { return 2 + (R + (c == kD1 ? 2 : 0) + (cur ? par : par ^ 1)) * wcap; };
   if (lane == 0) ring[0] = kRingNull;
 
   uint32_t chunk_cur = 0, chunk_end = 0;  // record chunk of this wave (16-byte units)
@@ -276,6 +282,7 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
       for (int c = 0; c < NC; ++c) { s_lo[r][c] = 1; s_hi[r][c] = -1; }
     }
     int s = 0;
+    par = 0;
     int status = 1;
     uint32_t prev_blk = kNoBlock;
     int dsteps = 0;  // below 2^31 for pairs of at most 1,024 bases
@@ -320,7 +327,7 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
       if (hi0 - lo0 + 3 > wcap) {
         status = kDevRingOverflow;
       } else {
-        const int b = base_of(0, kM) - (lo0 - 1);  // ring element of diagonal 0
+        const int b = mbase(0) - (lo0 - 1);  // ring element of diagonal 0
         int hitk = INT_MAX, hitoff = 0;
         for (int kb = lo0; kb <= hi0; kb += 64) {
           const int k = kb + lane;
@@ -371,7 +378,7 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
       s_top[0] = 1;
       s_lo[0][kM] = 0;
       s_hi[0][kM] = 0;
-      const int b = base_of(0, kM);
+      const int b = mbase(0);
       if (lane < 3) ring[b + lane] = (lane == 1) ? to_ring(v0, ring) : kRingNull;
       if (kend == 0 && v0 >= tlen) status = kDevCompleted;
       wave_sync();
@@ -397,6 +404,7 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
       for (int j = 1; j <= R; ++j) {
         const int ph = j % R;  // slot of this step (compile-time after unrolling)
         s += g;
+        par ^= 1;
         if constexpr (CUT) {
           // The scores between two multiples of g hold no wavefront: each is
           // a null step of the reference, whose extension ends the alignment
@@ -417,15 +425,15 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
         // neighbouring cells hold NULL (a pad or a cell trimmed away), so the
         // clamp may use them in place of the computed range.
         const LeanSrc mx = lean_src_m(CUT ? s_org[rx] : s_lo[rx][kM] - 1, CUT ? s_top[rx] : s_hi[rx][kM] + 1,
-                                      s_lo[rx][kM], s_hi[rx][kM], base_of(rx, kM), s_org[rx]);
+                                      s_lo[rx][kM], s_hi[rx][kM], mbase(rx), s_org[rx]);
         const LeanSrc mo = lean_src_m(CUT ? s_org[ro] : s_lo[ro][kM] - 1, CUT ? s_top[ro] : s_hi[ro][kM] + 1,
-                                      s_lo[ro][kM], s_hi[ro][kM], base_of(ro, kM), s_org[ro]);
-        const LeanSrc i1 = CUT ? lean_src_m(s_org[re], s_top[re], s_lo[re][kI1], s_hi[re][kI1], base_of(re, kI1),
+                                      s_lo[ro][kM], s_hi[ro][kM], mbase(ro), s_org[ro]);
+        const LeanSrc i1 = CUT ? lean_src_m(s_org[re], s_top[re], s_lo[re][kI1], s_hi[re][kI1], gbase(kI1, false),
                                             s_org[re])
-                               : lean_src_g(s_org[re], s_lo[re][kI1], s_hi[re][kI1], base_of(re, kI1));
-        const LeanSrc d1 = CUT ? lean_src_m(s_org[re], s_top[re], s_lo[re][kD1], s_hi[re][kD1], base_of(re, kD1),
+                               : lean_src_g(s_org[re], s_lo[re][kI1], s_hi[re][kI1], gbase(kI1, false));
+        const LeanSrc d1 = CUT ? lean_src_m(s_org[re], s_top[re], s_lo[re][kD1], s_hi[re][kD1], gbase(kD1, false),
                                             s_org[re])
-                               : lean_src_g(s_org[re], s_lo[re][kD1], s_hi[re][kD1], base_of(re, kD1));
+                               : lean_src_g(s_org[re], s_lo[re][kD1], s_hi[re][kD1], gbase(kD1, false));
         const bool vx = mx.llo <= mx.lhi, vo = mo.llo <= mo.lhi, vi = i1.llo <= i1.lhi, vd = d1.llo <= d1.lhi;
         if (!(vx || vo || vi || vd)) {  // null step (wavefront_compute_affine.c:236-243)
           if constexpr (CUT) {
@@ -473,9 +481,9 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
 
         // Outputs of this step go to slot ph, origin lo - 1.
         const int org = lo - 1;
-        int16_t* oM = ring + base_of(ph, kM) - org;
-        int16_t* oI = ring + base_of(ph, kI1) - org;
-        int16_t* oD = ring + base_of(ph, kD1) - org;
+        int16_t* oM = ring + mbase(ph) - org;
+        int16_t* oI = ring + gbase(kI1, true) - org;
+        int16_t* oD = ring + gbase(kD1, true) - org;
         // NULL in the two M cells just outside the computed range (and in
         // the gap cells there when every input is clamp-read).
         if (lane < 2) {
@@ -600,7 +608,7 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
                 if ((emptied >> (3 * slot + comp)) & 1u) {
                   b.clo = s_org[slot];
                   b.chi = s_top[slot];
-                  b.sb = base_of(slot, comp) - s_org[slot];
+                  b.sb = (comp == kM ? mbase(slot) : gbase(comp, false)) - s_org[slot];
                 }
                 return b;
               };
@@ -731,7 +739,7 @@ int lean_width(int wcap) {
 
 size_t lean_lds_bytes(const Penalties& pen, int wcap, int lmax_words) {
   if (!lean_structure(pen)) return 0;
-  return (size_t)2 * (((2 + pen.nslots * 3 * wcap) + 7) / 8 * 8) + (size_t)2 * lmax_words * sizeof(uint64_t);
+  return (size_t)2 * (((2 + (pen.nslots + 4) * wcap) + 7) / 8 * 8) + (size_t)2 * lmax_words * sizeof(uint64_t);
 }
 
 int lean_max_blocks_per_cu(const Penalties& pen, int wcap, size_t lds_bytes) {
