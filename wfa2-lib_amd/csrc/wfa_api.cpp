@@ -116,7 +116,14 @@ bool engine_init(Engine* e) {
   hipDeviceProp_t prop;
   if (!hip_ok(hipGetDeviceProperties(&prop, e->device), "hipGetDeviceProperties")) return false;
   e->num_cus = prop.multiProcessorCount;
-  if (!hip_ok(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "hipStreamCreate")) return false;
+  if (e->own_queue) {
+    std::vector<uint32_t> mask((e->num_cus + 31) / 32, 0xFFFFFFFFu);
+    if (e->num_cus % 32) mask.back() = (1u << (e->num_cus % 32)) - 1u;
+    if (!hip_ok(hipExtStreamCreateWithCUMask(&e->stream, (uint32_t)mask.size(), mask.data()), "hipStreamCreate"))
+      return false;
+  } else if (!hip_ok(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "hipStreamCreate")) {
+    return false;
+  }
   if (!e->counters.ensure(64)) return false;
   // Record arena: up to half of the free HBM (block indices address 64 GB),
   // or WFA_RECORD_ARENA_MB; each run sizes it from its pairs' lengths.
@@ -850,7 +857,10 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
     if (host_c.retry == 0) break;
     // Some jobs overflowed the LDS ring or the record arena: re-run only them.
     h_status.resize(js.n);
-    if (!hip_ok(hipMemcpy(h_status.data(), js.status, (size_t)js.n * 4, hipMemcpyDeviceToHost), "status"))
+    // copies on the aligner's stream, never the null stream (which would wait
+    // for the streams of other aligners)
+    if (!hip_ok(hipMemcpyAsync(h_status.data(), js.status, (size_t)js.n * 4, hipMemcpyDeviceToHost, st), "status") ||
+        !hip_ok(hipStreamSynchronize(st), "status"))
       return WF_STATUS_OOM;
     std::vector<PlanGroup> next;
     for (PlanGroup& g : *pending) {
@@ -880,7 +890,9 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
           if (e->rec_bytes * 2 > std::min(e->rec_cap, e->rec.cap + free_b / 2)) {
             fprintf(stderr, "[WFA] record arena exhausted (%zu MB)\n", e->rec_bytes >> 20);
             for (int i : arena_full) h_status[i] = WF_STATUS_OOM;
-            if (!hip_ok(hipMemcpy(js.status, h_status.data(), (size_t)js.n * 4, hipMemcpyHostToDevice), "status"))
+            if (!hip_ok(hipMemcpyAsync(js.status, h_status.data(), (size_t)js.n * 4, hipMemcpyHostToDevice, st),
+                        "status") ||
+                !hip_ok(hipStreamSynchronize(st), "status"))
               return WF_STATUS_OOM;
             host_c.fail += (unsigned)arena_full.size();
             arena_full.clear();

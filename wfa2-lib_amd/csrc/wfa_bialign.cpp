@@ -431,7 +431,9 @@ int bialign_run(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, RunTo
       d_pieces.release();
       if (!ok) return WF_STATUS_OOM;
     }
-    if (!hip_ok(hipMemcpy(b->d_ops_begin.p, begin.data(), (size_t)n * 4, hipMemcpyHostToDevice), "results"))
+    if (!hip_ok(hipMemcpyAsync(b->d_ops_begin.p, begin.data(), (size_t)n * 4, hipMemcpyHostToDevice, e->stream),
+                "results") ||
+        !hip_ok(hipStreamSynchronize(e->stream), "results"))
       return WF_STATUS_OOM;
     for (DevBuf* x : {&d_pl, &d_tl, &d_pp, &d_tp, &d_cb, &d_ce, &d_st, &d_sc, &d_last, &d_ds, &d_off, &d_beg, &d_ops})
       x->release();
@@ -444,7 +446,9 @@ int bialign_run(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, RunTo
     b->bi_score[i] = (records && status[i] != WF_STATUS_ALG_COMPLETED) ? failed_score(ps[i]) : ps[i].score;
     if (status[i] != WF_STATUS_ALG_COMPLETED) tot->fail++;
   }
-  if (n > 0 && !hip_ok(hipMemcpy(b->d_status.p, status.data(), (size_t)n * 4, hipMemcpyHostToDevice), "results"))
+  if (n > 0 && (!hip_ok(hipMemcpyAsync(b->d_status.p, status.data(), (size_t)n * 4, hipMemcpyHostToDevice, e->stream),
+                        "results") ||
+                 !hip_ok(hipStreamSynchronize(e->stream), "results")))
     return WF_STATUS_OOM;
   return WF_STATUS_OK;
 }

@@ -40,6 +40,10 @@ struct Engine {
   size_t rec_bytes = 0;     // record arena size for the next run (grows on demand)
   size_t rec_cap = 0;       // largest record arena this engine may allocate
   bool ok = false;
+  // Stream on a hardware queue of its own: HIP spreads ordinary streams over a
+  // few shared queues, on which kernels of different streams run in order;
+  // a stream created with a CU mask (here: every CU) gets a new queue.
+  bool own_queue = false;
   ::wfa_device_batch_t* scratch = nullptr;  // reused by wavefront_align / wavefront_align_batch
 };
 

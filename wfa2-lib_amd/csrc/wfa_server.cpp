@@ -95,8 +95,14 @@ void wfa_server_t::lane_main(int cls) {
     ++lane_errors;
     return;
   }
-  // The lane's record arena is its memory budget.
-  if (wfa_host::Engine* e = wfa_host::engine_of(wf)) e->rec_cap = cfg.lane_memory_bytes;
+  // The lane's record arena is its memory budget.  Each lane has a hardware
+  // queue of its own: on a shared queue, one lane's kernels (seconds for a
+  // long batch, hundreds of milliseconds for a medium one) would hold up the
+  // batches of the others (streams sharing a queue run in order).
+  if (wfa_host::Engine* e = wfa_host::engine_of(wf)) {
+    e->rec_cap = cfg.lane_memory_bytes;
+    e->own_queue = true;
+  }
   Queue& Q = queues[cls];
   std::vector<std::unique_ptr<Request>> batch;
   std::vector<wfa_pair_t> pairs;
