@@ -2,7 +2,10 @@
  * CPU oracle for the exact WFA hot path -- TEST INFRASTRUCTURE ONLY.
  *
  * A plain-C restatement of WFA2-lib v2.3 (smarco/WFA2-lib) wavefront_align for
- * gap-affine / gap-affine-2p, end-to-end, memory modes high/med/low/ultralow.
+ * the indel, edit, gap-linear, gap-affine and gap-affine-2p metrics; end-to-end,
+ * ends-free and extension forms; the heuristics none, wf-adaptive, wfmash,
+ * x-drop, z-drop, banded static and banded adaptive; memory modes
+ * high/med/low (one algorithm, identical CIGARs) and ultralow (BiWFA).
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
  * as the checker; the product library never links or calls it.
  *

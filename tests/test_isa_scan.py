@@ -1,5 +1,6 @@
 """The device code must write memory only through vector instructions: scan
-the gfx950 assembly of the kernels for scalar-cache writes."""
+the gfx950 assembly of every HIP source for scalar-cache writes, and check
+that the kernels each source is meant to hold were really compiled into it."""
 import os
 import re
 import shutil
@@ -10,18 +11,28 @@ import pytest
 from conftest import ROOT
 
 BAD = re.compile(r"^\s+s_(store|atomic|dcache_wb|buffer_store|buffer_atomic|scratch_store)", re.M)
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+CSRC = os.path.join(ROOT, "wfa2-lib_amd", "csrc")
+
+# source -> kernel names that must appear in its assembly
+SOURCES = {
+    "wfa_kernels.hip": ["wfa_forward_kernel", "wfa_backtrace_kernel", "wfa_reset_kernel"],
+    "wfa_lean.hip": ["wfa_forward_lean_kernel"],
+    "wfa_biwfa.hip": ["wfa_breakpoint_kernel", "reverse_pack_kernel", "gather_ops_kernel"],
+}
 
 
-@pytest.mark.skipif(shutil.which("hipcc") is None and not os.path.exists("/opt/rocm/bin/hipcc"),
-                    reason="hipcc not available")
-def test_no_scalar_cache_writes(tmp_path):
-    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
-    src = os.path.join(ROOT, "wfa2-lib_amd", "csrc", "wfa_kernels.hip")
-    out = tmp_path / "k.s"
-    subprocess.check_call([hipcc, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not available")
+@pytest.mark.parametrize("name", sorted(SOURCES))
+def test_no_scalar_cache_writes(tmp_path, name):
+    out = tmp_path / (name + ".s")
+    subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
                            "-Xclang", "-packed-fp32-ops", "--offload-device-only", "-S",
-                           "-I" + os.path.join(ROOT, "include"), src, "-o", str(out)],
+                           "-I" + os.path.join(ROOT, "include"), "-I" + CSRC, os.path.join(CSRC, name),
+                           "-o", str(out)],
                           stderr=subprocess.DEVNULL)
     asm = out.read_text()
-    assert "wfa_forward_kernel" in asm
-    assert not BAD.search(asm)
+    for kernel in SOURCES[name]:
+        assert kernel in asm, "%s: no %s in the assembly" % (name, kernel)
+    m = BAD.search(asm)
+    assert m is None, "%s: scalar-cache write: %s" % (name, m.group(0) if m else "")

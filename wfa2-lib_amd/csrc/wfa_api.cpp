@@ -1379,22 +1379,29 @@ extern "C" int wavefront_align_batch(wavefront_aligner_t* const wf, const wfa_pa
     return sup;
   }
   if (num_pairs <= 0) return WF_STATUS_ALG_COMPLETED;
+  // Every early failure leaves each result with that status and no CIGAR.
+  auto fail_all = [&](int st) {
+    for (int i = 0; i < num_pairs; ++i) results[i] = wfa_result_t{st, INT32_MIN, -1, -1, 0, 0};
+    return st;
+  };
   if (wf->alignment_scope == compute_alignment &&
       (cigar_arena == nullptr || cigar_arena_bytes < wavefront_align_batch_arena_bytes(pairs, num_pairs))) {
     fprintf(stderr, "[WFA] wavefront_align_batch: cigar_arena must hold %zu bytes\n",
             wavefront_align_batch_arena_bytes(pairs, num_pairs));
-    return WF_STATUS_OOM;
+    return fail_all(WF_STATUS_OOM);
   }
   // The aligner's resident batch: its device buffers, pinned staging and
   // plan buffers are reused from call to call (grown when too small).
   Engine* e = engine_of(wf);
-  if (!e || !engine_init(e)) return WF_STATUS_OOM;
+  if (!e || !engine_init(e)) return fail_all(WF_STATUS_OOM);
   if (!e->scratch) e->scratch = new (std::nothrow) wfa_device_batch_t();
   wfa_device_batch_t* b = e->scratch;
-  if (!b || !b->load(e, pairs, num_pairs)) return WF_STATUS_OOM;
+  if (!b || !b->load(e, pairs, num_pairs)) return fail_all(WF_STATUS_OOM);
   int rc = wfa_device_batch_run(wf, b);
-  if (rc == WF_STATUS_OOM && !b->have_results) return rc;
-  return wfa_device_batch_download(wf, b, results, cigar_arena, cigar_arena_bytes);
+  if (rc == WF_STATUS_OOM && !b->have_results) return fail_all(rc);
+  const int drc = wfa_device_batch_download(wf, b, results, cigar_arena, cigar_arena_bytes);
+  if (drc == WF_STATUS_OOM && !b->host_results) return fail_all(drc);
+  return drc;
 }
 
 // ------------------------------------------------------------ single pair
@@ -1437,7 +1444,7 @@ int align_one(wavefront_aligner_t* wf) {
     free(cigar->cigar_buffer);
     cigar->cigar_buffer = static_cast<uint32_t*>(malloc(sizeof(uint32_t) * cigar->max_operations));
   }
-  wfa_result_t r;
+  wfa_result_t r{WF_STATUS_OOM, INT32_MIN, -1, -1, 0, 0};
   const int rc = wavefront_align_batch(wf, &pr, 1, &r, cig ? cigar->operations : nullptr,
                                        cig ? (size_t)cigar->max_operations : 0);
   (void)rc;

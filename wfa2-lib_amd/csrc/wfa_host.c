@@ -153,7 +153,8 @@ char* wavefront_align_strerror(const int error_code) {
     case WF_STATUS_END_UNREACHABLE:
       return "[WFA] Alignment end unreachable under current configuration (due to heuristics like Z-drop)";
     case WF_STATUS_UNSUPPORTED:
-      return "[WFA] Configuration outside the exact device path (gap-affine/affine2p, end-to-end, no heuristic)";
+      return "[WFA] Configuration outside the device path (lambda sequences, or a form/heuristic/memory-mode "
+             "combination the device declines)";
     default: return "[WFA] Unknown error code";
   }
 }
