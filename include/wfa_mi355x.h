@@ -381,6 +381,7 @@ typedef struct {
   double forward_ms;
   double backtrace_ms;
   int forward_launches;
+  int64_t record_arena_bytes;  /* the aligner's backtrace-record arena after the run */
 } wfa_run_stats_t;
 int wfa_device_batch_stats(wfa_device_batch_t* batch, wfa_run_stats_t* stats);
 

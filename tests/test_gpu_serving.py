@@ -95,3 +95,40 @@ def test_submit_and_collect_by_id():
         st, sc, ops = al.align(ps.pattern(i), ps.text(i))
         assert (got[rid].status, got[rid].score, got[rid].ops) == (st, sc, ops)
         assert got[rid].submit <= got[rid].start <= got[rid].finish
+
+
+def test_serving_mix_of_the_configuration_equal_reference():
+    """The serving configuration's own mixture (60% 1 kbp / 5%, 35% 1-20 kbp /
+    1-15% with geometric indels, 5% 100 kbp / 15% through BiWFA), default
+    length classes, Poisson arrivals: every served result equals the
+    reference's (tests/golden/long_pairs.json) and the same request aligned
+    alone."""
+    import hashlib
+    import json
+    import os
+    from conftest import GOLDEN
+    s = json.load(open(os.path.join(GOLDEN, "long_pairs.json")))["sets"]["c4_mix"]
+    ps = datagen.generate(s["shape"], s["seed"], s["first_n"])
+    want = s["pairs"]
+    n = ps.n
+    is_long = np.array([max(int(ps.plen[i]), int(ps.tlen[i])) >= s["long_min_length"] for i in range(n)])
+    say("serving: configuration mix, %d requests (%d of 100 kbp)" % (n, int(is_long.sum())))
+    srv = wfa_amd.Server(long_min_length=s["long_min_length"], lane_memory_bytes=8 << 30)
+    arrivals = np.cumsum(np.random.default_rng(5).exponential(1.0 / 50.0, n))  # ~50 requests/s
+    srv.replay(ps, arrivals)
+    got = collect_all(srv, n, timeout_s=110)
+    srv.close()
+    assert sorted(got) == list(range(n))
+    for i in range(n):
+        w = want[i]
+        assert (got[i].status, got[i].score) == (w["status"], w["score"]), i
+        assert hashlib.sha256(got[i].ops).hexdigest() == w["sha256"], i
+    alone = {"high": wfa_amd.Aligner(metric="affine", penalties=(0, 4, 6, 2)),
+             "ultralow": wfa_amd.Aligner(metric="affine", penalties=(0, 4, 6, 2), memory="ultralow")}
+    for i in range(n):
+        st, sc, ops = alone["ultralow" if is_long[i] else "high"].align(ps.pattern(i), ps.text(i))
+        assert (st, sc, ops) == (got[i].status, got[i].score, got[i].ops), i
+    lat = np.array([got[i].latency for i in range(n)]) * 1e3
+    say("served %d requests: p50 %.1f ms, p99 %.1f ms (short p50 %.1f ms, long p50 %.1f ms)" % (
+        n, np.percentile(lat, 50), np.percentile(lat, 99), np.percentile(lat[~is_long], 50),
+        np.percentile(lat[is_long], 50)))

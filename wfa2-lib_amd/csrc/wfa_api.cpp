@@ -40,18 +40,24 @@ bool hip_ok(hipError_t e, const char* what) {
   return false;
 }
 
-bool DevBuf::ensure(size_t bytes) {
+bool DevBuf::ensure(size_t bytes, size_t max_bytes) {
   if (bytes <= cap && p) return true;
-  // Geometric growth: hipFree waits for all work on the device, so a buffer
-  // that is used again and again (serving lanes, repeated batches) should
-  // reallocate rarely.
-  const size_t want = std::max<size_t>(std::max<size_t>(bytes, 256), p ? std::min(2 * cap, cap + (size_t(4) << 30)) : 0);
+  // Geometric growth, clamped to max_bytes: a buffer used again and again
+  // (serving lanes, repeated batches) reallocates rarely.  If the grown size
+  // does not fit, the exact request is tried.
+  size_t want = std::max<size_t>(std::max<size_t>(bytes, 256), p ? std::min(2 * cap, cap + (size_t(4) << 30)) : 0);
+  want = std::max(bytes, std::min(want, max_bytes));
   if (p) (void)hipFree(p);
   p = nullptr;
   cap = 0;
   if (hipMalloc(&p, want) != hipSuccess) {
     p = nullptr;
-    return false;
+    if (want == bytes || hipMalloc(&p, bytes) != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      return false;
+    }
+    want = bytes;
   }
   cap = want;
   return true;
@@ -69,6 +75,25 @@ void PlanGroup::release() {
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
   ev[0] = ev[1] = ev[2] = nullptr;
+}
+
+PlanGroup& PlanGroup::operator=(PlanGroup&& o) noexcept {
+  if (this == &o) return *this;
+  release();
+  family = o.family;
+  block = o.block;
+  wcap = o.wcap;
+  lmax_words = o.lmax_words;
+  ids = std::move(o.ids);
+  d_ids = std::move(o.d_ids);
+  ids_on_device = o.ids_on_device;
+  est_rec_bytes = o.est_rec_bytes;
+  for (int j = 0; j < 3; ++j) {
+    ev[j] = o.ev[j];
+    o.ev[j] = nullptr;
+  }
+  o.ids_on_device = false;
+  return *this;
 }
 
 bool HostBuf::ensure(size_t bytes) {
@@ -91,7 +116,32 @@ void HostBuf::release() {
   cap = 0;
 }
 
-void recycle_plan(std::vector<PlanGroup>& old_plan, std::vector<PlanGroup>& new_plan) {
+PlanGroup take_group(Engine* e) {
+  PlanGroup g;
+  if (e && !e->group_pool.empty()) {
+    g = std::move(e->group_pool.back());
+    e->group_pool.pop_back();
+  }
+  g.family = 0;
+  g.block = 64;
+  g.wcap = 0;
+  g.lmax_words = 0;
+  g.ids.clear();
+  g.ids_on_device = false;
+  g.est_rec_bytes = -1;
+  return g;
+}
+
+void give_back_groups(Engine* e, std::vector<PlanGroup>& groups) {
+  for (auto& g : groups) {
+    g.ids.clear();
+    g.ids_on_device = false;
+    if (e) e->group_pool.push_back(std::move(g));
+  }
+  groups.clear();
+}
+
+void recycle_plan(Engine* e, std::vector<PlanGroup>& old_plan, std::vector<PlanGroup>& new_plan) {
   size_t i = 0;
   for (; i < old_plan.size() && i < new_plan.size(); ++i) {
     PlanGroup& o = old_plan[i];
@@ -100,8 +150,22 @@ void recycle_plan(std::vector<PlanGroup>& old_plan, std::vector<PlanGroup>& new_
     for (int j = 0; j < 3; ++j) std::swap(o.ev[j], n.ev[j]);
     n.ids_on_device = false;
   }
-  for (auto& g : old_plan) g.release();
-  old_plan.clear();
+  // groups beyond the old plan take pooled buffers and events
+  for (; i < new_plan.size(); ++i) {
+    if (!e || e->group_pool.empty()) break;
+    PlanGroup& n = new_plan[i];
+    PlanGroup& o = e->group_pool.back();
+    std::swap(o.d_ids, n.d_ids);
+    for (int j = 0; j < 3; ++j) std::swap(o.ev[j], n.ev[j]);
+    n.ids_on_device = false;
+    e->group_pool.pop_back();
+  }
+  give_back_groups(e, old_plan);
+}
+
+Engine::~Engine() {
+  for (auto& ev : bi.ev)
+    if (ev) (void)hipEventDestroy(ev);
 }
 
 Engine* engine_of(wavefront_aligner_t* wf) { return static_cast<Engine*>(wf->device); }
@@ -134,7 +198,8 @@ bool engine_init(Engine* e) {
   if (env) cap = std::min<size_t>(strtoull(env, nullptr, 10) << 20, kMaxArenaBytes);
   if (e->rec_cap) cap = std::min(cap, e->rec_cap);  // a cap set before the first run
   e->rec_cap = std::max<size_t>(cap, 64ull << 20);
-  e->rec_bytes = std::min<size_t>(8ull << 30, e->rec_cap);
+  // The arena starts at 64 MB; each run grows it to its pairs' estimate.
+  e->rec_bytes = std::min<size_t>(64ull << 20, e->rec_cap);
   e->ok = true;
   return true;
 }
@@ -590,8 +655,8 @@ const LeanTuning& lean_tuning() {
 
 namespace {
 
-PlanGroup make_large(const JobSet& js, std::vector<int> ids) {
-  PlanGroup g;
+PlanGroup make_large(Engine* e, const JobSet& js, std::vector<int> ids) {
+  PlanGroup g = take_group(e);
   g.family = wfa::kLarge;
   int pmax = 0, tmax = 0;
   for (int i : ids) { pmax = std::max(pmax, js.h_plen[i]); tmax = std::max(tmax, js.h_tlen[i]); }
@@ -721,7 +786,7 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
 
 // Device memory every group of the plan needs, allocated before any launch.
 bool reserve_for(Engine* e, std::vector<PlanGroup>& plan, int bits, const Penalties& pen, int cut, int records) {
-  if (records && !e->rec.ensure(e->rec_bytes)) {
+  if (records && !e->rec.ensure(e->rec_bytes, e->rec_cap)) {
     fprintf(stderr, "[WFA] cannot allocate the %zu MB record arena\n", e->rec_bytes >> 20);
     return false;
   }
@@ -797,7 +862,7 @@ std::vector<PlanGroup> make_plan(const JobSet& js, const std::vector<int>& ids, 
     plan.push_back(std::move(g));
   }
   for (auto& c : cls)
-    if (!c.empty()) plan.push_back(make_large(js, std::move(c)));
+    if (!c.empty()) plan.push_back(make_large(nullptr, js, std::move(c)));
   return plan;
 }
 
@@ -871,7 +936,7 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
       }
       if (!ring_over.empty()) {
         if (g.family == wfa::kLean && wfa::lean_width(g.wcap + 1) > 0) {
-          PlanGroup h;
+          PlanGroup h = take_group(e);
           h.family = wfa::kLean;
           h.block = 64;
           h.wcap = wfa::lean_width(g.wcap + 1);
@@ -879,7 +944,7 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
           h.ids = std::move(ring_over);
           next.push_back(std::move(h));
         } else {
-          next.push_back(make_large(js, std::move(ring_over)));
+          next.push_back(make_large(e, js, std::move(ring_over)));
         }
       }
       if (!arena_full.empty()) {
@@ -902,7 +967,7 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
           }
         }
         if (!arena_full.empty()) {
-          PlanGroup h;
+          PlanGroup h = take_group(e);
           h.family = g.family;
           h.block = g.block;
           h.wcap = g.wcap;
@@ -912,12 +977,12 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
         }
       }
     }
-    for (auto& g : retry_groups) g.release();
+    give_back_groups(e, retry_groups);
     retry_groups = std::move(next);
     pending = &retry_groups;
     if (retry_groups.empty()) break;
   }
-  for (auto& g : retry_groups) g.release();
+  give_back_groups(e, retry_groups);
   tot->dsteps += host_c.dsteps;
   tot->fail += host_c.fail;
   return WF_STATUS_OK;
@@ -1204,7 +1269,7 @@ extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_ba
                                   b->heur.cut()};
     if (b->plan_key != key) {
       std::vector<PlanGroup> plan = make_plan(b->jobs(), b->sorted, pen);
-      recycle_plan(b->plan, plan);
+      recycle_plan(e, b->plan, plan);
       b->plan = std::move(plan);
       b->plan_key = key;
     }
@@ -1216,6 +1281,7 @@ extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_ba
   b->stats.forward_ms = tot.forward_ms;
   b->stats.backtrace_ms = tot.backtrace_ms;
   b->stats.forward_launches = tot.launches;
+  b->stats.record_arena_bytes = (int64_t)e->rec.cap;
   b->have_results = true;
   b->host_results = false;
   if (tot.fail == 0) return WF_STATUS_ALG_COMPLETED;

@@ -93,7 +93,8 @@ bool ensure_reverse(Engine* e, wfa_device_batch_t* b) {
     len[i] = b->plen[i];
     len[n + i] = b->tlen[i];
   }
-  DevBuf d_pos, d_len;
+  DevBuf& d_pos = e->bi.pos;
+  DevBuf& d_len = e->bi.len;
   if (!d_pos.ensure(pos.size() * 8) || !d_len.ensure(len.size() * 4)) return false;
   hipStream_t st = e->stream;
   bool ok = hip_ok(hipMemcpyAsync(d_pos.p, pos.data(), pos.size() * 8, hipMemcpyHostToDevice, st), "reverse") &&
@@ -102,8 +103,6 @@ bool ensure_reverse(Engine* e, wfa_device_batch_t* b) {
                                             d_len.as<int>(), 2 * n, b->bits, b->rev_words.as<uint64_t>(), st),
                    "reverse pack") &&
             hip_ok(hipStreamSynchronize(st), "reverse pack");
-  d_pos.release();
-  d_len.release();
   b->have_reverse = ok;
   return ok;
 }
@@ -142,7 +141,8 @@ bool run_breakpoints(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, 
     if (per_cu <= 0) per_cu = 1;
     const int grid = (int)std::min<int64_t>((int64_t)ids.size(), (int64_t)per_cu * e->num_cus);
     const long long stride = 2LL * pen.nslots * ncomp * wcap;
-    DevBuf d_probs, d_out;
+    DevBuf& d_probs = e->bi.probs;
+    DevBuf& d_out = e->bi.out;
     if (!d_probs.ensure(probs.size() * sizeof(wfa::BiProblem)) || !d_out.ensure(probs.size() * sizeof(wfa::BiOutcome)) ||
         !e->ring.ensure((size_t)stride * grid * sizeof(int))) {
       fprintf(stderr, "[WFA] cannot allocate BiWFA breakpoint buffers\n");
@@ -169,9 +169,9 @@ bool run_breakpoints(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, 
     a.max_steps = wf->system.max_alignment_steps > 0 ? wf->system.max_alignment_steps : INT_MAX;
     a.heur = heur;  // the breakpoint aligners inherit the heuristic (wavefront_bialigner.c:53)
     a.pen = pen;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    (void)hipEventCreate(&e0);
-    (void)hipEventCreate(&e1);
+    for (auto& ev : e->bi.ev)
+      if (!ev && !hip_ok(hipEventCreate(&ev), "hipEventCreate")) return false;
+    hipEvent_t e0 = e->bi.ev[0], e1 = e->bi.ev[1];
     std::vector<wfa::BiOutcome> res(probs.size());
     wfa::Counters host_c{};
     bool ok = hip_ok(hipMemcpyAsync(d_probs.p, probs.data(), probs.size() * sizeof(wfa::BiProblem),
@@ -186,10 +186,6 @@ bool run_breakpoints(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, 
          hip_ok(hipStreamSynchronize(st), "breakpoint");
     float ms = 0;
     (void)hipEventElapsedTime(&ms, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    d_probs.release();
-    d_out.release();
     if (!ok) return false;
     tot->forward_ms += ms;
     tot->launches += 1;
@@ -306,7 +302,10 @@ int bialign_run(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, RunTo
       boff[j] = bops;
       bops += (int64_t)bpl[j] + btl[j];
     }
-    DevBuf d_pl, d_tl, d_pp, d_tp, d_cb, d_ce, d_st, d_sc, d_last, d_ds, d_off, d_beg, d_ops;
+    Engine::BiBuffers& bb = e->bi;
+    DevBuf &d_pl = bb.pl, &d_tl = bb.tl, &d_pp = bb.pp, &d_tp = bb.tp, &d_cb = bb.cb, &d_ce = bb.ce,
+           &d_st = bb.st, &d_sc = bb.sc, &d_last = bb.last, &d_ds = bb.ds, &d_off = bb.off, &d_beg = bb.beg,
+           &d_ops = bb.ops;
     hipStream_t st = e->stream;
     if (nb > 0) {
       const size_t m = (size_t)nb;
@@ -348,7 +347,7 @@ int bialign_run(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, RunTo
       std::vector<PlanGroup> plan = make_plan(js, ids, make_penalties(wf->penalties));
       RunTotals bt;
       const int rc = run_jobs(wf, e, js, plan, 1, &bt, bstatus);
-      for (auto& g : plan) g.release();
+      give_back_groups(e, plan);
       if (rc != WF_STATUS_OK) return rc;
       tot->forward_ms += bt.forward_ms;
       tot->backtrace_ms += bt.backtrace_ms;
@@ -421,22 +420,19 @@ int bialign_run(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, RunTo
       }
     }
     if (!pieces.empty()) {
-      DevBuf d_pieces;
+      DevBuf& d_pieces = e->bi.pieces;
       if (!d_pieces.ensure(pieces.size() * sizeof(wfa::OpsPiece))) return WF_STATUS_OOM;
       bool ok = hip_ok(hipMemcpyAsync(d_pieces.p, pieces.data(), pieces.size() * sizeof(wfa::OpsPiece),
                                       hipMemcpyHostToDevice, st), "pieces") &&
                 hip_ok(wfa::launch_gather_ops(d_pieces.as<wfa::OpsPiece>(), (int)pieces.size(),
                                               d_ops.p ? d_ops.as<char>() : nullptr, b->d_ops.as<char>(), st), "gather") &&
                 hip_ok(hipStreamSynchronize(st), "gather");
-      d_pieces.release();
       if (!ok) return WF_STATUS_OOM;
     }
     if (!hip_ok(hipMemcpyAsync(b->d_ops_begin.p, begin.data(), (size_t)n * 4, hipMemcpyHostToDevice, e->stream),
                 "results") ||
         !hip_ok(hipStreamSynchronize(e->stream), "results"))
       return WF_STATUS_OOM;
-    for (DevBuf* x : {&d_pl, &d_tl, &d_pp, &d_tp, &d_cb, &d_ce, &d_st, &d_sc, &d_last, &d_ds, &d_off, &d_beg, &d_ops})
-      x->release();
   }
   // Per-pair results into the batch (status on the device, scores on the host).
   std::vector<int> status(n);

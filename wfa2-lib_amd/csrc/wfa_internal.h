@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <utility>
 #include <vector>
 
 #include "wfa_device.h"
@@ -13,11 +14,23 @@ namespace wfa_host {
 
 bool hip_ok(hipError_t e, const char* what);
 
-// Grow-only device allocation.
+// Grow-only device allocation, freed by its owner's destructor.  hipFree
+// waits for the whole device (other aligners' kernels included), so buffers
+// that are used again and again grow geometrically up to max_bytes and are
+// never freed between runs.
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
-  bool ensure(size_t bytes);
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), cap(o.cap) { o.p = nullptr; o.cap = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; cap = o.cap; o.p = nullptr; o.cap = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  bool ensure(size_t bytes, size_t max_bytes = SIZE_MAX);
   void release();
   template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
 };
@@ -26,6 +39,39 @@ struct DevBuf {
 struct DevView {
   void* p = nullptr;
   template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+// Grow-only pinned host staging buffer (hipHostMalloc), freed by its owner.
+struct HostBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  HostBuf() = default;
+  HostBuf(const HostBuf&) = delete;
+  HostBuf& operator=(const HostBuf&) = delete;
+  ~HostBuf() { release(); }
+  bool ensure(size_t bytes);
+  void release();
+  template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+// One forward launch (+ backtrace) over a subset of jobs with one kernel family.
+struct PlanGroup {
+  int family = 0;
+  int block = 64;
+  int wcap = 0;
+  int lmax_words = 0;
+  std::vector<int> ids;
+  DevBuf d_ids;
+  bool ids_on_device = false;  // d_ids holds ids (uploaded once per plan)
+  double est_rec_bytes = -1;   // rough record footprint of the group (computed once)
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  PlanGroup() = default;
+  PlanGroup(const PlanGroup&) = delete;
+  PlanGroup& operator=(const PlanGroup&) = delete;
+  PlanGroup(PlanGroup&& o) noexcept { *this = std::move(o); }
+  PlanGroup& operator=(PlanGroup&& o) noexcept;
+  ~PlanGroup() { release(); }
+  void release();
 };
 
 // Per-aligner device state: one stream, the backtrace-record arena, the HBM
@@ -45,7 +91,28 @@ struct Engine {
   // a stream created with a CU mask (here: every CU) gets a new queue.
   bool own_queue = false;
   ::wfa_device_batch_t* scratch = nullptr;  // reused by wavefront_align / wavefront_align_batch
+  // Plan groups no longer in use (their order buffers and events), handed
+  // out again by take_group: retries and re-planning allocate nothing new.
+  std::vector<PlanGroup> group_pool;
+  // BiWFA: grow-only per-aligner buffers of the breakpoint launches, the base
+  // jobs and the CIGAR gather (wfa_bialign.cpp).
+  struct BiBuffers {
+    DevBuf pos, len, probs, out, pieces;
+    DevBuf pl, tl, pp, tp, cb, ce, st, sc, last, ds, off, beg, ops;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+  } bi;
+  ~Engine();
 };
+
+// A group with buffers and events from the engine's pool (or fresh ones),
+// its ids and plan fields cleared.
+PlanGroup take_group(Engine* e);
+// Returns groups' buffers and events to the engine's pool and clears the vector.
+void give_back_groups(Engine* e, std::vector<PlanGroup>& groups);
+// Moves the device buffers and events of old groups into new ones (same
+// index); what is left goes to the engine's pool, so re-planning allocates
+// nothing new when the plan does not grow.
+void recycle_plan(Engine* e, std::vector<PlanGroup>& old_plan, std::vector<PlanGroup>& new_plan);
 
 Engine* engine_of(wavefront_aligner_t* wf);
 bool engine_init(Engine* e);
@@ -95,33 +162,6 @@ struct JobSet {
 
 // The aligner's wf-adaptive parameters (none unless the strategy is wf-adaptive).
 wfa::HeurArgs heuristic_of(const wavefront_aligner_t* wf);
-
-// Grow-only pinned host staging buffer (hipHostMalloc).
-struct HostBuf {
-  void* p = nullptr;
-  size_t cap = 0;
-  bool ensure(size_t bytes);
-  void release();
-  template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
-};
-
-// One forward launch (+ backtrace) over a subset of jobs with one kernel family.
-struct PlanGroup {
-  int family = 0;
-  int block = 64;
-  int wcap = 0;
-  int lmax_words = 0;
-  std::vector<int> ids;
-  DevBuf d_ids;
-  bool ids_on_device = false;  // d_ids holds ids (uploaded once per plan)
-  double est_rec_bytes = -1;   // rough record footprint of the group (computed once)
-  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-  void release();
-};
-// Moves the device buffers and events of old groups into new ones (same
-// index), releasing what is left: re-planning allocates nothing new when the
-// plan does not grow.
-void recycle_plan(std::vector<PlanGroup>& old_plan, std::vector<PlanGroup>& new_plan);
 
 struct RunTotals {
   double forward_ms = 0;

@@ -148,7 +148,7 @@ class wfa_server_result_t(Structure):
 
 class wfa_run_stats_t(Structure):
     _fields_ = [("diagonal_steps", c_int64), ("forward_ms", c_double), ("backtrace_ms", c_double),
-                ("forward_launches", c_int)]
+                ("forward_launches", c_int), ("record_arena_bytes", c_int64)]
 
 
 _lib = None
