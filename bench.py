@@ -27,6 +27,64 @@ HBM_PEAK_GBS = 8000.0                 # MI355X HBM3E
 VALU_INT32_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12   # 256 CUs x 64 lanes x 2.4 GHz
 BYTES_PER_DSTEP = 44                  # gap-affine: 4 int32 in, 3 int32 out, 2 x 8 B sequence window
 OPS_PER_DSTEP = 20
+# per metric: (algorithmic bytes, integer ops) per diagonal-step
+DSTEP_COST = {"affine": (44, 20), "affine2p": (64, 30)}
+
+
+def host_cpu():
+    """The host's CPU: model name, logical CPUs, physical cores (distinct
+    (package, core) ids), the CPUs this process may run on, the cgroup CPU
+    quota, and the threads the CPU baselines use: the job's CPU share (the
+    affinity set, the quota and OMP_NUM_THREADS, whichever is smallest)."""
+    model, cores = None, set()
+    phys = core = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and model is None:
+                model = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+            elif not k and phys is not None:
+                cores.add((phys, core))
+                phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    logical = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = logical
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    share = affinity
+    if quota:
+        share = min(share, max(1, int(quota)))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        share = min(share, int(omp))
+    return {"model": model, "logical_cpus": logical, "physical_cores": len(cores) or None,
+            "affinity_cpus": affinity, "cgroup_cpu_quota": quota, "threads_used": share}
+
+
+_HOST = None
+
+
+def cpu_threads():
+    global _HOST
+    if _HOST is None:
+        _HOST = host_cpu()
+    return _HOST["threads_used"]
 
 
 def log(msg):
@@ -73,7 +131,7 @@ def cpu_baseline(ps, threads, res=None):
             "sample": "all %d pairs of the workload, wavefront_align of WFA2-lib v2.3 built -O3 "
                       "(%s) with one aligner per OpenMP thread, memory-high, score+CIGAR; "
                       "%.2f s wall" % (pairs.n, os.path.basename(O.ref_lib_path()), secs),
-            "gcups": ps.cells / secs / 1e9}
+            "gcups": ps.cells / secs / 1e9, "host": _HOST}
 
 
 # Configurations beyond the headline one (indel / edit / gap-linear, ends-free,
@@ -253,11 +311,14 @@ def batch_rate(al, ps, steps=3):
 
 
 LONG_WORKLOADS = {
-    # name: (datagen shape, metric, penalties, memory mode, default pairs, CPU sample pairs, description)
-    "affine2p-1-20kbp": (2, "affine2p", (0, 4, 6, 2, 24, 1), "high", 3000, 32,
+    # name: (datagen shape, metric, penalties, GPU memory mode, default pairs, CPU sample pairs,
+    #        CPU memory mode, description)
+    # The reference's memory mode med gives the CIGARs of memory high (tests/wfa.utest.sh:67-78) with
+    # O(s) memory per thread, so its CPU sample of the 1-20 kbp pairs runs in med on every thread.
+    "affine2p-1-20kbp": (2, "affine2p", (0, 4, 6, 2, 24, 1), "high", 3000, 1000, "med",
                          "dual-cost gap-affine (0,4,6,2,24,1) end-to-end pairs, L log-uniform in [1,20] kbp, "
                          "error ~U[1%,15%], geometric indels (mean 3), score+CIGAR, memory-high"),
-    "biwfa-100kbp": (3, "affine", (0, 4, 6, 2), "ultralow", 256, 8,
+    "biwfa-100kbp": (3, "affine", (0, 4, 6, 2), "ultralow", 256, 32, "ultralow",
                      "gap-affine (0,4,6,2) end-to-end pairs of 100 kbp, 15% errors, geometric indels "
                      "(mean 2), score+CIGAR, BiWFA (memory ultralow)"),
 }
@@ -265,17 +326,25 @@ LONG_WORKLOADS = {
 
 def long_workload(name, n, no_cpu):
     """One run over the first n pairs of a long-pair workload (a run takes
-    seconds, so one untimed warm-up on a small slice only), with the
-    reference timed on a small seeded sample of the same pairs and the GPU
-    results of that sample compared with its outputs."""
+    seconds, so one untimed warm-up on a small slice only).  Reports pairs/s,
+    equivalent GCUPS, the HBM and VALU fractions of the forward launches (the
+    contract's bytes and ops per diagonal-step over their HIP-event time),
+    and the reference on this host's CPU share over the first pairs of the
+    same set, whose results must equal the GPU's."""
     import numpy as np
     import wfa_amd
     from wfa_amd import datagen
-    shape, metric, pen, mem, n_def, n_cpu, desc = LONG_WORKLOADS[name]
+    shape, metric, pen, mem, n_def, n_cpu, cpu_mem, desc = LONG_WORKLOADS[name]
     n = n or n_def
     ps = datagen.generate(shape, 7, n)
     al = wfa_amd.Aligner(metric=metric, penalties=pen, memory=mem)
     al.align_batch(ps.subset(list(range(min(4, n)))))
+    if mem != "ultralow" and os.environ.get("WFA_AUTOTUNE", "1") != "0":
+        # the launch-shape tuner measures its candidates on pairs of the same
+        # workload (another seed), outside the timed run
+        t = time.perf_counter()
+        al.align_batch(datagen.generate(shape, 8, min(n, 800)))
+        log("%s: tuning run on %d pairs of seed 8 in %.2f s" % (name, min(n, 800), time.perf_counter() - t))
     db = al.upload(ps)
     t = time.perf_counter()
     rc = db.run()
@@ -284,37 +353,58 @@ def long_workload(name, n, no_cpu):
     if rc != 0:
         raise RuntimeError("run returned %d" % rc)
     res = db.download()
-    log("%s: %d pairs in %.2f s (%.1f pairs/s, %.3e diagonal-steps/s)" % (name, n, dt, n / dt, st.diagonal_steps / dt))
+    db.free()
+    bytes_ds, ops_ds = DSTEP_COST[metric]
+    W = int(st.diagonal_steps)
+    fwd_s = st.forward_ms * 1e-3
+    log("%s: %d pairs in %.2f s (%.1f pairs/s, %.3e diagonal-steps/s; forward %.2f s, backtrace %.2f s)"
+        % (name, n, dt, n / dt, W / dt, fwd_s, st.backtrace_ms * 1e-3))
     cpu = None
+    spot = None
     if not no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
         if O.ref_available():
-            idx = list(range(min(n_cpu, n)))
+            k = min(n_cpu, n)
+            idx = list(range(k))
             sub = O.Pairs([ps.pattern(i) for i in idx], [ps.text(i) for i in idx])
-            conf = O.Conf(metric, pen, mem)
-            threads = min(16, os.cpu_count() or 1, len(idx))
-            secs, _ = O.ref_time(conf, sub, threads)
-            ref = O.ref_align(conf, sub, threads)
+            threads = cpu_threads()
+            ref = O.ref_align(O.Conf(metric, pen, cpu_mem), sub, threads)
+            secs = ref.seconds
             bad = sum(1 for j, i in enumerate(idx) if res.status[i] != ref.status[j] or res.score[i] != ref.score[j]
                       or res.ops(i) != ref.ops(j))
+            spot = {"pairs": k, "differ": int(bad)}
             if bad:
-                raise RuntimeError("GPU results differ from the reference build on %d pairs" % bad)
+                raise RuntimeError("%s: GPU results differ from the reference build on %d pairs" % (name, bad))
             cells = float(sum(int(ps.plen[i]) * int(ps.tlen[i]) for i in idx))
-            cpu = {"value": len(idx) / secs, "unit": "pairs/s", "cores": threads, "kind": "reference",
-                   "sample": "first %d pairs, wavefront_align of the reference built -O3 with OpenMP, one "
-                             "aligner per thread; %.1f s wall; all %d identical to the GPU results"
-                             % (len(idx), secs, len(idx)), "gcups": cells / secs / 1e9}
-            log("reference: %.2f pairs/s on %d threads; %d pairs identical" % (len(idx) / secs, threads, len(idx)))
-    return {"metric": "pairs/s & equiv GCUPS, batched, exact CIGAR", "value": n / dt, "unit": "pairs/s",
-            "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": dt * 1e3, "higher_is_better": True,
+            cpu = {"value": k / secs, "unit": "pairs/s", "cores": threads, "kind": "reference",
+                   "sample": "first %d pairs of the set, wavefront_align of the reference built -O3 with OpenMP "
+                             "(%s), one aligner per thread, memory mode %s; %.1f s wall; all %d results identical "
+                             "to the GPU's" % (k, os.path.basename(O.ref_lib_path()), cpu_mem, secs, k),
+                   "gcups": cells / secs / 1e9, "host": _HOST}
+            log("%s reference: %.2f pairs/s on %d threads; %d pairs identical" % (name, k / secs, threads, k))
+    return {"workload": "%s: first %d pairs, %s" % (name, n, desc), "pairs": n, "value": n / dt,
+            "unit": "pairs/s", "seconds": dt, "sum_nm_cells": ps.cells, "gcups": ps.cells / dt / 1e9,
+            "diagonal_steps": W, "diagonal_steps_per_s": W / dt,
+            "forward_ms": st.forward_ms, "backtrace_ms": st.backtrace_ms, "launches": st.forward_launches,
+            "roofline": {"bytes_per_diagonal_step": bytes_ds, "ops_per_diagonal_step": ops_ds,
+                         "hbm_achieved_gbs": W * bytes_ds / fwd_s / 1e9 if fwd_s else None,
+                         "hbm_frac": W * bytes_ds / fwd_s / (HBM_PEAK_GBS * 1e9) if fwd_s else None,
+                         "valu_int32_frac": W * ops_ds / fwd_s / (VALU_INT32_PEAK_TOPS * 1e12) if fwd_s else None,
+                         "over": "forward launches (HIP events on the aligner's stream)"},
+            "parity_spot_check": spot, "cpu_baseline": cpu}
+
+
+def long_line(name, n, no_cpu):
+    """A long workload as a bench line of its own (--workload)."""
+    r = long_workload(name, n, no_cpu)
+    return {"metric": "pairs/s & equiv GCUPS, batched, exact CIGAR", "value": r["value"], "unit": "pairs/s",
+            "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": r["seconds"] * 1e3, "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "int32",
             "data": "synthetic (seeded SplitMix64 generator)",
-            "config": {"workload": "%s: first %d pairs, %s" % (name, n, desc), "pairs": n,
-                       "sum_nm_cells": ps.cells, "diagonal_steps": int(st.diagonal_steps)},
-            "gcups": ps.cells / dt / 1e9, "diagonal_steps_per_s": st.diagonal_steps / dt,
-            "forward_ms": st.forward_ms, "backtrace_ms": st.backtrace_ms, "launches": st.forward_launches,
-            "cpu_baseline": cpu}
+            "config": {"workload": r["workload"], "pairs": r["pairs"], "sum_nm_cells": r["sum_nm_cells"],
+                       "diagonal_steps": r["diagonal_steps"]},
+            "detail": r, "cpu_baseline": r["cpu_baseline"]}
 
 
 def copy_calibration(mb):
@@ -340,7 +430,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="open-loop serving run length (0: skip)")
-    ap.add_argument("--serve-lanes", default="1,1,1", help="short,medium,long lanes of the request loop")
+    ap.add_argument("--serve-lanes", default="2,2,2", help="short,medium,long lanes of the request loop")
     ap.add_argument("--extras", type=int, default=1,
                     help="0: only the short-read batch (plain form, 1 kbp case and serving skipped)")
     ap.add_argument("--seq-pairs", type=int, default=500000, help="pairs of the .seq -> .alg run (0: skip)")
@@ -349,6 +439,8 @@ def main():
     ap.add_argument("--workload", choices=("short-reads",) + tuple(LONG_WORKLOADS), default="short-reads",
                     help="short-reads: the headline batch; the others print their own line")
     ap.add_argument("--long-pairs", type=int, default=0, help="pairs of a long workload (0: its default)")
+    ap.add_argument("--long-sets", type=int, default=1,
+                    help="1: the default run also measures the 1-20 kbp and 100 kbp workloads on subsets")
     ap.add_argument("--mode", choices=("align", "copy"), default="align",
                     help="copy: one device copy of --copy-mb MiB (HBM counter calibration)")
     ap.add_argument("--copy-mb", type=int, default=1024)
@@ -360,7 +452,7 @@ def main():
         if args.gpus != 1:
             print("bench.py measures one GPU; run with --gpus 1.", file=sys.stderr)
             sys.exit(2)
-        print(json.dumps(long_workload(args.workload, args.long_pairs, args.no_cpu)))
+        print(json.dumps(long_line(args.workload, args.long_pairs, args.no_cpu)))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the pairs are independent, so N GPUs would each run "
@@ -399,12 +491,30 @@ def main():
     pairs_per_s = ps.n * args.steps / total
     gcups = ps.cells * args.steps / total / 1e9
 
-    # Results -> host (outside the timed region), for the PCIe-inclusive rate.
+    # Results -> host (outside the timed region), for the PCIe-inclusive
+    # rates: run-length CIGARs made on the device into a reused buffer
+    # (wfa_device_batch_download_runs), and every operation byte
+    # (wfa_device_batch_download).
+    runs_buf = None
+    d2h_runs = []
+    for _ in range(3):
+        t = time.perf_counter()
+        rres = db.download_runs(runs_buf)
+        d2h_runs.append(time.perf_counter() - t)
+        runs_buf = rres.runs.base if rres.runs.base is not None else rres.runs
+        db.run()  # the next download makes its runs again
+    d2h_runs_s = min(d2h_runs)
     t = time.perf_counter()
     res = db.download()
     d2h_s = time.perf_counter() - t
     if (res.status != 0).any():
         raise RuntimeError("some pairs did not complete")
+    import numpy as _np
+    for i in _np.random.default_rng(1).choice(ps.n, size=2000, replace=False):
+        if rres.ops(int(i)) != res.ops(int(i)) or rres.score[i] != res.score[i]:
+            raise RuntimeError("run-length CIGAR of pair %d differs from its operations" % i)
+    log("results to host: run-length CIGARs %.1f ms (%d runs), operation bytes %.1f ms"
+        % (d2h_runs_s * 1e3, len(rres.runs), d2h_s * 1e3))
 
     # Plain form: the same kernels, one pair per launch (wavefront_align),
     # against the batched form on 10^5 mixed-length pairs (the first 10^5
@@ -437,7 +547,7 @@ def main():
 
     variants = None
     if args.extras and args.variant_pairs > 0:
-        variants = variant_measurements(args.variant_pairs, 4000, min(16, os.cpu_count() or 1), args.no_cpu)
+        variants = variant_measurements(args.variant_pairs, 4000, cpu_threads(), args.no_cpu)
 
     seq_file = seq_file_measurement(args.seq_pairs) if args.extras and args.seq_pairs > 0 else None
 
@@ -447,6 +557,13 @@ def main():
         log("serving: p50 %.2f ms, p99 %.2f ms at %.1f requests/s"
             % (serve["p50_ms"], serve["p99_ms"], serve["rate_req_s"]))
 
+    # The 1-20 kbp dual-cost and the 100 kbp BiWFA workloads, over seeded
+    # subsets (sizes in each sub-line), with the reference on this host.
+    longs = {}
+    if args.extras and args.long_sets:
+        for key, name in (("affine2p_1_20kbp", "affine2p-1-20kbp"), ("biwfa_100kbp", "biwfa-100kbp")):
+            longs[key] = long_workload(name, 0, args.no_cpu)
+
     fwd_ms = float(np.mean(fwd))
     launches_per_step = float(np.mean(launches))
     alg_bytes = BYTES_PER_DSTEP * dsteps
@@ -455,7 +572,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         cpu = cpu_baseline(ps, threads, res)
         if cpu and cpu["parity_spot_check"]["differ"]:
             raise RuntimeError("GPU results differ from the reference build")
@@ -484,12 +601,20 @@ def main():
         "forward_ms_per_step": fwd_ms,
         "backtrace_ms_per_step": float(np.mean(bt)),
         "forward_launches_per_step": launches_per_step,
-        "pcie_inclusive_pairs_per_s": ps.n / (total / args.steps + d2h_s),
+        "pcie_inclusive_pairs_per_s": ps.n / (total / args.steps + d2h_runs_s),
+        "pcie_inclusive": {"runs_pairs_per_s": ps.n / (total / args.steps + d2h_runs_s),
+                           "ops_pairs_per_s": ps.n / (total / args.steps + d2h_s),
+                           "runs_download_ms": d2h_runs_s * 1e3, "ops_download_ms": d2h_s * 1e3,
+                           "runs": int(len(rres.runs)),
+                           "what": "one step plus the copy of every pair's status, score and CIGAR to host "
+                                   "memory: as run-length words made on the device (the reported rate), or as "
+                                   "one byte per operation"},
         "plain_gpu": plain,
         "kbp1_5pct": c0_out,
         "variants": variants,
         "seq_file": seq_file,
         "serve": serve,
+        "long_workloads": longs or None,
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,

@@ -373,6 +373,19 @@ int wfa_device_batch_run(wavefront_aligner_t* const wf_aligner, wfa_device_batch
 int wfa_device_batch_download(wavefront_aligner_t* const wf_aligner, wfa_device_batch_t* batch,
     wfa_result_t* results, char* cigar_arena, size_t cigar_arena_bytes);
 void wfa_device_batch_free(wfa_device_batch_t* batch);
+/* Run-length CIGARs, made on the device from the last run: each pair's
+ * CIGAR as the runs cigar_get_CIGAR returns with show_mismatches
+ * (alignment/cigar.c:180-239): (length << 4) | code, code 7 ('=') for
+ * matches, 8 (X), 1 (I), 2 (D), pairs back to back.  They are a few times
+ * smaller than the operation bytes, so fewer bytes cross PCIe.
+ * wfa_device_batch_cigar_runs returns the number of runs (to size `runs`),
+ * or a negative status; wfa_device_batch_download_runs fills results as
+ * wfa_device_batch_download does, with cigar_offset / cigar_length counting
+ * runs of `runs`.  Not for the extension form or z-drop (their CIGARs are
+ * trimmed on the host: WF_STATUS_UNSUPPORTED). */
+int64_t wfa_device_batch_cigar_runs(wavefront_aligner_t* const wf_aligner, wfa_device_batch_t* batch);
+int wfa_device_batch_download_runs(wavefront_aligner_t* const wf_aligner, wfa_device_batch_t* batch,
+    wfa_result_t* results, uint32_t* runs, size_t max_runs);
 /* Statistics of the last run: total diagonal-steps (sum over non-null score
    steps of hi-lo+1) and forward / backtrace kernel milliseconds (HIP events
    on the aligner's stream). */
@@ -385,6 +398,7 @@ typedef struct {
 } wfa_run_stats_t;
 int wfa_device_batch_stats(wfa_device_batch_t* batch, wfa_run_stats_t* stats);
 
+
 /* ------------------------------------------------------------ serving
  * No counterpart in the reference (its wavefront_align is synchronous per
  * pair).  A request loop for single-pair requests that arrive over time:
@@ -392,9 +406,13 @@ int wfa_device_batch_stats(wfa_device_batch_t* batch, wfa_run_stats_t* stats);
  * algorithm, long with BiWFA), and worker lanes -- each an aligner with its
  * own HIP stream and resident device buffers -- admit the queued requests of
  * their class into device batches, up to max_batch_pairs and up to the lane's
- * memory budget on each request's estimated footprint.  Batches of different
- * lanes run concurrently on the GPU.  Per request, results are identical to
- * wavefront_align with the same attributes. */
+ * memory budget on each request's estimated footprint: O(s^2) bytes of
+ * backtrace records for the memory-high classes and O(s) for BiWFA, with the
+ * score s estimated from the length, the penalties and error_prior.  Batches
+ * of different lanes run concurrently on the GPU; with two lanes per class,
+ * one admits new requests while the other's batch is on the device.  Per
+ * request, results are identical to wavefront_align with the same
+ * attributes. */
 typedef struct wfa_server_t wfa_server_t;
 typedef struct {
   int short_lanes;          /* lanes for requests with max(n, m) <= short_max_length */
@@ -404,6 +422,7 @@ typedef struct {
   int long_min_length;
   int max_batch_pairs;      /* requests admitted into one device batch */
   size_t lane_memory_bytes; /* per-lane budget: record arena and admission limit */
+  double error_prior;       /* expected error rate of a request, for its footprint estimate (0.1) */
 } wfa_server_config_t;
 
 typedef struct {

@@ -86,6 +86,16 @@ static void ref_shim_attributes(
  * ops_off[i] must leave room for plen[i]+tlen[i] bytes.  Returns the wall
  * time in seconds of the parallel alignment loop.
  */
+/* Pairs per dynamic-schedule chunk: 64 for large batches, fewer when the
+ * batch is small, so that every thread gets pairs (a batch of 32 long pairs
+ * in chunks of 64 would run on one thread). */
+static int shim_chunk(const int n,const int nt) {
+  int c = n / (8*nt);
+  if (c < 1) c = 1;
+  if (c > 64) c = 64;
+  return c;
+}
+
 double ref_align_batch_ex(
     const int metric,
     const int* const pen,
@@ -113,9 +123,10 @@ double ref_align_batch_ex(
   wavefront_aligner_t** aligners = calloc(nt,sizeof(wavefront_aligner_t*));
   int t;
   for (t=0;t<nt;++t) aligners[t] = wavefront_aligner_new(&attr);
+  const int chunk = shim_chunk(n,nt);
   const double t0 = omp_get_wtime();
   int i;
-  #pragma omp parallel for num_threads(nt) schedule(dynamic,64)
+  #pragma omp parallel for num_threads(nt) schedule(dynamic,chunk)
   for (i=0;i<n;++i) {
     wavefront_aligner_t* const wf = aligners[omp_get_thread_num()];
     status[i] = wavefront_align(wf,buf+p_off[i],plen[i],buf+t_off[i],tlen[i]);
@@ -183,9 +194,10 @@ double ref_align_time(
   int t;
   for (t=0;t<nt;++t) aligners[t] = wavefront_aligner_new(&attr);
   int64_t sum = 0;
+  const int chunk = shim_chunk(n,nt);
   const double t0 = omp_get_wtime();
   int i;
-  #pragma omp parallel for num_threads(nt) schedule(dynamic,64) reduction(+:sum)
+  #pragma omp parallel for num_threads(nt) schedule(dynamic,chunk) reduction(+:sum)
   for (i=0;i<n;++i) {
     wavefront_aligner_t* const wf = aligners[omp_get_thread_num()];
     wavefront_align(wf,buf+p_off[i],plen[i],buf+t_off[i],tlen[i]);

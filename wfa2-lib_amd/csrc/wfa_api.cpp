@@ -21,8 +21,11 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <map>
+#include <mutex>
 #include <numeric>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "wfa_internal.h"
@@ -42,6 +45,11 @@ bool hip_ok(hipError_t e, const char* what) {
 
 bool DevBuf::ensure(size_t bytes, size_t max_bytes) {
   if (bytes <= cap && p) return true;
+  if (region) {
+    p = region->take(bytes);
+    cap = p ? bytes : 0;
+    return p != nullptr;
+  }
   // Geometric growth, clamped to max_bytes: a buffer used again and again
   // (serving lanes, repeated batches) reallocates rarely.  If the grown size
   // does not fit, the exact request is tried.
@@ -64,7 +72,7 @@ bool DevBuf::ensure(size_t bytes, size_t max_bytes) {
 }
 
 void DevBuf::release() {
-  if (p) (void)hipFree(p);
+  if (p && !region) (void)hipFree(p);
   p = nullptr;
   cap = 0;
 }
@@ -84,6 +92,7 @@ PlanGroup& PlanGroup::operator=(PlanGroup&& o) noexcept {
   block = o.block;
   wcap = o.wcap;
   lmax_words = o.lmax_words;
+  lclass = o.lclass;
   ids = std::move(o.ids);
   d_ids = std::move(o.d_ids);
   ids_on_device = o.ids_on_device;
@@ -118,7 +127,9 @@ void HostBuf::release() {
 
 PlanGroup take_group(Engine* e) {
   PlanGroup g;
-  if (e && !e->group_pool.empty()) {
+  if (e && e->na) {
+    g.d_ids.region = &e->na->order;
+  } else if (e && !e->group_pool.empty()) {
     g = std::move(e->group_pool.back());
     e->group_pool.pop_back();
   }
@@ -129,6 +140,7 @@ PlanGroup take_group(Engine* e) {
   g.ids.clear();
   g.ids_on_device = false;
   g.est_rec_bytes = -1;
+  g.lclass = -1;
   return g;
 }
 
@@ -136,7 +148,7 @@ void give_back_groups(Engine* e, std::vector<PlanGroup>& groups) {
   for (auto& g : groups) {
     g.ids.clear();
     g.ids_on_device = false;
-    if (e) e->group_pool.push_back(std::move(g));
+    if (e && !g.d_ids.region) e->group_pool.push_back(std::move(g));
   }
   groups.clear();
 }
@@ -166,6 +178,8 @@ void recycle_plan(Engine* e, std::vector<PlanGroup>& old_plan, std::vector<PlanG
 Engine::~Engine() {
   for (auto& ev : bi.ev)
     if (ev) (void)hipEventDestroy(ev);
+  if (plain.exec) (void)hipGraphExecDestroy(plain.exec);
+  if (plain.graph) (void)hipGraphDestroy(plain.graph);
 }
 
 Engine* engine_of(wavefront_aligner_t* wf) { return static_cast<Engine*>(wf->device); }
@@ -173,6 +187,14 @@ wfa::Counters* counters_of(Engine* e) { return e->counters.as<wfa::Counters>(); 
 
 // Record blocks are addressed by 32-bit indices of 16-byte units.
 constexpr size_t kMaxArenaBytes = 0xFFFFFFF0ull * 16;
+// The arena for long pairs (HBM-ring family) is sized at L^2 / 2 bytes of
+// records per pair (the average over 1-20 kbp pairs of 1-15% error is about
+// L^2 / 4, up to 0.7 L^2 at 15%), and such a pair starts only while the arena
+// has L^2 / 4 bytes left (L^2 >> 6 units of 16 bytes).
+constexpr double kLargeRecordShareOfL2 = 2.0;
+// LDS for the two sequences of a pair of the HBM-ring family.
+constexpr size_t kLargeSeqLdsBytes = 64 * 1024;
+constexpr int kLargeDeferShift = 6;
 
 bool engine_init(Engine* e) {
   if (e->ok) return true;
@@ -661,7 +683,10 @@ PlanGroup make_large(Engine* e, const JobSet& js, std::vector<int> ids) {
   int pmax = 0, tmax = 0;
   for (int i : ids) { pmax = std::max(pmax, js.h_plen[i]); tmax = std::max(tmax, js.h_tlen[i]); }
   g.wcap = pmax + tmax + 8;
-  g.lmax_words = 0;
+  // Sequences in LDS when both fit in 64 KB (2-bit sequences up to ~130 kbp).
+  const int spw = 64 / js.bits;
+  const int lw = std::max(pmax, tmax) / spw + 3;
+  g.lmax_words = 2 * (size_t)lw * sizeof(uint64_t) <= kLargeSeqLdsBytes ? lw : 0;
   g.block = wfa::pick_block(wfa::kLarge, pmax + tmax);
   g.ids = std::move(ids);
   return g;
@@ -681,28 +706,25 @@ LaunchShape shape_of(Engine* e, const PlanGroup& g, int bits, const Penalties& p
   }
   if (per_cu <= 0) per_cu = 1;
   shape.grid = (int)std::min<int64_t>((int64_t)g.ids.size(), (int64_t)per_cu * e->num_cus);
+  if (e->na && g.family == wfa::kLarge) {
+    // no-allocation run: as many workgroups as rings fit in its ring share
+    const size_t per_block = (size_t)wfa::ring_slots(pen) * g.wcap * sizeof(int);
+    shape.grid = (int)std::min<size_t>((size_t)shape.grid, e->na->ring_bytes / per_block);
+  }
+  if (e->na) {
+    // ... and at most half the record arena in record chunks (one per
+    // resident workgroup), so that most workgroups get one
+    const size_t chunks = e->rec.cap / ((size_t)wfa::kChunkUnits * 16) / 2;
+    shape.grid = (int)std::max<size_t>(1, std::min<size_t>((size_t)shape.grid, chunks));
+  }
   return shape;
 }
 
-// Enqueues forward + backtrace of one group on the aligner's stream; with
-// fresh, the run counters were just cleared and need no reset of their own.
+// The kernel arguments of one group's forward and backtrace launches.
 // A group that takes jobs 0..n-1 in order passes no order array.
-int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGroup& grp, const Penalties& pen,
-                  int records, bool fresh) {
+void group_args(wavefront_aligner_t* wf, Engine* e, const JobSet& js, const PlanGroup& grp, const Penalties& pen,
+                int records, bool ident, ForwardArgs* pfa, BacktraceArgs* pba) {
   const int n = (int)grp.ids.size();
-  if (n == 0) return WF_STATUS_OK;
-  hipStream_t st = e->stream;
-  const LaunchShape shape = shape_of(e, grp, js.bits, pen, js.heur.cut());
-  bool ident = true;
-  for (int i = 0; i < n && ident; ++i) ident = grp.ids[i] == i;
-  if (!ident && !grp.ids_on_device) {
-    if (!grp.d_ids.ensure((size_t)n * 4)) return WF_STATUS_OOM;
-    if (!hip_ok(hipMemcpyAsync(grp.d_ids.p, grp.ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, st), "plan upload"))
-      return WF_STATUS_OOM;
-    grp.ids_on_device = true;
-  }
-  for (auto& ev : grp.ev)
-    if (!ev && !hip_ok(hipEventCreate(&ev), "hipEventCreate")) return WF_STATUS_OOM;
   wfa::Counters* c = counters_of(e);
   ForwardArgs fa{};
   fa.n_pairs = n;
@@ -733,7 +755,7 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
   fa.records = records;
   fa.comp_begin = wfa::kM;
   fa.comp_end = wfa::kM;
-  fa.defer_shift = 8;  // about L^2/16 bytes of records
+  fa.defer_shift = kLargeDeferShift;  // L^2 >> shift units of 16 B: about L^2/4 bytes of records
   fa.endsfree = js.ef.on;
   fa.pbf = js.ef.pbf;
   fa.pef = js.ef.pef;
@@ -744,43 +766,82 @@ int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGrou
   fa.out_end_rec = js.end_rec;
   fa.heur = js.heur;
   fa.pen = pen;
-  if (!fresh && !hip_ok(wfa::launch_reset_counters(c, 0, st), "reset")) return WF_STATUS_OOM;
-  (void)hipEventRecord(grp.ev[0], st);
-  const hipError_t lerr = grp.family == wfa::kLean ? wfa::launch_forward_lean(fa, shape, st)
-                                                   : wfa::launch_forward(fa, pen.ncomp, js.bits, shape, st);
-  if (!hip_ok(lerr, "forward launch")) return WF_STATUS_OOM;
-  (void)hipEventRecord(grp.ev[1], st);
+  *pfa = fa;
+  BacktraceArgs ba{};
+  ba.n_pairs = n;
+  ba.order = fa.order;
+  ba.plen = js.plen;
+  ba.tlen = js.tlen;
+  ba.status = js.status;
+  ba.score = js.score;
+  ba.last_block = js.last;
+  ba.rec_arena = e->rec.as<uint4>();
+  ba.rec_units = fa.rec_units;
+  ba.ops = js.ops;
+  ba.ops_off = js.ops_off;
+  ba.ops_begin = js.ops_begin;
+  ba.fail_count = &c->fail;
+  ba.comp_end_p = js.cend;
+  ba.comp_begin_p = js.cbeg;
+  ba.comp_begin = wfa::kM;
+  ba.comp_end = wfa::kM;
+  ba.words = js.words;
+  ba.pat_pos = js.ppos;
+  ba.txt_pos = js.tpos;
+  ba.bits = js.bits;
+  ba.endsfree = js.ef.on;
+  ba.end_k = js.end_k;
+  ba.end_off = js.end_off;
+  ba.end_rec = js.end_rec;
+  ba.pen = pen;
+  *pba = ba;
+}
+
+// Forward (and, with records, backtrace) launches of one group, with the
+// group's events around them when it has events.
+hipError_t launch_group(const PlanGroup& grp, const LaunchShape& shape, const ForwardArgs& fa,
+                        const BacktraceArgs& ba, int ncomp, int bits, int records, hipStream_t st) {
+  if (grp.ev[0]) (void)hipEventRecord(grp.ev[0], st);
+  hipError_t err = grp.family == wfa::kLean ? wfa::launch_forward_lean(fa, shape, st)
+                                            : wfa::launch_forward(fa, ncomp, bits, shape, st);
+  if (err != hipSuccess) return err;
+  if (grp.ev[1]) (void)hipEventRecord(grp.ev[1], st);
   if (records) {
-    BacktraceArgs ba{};
-    ba.n_pairs = n;
-    ba.order = ident ? nullptr : grp.d_ids.as<int>();
-    ba.plen = js.plen;
-    ba.tlen = js.tlen;
-    ba.status = js.status;
-    ba.score = js.score;
-    ba.last_block = js.last;
-    ba.rec_arena = e->rec.as<uint4>();
-    ba.rec_units = fa.rec_units;
-    ba.ops = js.ops;
-    ba.ops_off = js.ops_off;
-    ba.ops_begin = js.ops_begin;
-    ba.fail_count = &c->fail;
-    ba.comp_end_p = js.cend;
-    ba.comp_begin_p = js.cbeg;
-    ba.comp_begin = wfa::kM;
-    ba.comp_end = wfa::kM;
-    ba.words = js.words;
-    ba.pat_pos = js.ppos;
-    ba.txt_pos = js.tpos;
-    ba.bits = js.bits;
-    ba.endsfree = js.ef.on;
-    ba.end_k = js.end_k;
-    ba.end_off = js.end_off;
-    ba.end_rec = js.end_rec;
-    ba.pen = pen;
-    if (!hip_ok(wfa::launch_backtrace(ba, st), "backtrace launch")) return WF_STATUS_OOM;
+    err = wfa::launch_backtrace(ba, st);
+    if (err != hipSuccess) return err;
   }
-  (void)hipEventRecord(grp.ev[2], st);
+  if (grp.ev[2]) (void)hipEventRecord(grp.ev[2], st);
+  return hipSuccess;
+}
+
+// Enqueues forward + backtrace of one group on the aligner's stream; with
+// fresh, the run counters were just cleared and need no reset of their own.
+int enqueue_group(wavefront_aligner_t* wf, Engine* e, const JobSet& js, PlanGroup& grp, const Penalties& pen,
+                  int records, bool fresh) {
+  const int n = (int)grp.ids.size();
+  if (n == 0) return WF_STATUS_OK;
+  hipStream_t st = e->stream;
+  const LaunchShape shape = shape_of(e, grp, js.bits, pen, js.heur.cut());
+  bool ident = true;
+  for (int i = 0; i < n && ident; ++i) ident = grp.ids[i] == i;
+  if (!ident && !grp.ids_on_device) {
+    if (!grp.d_ids.ensure((size_t)n * 4)) {
+      fprintf(stderr, "[WFA] cannot allocate the launch order of %d pairs\n", n);
+      return WF_STATUS_OOM;
+    }
+    if (!hip_ok(hipMemcpyAsync(grp.d_ids.p, grp.ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, st), "plan upload"))
+      return WF_STATUS_OOM;
+    grp.ids_on_device = true;
+  }
+  if (!e->na)
+    for (auto& ev : grp.ev)
+      if (!ev && !hip_ok(hipEventCreate(&ev), "hipEventCreate")) return WF_STATUS_OOM;
+  ForwardArgs fa;
+  BacktraceArgs ba;
+  group_args(wf, e, js, grp, pen, records, ident, &fa, &ba);
+  if (!fresh && !hip_ok(wfa::launch_reset_counters(counters_of(e), 0, st), "reset")) return WF_STATUS_OOM;
+  if (!hip_ok(launch_group(grp, shape, fa, ba, pen.ncomp, js.bits, records, st), "forward/backtrace launch"))
+    return WF_STATUS_OOM;
   return WF_STATUS_OK;
 }
 
@@ -837,8 +898,8 @@ std::vector<PlanGroup> make_plan(const JobSet& js, const std::vector<int>& ids, 
       small_ids.push_back(i);
       lmax_small = std::max<int>(lmax_small, L / spw + 3);
     } else {
-      const int w = js.h_plen[i] + js.h_tlen[i];
-      cls[w <= 256 ? 0 : (w <= 4096 ? 1 : 2)].push_back(i);
+      const int blk = wfa::pick_block(wfa::kLarge, js.h_plen[i] + js.h_tlen[i]);
+      cls[blk == 64 ? 0 : (blk == 256 ? 1 : 2)].push_back(i);
     }
   }
   std::vector<PlanGroup> plan;
@@ -849,6 +910,7 @@ std::vector<PlanGroup> make_plan(const JobSet& js, const std::vector<int>& ids, 
     g.block = 64;
     g.wcap = wfa::lean_width(c == 0 && !js.ef.on ? lt.wcap_short : lt.wcap_long);
     g.lmax_words = lean_lmax[c];
+    g.lclass = c;
     g.ids = std::move(lean_ids[c]);
     plan.push_back(std::move(g));
   }
@@ -875,15 +937,18 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
   std::vector<PlanGroup>* pending = &plan;
   if (!hip_ok(wfa::launch_reset_counters(counters_of(e), 2, st), "reset")) return WF_STATUS_OOM;
   if (records) {
-    // Rough footprint: one byte per diagonal-step, about L^2/16 of them for
-    // the pairs at hand, plus a 64 KB chunk per resident workgroup.
+    // Rough footprint: one byte per diagonal-step plus 16 per score step.
+    // Diagonal-steps grow as L^2 e^2 (e the error rate): about L^2/16 for
+    // short reads of up to 10% error; long pairs take L^2/2 (see
+    // kLargeRecordShareOfL2); plus a 64 KB chunk per resident workgroup.
     double want = 0;
     for (PlanGroup& g : plan) {
       if (g.est_rec_bytes < 0) {
+        const double per_l2 = g.family == wfa::kLarge ? 1.0 / kLargeRecordShareOfL2 : 1.0 / 16;
         double w = 0;
         for (int i : g.ids) {
           const double L = std::max(js.h_plen[i], js.h_tlen[i]);
-          w += L * L / 16 + 8 * L;
+          w += L * L * per_l2 + 8 * L;
         }
         g.est_rec_bytes = w + (double)shape_of(e, g, js.bits, pen, js.heur.cut()).grid * wfa::kChunkUnits * 16;
       }
@@ -898,6 +963,8 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
       fprintf(stderr, "[WFA] batch did not converge (record arena too small?)\n");
       return WF_STATUS_OOM;
     }
+    // no-allocation run: the order arrays of the groups run before are done with
+    if (e->na && round > 0) e->na->order.used = 0;
     if (!reserve_for(e, *pending, js.bits, pen, js.heur.cut(), records)) return WF_STATUS_OOM;
     // the reset before the first round clears every counter
     if (round > 0 && !hip_ok(wfa::launch_reset_counters(counters_of(e), 1, st), "reset")) return WF_STATUS_OOM;
@@ -913,7 +980,7 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
       return WF_STATUS_OOM;
     for (PlanGroup& g : *pending) {
       float f = 0, t = 0;
-      if (g.ids.empty()) continue;
+      if (g.ids.empty() || !g.ev[0]) continue;
       (void)hipEventElapsedTime(&f, g.ev[0], g.ev[1]);
       (void)hipEventElapsedTime(&t, g.ev[1], g.ev[2]);
       tot->forward_ms += f;
@@ -1000,9 +1067,11 @@ bool download_results(Engine* e, wfa_device_batch_t* b, char* cigar_dst) {
     ok = b->h_res.ensure(bytes) &&
          hip_ok(hipMemcpyAsync(b->h_res.p, b->d_res.p, bytes, hipMemcpyDeviceToHost, st), "download");
   }
-  if (ok && cigar_dst && b->total_ops > 0)
+  const bool ops_copy = cigar_dst && b->total_ops > 0 && !b->h_ops;
+  if (cigar_dst && b->total_ops > 0 && b->h_ops) memcpy(cigar_dst, b->h_ops, (size_t)b->total_ops);
+  if (ok && ops_copy)
     ok = hip_ok(hipMemcpyAsync(cigar_dst, b->d_ops.p, b->total_ops, hipMemcpyDeviceToHost, st), "cigar download");
-  if (!res && !cigar_dst) return ok;
+  if (!res && !ops_copy) return ok;
   ok = ok && hip_ok(hipStreamSynchronize(st), "download sync");
   if (ok && !b->host_results) {
     b->status.resize(n);
@@ -1122,7 +1191,10 @@ extern "C" size_t wavefront_align_batch_arena_bytes(const wfa_pair_t* pairs, int
   return b;
 }
 
-bool wfa_device_batch_t::load(Engine* e, const wfa_pair_t* pairs, int count) {
+bool wfa_device_batch_t::load(Engine* e, const wfa_pair_t* pairs, int count, bool cigars, bool bucketed,
+                              size_t* upload_bytes) {
+  h_ops = nullptr;
+  runs_ready = false;
   n = count;
   bits = all_acgt(pairs, n) ? 2 : 8;
   plen.resize(n);
@@ -1146,7 +1218,12 @@ bool wfa_device_batch_t::load(Engine* e, const wfa_pair_t* pairs, int count) {
   words_used = nw;
   // per-pair inputs after the words: pat_pos, txt_pos, ops_off (int64), plen, tlen (int32)
   const size_t nn = std::max(n, 1);
-  const size_t meta_off = (size_t)std::max<int64_t>(nw, 1) * 8;
+  size_t meta_off = (size_t)std::max<int64_t>(nw, 1) * 8;
+  if (bucketed) {
+    size_t cw = 64;
+    while ((int64_t)cw < nw) cw *= 2;
+    meta_off = cw * 8;
+  }
   const size_t meta_bytes = nn * (3 * 8 + 2 * 4);
   if (!h_words.ensure(meta_off + meta_bytes)) {
     fprintf(stderr, "[WFA] pinned host allocation failed for a batch of %d pairs\n", n);
@@ -1178,7 +1255,8 @@ bool wfa_device_batch_t::load(Engine* e, const wfa_pair_t* pairs, int count) {
   have_reverse = false;
   have_results = host_results = ops_on_device = false;
   bool ok = words.ensure(meta_off + meta_bytes) && d_res.ensure(nn * 5 * 4) && d_last.ensure(nn * 4) &&
-            d_dsteps.ensure(nn * 8) && d_end_rec.ensure(nn * 4);
+            d_dsteps.ensure(nn * 8) && d_end_rec.ensure(nn * 4) &&
+            (!cigars || d_ops.ensure(std::max<int64_t>(total_ops, 1)));
   if (!ok) {
     fprintf(stderr, "[WFA] device allocation failed for a batch of %d pairs\n", n);
     return false;
@@ -1210,6 +1288,8 @@ bool wfa_device_batch_t::load(Engine* e, const wfa_pair_t* pairs, int count) {
     memcpy(hm + 3 * nn * 8 + nn * 4, tlen.data(), (size_t)n * 4);
   }
   hipStream_t st = e->stream;
+  if (upload_bytes) *upload_bytes = meta_off + meta_bytes;
+  if (bucketed) return true;
   ok = n == 0 || hip_ok(hipMemcpyAsync(words.p, host_words, meta_off + meta_bytes, hipMemcpyHostToDevice, st),
                         "upload");
   // No wait here: the runs are ordered after the copies on the same stream,
@@ -1223,7 +1303,7 @@ extern "C" wfa_device_batch_t* wfa_device_batch_upload(wavefront_aligner_t* cons
   if (!e || !engine_init(e) || n < 0) return nullptr;
   wfa_device_batch_t* b = new (std::nothrow) wfa_device_batch_t();
   if (!b) return nullptr;
-  if (!b->load(e, pairs, n)) {
+  if (!b->load(e, pairs, n, wf->alignment_scope == compute_alignment)) {
     delete b;
     return nullptr;
   }
@@ -1232,13 +1312,252 @@ extern "C" wfa_device_batch_t* wfa_device_batch_upload(wavefront_aligner_t* cons
 
 extern "C" void wfa_device_batch_free(wfa_device_batch_t* b) { delete b; }
 
-extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_batch_t* b) {
-  if (!b) return WF_STATUS_OOM;
-  presets_check(wf, b->plen.data(), b->tlen.data(), b->n);
-  const int sup = check_supported(wf);
-  if (sup != WF_STATUS_OK) return sup;
-  Engine* e = engine_of(wf);
-  if (!e || !engine_init(e)) return WF_STATUS_OOM;
+namespace {
+
+// ------------------------------------------------------------ autotuner
+// Launch shapes chosen by measurement on the workload itself and cached per
+// process: for a key (configuration, kernel family, length bin, error-rate
+// bin) the candidate -- the lean family's ring width, or the HBM-ring
+// family's threads per pair -- with the least kernel time per diagonal-step.
+// A group with no cached choice and enough pairs is split into strided
+// slices, one per candidate; each slice runs (all of its retries included)
+// as a run of its own, so every pair is aligned once and the slices see the
+// same mix of lengths and error rates.  The error-rate bin is measured on
+// the slices' results (log2 of 64 x the mean wavefront score per base); a
+// later plan predicts it as the one last measured for the length bin.
+// WFA_AUTOTUNE=0 turns the tuner off (fixed shapes); WFA_AUTOTUNE_LOG=1
+// prints each measurement.
+struct TuneKey {
+  std::vector<int> conf;
+  int family, lbin, ebin;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(conf, family, lbin, ebin) < std::tie(o.conf, o.family, o.lbin, o.ebin);
+  }
+};
+struct TuneState {
+  std::mutex mu;
+  std::map<TuneKey, int> choice;     // full key -> candidate
+  std::map<TuneKey, int> last_ebin;  // key with ebin = -1 -> error bin last measured
+};
+TuneState& tuner() {
+  static TuneState t;
+  return t;
+}
+bool env_on(const char* name, bool dflt) {
+  const char* v = getenv(name);
+  return v ? v[0] != '0' : dflt;
+}
+constexpr int kLeanSliceMin = 16384;   // pairs per candidate slice (lean family)
+constexpr int kLargeSliceMin = 48;     // pairs per candidate slice (HBM-ring family)
+constexpr int kLargeBlocks[3] = {64, 256, 1024};
+
+// HBM-ring family length bins: log2(L / 512), up to 10.
+int large_lbin(int L) {
+  int b = 0;
+  while (b < 10 && (512 << (b + 1)) <= L) ++b;
+  return b;
+}
+
+// One candidate slice of a group being measured.
+struct TuneTrial {
+  TuneKey base;   // ebin = -1
+  int candidate;
+  PlanGroup group;
+};
+
+int predicted_ebin(TuneState& T, const TuneKey& base) {
+  auto it = T.last_ebin.find(base);
+  return it == T.last_ebin.end() ? -1 : it->second;
+}
+
+// Applies cached choices to the plan and moves groups that need measuring
+// into trials.
+void tune_plan(Engine* e, const JobSet& js, const std::vector<int>& conf, std::vector<PlanGroup>& plan,
+               std::vector<TuneTrial>* trials) {
+  TuneState& T = tuner();
+  std::lock_guard<std::mutex> lk(T.mu);
+  std::vector<PlanGroup> out;
+  std::vector<int> large_ids;
+  for (PlanGroup& g : plan) {
+    if (g.family == wfa::kLarge) {
+      large_ids.insert(large_ids.end(), g.ids.begin(), g.ids.end());
+      continue;
+    }
+    if (g.family != wfa::kLean || g.lclass < 0 || js.ef.on) {
+      out.push_back(std::move(g));
+      continue;
+    }
+    const int cands[2] = {g.lclass == 0 ? 224 : 384, g.lclass == 0 ? 384 : 768};
+    TuneKey base{conf, wfa::kLean, g.lclass, -1};
+    const int eb = predicted_ebin(T, base);
+    TuneKey key = base;
+    key.ebin = eb;
+    auto it = eb >= 0 ? T.choice.find(key) : T.choice.end();
+    if (it != T.choice.end()) {
+      g.wcap = it->second;
+      out.push_back(std::move(g));
+    } else if ((int)g.ids.size() >= 2 * kLeanSliceMin) {
+      for (int c = 0; c < 2; ++c) {
+        TuneTrial t{base, cands[c], take_group(e)};
+        t.group.family = wfa::kLean;
+        t.group.block = 64;
+        t.group.wcap = cands[c];
+        t.group.lmax_words = g.lmax_words;
+        t.group.lclass = g.lclass;
+        for (size_t i = c; i < g.ids.size(); i += 2) t.group.ids.push_back(g.ids[i]);
+        trials->push_back(std::move(t));
+      }
+      g.ids.clear();
+    } else {
+      out.push_back(std::move(g));
+    }
+  }
+  // HBM-ring family: per length bin, the cached threads per pair, a
+  // measurement, or the fixed rule; bins with one block size share a launch.
+  std::map<int, std::vector<int>> by_block;
+  std::map<int, std::vector<int>> bins;
+  for (int i : large_ids) bins[large_lbin(std::max(js.h_plen[i], js.h_tlen[i]))].push_back(i);
+  for (auto& kv : bins) {
+    std::vector<int>& ids = kv.second;
+    TuneKey base{conf, wfa::kLarge, kv.first, -1};
+    const int eb = predicted_ebin(T, base);
+    TuneKey key = base;
+    key.ebin = eb;
+    auto it = eb >= 0 ? T.choice.find(key) : T.choice.end();
+    if (it != T.choice.end()) {
+      auto& v = by_block[it->second];
+      v.insert(v.end(), ids.begin(), ids.end());
+    } else if ((int)ids.size() >= 3 * kLargeSliceMin) {
+      for (int c = 0; c < 3; ++c) {
+        std::vector<int> sl;
+        for (size_t i = c; i < ids.size(); i += 3) sl.push_back(ids[i]);
+        TuneTrial t{base, kLargeBlocks[c], make_large(e, js, std::move(sl))};
+        t.group.block = kLargeBlocks[c];
+        trials->push_back(std::move(t));
+      }
+    } else {
+      for (int i : ids) by_block[wfa::pick_block(wfa::kLarge, js.h_plen[i] + js.h_tlen[i])].push_back(i);
+    }
+  }
+  for (auto& kv : by_block) {
+    std::vector<int>& ids = kv.second;
+    std::stable_sort(ids.begin(), ids.end(), [&](int x, int y) {
+      return (int64_t)js.h_plen[x] + js.h_tlen[x] > (int64_t)js.h_plen[y] + js.h_tlen[y];
+    });
+    PlanGroup g = make_large(e, js, std::move(ids));
+    g.block = kv.first;
+    out.push_back(std::move(g));
+  }
+  give_back_groups(e, plan);
+  plan = std::move(out);
+}
+
+// Runs every trial slice as a run of its own, then stores the choice of each
+// measured key.
+int run_trials(wavefront_aligner_t* wf, Engine* e, const JobSet& js, int records, std::vector<TuneTrial>& trials,
+               RunTotals* tot) {
+  struct Cost {
+    double ms = 0;
+    unsigned long long dsteps = 0;
+  };
+  std::vector<Cost> costs(trials.size());
+  std::vector<int> scratch;
+  for (size_t t = 0; t < trials.size(); ++t) {
+    std::vector<PlanGroup> one;
+    one.push_back(std::move(trials[t].group));
+    RunTotals rt;
+    const int rc = run_jobs(wf, e, js, one, records, &rt, scratch);
+    trials[t].group = std::move(one[0]);
+    if (rc != WF_STATUS_OK) return rc;
+    costs[t].ms = rt.forward_ms + rt.backtrace_ms;
+    costs[t].dsteps = rt.dsteps;
+    tot->forward_ms += rt.forward_ms;
+    tot->backtrace_ms += rt.backtrace_ms;
+    tot->launches += rt.launches;
+    tot->dsteps += rt.dsteps;
+    tot->fail += rt.fail;
+  }
+  // error-rate bins from the slices' scores
+  std::vector<int> score(js.n);
+  if (!hip_ok(hipMemcpyAsync(score.data(), js.score, (size_t)js.n * 4, hipMemcpyDeviceToHost, e->stream), "scores") ||
+      !hip_ok(hipStreamSynchronize(e->stream), "scores"))
+    return WF_STATUS_OOM;
+  const bool log = env_on("WFA_AUTOTUNE_LOG", false);
+  TuneState& T = tuner();
+  std::lock_guard<std::mutex> lk(T.mu);
+  for (size_t t = 0; t < trials.size();) {
+    size_t u = t;
+    double sum = 0;
+    long long cnt = 0;
+    while (u < trials.size() && !(trials[t].base < trials[u].base) && !(trials[u].base < trials[t].base)) {
+      for (int i : trials[u].group.ids) {
+        sum += (double)score[i] / std::max(1, std::max(js.h_plen[i], js.h_tlen[i]));
+        ++cnt;
+      }
+      ++u;
+    }
+    const double mean = cnt ? sum / cnt : 0;
+    int eb = 0;
+    while (eb < 12 && 64.0 * mean >= (double)(2 << eb)) ++eb;
+    int best = trials[t].candidate;
+    double best_cost = 1e300;
+    for (size_t v = t; v < u; ++v) {
+      const double c = costs[v].dsteps ? costs[v].ms / (double)costs[v].dsteps : 1e300;
+      if (log)
+        fprintf(stderr, "[WFA] autotune family %d length bin %d error bin %d: candidate %d, %zu pairs, %.3f ms, "
+                "%.3e ms per diagonal-step\n", trials[v].base.family, trials[v].base.lbin, eb, trials[v].candidate,
+                trials[v].group.ids.size(), costs[v].ms, c);
+      if (c < best_cost) {
+        best_cost = c;
+        best = trials[v].candidate;
+      }
+    }
+    TuneKey key = trials[t].base;
+    key.ebin = eb;
+    T.choice[key] = best;
+    T.last_ebin[trials[t].base] = eb;
+    if (log) fprintf(stderr, "[WFA] autotune: chose %d\n", best);
+    t = u;
+  }
+  std::vector<PlanGroup> done;
+  for (auto& tr : trials) done.push_back(std::move(tr.group));
+  give_back_groups(e, done);
+  return WF_STATUS_OK;
+}
+
+}  // namespace
+
+namespace {
+
+// Final per-pair statuses of a run with failures: internal codes the host
+// reports differently, and the first failing status.
+int settle_statuses(wavefront_aligner_t* wf, std::vector<int>& status) {
+  int first_fail = WF_STATUS_ALG_COMPLETED;
+  for (size_t i = 0; i < status.size(); ++i) {
+    if (status[i] == wfa::kDevBadTrace) {
+      fprintf(stderr, "[WFA] internal error: inconsistent backtrace for pair %zu\n", i);
+      status[i] = WF_STATUS_UNATTAINABLE;
+    }
+    if (status[i] == wfa::kDevStaleTrace) {
+      // The reference's backtrace (wavefront_backtrace.c:64-219) would read
+      // cells of a wavefront the cut-off emptied, which the compute step
+      // ignored; that trace is not replayed.
+      if (wf->system.verbose >= 0)
+        fprintf(stderr, "[WFA] pair %zu: backtrace through a wavefront emptied by the heuristic cut-off "
+                "is not supported\n", i);
+      status[i] = WF_STATUS_UNSUPPORTED;
+    }
+    // an end made unreachable (or z-dropped) by the cut-off is a partial
+    // result, not a failure
+    if (status[i] == wfa::kDevUnreachable || status[i] == wfa::kDevZdropped) continue;
+    if (status[i] != WF_STATUS_ALG_COMPLETED && first_fail == WF_STATUS_ALG_COMPLETED) first_fail = status[i];
+  }
+  return first_fail;
+}
+
+// wfa_device_batch_run, and its no-allocation form when the engine's
+// buffers are views into a fixed region (e->na set).
+int batch_run(wavefront_aligner_t* wf, wfa_device_batch_t* b, Engine* e) {
   const Penalties pen = make_penalties(wf->penalties);
   const int records = (wf->alignment_scope == compute_alignment) ? 1 : 0;
   b->records = records;
@@ -1247,13 +1566,15 @@ extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_ba
   b->stats = wfa_run_stats_t{};
   b->have_results = false;
   b->host_results = false;
+  b->h_ops = nullptr;
+  b->runs_ready = false;
   b->first_fail = WF_STATUS_ALG_COMPLETED;
   if (b->n == 0) {
     b->have_results = true;
     b->host_results = true;
     return WF_STATUS_ALG_COMPLETED;
   }
-  if (records && !b->d_ops.ensure(std::max<int64_t>(b->total_ops, 1))) {
+  if (records && !e->na && !b->d_ops.ensure(std::max<int64_t>(b->total_ops, 1))) {
     fprintf(stderr, "[WFA] cannot allocate %lld bytes of CIGAR output\n", (long long)b->total_ops);
     return WF_STATUS_OOM;
   }
@@ -1264,14 +1585,30 @@ extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_ba
   if (b->bi_mode) {
     rc = bialign_run(wf, e, b, &tot);
     if (rc == WF_STATUS_OOM) return rc;
+  } else if (e->na) {
+    // a plan of its own, whose order arrays come from the region
+    std::vector<PlanGroup> plan = make_plan(b->jobs(), b->sorted, pen);
+    for (PlanGroup& g : plan) g.d_ids.region = &e->na->order;
+    std::vector<int> scratch;
+    rc = run_jobs(wf, e, b->jobs(), plan, records, &tot, scratch);
+    give_back_groups(e, plan);
+    if (rc != WF_STATUS_OK) return rc;
   } else {
     const std::vector<int> key = {pen.metric, pen.x, pen.o1, pen.e1, pen.o2, pen.e2, b->bits, b->ef.on,
-                                  b->heur.cut()};
+                                  b->heur.cut(), records};
     if (b->plan_key != key) {
       std::vector<PlanGroup> plan = make_plan(b->jobs(), b->sorted, pen);
+      std::vector<TuneTrial> trials;
+      if (env_on("WFA_AUTOTUNE", true)) tune_plan(e, b->jobs(), key, plan, &trials);
+      if (!trials.empty()) {
+        // measure the candidates on their slices (aligning those pairs)
+        rc = run_trials(wf, e, b->jobs(), records, trials, &tot);
+        if (rc != WF_STATUS_OK) return rc;
+      }
       recycle_plan(e, b->plan, plan);
       b->plan = std::move(plan);
-      b->plan_key = key;
+      // after a measurement the next run plans again, with the choices
+      b->plan_key = trials.empty() ? key : std::vector<int>();
     }
     std::vector<int> scratch;
     rc = run_jobs(wf, e, b->jobs(), b->plan, records, &tot, scratch);
@@ -1285,30 +1622,38 @@ extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_ba
   b->have_results = true;
   b->host_results = false;
   if (tot.fail == 0) return WF_STATUS_ALG_COMPLETED;
-  if (!download_results(e, b)) return WF_STATUS_OOM;
-  int first_fail = WF_STATUS_ALG_COMPLETED;
-  for (int i = 0; i < b->n; ++i) {
-    if (b->status[i] == wfa::kDevBadTrace) {
-      fprintf(stderr, "[WFA] internal error: inconsistent backtrace for pair %d\n", i);
-      b->status[i] = WF_STATUS_UNATTAINABLE;
-    }
-    if (b->status[i] == wfa::kDevStaleTrace) {
-      // The reference's backtrace (wavefront_backtrace.c:64-219) would read
-      // cells of a wavefront the cut-off emptied, which the compute step
-      // ignored; that trace is not replayed.
-      if (wf->system.verbose >= 0)
-        fprintf(stderr, "[WFA] pair %d: backtrace through a wavefront emptied by the heuristic cut-off "
-                "is not supported\n", i);
-      b->status[i] = WF_STATUS_UNSUPPORTED;
-    }
-    // an end made unreachable (or z-dropped) by the cut-off is a partial
-    // result, not a failure
-    if (b->status[i] == wfa::kDevUnreachable || b->status[i] == wfa::kDevZdropped) continue;
-    if (b->status[i] != WF_STATUS_ALG_COMPLETED && first_fail == WF_STATUS_ALG_COMPLETED) first_fail = b->status[i];
+  if (e->na) {
+    // statuses through pageable host memory (no pinned staging), settled,
+    // and written back for the later download
+    std::vector<int> st(b->n);
+    if (!hip_ok(hipMemcpyAsync(st.data(), b->d_status.p, (size_t)b->n * 4, hipMemcpyDeviceToHost, e->stream),
+                "status") ||
+        !hip_ok(hipStreamSynchronize(e->stream), "status"))
+      return WF_STATUS_OOM;
+    b->first_fail = settle_statuses(wf, st);
+    if (!hip_ok(hipMemcpyAsync(b->d_status.p, st.data(), (size_t)b->n * 4, hipMemcpyHostToDevice, e->stream),
+                "status") ||
+        !hip_ok(hipStreamSynchronize(e->stream), "status"))
+      return WF_STATUS_OOM;
+    return b->first_fail;
   }
-  b->first_fail = first_fail;
-  return first_fail;
+  if (!download_results(e, b)) return WF_STATUS_OOM;
+  b->first_fail = settle_statuses(wf, b->status);
+  return b->first_fail;
 }
+
+}  // namespace
+
+extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_batch_t* b) {
+  if (!b) return WF_STATUS_OOM;
+  presets_check(wf, b->plen.data(), b->tlen.data(), b->n);
+  const int sup = check_supported(wf);
+  if (sup != WF_STATUS_OK) return sup;
+  Engine* e = engine_of(wf);
+  if (!e || !engine_init(e)) return WF_STATUS_OOM;
+  return batch_run(wf, b, e);
+}
+
 
 extern "C" int wfa_device_batch_stats(wfa_device_batch_t* b, wfa_run_stats_t* stats) {
   if (!b || !stats || !b->have_results) return -1;
@@ -1434,6 +1779,229 @@ extern "C" int wfa_device_batch_download(wavefront_aligner_t* const wf, wfa_devi
   return first_fail;
 }
 
+namespace {
+
+// Single-pair graphs can be turned off with WFA_PLAIN_GRAPH=0 (read at
+// every call, so a process can compare both forms).
+bool plain_graphs_enabled() {
+  const char* v = getenv("WFA_PLAIN_GRAPH");
+  return !(v && v[0] == '0');
+}
+
+template <typename T>
+void key_add(std::vector<unsigned char>& k, const T& v) {
+  const unsigned char* p = reinterpret_cast<const unsigned char*>(&v);
+  k.insert(k.end(), p, p + sizeof(T));
+}
+
+// One pair through a captured graph (see Engine::PlainGraph).  Returns
+// WF_STATUS_OK when the pair's results are ready in the batch (on the host),
+// a failing status on a HIP error, or 1 when the pair must take the general
+// path (a configuration with more than one launch group, or a retry).
+int plain_run(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, const wfa_pair_t* pair) {
+  const int records = (wf->alignment_scope == compute_alignment) ? 1 : 0;
+  size_t upload = 0;
+  if (!b->load(e, pair, 1, records != 0, true, &upload)) return WF_STATUS_OOM;
+  const Penalties pen = make_penalties(wf->penalties);
+  b->records = records;
+  b->ef = ends_free_of(wf);
+  b->heur = heuristic_of(wf);
+  b->stats = wfa_run_stats_t{};
+  b->bi_mode = false;
+  b->first_fail = WF_STATUS_ALG_COMPLETED;
+  hipStream_t st = e->stream;
+  const JobSet js = b->jobs();
+  std::vector<PlanGroup> plan = make_plan(js, b->sorted, pen);
+  auto general = [&]() {
+    // the staged inputs go up uncaptured; the caller runs the general path
+    return hip_ok(hipMemcpyAsync(b->words.p, b->h_words.p, upload, hipMemcpyHostToDevice, st), "upload")
+               ? 1 : WF_STATUS_OOM;
+  };
+  if (plan.size() != 1) return general();
+  PlanGroup& g = plan[0];
+  if (g.family != wfa::kLarge) g.lmax_words = (int)(upload / 8);  // the bucket's words: a capacity, not a length
+  if (!reserve_for(e, plan, js.bits, pen, js.heur.cut(), records)) return WF_STATUS_OOM;
+  size_t ops_cap = 1024;
+  while ((int64_t)ops_cap < b->total_ops) ops_cap *= 2;
+  if ((records && !b->d_ops.ensure(ops_cap)) || !e->plain_out.ensure(256 + ops_cap)) return WF_STATUS_OOM;
+  const LaunchShape shape = shape_of(e, g, js.bits, pen, js.heur.cut());
+  ForwardArgs fa;
+  BacktraceArgs ba;
+  group_args(wf, e, js, g, pen, records, true, &fa, &ba);
+  char* out = e->plain_out.as<char>();
+  std::vector<unsigned char> key;
+  key_add(key, fa);
+  key_add(key, ba);
+  key_add(key, shape);
+  key_add(key, g.family);
+  key_add(key, js.bits);
+  key_add(key, records);
+  key_add(key, upload);
+  key_add(key, ops_cap);
+  key_add(key, b->words.p);
+  key_add(key, b->h_words.p);
+  key_add(key, b->d_res.p);
+  key_add(key, b->d_ops.p);
+  key_add(key, out);
+  Engine::PlainGraph& pg = e->plain;
+  if (pg.key != key) {
+    if (pg.seen != key) {
+      pg.seen = key;  // first sighting: run uncaptured
+      return general();
+    }
+    // second sighting in a row: capture it
+    if (pg.exec) (void)hipGraphExecDestroy(pg.exec);
+    if (pg.graph) (void)hipGraphDestroy(pg.graph);
+    pg.exec = nullptr;
+    pg.graph = nullptr;
+    pg.key.clear();
+    hipGraph_t graph = nullptr;
+    if (!hip_ok(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed), "graph capture")) {
+      pg.off = true;
+      return general();
+    }
+    bool ok = hipMemcpyAsync(b->words.p, b->h_words.p, upload, hipMemcpyHostToDevice, st) == hipSuccess &&
+              wfa::launch_reset_counters(counters_of(e), 2, st) == hipSuccess &&
+              launch_group(g, shape, fa, ba, pen.ncomp, js.bits, records, st) == hipSuccess &&
+              hipMemcpyAsync(out, b->d_res.p, 5 * sizeof(int), hipMemcpyDeviceToHost, st) == hipSuccess &&
+              hipMemcpyAsync(out + 64, e->counters.p, sizeof(wfa::Counters), hipMemcpyDeviceToHost, st) ==
+                  hipSuccess &&
+              (!records || hipMemcpyAsync(out + 256, b->d_ops.p, ops_cap, hipMemcpyDeviceToHost, st) == hipSuccess);
+    const hipError_t ec = hipStreamEndCapture(st, &graph);
+    hipGraphExec_t exec = nullptr;
+    if (ok && ec == hipSuccess && graph && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+      pg.graph = graph;
+      pg.exec = exec;
+      pg.key = key;
+    } else {
+      if (graph) (void)hipGraphDestroy(graph);
+      (void)hipGetLastError();
+      fprintf(stderr, "[WFA] single-pair graph capture failed; single pairs run uncaptured\n");
+      pg.off = true;
+      return general();
+    }
+  }
+  if (!hip_ok(hipGraphLaunch(pg.exec, st), "graph launch") || !hip_ok(hipStreamSynchronize(st), "graph"))
+    return WF_STATUS_OOM;
+  ++pg.replays;
+  wfa::Counters c;
+  memcpy(&c, out + 64, sizeof(c));
+  if (c.retry != 0) return 1;  // a ring or the arena overflowed: the general path retries (inputs are up)
+  b->status.assign(1, 0);
+  b->score.assign(1, 0);
+  b->ops_begin.assign(1, 0);
+  b->end_k.assign(1, 0);
+  b->end_off.assign(1, 0);
+  memcpy(b->status.data(), out, 4);
+  memcpy(b->score.data(), out + 4, 4);
+  memcpy(b->ops_begin.data(), out + 8, 4);
+  memcpy(b->end_k.data(), out + 12, 4);
+  memcpy(b->end_off.data(), out + 16, 4);
+  b->host_results = true;
+  b->have_results = true;
+  b->runs_ready = false;
+  b->ops_on_device = records != 0;
+  b->h_ops = records ? out + 256 : nullptr;
+  b->stats.diagonal_steps = (int64_t)c.dsteps;
+  b->stats.forward_launches = 1;
+  b->stats.record_arena_bytes = (int64_t)e->rec.cap;
+  if (c.fail) b->first_fail = settle_statuses(wf, b->status);
+  return WF_STATUS_OK;
+}
+
+}  // namespace
+
+namespace {
+
+// Builds the run-length CIGARs of the batch's last run on the device (once
+// per run) and brings the per-pair run offsets to the host.
+int make_runs(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b) {
+  if (b->runs_ready) return WF_STATUS_OK;
+  if (!b->have_results) return WF_STATUS_UNATTAINABLE;
+  const bool extension = wf->alignment_form.extension && !b->bi_mode;
+  if (extension || (b->heur.zdrop && !b->bi_mode)) {
+    fprintf(stderr, "[WFA] run-length CIGARs are not made for the extension form or z-drop (their CIGARs are "
+            "trimmed on the host): use wfa_device_batch_download\n");
+    return WF_STATUS_UNSUPPORTED;
+  }
+  const int n = b->n;
+  b->h_run_off.assign((size_t)n + 1, 0);
+  b->total_runs = 0;
+  if (n == 0 || !b->records || !b->ops_on_device) {
+    b->runs_ready = true;
+    return WF_STATUS_OK;
+  }
+  hipStream_t st = e->stream;
+  wfa::CigarRunsArgs a{};
+  a.n = n;
+  a.keep_failed = b->bi_mode ? 1 : 0;
+  a.ops = b->d_ops.as<char>();
+  a.ops_off = b->d_ops_off.as<int64_t>();
+  a.ops_begin = b->d_ops_begin.as<int>();
+  a.plen = b->d_plen.as<int>();
+  a.tlen = b->d_tlen.as<int>();
+  a.status = b->d_status.as<int>();
+  if (!b->d_run_counts.ensure((size_t)n * 8) || !b->d_run_off.ensure(((size_t)n + 1) * 8)) return WF_STATUS_OOM;
+  a.counts = b->d_run_counts.as<long long>();
+  a.offsets = b->d_run_off.as<long long>();
+  size_t temp = 0;
+  if (!hip_ok(wfa::launch_cigar_runs_count(a, nullptr, &temp, st), "scan size") ||
+      !b->d_scan_temp.ensure(std::max<size_t>(temp, 256)))
+    return WF_STATUS_OOM;
+  temp = b->d_scan_temp.cap;
+  if (!hip_ok(wfa::launch_cigar_runs_count(a, b->d_scan_temp.p, &temp, st), "run count") ||
+      !hip_ok(hipMemcpyAsync(b->h_run_off.data(), b->d_run_off.p, ((size_t)n + 1) * 8, hipMemcpyDeviceToHost, st),
+              "run offsets") ||
+      !hip_ok(hipStreamSynchronize(st), "run offsets"))
+    return WF_STATUS_OOM;
+  b->total_runs = b->h_run_off[n];
+  if (b->total_runs > 0) {
+    if (!b->d_runs.ensure((size_t)b->total_runs * 4)) return WF_STATUS_OOM;
+    a.runs = b->d_runs.as<uint32_t>();
+    if (!hip_ok(wfa::launch_cigar_runs_write(a, st), "run write")) return WF_STATUS_OOM;
+  }
+  b->runs_ready = true;
+  return WF_STATUS_OK;
+}
+
+}  // namespace
+
+extern "C" int64_t wfa_device_batch_cigar_runs(wavefront_aligner_t* const wf, wfa_device_batch_t* b) {
+  if (!wf || !b) return WF_STATUS_OOM;
+  Engine* e = engine_of(wf);
+  if (!e || !e->ok) return WF_STATUS_OOM;
+  const int rc = make_runs(wf, e, b);
+  return rc == WF_STATUS_OK ? (int64_t)b->total_runs : (int64_t)rc;
+}
+
+extern "C" int wfa_device_batch_download_runs(wavefront_aligner_t* const wf, wfa_device_batch_t* b,
+                                              wfa_result_t* results, uint32_t* runs, size_t max_runs) {
+  if (!wf || !b || !b->have_results) return WF_STATUS_UNATTAINABLE;
+  Engine* e = engine_of(wf);
+  if (!e || !e->ok) return WF_STATUS_OOM;
+  int rc = make_runs(wf, e, b);
+  if (rc != WF_STATUS_OK) return rc;
+  if (b->total_runs > 0 && (runs == nullptr || max_runs < (size_t)b->total_runs)) {
+    fprintf(stderr, "[WFA] wfa_device_batch_download_runs: runs must hold %lld entries\n",
+            (long long)b->total_runs);
+    return WF_STATUS_OOM;
+  }
+  // the runs go down on the stream while the host settles the results
+  if (b->total_runs > 0 &&
+      !hip_ok(hipMemcpyAsync(runs, b->d_runs.p, (size_t)b->total_runs * 4, hipMemcpyDeviceToHost, e->stream),
+              "run download"))
+    return WF_STATUS_OOM;
+  rc = wfa_device_batch_download(wf, b, results, nullptr, 0);
+  if (!hip_ok(hipStreamSynchronize(e->stream), "run download")) return WF_STATUS_OOM;
+  for (int i = 0; i < b->n; ++i) {
+    wfa_result_t& r = results[i];
+    const long long cnt = b->h_run_off[i + 1] - b->h_run_off[i];
+    r.cigar_offset = b->h_run_off[i];
+    r.cigar_length = r.cigar_length > 0 ? (int)cnt : 0;
+  }
+  return rc;
+}
+
 extern "C" int wavefront_align_batch(wavefront_aligner_t* const wf, const wfa_pair_t* pairs, int num_pairs,
                                      wfa_result_t* results, char* cigar_arena, size_t cigar_arena_bytes) {
   if (num_pairs > 0) presets_check(wf, pairs, num_pairs);
@@ -1462,8 +2030,17 @@ extern "C" int wavefront_align_batch(wavefront_aligner_t* const wf, const wfa_pa
   if (!e || !engine_init(e)) return fail_all(WF_STATUS_OOM);
   if (!e->scratch) e->scratch = new (std::nothrow) wfa_device_batch_t();
   wfa_device_batch_t* b = e->scratch;
-  if (!b || !b->load(e, pairs, num_pairs)) return fail_all(WF_STATUS_OOM);
-  int rc = wfa_device_batch_run(wf, b);
+  if (!b) return fail_all(WF_STATUS_OOM);
+  int rc;
+  if (num_pairs == 1 && wf->memory_mode != wavefront_memory_ultralow && !e->plain.off && plain_graphs_enabled()) {
+    rc = plain_run(wf, e, b, pairs);
+    if (rc == WF_STATUS_OK) return wfa_device_batch_download(wf, b, results, cigar_arena, cigar_arena_bytes);
+    if (rc != 1) return fail_all(rc);
+    rc = batch_run(wf, b, e);  // inputs already on the device
+  } else {
+    if (!b->load(e, pairs, num_pairs, wf->alignment_scope == compute_alignment)) return fail_all(WF_STATUS_OOM);
+    rc = wfa_device_batch_run(wf, b);
+  }
   if (rc == WF_STATUS_OOM && !b->have_results) return fail_all(rc);
   const int drc = wfa_device_batch_download(wf, b, results, cigar_arena, cigar_arena_bytes);
   if (drc == WF_STATUS_OOM && !b->host_results) return fail_all(drc);

@@ -256,6 +256,28 @@ hipError_t launch_reverse_pack(const uint64_t* words, const int64_t* src_pos, co
                                int n, int bits, uint64_t* out, hipStream_t stream);
 int breakpoint_max_blocks_per_cu(int ncomp, int bits, int block, bool cut);
 
+// Run-length CIGARs (wfa_cigar.hip): per pair the runs of its operations
+// ops[ops_off + ops_begin .. ops_off + plen + tlen) as (length << 4) | SAM
+// code (7 '=', 8 X, 1 I, 2 D), pairs back to back.  A pair whose status is
+// not 0 has none unless keep_failed (BiWFA keeps the CIGAR appended before a
+// failure).
+struct CigarRunsArgs {
+  int n;
+  int keep_failed;
+  const char* ops;
+  const int64_t* ops_off;
+  const int* ops_begin;
+  const int* plen;
+  const int* tlen;
+  const int* status;           // null: every pair has its CIGAR
+  long long* counts;           // runs per pair
+  long long* offsets;          // n + 1 entries: offsets[i] = runs before pair i
+  uint32_t* runs;
+};
+// Counting pass and scan (temp == null: only sets *temp_bytes, the scan's storage).
+hipError_t launch_cigar_runs_count(const CigarRunsArgs& a, void* temp, size_t* temp_bytes, hipStream_t stream);
+hipError_t launch_cigar_runs_write(const CigarRunsArgs& a, hipStream_t stream);
+
 // Lean family: whether the penalties' slot structure has an instance, its
 // dynamic LDS, occupancy and launch (ForwardArgs as for launch_forward; the
 // per-pair component arrays must be null and comp_begin = comp_end = M).

@@ -14,19 +14,39 @@ namespace wfa_host {
 
 bool hip_ok(hipError_t e, const char* what);
 
+// A fixed range of device memory handed out front to back, 256-byte aligned
+// (no HIP allocation; the no-allocation entry point carves its buffers from
+// the library's static device array this way).
+struct ScratchRegion {
+  char* base = nullptr;
+  size_t bytes = 0, used = 0;
+  void* take(size_t n) {
+    const size_t at = (used + 255) & ~(size_t)255;
+    if (at > bytes || n > bytes - at) return nullptr;
+    used = at + n;
+    return base + at;
+  }
+};
+
 // Grow-only device allocation, freed by its owner's destructor.  hipFree
 // waits for the whole device (other aligners' kernels included), so buffers
 // that are used again and again grow geometrically up to max_bytes and are
-// never freed between runs.
+// never freed between runs.  A buffer bound to a ScratchRegion takes its
+// memory from the region instead and never allocates or frees.
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
+  ScratchRegion* region = nullptr;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), cap(o.cap) { o.p = nullptr; o.cap = 0; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), cap(o.cap), region(o.region) { o.p = nullptr; o.cap = 0; o.region = nullptr; }
   DevBuf& operator=(DevBuf&& o) noexcept {
-    if (this != &o) { release(); p = o.p; cap = o.cap; o.p = nullptr; o.cap = 0; }
+    if (this != &o) {
+      release();
+      p = o.p; cap = o.cap; region = o.region;
+      o.p = nullptr; o.cap = 0; o.region = nullptr;
+    }
     return *this;
   }
   ~DevBuf() { release(); }
@@ -64,6 +84,7 @@ struct PlanGroup {
   DevBuf d_ids;
   bool ids_on_device = false;  // d_ids holds ids (uploaded once per plan)
   double est_rec_bytes = -1;   // rough record footprint of the group (computed once)
+  int lclass = -1;             // lean family: length class (0 short, 1 long)
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   PlanGroup() = default;
   PlanGroup(const PlanGroup&) = delete;
@@ -101,6 +122,28 @@ struct Engine {
     DevBuf pl, tl, pp, tp, cb, ce, st, sc, last, ds, off, beg, ops;
     hipEvent_t ev[2] = {nullptr, nullptr};
   } bi;
+  // Set for the duration of a no-allocation run: plan groups take their
+  // order arrays from `order`, and the HBM-ring family's grid is limited so
+  // that its rings fit in ring_bytes (the engine's rec/ring/counters buffers
+  // are then views into the same static array).  No events are recorded.
+  struct NoAlloc {
+    ScratchRegion order;
+    size_t ring_bytes = 0;
+  }* na = nullptr;
+  // Single-pair calls (wavefront_align) replay one captured graph per launch
+  // configuration: the upload, the counter reset, the forward and backtrace
+  // launches and the downloads of the results, counters and CIGAR, under one
+  // launch and one wait.  A configuration is captured the second time in a
+  // row it is seen; `key` holds the bytes of the arguments and copies the
+  // graph was captured with.
+  struct PlainGraph {
+    std::vector<unsigned char> key, seen;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool off = false;      // a capture failed: single pairs run uncaptured from then on
+    long long replays = 0;
+  } plain;
+  HostBuf plain_out;       // pinned: results (0), counters (64) and CIGAR (256) of the single pair
   ~Engine();
 };
 
@@ -219,8 +262,19 @@ struct wfa_device_batch_t {
   wfa_host::HostBuf h_res;                 // pinned staging of the results
   wfa_host::JobSet jobs() const;
   // (Re)fills the batch with new pairs, reusing every buffer that is large
-  // enough.  Returns false on an allocation or copy failure.
-  bool load(wfa_host::Engine* e, const wfa_pair_t* pairs, int n);
+  // enough; with cigars, the CIGAR output buffer is sized as well.  Returns
+  // false on an allocation or copy failure.
+  // With bucketed, the per-pair inputs start at a power-of-two word offset
+  // (so that pairs of similar lengths share one layout) and nothing is
+  // copied: *upload_bytes tells how much of the staging buffer to upload.
+  bool load(wfa_host::Engine* e, const wfa_pair_t* pairs, int n, bool cigars, bool bucketed = false,
+            size_t* upload_bytes = nullptr);
+  const char* h_ops = nullptr;             // a host copy of the operations (single-pair graph), or null
+  // run-length CIGARs of the last run (wfa_device_batch_download_runs), made on first request
+  wfa_host::DevBuf d_run_counts, d_run_off, d_runs, d_scan_temp;
+  bool runs_ready = false;
+  long long total_runs = 0;
+  std::vector<long long> h_run_off;
   ~wfa_device_batch_t();
 };
 

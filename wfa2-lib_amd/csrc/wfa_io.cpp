@@ -38,7 +38,9 @@ struct Batch {
   std::vector<int64_t> poff, toff;
   std::vector<int> plen, tlen;
   std::vector<wfa_result_t> res;
-  std::vector<char> arena;          // CIGAR operations
+  std::vector<char> arena;          // CIGAR operations (byte form)
+  std::vector<uint32_t> runs;       // or run-length CIGARs made on the device
+  bool by_runs = false;             // res[i].cigar_offset / cigar_length index runs
   int n() const { return (int)plen.size(); }
 };
 
@@ -132,6 +134,43 @@ int cigar_score_of(const wavefront_penalties_t& p, const char* ops, int n) {
   }
 }
 
+// The same scores over run-length CIGARs ((length << 4) | SAM code: 7 '=',
+// 8 X, 1 I, 2 D): a run of one gap type opens once, as the byte loops of
+// cigar_score_gap_affine see it (no two runs of one type are adjacent).
+int runs_score_of(const wavefront_penalties_t& p, const uint32_t* runs, int n) {
+  long long score = 0;
+  for (int i = 0; i < n; ++i) {
+    const int len = (int)(runs[i] >> 4);
+    const uint32_t c = runs[i] & 15u;
+    const bool match = c == 7u, mism = c == 8u, gap = c == 1u || c == 2u;
+    switch (p.distance_metric) {
+      case indel:
+      case edit:
+        if (!match) score += len;
+        break;
+      case gap_linear:
+        score -= (long long)len * (match ? p.linear_penalties.match
+                                         : mism ? p.linear_penalties.mismatch : p.linear_penalties.indel);
+        break;
+      case gap_affine: {
+        const affine_penalties_t& a = p.affine_penalties;
+        if (match) score -= (long long)a.match * len;
+        else if (mism) score -= (long long)a.mismatch * len;
+        else if (gap) score -= a.gap_opening + (long long)a.gap_extension * len;
+        break;
+      }
+      default: {
+        const affine2p_penalties_t& a = p.affine2p_penalties;
+        if (match) score -= (long long)a.match * len;
+        else if (mism) score -= (long long)a.mismatch * len;
+        else if (gap) score -= std::min((long long)a.gap_opening1 + (long long)a.gap_extension1 * len,
+                                        (long long)a.gap_opening2 + (long long)a.gap_extension2 * len);
+      }
+    }
+  }
+  return (int)score;
+}
+
 void append_int(std::string& out, long long v) {
   char tmp[24];
   int n = 0;
@@ -149,11 +188,16 @@ void append_int(std::string& out, long long v) {
 void format_lines(const Batch& b, int lo, int hi, bool score_only, bool full_output, const wavefront_penalties_t& pen,
                   std::string& text) {
   text.clear();
+  static const char kRunOp[16] = {'M', 'I', 'D', 'N', '?', '?', '?', 'M', 'X', '?', '?', '?', '?', '?', '?', '?'};
   for (int i = lo; i < hi; ++i) {
     const wfa_result_t& r = b.res[i];
     const bool has_cigar = !score_only && r.cigar_length > 0;
-    const char* ops = b.arena.data() + (has_cigar ? r.cigar_offset : 0);
-    const int score = score_only ? r.score : (has_cigar ? cigar_score_of(pen, ops, r.cigar_length) : -1);
+    const char* ops = b.by_runs ? nullptr : b.arena.data() + (has_cigar ? r.cigar_offset : 0);
+    const uint32_t* runs = b.by_runs ? b.runs.data() + (has_cigar ? r.cigar_offset : 0) : nullptr;
+    const int score = score_only ? r.score
+                                 : (!has_cigar ? -1
+                                               : (b.by_runs ? runs_score_of(pen, runs, r.cigar_length)
+                                                            : cigar_score_of(pen, ops, r.cigar_length)));
     if (full_output) {
       append_int(text, b.plen[i]);
       text += '\t';
@@ -170,6 +214,11 @@ void format_lines(const Batch& b, int lo, int hi, bool score_only, bool full_out
     }
     if (!has_cigar) {
       text += '-';
+    } else if (b.by_runs) {
+      for (int j = 0; j < r.cigar_length; ++j) {
+        append_int(text, (long long)(runs[j] >> 4));
+        text += kRunOp[runs[j] & 15u];
+      }
     } else {
       int j = 0;
       while (j < r.cigar_length) {
@@ -261,6 +310,12 @@ extern "C" int wfa_align_seq_file(wavefront_aligner_t* const wf, const char* in_
     }
   });
 
+  // CIGARs come back run-length (made on the device) except where the host
+  // trims them (extension form, z-drop).
+  const bool by_runs = !score_only && !wf->alignment_form.extension &&
+                       !((wf->heuristic.strategy & wf_heuristic_zdrop) && wf->memory_mode != wavefront_memory_ultralow);
+  wfa_host::Engine* e = wfa_host::engine_of(wf);
+  std::unique_ptr<wfa_device_batch_t> db;
   int first_fail = WF_STATUS_ALG_COMPLETED;
   for (;;) {
     std::unique_ptr<Batch> b = to_align.get();
@@ -274,9 +329,30 @@ extern "C" int wfa_align_seq_file(wavefront_aligner_t* const wf, const char* in_
       ops += (size_t)b->plen[i] + b->tlen[i];
     }
     b->res.resize(n);
-    b->arena.resize(std::max<size_t>(ops, 1));
-    const int rc = wavefront_align_batch(wf, pr.data(), n, b->res.data(), score_only ? nullptr : b->arena.data(),
-                                         score_only ? 0 : b->arena.size());
+    int rc;
+    if (by_runs && e && wfa_host::engine_init(e)) {
+      if (!db) db.reset(new (std::nothrow) wfa_device_batch_t());
+      rc = WF_STATUS_OOM;
+      if (db && db->load(e, pr.data(), n, true)) {
+        rc = wfa_device_batch_run(wf, db.get());
+        if (db->have_results) {
+          const int64_t total = wfa_device_batch_cigar_runs(wf, db.get());
+          if (total >= 0) {
+            b->runs.resize((size_t)std::max<int64_t>(total, 1));
+            rc = wfa_device_batch_download_runs(wf, db.get(), b->res.data(), b->runs.data(), b->runs.size());
+            b->by_runs = true;
+          } else {
+            rc = (int)total;
+          }
+        }
+      }
+      if (!b->by_runs)
+        for (auto& r : b->res) r = wfa_result_t{rc, INT32_MIN, -1, -1, 0, 0};
+    } else {
+      b->arena.resize(std::max<size_t>(ops, 1));
+      rc = wavefront_align_batch(wf, pr.data(), n, b->res.data(), score_only ? nullptr : b->arena.data(),
+                                 score_only ? 0 : b->arena.size());
+    }
     if (rc != WF_STATUS_ALG_COMPLETED && first_fail == WF_STATUS_ALG_COMPLETED) first_fail = rc;
     pairs += n;
     align_s += now_s() - t;

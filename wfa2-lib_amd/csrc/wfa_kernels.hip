@@ -17,14 +17,19 @@
 // compute that cell, so the winner (and its tie-break M > D2 > D1 > I2 > I1,
 // extension before opening) is decided here once, and stored as 7 bits next
 // to the cell's pre-extension M offset.  The backtrace kernel replays them.
+#include <cstdio>
+#include <cstdlib>
+
 #include "wfa_common.h"
 
 namespace wfa {
 namespace {
 
 // CUT: 0 exact; 1 wf-adaptive, wfmash and the bands; 2 also x-drop / z-drop.
+// The 1024-thread instance is held to 64 VGPRs (8 waves per SIMD), so two
+// workgroups, two pairs, share a CU.
 template <int NC, int BLOCK, bool SMALL, int BITS, int CUT>
-__global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
+__global__ void __launch_bounds__(BLOCK, BLOCK == 1024 ? 8 : 1) wfa_forward_kernel(ForwardArgs a) {
   constexpr bool HC = CUT != 0;
   // The LDS family stores the ring as int16, the HBM family as int32.
   using RT = typename std::conditional<SMALL, int16_t, int>::type;
@@ -34,6 +39,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
   // One wave and the ring in LDS: no workgroup barrier is ever needed.
   constexpr bool WAVE_ONLY = SMALL && NW == 1;
   __shared__ int sh_bcast[2];
+  __shared__ int sh_end[2];  // the end cell's value, by step parity (see the termination test)
   __shared__ int sh_red[NW > 1 ? NW : 1][2 * NC + 1];
   __shared__ int sh_hit[NW > 1 ? NW : 1];
   BlockRed<BLOCK> red{sh_hit};
@@ -154,7 +160,10 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
     const int pb = (int)(ppos % SPW), tb = (int)(tpos % SPW);
     const uint64_t* P;
     const uint64_t* T;
-    if constexpr (SMALL) {
+    // The LDS family, and the HBM-ring family when the host gave room
+    // (lmax_words > 0), read the sequences from LDS: the extension's
+    // windows then cost an LDS round trip instead of an L2 one.
+    if (SMALL || a.lmax_words > 0) {
       const uint64_t* gP = a.words + ppos / SPW;
       const uint64_t* gT = a.words + tpos / SPW;
       const int nwp = (pb + plen) / SPW + 2, nwt = (tb + tlen) / SPW + 2;
@@ -396,6 +405,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
             if (rec) rec[k - lo] = (uint8_t)src;
             outM[k] = to_ring(m, ring);
             vM = (m != kOffsetNull);
+            if (k == kend) sh_end[s & 1] = (int)to_ring(m, ring);
           }
         } else if (k <= hi) {
           const int om_l = rd(ring, mo1, k - 1);
@@ -436,6 +446,11 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
           if constexpr (NC == 5) {
             if (has_i2) { outI2[k] = to_ring(ins2, ring); vI2 = in_bounds(ins2, k, plen, tlen); }
             if (has_d2) { outD2[k] = to_ring(del2, ring); vD2 = in_bounds(del2, k, plen, tlen); }
+          }
+          if (k == kend) {
+            // the end component's value, for the termination test after the step
+            const int ve = cend == kM ? m : cend == kI1 ? ins1 : cend == kD1 ? del1 : cend == kI2 ? ins2 : del2;
+            sh_end[s & 1] = (int)to_ring(ve, ring);
           }
         }
         const int kw = kb + wid * 64;
@@ -539,8 +554,11 @@ __global__ void __launch_bounds__(BLOCK) wfa_forward_kernel(ForwardArgs a) {
           if (ce == kD2) { pce = has_d2; rce = rD2; }
         }
         if (pce && !rce.null() && kend >= rce.first && kend <= rce.last) {
-          // the same LDS word for every lane: keep the loop control scalar
-          const int vend = uniform((int)ring[ring_base(pen, wcap, ce, rp, 0) + (kend - lo)]);
+          // The lane of k_end left the value in sh_end (by step parity, so a
+          // wave already in the next step cannot overwrite it before every wave
+          // read it: one block barrier per step): no ring round trip.  The
+          // same LDS word for every lane: keep the loop control scalar.
+          const int vend = uniform(sh_end[s & 1]);
           if (vend >= tlen) status = kDevCompleted;
         }
       }
@@ -891,18 +909,24 @@ const void* select_forward(int family, int ncomp, int bits, int block, int cut) 
 }  // namespace
 
 size_t forward_lds_bytes(int family, int ring_slots, int wcap, int lmax_words) {
-  size_t b = 0;
-  if (family == kSmall) {
-    b += (size_t)2 * lmax_words * sizeof(uint64_t);
-    b += (size_t)ring_slots * wcap * sizeof(int16_t);
-  }
+  size_t b = (size_t)2 * lmax_words * sizeof(uint64_t);  // sequences (HBM-ring family: 0 when they do not fit)
+  if (family == kSmall) b += (size_t)ring_slots * wcap * sizeof(int16_t);
   return b;
 }
 
+// Threads per pair of the HBM-ring family by n + m: 64 up to t64, 256 up to
+// t256, else 1024.  WFA_LARGE_BLOCKS="t64,t256" overrides the limits (tuning).
 int pick_block(int family, int max_width) {
+  static const int* lim = [] {
+    static int t[2] = {256, 4096};
+    const char* v = getenv("WFA_LARGE_BLOCKS");
+    int a = 0, b = 0;
+    if (v && sscanf(v, "%d,%d", &a, &b) == 2 && a >= 0 && b >= a) { t[0] = a; t[1] = b; }
+    return t;
+  }();
   if (family == kSmall) return 64;
-  if (max_width <= 256) return 64;
-  if (max_width <= 4096) return 256;
+  if (max_width <= lim[0]) return 64;
+  if (max_width <= lim[1]) return 256;
   return 1024;
 }
 

@@ -7,8 +7,9 @@
 // stream, a resident batch and a record arena) and runs back-to-back device
 // batches.  A free lane admits the queued requests of its class in arrival
 // order, up to a pair count and up to its device-memory budget on the
-// estimated footprint of each request (about L^2/16 bytes of backtrace
-// records for memory-high, O(L) for BiWFA), so requests that arrive while
+// estimated footprint of each request (O(s^2) bytes of backtrace records for
+// memory-high, O(s) for BiWFA, s estimated from the length and an error
+// prior), so requests that arrive while
 // other batches are on the GPU join the next batch of any free lane and the
 // batches of different lanes overlap on the device.  Results and their CIGARs
 // are queued for wfa_server_collect with submit / start / finish times.
@@ -79,10 +80,38 @@ struct wfa_server_t {
     if (L <= cfg.short_max_length) return kShort;
     return kMedium;
   }
-  static double footprint(int cls, int plen, int tlen) {
+  // Estimated alignment score of a request: error_prior * L errors, equally
+  // substitutions, 1-symbol insertions and 1-symbol deletions, at the
+  // aligner's penalties.
+  double score_estimate(double L) const {
+    double sub = 1, gap = 1;
+    switch (attr.distance_metric) {
+      case indel: sub = 2; gap = 1; break;
+      case edit: sub = 1; gap = 1; break;
+      case gap_linear: sub = attr.linear_penalties.mismatch; gap = attr.linear_penalties.indel; break;
+      case gap_affine:
+        sub = attr.affine_penalties.mismatch;
+        gap = attr.affine_penalties.gap_opening + attr.affine_penalties.gap_extension;
+        break;
+      default:
+        sub = attr.affine2p_penalties.mismatch;
+        gap = std::min(attr.affine2p_penalties.gap_opening1 + attr.affine2p_penalties.gap_extension1,
+                       attr.affine2p_penalties.gap_opening2 + attr.affine2p_penalties.gap_extension2);
+    }
+    return std::max(1.0, cfg.error_prior * L * (sub + 2 * gap) / 3);
+  }
+  // Device bytes a request is expected to take.  Memory-high: its backtrace
+  // records, one byte per diagonal-step plus 16 per score step, with about
+  // 0.2 s^2 diagonal-steps (9 L^2 e^2 measured on gap-affine (0,4,6,2) reads,
+  // where s is about 6.7 e L), plus its packed sequences and CIGAR: O(s^2).
+  // BiWFA: the breakpoint rings of both directions (5 slots x 3 components x
+  // (n + m) int32 each, the device's allocation) and the records of its
+  // base cases (scores under 250): O(s) beyond the sequences.
+  double footprint(int cls, int plen, int tlen) const {
     const double L = std::max(plen, tlen);
-    if (cls == kLong) return 64.0 * L + 1e6;
-    return L * L / 16 + 8 * L + 4096;
+    const double s = score_estimate(L);
+    if (cls == kLong) return 2.0 * 5 * 3 * 4 * (plen + tlen + 8.0) + 64.0 * s + 2 * L + (1 << 20);
+    return 0.2 * s * s + 16 * s + 2 * L + 4096;
   }
   void lane_main(int cls);
 };
@@ -158,12 +187,14 @@ void wfa_server_t::lane_main(int cls) {
 }
 
 extern "C" void wfa_server_config_default(wfa_server_config_t* c) {
-  // One lane per class: each lane is a HIP stream, and with HIP's default of
-  // four hardware queues per process more streams share queues (kernels of
-  // streams on one queue run in order).
-  c->short_lanes = 1;
-  c->medium_lanes = 1;
-  c->long_lanes = 1;
+  // Two lanes per class: while one lane's batch is on the device, the other
+  // admits the requests that arrive meanwhile.  Every lane's stream has a
+  // hardware queue of its own (CU-masked stream), so the batches of
+  // different lanes overlap on the device.
+  c->short_lanes = 2;
+  c->medium_lanes = 2;
+  c->long_lanes = 2;
+  c->error_prior = 0.1;
   c->short_max_length = 2048;
   c->long_min_length = 32768;
   c->max_batch_pairs = 8192;
@@ -178,6 +209,7 @@ extern "C" wfa_server_t* wfa_server_create(const wavefront_aligner_attr_t* attri
   else wfa_server_config_default(&s->cfg);
   s->attr = attributes ? *attributes : wavefront_aligner_attr_default;
   if (s->cfg.max_batch_pairs < 1) s->cfg.max_batch_pairs = 1;
+  if (!(s->cfg.error_prior > 0 && s->cfg.error_prior <= 1)) s->cfg.error_prior = 0.1;
   const int counts[kClasses] = {std::max(1, s->cfg.short_lanes), std::max(1, s->cfg.medium_lanes),
                                 std::max(1, s->cfg.long_lanes)};
   for (int c = 0; c < kClasses; ++c)
@@ -193,7 +225,7 @@ extern "C" int64_t wfa_server_submit(wfa_server_t* s, const char* pattern, int p
   r->text.assign(text, (size_t)tlen);
   r->submit = now_s();
   const int cls = s->classify(plen, tlen);
-  r->footprint = wfa_server_t::footprint(cls, plen, tlen);
+  r->footprint = s->footprint(cls, plen, tlen);
   const int64_t id = r->id;
   s->outstanding.fetch_add(1);
   {

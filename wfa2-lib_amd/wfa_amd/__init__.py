@@ -137,7 +137,7 @@ class wfa_result_t(Structure):
 class wfa_server_config_t(Structure):
     _fields_ = [("short_lanes", c_int), ("medium_lanes", c_int), ("long_lanes", c_int),
                 ("short_max_length", c_int), ("long_min_length", c_int), ("max_batch_pairs", c_int),
-                ("lane_memory_bytes", ctypes.c_size_t)]
+                ("lane_memory_bytes", ctypes.c_size_t), ("error_prior", c_double)]
 
 
 class wfa_server_result_t(Structure):
@@ -174,6 +174,10 @@ def lib():
         L.wfa_device_batch_download.argtypes = [POINTER(wavefront_aligner_t), c_void_p, c_void_p, c_void_p,
                                                 ctypes.c_size_t]
         L.wfa_device_batch_free.argtypes = [c_void_p]
+        L.wfa_device_batch_cigar_runs.restype = c_int64
+        L.wfa_device_batch_cigar_runs.argtypes = [POINTER(wavefront_aligner_t), c_void_p]
+        L.wfa_device_batch_download_runs.argtypes = [POINTER(wavefront_aligner_t), c_void_p, c_void_p, c_void_p,
+                                                     ctypes.c_size_t]
         L.wfa_device_batch_stats.argtypes = [c_void_p, POINTER(wfa_run_stats_t)]
         L.wfa_server_config_default.argtypes = [POINTER(wfa_server_config_t)]
         L.wfa_server_create.restype = c_void_p
@@ -198,20 +202,58 @@ def default_attributes() -> wavefront_aligner_attr_t:
 
 
 class BatchResult:
-    """Per-pair results of one batch: numpy arrays plus the CIGAR arena."""
+    """Per-pair results of one batch: numpy views of the wfa_result_t array
+    and the CIGAR arena (filled in place by the library: no zero-fill, no
+    second copy)."""
 
     def __init__(self, n, results, arena):
         self.n = n
         self._r = results
         self.arena = arena
-        self.status = np.array([results[i].status for i in range(n)], dtype=np.int32)
-        self.score = np.array([results[i].score for i in range(n)], dtype=np.int32)
-        self.cigar_offset = np.array([results[i].cigar_offset for i in range(n)], dtype=np.int64)
-        self.cigar_length = np.array([results[i].cigar_length for i in range(n)], dtype=np.int32)
+        self.status = results["status"][:n]
+        self.score = results["score"][:n]
+        self.cigar_offset = results["cigar_offset"][:n]
+        self.cigar_length = results["cigar_length"][:n]
 
     def ops(self, i) -> bytes:
+        if self.arena is None:
+            return b""
         o = int(self.cigar_offset[i])
-        return self.arena[o:o + int(self.cigar_length[i])] if self.arena is not None else b""
+        return self.arena[o:o + int(self.cigar_length[i])].tobytes()
+
+
+# wfa_result_t as a numpy record (int status, score, end_v, end_h; int64 offset; int length; padding)
+_RESULT_DTYPE = np.dtype([("status", np.int32), ("score", np.int32), ("end_v", np.int32), ("end_h", np.int32),
+                          ("cigar_offset", np.int64), ("cigar_length", np.int32), ("pad", np.int32)])
+
+_SAM_OPS = {7: "M", 8: "X", 1: "I", 2: "D"}
+
+
+class RunsResult:
+    """Results with run-length CIGARs (wfa_device_batch_download_runs)."""
+
+    def __init__(self, n, res, runs, rc):
+        self.n = n
+        self.rc = rc
+        self.status = res["status"]
+        self.score = res["score"]
+        self.cigar_offset = res["cigar_offset"]
+        self.cigar_length = res["cigar_length"]
+        self.runs = runs
+
+    def pair_runs(self, i):
+        o = int(self.cigar_offset[i])
+        return self.runs[o:o + int(self.cigar_length[i])]
+
+    def rle(self, i) -> str:
+        """The CIGAR as cigar_sprint prints it with matches ("12M1X9M"), "-" when empty."""
+        r = self.pair_runs(i)
+        if len(r) == 0:
+            return "-"
+        return "".join("%d%s" % (int(x) >> 4, _SAM_OPS.get(int(x) & 15, "?")) for x in r)
+
+    def ops(self, i) -> bytes:
+        return b"".join(_SAM_OPS.get(int(x) & 15, "?").encode() * (int(x) >> 4) for x in self.pair_runs(i))
 
 
 class _Pairs:
@@ -266,12 +308,27 @@ class DeviceBatch:
 
     def download(self, cigars=True) -> BatchResult:
         n = self.pairs.n
-        res = (wfa_result_t * max(n, 1))()
-        arena = ctypes.create_string_buffer(self.pairs.total + 1) if cigars else None
-        lib().wfa_device_batch_download(self.al.wf, self.h, ctypes.addressof(res),
-                                        ctypes.addressof(arena) if arena is not None else None,
+        res = np.empty(max(n, 1), dtype=_RESULT_DTYPE)
+        arena = np.empty(self.pairs.total + 1, dtype=np.uint8) if cigars else None
+        lib().wfa_device_batch_download(self.al.wf, self.h, res.ctypes.data,
+                                        arena.ctypes.data if arena is not None else None,
                                         self.pairs.total + 1 if arena is not None else 0)
-        return BatchResult(n, res, arena.raw if arena is not None else None)
+        return BatchResult(n, res, arena)
+
+    def download_runs(self, runs=None):
+        """wfa_device_batch_download_runs: status, score, run offsets and
+        counts per pair (numpy arrays) and the runs ((length << 4) | SAM code,
+        uint32), into `runs` when it is a large enough uint32 array (a pinned
+        or reused buffer avoids the allocation)."""
+        total = lib().wfa_device_batch_cigar_runs(self.al.wf, self.h)
+        if total < 0:
+            raise RuntimeError("wfa_device_batch_cigar_runs returned %d" % total)
+        n = self.pairs.n
+        res = np.empty(max(n, 1), dtype=_RESULT_DTYPE)
+        if runs is None or len(runs) < total:
+            runs = np.empty(max(total, 1), dtype=np.uint32)
+        rc = lib().wfa_device_batch_download_runs(self.al.wf, self.h, res.ctypes.data, runs.ctypes.data, len(runs))
+        return RunsResult(n, res[:n], runs[:total], rc)
 
     def free(self):
         if self.h:
@@ -359,12 +416,12 @@ class Aligner:
         """wavefront_align_batch over host buffers."""
         pr = _Pairs(patterns, texts)
         n = pr.n
-        res = (wfa_result_t * max(n, 1))()
-        arena = None if self.score_only else ctypes.create_string_buffer(pr.total + 1)
-        lib().wavefront_align_batch(self.wf, pr.address, n, ctypes.addressof(res),
-                                    ctypes.addressof(arena) if arena is not None else None,
+        res = np.empty(max(n, 1), dtype=_RESULT_DTYPE)
+        arena = None if self.score_only else np.empty(pr.total + 1, dtype=np.uint8)
+        lib().wavefront_align_batch(self.wf, pr.address, n, res.ctypes.data,
+                                    arena.ctypes.data if arena is not None else None,
                                     pr.total + 1 if arena is not None else 0)
-        return BatchResult(n, res, arena.raw if arena is not None else None)
+        return BatchResult(n, res, arena)
 
     def upload(self, patterns, texts=None) -> DeviceBatch:
         return DeviceBatch(self, patterns, texts)
