@@ -398,6 +398,22 @@ typedef struct {
 } wfa_run_stats_t;
 int wfa_device_batch_stats(wfa_device_batch_t* batch, wfa_run_stats_t* stats);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same run as wfa_device_batch_run, with every device
+ * buffer beyond the batch's inputs and outputs (run counters, launch order
+ * arrays, HBM wavefront rings, backtrace-record arena) taken from one static
+ * 256 MB device array per GPU, held until the process ends.  No HIP
+ * allocation or free is made.  The batch must have been uploaded with this
+ * aligner (wfa_device_batch_upload sizes its CIGAR output for aligners that
+ * compute alignments).  Limits, each refused with a [WFA] message before any
+ * device work: at most about 20 M pairs per call; each pair's wavefront ring
+ * (4 x ring slots x (n + m + 8) bytes) within 64 MB; memory mode ultralow is
+ * refused (WF_STATUS_UNSUPPORTED).  Batches whose records exceed the arena
+ * run in waves; a pair whose own records do not fit ends WF_STATUS_OOM.
+ * Calls that overlap on one GPU share the array and race: serialise them.
+ * No kernel timings are kept (wfa_run_stats_t forward_ms / backtrace_ms 0). */
+int wfa_device_batch_run_noalloc(wavefront_aligner_t* const wf_aligner, wfa_device_batch_t* batch);
 
 /* ------------------------------------------------------------ serving
  * No counterpart in the reference (its wavefront_align is synchronous per

@@ -1654,6 +1654,87 @@ extern "C" int wfa_device_batch_run(wavefront_aligner_t* const wf, wfa_device_ba
   return batch_run(wf, b, e);
 }
 
+namespace wfa_host {
+
+// The no-allocation run (wfa_device_batch_run_noalloc, wfa_noalloc.hip): the
+// region [scratch, scratch + bytes) is split into the run counters, the plan
+// groups' order arrays (8 bytes per pair covers the first plan and every
+// retry plan), the HBM-ring share and the record arena; the engine's own
+// buffers are swapped out for views of these parts during the run and
+// swapped back afterwards.
+int device_batch_run_in(wavefront_aligner_t* wf, wfa_device_batch_t* b, char* scratch, size_t bytes) {
+  constexpr size_t kRingShare = 64ull << 20;
+  constexpr size_t kMinArena = 32ull << 20;
+  if (!b) return WF_STATUS_OOM;
+  presets_check(wf, b->plen.data(), b->tlen.data(), b->n);
+  const int sup = check_supported(wf);
+  if (sup != WF_STATUS_OK) return sup;
+  Engine* e = engine_of(wf);
+  if (!e || !e->ok) {
+    fprintf(stderr, "[WFA] wfa_device_batch_run_noalloc: the batch must be uploaded with this aligner first\n");
+    return WF_STATUS_OOM;
+  }
+  if (wf->memory_mode == wavefront_memory_ultralow) {
+    fprintf(stderr, "[WFA] wfa_device_batch_run_noalloc: memory mode ultralow (BiWFA) is not available "
+            "without allocation; use wfa_device_batch_run\n");
+    return WF_STATUS_UNSUPPORTED;
+  }
+  const int records = (wf->alignment_scope == compute_alignment) ? 1 : 0;
+  if (records && b->d_ops.cap < (size_t)std::max<int64_t>(b->total_ops, 1)) {
+    fprintf(stderr, "[WFA] wfa_device_batch_run_noalloc: the batch has no CIGAR output buffer (upload it with "
+            "an aligner that computes alignments)\n");
+    return WF_STATUS_OOM;
+  }
+  const size_t order_bytes = ((size_t)b->n * 8 + 4096 + 255) & ~(size_t)255;
+  if (256 + order_bytes + kRingShare + kMinArena > bytes) {
+    fprintf(stderr, "[WFA] wfa_device_batch_run_noalloc: %d pairs exceed the %zu MB scratch array "
+            "(at most %zu pairs per call)\n", b->n, bytes >> 20,
+            (size_t)((bytes - 256 - kRingShare - kMinArena - 4096) / 8));
+    return WF_STATUS_OOM;
+  }
+  const Penalties pen = make_penalties(wf->penalties);
+  for (int i = 0; i < b->n; ++i) {
+    const size_t ring = (size_t)wfa::ring_slots(pen) * ((size_t)b->plen[i] + b->tlen[i] + 8) * sizeof(int);
+    if (ring > kRingShare) {
+      fprintf(stderr, "[WFA] wfa_device_batch_run_noalloc: pair %d (%d + %d symbols) needs a %zu MB wavefront "
+              "ring, more than the %zu MB share of the scratch array\n", i, b->plen[i], b->tlen[i], ring >> 20,
+              kRingShare >> 20);
+      return WF_STATUS_OOM;
+    }
+  }
+  // counters | order arrays | ring share | record arena
+  Engine::NoAlloc na;
+  na.order.base = scratch + 256;
+  na.order.bytes = order_bytes;
+  na.ring_bytes = kRingShare;
+  ScratchRegion fixed;  // binds the views below: they never allocate or free
+  DevBuf counters, ring, rec;
+  counters.p = scratch;
+  counters.cap = 256;
+  counters.region = &fixed;
+  ring.p = scratch + 256 + order_bytes;
+  ring.cap = kRingShare;
+  ring.region = &fixed;
+  rec.p = scratch + 256 + order_bytes + kRingShare;
+  rec.cap = (bytes - 256 - order_bytes - kRingShare) & ~(size_t)255;
+  rec.region = &fixed;
+  std::swap(e->counters, counters);
+  std::swap(e->ring, ring);
+  std::swap(e->rec, rec);
+  const size_t rec_bytes = e->rec_bytes, rec_cap = e->rec_cap;
+  e->rec_bytes = e->rec_cap = e->rec.cap;
+  e->na = &na;
+  const int rc = batch_run(wf, b, e);
+  e->na = nullptr;
+  e->rec_bytes = rec_bytes;
+  e->rec_cap = rec_cap;
+  std::swap(e->counters, counters);
+  std::swap(e->ring, ring);
+  std::swap(e->rec, rec);
+  return rc;
+}
+
+}  // namespace wfa_host
 
 extern "C" int wfa_device_batch_stats(wfa_device_batch_t* b, wfa_run_stats_t* stats) {
   if (!b || !stats || !b->have_results) return -1;

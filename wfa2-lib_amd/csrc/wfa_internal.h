@@ -285,4 +285,7 @@ int bialign_run(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, RunTo
 // Copies the results to the host (once per run) and, when cigar_dst is
 // given, the operations arena with them, under one wait.
 bool download_results(Engine* e, wfa_device_batch_t* b, char* cigar_dst = nullptr);
+// wfa_device_batch_run with every device buffer beyond the batch's inputs and
+// outputs carved from [scratch, scratch + scratch_bytes): no HIP allocation.
+int device_batch_run_in(wavefront_aligner_t* wf, wfa_device_batch_t* b, char* scratch, size_t scratch_bytes);
 }  // namespace wfa_host

@@ -171,6 +171,7 @@ def lib():
         L.wfa_device_batch_upload.restype = c_void_p
         L.wfa_device_batch_upload.argtypes = [POINTER(wavefront_aligner_t), c_void_p, c_int]
         L.wfa_device_batch_run.argtypes = [POINTER(wavefront_aligner_t), c_void_p]
+        L.wfa_device_batch_run_noalloc.argtypes = [POINTER(wavefront_aligner_t), c_void_p]
         L.wfa_device_batch_download.argtypes = [POINTER(wavefront_aligner_t), c_void_p, c_void_p, c_void_p,
                                                 ctypes.c_size_t]
         L.wfa_device_batch_free.argtypes = [c_void_p]
@@ -300,6 +301,10 @@ class DeviceBatch:
 
     def run(self) -> int:
         return lib().wfa_device_batch_run(self.al.wf, self.h)
+
+    def run_noalloc(self) -> int:
+        """wfa_device_batch_run_noalloc: the same run with no device allocation."""
+        return lib().wfa_device_batch_run_noalloc(self.al.wf, self.h)
 
     def stats(self) -> wfa_run_stats_t:
         s = wfa_run_stats_t()
