@@ -187,14 +187,16 @@ wfa::Counters* counters_of(Engine* e) { return e->counters.as<wfa::Counters>(); 
 
 // Record blocks are addressed by 32-bit indices of 16-byte units.
 constexpr size_t kMaxArenaBytes = 0xFFFFFFF0ull * 16;
-// The arena for long pairs (HBM-ring family) is sized at L^2 / 2 bytes of
-// records per pair (the average over 1-20 kbp pairs of 1-15% error is about
-// L^2 / 4, up to 0.7 L^2 at 15%), and such a pair starts only while the arena
-// has L^2 / 4 bytes left (L^2 >> 6 units of 16 bytes).
-constexpr double kLargeRecordShareOfL2 = 2.0;
+// The arena for long pairs (HBM-ring family) is sized at L^2 bytes of
+// records per pair (1-20 kbp pairs of 1-15% error average about L^2 / 4 and
+// reach 0.7 L^2 at 15%; a sample that runs out of arena re-runs its last
+// pairs almost alone, a launch at a few percent occupancy), and such a pair
+// starts only while the arena has L^2 / 2 bytes left (L^2 >> 5 units of 16
+// bytes).
+constexpr double kLargeRecordShareOfL2 = 1.0;
 // LDS for the two sequences of a pair of the HBM-ring family.
 constexpr size_t kLargeSeqLdsBytes = 64 * 1024;
-constexpr int kLargeDeferShift = 6;
+constexpr int kLargeDeferShift = 5;
 
 bool engine_init(Engine* e) {
   if (e->ok) return true;
@@ -755,7 +757,7 @@ void group_args(wavefront_aligner_t* wf, Engine* e, const JobSet& js, const Plan
   fa.records = records;
   fa.comp_begin = wfa::kM;
   fa.comp_end = wfa::kM;
-  fa.defer_shift = kLargeDeferShift;  // L^2 >> shift units of 16 B: about L^2/4 bytes of records
+  fa.defer_shift = kLargeDeferShift;  // L^2 >> shift units of 16 B: L^2/2 bytes of records
   fa.endsfree = js.ef.on;
   fa.pbf = js.ef.pbf;
   fa.pef = js.ef.pef;
@@ -939,7 +941,7 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
   if (records) {
     // Rough footprint: one byte per diagonal-step plus 16 per score step.
     // Diagonal-steps grow as L^2 e^2 (e the error rate): about L^2/16 for
-    // short reads of up to 10% error; long pairs take L^2/2 (see
+    // short reads of up to 10% error; long pairs take L^2 (see
     // kLargeRecordShareOfL2); plus a 64 KB chunk per resident workgroup.
     double want = 0;
     for (PlanGroup& g : plan) {
@@ -1347,8 +1349,12 @@ bool env_on(const char* name, bool dflt) {
   const char* v = getenv(name);
   return v ? v[0] != '0' : dflt;
 }
-constexpr int kLeanSliceMin = 16384;   // pairs per candidate slice (lean family)
-constexpr int kLargeSliceMin = 48;     // pairs per candidate slice (HBM-ring family)
+// Pairs per candidate slice: enough to fill the GPU, so that a slice
+// measures throughput rather than the latency of its longest pairs (a slice
+// of 50-100 long pairs favoured the widest workgroups and lost 6% against
+// the fixed rule on 1-20 kbp pairs).
+constexpr int kLeanSliceMin = 16384;   // lean family: about two waves per SIMD
+constexpr int kLargeSliceMin = 512;    // HBM-ring family: 2 x the 1024-thread pairs in flight
 constexpr int kLargeBlocks[3] = {64, 256, 1024};
 
 // HBM-ring family length bins: log2(L / 512), up to 10.
@@ -1428,9 +1434,12 @@ void tune_plan(Engine* e, const JobSet& js, const std::vector<int>& conf, std::v
       auto& v = by_block[it->second];
       v.insert(v.end(), ids.begin(), ids.end());
     } else if ((int)ids.size() >= 3 * kLargeSliceMin) {
-      for (int c = 0; c < 3; ++c) {
+      // 64 threads per pair only up to 2 kbp: on longer pairs its slice runs
+      // for seconds (one wave steps through thousands of diagonals per step)
+      const int c0 = kv.first <= 1 ? 0 : 1, nc = 3 - c0;
+      for (int c = c0; c < 3; ++c) {
         std::vector<int> sl;
-        for (size_t i = c; i < ids.size(); i += 3) sl.push_back(ids[i]);
+        for (size_t i = c - c0; i < ids.size(); i += nc) sl.push_back(ids[i]);
         TuneTrial t{base, kLargeBlocks[c], make_large(e, js, std::move(sl))};
         t.group.block = kLargeBlocks[c];
         trials->push_back(std::move(t));

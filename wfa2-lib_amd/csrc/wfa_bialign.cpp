@@ -109,6 +109,10 @@ bool ensure_reverse(Engine* e, wfa_device_batch_t* b) {
 
 int bp_block(int size) { return size <= 512 ? 64 : (size <= 8192 ? 256 : 1024); }
 
+// LDS for the four sequences of a breakpoint problem (2-bit sequences of a
+// 100 kbp pair take 100 KB of gfx950's 160 KB per CU).
+constexpr size_t kBreakpointSeqLdsBytes = 120 * 1024;
+
 // Runs the breakpoint search of `nodes` (grouped by size) and returns the outcomes.
 bool run_breakpoints(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, const std::vector<Node>& nodes,
                      std::vector<wfa::BiOutcome>& out, RunTotals* tot) {
@@ -137,7 +141,17 @@ bool run_breakpoints(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, 
     }
     const int wcap = wmax + 8;
     const wfa::HeurArgs heur = heuristic_of(wf);
-    int per_cu = wfa::breakpoint_max_blocks_per_cu(ncomp, b->bits, block, heur.cut() != 0);
+    // The four sequences of the launch's longest problem in LDS when they fit.
+    const int spw = 64 / b->bits;
+    int lw = 0;
+    for (const wfa::BiProblem& p : probs) lw = std::max(lw, std::max(p.pe - p.pb, p.te - p.tb) / spw + 3);
+    if (wfa::breakpoint_lds_bytes(lw) > kBreakpointSeqLdsBytes) lw = 0;
+    int per_cu = wfa::breakpoint_max_blocks_per_cu(ncomp, b->bits, block, heur.cut() != 0,
+                                                   wfa::breakpoint_lds_bytes(lw));
+    if (per_cu <= 0 && lw > 0) {
+      lw = 0;
+      per_cu = wfa::breakpoint_max_blocks_per_cu(ncomp, b->bits, block, heur.cut() != 0);
+    }
     if (per_cu <= 0) per_cu = 1;
     const int grid = (int)std::min<int64_t>((int64_t)ids.size(), (int64_t)per_cu * e->num_cus);
     const long long stride = 2LL * pen.nslots * ncomp * wcap;
@@ -167,6 +181,7 @@ bool run_breakpoints(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, 
     a.queue = &c->queue;
     a.dsteps_total = &c->dsteps;
     a.max_steps = wf->system.max_alignment_steps > 0 ? wf->system.max_alignment_steps : INT_MAX;
+    a.lmax_words = lw;
     a.heur = heur;  // the breakpoint aligners inherit the heuristic (wavefront_bialigner.c:53)
     a.pen = pen;
     for (auto& ev : e->bi.ev)

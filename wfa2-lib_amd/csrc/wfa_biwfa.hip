@@ -269,10 +269,19 @@ __device__ int side_step(Side<NC, CUT>& sd, int s, int plen, int tlen, const Pen
       rD2.add(__ballot(vD2), kw);
     }
   }
-  if (want_max) *max_ak = red.max_all(best_ak);
+  // The furthest antidiagonal rides along with the range merge (one block
+  // barrier for both): a wave maximum first, then the merge over waves.
+  if (want_max) {
+    for (int o = 32; o > 0; o >>= 1) best_ak = max(best_ak, __shfl_xor(best_ak, o, 64));
+    best_ak = uniform(best_ak);
+  }
   if (prune_try) wmin = red.min_all(wmin);
+  if constexpr (NW == 1) {
+    if (want_max) *max_ak = best_ak;
+  }
   if constexpr (NW > 1) {
     if (lane == 0) {
+      sh_red[wid][2 * NC] = best_ak;
       sh_red[wid][0] = rM.first;  sh_red[wid][1] = rM.last;
       if constexpr (NC > 1) {
         sh_red[wid][2] = rI1.first; sh_red[wid][3] = rI1.last;
@@ -285,8 +294,10 @@ __device__ int side_step(Side<NC, CUT>& sd, int s, int plen, int tlen, const Pen
     }
     __syncthreads();
     rM.init(); rI1.init(); rD1.init(); rI2.init(); rD2.init();
+    int ak_all = 0;
 #pragma unroll
     for (int w = 0; w < NW; ++w) {
+      ak_all = max(ak_all, sh_red[w][2 * NC]);
       rM.merge(sh_red[w][0], sh_red[w][1]);
       if constexpr (NC > 1) {
         rI1.merge(sh_red[w][2], sh_red[w][3]);
@@ -297,6 +308,7 @@ __device__ int side_step(Side<NC, CUT>& sd, int s, int plen, int tlen, const Pen
         rD2.merge(sh_red[w][8], sh_red[w][9]);
       }
     }
+    if (want_max) *max_ak = ak_all;
   }
   __syncthreads();  // ring writes of this score visible to the whole workgroup
   if constexpr (NC == 1) {
@@ -494,6 +506,7 @@ __device__ int side_extend0(Side<NC, CUT>& sd, int plen, int tlen, int wcap, con
 template <int NC, int BLOCK, int BITS, bool CUT>
 __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
   constexpr int SPW = 64 / BITS;
+  extern __shared__ __align__(16) unsigned char bp_smem[];
   __shared__ int sh_red[BLOCK / 64][2 * NC + 2];
   __shared__ int sh_min[BLOCK / 64 > 0 ? BLOCK / 64 : 1];
   __shared__ int sh_q;
@@ -514,6 +527,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
     Side<NC, CUT> f, r;
     f.ring = a.ring + (long long)blockIdx.x * a.ring_stride;
     r.ring = f.ring + (long long)pen.nslots * NC * wcap;
+    __syncthreads();  // the previous sub-problem is done with the rings and the LDS sequences
     {
       const long long pp = a.fwd_ppos[pair] + pr.pb, tp = a.fwd_tpos[pair] + pr.tb;
       f.P = a.words + pp / SPW; f.pb = (int)(pp % SPW);
@@ -521,8 +535,25 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
       const long long rp = a.rev_ppos[pair] + (plen_all - pr.pe), rt = a.rev_tpos[pair] + (tlen_all - pr.te);
       r.P = a.rev_words + rp / SPW; r.pb = (int)(rp % SPW);
       r.T = a.rev_words + rt / SPW; r.tb = (int)(rt % SPW);
+      if (a.lmax_words > 0) {
+        // The four sequences in LDS (the host gave room for the longest
+        // problem of the launch): extension windows cost an LDS round trip.
+        const int nwp = (f.pb + plen) / SPW + 2, nwt = (f.tb + tlen) / SPW + 2;
+        const int nrp = (r.pb + plen) / SPW + 2, nrt = (r.tb + tlen) / SPW + 2;
+        uint64_t* s0 = reinterpret_cast<uint64_t*>(bp_smem);
+        uint64_t* s1 = s0 + a.lmax_words;
+        uint64_t* s2 = s1 + a.lmax_words;
+        uint64_t* s3 = s2 + a.lmax_words;
+        for (int i = threadIdx.x; i < nwp; i += BLOCK) s0[i] = f.P[i];
+        for (int i = threadIdx.x; i < nwt; i += BLOCK) s1[i] = f.T[i];
+        for (int i = threadIdx.x; i < nrp; i += BLOCK) s2[i] = r.P[i];
+        for (int i = threadIdx.x; i < nrt; i += BLOCK) s3[i] = r.T[i];
+        f.P = s0;
+        f.T = s1;
+        r.P = s2;
+        r.T = s3;
+      }
     }
-    __syncthreads();  // the previous sub-problem is done with the rings
     side_init(f, pr.comp_begin, pr.comp_end, wcap, a.heur.steps);
     side_init(r, pr.comp_end, pr.comp_begin, wcap, a.heur.steps);
     __syncthreads();
@@ -695,19 +726,29 @@ hipError_t launch_gather_ops(const OpsPiece* pieces, int n, const char* src, cha
   return hipGetLastError();
 }
 
-int breakpoint_max_blocks_per_cu(int ncomp, int bits, int block, bool cut) {
+size_t breakpoint_lds_bytes(int lmax_words) { return (size_t)4 * lmax_words * sizeof(uint64_t); }
+
+int breakpoint_max_blocks_per_cu(int ncomp, int bits, int block, bool cut, size_t lds_bytes) {
   const void* f = select_breakpoint(ncomp, bits, block, cut);
   if (!f) return 0;
+  if (lds_bytes > 48 * 1024 &&
+      hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) != hipSuccess)
+    return 0;
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, 0) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, lds_bytes) != hipSuccess) return 0;
   return n;
 }
 
 hipError_t launch_breakpoint(const BiArgs& a, int ncomp, int bits, int block, int grid, hipStream_t stream) {
   const void* f = select_breakpoint(ncomp, bits, block, a.heur.cut() != 0);
   if (!f) return hipErrorInvalidValue;
+  const size_t lds = breakpoint_lds_bytes(a.lmax_words);
+  if (lds > 48 * 1024) {
+    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
   void* args[] = {const_cast<BiArgs*>(&a)};
-  return hipLaunchKernel(f, dim3(grid), dim3(block), args, 0, stream);
+  return hipLaunchKernel(f, dim3(grid), dim3(block), args, lds, stream);
 }
 
 }  // namespace wfa

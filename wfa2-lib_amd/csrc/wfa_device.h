@@ -238,10 +238,13 @@ struct BiArgs {
   unsigned int* queue;
   unsigned long long* dsteps_total;
   int max_steps;
+  int lmax_words;              // LDS words per sequence (four: forward and reversed pattern and text); 0: global
   HeurArgs heur;
   Penalties pen;
 };
 hipError_t launch_breakpoint(const BiArgs& a, int ncomp, int bits, int block, int grid, hipStream_t stream);
+// Dynamic LDS of a breakpoint launch: the four sequences of a problem.
+size_t breakpoint_lds_bytes(int lmax_words);
 // A piece of a final CIGAR: len operations copied from src (or, when src < 0,
 // len copies of op) to dst.
 struct OpsPiece {
@@ -254,7 +257,7 @@ struct OpsPiece {
 hipError_t launch_gather_ops(const OpsPiece* pieces, int n, const char* src, char* dst, hipStream_t stream);
 hipError_t launch_reverse_pack(const uint64_t* words, const int64_t* src_pos, const int64_t* dst_pos, const int* len,
                                int n, int bits, uint64_t* out, hipStream_t stream);
-int breakpoint_max_blocks_per_cu(int ncomp, int bits, int block, bool cut);
+int breakpoint_max_blocks_per_cu(int ncomp, int bits, int block, bool cut, size_t lds_bytes = 0);
 
 // Run-length CIGARs (wfa_cigar.hip): per pair the runs of its operations
 // ops[ops_off + ops_begin .. ops_off + plen + tlen) as (length << 4) | SAM

@@ -380,8 +380,36 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 1024 ? 8 : 1) wfa_forward_kern
       WaveRange rM, rI1, rD1, rI2, rD2;
       rM.init(); rI1.init(); rD1.init(); rI2.init(); rD2.init();
       int hitk = INT_MAX;  // ends-free: first diagonal that ends the alignment
+      // The ring reads of a chunk of diagonals (NC > 1), issued one chunk
+      // ahead: they read the slots of earlier steps, never the one this step
+      // writes, so their latency overlaps the current chunk's compute,
+      // extension and stores.
+      struct Reads {
+        int om_l, om_r, ei1, ed1, mx, o2l, o2r, ei2, ed2;
+      };
+      auto reads = [&](int k) {
+        Reads r{kOffsetNull, kOffsetNull, kOffsetNull, kOffsetNull, kOffsetNull,
+                kOffsetNull, kOffsetNull, kOffsetNull, kOffsetNull};
+        if (NC > 1 && k <= hi) {
+          r.om_l = rd(ring, mo1, k - 1);
+          r.om_r = rd(ring, mo1, k + 1);
+          r.ei1 = rd(ring, i1, k - 1);
+          r.ed1 = rd(ring, d1, k + 1);
+          r.mx = rd(ring, mx, k);
+          if constexpr (NC == 5) {
+            r.o2l = rd(ring, mo2, k - 1);
+            r.o2r = rd(ring, mo2, k + 1);
+            r.ei2 = rd(ring, i2, k - 1);
+            r.ed2 = rd(ring, d2, k + 1);
+          }
+        }
+        return r;
+      };
+      Reads next = reads(lo + tid);
       for (int kb = lo; kb <= hi; kb += BLOCK) {
         const int k = kb + tid;
+        const Reads cur_r = next;
+        if constexpr (NC > 1) next = reads(k + BLOCK);
         bool vM = false, vI1 = false, vD1 = false, vI2 = false, vD2 = false;
         bool hit = false;
         if constexpr (NC == 1) {
@@ -408,21 +436,21 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 1024 ? 8 : 1) wfa_forward_kern
             if (k == kend) sh_end[s & 1] = (int)to_ring(m, ring);
           }
         } else if (k <= hi) {
-          const int om_l = rd(ring, mo1, k - 1);
-          const int om_r = rd(ring, mo1, k + 1);
-          const int ei1 = rd(ring, i1, k - 1);
-          const int ed1 = rd(ring, d1, k + 1);
+          const int om_l = cur_r.om_l;
+          const int om_r = cur_r.om_r;
+          const int ei1 = cur_r.ei1;
+          const int ed1 = cur_r.ed1;
           const int ins1 = max(om_l, ei1) + 1;
           const int del1 = max(om_r, ed1);
-          int best = rd(ring, mx, k) + 1;
+          int best = cur_r.mx + 1;
           uint32_t src = kSrcX;
           uint32_t bits = (ei1 >= om_l ? kRecI1Ext : 0u) | (ed1 >= om_r ? kRecD1Ext : 0u);
           int ins2 = kOffsetNull, del2 = kOffsetNull;
           if constexpr (NC == 5) {
-            const int o2l = rd(ring, mo2, k - 1);
-            const int o2r = rd(ring, mo2, k + 1);
-            const int ei2 = rd(ring, i2, k - 1);
-            const int ed2 = rd(ring, d2, k + 1);
+            const int o2l = cur_r.o2l;
+            const int o2r = cur_r.o2r;
+            const int ei2 = cur_r.ei2;
+            const int ed2 = cur_r.ed2;
             ins2 = max(o2l, ei2) + 1;
             del2 = max(o2r, ed2);
             bits |= (ei2 >= o2l ? kRecI2Ext : 0u) | (ed2 >= o2r ? kRecD2Ext : 0u);
