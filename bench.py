@@ -435,7 +435,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="open-loop serving run length (0: skip)")
-    ap.add_argument("--serve-lanes", default="2,2,2", help="short,medium,long lanes of the request loop")
+    ap.add_argument("--serve-lanes", default="2,2,1", help="short,medium,long lanes of the request loop")
     ap.add_argument("--extras", type=int, default=1,
                     help="0: only the short-read batch (plain form, 1 kbp case and serving skipped)")
     ap.add_argument("--seq-pairs", type=int, default=500000, help="pairs of the .seq -> .alg run (0: skip)")

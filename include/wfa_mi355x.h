@@ -439,6 +439,13 @@ typedef struct {
   int max_batch_pairs;      /* requests admitted into one device batch */
   size_t lane_memory_bytes; /* per-lane budget: record arena and admission limit */
   double error_prior;       /* expected error rate of a request, for its footprint estimate (0.1) */
+  /* Shares of the device's CUs per class (0.125, 0.375, 0.5): the CUs are
+     split short | medium | long from the top (short and medium must not
+     overlap: short + medium <= 1; long takes the lowest CUs and may overlap
+     them), so one class's long-running kernels never hold another class's
+     batches off the CUs.  Each share is at least 0.125 (one of the eight
+     accelerator dies) and at most 1; wfa_server_create refuses others. */
+  double short_cu_share, medium_cu_share, long_cu_share;
 } wfa_server_config_t;
 
 typedef struct {

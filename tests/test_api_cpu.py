@@ -51,6 +51,15 @@ def test_unsupported_configurations_are_refused(kw):
     assert ops == b""
 
 
+@pytest.mark.parametrize("shares", [(1.0, 1.0, 1.0), (0.5, 0.625, 0.5), (0.0625, 0.375, 0.5), (0.125, 0.375, 1.5)])
+def test_server_refuses_cu_shares_that_do_not_split_the_device(shares):
+    """Short and medium take disjoint CU ranges of at least one die's share
+    each; wfa_server_create refuses other splits before starting any lane."""
+    kw = dict(zip(("short_cu_share", "medium_cu_share", "long_cu_share"), shares))
+    with pytest.raises(RuntimeError, match="wfa_server_create failed"):
+        wfa_amd.Server(**kw)
+
+
 def test_strerror_messages():
     L = wfa_amd.lib()
     assert L.wavefront_align_strerror(0) == b"[WFA] Alignment completed successfully"

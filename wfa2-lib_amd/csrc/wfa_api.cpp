@@ -43,6 +43,8 @@ bool hip_ok(hipError_t e, const char* what) {
   return false;
 }
 
+thread_local bool t_keep_retired = false;
+
 bool DevBuf::ensure(size_t bytes, size_t max_bytes) {
   if (bytes <= cap && p) return true;
   if (region) {
@@ -55,7 +57,8 @@ bool DevBuf::ensure(size_t bytes, size_t max_bytes) {
   // does not fit, the exact request is tried.
   size_t want = std::max<size_t>(std::max<size_t>(bytes, 256), p ? std::min(2 * cap, cap + (size_t(4) << 30)) : 0);
   want = std::max(bytes, std::min(want, max_bytes));
-  if (p) (void)hipFree(p);
+  if (p && t_keep_retired) retired.push_back(p);
+  else if (p) (void)hipFree(p);
   p = nullptr;
   cap = 0;
   if (hipMalloc(&p, want) != hipSuccess) {
@@ -73,6 +76,8 @@ bool DevBuf::ensure(size_t bytes, size_t max_bytes) {
 
 void DevBuf::release() {
   if (p && !region) (void)hipFree(p);
+  for (void* q : retired) (void)hipFree(q);
+  retired.clear();
   p = nullptr;
   cap = 0;
 }
@@ -108,7 +113,8 @@ PlanGroup& PlanGroup::operator=(PlanGroup&& o) noexcept {
 bool HostBuf::ensure(size_t bytes) {
   if (bytes <= cap && p) return true;
   const size_t want = std::max<size_t>(std::max<size_t>(bytes, 4096), p ? 2 * cap : 0);
-  if (p) (void)hipHostFree(p);
+  if (p && t_keep_retired) retired.push_back(p);
+  else if (p) (void)hipHostFree(p);
   p = nullptr;
   cap = 0;
   if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
@@ -121,6 +127,8 @@ bool HostBuf::ensure(size_t bytes) {
 
 void HostBuf::release() {
   if (p) (void)hipHostFree(p);
+  for (void* q : retired) (void)hipHostFree(q);
+  retired.clear();
   p = nullptr;
   cap = 0;
 }
@@ -205,10 +213,14 @@ bool engine_init(Engine* e) {
   if (!hip_ok(hipGetDeviceProperties(&prop, e->device), "hipGetDeviceProperties")) return false;
   e->num_cus = prop.multiProcessorCount;
   if (e->own_queue) {
-    std::vector<uint32_t> mask((e->num_cus + 31) / 32, 0xFFFFFFFFu);
-    if (e->num_cus % 32) mask.back() = (1u << (e->num_cus % 32)) - 1u;
+    const int n = e->num_cus;
+    int lo = std::max(0, std::min(n - 1, (int)(e->cu_lo * n)));
+    int hi = std::max(lo + 1, std::min(n, (int)(e->cu_hi * n + 0.999)));
+    std::vector<uint32_t> mask((n + 31) / 32, 0u);
+    for (int c = lo; c < hi; ++c) mask[c / 32] |= 1u << (c % 32);
     if (!hip_ok(hipExtStreamCreateWithCUMask(&e->stream, (uint32_t)mask.size(), mask.data()), "hipStreamCreate"))
       return false;
+    e->num_cus = hi - lo;
   } else if (!hip_ok(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "hipStreamCreate")) {
     return false;
   }
@@ -590,13 +602,12 @@ Trim maxtrim(const wavefront_penalties_t& p, const char* ops, int n) {
 }
 
 bool all_acgt(const wfa_pair_t* pairs, int n) {
-  static bool lut_init = false;
-  static bool lut[256];
-  if (!lut_init) {
-    for (int i = 0; i < 256; ++i) lut[i] = false;
-    lut['A'] = lut['C'] = lut['G'] = lut['T'] = true;
-    lut_init = true;
-  }
+  // built once, thread-safely (serving lanes call this concurrently)
+  static const struct Lut {
+    bool v[256];
+    Lut() : v() { v['A'] = v['C'] = v['G'] = v['T'] = true; }
+  } lut_table;
+  const bool* lut = lut_table.v;
   for (int i = 0; i < n; ++i) {
     const unsigned char* p = reinterpret_cast<const unsigned char*>(pairs[i].pattern);
     for (int j = 0; j < pairs[i].pattern_length; ++j) if (!lut[p[j]]) return false;

@@ -28,6 +28,14 @@ struct ScratchRegion {
   }
 };
 
+// Set on threads whose aligners run beside other aligners' kernels (the
+// serving lanes).  hipFree and hipHostFree wait for every stream of the
+// device: a short lane growing a buffer would wait for a long lane's
+// seconds-long BiWFA kernels.  On such threads a buffer that grows keeps its
+// old allocation until its owner releases it (at most about its current
+// size more, as buffers grow geometrically).
+extern thread_local bool t_keep_retired;
+
 // Grow-only device allocation, freed by its owner's destructor.  hipFree
 // waits for the whole device (other aligners' kernels included), so buffers
 // that are used again and again grow geometrically up to max_bytes and are
@@ -37,14 +45,17 @@ struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
   ScratchRegion* region = nullptr;
+  std::vector<void*> retired;  // outgrown allocations kept under t_keep_retired
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), cap(o.cap), region(o.region) { o.p = nullptr; o.cap = 0; o.region = nullptr; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), cap(o.cap), region(o.region), retired(std::move(o.retired)) {
+    o.p = nullptr; o.cap = 0; o.region = nullptr;
+  }
   DevBuf& operator=(DevBuf&& o) noexcept {
     if (this != &o) {
       release();
-      p = o.p; cap = o.cap; region = o.region;
+      p = o.p; cap = o.cap; region = o.region; retired = std::move(o.retired);
       o.p = nullptr; o.cap = 0; o.region = nullptr;
     }
     return *this;
@@ -65,6 +76,7 @@ struct DevView {
 struct HostBuf {
   void* p = nullptr;
   size_t cap = 0;
+  std::vector<void*> retired;  // outgrown allocations kept under t_keep_retired
   HostBuf() = default;
   HostBuf(const HostBuf&) = delete;
   HostBuf& operator=(const HostBuf&) = delete;
@@ -111,6 +123,10 @@ struct Engine {
   // few shared queues, on which kernels of different streams run in order;
   // a stream created with a CU mask (here: every CU) gets a new queue.
   bool own_queue = false;
+  // With own_queue, the stream's CU mask covers the CUs [cu_lo, cu_hi) x CU
+  // count (fractions); num_cus is then the count of those CUs, and the
+  // persistent launches size their grids to them.
+  double cu_lo = 0.0, cu_hi = 1.0;
   ::wfa_device_batch_t* scratch = nullptr;  // reused by wavefront_align / wavefront_align_batch
   // Plan groups no longer in use (their order buffers and events), handed
   // out again by take_group: retries and re-planning allocate nothing new.

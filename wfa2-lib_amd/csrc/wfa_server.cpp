@@ -117,6 +117,7 @@ struct wfa_server_t {
 };
 
 void wfa_server_t::lane_main(int cls) {
+  wfa_host::t_keep_retired = true;  // no hipFree while other lanes' kernels run
   wavefront_aligner_attr_t a = attr;
   a.memory_mode = cls == kLong ? wavefront_memory_ultralow : wavefront_memory_high;
   wavefront_aligner_t* wf = wavefront_aligner_new(&a);
@@ -131,6 +132,11 @@ void wfa_server_t::lane_main(int cls) {
   if (wfa_host::Engine* e = wfa_host::engine_of(wf)) {
     e->rec_cap = cfg.lane_memory_bytes;
     e->own_queue = true;
+    // CUs from the top: short | medium | long
+    const double s = cfg.short_cu_share, m = cfg.medium_cu_share, l = cfg.long_cu_share;
+    if (cls == kShort) { e->cu_lo = 1.0 - s; e->cu_hi = 1.0; }
+    else if (cls == kMedium) { e->cu_lo = 1.0 - s - m; e->cu_hi = 1.0 - s; }
+    else { e->cu_lo = 0.0; e->cu_hi = l; }
   }
   Queue& Q = queues[cls];
   std::vector<std::unique_ptr<Request>> batch;
@@ -187,14 +193,21 @@ void wfa_server_t::lane_main(int cls) {
 }
 
 extern "C" void wfa_server_config_default(wfa_server_config_t* c) {
-  // Two lanes per class: while one lane's batch is on the device, the other
-  // admits the requests that arrive meanwhile.  Every lane's stream has a
-  // hardware queue of its own (CU-masked stream), so the batches of
-  // different lanes overlap on the device.
+  // Two lanes for the short and medium classes: while one lane's batch is on
+  // the device, the other admits the requests that arrive meanwhile.  Every
+  // lane's stream has a hardware queue of its own (CU-masked stream), so the
+  // batches of different lanes overlap on the device.
   c->short_lanes = 2;
   c->medium_lanes = 2;
-  c->long_lanes = 2;
+  c->long_lanes = 1;
   c->error_prior = 0.1;
+  // The persistent kernels of a batch hold their CUs until the batch ends
+  // (seconds for 100 kbp BiWFA): without a split, the p99 of short requests
+  // was 1.5-2 s of waiting for CUs.  Short requests need few CUs; long ones
+  // the most (one CU per 100 kbp problem for seconds).
+  c->short_cu_share = 0.125;
+  c->medium_cu_share = 0.375;
+  c->long_cu_share = 0.5;
   c->short_max_length = 2048;
   c->long_min_length = 32768;
   c->max_batch_pairs = 8192;
@@ -210,6 +223,15 @@ extern "C" wfa_server_t* wfa_server_create(const wavefront_aligner_attr_t* attri
   s->attr = attributes ? *attributes : wavefront_aligner_attr_default;
   if (s->cfg.max_batch_pairs < 1) s->cfg.max_batch_pairs = 1;
   if (!(s->cfg.error_prior > 0 && s->cfg.error_prior <= 1)) s->cfg.error_prior = 0.1;
+  const double sh_s = s->cfg.short_cu_share, sh_m = s->cfg.medium_cu_share, sh_l = s->cfg.long_cu_share;
+  bool shares_ok = sh_s + sh_m <= 1.0 + 1e-9;
+  for (double sh : {sh_s, sh_m, sh_l}) shares_ok = shares_ok && sh >= 0.125 && sh <= 1.0;
+  if (!shares_ok) {
+    fprintf(stderr, "[WFA] wfa_server_create: CU shares (%g, %g, %g) must each lie in [0.125, 1] with short + "
+            "medium <= 1\n", sh_s, sh_m, sh_l);
+    delete s;
+    return nullptr;
+  }
   const int counts[kClasses] = {std::max(1, s->cfg.short_lanes), std::max(1, s->cfg.medium_lanes),
                                 std::max(1, s->cfg.long_lanes)};
   for (int c = 0; c < kClasses; ++c)

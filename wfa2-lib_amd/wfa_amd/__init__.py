@@ -137,7 +137,8 @@ class wfa_result_t(Structure):
 class wfa_server_config_t(Structure):
     _fields_ = [("short_lanes", c_int), ("medium_lanes", c_int), ("long_lanes", c_int),
                 ("short_max_length", c_int), ("long_min_length", c_int), ("max_batch_pairs", c_int),
-                ("lane_memory_bytes", ctypes.c_size_t), ("error_prior", c_double)]
+                ("lane_memory_bytes", ctypes.c_size_t), ("error_prior", c_double),
+                ("short_cu_share", c_double), ("medium_cu_share", c_double), ("long_cu_share", c_double)]
 
 
 class wfa_server_result_t(Structure):
