@@ -275,7 +275,7 @@ __device__ __forceinline__ int16_t to_ring(int v, int16_t*) {
 __device__ __forceinline__ int to_ring(int v, int*) { return v; }
 
 __device__ __forceinline__ bool in_bounds(int off, int k, int plen, int tlen) {
-  return (unsigned)off <= (unsigned)tlen && (unsigned)(off - k) <= (unsigned)plen;
+  return ((unsigned)off <= (unsigned)tlen) & ((unsigned)(off - k) <= (unsigned)plen);  // no branch
 }
 
 // 64-bit window of symbols starting at symbol `pos` (BITS bits per symbol,

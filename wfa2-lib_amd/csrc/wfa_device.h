@@ -74,7 +74,7 @@ constexpr uint32_t kRecI2Ext = 1u << 5;
 constexpr uint32_t kRecD2Ext = 1u << 6;
 constexpr uint32_t kRecStale = 1u << 7;
 __host__ __device__ inline unsigned long long rec_units_for(int width) {
-  return 1 + (unsigned long long)((width + 15) / 16);
+  return 1 + (unsigned long long)((unsigned)(width + 15) / 16u);  // width >= 0
 }
 enum RecSrc : uint32_t { kSrcX = 0, kSrcD2 = 1, kSrcD1 = 2, kSrcI2 = 3, kSrcI1 = 4 };
 constexpr uint32_t kNoBlock = 0xFFFFFFFFu;

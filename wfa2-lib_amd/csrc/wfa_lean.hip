@@ -34,37 +34,38 @@ constexpr int16_t kRingNull = -16384;  // to_ring(kOffsetNull)
 // written NULL before each step.  I1/D1 inputs hold raw values outside their
 // trimmed range, so a read checks the range (wavefront_compute_init_ends,
 // wavefront_compute.c:498-575, gives the reference the same NULLs).
+//
+// Every empty range is kept as (1, -1), the null wavefront's limits, so a
+// range is its own limits contribution whether or not it is empty.
 struct LeanSrc {
   int llo, lhi;   // limits contribution; valid iff llo <= lhi
   int clo, chi;   // reads clamp k into [clo, chi]
   int sb;         // ring element of diagonal 0
 };
-constexpr int kFarOff = 1 << 20;  // clamp bounds of an absent gap input (reads land on the NULL cell)
 
 // M input of a slot whose ring origin is org, with limits [lo, hi], read
 // through a clamp to [cl, ch] (cells holding NULL at both bounds).
 __device__ __forceinline__ LeanSrc lean_src_m(int cl, int ch, int lo, int hi, int base, int org) {
   LeanSrc r;
   const bool v = lo <= hi;
-  r.llo = v ? lo : 1;
-  r.lhi = v ? hi : -1;
+  r.llo = lo;
+  r.lhi = hi;
   r.clo = v ? cl : 0;
   r.chi = v ? ch : 0;
   r.sb = v ? base - org : 0;  // element 0 of the ring is a NULL cell
   return r;
 }
 
-// I1 or D1 input with trimmed range [lo, hi].
-__device__ __forceinline__ LeanSrc lean_src_g(int org, int lo, int hi, int base) {
-  LeanSrc r;
-  const bool v = lo <= hi;
-  r.llo = v ? lo : 1;
-  r.lhi = v ? hi : -1;
-  r.clo = v ? lo : kFarOff;
-  r.chi = v ? hi : -kFarOff;
-  r.sb = v ? base - org : kFarOff;  // absent: sb + clamp = 0, the NULL cell
-  return r;
+// Without a cut-off the ring element of diagonal 0 is kept per slot, chosen
+// when the slot is written (mb: M; gb: the I1 or D1 cells of the last
+// step): an empty range (1, -1) then reads element 0, the NULL cell, with
+// no selects per read.  M reads clamp to [lo - 1, hi + 1], i.e. [0, 0] when
+// empty (element mb = 0).  Gap reads clamp to [lo, hi] and take NULL unless
+// the clamp kept k; empty, the clamp gives -1 and gb = 1 reads element 0.
+__device__ __forceinline__ LeanSrc lean_src_m_kept(int lo, int hi, int mb) {
+  return LeanSrc{lo, hi, lo - 1, hi + 1, mb};
 }
+__device__ __forceinline__ LeanSrc lean_src_g_kept(int lo, int hi, int gb) { return LeanSrc{lo, hi, lo, hi, gb}; }
 
 __device__ __forceinline__ int lean_rd(const int16_t* ring, const LeanSrc& s, int k) {
   return (int)ring[s.sb + min(max(k, s.clo), s.chi)];
@@ -272,12 +273,16 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
       wave_sync();
     }
 
-    // Ring metadata (uniform): origin and trimmed ranges per slot.
+    // Ring metadata (uniform): origin and trimmed ranges per slot; without a
+    // cut-off, the ring elements of diagonal 0 (s_mb per slot, s_gb for the
+    // I1 and D1 cells of the last step; see lean_src_m_kept).
     int s_org[R], s_top[R], s_lo[R][NC], s_hi[R][NC];
+    int s_mb[R], s_gb[2] = {1, 1};
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       s_org[r] = 0;
       s_top[r] = 0;
+      s_mb[r] = 0;
 #pragma unroll
       for (int c = 0; c < NC; ++c) { s_lo[r][c] = 1; s_hi[r][c] = -1; }
     }
@@ -349,6 +354,7 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
         s_top[0] = hi0 + 1;
         s_lo[0][kM] = lo0;
         s_hi[0][kM] = hi0;
+        s_mb[0] = b;
         if (hitk != INT_MAX) {
           status = kDevCompleted;
           end_k = hitk;
@@ -379,6 +385,7 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
       s_lo[0][kM] = 0;
       s_hi[0][kM] = 0;
       const int b = mbase(0);
+      s_mb[0] = b + 1;
       if (lane < 3) ring[b + lane] = (lane == 1) ? to_ring(v0, ring) : kRingNull;
       if (kend == 0 && v0 >= tlen) status = kDevCompleted;
       wave_sync();
@@ -424,16 +431,18 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
         // Without a cut-off the M ranges are the trimmed ranges, whose
         // neighbouring cells hold NULL (a pad or a cell trimmed away), so the
         // clamp may use them in place of the computed range.
-        const LeanSrc mx = lean_src_m(CUT ? s_org[rx] : s_lo[rx][kM] - 1, CUT ? s_top[rx] : s_hi[rx][kM] + 1,
-                                      s_lo[rx][kM], s_hi[rx][kM], mbase(rx), s_org[rx]);
-        const LeanSrc mo = lean_src_m(CUT ? s_org[ro] : s_lo[ro][kM] - 1, CUT ? s_top[ro] : s_hi[ro][kM] + 1,
-                                      s_lo[ro][kM], s_hi[ro][kM], mbase(ro), s_org[ro]);
+        // With a cut-off every input is clamp-read over its computed range
+        // [org, top] (cells outside the trimmed range hold NULL).
+        const LeanSrc mx = CUT ? lean_src_m(s_org[rx], s_top[rx], s_lo[rx][kM], s_hi[rx][kM], mbase(rx), s_org[rx])
+                               : lean_src_m_kept(s_lo[rx][kM], s_hi[rx][kM], s_mb[rx]);
+        const LeanSrc mo = CUT ? lean_src_m(s_org[ro], s_top[ro], s_lo[ro][kM], s_hi[ro][kM], mbase(ro), s_org[ro])
+                               : lean_src_m_kept(s_lo[ro][kM], s_hi[ro][kM], s_mb[ro]);
         const LeanSrc i1 = CUT ? lean_src_m(s_org[re], s_top[re], s_lo[re][kI1], s_hi[re][kI1], gbase(kI1, false),
                                             s_org[re])
-                               : lean_src_g(s_org[re], s_lo[re][kI1], s_hi[re][kI1], gbase(kI1, false));
+                               : lean_src_g_kept(s_lo[re][kI1], s_hi[re][kI1], s_gb[0]);
         const LeanSrc d1 = CUT ? lean_src_m(s_org[re], s_top[re], s_lo[re][kD1], s_hi[re][kD1], gbase(kD1, false),
                                             s_org[re])
-                               : lean_src_g(s_org[re], s_lo[re][kD1], s_hi[re][kD1], gbase(kD1, false));
+                               : lean_src_g_kept(s_lo[re][kD1], s_hi[re][kD1], s_gb[1]);
         const bool vx = mx.llo <= mx.lhi, vo = mo.llo <= mo.lhi, vi = i1.llo <= i1.lhi, vd = d1.llo <= d1.lhi;
         if (!(vx || vo || vi || vd)) {  // null step (wavefront_compute_affine.c:236-243)
           if constexpr (CUT) {
@@ -442,6 +451,8 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
           }
 #pragma unroll
           for (int c = 0; c < NC; ++c) { s_lo[ph][c] = 1; s_hi[ph][c] = -1; }
+          s_mb[ph] = 0;
+          s_gb[0] = s_gb[1] = 1;
           continue;
         }
         if constexpr (CUT) nulls = 0;
@@ -474,9 +485,10 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
           }
           const uint32_t blk = chunk_cur;
           chunk_cur += units;
-          if (lane == 0) a.rec_arena[blk] = make_uint4((unsigned)s, (unsigned)lo, (unsigned)hi, prev_blk);
+          uint4* const hdr = args_now()->rec_arena + blk;
+          if (lane == 0) *hdr = make_uint4((unsigned)s, (unsigned)lo, (unsigned)hi, prev_blk);
           prev_blk = blk;
-          rec = reinterpret_cast<uint8_t*>(a.rec_arena + blk + 1);
+          rec = reinterpret_cast<uint8_t*>(hdr + 1);
         }
 
         // Outputs of this step go to slot ph, origin lo - 1.
@@ -521,7 +533,7 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
           // null-after-max, then the extension of the valid M cells: one
           // 16-symbol window for every lane, more only for lanes whose run
           // fills it (uniform branch).
-          const bool okM = act && in_bounds(best, k, plen, tlen);
+          const bool okM = act & in_bounds(best, k, plen, tlen);
           int h = okM ? best : 0;
           int rem = okM ? min(plen - (best - k), tlen - best) : 0;
           int pp = okM ? pb + best - k : 0, pt = okM ? tb + best : 0;
@@ -529,7 +541,7 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
             const uint32_t x = win16(seqP, pp) ^ win16(seqT, pt);
             const int n = min(x ? (int)(__builtin_ctz(x) >> 1) : 16, rem);
             h += n;
-            bool more = n == 16 && rem > 16;
+            bool more = (n == 16) & (rem > 16);
             if (__ballot(more)) {
               while (more) {
                 rem -= 16;
@@ -543,8 +555,8 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
             }
           }
           if constexpr (CUT) dl = okM ? end_distance(h, k, plen, tlen, wmash, mfactor) : kNullDistance;
-          const bool vI = act && has_i1 && in_bounds(ins1, k, plen, tlen);
-          const bool vD = act && has_d1 && in_bounds(del1, k, plen, tlen);
+          const bool vI = act & has_i1 & in_bounds(ins1, k, plen, tlen);
+          const bool vD = act & has_d1 & in_bounds(del1, k, plen, tlen);
           if (act) {
             oM[k] = to_ring(okM ? h : kOffsetNull, ring);
             if (has_i1) oI[k] = to_ring(ins1, ring);
@@ -558,7 +570,7 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
             if (fb) fin_off = __builtin_amdgcn_readlane(h, (int)__builtin_ctzll(fb));
             return fb;
           } else {
-            return __ballot(okM && k == kend && h >= tlen);
+            return __ballot(okM & (k == kend) & (h >= tlen));
           }
         };
         // Trimmed ranges (wavefront_compute.c:579-613) from the ballots;
@@ -634,6 +646,11 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
         s_lo[ph][kM] = mlo; s_hi[ph][kM] = mhi;
         s_lo[ph][kI1] = ilo; s_hi[ph][kI1] = ihi;
         s_lo[ph][kD1] = dlo; s_hi[ph][kD1] = dhi;
+        if constexpr (!CUT) {
+          s_mb[ph] = mlo <= mhi ? mbase(ph) - org : 0;
+          s_gb[0] = ilo <= ihi ? gbase(kI1, true) - org : 1;
+          s_gb[1] = dlo <= dhi ? gbase(kD1, true) - org : 1;
+        }
         wave_sync();  // ring writes of this step before the next step's reads
         // M[k_end] reached the end of the text: a valid cell at k_end lies in
         // the trimmed range, so the lane that computed it decides.
