@@ -1527,7 +1527,9 @@ int run_trials(wavefront_aligner_t* wf, Engine* e, const JobSet& js, int records
         fprintf(stderr, "[WFA] autotune family %d length bin %d error bin %d: candidate %d, %zu pairs, %.3f ms, "
                 "%.3e ms per diagonal-step\n", trials[v].base.family, trials[v].base.lbin, eb, trials[v].candidate,
                 trials[v].group.ids.size(), costs[v].ms, c);
-      if (c < best_cost) {
+      // lean family: a wider ring (fewer waves per CU) only if it is
+      // measurably faster, not on a tie within the slices' noise
+      if (trials[v].base.family == wfa::kLean && v > t ? c * 1.03 < best_cost : c < best_cost) {
         best_cost = c;
         best = trials[v].candidate;
       }
