@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -109,6 +110,12 @@ bool ensure_reverse(Engine* e, wfa_device_batch_t* b) {
 
 int bp_block(int size) { return size <= 512 ? 64 : (size <= 8192 ? 256 : 1024); }
 
+// WFA_PAIR_SEARCH=0 turns paired-workgroup breakpoint searches off.
+bool pair_searches_enabled() {
+  const char* v = getenv("WFA_PAIR_SEARCH");
+  return !v || v[0] != '0';
+}
+
 // LDS for the four sequences of a breakpoint problem (2-bit sequences of a
 // 100 kbp pair take 100 KB of gfx950's 160 KB per CU).
 constexpr size_t kBreakpointSeqLdsBytes = 120 * 1024;
@@ -129,83 +136,134 @@ bool run_breakpoints(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, 
     groups[bl == 64 ? 0 : (bl == 256 ? 1 : 2)].push_back(i);
   }
   hipStream_t st = e->stream;
+  const wfa::HeurArgs heur = heuristic_of(wf);
   for (int gi = 0; gi < 3; ++gi) {
-    const std::vector<int>& ids = groups[gi];
+    std::vector<int> ids = groups[gi];
     if (ids.empty()) continue;
     const int block = gi == 0 ? 64 : (gi == 1 ? 256 : 1024);
-    std::vector<wfa::BiProblem> probs(ids.size());
-    int wmax = 0;
-    for (size_t j = 0; j < ids.size(); ++j) {
-      probs[j] = nodes[ids[j]].prob;
-      wmax = std::max(wmax, (probs[j].pe - probs[j].pb) + (probs[j].te - probs[j].tb));
+    // One launch over the sub-problems `sel` (indices into nodes); paired:
+    // two workgroups per sub-problem.  Outcomes go to out[].
+    auto launch = [&](const std::vector<int>& sel, bool paired) -> bool {
+      std::vector<wfa::BiProblem> probs(sel.size());
+      int wmax = 0;
+      for (size_t j = 0; j < sel.size(); ++j) {
+        probs[j] = nodes[sel[j]].prob;
+        wmax = std::max(wmax, (probs[j].pe - probs[j].pb) + (probs[j].te - probs[j].tb));
+      }
+      const int wcap = wmax + 8;
+      // The four sequences of the launch's longest problem in LDS when they fit.
+      const int spw = 64 / b->bits;
+      int lw = 0;
+      for (const wfa::BiProblem& p : probs) lw = std::max(lw, std::max(p.pe - p.pb, p.te - p.tb) / spw + 3);
+      if (wfa::breakpoint_lds_bytes(lw) > kBreakpointSeqLdsBytes) lw = 0;
+      int grid = 0;
+      wfa::Penalties lpen = pen;
+      if (paired) {
+        lpen.nslots = pen.nslots + 1;  // room for the speculative reverse step
+        grid = 2 * (int)probs.size();
+      } else {
+        int per_cu = wfa::breakpoint_max_blocks_per_cu(ncomp, b->bits, block, heur.cut() != 0,
+                                                       wfa::breakpoint_lds_bytes(lw));
+        if (per_cu <= 0 && lw > 0) {
+          lw = 0;
+          per_cu = wfa::breakpoint_max_blocks_per_cu(ncomp, b->bits, block, heur.cut() != 0);
+        }
+        if (per_cu <= 0) per_cu = 1;
+        grid = (int)std::min<int64_t>((int64_t)probs.size(), (int64_t)per_cu * e->num_cus);
+      }
+      const long long stride = 2LL * lpen.nslots * ncomp * wcap;
+      const size_t ring_units = paired ? probs.size() : (size_t)grid;  // teams or workgroups
+      DevBuf& d_probs = e->bi.probs;
+      DevBuf& d_out = e->bi.out;
+      const size_t hand_ints = wfa::breakpoint_pair_handover_ints(ncomp);
+      if (!d_probs.ensure(probs.size() * sizeof(wfa::BiProblem)) ||
+          !d_out.ensure(probs.size() * sizeof(wfa::BiOutcome)) ||
+          !e->ring.ensure((size_t)stride * ring_units * sizeof(int)) ||
+          (paired && (!e->bi.mail.ensure(probs.size() * 4 * sizeof(unsigned long long)) ||
+                      !e->bi.hand.ensure(probs.size() * hand_ints * sizeof(int))))) {
+        fprintf(stderr, "[WFA] cannot allocate BiWFA breakpoint buffers\n");
+        return false;
+      }
+      wfa::Counters* c = counters_of(e);
+      wfa::BiArgs a{};
+      a.n = (int)probs.size();
+      a.probs = d_probs.as<wfa::BiProblem>();
+      a.words = b->words.as<uint64_t>();
+      a.rev_words = b->rev_words.as<uint64_t>();
+      a.fwd_ppos = b->d_patw.as<int64_t>();
+      a.fwd_tpos = b->d_txtw.as<int64_t>();
+      a.rev_ppos = b->d_patw.as<int64_t>();
+      a.rev_tpos = b->d_txtw.as<int64_t>();
+      a.plen_total = b->d_plen.as<int>();
+      a.tlen_total = b->d_tlen.as<int>();
+      a.out = d_out.as<wfa::BiOutcome>();
+      a.ring = e->ring.as<int>();
+      a.ring_stride = stride;
+      a.wcap = wcap;
+      a.queue = &c->queue;
+      a.dsteps_total = &c->dsteps;
+      a.max_steps = wf->system.max_alignment_steps > 0 ? wf->system.max_alignment_steps : INT_MAX;
+      a.lmax_words = lw;
+      a.heur = heur;  // the breakpoint aligners inherit the heuristic (wavefront_bialigner.c:53)
+      a.pen = lpen;
+      a.mail = paired ? e->bi.mail.as<unsigned long long>() : nullptr;
+      a.handover = paired ? e->bi.hand.as<int>() : nullptr;
+      for (auto& ev : e->bi.ev)
+        if (!ev && !hip_ok(hipEventCreate(&ev), "hipEventCreate")) return false;
+      hipEvent_t e0 = e->bi.ev[0], e1 = e->bi.ev[1];
+      std::vector<wfa::BiOutcome> res(probs.size());
+      wfa::Counters host_c{};
+      bool ok = hip_ok(hipMemcpyAsync(d_probs.p, probs.data(), probs.size() * sizeof(wfa::BiProblem),
+                                      hipMemcpyHostToDevice, st), "bi upload") &&
+                hip_ok(wfa::launch_reset_counters(c, 2, st), "reset") &&
+                (!paired || hip_ok(hipMemsetAsync(a.mail, 0, probs.size() * 4 * sizeof(unsigned long long), st),
+                                   "mailboxes"));
+      (void)hipEventRecord(e0, st);
+      ok = ok && hip_ok(paired ? wfa::launch_breakpoint_pair(a, ncomp, b->bits, block, st)
+                               : wfa::launch_breakpoint(a, ncomp, b->bits, block, grid, st),
+                        "breakpoint launch");
+      (void)hipEventRecord(e1, st);
+      ok = ok && hip_ok(hipMemcpyAsync(res.data(), d_out.p, res.size() * sizeof(wfa::BiOutcome),
+                                       hipMemcpyDeviceToHost, st), "bi download") &&
+           hip_ok(hipMemcpyAsync(&host_c, e->counters.p, sizeof(host_c), hipMemcpyDeviceToHost, st), "counters") &&
+           hip_ok(hipStreamSynchronize(st), "breakpoint");
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (!ok) return false;
+      tot->forward_ms += ms;
+      tot->launches += 1;
+      tot->dsteps += host_c.dsteps;
+      for (size_t j = 0; j < sel.size(); ++j) out[sel[j]] = res[j];
+      return true;
+    };
+    // Paired workgroups when every team fits on the device at once beside
+    // nothing else of this launch (large sub-problems, fewer than half the
+    // workgroups the CUs hold); not with a heuristic cut-off, or where the
+    // extra ring slot would pass the 64 slots of the slot metadata.
+    bool paired = false;
+    if (block == 1024 && heur.cut() == 0 && pen.nslots + 1 <= 64 && pair_searches_enabled()) {
+      const int spw = 64 / b->bits;
+      int lw = 0;
+      for (int i : ids) {
+        const wfa::BiProblem& p = nodes[i].prob;
+        lw = std::max(lw, std::max(p.pe - p.pb, p.te - p.tb) / spw + 3);
+      }
+      if (wfa::breakpoint_lds_bytes(lw) > kBreakpointSeqLdsBytes) lw = 0;
+      const int per_cu = wfa::breakpoint_pair_max_blocks_per_cu(ncomp, b->bits, block, wfa::breakpoint_lds_bytes(lw));
+      paired = per_cu > 0 && 2 * (int64_t)ids.size() <= (int64_t)per_cu * e->num_cus;
     }
-    const int wcap = wmax + 8;
-    const wfa::HeurArgs heur = heuristic_of(wf);
-    // The four sequences of the launch's longest problem in LDS when they fit.
-    const int spw = 64 / b->bits;
-    int lw = 0;
-    for (const wfa::BiProblem& p : probs) lw = std::max(lw, std::max(p.pe - p.pb, p.te - p.tb) / spw + 3);
-    if (wfa::breakpoint_lds_bytes(lw) > kBreakpointSeqLdsBytes) lw = 0;
-    int per_cu = wfa::breakpoint_max_blocks_per_cu(ncomp, b->bits, block, heur.cut() != 0,
-                                                   wfa::breakpoint_lds_bytes(lw));
-    if (per_cu <= 0 && lw > 0) {
-      lw = 0;
-      per_cu = wfa::breakpoint_max_blocks_per_cu(ncomp, b->bits, block, heur.cut() != 0);
+    if (!launch(ids, paired)) return false;
+    if (paired) {
+      // a team whose partner did not answer in time: search again on one workgroup
+      std::vector<int> again;
+      for (int i : ids)
+        if (out[i].status == wfa::kBiPairAbort) again.push_back(i);
+      if (!again.empty()) {
+        fprintf(stderr, "[WFA] %zu paired breakpoint searches timed out; searching them again on one workgroup\n",
+                again.size());
+        if (!launch(again, false)) return false;
+      }
     }
-    if (per_cu <= 0) per_cu = 1;
-    const int grid = (int)std::min<int64_t>((int64_t)ids.size(), (int64_t)per_cu * e->num_cus);
-    const long long stride = 2LL * pen.nslots * ncomp * wcap;
-    DevBuf& d_probs = e->bi.probs;
-    DevBuf& d_out = e->bi.out;
-    if (!d_probs.ensure(probs.size() * sizeof(wfa::BiProblem)) || !d_out.ensure(probs.size() * sizeof(wfa::BiOutcome)) ||
-        !e->ring.ensure((size_t)stride * grid * sizeof(int))) {
-      fprintf(stderr, "[WFA] cannot allocate BiWFA breakpoint buffers\n");
-      return false;
-    }
-    wfa::Counters* c = counters_of(e);
-    wfa::BiArgs a{};
-    a.n = (int)probs.size();
-    a.probs = d_probs.as<wfa::BiProblem>();
-    a.words = b->words.as<uint64_t>();
-    a.rev_words = b->rev_words.as<uint64_t>();
-    a.fwd_ppos = b->d_patw.as<int64_t>();
-    a.fwd_tpos = b->d_txtw.as<int64_t>();
-    a.rev_ppos = b->d_patw.as<int64_t>();
-    a.rev_tpos = b->d_txtw.as<int64_t>();
-    a.plen_total = b->d_plen.as<int>();
-    a.tlen_total = b->d_tlen.as<int>();
-    a.out = d_out.as<wfa::BiOutcome>();
-    a.ring = e->ring.as<int>();
-    a.ring_stride = stride;
-    a.wcap = wcap;
-    a.queue = &c->queue;
-    a.dsteps_total = &c->dsteps;
-    a.max_steps = wf->system.max_alignment_steps > 0 ? wf->system.max_alignment_steps : INT_MAX;
-    a.lmax_words = lw;
-    a.heur = heur;  // the breakpoint aligners inherit the heuristic (wavefront_bialigner.c:53)
-    a.pen = pen;
-    for (auto& ev : e->bi.ev)
-      if (!ev && !hip_ok(hipEventCreate(&ev), "hipEventCreate")) return false;
-    hipEvent_t e0 = e->bi.ev[0], e1 = e->bi.ev[1];
-    std::vector<wfa::BiOutcome> res(probs.size());
-    wfa::Counters host_c{};
-    bool ok = hip_ok(hipMemcpyAsync(d_probs.p, probs.data(), probs.size() * sizeof(wfa::BiProblem),
-                                    hipMemcpyHostToDevice, st), "bi upload") &&
-              hip_ok(wfa::launch_reset_counters(c, 2, st), "reset");
-    (void)hipEventRecord(e0, st);
-    ok = ok && hip_ok(wfa::launch_breakpoint(a, ncomp, b->bits, block, grid, st), "breakpoint launch");
-    (void)hipEventRecord(e1, st);
-    ok = ok && hip_ok(hipMemcpyAsync(res.data(), d_out.p, res.size() * sizeof(wfa::BiOutcome), hipMemcpyDeviceToHost, st),
-                      "bi download") &&
-         hip_ok(hipMemcpyAsync(&host_c, e->counters.p, sizeof(host_c), hipMemcpyDeviceToHost, st), "counters") &&
-         hip_ok(hipStreamSynchronize(st), "breakpoint");
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    if (!ok) return false;
-    tot->forward_ms += ms;
-    tot->launches += 1;
-    tot->dsteps += host_c.dsteps;
-    for (size_t j = 0; j < ids.size(); ++j) out[ids[j]] = res[j];
   }
   return true;
 }

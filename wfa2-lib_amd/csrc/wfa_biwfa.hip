@@ -459,8 +459,10 @@ __device__ void overlap(const Side<NC, CUT>& a0, const Side<NC, CUT>& a1, int s0
   }
 }
 
+// own_ring: this workgroup writes the side's score-0 cell (a paired
+// workgroup sets up its partner's side in registers only).
 template <int NC, bool CUT>
-__device__ void side_init(Side<NC, CUT>& sd, int cb, int ce, int wcap, int steps) {
+__device__ void side_init(Side<NC, CUT>& sd, int cb, int ce, int wcap, int steps, bool own_ring = true) {
   const int lane = threadIdx.x & 63;
   sd.meta.tag = (lane == 0) ? 0 : -1;
   sd.meta.alo = 0;
@@ -479,7 +481,7 @@ __device__ void side_init(Side<NC, CUT>& sd, int cb, int ce, int wcap, int steps
   sd.cb = cb;
   sd.ce = ce;
   sd.end_score = -1;
-  if (threadIdx.x == 0) sd.ring[(0 * NC + cb) * wcap] = 0;
+  if (own_ring && threadIdx.x == 0) sd.ring[(0 * NC + cb) * wcap] = 0;
 }
 
 // Extension of the score-0 wavefront (only M can extend).
@@ -503,9 +505,113 @@ __device__ int side_extend0(Side<NC, CUT>& sd, int plen, int tlen, int wcap, con
   return 0;
 }
 
+// Sequences and rings of sub-problem pr for both sides (ring: this
+// workgroup's or team's region, forward then reverse); with a.lmax_words > 0
+// the four sequences (forward and reversed pattern and text) are copied into
+// LDS, so extension windows cost an LDS round trip.  The caller separates the
+// previous sub-problem's use of the LDS with a barrier.
+template <int NC, int BLOCK, int BITS, bool CUT>
+__device__ __forceinline__ void setup_sides(const BiArgs& a, const BiProblem& pr, int* ring, int ring_side,
+                                            unsigned char* smem, Side<NC, CUT>& f, Side<NC, CUT>& r) {
+  constexpr int SPW = 64 / BITS;
+  const int pair = pr.pair;
+  const int plen = pr.pe - pr.pb, tlen = pr.te - pr.tb;
+  const int plen_all = a.plen_total[pair], tlen_all = a.tlen_total[pair];
+  f.ring = ring;
+  r.ring = ring + ring_side;
+  const long long pp = a.fwd_ppos[pair] + pr.pb, tp = a.fwd_tpos[pair] + pr.tb;
+  f.P = a.words + pp / SPW; f.pb = (int)(pp % SPW);
+  f.T = a.words + tp / SPW; f.tb = (int)(tp % SPW);
+  const long long rp = a.rev_ppos[pair] + (plen_all - pr.pe), rt = a.rev_tpos[pair] + (tlen_all - pr.te);
+  r.P = a.rev_words + rp / SPW; r.pb = (int)(rp % SPW);
+  r.T = a.rev_words + rt / SPW; r.tb = (int)(rt % SPW);
+  if (a.lmax_words > 0) {
+    const int nwp = (f.pb + plen) / SPW + 2, nwt = (f.tb + tlen) / SPW + 2;
+    const int nrp = (r.pb + plen) / SPW + 2, nrt = (r.tb + tlen) / SPW + 2;
+    uint64_t* s0 = reinterpret_cast<uint64_t*>(smem);
+    uint64_t* s1 = s0 + a.lmax_words;
+    uint64_t* s2 = s1 + a.lmax_words;
+    uint64_t* s3 = s2 + a.lmax_words;
+    for (int i = threadIdx.x; i < nwp; i += BLOCK) s0[i] = f.P[i];
+    for (int i = threadIdx.x; i < nwt; i += BLOCK) s1[i] = f.T[i];
+    for (int i = threadIdx.x; i < nrp; i += BLOCK) s2[i] = r.P[i];
+    for (int i = threadIdx.x; i < nrt; i += BLOCK) s3[i] = r.T[i];
+    f.P = s0;
+    f.T = s1;
+    r.P = s2;
+    r.T = s3;
+  }
+}
+
+// Phase 2 of wavefront_bialign_find_breakpoint (wavefront_bialign.c:
+// 1046-1080): overlap scans against the other side's last scores, one step
+// of the side scanned against, alternating from where phase 1 stopped.
+// Returns the search status (kBiOk when the bound stops it).
+template <int NC, int BLOCK, int BITS, bool CUT>
+__device__ int find_phase2(Side<NC, CUT>& f, Side<NC, CUT>& r, int& sf, int& sr, bool last_forward, int plen,
+                           int tlen, const Penalties& pen, const BiArgs& a, int wcap, int s_cap, Bp& bp,
+                           long long* dsteps, int (*sh_red)[2 * NC + 2], BlockRed<BLOCK>& red) {
+  const int scope = pen.scope;
+  // wavefront_bialign_overlap_gopen_adjust (wavefront_bialign.c:960-973)
+  const int gap_open = (NC == 5) ? max(pen.o1, pen.o2) : (NC == 3 ? pen.o1 : 0);
+  int ak, rc;
+  while (true) {
+    if (last_forward) {
+      const int min_sr = (sr > scope - 1) ? sr - (scope - 1) : 0;
+      if (sf + min_sr - gap_open >= bp.score) break;
+      overlap<NC, BLOCK, CUT>(f, r, sf, sr, true, plen, tlen, pen, wcap, bp, red);
+      ++sr;
+      rc = side_step<NC, BLOCK, BITS>(r, sr, plen, tlen, pen, a.heur, wcap, false, &ak, dsteps, sh_red, red);
+      if (rc < 0) return kDevRingOverflow;
+      if (rc == 2) return kDevUnreachable;
+      if (rc > 0) return kBiEndReached;
+    }
+    const int min_sf = (sf > scope - 1) ? sf - (scope - 1) : 0;
+    if (min_sf + sr - gap_open >= bp.score) break;
+    overlap<NC, BLOCK, CUT>(r, f, sr, sf, false, plen, tlen, pen, wcap, bp, red);
+    ++sf;
+    rc = side_step<NC, BLOCK, BITS>(f, sf, plen, tlen, pen, a.heur, wcap, false, &ak, dsteps, sh_red, red);
+    if (rc < 0) return kDevRingOverflow;
+    if (rc == 2) return kDevUnreachable;
+    if (rc > 0) return kBiEndReached;
+    if (sr + sf >= a.max_steps) return kDevMaxSteps;
+    // both sides emptied and no breakpoint yet: nothing changes until the step limit
+    if (f.dead && r.dead && bp.score == INT_MAX) return kDevMaxSteps;
+    if (sf > s_cap) return kDevBadTrace;
+    last_forward = true;
+  }
+  return kBiOk;
+}
+
+template <int NC, bool CUT>
+__device__ __forceinline__ void write_outcome(BiOutcome* out, int status, const Bp& bp, int fwd_end, int rev_end,
+                                              long long dsteps) {
+  BiOutcome o;
+  o.status = status;
+  o.score = bp.score;
+  o.score_forward = bp.score_forward;
+  o.score_reverse = bp.score_reverse;
+  o.k_forward = bp.k_forward;
+  o.k_reverse = bp.k_reverse;
+  o.offset_forward = bp.offset_forward;
+  o.offset_reverse = bp.offset_reverse;
+  o.component = bp.component;
+  o.fwd_end_score = fwd_end;
+  o.rev_end_score = rev_end;
+  o.pad = 0;
+  o.dsteps = dsteps;
+  *out = o;
+}
+
+__device__ __forceinline__ void bp_clear(Bp& bp) {
+  bp.score = INT_MAX;
+  bp.score_forward = bp.score_reverse = bp.k_forward = bp.k_reverse = 0;
+  bp.offset_forward = bp.offset_reverse = 0;
+  bp.component = kM;
+}
+
 template <int NC, int BLOCK, int BITS, bool CUT>
 __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
-  constexpr int SPW = 64 / BITS;
   extern __shared__ __align__(16) unsigned char bp_smem[];
   __shared__ int sh_red[BLOCK / 64][2 * NC + 2];
   __shared__ int sh_min[BLOCK / 64 > 0 ? BLOCK / 64 : 1];
@@ -521,47 +627,16 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
     __syncthreads();
     if (q >= a.n) break;
     const BiProblem pr = a.probs[q];
-    const int pair = pr.pair;
     const int plen = pr.pe - pr.pb, tlen = pr.te - pr.tb;
-    const int plen_all = a.plen_total[pair], tlen_all = a.tlen_total[pair];
     Side<NC, CUT> f, r;
-    f.ring = a.ring + (long long)blockIdx.x * a.ring_stride;
-    r.ring = f.ring + (long long)pen.nslots * NC * wcap;
     __syncthreads();  // the previous sub-problem is done with the rings and the LDS sequences
-    {
-      const long long pp = a.fwd_ppos[pair] + pr.pb, tp = a.fwd_tpos[pair] + pr.tb;
-      f.P = a.words + pp / SPW; f.pb = (int)(pp % SPW);
-      f.T = a.words + tp / SPW; f.tb = (int)(tp % SPW);
-      const long long rp = a.rev_ppos[pair] + (plen_all - pr.pe), rt = a.rev_tpos[pair] + (tlen_all - pr.te);
-      r.P = a.rev_words + rp / SPW; r.pb = (int)(rp % SPW);
-      r.T = a.rev_words + rt / SPW; r.tb = (int)(rt % SPW);
-      if (a.lmax_words > 0) {
-        // The four sequences in LDS (the host gave room for the longest
-        // problem of the launch): extension windows cost an LDS round trip.
-        const int nwp = (f.pb + plen) / SPW + 2, nwt = (f.tb + tlen) / SPW + 2;
-        const int nrp = (r.pb + plen) / SPW + 2, nrt = (r.tb + tlen) / SPW + 2;
-        uint64_t* s0 = reinterpret_cast<uint64_t*>(bp_smem);
-        uint64_t* s1 = s0 + a.lmax_words;
-        uint64_t* s2 = s1 + a.lmax_words;
-        uint64_t* s3 = s2 + a.lmax_words;
-        for (int i = threadIdx.x; i < nwp; i += BLOCK) s0[i] = f.P[i];
-        for (int i = threadIdx.x; i < nwt; i += BLOCK) s1[i] = f.T[i];
-        for (int i = threadIdx.x; i < nrp; i += BLOCK) s2[i] = r.P[i];
-        for (int i = threadIdx.x; i < nrt; i += BLOCK) s3[i] = r.T[i];
-        f.P = s0;
-        f.T = s1;
-        r.P = s2;
-        r.T = s3;
-      }
-    }
+    setup_sides<NC, BLOCK, BITS>(a, pr, a.ring + (long long)blockIdx.x * a.ring_stride,
+                                 pen.nslots * NC * wcap, bp_smem, f, r);
     side_init(f, pr.comp_begin, pr.comp_end, wcap, a.heur.steps);
     side_init(r, pr.comp_end, pr.comp_begin, wcap, a.heur.steps);
     __syncthreads();
     Bp bp;
-    bp.score = INT_MAX;
-    bp.score_forward = bp.score_reverse = bp.k_forward = bp.k_reverse = 0;
-    bp.offset_forward = bp.offset_reverse = 0;
-    bp.component = kM;
+    bp_clear(bp);
     int status = kBiOk;
     long long dsteps = 0;
     const int s_cap = (pen.x + pen.o1 + pen.e1 + (NC == 5 ? pen.o2 + pen.e2 : 0) + 2) * (plen + tlen + 2);
@@ -594,64 +669,221 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
       if (f.dead && r.dead) { status = kDevMaxSteps; goto done; }
       if (sf > s_cap) { status = kDevBadTrace; goto done; }
     }
-    // phase 2 (:1046-1080)
-    {
-      const int scope = pen.scope;
-      // wavefront_bialign_overlap_gopen_adjust (wavefront_bialign.c:960-973)
-      const int gap_open = (NC == 5) ? max(pen.o1, pen.o2) : (NC == 3 ? pen.o1 : 0);
-      while (true) {
-        if (last_forward) {
-          const int min_sr = (sr > scope - 1) ? sr - (scope - 1) : 0;
-          if (sf + min_sr - gap_open >= bp.score) break;
-          overlap<NC, BLOCK, CUT>(f, r, sf, sr, true, plen, tlen, pen, wcap, bp, red);
-          ++sr;
-          rc = side_step<NC, BLOCK, BITS>(r, sr, plen, tlen, pen, a.heur, wcap, false, &ak, &dsteps, sh_red, red);
-          if (rc < 0) { status = kDevRingOverflow; goto done; }
-          if (rc == 2) { status = kDevUnreachable; goto done; }
-          if (rc > 0) { status = kBiEndReached; goto done; }
-        }
-        const int min_sf = (sf > scope - 1) ? sf - (scope - 1) : 0;
-        if (min_sf + sr - gap_open >= bp.score) break;
-        overlap<NC, BLOCK, CUT>(r, f, sr, sf, false, plen, tlen, pen, wcap, bp, red);
-        ++sf;
-        rc = side_step<NC, BLOCK, BITS>(f, sf, plen, tlen, pen, a.heur, wcap, false, &ak, &dsteps, sh_red, red);
-        if (rc < 0) { status = kDevRingOverflow; goto done; }
-        if (rc == 2) { status = kDevUnreachable; goto done; }
-        if (rc > 0) { status = kBiEndReached; goto done; }
-        if (sr + sf >= a.max_steps) { status = kDevMaxSteps; goto done; }
-        // both sides emptied and no breakpoint yet: nothing changes until the step limit
-        if (f.dead && r.dead && bp.score == INT_MAX) { status = kDevMaxSteps; goto done; }
-        if (sf > s_cap) { status = kDevBadTrace; goto done; }
-        last_forward = true;
-      }
-    }
+    status = find_phase2<NC, BLOCK, BITS>(f, r, sf, sr, last_forward, plen, tlen, pen, a, wcap, s_cap, bp, &dsteps,
+                                          sh_red, red);
   done:
-    if (threadIdx.x == 0) {
-      BiOutcome o;
-      o.status = status;
-      o.score = bp.score;
-      o.score_forward = bp.score_forward;
-      o.score_reverse = bp.score_reverse;
-      o.k_forward = bp.k_forward;
-      o.k_reverse = bp.k_reverse;
-      o.offset_forward = bp.offset_forward;
-      o.offset_reverse = bp.offset_reverse;
-      o.component = bp.component;
-      o.fwd_end_score = f.end_score;
-      o.rev_end_score = r.end_score;
-      o.pad = 0;
-      o.dsteps = dsteps;
-      a.out[q] = o;
-    }
+    if (threadIdx.x == 0) write_outcome<NC, CUT>(a.out + q, status, bp, f.end_score, r.end_score, dsteps);
     wg_dsteps += dsteps;
     __syncthreads();
   }
   if (threadIdx.x == 0 && wg_dsteps) atomicAdd(a.dsteps_total, (unsigned long long)wg_dsteps);
 }
 
+// ---------------------------------------------------------------------------
+// Paired workgroups: the forward and the reverse side of one large
+// sub-problem on two workgroups (a team: blocks 2t and 2t + 1, one
+// sub-problem per team), for launches with fewer sub-problems than half the
+// workgroups the CUs hold (the top levels of small batches, a serving lane's
+// long requests).  Phase 1 advances both sides at once, one score each per
+// round: round i computes forward score i and, speculatively, reverse score
+// i, then the two exchange their furthest antidiagonals and step statuses
+// and both apply the reference's order to the pair (forward step, its end
+// test and the meeting test, then the reverse step and its tests).  When the
+// meeting test stops phase 1 after the forward step, the reverse workgroup
+// takes its speculative step back (its ring has one slot more than the score
+// scope needs, so the step overwrote nothing phase 2 reads).  The reverse
+// side's state then goes to the forward workgroup, which runs phase 2 alone.
+//
+// Exchanges use agent-scope atomics on a per-team mailbox (one 64-bit word
+// per side and round parity: round number, step status, antidiagonal); the
+// hand-over of the reverse side's rings and slot metadata is ordered by a
+// device-scope release fence before its mailbox word and an acquire fence
+// after it.  Every wait is bounded by kPairWaitTicks of the 100 MHz device
+// clock; a team whose partner does not answer in time reports kBiPairAbort
+// and the host searches that sub-problem again on one workgroup.
+constexpr unsigned long long kPairWaitTicks = 400000000ull;  // 4 s
+
+// Handover record of the reverse side: per lane of wave 0, its slot's
+// metadata, then the side's scalars.
+template <int NC>
+struct PairHandover {
+  static constexpr int kLaneInts = 3 + 2 * NC;
+  static constexpr int kInts = 64 * kLaneInts + 8;
+};
+
+// Publishes `payload` for exchange `seq` and waits for the partner's word of
+// the same exchange (parity slots: a partner can be at most one exchange
+// ahead).  Returns false on timeout.  Every thread of the workgroup calls it.
+__device__ __forceinline__ bool pair_exchange(unsigned long long* box, int role, unsigned seq, unsigned payload,
+                                              unsigned* got, unsigned long long* sh) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long* mine = box + 2 * role + (seq & 1u);
+    unsigned long long* other = box + 2 * (role ^ 1) + (seq & 1u);
+    __hip_atomic_store(mine, ((unsigned long long)seq << 32) | payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+    unsigned long long w = 0;
+    bool ok = false;
+    for (;;) {
+      w = __hip_atomic_load(other, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if ((unsigned)(w >> 32) == seq) { ok = true; break; }
+      if (__builtin_amdgcn_s_memrealtime() - t0 > kPairWaitTicks) break;
+      __builtin_amdgcn_s_sleep(2);
+    }
+    *sh = ok ? (w & 0xFFFFFFFFull) : ~0ull;
+  }
+  __syncthreads();
+  const unsigned long long w = *sh;
+  __syncthreads();  // *sh is reused by the next exchange
+  if (w == ~0ull) return false;
+  *got = (unsigned)w;
+  return true;
+}
+
+// payload: step status + 1 in the top 4 bits, antidiagonal below (< 2^28)
+__device__ __forceinline__ unsigned pair_payload(int rc, int ak) {
+  return ((unsigned)(rc + 1) << 28) | ((unsigned)ak & 0x0FFFFFFFu);
+}
+__device__ __forceinline__ int payload_rc(unsigned w) { return (int)(w >> 28) - 1; }
+__device__ __forceinline__ int payload_ak(unsigned w) { return (int)(w & 0x0FFFFFFFu); }
+
+template <int NC, int BLOCK, int BITS>
+__global__ void __launch_bounds__(BLOCK) wfa_breakpoint_pair_kernel(BiArgs a) {
+  constexpr bool CUT = false;
+  extern __shared__ __align__(16) unsigned char bp_smem[];
+  __shared__ int sh_red[BLOCK / 64][2 * NC + 2];
+  __shared__ int sh_min[BLOCK / 64 > 0 ? BLOCK / 64 : 1];
+  __shared__ unsigned long long sh_word;
+  BlockRed<BLOCK> red{sh_min};
+  const Penalties pen = a.pen;  // nslots: one more than the score scope needs
+  const int wcap = a.wcap;
+  const int team = blockIdx.x >> 1, role = blockIdx.x & 1;  // role 0: forward side, 1: reverse side
+  const int q = team;
+  if (q >= a.n) return;
+  unsigned long long* box = a.mail + 4 * team;
+  int* hand = a.handover + (long long)team * PairHandover<NC>::kInts;
+  unsigned seq = 0;
+  unsigned got = 0;
+  const BiProblem pr = a.probs[q];
+  const int plen = pr.pe - pr.pb, tlen = pr.te - pr.tb;
+  Side<NC, CUT> f, r;
+  setup_sides<NC, BLOCK, BITS>(a, pr, a.ring + (long long)team * a.ring_stride, pen.nslots * NC * wcap, bp_smem,
+                               f, r);
+  side_init(f, pr.comp_begin, pr.comp_end, wcap, a.heur.steps, role == 0);
+  side_init(r, pr.comp_end, pr.comp_begin, wcap, a.heur.steps, role == 1);
+  Side<NC, CUT>& me = role ? r : f;
+  __syncthreads();
+  Bp bp;
+  bp_clear(bp);
+  int status = kBiOk;
+  long long dsteps = 0;
+  const int s_cap = (pen.x + pen.o1 + pen.e1 + (NC == 5 ? pen.o2 + pen.e2 : 0) + 2) * (plen + tlen + 2);
+  int sf = 0, sr = 0, fmax = 0, rmax = 0, ak = 0;
+  int fwd_end = -1, rev_end = -1;
+  const int max_ad = plen + tlen - 1;
+  bool last_forward = false;
+  int rc_me, rcf, rcr, akf, akr;
+  // score 0 of both sides, then the reference's order: forward end test, reverse end test
+  rc_me = side_extend0<NC, BLOCK, BITS>(me, plen, tlen, wcap, pen, a.heur, red, &ak);
+  if (!pair_exchange(box, role, ++seq, pair_payload(rc_me, ak), &got, &sh_word)) { status = kBiPairAbort; goto done; }
+  rcf = role ? payload_rc(got) : rc_me;  akf = role ? payload_ak(got) : ak;
+  rcr = role ? rc_me : payload_rc(got);  akr = role ? ak : payload_ak(got);
+  fmax = akf;
+  if (rcf) { status = kBiEndReached; fwd_end = 0; goto done; }
+  rmax = akr;
+  if (rcr) { status = kBiEndReached; rev_end = 0; goto done; }
+  // phase 1 (wavefront_bialign.c:1003-1044), one round per score of both sides
+  while (true) {
+    if (fmax + rmax >= max_ad) break;
+    ++sf;
+    ++sr;
+    // the reverse side's state before its (speculative) step
+    const int cur0 = r.cur, end0 = r.end_score;
+    const long long ds0 = dsteps;
+    rc_me = side_step<NC, BLOCK, BITS>(me, role ? sr : sf, plen, tlen, pen, a.heur, wcap, true, &ak, &dsteps, sh_red,
+                                       red);
+    if (!pair_exchange(box, role, ++seq, pair_payload(rc_me, ak), &got, &sh_word)) { status = kBiPairAbort; goto done; }
+    rcf = role ? payload_rc(got) : rc_me;  akf = role ? payload_ak(got) : ak;
+    rcr = role ? rc_me : payload_rc(got);  akr = role ? ak : payload_ak(got);
+    if (rcf < 0) { status = kDevRingOverflow; goto done; }
+    if (rcf == 2) { status = kDevUnreachable; goto done; }
+    if (fmax < akf) fmax = akf;
+    last_forward = true;
+    if (rcf > 0) { status = kBiEndReached; fwd_end = sf; goto done; }
+    if (fmax + rmax >= max_ad) {
+      // the reverse step of this round never happened: take it back
+      --sr;
+      if (role) {
+        r.cur = cur0;
+        r.end_score = end0;
+        dsteps = ds0;
+      }
+      break;
+    }
+    if (rcr < 0) { status = kDevRingOverflow; goto done; }
+    if (rcr == 2) { status = kDevUnreachable; goto done; }
+    if (rmax < akr) rmax = akr;
+    last_forward = false;
+    if (rcr > 0) { status = kBiEndReached; rev_end = sr; goto done; }
+    if (sr + sf >= a.max_steps) { status = kDevMaxSteps; goto done; }
+    if (sf > s_cap) { status = kDevBadTrace; goto done; }
+  }
+  // hand the reverse side to the forward workgroup, which runs phase 2
+  if (role) {
+    if (threadIdx.x < 64) {
+      int* h = hand + threadIdx.x * PairHandover<NC>::kLaneInts;
+      h[0] = r.meta.tag;
+      h[1] = r.meta.alo;
+      h[2] = (int)r.meta.flags;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) { h[3 + c] = r.meta.lo[c]; h[3 + NC + c] = r.meta.hi[c]; }
+    }
+    if (threadIdx.x == 0) {
+      int* hs = hand + 64 * PairHandover<NC>::kLaneInts;
+      hs[0] = r.cur;
+      hs[1] = r.end_score;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) __threadfence();  // the rings and the record above, device-wide, before the word
+    (void)pair_exchange(box, role, ++seq, 0u, &got, &sh_word);
+    goto done;
+  }
+  if (!pair_exchange(box, role, ++seq, 0u, &got, &sh_word)) { status = kBiPairAbort; goto done; }
+  if (threadIdx.x == 0) __threadfence();  // ... and after it, before reading them
+  __syncthreads();
+  {
+    const int lane = threadIdx.x & 63;
+    const int* h = hand + lane * PairHandover<NC>::kLaneInts;
+    r.meta.tag = h[0];
+    r.meta.alo = h[1];
+    r.meta.flags = (unsigned)h[2];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) { r.meta.lo[c] = h[3 + c]; r.meta.hi[c] = h[3 + NC + c]; }
+    const int* hs = hand + 64 * PairHandover<NC>::kLaneInts;
+    r.cur = uniform(hs[0]);
+    r.end_score = uniform(hs[1]);
+  }
+  status = find_phase2<NC, BLOCK, BITS>(f, r, sf, sr, last_forward, plen, tlen, pen, a, wcap, s_cap, bp, &dsteps,
+                                        sh_red, red);
+  fwd_end = f.end_score;
+  rev_end = r.end_score;
+done:
+  if (role == 0 && threadIdx.x == 0) write_outcome<NC, CUT>(a.out + q, status, bp, fwd_end, rev_end, dsteps);
+  if (threadIdx.x == 0 && dsteps) atomicAdd(a.dsteps_total, (unsigned long long)dsteps);
+}
+
 template <int NC, int BLOCK, int BITS, bool CUT>
 const void* bp_fn() {
   return reinterpret_cast<const void*>(&wfa_breakpoint_kernel<NC, BLOCK, BITS, CUT>);
+}
+
+const void* select_breakpoint_pair(int ncomp, int bits, int block) {
+#define WFA_PSEL(NC, BL, BI) \
+  if (ncomp == NC && block == BL && bits == BI) return reinterpret_cast<const void*>(&wfa_breakpoint_pair_kernel<NC, BL, BI>);
+  WFA_PSEL(1, 1024, 2) WFA_PSEL(1, 1024, 8) WFA_PSEL(3, 1024, 2) WFA_PSEL(3, 1024, 8) WFA_PSEL(5, 1024, 2)
+  WFA_PSEL(5, 1024, 8)
+#undef WFA_PSEL
+  return nullptr;
 }
 
 const void* select_breakpoint(int ncomp, int bits, int block, bool cut) {
@@ -737,6 +969,33 @@ int breakpoint_max_blocks_per_cu(int ncomp, int bits, int block, bool cut, size_
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, lds_bytes) != hipSuccess) return 0;
   return n;
+}
+
+int breakpoint_pair_max_blocks_per_cu(int ncomp, int bits, int block, size_t lds_bytes) {
+  const void* f = select_breakpoint_pair(ncomp, bits, block);
+  if (!f) return 0;
+  if (lds_bytes > 48 * 1024 &&
+      hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) != hipSuccess)
+    return 0;
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, lds_bytes) != hipSuccess) return 0;
+  return n;
+}
+
+size_t breakpoint_pair_handover_ints(int ncomp) {
+  return ncomp == 1 ? PairHandover<1>::kInts : (ncomp == 3 ? PairHandover<3>::kInts : PairHandover<5>::kInts);
+}
+
+hipError_t launch_breakpoint_pair(const BiArgs& a, int ncomp, int bits, int block, hipStream_t stream) {
+  const void* f = select_breakpoint_pair(ncomp, bits, block);
+  if (!f || a.heur.cut() != 0 || !a.mail || !a.handover) return hipErrorInvalidValue;
+  const size_t lds = breakpoint_lds_bytes(a.lmax_words);
+  if (lds > 48 * 1024) {
+    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  void* args[] = {const_cast<BiArgs*>(&a)};
+  return hipLaunchKernel(f, dim3(2 * a.n), dim3(block), args, lds, stream);
 }
 
 hipError_t launch_breakpoint(const BiArgs& a, int ncomp, int bits, int block, int grid, hipStream_t stream) {

@@ -208,7 +208,9 @@ struct BiProblem {
   int comp_begin, comp_end;
   int pad;
 };
-enum BiStatus : int { kBiOk = 0, kBiEndReached = -2 };
+// kBiPairAbort: a paired-workgroup search whose partner did not answer in
+// time (the host searches the sub-problem again on one workgroup)
+enum BiStatus : int { kBiOk = 0, kBiEndReached = -2, kBiPairAbort = -3 };
 struct BiOutcome {             // wf_bialign_breakpoint_t plus the search status
   int status;                  // kBiOk, kBiEndReached, kDevMaxSteps, kDevRingOverflow, kDevBadTrace
   int score, score_forward, score_reverse;
@@ -241,8 +243,19 @@ struct BiArgs {
   int lmax_words;              // LDS words per sequence (four: forward and reversed pattern and text); 0: global
   HeurArgs heur;
   Penalties pen;
+  // paired workgroups (launch_breakpoint_pair): per team, 4 mailbox words
+  // (zeroed before the launch) and the reverse side's handover record
+  unsigned long long* mail;
+  int* handover;
 };
 hipError_t launch_breakpoint(const BiArgs& a, int ncomp, int bits, int block, int grid, hipStream_t stream);
+// Paired workgroups, one sub-problem per team of two (2 * a.n workgroups of
+// `block` threads; 1,024 only), no heuristic cut-off; a.pen.nslots must be
+// one more than the penalties' score scope needs, and every workgroup must
+// fit on the device at once (the caller checks against the occupancy).
+hipError_t launch_breakpoint_pair(const BiArgs& a, int ncomp, int bits, int block, hipStream_t stream);
+int breakpoint_pair_max_blocks_per_cu(int ncomp, int bits, int block, size_t lds_bytes);
+size_t breakpoint_pair_handover_ints(int ncomp);
 // Dynamic LDS of a breakpoint launch: the four sequences of a problem.
 size_t breakpoint_lds_bytes(int lmax_words);
 // A piece of a final CIGAR: len operations copied from src (or, when src < 0,

@@ -136,6 +136,7 @@ struct Engine {
   struct BiBuffers {
     DevBuf pos, len, probs, out, pieces;
     DevBuf pl, tl, pp, tp, cb, ce, st, sc, last, ds, off, beg, ops;
+    DevBuf mail, hand;  // paired-workgroup searches: mailboxes and handover records
     hipEvent_t ev[2] = {nullptr, nullptr};
   } bi;
   // Set for the duration of a no-allocation run: plan groups take their
