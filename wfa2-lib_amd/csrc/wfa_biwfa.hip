@@ -25,10 +25,11 @@ struct Side {
   bool dead;               // indel / edit: the cut-off emptied the wavefront
   int cur;                 // ring slot of the latest multiple-of-g score
   int* ring;
-  const uint64_t* P;
+  const uint64_t* P;       // sequences in global memory ...
   int pb;
   const uint64_t* T;
   int tb;
+  int lds_p, lds_t;        // ... or, when >= 0, at these words of the LDS sequence area
   int cb, ce;              // begin / end component of this direction
   int end_score;           // score at which this side reached its end, -1 if not
 };
@@ -38,6 +39,21 @@ struct Bp {  // wf_bialign_breakpoint_t
   int k_forward, k_reverse, offset_forward, offset_reverse;
   int component;
 };
+
+// Dynamic LDS of the breakpoint kernels: the sequences of the sub-problem.
+extern __shared__ __align__(16) unsigned char bp_smem[];
+
+// Extension of diagonal k from offset h on one side.  With the sequences in
+// LDS the windows are read through an LDS pointer (ds_read); a generic
+// pointer would make every window a flat load that waits on both counters.
+template <int BITS, int NC, bool CUT>
+__device__ __forceinline__ int side_lcp(const Side<NC, CUT>& sd, int plen, int tlen, int k, int h) {
+  if (sd.lds_p >= 0) {
+    const uint64_t* W = reinterpret_cast<const uint64_t*>(bp_smem);
+    return lcp_extend<BITS>(W + sd.lds_p, sd.pb, W + sd.lds_t, sd.tb, plen, tlen, k, h);
+  }
+  return lcp_extend<BITS>(sd.P, sd.pb, sd.T, sd.tb, plen, tlen, k, h);
+}
 
 // The stored wavefront of component c at score s of a side, if present:
 // its trimmed range and the ring offset of diagonal 0.
@@ -228,7 +244,7 @@ __device__ int side_step(Side<NC, CUT>& sd, int s, int plen, int tlen, const Pen
         if (!in_bounds(m, k, plen, tlen)) m = kOffsetNull;
         if (m != kOffsetNull) {
           if (prune_try) wmin = min(wmin, max(plen - (m - k), tlen - m));
-          m = lcp_extend<BITS>(sd.P, sd.pb, sd.T, sd.tb, plen, tlen, k, m);
+          m = side_lcp<BITS>(sd, plen, tlen, k, m);
           best_ak = max(best_ak, 2 * m - k);
         }
         outM[k] = m;
@@ -246,7 +262,7 @@ __device__ int side_step(Side<NC, CUT>& sd, int s, int plen, int tlen, const Pen
       }
       if (!in_bounds(m, k, plen, tlen)) m = kOffsetNull;
       if (m != kOffsetNull) {
-        m = lcp_extend<BITS>(sd.P, sd.pb, sd.T, sd.tb, plen, tlen, k, m);
+        m = side_lcp<BITS>(sd, plen, tlen, k, m);
         best_ak = max(best_ak, 2 * m - k);
       }
       outM[k] = m;
@@ -491,7 +507,7 @@ __device__ int side_extend0(Side<NC, CUT>& sd, int plen, int tlen, int wcap, con
   int v0 = 0;
   *max_ak = 0;
   if (sd.cb != kM) return 0;  // M absent at score 0: nothing to extend, no termination test
-  v0 = uniform(lcp_extend<BITS>(sd.P, sd.pb, sd.T, sd.tb, plen, tlen, 0, 0));
+  v0 = uniform(side_lcp<BITS>(sd, plen, tlen, 0, 0));
   __syncthreads();
   if (threadIdx.x == 0) sd.ring[(0 * NC + kM) * wcap] = v0;
   __syncthreads();
@@ -512,7 +528,7 @@ __device__ int side_extend0(Side<NC, CUT>& sd, int plen, int tlen, int wcap, con
 // previous sub-problem's use of the LDS with a barrier.
 template <int NC, int BLOCK, int BITS, bool CUT>
 __device__ __forceinline__ void setup_sides(const BiArgs& a, const BiProblem& pr, int* ring, int ring_side,
-                                            unsigned char* smem, Side<NC, CUT>& f, Side<NC, CUT>& r) {
+                                            Side<NC, CUT>& f, Side<NC, CUT>& r) {
   constexpr int SPW = 64 / BITS;
   const int pair = pr.pair;
   const int plen = pr.pe - pr.pb, tlen = pr.te - pr.tb;
@@ -525,10 +541,11 @@ __device__ __forceinline__ void setup_sides(const BiArgs& a, const BiProblem& pr
   const long long rp = a.rev_ppos[pair] + (plen_all - pr.pe), rt = a.rev_tpos[pair] + (tlen_all - pr.te);
   r.P = a.rev_words + rp / SPW; r.pb = (int)(rp % SPW);
   r.T = a.rev_words + rt / SPW; r.tb = (int)(rt % SPW);
+  f.lds_p = f.lds_t = r.lds_p = r.lds_t = -1;
   if (a.lmax_words > 0) {
     const int nwp = (f.pb + plen) / SPW + 2, nwt = (f.tb + tlen) / SPW + 2;
     const int nrp = (r.pb + plen) / SPW + 2, nrt = (r.tb + tlen) / SPW + 2;
-    uint64_t* s0 = reinterpret_cast<uint64_t*>(smem);
+    uint64_t* s0 = reinterpret_cast<uint64_t*>(bp_smem);
     uint64_t* s1 = s0 + a.lmax_words;
     uint64_t* s2 = s1 + a.lmax_words;
     uint64_t* s3 = s2 + a.lmax_words;
@@ -536,10 +553,10 @@ __device__ __forceinline__ void setup_sides(const BiArgs& a, const BiProblem& pr
     for (int i = threadIdx.x; i < nwt; i += BLOCK) s1[i] = f.T[i];
     for (int i = threadIdx.x; i < nrp; i += BLOCK) s2[i] = r.P[i];
     for (int i = threadIdx.x; i < nrt; i += BLOCK) s3[i] = r.T[i];
-    f.P = s0;
-    f.T = s1;
-    r.P = s2;
-    r.T = s3;
+    f.lds_p = 0;
+    f.lds_t = a.lmax_words;
+    r.lds_p = 2 * a.lmax_words;
+    r.lds_t = 3 * a.lmax_words;
   }
 }
 
@@ -612,7 +629,6 @@ __device__ __forceinline__ void bp_clear(Bp& bp) {
 
 template <int NC, int BLOCK, int BITS, bool CUT>
 __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
-  extern __shared__ __align__(16) unsigned char bp_smem[];
   __shared__ int sh_red[BLOCK / 64][2 * NC + 2];
   __shared__ int sh_min[BLOCK / 64 > 0 ? BLOCK / 64 : 1];
   __shared__ int sh_q;
@@ -631,7 +647,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
     Side<NC, CUT> f, r;
     __syncthreads();  // the previous sub-problem is done with the rings and the LDS sequences
     setup_sides<NC, BLOCK, BITS>(a, pr, a.ring + (long long)blockIdx.x * a.ring_stride,
-                                 pen.nslots * NC * wcap, bp_smem, f, r);
+                                 pen.nslots * NC * wcap, f, r);
     side_init(f, pr.comp_begin, pr.comp_end, wcap, a.heur.steps);
     side_init(r, pr.comp_end, pr.comp_begin, wcap, a.heur.steps);
     __syncthreads();
@@ -750,7 +766,6 @@ __device__ __forceinline__ int payload_ak(unsigned w) { return (int)(w & 0x0FFFF
 template <int NC, int BLOCK, int BITS>
 __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_pair_kernel(BiArgs a) {
   constexpr bool CUT = false;
-  extern __shared__ __align__(16) unsigned char bp_smem[];
   __shared__ int sh_red[BLOCK / 64][2 * NC + 2];
   __shared__ int sh_min[BLOCK / 64 > 0 ? BLOCK / 64 : 1];
   __shared__ unsigned long long sh_word;
@@ -767,8 +782,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_pair_kernel(BiArgs a) {
   const BiProblem pr = a.probs[q];
   const int plen = pr.pe - pr.pb, tlen = pr.te - pr.tb;
   Side<NC, CUT> f, r;
-  setup_sides<NC, BLOCK, BITS>(a, pr, a.ring + (long long)team * a.ring_stride, pen.nslots * NC * wcap, bp_smem,
-                               f, r);
+  setup_sides<NC, BLOCK, BITS>(a, pr, a.ring + (long long)team * a.ring_stride, pen.nslots * NC * wcap, f, r);
   side_init(f, pr.comp_begin, pr.comp_end, wcap, a.heur.steps, role == 0);
   side_init(r, pr.comp_end, pr.comp_begin, wcap, a.heur.steps, role == 1);
   Side<NC, CUT>& me = role ? r : f;

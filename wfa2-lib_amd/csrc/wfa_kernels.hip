@@ -158,24 +158,25 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 1024 ? 8 : 1) wfa_forward_kern
     // Sequences: word-aligned base pointer plus a symbol offset below SPW.
     const long long ppos = a.pat_pos[p], tpos = a.txt_pos[p];
     const int pb = (int)(ppos % SPW), tb = (int)(tpos % SPW);
-    const uint64_t* P;
-    const uint64_t* T;
     // The LDS family, and the HBM-ring family when the host gave room
     // (lmax_words > 0), read the sequences from LDS: the extension's
-    // windows then cost an LDS round trip instead of an L2 one.
-    if (SMALL || a.lmax_words > 0) {
-      const uint64_t* gP = a.words + ppos / SPW;
-      const uint64_t* gT = a.words + tpos / SPW;
+    // windows then cost an LDS round trip instead of an L2 one.  The two
+    // cases keep separate pointers (seqP/seqT into LDS, gP/gT in global
+    // memory) so that each extension is compiled with its memory's loads; a
+    // pointer that may be either would make every window a flat load.
+    const uint64_t* gP = a.words + ppos / SPW;
+    const uint64_t* gT = a.words + tpos / SPW;
+    const bool lds_seq = SMALL || a.lmax_words > 0;
+    if (lds_seq) {
       const int nwp = (pb + plen) / SPW + 2, nwt = (tb + tlen) / SPW + 2;
       block_sync();  // the previous pair is done reading the sequences
       for (int i = tid; i < nwp; i += BLOCK) seqP[i] = gP[i];
       for (int i = tid; i < nwt; i += BLOCK) seqT[i] = gT[i];
-      P = seqP;
-      T = seqT;
-    } else {
-      P = a.words + ppos / SPW;
-      T = a.words + tpos / SPW;
     }
+    auto extend = [&](int k, int h) {
+      return lds_seq ? lcp_extend<BITS>(seqP, pb, seqT, tb, plen, tlen, k, h)
+                     : lcp_extend<BITS>(gP, pb, gT, tb, plen, tlen, k, h);
+    };
     // Score 0: the begin component holds offset 0 at k = 0 (wavefront_aligner.c:252-380).
     meta.tag = (lane == 0) ? 0 : -1;
     meta.alo = 0;
@@ -226,7 +227,7 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 1024 ? 8 : 1) wfa_forward_kern
           const int k = kb + tid;
           bool hit = false;
           if (k <= hi0) {
-            const int m = lcp_extend<BITS>(P, pb, T, tb, plen, tlen, k, k > 0 ? k : 0);
+            const int m = extend(k, k > 0 ? k : 0);
             ring[(0 * NC + kM) * wcap + (k - lo0)] = to_ring(m, ring);
             hit = endsfree_done(m, k, plen, tlen, a.pef, a.tef);
           }
@@ -249,7 +250,7 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 1024 ? 8 : 1) wfa_forward_kern
       }
     } else if (status == 1) {
       int v0 = 0;
-      if (tid == 0 && cbeg == kM) v0 = lcp_extend<BITS>(P, pb, T, tb, plen, tlen, 0, 0);
+      if (tid == 0 && cbeg == kM) v0 = extend(0, 0);
       v0 = broadcast0(v0, 1);
       if (tid == 0) ring[ring_base(pen, wcap, cbeg, rp, 0)] = to_ring(v0, ring);
       if (cbeg == kM && cend == kM && kend == 0 && v0 >= tlen) status = kDevCompleted;
@@ -427,7 +428,7 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 1024 ? 8 : 1) wfa_forward_kern
             if (!in_bounds(m, k, plen, tlen)) m = kOffsetNull;
             if (m != kOffsetNull) {
               if (prune_try) wmin = min(wmin, max(plen - (m - k), tlen - m));
-              m = lcp_extend<BITS>(P, pb, T, tb, plen, tlen, k, m);
+              m = extend(k, m);
               hit = a.endsfree && endsfree_done(m, k, plen, tlen, a.pef, a.tef);
             }
             if (rec) rec[k - lo] = (uint8_t)src;
@@ -463,7 +464,7 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 1024 ? 8 : 1) wfa_forward_kern
           int m = best;
           if (!in_bounds(m, k, plen, tlen)) m = kOffsetNull;  // null-after-max
           if (m != kOffsetNull) {
-            m = lcp_extend<BITS>(P, pb, T, tb, plen, tlen, k, m);
+            m = extend(k, m);
             hit = a.endsfree && endsfree_done(m, k, plen, tlen, a.pef, a.tef);
           }
           if (rec) rec[k - lo] = (uint8_t)(src | bits);
