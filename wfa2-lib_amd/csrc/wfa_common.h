@@ -143,7 +143,9 @@ template <typename RingPtr>
 __device__ __forceinline__ int rd(RingPtr ring, const Src& s, int k) {
   const int i = k - s.lo;
   const bool in = (unsigned)i <= (unsigned)s.span;
-  const int v = (int)ring[s.base + (in ? i : 0)];
+  // base + i is a non-negative element of the ring: an unsigned index lets
+  // a global ring be read with a 32-bit offset from its uniform base
+  const int v = (int)ring[(unsigned)(s.base + (in ? i : 0))];
   return in ? v : kOffsetNull;
 }
 

@@ -29,7 +29,7 @@ namespace {
 // The 1024-thread instance is held to 64 VGPRs (8 waves per SIMD), so two
 // workgroups, two pairs, share a CU.
 template <int NC, int BLOCK, bool SMALL, int BITS, int CUT>
-__global__ void __launch_bounds__(BLOCK, BLOCK == 1024 ? 8 : 1) wfa_forward_kernel(ForwardArgs a) {
+__global__ void __launch_bounds__(BLOCK, BLOCK == 1024 ? 4 : 1) wfa_forward_kernel(ForwardArgs a) {
   constexpr bool HC = CUT != 0;
   // The LDS family stores the ring as int16, the HBM family as int32.
   using RT = typename std::conditional<SMALL, int16_t, int>::type;
