@@ -320,7 +320,7 @@ LONG_WORKLOADS = {
     #        CPU memory mode, description)
     # The reference's memory mode med gives the CIGARs of memory high (tests/wfa.utest.sh:67-78) with
     # O(s) memory per thread, so its CPU sample of the 1-20 kbp pairs runs in med on every thread.
-    "affine2p-1-20kbp": (2, "affine2p", (0, 4, 6, 2, 24, 1), "high", 1000, 1000, "med",
+    "affine2p-1-20kbp": (2, "affine2p", (0, 4, 6, 2, 24, 1), "high", 4000, 1000, "med",
                          "dual-cost gap-affine (0,4,6,2,24,1) end-to-end pairs, L log-uniform in [1,20] kbp, "
                          "error ~U[1%,15%], geometric indels (mean 3), score+CIGAR, memory-high"),
     "biwfa-100kbp": (3, "affine", (0, 4, 6, 2), "ultralow", 256, 32, "ultralow",
