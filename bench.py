@@ -239,7 +239,7 @@ def serve_measurement(duration_s, seed, log, lanes=(1, 1, 1)):
     from wfa_amd import datagen
 
     cfg = dict(short_max_length=2048, long_min_length=32768, short_lanes=lanes[0], medium_lanes=lanes[1],
-               long_lanes=lanes[2], max_batch_pairs=4096, lane_memory_bytes=24 << 30)
+               long_lanes=lanes[2], max_batch_pairs=4096)
 
     srv = wfa_amd.Server(**cfg)
 
