@@ -73,3 +73,23 @@ def test_paired_search_vs_oracle(metric, pen):
     assert np.array_equal(got.status, ref.status) and np.array_equal(got.score, ref.score)
     assert all(got.ops(i) == ref.ops(i) for i in range(ps.n))
     assert st.diagonal_steps == int(ref.dsteps.sum())
+
+
+def test_timed_out_paired_searches_fall_back_to_one_workgroup():
+    """A wait bound of a few device-clock ticks makes paired searches give
+    up (kBiPairAbort) instead of waiting for their partner; the host then
+    searches those sub-problems again on one workgroup, with the same
+    results.  Every wait stays bounded, so no workgroup can hang."""
+    ps = long_pairs(834, 4, 8000, 14000)
+    say("paired BiWFA with a 2-tick wait bound: %d pairs" % ps.n)
+    old = os.environ.get("WFA_PAIR_WAIT_TICKS")
+    os.environ["WFA_PAIR_WAIT_TICKS"] = "2"
+    try:
+        got, st = run("affine", (0, 4, 6, 2), ps, True)
+    finally:
+        if old is None:
+            del os.environ["WFA_PAIR_WAIT_TICKS"]
+        else:
+            os.environ["WFA_PAIR_WAIT_TICKS"] = old
+    ref, st0 = run("affine", (0, 4, 6, 2), ps, False)
+    same(got, ref, ps.n)

@@ -110,6 +110,14 @@ bool ensure_reverse(Engine* e, wfa_device_batch_t* b) {
 
 int bp_block(int size) { return size <= 512 ? 64 : (size <= 8192 ? 256 : 1024); }
 
+// Bound of every wait of a paired search: 4 s of the 100 MHz device clock.
+// WFA_PAIR_WAIT_TICKS sets another (tests force the one-workgroup fallback
+// with a bound of a few ticks).
+unsigned long long pair_wait_ticks() {
+  const char* v = getenv("WFA_PAIR_WAIT_TICKS");
+  return v ? strtoull(v, nullptr, 10) : 400000000ull;
+}
+
 // WFA_PAIR_SEARCH=0 turns paired-workgroup breakpoint searches off.
 bool pair_searches_enabled() {
   const char* v = getenv("WFA_PAIR_SEARCH");
@@ -208,6 +216,7 @@ bool run_breakpoints(wavefront_aligner_t* wf, Engine* e, wfa_device_batch_t* b, 
       a.pen = lpen;
       a.mail = paired ? e->bi.mail.as<unsigned long long>() : nullptr;
       a.handover = paired ? e->bi.hand.as<int>() : nullptr;
+      a.pair_wait_ticks = pair_wait_ticks();
       for (auto& ev : e->bi.ev)
         if (!ev && !hip_ok(hipEventCreate(&ev), "hipEventCreate")) return false;
       hipEvent_t e0 = e->bi.ev[0], e1 = e->bi.ev[1];

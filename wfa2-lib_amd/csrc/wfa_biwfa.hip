@@ -714,10 +714,10 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
 // per side and round parity: round number, step status, antidiagonal); the
 // hand-over of the reverse side's rings and slot metadata is ordered by a
 // device-scope release fence before its mailbox word and an acquire fence
-// after it.  Every wait is bounded by kPairWaitTicks of the 100 MHz device
-// clock; a team whose partner does not answer in time reports kBiPairAbort
-// and the host searches that sub-problem again on one workgroup.
-constexpr unsigned long long kPairWaitTicks = 400000000ull;  // 4 s
+// after it.  Every wait is bounded by BiArgs::pair_wait_ticks of the 100 MHz
+// device clock (4 s unless the host asks for less); a team whose partner
+// does not answer in time reports kBiPairAbort and the host searches that
+// sub-problem again on one workgroup.
 
 // Handover record of the reverse side: per lane of wave 0, its slot's
 // metadata, then the side's scalars.
@@ -731,7 +731,7 @@ struct PairHandover {
 // the same exchange (parity slots: a partner can be at most one exchange
 // ahead).  Returns false on timeout.  Every thread of the workgroup calls it.
 __device__ __forceinline__ bool pair_exchange(unsigned long long* box, int role, unsigned seq, unsigned payload,
-                                              unsigned* got, unsigned long long* sh) {
+                                              unsigned* got, unsigned long long* sh, unsigned long long wait_ticks) {
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned long long* mine = box + 2 * role + (seq & 1u);
@@ -743,7 +743,7 @@ __device__ __forceinline__ bool pair_exchange(unsigned long long* box, int role,
     for (;;) {
       w = __hip_atomic_load(other, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if ((unsigned)(w >> 32) == seq) { ok = true; break; }
-      if (__builtin_amdgcn_s_memrealtime() - t0 > kPairWaitTicks) break;
+      if (__builtin_amdgcn_s_memrealtime() - t0 > wait_ticks) break;
       __builtin_amdgcn_s_sleep(2);
     }
     *sh = ok ? (w & 0xFFFFFFFFull) : ~0ull;
@@ -799,7 +799,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_pair_kernel(BiArgs a) {
   int rc_me, rcf, rcr, akf, akr;
   // score 0 of both sides, then the reference's order: forward end test, reverse end test
   rc_me = side_extend0<NC, BLOCK, BITS>(me, plen, tlen, wcap, pen, a.heur, red, &ak);
-  if (!pair_exchange(box, role, ++seq, pair_payload(rc_me, ak), &got, &sh_word)) { status = kBiPairAbort; goto done; }
+  if (!pair_exchange(box, role, ++seq, pair_payload(rc_me, ak), &got, &sh_word, a.pair_wait_ticks)) { status = kBiPairAbort; goto done; }
   rcf = role ? payload_rc(got) : rc_me;  akf = role ? payload_ak(got) : ak;
   rcr = role ? rc_me : payload_rc(got);  akr = role ? ak : payload_ak(got);
   fmax = akf;
@@ -816,7 +816,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_pair_kernel(BiArgs a) {
     const long long ds0 = dsteps;
     rc_me = side_step<NC, BLOCK, BITS>(me, role ? sr : sf, plen, tlen, pen, a.heur, wcap, true, &ak, &dsteps, sh_red,
                                        red);
-    if (!pair_exchange(box, role, ++seq, pair_payload(rc_me, ak), &got, &sh_word)) { status = kBiPairAbort; goto done; }
+    if (!pair_exchange(box, role, ++seq, pair_payload(rc_me, ak), &got, &sh_word, a.pair_wait_ticks)) { status = kBiPairAbort; goto done; }
     rcf = role ? payload_rc(got) : rc_me;  akf = role ? payload_ak(got) : ak;
     rcr = role ? rc_me : payload_rc(got);  akr = role ? ak : payload_ak(got);
     if (rcf < 0) { status = kDevRingOverflow; goto done; }
@@ -859,10 +859,10 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_pair_kernel(BiArgs a) {
     }
     __syncthreads();
     if (threadIdx.x == 0) __threadfence();  // the rings and the record above, device-wide, before the word
-    (void)pair_exchange(box, role, ++seq, 0u, &got, &sh_word);
+    (void)pair_exchange(box, role, ++seq, 0u, &got, &sh_word, a.pair_wait_ticks);
     goto done;
   }
-  if (!pair_exchange(box, role, ++seq, 0u, &got, &sh_word)) { status = kBiPairAbort; goto done; }
+  if (!pair_exchange(box, role, ++seq, 0u, &got, &sh_word, a.pair_wait_ticks)) { status = kBiPairAbort; goto done; }
   if (threadIdx.x == 0) __threadfence();  // ... and after it, before reading them
   __syncthreads();
   {

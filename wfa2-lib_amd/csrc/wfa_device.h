@@ -247,6 +247,7 @@ struct BiArgs {
   // (zeroed before the launch) and the reverse side's handover record
   unsigned long long* mail;
   int* handover;
+  unsigned long long pair_wait_ticks;  // bound of every wait, 100 MHz device clock ticks
 };
 hipError_t launch_breakpoint(const BiArgs& a, int ncomp, int bits, int block, int grid, hipStream_t stream);
 // Paired workgroups, one sub-problem per team of two (2 * a.n workgroups of
