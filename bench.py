@@ -461,7 +461,7 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the pairs are independent, so N GPUs would each run "
-              "their own share with no exchange (see DESIGN.md); run with --gpus 1.", file=sys.stderr)
+              "their own share with no exchange; run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
