@@ -618,42 +618,123 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 1024 ? 4 : 1) wfa_forward_kern
   if (tid == 0 && wg_dsteps) atomicAdd(a.dsteps_total, wg_dsteps);
 }
 
+// Sixteen bytes of a CIGAR slot gathered in registers before one aligned
+// 16-byte store (the backtrace is bound by memory transactions: one lane per
+// pair, so every store of a wave touches 64 different lines).
+struct OpChunk {
+  uint32_t a0 = 0u, a1 = 0u, a2 = 0u, a3 = 0u;
+  // Byte b (0..15).  The halves are combined before the select: a select
+  // between two members would become a load from a selected address, which
+  // keeps the chunk in memory instead of registers.
+  __device__ __forceinline__ unsigned byte_at(unsigned b) const {
+    const uint64_t lo = ((uint64_t)a1 << 32) | a0, hi = ((uint64_t)a3 << 32) | a2;
+    return (unsigned)(((b & 8u) ? hi : lo) >> (8u * (b & 7u))) & 0xFFu;
+  }
+  __device__ __forceinline__ static uint32_t span(unsigned from, unsigned to, unsigned j) {
+    const unsigned lo = from < 4 * j ? 0u : from > 4 * j + 4 ? 4u : from - 4 * j;
+    const unsigned hi = to < 4 * j ? 0u : to > 4 * j + 4 ? 4u : to - 4 * j;
+    return (uint32_t)((1ull << (8 * hi)) - (1ull << (8 * lo)));
+  }
+  // Bytes [from, to) (0 <= from <= to <= 16), zero before, set to c4's byte
+  // (c4 holds it four times).
+  __device__ __forceinline__ void fill(unsigned from, unsigned to, uint32_t c4) {
+    a0 |= c4 & span(from, to, 0);
+    a1 |= c4 & span(from, to, 1);
+    a2 |= c4 & span(from, to, 2);
+    a3 |= c4 & span(from, to, 3);
+  }
+  __device__ __forceinline__ void store(unsigned char* at) const {
+    *reinterpret_cast<uint4*>(at) = make_uint4(a0, a1, a2, a3);
+  }
+  __device__ __forceinline__ void clear() { a0 = a1 = a2 = a3 = 0u; }
+};
+
+__device__ __forceinline__ unsigned chunk_phase(const unsigned char* at) {
+  return (unsigned)(reinterpret_cast<uintptr_t>(at) & 15u);
+}
+
 // Forward writer of the replay: consecutive bytes from position w0 of a
-// slot are combined into aligned 4-byte stores (one memory transaction per
-// four operations instead of one per operation).  A dword that reaches
-// below w0 is written byte by byte, and so is the partial dword at the end
-// (finish), so no byte outside [w0, w) is touched: the neighbouring slots
-// belong to other lanes.  Positions below w may sit in the combiner until
-// their dword fills; the replay only reads positions at or above w.
+// slot are combined into aligned 16-byte stores (a run of matches fills
+// whole chunks directly).  A chunk that reaches below w0 is written byte by
+// byte, and so is the partial chunk at the end (finish), so no byte outside
+// [w0, w) is touched: the neighbouring slots belong to other lanes.
+// Positions below w may sit in the chunk until it fills; the replay only
+// reads positions at or above w.
 struct OpWriter {
   unsigned char* out;
   int w, w0;
-  uint32_t acc;
-  __device__ __forceinline__ unsigned phase() const { return (unsigned)(reinterpret_cast<uintptr_t>(out + w) & 3u); }
-  __device__ __forceinline__ void put(unsigned c) {
-    const unsigned d = phase();
-    acc |= (c & 0xFFu) << (8u * d);
-    ++w;
-    if (d == 3u) {
-      if (w - 4 >= w0) {
-        *reinterpret_cast<uint32_t*>(out + w - 4) = acc;
-      } else {
-        for (int i = w0; i < w; ++i) out[i] = (unsigned char)(acc >> (8u * (3u - (unsigned)(w - 1 - i))));
-      }
-      acc = 0;
+  OpChunk c;  // the chunk holding position w: its bytes below w
+  __device__ __forceinline__ unsigned phase() const { return chunk_phase(out + w); }
+  // The chunk [w - 16, w) is complete (w is 16-aligned).
+  __device__ __forceinline__ void flush() {
+    if (w - 16 >= w0) {
+      c.store(out + w - 16);
+    } else {
+      for (int i = w0; i < w; ++i) out[i] = (unsigned char)c.byte_at((unsigned)(i - (w - 16)));
     }
+    c.clear();
   }
-  // m copies of c: bytes up to the next dword boundary, then whole dwords.
-  __device__ __forceinline__ void run(unsigned c, int m) {
-    while (m > 0 && phase() != 0u) { put(c); --m; }
-    const uint32_t c4 = (c & 0xFFu) * 0x01010101u;
-    for (; m >= 4; m -= 4, w += 4) *reinterpret_cast<uint32_t*>(out + w) = c4;
-    for (; m > 0; --m) put(c);
+  __device__ __forceinline__ void put(unsigned op) {
+    const unsigned d = phase();
+    c.fill(d, d + 1, (op & 0xFFu) * 0x01010101u);
+    ++w;
+    if (d == 15u) flush();
+  }
+  // m copies of op: the rest of the current chunk, then whole chunks, then
+  // the beginning of the last one.
+  __device__ __forceinline__ void run(unsigned op, int m) {
+    if (m <= 0) return;
+    const uint32_t c4 = (op & 0xFFu) * 0x01010101u;
+    const unsigned d = phase();
+    if (d != 0u) {
+      const unsigned take = (unsigned)m < 16u - d ? (unsigned)m : 16u - d;
+      c.fill(d, d + take, c4);
+      w += (int)take;
+      m -= (int)take;
+      if (d + take < 16u) return;
+      flush();
+    }
+    for (; m >= 16; m -= 16, w += 16) *reinterpret_cast<uint4*>(out + w) = make_uint4(c4, c4, c4, c4);
+    if (m > 0) {
+      c.fill(0u, (unsigned)m, c4);
+      w += m;
+    }
   }
   __device__ __forceinline__ void finish() {
     const int d = (int)phase();
-    for (int i = max(w - d, w0); i < w; ++i) out[i] = (unsigned char)(acc >> (8 * (d - (w - i))));
-    acc = 0;
+    for (int i = max(w - d, w0); i < w; ++i) out[i] = (unsigned char)c.byte_at((unsigned)(i - (w - d)));
+    c.clear();
+  }
+};
+
+// Backward writer of the walk: bytes written at descending positions below
+// top are combined into aligned 16-byte stores.  A chunk that reaches top
+// (the bytes from top up belong to the trailing gaps or to another slot) is
+// written byte by byte, and so is the partial chunk at the end (finish).
+struct OpWriterBack {
+  unsigned char* out;
+  int p, top;
+  OpChunk c;  // the chunk holding position p: its bytes from p up
+  __device__ __forceinline__ void put(unsigned op) {
+    --p;
+    const unsigned d = chunk_phase(out + p);
+    c.fill(d, d + 1, (op & 0xFFu) * 0x01010101u);
+    if (d == 0u) {
+      if (p + 16 <= top) {
+        c.store(out + p);
+      } else {
+        for (int i = p; i < top; ++i) out[i] = (unsigned char)c.byte_at((unsigned)(i - p));
+      }
+      c.clear();
+    }
+  }
+  __device__ __forceinline__ void finish() {
+    const int d = (int)chunk_phase(out + p);
+    if (d != 0) {
+      const int c0 = p - d;
+      for (int i = p; i < min(c0 + 16, top); ++i) out[i] = (unsigned char)c.byte_at((unsigned)(i - c0));
+    }
+    c.clear();
   }
 };
 
@@ -735,8 +816,9 @@ __global__ void __launch_bounds__(256) wfa_backtrace_kernel(BacktraceArgs a) {
   uint32_t blk = a.last_block[p];
   uint4 hdr = make_uint4(0, 0, 0, kNoBlock);
   if (blk != kNoBlock && blk < a.rec_units) hdr = a.rec_arena[blk];
+  OpWriterBack bw{out, pos, pos, OpChunk{}};
   while (s > 0 && !bad && !direct) {
-    if (pos <= 0) {
+    if (bw.p <= 0) {
       bad = true;
       break;
     }
@@ -762,7 +844,7 @@ __global__ void __launch_bounds__(256) wfa_backtrace_kernel(BacktraceArgs a) {
     if (comp == kM) {
       switch (rec & kRecSrcMask) {
         case kSrcX:
-          out[--pos] = (unsigned char)('X' | (mark ? 0x80 : 0));
+          bw.put('X' | (mark ? 0x80u : 0u));
           ++nx;
           s -= pen.x;
           mark = true;
@@ -777,7 +859,7 @@ __global__ void __launch_bounds__(256) wfa_backtrace_kernel(BacktraceArgs a) {
     const bool ins = comp == kI1 || comp == kI2;
     const bool one = comp == kI1 || comp == kD1;
     const uint32_t ext_bit = comp == kI1 ? kRecI1Ext : comp == kD1 ? kRecD1Ext : comp == kI2 ? kRecI2Ext : kRecD2Ext;
-    out[--pos] = (unsigned char)((ins ? 'I' : 'D') | (mark ? 0x80 : 0));
+    bw.put((ins ? 'I' : 'D') | (mark ? 0x80u : 0u));
     ++nid;
     k += ins ? -1 : 1;
     if (rec & ext_bit) {
@@ -789,6 +871,8 @@ __global__ void __launch_bounds__(256) wfa_backtrace_kernel(BacktraceArgs a) {
       mark = true;
     }
   }
+  bw.finish();
+  pos = bw.p;
   // The walk ends at score 0: at the origin, or (ends-free) at a free
   // beginning cell (v=0, h=k) or (v=-k, h=0) of the score-0 wavefront.
   if (!bad && !direct && (s != 0 || comp != cbeg || (k != 0 && !a.endsfree))) bad = true;
@@ -845,7 +929,7 @@ __global__ void __launch_bounds__(256) wfa_backtrace_kernel(BacktraceArgs a) {
       len = bad ? 0 : rest / 2 + (cap - pos) + nlead;
     }
     start = bad ? cap : cap - len;
-    OpWriter ow{out, start, start, 0u};
+    OpWriter ow{out, start, start, OpChunk{}};
     int v = 0, h = 0;
     if (!bad) {
       ow.run(k > 0 ? 'I' : 'D', nlead);  // free leading gaps
@@ -868,8 +952,22 @@ __global__ void __launch_bounds__(256) wfa_backtrace_kernel(BacktraceArgs a) {
     if (!bad && cbeg == kM) {
       if (zd && pos == cap - ntrail) run_to_end(); else extend();
     }
+    // The operations are read 16 bytes at a time: an aligned 16-byte chunk
+    // that holds a byte of the slot lies in the same page as that byte.
+    // Bytes of the cached chunk below the position read may since have been
+    // overwritten by the writer (which stays below it); they are not read
+    // again.
+    uintptr_t cached = ~(uintptr_t)0;
+    uint4 chunk = make_uint4(0u, 0u, 0u, 0u);
     for (int i = pos; i < cap && !bad; ++i) {
-      const unsigned char b = out[i];
+      const uintptr_t ad = reinterpret_cast<uintptr_t>(out + i);
+      const unsigned o = (unsigned)(ad & 15u);
+      if ((ad & ~(uintptr_t)15) != cached) {
+        cached = ad & ~(uintptr_t)15;
+        chunk = *reinterpret_cast<const uint4*>(out + (i - (int)o));
+      }
+      const uint32_t lo = (o & 4u) ? chunk.y : chunk.x, hi = (o & 4u) ? chunk.w : chunk.z;
+      const unsigned char b = (unsigned char)(((o & 8u) ? hi : lo) >> (8u * (o & 3u)));
       const unsigned char op = b & 0x7F;
       if (ow.w > i) { bad = true; break; }  // inconsistent records: never overrun unread operations
       ow.put(op);
