@@ -222,6 +222,11 @@ constexpr int lean_ring_elems() {
 // kernel's probed_records does.
 template <int R, int DQX, int DQO, int DQE, int WCAP, bool REC, bool EF, bool CUT>
 __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
+  // The ring is a fixed-size LDS array, whose address the compiler knows
+  // (an address in dynamic LDS is resolved at load time: every ring and
+  // sequence address then costs an add of it); the sequences, sized by the
+  // batch's longest pair, follow in dynamic LDS.
+  __shared__ __align__(16) int16_t ring_lds[lean_ring_elems<R, WCAP>()];
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int NC = 3;
   static_assert(R * NC <= 32, "emptied mask");
@@ -232,8 +237,8 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
 
   // LDS: the ring (element 0: NULL cell read by absent sources; slots from
   // element 2), then both sequences.
-  int16_t* ring = reinterpret_cast<int16_t*>(smem);
-  uint64_t* seqP64 = reinterpret_cast<uint64_t*>(smem + 2 * lean_ring_elems<R, WCAP>());
+  int16_t* ring = ring_lds;
+  uint64_t* seqP64 = reinterpret_cast<uint64_t*>(smem);
   uint64_t* seqT64 = seqP64 + a.lmax_words;
   const uint32_t* seqP = reinterpret_cast<const uint32_t*>(seqP64);
   const uint32_t* seqT = reinterpret_cast<const uint32_t*>(seqT64);
@@ -498,8 +503,12 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
         int16_t* oD = ring + gbase(kD1, true) - org;
         // NULL in the two M cells just outside the computed range (and in
         // the gap cells there when every input is clamp-read).
-        if (lane < 2) {
-          const int kk = lane ? hi + 1 : lo - 1;
+        // (lane < 2 compared here from an opaque copy of the lane: a
+        // compare hoisted out of the step loop is kept as a spilled mask)
+        int pad_lane = lane;
+        asm volatile("" : "+v"(pad_lane));
+        if (pad_lane < 2) {
+          const int kk = pad_lane ? hi + 1 : lo - 1;
           oM[kk] = kRingNull;
           if constexpr (CUT) {
             oI[kk] = kRingNull;
@@ -755,8 +764,9 @@ int lean_width(int wcap) {
 }
 
 size_t lean_lds_bytes(const Penalties& pen, int wcap, int lmax_words) {
+  (void)wcap;  // the ring is static LDS of each instance
   if (!lean_structure(pen)) return 0;
-  return (size_t)2 * (((2 + (pen.nslots + 4) * wcap) + 7) / 8 * 8) + (size_t)2 * lmax_words * sizeof(uint64_t);
+  return (size_t)2 * lmax_words * sizeof(uint64_t);
 }
 
 int lean_max_blocks_per_cu(const Penalties& pen, int wcap, size_t lds_bytes) {
