@@ -212,16 +212,29 @@ def seq_file_measurement(n):
             for i in range(ps.n):
                 f.write(b">" + ps.pattern(i) + b"\n<" + ps.text(i) + b"\n")
         al = wfa_amd.Aligner(metric="affine", penalties=(0, 4, 6, 2))
-        rc, st = al.align_seq_file(src, dst)
-        if rc != 0 or st.pairs != ps.n:
-            raise RuntimeError("seq file run failed (%d, %d pairs)" % (rc, st.pairs))
+        # The first file of a new aligner also pays its set-up (stream,
+        # device and pinned buffers, the launch-shape tuner's trials); the
+        # second, the same file again, is the rate of a process that streams
+        # file after file.  Both are reported; value is the second.
+        runs = []
+        for _ in range(2):
+            rc, st = al.align_seq_file(src, dst)
+            if rc != 0 or st.pairs != ps.n:
+                raise RuntimeError("seq file run failed (%d, %d pairs)" % (rc, st.pairs))
+            runs.append(st)
+        cold, st = runs
         out = {"pairs": int(st.pairs), "value": st.pairs / st.seconds, "unit": "pairs/s",
                "input_mb_s": st.bytes_in / st.seconds / 1e6, "seconds": st.seconds,
                "read_s": st.read_seconds, "align_s": st.align_seconds, "write_s": st.write_seconds,
+               "first_file": {"value": cold.pairs / cold.seconds, "seconds": cold.seconds,
+                              "read_s": cold.read_seconds, "align_s": cold.align_seconds,
+                              "write_s": cold.write_seconds},
                "sample": "%d short-read pairs (gap-affine 0,4,6,2, score+CIGAR) from a local .seq file "
-                         "of %.0f MB to a .alg file of %.0f MB" % (ps.n, st.bytes_in / 1e6, st.bytes_out / 1e6)}
-        log(".seq -> .alg: %.3e pairs/s (%.0f MB/s in; read %.2f s, align %.2f s, write %.2f s, total %.2f s)"
-            % (out["value"], out["input_mb_s"], st.read_seconds, st.align_seconds, st.write_seconds, st.seconds))
+                         "of %.0f MB to a .alg file of %.0f MB, the second file of the aligner (first_file: "
+                         "the first, with the aligner's set-up)" % (ps.n, st.bytes_in / 1e6, st.bytes_out / 1e6)}
+        log(".seq -> .alg: %.3e pairs/s (%.0f MB/s in; read %.2f s, align %.2f s, write %.2f s, total %.2f s; "
+            "first file %.2f s)" % (out["value"], out["input_mb_s"], st.read_seconds, st.align_seconds,
+                                    st.write_seconds, st.seconds, cold.seconds))
         return out
     finally:
         shutil.rmtree(tmp, ignore_errors=True)

@@ -483,7 +483,7 @@ void wfa_server_destroy(wfa_server_t* server);
  * The input and output formats of the reference's align_benchmark tool,
  * streamed through the device: a .seq file of line pairs (">pattern",
  * "<text"; tools/align_benchmark/align_benchmark.c:313-339) is aligned in
- * batches of batch_pairs pairs (0: 262144) with the aligner's configuration,
+ * batches of batch_pairs pairs (0: 65536) with the aligner's configuration,
  * and one .alg line per pair is written (benchmark_utils.c:103-179):
  * "score\tCIGAR", or with full_output "plen\ttlen\tscore\tpattern\ttext\tCIGAR".
  * Parsing, device work and formatting of consecutive batches overlap.

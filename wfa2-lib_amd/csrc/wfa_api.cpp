@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cstdio>
 #include <cstdlib>
@@ -608,13 +609,30 @@ bool all_acgt(const wfa_pair_t* pairs, int n) {
     Lut() : v() { v['A'] = v['C'] = v['G'] = v['T'] = true; }
   } lut_table;
   const bool* lut = lut_table.v;
-  for (int i = 0; i < n; ++i) {
-    const unsigned char* p = reinterpret_cast<const unsigned char*>(pairs[i].pattern);
-    for (int j = 0; j < pairs[i].pattern_length; ++j) if (!lut[p[j]]) return false;
-    const unsigned char* t = reinterpret_cast<const unsigned char*>(pairs[i].text);
-    for (int j = 0; j < pairs[i].text_length; ++j) if (!lut[t[j]]) return false;
-  }
-  return true;
+  auto scan = [&](int i0, int i1, const std::atomic<bool>* stop) {
+    for (int i = i0; i < i1; ++i) {
+      if (stop && (i & 1023) == 0 && stop->load(std::memory_order_relaxed)) return false;
+      const unsigned char* p = reinterpret_cast<const unsigned char*>(pairs[i].pattern);
+      for (int j = 0; j < pairs[i].pattern_length; ++j) if (!lut[p[j]]) return false;
+      const unsigned char* t = reinterpret_cast<const unsigned char*>(pairs[i].text);
+      for (int j = 0; j < pairs[i].text_length; ++j) if (!lut[t[j]]) return false;
+    }
+    return true;
+  };
+  int64_t symbols = 0;
+  for (int i = 0; i < n; ++i) symbols += (int64_t)pairs[i].pattern_length + pairs[i].text_length;
+  // large batches are scanned by up to 16 threads, one range of pairs each
+  const int nt = (int)std::min<int64_t>({16, (int64_t)std::max(1u, std::thread::hardware_concurrency()),
+                                         symbols / (int64_t(1) << 22) + 1, (int64_t)std::max(n, 1)});
+  if (nt <= 1) return scan(0, n, nullptr);
+  std::atomic<bool> bad(false);
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back([&, t] {
+      if (!scan((int)((int64_t)n * t / nt), (int)((int64_t)n * (t + 1) / nt), &bad)) bad.store(true);
+    });
+  for (auto& x : th) x.join();
+  return !bad.load();
 }
 
 inline int64_t nwords(int len, int bits) { return len / (64 / bits) + 2; }
@@ -1259,11 +1277,28 @@ bool wfa_device_batch_t::load(Engine* e, const wfa_pair_t* pairs, int count, boo
     for (int t = 0; t < nt; ++t) th.emplace_back(pack, t);
     for (auto& x : th) x.join();
   }
+  // Longest pairs first, ties in input order: a counting sort over
+  // plen + tlen when the longest is within a few times the pair count, else
+  // a stable sort.
   sorted.resize(n);
-  std::iota(sorted.begin(), sorted.end(), 0);
-  std::stable_sort(sorted.begin(), sorted.end(), [&](int x, int y) {
-    return (int64_t)plen[x] + tlen[x] > (int64_t)plen[y] + tlen[y];
-  });
+  int64_t lmax = 0;
+  for (int i = 0; i < n; ++i) lmax = std::max<int64_t>(lmax, (int64_t)plen[i] + tlen[i]);
+  if (lmax <= std::max<int64_t>(4096, 8 * (int64_t)n)) {
+    std::vector<int> start((size_t)lmax + 2, 0);
+    for (int i = 0; i < n; ++i) ++start[(size_t)(lmax - (plen[i] + tlen[i]))];
+    int acc = 0;
+    for (auto& c : start) {
+      const int cnt = c;
+      c = acc;
+      acc += cnt;
+    }
+    for (int i = 0; i < n; ++i) sorted[(size_t)start[(size_t)(lmax - (plen[i] + tlen[i]))]++] = i;
+  } else {
+    std::iota(sorted.begin(), sorted.end(), 0);
+    std::stable_sort(sorted.begin(), sorted.end(), [&](int x, int y) {
+      return (int64_t)plen[x] + tlen[x] > (int64_t)plen[y] + tlen[y];
+    });
+  }
   plan_key.clear();  // new pairs: re-plan on the next run
   have_reverse = false;
   have_results = host_results = ops_on_device = false;

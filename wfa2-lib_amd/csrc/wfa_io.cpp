@@ -70,43 +70,63 @@ class Channel {
   std::deque<std::unique_ptr<Batch>> q_;
 };
 
-// getline() over a FILE with a large buffer; returns false at end of file.
+// Lines of a FILE read through a large buffer.  A line is returned as a view
+// into the buffer (with its '\n'), valid until the next call; one that
+// straddles two reads is assembled in a side string.
 class LineReader {
  public:
   explicit LineReader(FILE* f) : f_(f), buf_(size_t(1) << 22) {}
-  bool next(std::string& line) {
-    line.clear();
-    bool any = false;
+  bool next(const char** p, size_t* len) {
+    if (carry_done_) {
+      carry_.clear();
+      carry_done_ = false;
+    }
     for (;;) {
-      if (pos_ == end_) {
-        end_ = fread(buf_.data(), 1, buf_.size(), f_);
-        pos_ = 0;
-        bytes_ += end_;
-        if (end_ == 0) return any;
+      if (pos_ < end_) {
+        const char* b = buf_.data() + pos_;
+        const void* nl = memchr(b, '\n', end_ - pos_);
+        if (nl) {
+          const size_t take = (size_t)((const char*)nl - b) + 1;
+          pos_ += take;
+          if (carry_.empty()) {
+            *p = b;
+            *len = take;
+            return true;
+          }
+          carry_.append(b, take);
+          return give_carry(p, len);
+        }
+        carry_.append(b, end_ - pos_);
+        pos_ = end_;
       }
-      const char* b = buf_.data() + pos_;
-      const void* nl = memchr(b, '\n', end_ - pos_);
-      const size_t take = nl ? (size_t)((const char*)nl - b) + 1 : end_ - pos_;
-      line.append(b, take);
-      pos_ += take;
-      any = true;
-      if (nl) return true;
+      end_ = fread(buf_.data(), 1, buf_.size(), f_);
+      pos_ = 0;
+      bytes_ += end_;
+      if (end_ == 0) return carry_.empty() ? false : give_carry(p, len);
     }
   }
   int64_t bytes() const { return bytes_; }
 
  private:
+  bool give_carry(const char** p, size_t* len) {
+    *p = carry_.data();
+    *len = carry_.size();
+    carry_done_ = true;
+    return true;
+  }
   FILE* f_;
   std::vector<char> buf_;
   size_t pos_ = 0, end_ = 0;
   int64_t bytes_ = 0;
+  std::string carry_;
+  bool carry_done_ = false;
 };
 
 // A line of the .seq format: drop the tag character and one trailing '\n'.
-void strip(const std::string& line, const char** p, int* len) {
-  int n = (int)line.size() - 1;
-  if (!line.empty() && line.back() == '\n') --n;
-  *p = line.data() + (line.empty() ? 0 : 1);
+void strip(const char* line, size_t size, const char** p, int* len) {
+  int n = (int)size - 1;
+  if (size > 0 && line[size - 1] == '\n') --n;
+  *p = line + (size == 0 ? 0 : 1);
   *len = n > 0 ? n : 0;
 }
 
@@ -249,7 +269,7 @@ extern "C" int wfa_align_seq_file(wavefront_aligner_t* const wf, const char* in_
     fclose(in);
     return WF_STATUS_UNATTAINABLE;
   }
-  if (batch_pairs <= 0) batch_pairs = 1 << 18;
+  if (batch_pairs <= 0) batch_pairs = 1 << 16;
   const bool score_only = wf->alignment_scope == compute_score;
   const wavefront_penalties_t pen = wf->penalties;
   const double t0 = now_s();
@@ -259,20 +279,38 @@ extern "C" int wfa_align_seq_file(wavefront_aligner_t* const wf, const char* in_
 
   std::thread reader([&] {
     LineReader lr(in);
-    std::string l1, l2;
+    bool eof = false;
     for (;;) {
       const double t = now_s();
       auto b = std::make_unique<Batch>();
-      while (b->n() < batch_pairs && b->seqs.size() < (size_t(64) << 20)) {
-        if (!lr.next(l1) || !lr.next(l2)) break;
+      b->seqs.reserve(size_t(8) << 20);
+      b->poff.reserve(batch_pairs);
+      b->toff.reserve(batch_pairs);
+      b->plen.reserve(batch_pairs);
+      b->tlen.reserve(batch_pairs);
+      while (!eof && b->n() < batch_pairs && b->seqs.size() < (size_t(64) << 20)) {
+        // each line is copied out before the next is read (views die then)
+        const char* l;
+        size_t ll;
         const char* p;
-        const char* q;
-        int pl, ql;
-        strip(l1, &p, &pl);
-        strip(l2, &q, &ql);
-        b->poff.push_back((int64_t)b->seqs.size());
-        b->plen.push_back(pl);
+        int pl;
+        if (!lr.next(&l, &ll)) {
+          eof = true;
+          break;
+        }
+        strip(l, ll, &p, &pl);
+        const size_t at = b->seqs.size();
         b->seqs.append(p, pl);
+        if (!lr.next(&l, &ll)) {  // a pattern without its text is dropped
+          b->seqs.resize(at);
+          eof = true;
+          break;
+        }
+        const char* q;
+        int ql;
+        strip(l, ll, &q, &ql);
+        b->poff.push_back((int64_t)at);
+        b->plen.push_back(pl);
         b->toff.push_back((int64_t)b->seqs.size());
         b->tlen.push_back(ql);
         b->seqs.append(q, ql);
