@@ -562,6 +562,29 @@ def main():
         c0_rate, c0_dt, _ = batch_rate(al, c0)
         log("1 kbp/5%%: %d pairs in %.2f ms (%.3e pairs/s)" % (c0.n, c0_dt * 1e3, c0_rate))
         c0_out = {"pairs": c0.n, "value": c0_rate, "unit": "pairs/s", "gcups": c0.cells * c0_rate / c0.n / 1e9}
+        if not args.no_cpu:
+            # every pair against the reference build (10,000 pairs take it
+            # well under a second on the host's cores)
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            try:
+                import oracle as O
+            except Exception as exc:  # pragma: no cover
+                O = None
+                log("reference build unavailable: %s" % exc)
+            if O is not None and O.ref_available():
+                threads = cpu_threads()
+                ref = O.ref_align(O.Conf("affine", (0, 4, 6, 2), "high"), O.Pairs(c0.patterns(), c0.texts()),
+                                  threads)
+                got = al.align_batch(c0)
+                differ = sum(1 for i in range(c0.n) if got.status[i] != ref.status[i]
+                             or got.score[i] != ref.score[i] or got.ops(i) != ref.ops(i))
+                c0_out["cpu_reference"] = {"value": c0.n / ref.seconds, "unit": "pairs/s", "cores": threads,
+                                           "sample": "all %d pairs, %s" % (c0.n, os.path.basename(O.ref_lib_path()))}
+                c0_out["parity_check"] = {"pairs": c0.n, "differ": differ}
+                log("1 kbp/5%%: every pair against the reference build: %d differ (reference %.3e pairs/s on %d "
+                    "threads)" % (differ, c0.n / ref.seconds, threads))
+                if differ:
+                    raise RuntimeError("1 kbp/5%: GPU results differ from the reference build")
 
     variants = None
     if args.extras and args.variant_pairs > 0:
