@@ -165,6 +165,18 @@ __device__ __forceinline__ void lean_wfadaptive_regs(int d, int lo, int mlo, int
   *chi = mhi <= bottom ? mhi : max(last, bottom);
 }
 
+// Matching symbols at the start of a 16-symbol window whose xor is x, capped
+// at cap: v_ffbl gives -1 for x == 0, which the shift makes 0x7fffffff, so
+// one three-way minimum takes both the full window (16) and the cap.  (As a
+// count-trailing-zeros the compiler would add a compare and a select for
+// x == 0; the instruction is written out so it keeps its own zero result.)
+__device__ __forceinline__ int run16(uint32_t x, int cap) {
+  uint32_t f;
+  asm("v_ffbl_b32 %0, %1" : "=v"(f) : "v"(x));
+  const int t = (int)(f >> 1);
+  return min(min(t, 16), cap);
+}
+
 // 16 symbols of a 2-bit packed sequence starting at symbol pos (dword view).
 __device__ __forceinline__ uint32_t win16(const uint32_t* W, int pos) {
   const unsigned w = (unsigned)pos >> 4;
@@ -178,7 +190,7 @@ __device__ __forceinline__ int lcp16(const uint32_t* P, int pb, const uint32_t* 
     const int rem = min(plen - v, tlen - h);
     if (rem <= 0) return h;
     const uint32_t x = win16(P, pb + v) ^ win16(T, tb + h);
-    const int n = x ? (int)(__builtin_ctz(x) >> 1) : 16;
+    const int n = run16(x, 16);
     if (n >= rem) return h + rem;
     h += n;
     v += n;
@@ -538,7 +550,12 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
           if (del1 > best) { best = del1; src = kSrcD1; }
           if (ins1 > best) { best = ins1; src = kSrcI1; }
           const uint32_t bits = (ei1 >= om_l ? kRecI1Ext : 0u) | (ed1 >= om_r ? kRecD1Ext : 0u);
-          if (REC && act) rec[(unsigned)(k - lo)] = (uint8_t)(src | bits);
+          // The record byte is made before the branch of its store (an
+          // opaque copy keeps it there): inside, its compare-select chain
+          // has nothing to interleave with and waits on every select.
+          uint32_t rb = src | bits;
+          asm volatile("" : "+v"(rb));
+          if (REC && act) rec[(unsigned)(k - lo)] = (uint8_t)rb;
           // null-after-max, then the extension of the valid M cells: one
           // 16-symbol window for every lane, more only for lanes whose run
           // fills it (uniform branch).
@@ -548,16 +565,18 @@ __global__ void __launch_bounds__(64) wfa_forward_lean_kernel(ForwardArgs a) {
           int pp = okM ? pb + best - k : 0, pt = okM ? tb + best : 0;
           {
             const uint32_t x = win16(seqP, pp) ^ win16(seqT, pt);
-            const int n = min(x ? (int)(__builtin_ctz(x) >> 1) : 16, rem);
+            const int n = run16(x, rem);
             h += n;
             bool more = (n == 16) & (rem > 16);
-            if (__ballot(more)) {
+            // (the AND of two single-compare ballots: a ballot of the combined
+            // condition is taken from a 0/1 vector value compared back)
+            if (__ballot(n == 16) & __ballot(rem > 16)) {
               while (more) {
                 rem -= 16;
                 pp += 16;
                 pt += 16;
                 const uint32_t y = win16(seqP, pp) ^ win16(seqT, pt);
-                const int n2 = min(y ? (int)(__builtin_ctz(y) >> 1) : 16, rem);
+                const int n2 = run16(y, rem);
                 h += n2;
                 more = n2 == 16 && rem > 16;
               }
