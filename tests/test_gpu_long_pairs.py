@@ -72,3 +72,63 @@ def test_biwfa_full_100kbp_equal_reference():
     got = al.align_batch(ps)
     for i in range(ps.n):
         check(got, i, want[i])
+
+
+
+
+WAVES_CHILD = r"""
+import hashlib, json, sys
+import numpy as np
+sys.path.insert(0, %r)
+import wfa_amd
+from wfa_amd import datagen
+# twelve pairs with texts of 17,000 bases and 7%% errors (substitutions,
+# 1-base insertions and deletions), dual-cost gap-affine, memory mode high
+rng = np.random.default_rng(17)
+pats, txts = [], []
+for _ in range(12):
+    t = rng.choice(list(b"ACGT"), 17000).astype(np.uint8)
+    p = []
+    for c in t:
+        r = rng.random()
+        if r < 0.07 / 3:
+            p.append(int(rng.choice(list(b"ACGT"))))
+        elif r < 0.14 / 3:
+            p.append(int(c)); p.append(int(rng.choice(list(b"ACGT"))))
+        elif r < 0.07:
+            continue
+        else:
+            p.append(int(c))
+    pats.append(bytes(p)); txts.append(bytes(t))
+ps = datagen.PairSet.from_lists(pats, txts)
+al = wfa_amd.Aligner(metric="affine2p", penalties=(0, 4, 6, 2, 24, 1))
+db = al.upload(ps)
+rc = db.run()
+got = db.download()
+print(json.dumps({"rc": rc, "launches": db.stats().forward_launches, "dsteps": db.stats().diagonal_steps,
+                  "res": [[int(got.status[i]), int(got.score[i]), hashlib.sha256(got.ops(i)).hexdigest()]
+                          for i in range(ps.n)]}))
+"""
+
+
+def test_long_pairs_in_record_arena_waves_agree():
+    """Twelve long pairs whose records (about 560 MB together, under 64 MB
+    each) do not fit a 64 MB record arena at once: run together they fill it
+    before any finishes, so they are re-run one workgroup at a time, in
+    waves; the results equal those of one launch with a large arena."""
+    import subprocess
+    from conftest import ROOT
+    code = WAVES_CHILD % os.path.join(ROOT, "wfa2-lib_amd")
+    out = {}
+    for name, env in (("one launch", {}), ("arena waves", {"WFA_RECORD_ARENA_MB": "64"})):
+        say("long pairs: %s" % name)
+        r = subprocess.run([sys.executable, "-c", code], env=dict(os.environ, **env), capture_output=True,
+                           timeout=240)
+        assert r.returncode == 0, (name, r.stderr.decode()[-2000:])
+        out[name] = json.loads(r.stdout.decode().strip().splitlines()[-1])
+        say("  %s: %d launches, %d diagonal-steps" % (name, out[name]["launches"], out[name]["dsteps"]))
+        assert out[name]["rc"] == 0, (name, out[name])
+    assert out["arena waves"]["launches"] > out["one launch"]["launches"]
+    assert out["arena waves"]["res"] == out["one launch"]["res"]
+    assert out["arena waves"]["dsteps"] == out["one launch"]["dsteps"]
+    assert all(r[0] == 0 for r in out["one launch"]["res"])

@@ -98,6 +98,7 @@ PlanGroup& PlanGroup::operator=(PlanGroup&& o) noexcept {
   block = o.block;
   wcap = o.wcap;
   lmax_words = o.lmax_words;
+  max_grid = o.max_grid;
   lclass = o.lclass;
   ids = std::move(o.ids);
   d_ids = std::move(o.d_ids);
@@ -146,6 +147,7 @@ PlanGroup take_group(Engine* e) {
   g.block = 64;
   g.wcap = 0;
   g.lmax_words = 0;
+  g.max_grid = 0;
   g.ids.clear();
   g.ids_on_device = false;
   g.est_rec_bytes = -1;
@@ -725,6 +727,7 @@ PlanGroup make_large(Engine* e, const JobSet& js, std::vector<int> ids) {
 
 LaunchShape shape_of(Engine* e, const PlanGroup& g, int bits, const Penalties& pen, int cut) {
   LaunchShape shape;
+  memset(&shape, 0, sizeof(shape));  // padding too: shapes are compared as bytes (graph keys)
   shape.family = g.family;
   shape.block = g.block;
   int per_cu = 0;
@@ -737,6 +740,7 @@ LaunchShape shape_of(Engine* e, const PlanGroup& g, int bits, const Penalties& p
   }
   if (per_cu <= 0) per_cu = 1;
   shape.grid = (int)std::min<int64_t>((int64_t)g.ids.size(), (int64_t)per_cu * e->num_cus);
+  if (g.max_grid > 0) shape.grid = std::min(shape.grid, g.max_grid);
   if (e->na && g.family == wfa::kLarge) {
     // no-allocation run: as many workgroups as rings fit in its ring share
     const size_t per_block = (size_t)wfa::ring_slots(pen) * g.wcap * sizeof(int);
@@ -1046,11 +1050,17 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
         }
       }
       if (!arena_full.empty()) {
+        int max_grid = g.max_grid;
         if (arena_full.size() == g.ids.size()) {
-          // not a single job of the group fitted: grow the arena
+          // not a single job of the group fitted: grow the arena; when it
+          // cannot grow, run the jobs one workgroup at a time (jobs that ran
+          // at once can fill the arena together although each fits alone)
           size_t free_b = 0, total_b = 0;
           (void)hipMemGetInfo(&free_b, &total_b);
-          if (e->rec_bytes * 2 > std::min(e->rec_cap, e->rec.cap + free_b / 2)) {
+          if (e->rec_bytes * 2 > std::min(e->rec_cap, e->rec.cap + free_b / 2) && g.max_grid != 1 &&
+              g.ids.size() > 1) {
+            max_grid = 1;
+          } else if (e->rec_bytes * 2 > std::min(e->rec_cap, e->rec.cap + free_b / 2)) {
             fprintf(stderr, "[WFA] record arena exhausted (%zu MB)\n", e->rec_bytes >> 20);
             for (int i : arena_full) h_status[i] = WF_STATUS_OOM;
             if (!hip_ok(hipMemcpyAsync(js.status, h_status.data(), (size_t)js.n * 4, hipMemcpyHostToDevice, st),
@@ -1070,6 +1080,7 @@ int run_jobs(wavefront_aligner_t* wf, Engine* e, const JobSet& js, std::vector<P
           h.block = g.block;
           h.wcap = g.wcap;
           h.lmax_words = g.lmax_words;
+          h.max_grid = max_grid;
           h.ids = std::move(arena_full);
           next.push_back(std::move(h));
         }

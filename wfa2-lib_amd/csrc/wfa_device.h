@@ -174,10 +174,10 @@ struct BacktraceArgs {
 enum Family : int { kSmall = 0, kLarge = 1, kLean = 2 };
 
 struct LaunchShape {
-  int family;
-  int block;        // threads per workgroup
-  int grid;         // persistent workgroups
-  size_t lds_bytes; // dynamic LDS
+  int family = 0;
+  int block = 0;         // threads per workgroup
+  int grid = 0;          // persistent workgroups
+  size_t lds_bytes = 0;  // dynamic LDS
 };
 
 // Returns hipSuccess or the launch error.  bits: 2 (ACGT packed) or 8 (bytes).

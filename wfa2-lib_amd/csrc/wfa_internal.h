@@ -92,6 +92,7 @@ struct PlanGroup {
   int block = 64;
   int wcap = 0;
   int lmax_words = 0;
+  int max_grid = 0;            // cap on the persistent grid (0: none), set when jobs must run one at a time
   std::vector<int> ids;
   DevBuf d_ids;
   bool ids_on_device = false;  // d_ids holds ids (uploaded once per plan)
