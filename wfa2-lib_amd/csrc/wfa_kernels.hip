@@ -39,8 +39,11 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 1024 ? 4 : 1) wfa_forward_kern
   // One wave and the ring in LDS: no workgroup barrier is ever needed.
   constexpr bool WAVE_ONLY = SMALL && NW == 1;
   __shared__ int sh_bcast[2];
-  __shared__ int sh_end[2];  // the end cell's value, by step parity (see the termination test)
-  __shared__ int sh_red[NW > 1 ? NW : 1][2 * NC + 1];
+  __shared__ int sh_end[2];  // the end cell's value, by computed-step parity (see the termination test)
+  // per-wave ranges of a step, by the parity of the computed steps: a wave
+  // already in the next computed step writes the other half, so one barrier
+  // per step separates the writes from every wave's merge (sh_end likewise)
+  __shared__ int sh_red[2][NW > 1 ? NW : 1][2 * NC + 1];
   __shared__ int sh_hit[NW > 1 ? NW : 1];
   BlockRed<BLOCK> red{sh_hit};
 
@@ -194,6 +197,7 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 1024 ? 4 : 1) wfa_forward_kern
     RingPos rp{0, 0, 0};  // rp.cur == cur: the M ring shares the metadata slots
     int s = 0;
     int status = 1;  // running
+    int nb = 0;      // computed (non-null) steps: the parity of sh_red and sh_end
     uint32_t prev_blk = kNoBlock;
     long long dsteps = 0;
     if constexpr (!SMALL) {
@@ -434,7 +438,7 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 1024 ? 4 : 1) wfa_forward_kern
             if (rec) rec[k - lo] = (uint8_t)src;
             outM[k] = to_ring(m, ring);
             vM = (m != kOffsetNull);
-            if (k == kend) sh_end[s & 1] = (int)to_ring(m, ring);
+            if (k == kend) sh_end[nb & 1] = (int)to_ring(m, ring);
           }
         } else if (k <= hi) {
           const int om_l = cur_r.om_l;
@@ -479,7 +483,7 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 1024 ? 4 : 1) wfa_forward_kern
           if (k == kend) {
             // the end component's value, for the termination test after the step
             const int ve = cend == kM ? m : cend == kI1 ? ins1 : cend == kD1 ? del1 : cend == kI2 ? ins2 : del2;
-            sh_end[s & 1] = (int)to_ring(ve, ring);
+            sh_end[nb & 1] = (int)to_ring(ve, ring);
           }
         }
         const int kw = kb + wid * 64;
@@ -503,34 +507,37 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 1024 ? 4 : 1) wfa_forward_kern
       }
       if constexpr (NW > 1) {
         if (lane == 0) {
-          sh_red[wid][0] = rM.first;  sh_red[wid][1] = rM.last;
-          if constexpr (NC == 1) sh_red[wid][2] = wmin;
+          sh_red[nb & 1][wid][0] = rM.first;  sh_red[nb & 1][wid][1] = rM.last;
+          if constexpr (NC == 1) sh_red[nb & 1][wid][2] = wmin;
           if constexpr (NC > 1) {
-            sh_red[wid][2] = rI1.first; sh_red[wid][3] = rI1.last;
-            sh_red[wid][4] = rD1.first; sh_red[wid][5] = rD1.last;
+            sh_red[nb & 1][wid][2] = rI1.first; sh_red[nb & 1][wid][3] = rI1.last;
+            sh_red[nb & 1][wid][4] = rD1.first; sh_red[nb & 1][wid][5] = rD1.last;
           }
           if constexpr (NC == 5) {
-            sh_red[wid][6] = rI2.first; sh_red[wid][7] = rI2.last;
-            sh_red[wid][8] = rD2.first; sh_red[wid][9] = rD2.last;
+            sh_red[nb & 1][wid][6] = rI2.first; sh_red[nb & 1][wid][7] = rI2.last;
+            sh_red[nb & 1][wid][8] = rD2.first; sh_red[nb & 1][wid][9] = rD2.last;
           }
         }
         __syncthreads();
         rM.init(); rI1.init(); rD1.init(); rI2.init(); rD2.init();
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
-          rM.merge(sh_red[w][0], sh_red[w][1]);
-          if constexpr (NC == 1) wmin = min(wmin, sh_red[w][2]);
+          rM.merge(sh_red[nb & 1][w][0], sh_red[nb & 1][w][1]);
+          if constexpr (NC == 1) wmin = min(wmin, sh_red[nb & 1][w][2]);
           if constexpr (NC > 1) {
-            rI1.merge(sh_red[w][2], sh_red[w][3]);
-            rD1.merge(sh_red[w][4], sh_red[w][5]);
+            rI1.merge(sh_red[nb & 1][w][2], sh_red[nb & 1][w][3]);
+            rD1.merge(sh_red[nb & 1][w][4], sh_red[nb & 1][w][5]);
           }
           if constexpr (NC == 5) {
-            rI2.merge(sh_red[w][6], sh_red[w][7]);
-            rD2.merge(sh_red[w][8], sh_red[w][9]);
+            rI2.merge(sh_red[nb & 1][w][6], sh_red[nb & 1][w][7]);
+            rD2.merge(sh_red[nb & 1][w][8], sh_red[nb & 1][w][9]);
           }
         }
       }
-      block_sync();  // ring values of this step visible to every lane; sh_red free again
+      // (no second barrier: the one above already orders this step's ring
+      // writes before every read of the next step, and the next step's
+      // range writes go to the other half of sh_red)
+      if constexpr (NW == 1) block_sync();
       if constexpr (CUT) {
         // The reference's backtrace also probes the readable cells of an
         // input the cut-off emptied (a null wavefront to compute,
@@ -583,11 +590,12 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 1024 ? 4 : 1) wfa_forward_kern
           if (ce == kD2) { pce = has_d2; rce = rD2; }
         }
         if (pce && !rce.null() && kend >= rce.first && kend <= rce.last) {
-          // The lane of k_end left the value in sh_end (by step parity, so a
-          // wave already in the next step cannot overwrite it before every wave
-          // read it: one block barrier per step): no ring round trip.  The
-          // same LDS word for every lane: keep the loop control scalar.
-          const int vend = uniform(sh_end[s & 1]);
+          // The lane of k_end left the value in sh_end (by the parity of
+          // the computed steps, so a wave already in the next one cannot
+          // overwrite it before every wave read it: one block barrier per
+          // step): no ring round trip.  The same LDS word for every lane:
+          // keep the loop control scalar.
+          const int vend = uniform(sh_end[nb & 1]);
           if (vend >= tlen) status = kDevCompleted;
         }
       }
@@ -597,6 +605,7 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 1024 ? 4 : 1) wfa_forward_kern
         if (tid == 0 && a.out_end_rec) a.out_end_rec[p] = er;
       }
       if (status == 1 && s >= s_cap) status = kDevBadTrace;  // unreachable for valid input
+      ++nb;
     }
     if (CUT == 2 && status == kDevZdropped) {  // alignment_end_pos of the z-drop (wavefront_heuristic.c:424-431)
       end_k = xs.k;
