@@ -150,9 +150,19 @@ __device__ bool side_cutoff(Side<NC, CUT>& sd, const HeurArgs& heur, BlockRed<BL
 // score scope, wavefront_extend.c:184-191), -1 on an internal error, 0
 // otherwise.  *max_ak gets the largest antidiagonal 2*offset-k of the
 // extended M wavefront (0 when absent).
+// The per-wave ranges of a side step, double-buffered: consecutive computed
+// side steps of a workgroup write alternate halves, so one barrier per step
+// separates a wave's writes from every wave's merge (a wave already in the
+// next step writes the other half).
+template <int NC, int NW>
+struct RedRows {
+  int (*buf)[NW][2 * NC + 2];  // two halves
+  int par;                     // computed steps so far (uniform)
+};
+
 template <int NC, int BLOCK, int BITS, bool CUT>
 __device__ int side_step(Side<NC, CUT>& sd, int s, int plen, int tlen, const Penalties& pen, const HeurArgs& heur, int wcap,
-                         bool want_max, int* max_ak, long long* dsteps, int (*sh_red)[2 * NC + 2],
+                         bool want_max, int* max_ak, long long* dsteps, RedRows<NC, BLOCK / 64>& rr,
                          BlockRed<BLOCK>& red) {
   constexpr int NW = BLOCK / 64;
   const int tid = threadIdx.x, lane = tid & 63, wid = uniform(tid >> 6);
@@ -296,6 +306,7 @@ __device__ int side_step(Side<NC, CUT>& sd, int s, int plen, int tlen, const Pen
     if (want_max) *max_ak = best_ak;
   }
   if constexpr (NW > 1) {
+    int (*sh_red)[2 * NC + 2] = rr.buf[rr.par & 1];
     if (lane == 0) {
       sh_red[wid][2 * NC] = best_ak;
       sh_red[wid][0] = rM.first;  sh_red[wid][1] = rM.last;
@@ -325,8 +336,11 @@ __device__ int side_step(Side<NC, CUT>& sd, int s, int plen, int tlen, const Pen
       }
     }
     if (want_max) *max_ak = ak_all;
+    ++rr.par;
   }
-  __syncthreads();  // ring writes of this score visible to the whole workgroup
+  // (one barrier: the one before the merge already makes this score's ring
+  // writes visible to the whole workgroup)
+  if constexpr (NW == 1) __syncthreads();
   if constexpr (NC == 1) {
     if (prune_try && !rM.null()) {
       const int first0 = rM.first, last0 = rM.last;
@@ -567,7 +581,7 @@ __device__ __forceinline__ void setup_sides(const BiArgs& a, const BiProblem& pr
 template <int NC, int BLOCK, int BITS, bool CUT>
 __device__ int find_phase2(Side<NC, CUT>& f, Side<NC, CUT>& r, int& sf, int& sr, bool last_forward, int plen,
                            int tlen, const Penalties& pen, const BiArgs& a, int wcap, int s_cap, Bp& bp,
-                           long long* dsteps, int (*sh_red)[2 * NC + 2], BlockRed<BLOCK>& red) {
+                           long long* dsteps, RedRows<NC, BLOCK / 64>& rr, BlockRed<BLOCK>& red) {
   const int scope = pen.scope;
   // wavefront_bialign_overlap_gopen_adjust (wavefront_bialign.c:960-973)
   const int gap_open = (NC == 5) ? max(pen.o1, pen.o2) : (NC == 3 ? pen.o1 : 0);
@@ -578,7 +592,7 @@ __device__ int find_phase2(Side<NC, CUT>& f, Side<NC, CUT>& r, int& sf, int& sr,
       if (sf + min_sr - gap_open >= bp.score) break;
       overlap<NC, BLOCK, CUT>(f, r, sf, sr, true, plen, tlen, pen, wcap, bp, red);
       ++sr;
-      rc = side_step<NC, BLOCK, BITS>(r, sr, plen, tlen, pen, a.heur, wcap, false, &ak, dsteps, sh_red, red);
+      rc = side_step<NC, BLOCK, BITS>(r, sr, plen, tlen, pen, a.heur, wcap, false, &ak, dsteps, rr, red);
       if (rc < 0) return kDevRingOverflow;
       if (rc == 2) return kDevUnreachable;
       if (rc > 0) return kBiEndReached;
@@ -587,7 +601,7 @@ __device__ int find_phase2(Side<NC, CUT>& f, Side<NC, CUT>& r, int& sf, int& sr,
     if (min_sf + sr - gap_open >= bp.score) break;
     overlap<NC, BLOCK, CUT>(r, f, sr, sf, false, plen, tlen, pen, wcap, bp, red);
     ++sf;
-    rc = side_step<NC, BLOCK, BITS>(f, sf, plen, tlen, pen, a.heur, wcap, false, &ak, dsteps, sh_red, red);
+    rc = side_step<NC, BLOCK, BITS>(f, sf, plen, tlen, pen, a.heur, wcap, false, &ak, dsteps, rr, red);
     if (rc < 0) return kDevRingOverflow;
     if (rc == 2) return kDevUnreachable;
     if (rc > 0) return kBiEndReached;
@@ -629,7 +643,8 @@ __device__ __forceinline__ void bp_clear(Bp& bp) {
 
 template <int NC, int BLOCK, int BITS, bool CUT>
 __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
-  __shared__ int sh_red[BLOCK / 64][2 * NC + 2];
+  __shared__ int sh_red[2][BLOCK / 64][2 * NC + 2];
+  RedRows<NC, BLOCK / 64> rr{sh_red, 0};
   __shared__ int sh_min[BLOCK / 64 > 0 ? BLOCK / 64 : 1];
   __shared__ int sh_q;
   BlockRed<BLOCK> red{sh_min};
@@ -666,7 +681,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
     while (true) {
       if (fmax + rmax >= max_ad) break;
       ++sf;
-      rc = side_step<NC, BLOCK, BITS>(f, sf, plen, tlen, pen, a.heur, wcap, true, &ak, &dsteps, sh_red, red);
+      rc = side_step<NC, BLOCK, BITS>(f, sf, plen, tlen, pen, a.heur, wcap, true, &ak, &dsteps, rr, red);
       if (rc < 0) { status = kDevRingOverflow; goto done; }
       if (rc == 2) { status = kDevUnreachable; goto done; }
       if (fmax < ak) fmax = ak;
@@ -674,7 +689,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
       if (rc > 0) { status = kBiEndReached; goto done; }
       if (fmax + rmax >= max_ad) break;
       ++sr;
-      rc = side_step<NC, BLOCK, BITS>(r, sr, plen, tlen, pen, a.heur, wcap, true, &ak, &dsteps, sh_red, red);
+      rc = side_step<NC, BLOCK, BITS>(r, sr, plen, tlen, pen, a.heur, wcap, true, &ak, &dsteps, rr, red);
       if (rc < 0) { status = kDevRingOverflow; goto done; }
       if (rc == 2) { status = kDevUnreachable; goto done; }
       if (rmax < ak) rmax = ak;
@@ -686,7 +701,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_kernel(BiArgs a) {
       if (sf > s_cap) { status = kDevBadTrace; goto done; }
     }
     status = find_phase2<NC, BLOCK, BITS>(f, r, sf, sr, last_forward, plen, tlen, pen, a, wcap, s_cap, bp, &dsteps,
-                                          sh_red, red);
+                                          rr, red);
   done:
     if (threadIdx.x == 0) write_outcome<NC, CUT>(a.out + q, status, bp, f.end_score, r.end_score, dsteps);
     wg_dsteps += dsteps;
@@ -766,7 +781,8 @@ __device__ __forceinline__ int payload_ak(unsigned w) { return (int)(w & 0x0FFFF
 template <int NC, int BLOCK, int BITS>
 __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_pair_kernel(BiArgs a) {
   constexpr bool CUT = false;
-  __shared__ int sh_red[BLOCK / 64][2 * NC + 2];
+  __shared__ int sh_red[2][BLOCK / 64][2 * NC + 2];
+  RedRows<NC, BLOCK / 64> rr{sh_red, 0};
   __shared__ int sh_min[BLOCK / 64 > 0 ? BLOCK / 64 : 1];
   __shared__ unsigned long long sh_word;
   BlockRed<BLOCK> red{sh_min};
@@ -814,7 +830,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_pair_kernel(BiArgs a) {
     // the reverse side's state before its (speculative) step
     const int cur0 = r.cur, end0 = r.end_score;
     const long long ds0 = dsteps;
-    rc_me = side_step<NC, BLOCK, BITS>(me, role ? sr : sf, plen, tlen, pen, a.heur, wcap, true, &ak, &dsteps, sh_red,
+    rc_me = side_step<NC, BLOCK, BITS>(me, role ? sr : sf, plen, tlen, pen, a.heur, wcap, true, &ak, &dsteps, rr,
                                        red);
     if (!pair_exchange(box, role, ++seq, pair_payload(rc_me, ak), &got, &sh_word, a.pair_wait_ticks)) { status = kBiPairAbort; goto done; }
     rcf = role ? payload_rc(got) : rc_me;  akf = role ? payload_ak(got) : ak;
@@ -878,7 +894,7 @@ __global__ void __launch_bounds__(BLOCK) wfa_breakpoint_pair_kernel(BiArgs a) {
     r.end_score = uniform(hs[1]);
   }
   status = find_phase2<NC, BLOCK, BITS>(f, r, sf, sr, last_forward, plen, tlen, pen, a, wcap, s_cap, bp, &dsteps,
-                                        sh_red, red);
+                                        rr, red);
   fwd_end = f.end_score;
   rev_end = r.end_score;
 done:
