@@ -26,8 +26,9 @@ namespace wfa {
 namespace {
 
 // CUT: 0 exact; 1 wf-adaptive, wfmash and the bands; 2 also x-drop / z-drop.
-// The 1024-thread instance is held to 64 VGPRs (8 waves per SIMD), so two
-// workgroups, two pairs, share a CU.
+// The 1024-thread instance asks for at least 4 waves per SIMD, which leaves
+// it up to 128 VGPRs (it uses about 80): one workgroup, one pair, per CU,
+// and registers enough for a chunk's ring reads to go out together.
 template <int NC, int BLOCK, bool SMALL, int BITS, int CUT>
 __global__ void __launch_bounds__(BLOCK, BLOCK == 1024 ? 4 : 1) wfa_forward_kernel(ForwardArgs a) {
   constexpr bool HC = CUT != 0;
